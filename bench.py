@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Benchmark: walk-on-boundary paths/s on one MI355X.
+
+Default workload (icosphere_neumann_3d): level-5 icosphere (20,480 triangles) radially
+displaced to r = 1 + 0.2 sin(3 theta) cos(2 phi), Neumann data of u = x^2 - y^2 + 0.5 z,
+direct BIE, interior, backward estimator, path length 6, 100,000 interior points x 256
+paths per step (25.6 M paths).  A step is one wob_estimate_domain call over all points
+(walk kernel + FP64 per-point reduction), inputs resident in HBM, fresh Philox streams per
+step (path_base advances).  Prints ONE JSON line.
+
+Also reported: the walk kernel's roofline position from its own HIP-event duration (the
+library brackets the kernel on its launch stream) and the algorithmic work model in
+wobtoolbox_amd/roofline.py, and a CPU baseline: the OpenMP restatement of the reference
+(oracle/, reference hit-pick and per-point mt19937 streams) timed on a bounded sample of
+the same workload on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "WoB paths/s and time-to-RMSE 1e-3 (learned CV vs plain WoB), 1 MI355X"
+
+
+def cpu_baseline(workload, cores, target_s=10.0):
+    """Reference-mode OpenMP oracle on a deterministic subset (every 64th point, full paths)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    sc = oracle_lib.OracleScene(workload.elements)
+    paths = workload.solver["num_sample_paths"]
+    idx = np.arange(0, len(workload.points), 64, dtype=np.uint32)
+    # calibrate on a slice, then size the run to about target_s seconds
+    cal = idx[: max(cores * 2, 32)]
+    t0 = time.perf_counter()
+    sc.estimate_domain(workload.solver, workload.points[cal], paths, seed=0, threads=cores, pick_mode=0, rng_mode=1,
+                       point_index=cal)
+    dt = time.perf_counter() - t0
+    rate = len(cal) * paths / dt
+    n = int(min(len(idx), max(cores, target_s * rate / paths)))
+    sel = idx[:n]
+    t0 = time.perf_counter()
+    sc.estimate_domain(workload.solver, workload.points[sel], paths, seed=0, threads=cores, pick_mode=0, rng_mode=1,
+                       point_index=sel)
+    dt = time.perf_counter() - t0
+    return {"value": n * paths / dt, "unit": "paths/s", "cores": cores, "kind": "port",
+            "sample": f"{n} points (every 64th evaluation point) x {paths} paths = {n * paths} paths in {dt:.1f} s; "
+                      f"reference hit pick (64-entry buffer), per-point mt19937 seeded as runtime_utils.cuh:89-103, "
+                      f"OpenMP static schedule over points"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="icosphere_neumann_3d", choices=["icosphere_neumann_3d", "circle_dirichlet_2d"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--paths-per-lane", type=int, default=0)
+    ap.add_argument("--block-size", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested). "
+              "Points would shard across GPUs with no exchange step; only one GPU is measured.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import wobtoolbox_amd as wob
+    from wobtoolbox_amd import geometry as G
+    from wobtoolbox_amd.roofline import MI355X_PEAKS, workload_model
+
+    print(f"[bench] building workload {args.workload}", file=sys.stderr, flush=True)
+    w = G.icosphere_neumann_3d() if args.workload == "icosphere_neumann_3d" else G.circle_dirichlet_2d()
+    paths = w.solver["num_sample_paths"]
+    dev = torch.device("cuda:0")
+    scene = wob.SceneHost(w.elements)
+    est = wob.Estimator.from_config(scene, w.solver, paths_per_lane=args.paths_per_lane, block_size=args.block_size)
+    pts = torch.as_tensor(w.points).to(dev)
+    n = pts.shape[0]
+    s = torch.zeros((n, w.elements.dim + 1), dtype=torch.float64, device=dev)
+    q = torch.zeros_like(s)
+    stream = torch.cuda.current_stream()
+
+    step = 0
+    print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        est.compute_domain(pts, paths, seed=0, path_base=step * paths, out=(s, q))
+        step += 1
+    torch.cuda.synchronize()
+    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        est.compute_domain(pts, paths, seed=0, path_base=step * paths, out=(s, q))
+        step += 1
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    total_paths = args.steps * n * paths
+    value = total_paths / elapsed
+
+    # walk-kernel duration per launch, HIP events on the launch stream (inside the library)
+    est.time_walk = True
+    walk_ms = []
+    for _ in range(max(3, min(args.steps, 10))):
+        est.compute_domain(pts, paths, seed=0, path_base=step * paths, out=(s, q))
+        walk_ms.append(est.last_launch.walk_ms)
+        step += 1
+    est.time_walk = False
+    avg_walk_s = float(np.mean(walk_ms)) / 1e3
+    flop, byts = workload_model(w.name, w.elements.n, w.solver["path_length"])
+    launch_paths = n * paths
+    gbps = byts * launch_paths / avg_walk_s / 1e9
+    tflops = flop * launch_paths / avg_walk_s / 1e12
+
+    # accuracy of what was computed (all steps accumulate into s, q)
+    m = (s / (step * paths)).cpu().numpy()
+    ex = w.exact(w.points)
+    d = m[:, 0] - ex[:, 0]
+    if w.solver["problem"] == "NeumannProblem":
+        d = d - d.mean()
+    rmse = float(np.sqrt((d ** 2).mean()))
+
+    cores = min(16, os.cpu_count() or 1)
+    cpu = None
+    if not args.no_cpu_baseline:
+        print(f"[bench] cpu baseline on {cores} threads", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(w, cores)
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "paths/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": w.name, "elements": w.elements.n, "points": n, "paths_per_point": paths,
+                   "paths_per_step": launch_paths, "path_length": w.solver["path_length"],
+                   "bie_model": w.solver["bie_model"], "problem": w.solver["problem"],
+                   "estimator": w.solver["estimator"], "paths_per_lane": est.last_launch.used_paths_per_lane,
+                   "work_items": est.last_launch.used_work_items},
+        "roofline": {"bound": "hbm", "achieved": gbps, "peak": MI355X_PEAKS["hbm_GBps"], "unit": "GB/s",
+                     "frac": gbps / MI355X_PEAKS["hbm_GBps"], "traffic": None,
+                     "kernel": "walk_kernel", "avg_kernel_ms": avg_walk_s * 1e3,
+                     "algorithmic_bytes_per_path": byts, "algorithmic_flop_per_path": flop,
+                     "fp32_valu": {"achieved": tflops, "peak": MI355X_PEAKS["fp32_TFLOPs"], "unit": "TFLOP/s",
+                                   "frac": tflops / MI355X_PEAKS["fp32_TFLOPs"]}},
+        "cpu_baseline": cpu,
+        "accuracy": {"rmse_u": rmse, "paths_per_point_total": step * paths,
+                     "note": "Neumann problem: u compared after removing the mean offset"
+                     if w.solver["problem"] == "NeumannProblem" else "u compared directly"},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,184 @@
+/*
+ * wob_c.h -- C ABI of the MI355X walk-on-boundary engine (libwob_hip.so).
+ *
+ * The reference (rsugimoto/WoBToolbox) is a header-only C++ template library whose device
+ * work is a thrust::for_each over evaluation points.  Its user-facing surface on the hot
+ * path is:
+ *   SceneHost<S,Dim,IsVec>(std::vector<ElementXs>)          wob/wob/scene.cuh:264-299
+ *   SceneHost::set_volume_cache / get_device_repr             wob/wob/scene.cuh:301-318
+ *   Estimator<...>::compute_sample_path_contribution_domain    wob/wob/estimator.cuh:154-191
+ *   Estimator<...>::compute_sample_path_contribution_boundary  wob/wob/estimator.cuh:193-222
+ *   utils::run<DeviceFunctor>(args, n, bie, domain, est, prob) wob/wob/runtime_utils.cuh:105-153
+ *   utils::random_states_init                                  wob/wob/runtime_utils.cuh:89-103
+ *   string_to_*_type / *_type_to_string                        bie_models.cuh:24-51, estimator.cuh:25-54
+ * Arbitrary user __device__ functors cannot cross a C ABI; the entry points below run the
+ * library's own estimators over caller-supplied evaluation points and accumulate per-point
+ * sums the way the reference's app functors do (apps/general_2d.cu:40-71).
+ *
+ * Conventions
+ *   - All pointers named d_* are device pointers (HBM); everything else is host memory.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *   - No exceptions cross the ABI; every entry point returns a wob_status and records a
+ *     message retrievable with wob_last_error_message() (thread-local).
+ *   - Element records follow ElementXs (scene.cuh:378-382, 646-650): per element its Dim
+ *     vertices, one normal, Dim boundary values, a boundary type, Dim Robin coefficients.
+ *   - Boundary point records follow BoundaryPoint (points.hpp:32-39) as 2*Dim+4 floats:
+ *     [p (Dim), n (Dim), interior_angle, boundary_value, boundary_type, robin_alpha].
+ */
+#ifndef WOB_C_H
+#define WOB_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum wob_status {
+    WOB_OK = 0,
+    WOB_INVALID_ARG = 1,        /* null pointer, zero size, size mismatch */
+    WOB_INVALID_ENUM = 2,       /* enum value out of range (the reference silently launches nothing:
+                                   runtime_utils.cuh:117-120) */
+    WOB_UNSUPPORTED_COMBO = 3,  /* e.g. double layer with non-Dirichlet data (bie_models.cuh:383, 394) */
+    WOB_OOM = 4,
+    WOB_DEVICE = 5,             /* a HIP runtime error */
+    WOB_CANCELLED = 6
+} wob_status;
+
+/* Enum values match the reference's enums. */
+enum { WOB_DIRICHLET = 0, WOB_NEUMANN = 1, WOB_ROBIN = 2 };                               /* points.hpp:22 */
+enum { WOB_POISSON_KERNEL = 0 };                                                           /* equation_kernel.cuh:20 */
+enum { WOB_SINGLE_LAYER = 0, WOB_DOUBLE_LAYER = 1, WOB_DIRECT = 2 };                      /* bie_models.cuh:21 */
+enum { WOB_INTERIOR = 0, WOB_EXTERIOR = 1 };                                               /* bie_models.cuh:22 */
+enum { WOB_BACKWARD = 0, WOB_FORWARD = 1 };                                                /* estimator.cuh:24 */
+enum { WOB_DIRICHLET_PROBLEM = 0, WOB_NEUMANN_PROBLEM = 1, WOB_ROBIN_PROBLEM = 2, WOB_MIXED_PROBLEM = 3 }; /* estimator.cuh:38 */
+
+typedef struct wob_scene wob_scene;
+
+/* Replaces SceneHost(const std::vector<ElementXs>&) (scene.cuh:264-299).  Host arrays are
+ * copied; nothing refers to caller memory after the call.  Builds the wide BVH, the area and
+ * boundary-value CDFs (FP32 Kahan sums, scene.cuh:271-298) and their guide tables.
+ *   vertices        n_elements * dim * dim floats
+ *   normals         n_elements * dim
+ *   boundary_values n_elements * dim
+ *   boundary_types  n_elements  (WOB_DIRICHLET / WOB_NEUMANN / WOB_ROBIN)
+ *   robin_alphas    n_elements * dim (may be NULL: zeros)                                   */
+typedef struct wob_scene_desc {
+    uint32_t dim;
+    uint32_t n_elements;
+    const float *vertices;
+    const float *normals;
+    const float *boundary_values;
+    const int32_t *boundary_types;
+    const float *robin_alphas;
+} wob_scene_desc;
+
+wob_status wob_scene_create(const wob_scene_desc *desc, wob_scene **out_scene);
+wob_status wob_scene_destroy(wob_scene *scene);
+
+/* Scene facts: dim, element count, BVH node count, BVH depth, totals (area, boundary value). */
+typedef struct wob_scene_info {
+    uint32_t dim, n_elements, n_nodes, depth;
+    float total_area, total_boundary_value;
+    uint64_t device_bytes;
+} wob_scene_info;
+wob_status wob_scene_get_info(const wob_scene *scene, wob_scene_info *info);
+
+/* Replaces SceneHost::set_volume_cache(grid_res, domain_size, cache) (scene.cuh:301-306,
+ * 206-240): a grid source over (-domain_size/2, domain_size/2)^dim sampled proportionally
+ * to |cache|.  cache: grid_res^dim host floats.                                            */
+wob_status wob_scene_set_volume_grid(wob_scene *scene, uint32_t grid_res, float domain_size, const float *cache);
+
+/* Estimator configuration: the reference's Estimator template and constructor arguments
+ * (estimator.cuh:118-152) plus the JSON keys of the apps (general_2d.cu:174-195).           */
+typedef struct wob_estimator_desc {
+    int32_t equation_kernel;      /* WOB_POISSON_KERNEL */
+    int32_t bie_model;            /* WOB_SINGLE_LAYER / WOB_DOUBLE_LAYER / WOB_DIRECT */
+    int32_t domain;               /* WOB_INTERIOR / WOB_EXTERIOR */
+    int32_t estimator;            /* WOB_BACKWARD / WOB_FORWARD */
+    int32_t problem;              /* WOB_*_PROBLEM */
+    uint32_t path_length;
+    uint32_t num_resampling_candidates;   /* 1 = plain uniform sampling */
+    uint32_t num_volume_samples;
+    float eps;                            /* kernel clamp, default FLT_EPSILON */
+    float first_kind_equation_scaling_constant;  /* default 1 */
+} wob_estimator_desc;
+
+/* Launch tuning and timing (in/out; may be NULL).  paths_per_lane: paths one lane
+ * accumulates before it writes its partial sums; block_size: threads per workgroup
+ * (64..256); 0 = automatic.  With WOB_LAUNCH_TIME_WALK in flags the call brackets the walk
+ * kernel with HIP events on the launch stream, waits for the end event and writes the
+ * kernel's duration to walk_ms (and the work-item geometry it chose to the other outputs). */
+enum { WOB_LAUNCH_TIME_WALK = 1 };
+typedef struct wob_launch_config {
+    uint32_t paths_per_lane;
+    uint32_t block_size;
+    uint32_t flags;
+    float walk_ms;              /* out */
+    uint32_t used_paths_per_lane; /* out */
+    uint32_t used_work_items;     /* out */
+} wob_launch_config;
+
+/* Domain estimates: replaces utils::run<DeviceFunctor> with an app functor that calls
+ * compute_sample_path_contribution_domain num_sample_paths times per point and adds the
+ * Kahan sum into per-point buffers (apps/general_2d.cu:44-70).
+ *   d_points   n_points * dim floats (device)
+ *   point_base global index of d_points[0] (RNG counter word; lets callers split point sets)
+ *   path_base  global index of the first path of every point in this call (progressive passes
+ *              use path_base = pass * paths_per_point so every path has its own stream)
+ *   stream_id  RNG stream word (0 for a single scene; batch entries use their scene index)
+ *   d_sum, d_sumsq  n_points * (dim + 1) doubles [u, grad u] each, ACCUMULATED into (+=)
+ * Per path the result is [u, grad u]; per lane paths are Kahan-summed in FP32, chunks are
+ * reduced per point in FP64 in a fixed order (deterministic for fixed inputs).             */
+wob_status wob_estimate_domain(const wob_scene *scene, const wob_estimator_desc *est, const float *d_points,
+                               uint32_t n_points, uint32_t point_base, uint32_t paths_per_point, uint64_t seed,
+                               uint64_t path_base, uint32_t stream_id, double *d_sum, double *d_sumsq,
+                               wob_launch_config *launch, void *stream);
+
+/* On-boundary estimates: compute_sample_path_contribution_boundary (estimator.cuh:193-222,
+ * forward :456-496), as used by apps/boundary_estimate.cu:65-98.
+ *   d_records  n_points * (2*dim + 4) floats (BoundaryPoint records, device)
+ *   d_sum, d_sumsq  n_points doubles each, accumulated into.                                 */
+wob_status wob_estimate_boundary(const wob_scene *scene, const wob_estimator_desc *est, const float *d_records,
+                                 uint32_t n_points, uint32_t point_base, uint32_t paths_per_point, uint64_t seed,
+                                 uint64_t path_base, uint32_t stream_id, double *d_sum, double *d_sumsq,
+                                 wob_launch_config *launch, void *stream);
+
+/* All-hit line query (the lbvh query_line_intersect call of scene.cuh:481-486 / 787-790,
+ * without its 64-hit cap).  For each line i (origin, direction; device arrays of n*dim
+ * floats): d_counts[i] = number of hits; the first max_hits hits are written as element ids
+ * (caller's element order) and line parameters t, in traversal order.                      */
+wob_status wob_intersect_lines(const wob_scene *scene, const float *d_origins, const float *d_dirs, uint32_t n,
+                               uint32_t max_hits, uint32_t *d_counts, uint32_t *d_ids, float *d_t, void *stream);
+
+/* Deterministic Laplace kernels (equation_kernel.cuh:67-145).  Per sample, out holds
+ * [G, dG_dx (dim), dG_dny, d2G_dxdny (dim), d2G_dnxdny] = 3 + 2*dim floats.                 */
+wob_status wob_eval_kernels(uint32_t dim, uint32_t n, const float *d_y, const float *d_x, const float *d_ny,
+                            const float *d_nx, float eps, float *d_out, void *stream);
+
+/* Deterministic BIE weights (bie_models.cuh) for given pairs of boundary records x, y, the
+ * inverse pdf, a domain point and the uniform that drives the first-kind splitting.
+ * Per pair, out holds [fredholm_known, fredholm_unknown, split_taken,
+ * solution_domain_unknown, gradient_domain_unknown (dim), solution_boundary_unknown].       */
+wob_status wob_eval_weights(uint32_t dim, int32_t bie_model, int32_t domain, float eps, float fkesc, uint32_t n,
+                            const float *d_x, const float *d_y, const float *d_inv_pdf, const float *d_xdomain,
+                            const float *d_u_split, float *d_out, void *stream);
+
+/* Counter-based RNG used by every estimator: Philox4x32-10 (Salmon et al., SC'11).
+ * out[4*i..] = philox(ctr[4*i..], key[2*i..]) for i < n (device arrays).                    */
+wob_status wob_philox4x32_10(uint32_t n, const uint32_t *d_ctr, const uint32_t *d_key, uint32_t *d_out, void *stream);
+
+/* String <-> enum maps of the reference (bie_models.cuh:24-51, estimator.cuh:25-54,
+ * equation_kernel.cuh:25-34).  Unknown strings return -1.                                   */
+int32_t wob_string_to_enum(const char *s);
+const char *wob_enum_to_string(const char *kind, int32_t value);
+
+const char *wob_last_error_message(void);
+const char *wob_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WOB_C_H */

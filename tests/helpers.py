@@ -1,0 +1,106 @@
+"""Shared test helpers: small scenes, random kernel inputs, ULP distances."""
+from __future__ import annotations
+
+import numpy as np
+
+from wobtoolbox_amd import geometry as G
+
+FLT_EPS = float(np.finfo(np.float32).eps)
+
+
+def ulp_distance(a, b):
+    """|a - b| in units of the FP32 spacing at max(|a|, |b|)."""
+    a = np.asarray(a, np.float32).astype(np.float64)
+    b = np.asarray(b, np.float32).astype(np.float64)
+    scale = np.maximum(np.abs(a), np.abs(b))
+    sp = np.spacing(scale.astype(np.float32)).astype(np.float64)
+    sp = np.where(sp == 0, np.finfo(np.float32).tiny, sp)
+    return np.abs(a - b) / sp
+
+
+def kernel_scales(dim, y, x):
+    """Natural magnitude of each kernel output column (the size of its terms), per sample:
+    G, |grad G| for dG_dx and dG_dny, |grad^2 G| for d2G_dxdny and d2G_dnxdny."""
+    r = np.linalg.norm(np.asarray(y, np.float64) - np.asarray(x, np.float64), axis=1)
+    if dim == 2:
+        g0 = np.maximum(1.0, np.abs(np.log(r))) / (2 * np.pi)
+        g1 = 1.0 / (2 * np.pi * r)
+        g2 = 2.0 / (2 * np.pi * r * r)
+    else:
+        g0 = 1.0 / (4 * np.pi * r)
+        g1 = 1.0 / (4 * np.pi * r * r)
+        g2 = 3.0 / (4 * np.pi * r ** 3)
+    return np.stack([g0] + [g1] * (dim + 1) + [g2] * (dim + 1), 1)
+
+
+def random_kernel_inputs(dim, n, rng, min_dist=1e-2):
+    y = rng.uniform(-1, 1, (n, dim))
+    x = rng.uniform(-1, 1, (n, dim))
+    d = np.linalg.norm(y - x, axis=1)
+    x[d < min_dist] += min_dist
+    ny = rng.normal(size=(n, dim)); ny /= np.linalg.norm(ny, axis=1, keepdims=True)
+    nx = rng.normal(size=(n, dim)); nx /= np.linalg.norm(nx, axis=1, keepdims=True)
+    return [a.astype(np.float32) for a in (y, x, ny, nx)]
+
+
+def small_circle(n=64, value=None):
+    th = 2 * np.pi * np.arange(n) / n
+    v = np.stack([np.cos(th), np.sin(th)], 1)
+    fn = (lambda p, nr, ty: np.full(len(p), value)) if value is not None else \
+        (lambda p, nr, ty: G.exact_exp_cos_2d(p)[:, 0])
+    return G.polygon_elements(v, G.DIRICHLET, fn)
+
+
+def small_icosphere(level=3, kind="neumann"):
+    V, F = G.icosphere(level)
+    V = G.displace_radial(V)
+    if kind == "neumann":
+        return G.mesh_elements(V, F, G.NEUMANN, lambda p, nr, ty: (G.exact_quadratic_3d(p)[:, 1:] * nr).sum(1)), V, F
+    cz = V[F].mean(1)[:, 2]
+    types = np.where(cz > 0, G.DIRICHLET, G.NEUMANN).astype(np.int32)
+
+    def data(p, nr, ty):
+        e = G.exact_expcos_xz_3d(p)
+        return np.where(ty == G.DIRICHLET, e[:, 0], (e[:, 1:] * nr).sum(1))
+    return G.mesh_elements(V, F, types, data), V, F
+
+
+def star_scene(seed=0, n_segments=400, kind=1):
+    """One exterior star polygon scene (kind 0 Neumann, 1 Robin, 2 mixed) with few points."""
+    ws = G.star_exterior_batch_2d(n_scenes=1, seed=seed, min_segments=n_segments, max_segments=n_segments,
+                                  min_points=64, max_points=64)
+    return ws[0]
+
+
+def boundary_records(el, n, rng):
+    """Uniform boundary points as BoundaryPoint records [p, n, 0.5, value, type, alpha]."""
+    D = el.dim
+    areas = []
+    for e in range(el.n):
+        v = el.vertices[e].astype(np.float64)
+        areas.append(np.linalg.norm(v[1] - v[0]) if D == 2 else 0.5 * np.linalg.norm(np.cross(v[1] - v[0], v[2] - v[0])))
+    areas = np.array(areas)
+    idx = rng.choice(el.n, size=n, p=areas / areas.sum())
+    rec = np.zeros((n, 2 * D + 4), np.float32)
+    for i, e in enumerate(idx):
+        v = el.vertices[e].astype(np.float64)
+        if D == 2:
+            u = rng.uniform(0.05, 0.95)
+            w = np.array([1 - u, u])
+        else:
+            a, b = rng.uniform(0.05, 0.9, 2)
+            if a + b > 0.95:
+                a, b = 0.95 - b * 0.5, b * 0.5
+            w = np.array([1 - a - b, a, b])
+        rec[i, :D] = w @ v
+        rec[i, D:2 * D] = el.normals[e]
+        rec[i, 2 * D] = 0.5
+        rec[i, 2 * D + 1] = w @ el.boundary_values[e]
+        rec[i, 2 * D + 2] = el.boundary_types[e]
+        rec[i, 2 * D + 3] = w @ el.robin_alphas[e]
+    return rec
+
+
+def mean_se(s, q, n):
+    m = s / n
+    return m, np.sqrt(np.maximum(q / n - m * m, 0) / n)

@@ -1,0 +1,173 @@
+"""ctypes binding of the CPU oracle (oracle/build/libwob_oracle.so).
+
+Test infrastructure: imported only by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, always as the checker or the CPU baseline, never as the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "build", "libwob_oracle.so")
+
+ENUMS = {
+    "IndirectSingleLayer": 0, "IndirectDoubleLayer": 1, "Direct": 2,
+    "InteriorDomain": 0, "ExteriorDomain": 1,
+    "BackwardEstimator": 0, "ForwardEstimator": 1,
+    "DirichletProblem": 0, "NeumannProblem": 1, "RobinProblem": 2, "MixedProblem": 3,
+}
+
+
+class EstDesc(C.Structure):
+    _fields_ = [("bie_model", C.c_int32), ("domain", C.c_int32), ("estimator", C.c_int32), ("problem", C.c_int32),
+                ("path_length", C.c_uint32), ("num_resampling_candidates", C.c_uint32),
+                ("num_volume_samples", C.c_uint32), ("eps", C.c_float),
+                ("first_kind_equation_scaling_constant", C.c_float), ("pick_mode", C.c_int32),
+                ("rng_mode", C.c_int32), ("use_bvh", C.c_int32)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.orc_scene_create.restype = P
+        L.orc_scene_create.argtypes = [C.c_int, C.c_int, P, P, P, P, P]
+        L.orc_scene_destroy.argtypes = [P]
+        L.orc_scene_totals.argtypes = [P, P]
+        L.orc_scene_cdfs.argtypes = [P, P, P]
+        L.orc_scene_set_volume_grid.argtypes = [P, C.c_uint, C.c_float, P]
+        L.orc_intersect_line.argtypes = [P, P, P, C.c_int, C.c_int, P, P]
+        L.orc_intersect_line.restype = C.c_int
+        L.orc_estimate_domain.argtypes = [P, C.POINTER(EstDesc), P, P, C.c_int, C.c_uint, C.c_uint64, C.c_uint64,
+                                          C.c_uint, C.c_int, P, P, P]
+        L.orc_estimate_boundary.argtypes = [P, C.POINTER(EstDesc), P, C.c_int, C.c_uint, C.c_uint64, C.c_uint64,
+                                            C.c_uint, C.c_int, P, P, P]
+        L.orc_eval_kernels_f32.argtypes = [C.c_int, C.c_int, P, P, P, P, C.c_float, P]
+        L.orc_eval_kernels_f64.argtypes = [C.c_int, C.c_int, P, P, P, P, C.c_double, P]
+        L.orc_eval_weights.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, P, P, P, P, P, P]
+        L.orc_philox4x32_10.argtypes = [P, P, P]
+        L.orc_triple32.argtypes = [C.c_uint32]
+        L.orc_triple32.restype = C.c_uint32
+        L.orc_sample_line.argtypes = [P, P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, P]
+        L.orc_sample_line.restype = C.c_float
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def est_desc(solver: dict, pick_mode=1, rng_mode=0, use_bvh=1, num_volume_samples=0) -> EstDesc:
+    return EstDesc(ENUMS[solver["bie_model"]], ENUMS[solver["domain"]], ENUMS[solver["estimator"]],
+                   ENUMS[solver["problem"]], solver["path_length"], solver.get("num_resampling_candidates", 1),
+                   num_volume_samples, solver.get("eps", float(np.finfo(np.float32).eps)),
+                   solver.get("first_kind_equation_scaling_constant", 1.0), pick_mode, rng_mode, use_bvh)
+
+
+class OracleScene:
+    def __init__(self, el):
+        self.el = el
+        self.dim = el.dim
+        self.h = lib().orc_scene_create(el.dim, el.n, _p(el.vertices), _p(el.normals), _p(el.boundary_values),
+                                        _p(el.boundary_types), _p(el.robin_alphas))
+        assert self.h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def totals(self):
+        t = np.zeros(2, np.float32)
+        lib().orc_scene_totals(self.h, _p(t))
+        return t
+
+    def cdfs(self):
+        a = np.zeros(self.el.n, np.float32)
+        b = np.zeros(self.el.n, np.float32)
+        lib().orc_scene_cdfs(self.h, _p(a), _p(b))
+        return a, b
+
+    def set_volume_grid(self, grid_res, domain_size, cache):
+        cache = np.ascontiguousarray(cache, np.float32)
+        lib().orc_scene_set_volume_grid(self.h, grid_res, domain_size, _p(cache))
+
+    def intersect(self, origin, direction, use_bvh=True, cap=256):
+        o = np.ascontiguousarray(origin, np.float32)
+        d = np.ascontiguousarray(direction, np.float32)
+        ids = np.zeros(cap, np.int32)
+        ts = np.zeros(cap, np.float32)
+        c = lib().orc_intersect_line(self.h, _p(o), _p(d), int(use_bvh), cap, _p(ids), _p(ts))
+        m = min(c, cap)
+        return c, ids[:m], ts[:m]
+
+    def estimate_domain(self, solver, points, paths, seed=0, path_base=0, stream=0, threads=0, pick_mode=1,
+                        rng_mode=0, point_index=None, return_paths=False, num_volume_samples=0):
+        pts = np.ascontiguousarray(points, np.float32)
+        n = pts.shape[0]
+        D = self.dim
+        s = np.zeros((n, D + 1), np.float64)
+        q = np.zeros((n, D + 1), np.float64)
+        pi = None if point_index is None else np.ascontiguousarray(point_index, np.uint32)
+        po = np.zeros((n, paths, D + 1), np.float32) if return_paths else None
+        d = est_desc(solver, pick_mode, rng_mode, 1, num_volume_samples)
+        lib().orc_estimate_domain(self.h, C.byref(d), _p(pts), _p(pi), n, paths, seed, path_base, stream, threads,
+                                  _p(s), _p(q), _p(po))
+        return (s, q, po) if return_paths else (s, q)
+
+    def estimate_boundary(self, solver, records, paths, seed=0, path_base=0, stream=0, threads=0, pick_mode=1,
+                          return_paths=False):
+        r = np.ascontiguousarray(records, np.float32)
+        n = r.shape[0]
+        s = np.zeros(n, np.float64)
+        q = np.zeros(n, np.float64)
+        po = np.zeros((n, paths), np.float32) if return_paths else None
+        d = est_desc(solver, pick_mode, 0, 1)
+        lib().orc_estimate_boundary(self.h, C.byref(d), _p(r), n, paths, seed, path_base, stream, threads, _p(s),
+                                    _p(q), _p(po))
+        return (s, q, po) if return_paths else (s, q)
+
+
+def eval_kernels(dim, y, x, ny, nx, eps, f64=False):
+    n = y.shape[0]
+    W = 3 + 2 * dim
+    if f64:
+        a = [np.ascontiguousarray(v, np.float64) for v in (y, x, ny, nx)]
+        out = np.zeros((n, W), np.float64)
+        lib().orc_eval_kernels_f64(dim, n, *[_p(v) for v in a], float(eps), _p(out))
+    else:
+        a = [np.ascontiguousarray(v, np.float32) for v in (y, x, ny, nx)]
+        out = np.zeros((n, W), np.float32)
+        lib().orc_eval_kernels_f32(dim, n, *[_p(v) for v in a], float(eps), _p(out))
+    return out
+
+
+def eval_weights(dim, bie, domain, eps, fkesc, xb, yb, inv_pdf, xd, u_split):
+    n = xb.shape[0]
+    out = np.zeros((n, 5 + dim), np.float32)
+    a = [np.ascontiguousarray(v, np.float32) for v in (xb, yb, inv_pdf, xd, u_split)]
+    lib().orc_eval_weights(dim, bie, domain, float(eps), float(fkesc), n, *[_p(v) for v in a], _p(out))
+    return out
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    o = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(_p(c), _p(k), _p(o))
+    return o

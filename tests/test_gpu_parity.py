@@ -1,0 +1,258 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the same inputs.
+
+Deterministic sub-kernels (Philox, Laplace kernels, BIE weights, all-hit line lists) are
+compared directly; the walk is compared per point against the oracle run with the same
+Philox streams and the same order-independent hit pick, and against the analytic
+harmonic solutions within standard errors.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import FLT_EPS, boundary_records, kernel_scales, mean_se, random_kernel_inputs, small_circle, \
+    small_icosphere, star_scene
+from wobtoolbox_amd import geometry as G
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _dev(a, gpu):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a)).to(gpu)
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_philox_device_matches_oracle_and_kat(gpu, oracle):
+    import wobtoolbox_amd as w
+    kat = json.load(open(os.path.join(HERE, "golden", "philox_kat.json")))["vectors"]
+    rng = np.random.default_rng(0)
+    ctr = rng.integers(0, 2 ** 32, (997, 4), dtype=np.uint64).astype(np.uint32)
+    key = rng.integers(0, 2 ** 32, (997, 2), dtype=np.uint64).astype(np.uint32)
+    for i, v in enumerate(kat):
+        ctr[i] = [int(x, 16) for x in v["ctr"]]
+        key[i] = [int(x, 16) for x in v["key"]]
+    out = w.philox4x32_10(_dev(ctr.view(np.int32), gpu), _dev(key.view(np.int32), gpu)).cpu().numpy().view(np.uint32)
+    for i, v in enumerate(kat):
+        assert out[i].tolist() == [int(x, 16) for x in v["out"]]
+    ref = np.stack([oracle.philox(ctr[i], key[i]) for i in range(len(ctr))])
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_kernels_device_vs_oracle(dim, gpu, oracle):
+    import wobtoolbox_amd as w
+    rng = np.random.default_rng(10 + dim)
+    y, x, ny, nx = random_kernel_inputs(dim, 50000, rng)
+    eps = FLT_EPS
+    dev = w.eval_kernels(dim, _dev(y, gpu), _dev(x, gpu), _dev(ny, gpu), _dev(nx, gpu), eps).cpu().numpy()
+    ref = oracle.eval_kernels(dim, y, x, ny, nx, eps)
+    ref64 = oracle.eval_kernels(dim, y, x, ny, nx, eps, f64=True)
+    sc = kernel_scales(dim, y, x)
+    ulp_vs_oracle = np.abs(dev.astype(np.float64) - ref) / (FLT_EPS * sc)
+    ulp_vs_exact = np.abs(dev.astype(np.float64) - ref64) / (FLT_EPS * sc)
+    _say(f"dim {dim}: device vs oracle, units of FLT_EPS x term magnitude: max {ulp_vs_oracle.max():.2f} "
+         f"q99.9 {np.quantile(ulp_vs_oracle, 0.999):.2f}; vs FP64 max {ulp_vs_exact.max():.2f}")
+    assert np.quantile(ulp_vs_oracle, 0.999) <= 2.0
+    assert ulp_vs_oracle.max() <= 8.0
+    assert ulp_vs_exact.max() <= 8.0
+
+
+def _random_records(dim, n, rng, types=(0, 1, 2)):
+    r = np.zeros((n, 2 * dim + 4), np.float32)
+    r[:, :dim] = rng.uniform(-1, 1, (n, dim))
+    nn = rng.normal(size=(n, dim)); nn /= np.linalg.norm(nn, axis=1, keepdims=True)
+    r[:, dim:2 * dim] = nn
+    r[:, 2 * dim] = 0.5
+    r[:, 2 * dim + 1] = rng.uniform(-2, 2, n)
+    r[:, 2 * dim + 2] = rng.choice(types, n)
+    r[:, 2 * dim + 3] = rng.uniform(0.5, 2, n)
+    return r
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("bie", [0, 1, 2])
+@pytest.mark.parametrize("domain", [0, 1])
+def test_weights_device_vs_oracle(dim, bie, domain, gpu, oracle):
+    import wobtoolbox_amd as w
+    rng = np.random.default_rng(100 * dim + 10 * bie + domain)
+    n = 4000
+    x = _random_records(dim, n, rng, types=(0,) if bie == 1 else (0, 1, 2))
+    y = _random_records(dim, n, rng)
+    inv = rng.uniform(0.1, 10, n).astype(np.float32)
+    xd = rng.uniform(-1, 1, (n, dim)).astype(np.float32)
+    us = rng.uniform(0, 1, n).astype(np.float32)
+    fk = 2.0
+    dev = w.eval_weights(dim, bie, domain, 1e-5, fk, _dev(x, gpu), _dev(y, gpu), _dev(inv, gpu), _dev(xd, gpu),
+                         _dev(us, gpu)).cpu().numpy()
+    ref = oracle.eval_weights(dim, bie, domain, 1e-5, fk, x, y, inv, xd, us)
+    assert np.array_equal(dev[:, 2], ref[:, 2])          # identical split decisions
+    # magnitude of each weight: |inv| x kernel term sizes
+    r = np.linalg.norm(y[:, :dim].astype(np.float64) - x[:, :dim], axis=1)
+    rd = np.linalg.norm(y[:, :dim].astype(np.float64) - xd, axis=1)
+    g = (lambda rr: np.maximum(1, np.abs(np.log(rr))) / (2 * np.pi) + 1 / (2 * np.pi * rr)) if dim == 2 else \
+        (lambda rr: 1 / (4 * np.pi * rr) + 1 / (4 * np.pi * rr * rr))
+    h = (lambda rr: 2 / (2 * np.pi * rr * rr)) if dim == 2 else (lambda rr: 3 / (4 * np.pi * rr ** 3))
+    big = 4 * (1 + fk) * inv * (1 + 2 * np.abs(y[:, 2 * dim + 3]))
+    scale = np.stack([4 * fk * (np.abs(x[:, 2 * dim + 1]) + big * g(r) * np.abs(y[:, 2 * dim + 1])) + 1,
+                      big * g(r) + 1, np.ones(n), big * g(rd)] + [big * (g(rd) + h(rd))] * dim + [big * g(r)], 1)
+    err = np.abs(dev.astype(np.float64) - ref) / (FLT_EPS * scale)
+    _say(f"dim {dim} bie {bie} domain {domain}: weights err / (FLT_EPS x term) max {err.max():.2f}")
+    assert err.max() <= 8.0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_intersections_device_vs_brute_force(dim, gpu, oracle):
+    import wobtoolbox_amd as w
+    el = G.circle_dirichlet_2d().elements if dim == 2 else G.icosphere_neumann_3d(n_points=10).elements
+    sc = oracle.OracleScene(el)
+    scene = w.SceneHost(el)
+    rng = np.random.default_rng(7 + dim)
+    n = 3000
+    o = rng.uniform(-0.9, 0.9, (n, dim)).astype(np.float32)
+    rec = boundary_records(el, n // 3, rng)
+    o[: n // 3] = rec[:, :dim]                       # a third start on the boundary, as bounces do
+    d = rng.normal(size=(n, dim)); d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d = d.astype(np.float32)
+    cnt, ids, ts = scene.intersect_lines(_dev(o, gpu), _dev(d, gpu), max_hits=32)
+    cnt, ids = cnt.cpu().numpy(), ids.cpu().numpy()
+    mism = 0
+    for i in range(n):
+        c, ri, rt = sc.intersect(o[i], d[i], use_bvh=False, cap=64)
+        if c != cnt[i] or sorted(ri.tolist()) != sorted(ids[i, :cnt[i]].tolist()):
+            mism += 1
+    _say(f"dim {dim}: {mism} of {n} lines differ in hit set")
+    assert mism <= max(2, n // 500)
+
+
+def _compare_walk(gpu, oracle, el, solver, pts, paths, seed=3, point_index=None, rel=1e-3):
+    import wobtoolbox_amd as w
+    scene = w.SceneHost(el)
+    est = w.Estimator.from_config(scene, solver)
+    s, q = est.compute_domain(_dev(pts, gpu), paths, seed=seed)
+    s, q = s.cpu().numpy(), q.cpu().numpy()
+    sc = oracle.OracleScene(el)
+    so, qo = sc.estimate_domain(solver, pts, paths, seed=seed, pick_mode=1)
+    m, se = mean_se(s, q, paths)
+    mo, seo = mean_se(so, qo, paths)
+    diff = np.abs(m - mo)
+    tol = rel * (np.abs(mo) + se + 1e-3)
+    frac = (diff > tol).mean()
+    z = diff / np.sqrt(se ** 2 + seo ** 2 + 1e-30)
+    _say(f"{solver['bie_model']} {solver['problem']} {solver['estimator']}: points differing > {rel} rel: "
+         f"{frac:.4f}; max |z| {z.max():.3f}; median rel diff {np.median(diff / (np.abs(mo) + 1e-9)):.2e}")
+    return m, se, mo, seo, frac, z
+
+
+def test_walk_matched_circle_dirichlet(gpu, oracle):
+    w0 = G.circle_dirichlet_2d(grid_stride=201)
+    m, se, mo, seo, frac, z = _compare_walk(gpu, oracle, w0.elements, w0.solver, w0.points, 256)
+    assert frac < 0.05 and z.max() < 3.0
+
+
+def test_walk_matched_icosphere_neumann(gpu, oracle):
+    w1 = G.icosphere_neumann_3d(n_points=128)
+    m, se, mo, seo, frac, z = _compare_walk(gpu, oracle, w1.elements, w1.solver, w1.points, 64)
+    assert frac < 0.10 and z.max() < 3.0
+
+
+def test_walk_matched_single_layer_exterior_ris(gpu, oracle):
+    ws = star_scene(seed=4, n_segments=300, kind=1)
+    m, se, mo, seo, frac, z = _compare_walk(gpu, oracle, ws.elements, ws.solver, ws.points[:64], 128)
+    assert frac < 0.10 and z.max() < 3.0
+
+
+def test_walk_matched_forward_single_layer_neumann(gpu, oracle):
+    th = np.arange(256) * 2 * np.pi / 256
+    v = np.stack([np.cos(th), np.sin(th)], 1)
+    el = G.polygon_elements(v, G.NEUMANN, lambda p, nr, ty: (G.exact_exp_cos_2d(p)[:, 1:] * nr).sum(1))
+    solver = dict(bie_model="IndirectSingleLayer", domain="InteriorDomain", estimator="ForwardEstimator",
+                  problem="NeumannProblem", path_length=7)
+    pts = np.random.default_rng(2).uniform(-0.5, 0.5, (64, 2)).astype(np.float32)
+    m, se, mo, seo, frac, z = _compare_walk(gpu, oracle, el, solver, pts, 256)
+    assert frac < 0.10 and z.max() < 3.0
+
+
+@pytest.mark.parametrize("bie,problem", [("IndirectDoubleLayer", "DirichletProblem"), ("Direct", "MixedProblem"),
+                                         ("IndirectSingleLayer", "MixedProblem")])
+def test_walk_matched_uniform_sampling_3d(bie, problem, gpu, oracle):
+    el, V, F = small_icosphere(3, kind="mixed")
+    solver = dict(bie_model=bie, domain="InteriorDomain", estimator="BackwardEstimator", problem=problem,
+                  path_length=4, num_resampling_candidates=1, eps=1e-5, first_kind_equation_scaling_constant=2.0)
+    if bie == "IndirectDoubleLayer":
+        el, V, F = small_icosphere(3, kind="mixed")
+        el.boundary_types[:] = 0
+    pts = G.interior_points_star(48, np.random.default_rng(3), margin=0.05).astype(np.float32)
+    m, se, mo, seo, frac, z = _compare_walk(gpu, oracle, el, solver, pts, 64)
+    assert frac < 0.10 and z.max() < 3.0
+
+
+def test_boundary_estimator_matched(gpu, oracle):
+    import wobtoolbox_amd as w
+    el, V, F = small_icosphere(3, kind="mixed")
+    solver = dict(bie_model="IndirectSingleLayer", domain="InteriorDomain", estimator="BackwardEstimator",
+                  problem="MixedProblem", path_length=4, num_resampling_candidates=16, eps=1e-5,
+                  first_kind_equation_scaling_constant=2.0)
+    rec = boundary_records(el, 96, np.random.default_rng(8))
+    scene = w.SceneHost(el)
+    est = w.Estimator.from_config(scene, solver)
+    s, q = est.compute_boundary(_dev(rec, gpu), 128, seed=5)
+    so, qo = oracle.OracleScene(el).estimate_boundary(solver, rec, 128, seed=5)
+    m, se = mean_se(s.cpu().numpy(), q.cpu().numpy(), 128)
+    mo, seo = mean_se(so, qo, 128)
+    z = np.abs(m - mo) / np.sqrt(se ** 2 + seo ** 2 + 1e-30)
+    frac = (np.abs(m - mo) > 1e-3 * (np.abs(mo) + se + 1e-3)).mean()
+    _say(f"boundary estimator: frac differing {frac:.4f} max|z| {z.max():.3f}")
+    assert frac < 0.10 and z.max() < 3.0
+    dirichlet = rec[:, 8] == 0
+    np.testing.assert_allclose(m[dirichlet], rec[dirichlet, 7], rtol=1e-5)
+
+
+def test_circle_dirichlet_full_grid_analytic(gpu):
+    import torch
+    import wobtoolbox_amd as w
+    w0 = G.circle_dirichlet_2d()
+    scene = w.SceneHost(w0.elements)
+    est = w.Estimator.from_config(scene, w0.solver)
+    s, q = est.compute_domain(_dev(w0.points, gpu), 1024, seed=11)
+    torch.cuda.synchronize()
+    m, se = mean_se(s.cpu().numpy(), q.cpu().numpy(), 1024)
+    z = (m - w0.exact(w0.points)) / se
+    rmse = np.sqrt(((m[:, 0] - w0.exact(w0.points)[:, 0]) ** 2).mean())
+    _say(f"C0 full grid: {len(m)} points, rmse u {rmse:.4e}, mean se {se[:, 0].mean():.4e}, max|z| {np.abs(z).max():.2f}")
+    assert np.abs(z).max() < 6.0
+    assert abs(rmse / np.sqrt((se[:, 0] ** 2).mean()) - 1) < 0.1
+
+
+def test_determinism_and_progressive_passes(gpu):
+    import wobtoolbox_amd as w
+    w0 = G.circle_dirichlet_2d(grid_stride=50)
+    scene = w.SceneHost(w0.elements)
+    est = w.Estimator.from_config(scene, w0.solver)
+    p = _dev(w0.points, gpu)
+    a = est.compute_domain(p, 512, seed=1)[0].cpu().numpy()
+    b = est.compute_domain(p, 512, seed=1)[0].cpu().numpy()
+    assert np.array_equal(a, b)
+    h1 = est.compute_domain(p, 256, seed=1, path_base=0)
+    h2 = est.compute_domain(p, 256, seed=1, path_base=256, out=h1)
+    np.testing.assert_allclose(h2[0].cpu().numpy(), a, rtol=1e-5, atol=1e-5)
+
+
+def test_error_paths(gpu):
+    import wobtoolbox_amd as w
+    el = small_circle(32)
+    scene = w.SceneHost(el)
+    est = w.Estimator(scene, bie_model="IndirectDoubleLayer", problem="NeumannProblem")
+    with pytest.raises(w.WobError) as e:
+        est.compute_domain(_dev(np.zeros((4, 2), np.float32), gpu), 4)
+    assert e.value.status == 3
+    est = w.Estimator(scene, bie_model="IndirectDoubleLayer", path_length=0)
+    with pytest.raises(w.WobError):
+        est.compute_domain(_dev(np.zeros((4, 2), np.float32), gpu), 4)

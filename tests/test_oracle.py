@@ -1,0 +1,251 @@
+"""CPU tests that pin the oracle (oracle/wob_oracle.cpp) before it is trusted as the checker.
+
+Each check is chosen so that a plausible mistake (a dropped term, a sign, a swapped
+argument, an index) fails: Random123 known answers for Philox, FP64 closed forms plus
+finite differences for the kernels, brute force for the BVH queries, the double-layer
+known answers on convex domains (every path returns the constant datum exactly, bounce
+weights are -1), and analytic harmonic solutions for the estimators.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from helpers import FLT_EPS, boundary_records, kernel_scales, mean_se, random_kernel_inputs, small_circle, small_icosphere, \
+    star_scene, ulp_distance
+from wobtoolbox_amd import geometry as G
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_philox_known_answers(oracle):
+    kat = json.load(open(os.path.join(HERE, "golden", "philox_kat.json")))
+    for v in kat["vectors"]:
+        ctr = [int(x, 16) for x in v["ctr"]]
+        key = [int(x, 16) for x in v["key"]]
+        out = [int(x, 16) for x in v["out"]]
+        assert oracle.philox(ctr, key).tolist() == out
+
+
+def _fd_checks(dim, rng, oracle):
+    y, x, ny, nx = [a.astype(np.float64) for a in random_kernel_inputs(dim, 64, rng, min_dist=0.2)]
+    f = lambda Y, X, NY, NX: oracle.eval_kernels(dim, Y, X, NY, NX, 1e-12, f64=True)
+    base = f(y, x, ny, nx)
+    h = 1e-6
+    for k in range(dim):
+        e = np.zeros(dim); e[k] = h
+        dxp = f(y, x + e, ny, nx); dxm = f(y, x - e, ny, nx)
+        # dG/dx_k
+        np.testing.assert_allclose((dxp[:, 0] - dxm[:, 0]) / (2 * h), base[:, 1 + k], rtol=1e-6, atol=1e-7)
+        # d/dx_k of dG/dny
+        np.testing.assert_allclose((dxp[:, 1 + dim] - dxm[:, 1 + dim]) / (2 * h), base[:, 2 + dim + k], rtol=1e-5,
+                                   atol=1e-6)
+    # dG/dny = grad_y G . ny = -dG/dx . ny  (G depends on y - x)
+    np.testing.assert_allclose(base[:, 1 + dim], -(base[:, 1:1 + dim] * ny).sum(1), rtol=1e-12, atol=1e-14)
+    # d2G/dnx dny = nx . d2G/dx dny
+    np.testing.assert_allclose(base[:, 2 + 2 * dim], (base[:, 2 + dim:2 + 2 * dim] * nx).sum(1), rtol=1e-10,
+                               atol=1e-12)
+    # textbook G
+    r = np.linalg.norm(y - x, axis=1)
+    Gt = -np.log(r) / (2 * np.pi) if dim == 2 else 1 / (4 * np.pi * r)
+    np.testing.assert_allclose(base[:, 0], Gt, rtol=1e-14)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_kernel_closed_forms(dim, oracle):
+    rng = np.random.default_rng(dim)
+    _fd_checks(dim, rng, oracle)
+    y, x, ny, nx = random_kernel_inputs(dim, 20000, rng)
+    f32 = oracle.eval_kernels(dim, y, x, ny, nx, FLT_EPS)
+    f64 = oracle.eval_kernels(dim, y, x, ny, nx, FLT_EPS, f64=True)
+    # FP32 restatement vs FP64 closed form, relative to the magnitude of the terms involved
+    err = np.abs(f32 - f64) / kernel_scales(dim, y, x)
+    assert err.max() < 8 * FLT_EPS, err.max()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_kernel_eps_clamp(dim, oracle):
+    # coincident points: G uses max(|r|, eps) (equation_kernel.cuh:75, 116), derivatives are 0
+    y = np.zeros((1, dim), np.float32); n = np.ones((1, dim), np.float32) / np.sqrt(dim)
+    out = oracle.eval_kernels(dim, y, y, n, n, 1e-3)
+    if dim == 2:
+        assert abs(out[0, 0] - (-np.log(1e-3) / (2 * np.pi))) < 1e-5
+    else:
+        assert abs(out[0, 0] - 1 / (4 * np.pi * 1e-3)) < 1e-2
+    assert np.all(out[0, 1:1 + dim] == 0)
+
+
+def _random_lines(dim, n, rng, radius=0.9):
+    o = rng.uniform(-radius, radius, (n, dim))
+    d = rng.normal(size=(n, dim)); d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return o.astype(np.float32), d.astype(np.float32)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_bvh_query_equals_brute_force(dim, oracle):
+    el = small_circle(256) if dim == 2 else small_icosphere(3)[0]
+    sc = oracle.OracleScene(el)
+    rng = np.random.default_rng(11)
+    o, d = _random_lines(dim, 400, rng)
+    for i in range(len(o)):
+        cb, ib, tb = sc.intersect(o[i], d[i], use_bvh=True)
+        cf, i_f, tf = sc.intersect(o[i], d[i], use_bvh=False)
+        assert cb == cf and sorted(ib.tolist()) == sorted(i_f.tolist())
+
+
+def test_convex_hit_counts(oracle):
+    el = small_circle(128)
+    sc = oracle.OracleScene(el)
+    rng = np.random.default_rng(3)
+    o, d = _random_lines(2, 300, rng, radius=0.5)
+    counts = [sc.intersect(o[i], d[i])[0] for i in range(len(o))]
+    assert set(counts) == {2}
+    # from a point on the boundary (segment interior) every other line hit is unique
+    rec = boundary_records(el, 200, rng)
+    counts = [sc.intersect(rec[i, :2], d[i % len(d)])[0] for i in range(len(rec))]
+    assert set(counts) == {1}
+
+
+def test_double_layer_constant_data_every_path_exact(oracle):
+    # K1: constant Dirichlet data on a convex polygon, double layer: in exact arithmetic every
+    # path returns c.  The reference predicate (scene.cuh:431, |t| > 10 FLT_EPSILON) lets a line
+    # nearly parallel to the segment holding the current point re-hit that segment (the FP32
+    # point sits ~1e-7 off it), which adds a hit in a few percent of bounces; those paths take
+    # other values but the estimator stays unbiased.  Exactness is required for the rest.
+    el = small_circle(96, value=2.5)
+    sc = oracle.OracleScene(el)
+    pts = np.array([[0.1, 0.2], [-0.5, 0.3], [0.7, -0.1]], np.float32)
+    for L in (1, 2, 5, 8):
+        solver = dict(bie_model="IndirectDoubleLayer", domain="InteriorDomain", estimator="BackwardEstimator",
+                      problem="DirichletProblem", path_length=L)
+        n = 2048
+        s, q, paths = sc.estimate_domain(solver, pts, n, seed=7, return_paths=True)
+        exact = np.abs(paths[:, :, 0] - 2.5) <= 2e-5 * 2.5
+        assert exact.mean() > 1 - 0.04 * L, (L, exact.mean())
+        m, se = mean_se(s[:, 0], q[:, 0], n)
+        assert np.all(np.abs(m - 2.5) < 5 * se + 1e-6)
+        if L == 1:
+            assert exact.all()   # no bounce: only the two-hit domain sample
+
+
+def test_double_layer_weights_known_answer(oracle):
+    # K1 weights: from a boundary point of a convex polygon the bounce weight is exactly -1;
+    # from an interior point the domain weight is +1 (estimator.cuh:253-263, bie_models.cuh:341-344, 386-387).
+    el = small_circle(64)
+    rng = np.random.default_rng(4)
+    x = boundary_records(el, 100, rng)
+    sc = oracle.OracleScene(el)
+    ys = np.zeros_like(x)
+    inv = np.zeros(len(x), np.float32)
+    for i in range(len(x)):
+        rec = np.zeros(8, np.float32)
+        inv[i] = oracle.lib().orc_sample_line(sc.h, x[i, :2].ctypes.data, 5, i, 0, 0, 1, rec.ctypes.data)
+        ys[i] = rec
+    xd = np.zeros((len(x), 2), np.float32) + np.array([0.1, -0.2], np.float32)
+    w = oracle.eval_weights(2, 1, 0, FLT_EPS, 1.0, x, ys, inv, xd, np.full(len(x), 0.5, np.float32))
+    # -1 except for the reference's self re-hit (two hits, weight magnitude 2)
+    one = np.abs(w[:, 1] + 1.0) < 1e-4
+    assert one.mean() > 0.9
+    np.testing.assert_allclose(np.abs(w[~one, 1]), 2.0, rtol=1e-4)
+    np.testing.assert_allclose(w[:, 0], 2.0 * x[:, 5], rtol=1e-6)   # fredholm_known = 2 g(x)
+
+
+def test_direct_neumann_weights_known_answer(oracle):
+    # K2: direct model, Neumann points, line sampling: fredholm_unknown = +N sign(cos)
+    el, V, F = small_icosphere(2)
+    sc = oracle.OracleScene(el)
+    rng = np.random.default_rng(9)
+    x = boundary_records(el, 200, rng)
+    ys = np.zeros_like(x)
+    inv = np.zeros(len(x), np.float32)
+    for i in range(len(x)):
+        rec = np.zeros(10, np.float32)
+        inv[i] = oracle.lib().orc_sample_line(sc.h, x[i, :3].ctypes.data, 5, i, 0, 0, 1, rec.ctypes.data)
+        ys[i] = rec
+    xd = np.zeros((len(x), 3), np.float32)
+    w = oracle.eval_weights(3, 2, 0, FLT_EPS, 1.0, x, ys, inv, xd, np.full(len(x), 0.5, np.float32))
+    r = ys[:, :3].astype(np.float64) - x[:, :3]
+    cos = (r * ys[:, 3:6]).sum(1) / np.linalg.norm(r, axis=1)
+    fu = w[:, 1]
+    ok = inv > 0
+    n_hits = np.rint(np.abs(fu[ok]))
+    assert np.all(n_hits >= 1)
+    np.testing.assert_allclose(np.abs(fu[ok]), n_hits, rtol=1e-3)
+    assert np.all(np.sign(fu[ok]) == np.sign(cos[ok]))
+
+
+def test_circle_dirichlet_unbiased(oracle):
+    w = G.circle_dirichlet_2d(grid_stride=997)
+    sc = oracle.OracleScene(w.elements)
+    n = 4096
+    s, q = sc.estimate_domain(w.solver, w.points, n, seed=1)
+    m, se = mean_se(s, q, n)
+    z = (m - w.exact(w.points)) / se
+    assert np.abs(z).max() < 5.0
+    assert abs(z.mean()) < 4.0 / np.sqrt(z.size)
+    # the reference-mode pick (64-entry buffer, pick drawn after the query) is distributionally the same
+    s2, q2 = sc.estimate_domain(w.solver, w.points, n, seed=2, pick_mode=0)
+    m2, se2 = mean_se(s2, q2, n)
+    z2 = (m2 - m) / np.sqrt(se ** 2 + se2 ** 2)
+    assert np.abs(z2).max() < 5.0
+
+
+def test_direct_neumann_unbiased(oracle):
+    el, V, F = small_icosphere(3)
+    sc = oracle.OracleScene(el)
+    pts = G.interior_points_star(40, np.random.default_rng(5), margin=0.05).astype(np.float32)
+    solver = dict(bie_model="Direct", domain="InteriorDomain", estimator="BackwardEstimator",
+                  problem="NeumannProblem", path_length=6)
+    n = 8192
+    s, q = sc.estimate_domain(solver, pts, n, seed=3)
+    m, se = mean_se(s, q, n)
+    ex = G.exact_quadratic_3d(pts)
+    d = m[:, 0] - ex[:, 0]
+    d -= d.mean()    # Neumann data fixes u up to a constant
+    assert np.abs(d / se[:, 0]).max() < 5.0
+    assert np.abs((m[:, 1:] - ex[:, 1:]) / se[:, 1:]).max() < 5.0
+
+
+def test_single_layer_exterior_ris(oracle):
+    w = star_scene(seed=4, n_segments=300, kind=1)
+    sc = oracle.OracleScene(w.elements)
+    pts = w.points[:24]
+    n = 4096
+    s, q = sc.estimate_domain(w.solver, pts, n, seed=4)
+    m, se = mean_se(s, q, n)
+    ex = w.exact(pts)
+    z = (m - ex) / se
+    # truncated series (path_length 4-7) is slightly biased; the bias stays within a few SE at this n
+    assert np.abs(z[:, 0]).max() < 5.5, np.abs(z).max()
+
+
+def test_forward_single_layer_neumann(oracle):
+    # the reference's Neumann configs use the forward estimator (data/config_general_3d_interior_Neumann.json)
+    el = small_circle(128)
+    th = np.arange(128) * 2 * np.pi / 128
+    v = np.stack([np.cos(th), np.sin(th)], 1)
+    el = G.polygon_elements(v, G.NEUMANN, lambda p, nr, ty: (G.exact_exp_cos_2d(p)[:, 1:] * nr).sum(1))
+    sc = oracle.OracleScene(el)
+    pts = np.array([[0.1, 0.2], [-0.3, 0.4], [0.5, -0.2], [0.0, 0.0]], np.float32)
+    solver = dict(bie_model="IndirectSingleLayer", domain="InteriorDomain", estimator="ForwardEstimator",
+                  problem="NeumannProblem", path_length=7)
+    n = 16384
+    s, q = sc.estimate_domain(solver, pts, n, seed=5)
+    m, se = mean_se(s, q, n)
+    ex = G.exact_exp_cos_2d(pts)
+    # gradients are free of the additive constant
+    assert np.abs((m[:, 1:] - ex[:, 1:]) / se[:, 1:]).max() < 5.5
+
+
+def test_boundary_estimator_dirichlet_returns_data(oracle):
+    # single layer / direct at a Dirichlet point return the datum (bie_models.cuh:263, 499)
+    el, V, F = small_icosphere(2, kind="mixed")
+    sc = oracle.OracleScene(el)
+    rec = boundary_records(el, 50, np.random.default_rng(1))
+    rec = rec[rec[:, 8] == 0][:10]
+    solver = dict(bie_model="IndirectSingleLayer", domain="InteriorDomain", estimator="BackwardEstimator",
+                  problem="MixedProblem", path_length=4, num_resampling_candidates=8,
+                  first_kind_equation_scaling_constant=2.0, eps=1e-5)
+    s, q = sc.estimate_boundary(solver, rec, 16, seed=1)
+    np.testing.assert_allclose(s / 16, rec[:, 7], rtol=1e-5)

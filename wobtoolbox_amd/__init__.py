@@ -1,0 +1,234 @@
+"""MI355X walk-on-boundary engine: Python mirror of the reference's Scene / Estimator API.
+
+The reference (rsugimoto/WoBToolbox) exposes, on its hot path,
+  SceneHost(elements)                                        wob/wob/scene.cuh:264-299
+  Estimator<..., BIE, Domain, Estimator, Problem>(...)       wob/wob/estimator.cuh:118-152
+  compute_sample_path_contribution_domain / _boundary        wob/wob/estimator.cuh:154-222
+  utils::run<DeviceFunctor>(args, n, bie, domain, est, prob) wob/wob/runtime_utils.cuh:105-153
+  string_to_*_type / *_type_to_string                        bie_models.cuh:24-51, estimator.cuh:25-54
+This package keeps those names and meanings; the work runs in libwob_hip.so (hand-written
+gfx950 HIP kernels behind the C ABI of include/wob_c.h).  PyTorch only provides device
+memory and streams.  There is no CPU fallback: a missing library or a missing GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _capi
+from ._capi import WobError, check, lib
+from .geometry import Elements
+
+__all__ = ["SceneHost", "Estimator", "WobError", "Elements", "string_to_bie_model_type", "string_to_domain_type",
+           "string_to_estimator_type", "string_to_problem_type", "string_to_equation_kernel_type",
+           "bie_model_type_to_string", "domain_type_to_string", "estimator_type_to_string",
+           "problem_type_to_string", "Dirichlet", "Neumann", "Robin"]
+
+Dirichlet, Neumann, Robin = 0, 1, 2
+PoissonKernel = 0
+IndirectSingleLayer, IndirectDoubleLayer, Direct = 0, 1, 2
+InteriorDomain, ExteriorDomain = 0, 1
+BackwardEstimator, ForwardEstimator = 0, 1
+DirichletProblem, NeumannProblem, RobinProblem, MixedProblem = 0, 1, 2, 3
+
+_BIE = ["IndirectSingleLayer", "IndirectDoubleLayer", "Direct"]
+_DOM = ["InteriorDomain", "ExteriorDomain"]
+_EST = ["BackwardEstimator", "ForwardEstimator"]
+_PROB = ["DirichletProblem", "NeumannProblem", "RobinProblem", "MixedProblem"]
+
+
+# string <-> enum (bie_models.cuh:24-51, estimator.cuh:25-54, equation_kernel.cuh:25-34);
+# unknown strings map to the *TypeCount value, as in the reference.
+def string_to_equation_kernel_type(s): return 0 if s == "PoissonKernel" else 1
+def string_to_bie_model_type(s): return _BIE.index(s) if s in _BIE else len(_BIE)
+def string_to_domain_type(s): return _DOM.index(s) if s in _DOM else len(_DOM)
+def string_to_estimator_type(s): return _EST.index(s) if s in _EST else len(_EST)
+def string_to_problem_type(s): return _PROB.index(s) if s in _PROB else len(_PROB)
+def bie_model_type_to_string(v): return _BIE[v] if 0 <= v < 3 else "InvalidBIEModelType"
+def domain_type_to_string(v): return _DOM[v] if 0 <= v < 2 else "InvalidDomainType"
+def estimator_type_to_string(v): return _EST[v] if 0 <= v < 2 else "InvalidEstimatorType"
+def problem_type_to_string(v): return _PROB[v] if 0 <= v < 4 else "InvalidProblemType"
+
+
+def _enum(v, conv):
+    return conv(v) if isinstance(v, str) else int(v)
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("the walk-on-boundary engine needs a GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream_ptr(stream):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+class SceneHost:
+    """Device scene: wide BVH, CDFs and element data (SceneHost, scene.cuh:264-326)."""
+
+    def __init__(self, elements: Elements):
+        _torch()
+        el = elements.astype_contiguous()
+        self.elements = el
+        self.dim = el.dim
+        d = _capi.SceneDesc(el.dim, el.n, el.vertices.ctypes.data, el.normals.ctypes.data,
+                            el.boundary_values.ctypes.data, el.boundary_types.ctypes.data,
+                            el.robin_alphas.ctypes.data)
+        h = C.c_void_p()
+        check(lib().wob_scene_create(C.byref(d), C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> dict:
+        i = _capi.SceneInfo()
+        check(lib().wob_scene_get_info(self._h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in i._fields_}
+
+    def intersect_lines(self, origins, dirs, max_hits=64, stream=None):
+        """All-hit line query: (counts, ids, t) for device tensors of lines."""
+        torch = _torch()
+        n = origins.shape[0]
+        counts = torch.zeros(n, dtype=torch.int32, device=origins.device)
+        ids = torch.zeros((n, max_hits), dtype=torch.int32, device=origins.device)
+        ts = torch.zeros((n, max_hits), dtype=torch.float32, device=origins.device)
+        check(lib().wob_intersect_lines(self._h, _ptr(origins.contiguous()), _ptr(dirs.contiguous()), n, max_hits,
+                                        _ptr(counts), _ptr(ids), _ptr(ts), _stream_ptr(stream)))
+        return counts, ids, ts
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().wob_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class Estimator:
+    """The reference's Estimator template arguments and constructor parameters."""
+
+    scene: SceneHost
+    bie_model: object = "IndirectDoubleLayer"
+    domain: object = "InteriorDomain"
+    estimator: object = "BackwardEstimator"
+    problem: object = "DirichletProblem"
+    path_length: int = 4
+    num_resampling_candidates: int = 1
+    eps: float = float(np.finfo(np.float32).eps)
+    first_kind_equation_scaling_constant: float = 1.0
+    num_volume_samples: int = 0
+    paths_per_lane: int = 0
+    block_size: int = 0
+
+    @classmethod
+    def from_config(cls, scene, cfg: dict, **overrides):
+        keys = ["bie_model", "domain", "estimator", "problem", "path_length", "num_resampling_candidates", "eps",
+                "first_kind_equation_scaling_constant", "num_volume_samples"]
+        kw = {k: cfg[k] for k in keys if k in cfg}
+        kw.update(overrides)
+        return cls(scene, **kw)
+
+    def desc(self) -> _capi.EstimatorDesc:
+        return _capi.EstimatorDesc(
+            PoissonKernel, _enum(self.bie_model, string_to_bie_model_type), _enum(self.domain, string_to_domain_type),
+            _enum(self.estimator, string_to_estimator_type), _enum(self.problem, string_to_problem_type),
+            int(self.path_length), int(self.num_resampling_candidates), int(self.num_volume_samples), float(self.eps),
+            float(self.first_kind_equation_scaling_constant))
+
+    time_walk: bool = False
+    last_launch: object = None
+
+    def _launch(self):
+        lc = _capi.LaunchConfig(int(self.paths_per_lane), int(self.block_size),
+                                _capi.LAUNCH_TIME_WALK if self.time_walk else 0, 0.0, 0, 0)
+        self.last_launch = lc
+        return lc
+
+    def compute_domain(self, points, paths_per_point, seed=0, path_base=0, point_base=0, stream_id=0, out=None,
+                       stream=None):
+        """Accumulate [u, grad u] path sums and squared sums at device points (n, dim).
+
+        Returns (sum, sumsq) FP64 tensors of shape (n, dim + 1), added into `out` if given.
+        """
+        torch = _torch()
+        pts = points.contiguous()
+        n = pts.shape[0]
+        V = self.scene.dim + 1
+        if out is None:
+            s = torch.zeros((n, V), dtype=torch.float64, device=pts.device)
+            q = torch.zeros((n, V), dtype=torch.float64, device=pts.device)
+        else:
+            s, q = out
+        d, lc = self.desc(), self._launch()
+        check(lib().wob_estimate_domain(self.scene.handle, C.byref(d), _ptr(pts), n, point_base, paths_per_point,
+                                        seed, path_base, stream_id, _ptr(s), _ptr(q), C.byref(lc),
+                                        _stream_ptr(stream)))
+        return s, q
+
+    def compute_boundary(self, records, paths_per_point, seed=0, path_base=0, point_base=0, stream_id=0, out=None,
+                         stream=None):
+        """Accumulate on-boundary path sums at BoundaryPoint records (n, 2*dim + 4)."""
+        torch = _torch()
+        rec = records.contiguous()
+        n = rec.shape[0]
+        if out is None:
+            s = torch.zeros(n, dtype=torch.float64, device=rec.device)
+            q = torch.zeros(n, dtype=torch.float64, device=rec.device)
+        else:
+            s, q = out
+        d, lc = self.desc(), self._launch()
+        check(lib().wob_estimate_boundary(self.scene.handle, C.byref(d), _ptr(rec), n, point_base, paths_per_point,
+                                          seed, path_base, stream_id, _ptr(s), _ptr(q), C.byref(lc),
+                                          _stream_ptr(stream)))
+        return s, q
+
+
+def mean_and_se(s, q, n_paths):
+    """Per-point mean and standard error from path sums (FP64)."""
+    m = s / n_paths
+    var = (q / n_paths - m * m).clamp_min(0) if hasattr(s, "clamp_min") else np.maximum(q / n_paths - m * m, 0)
+    return m, (var / n_paths) ** 0.5
+
+
+def eval_kernels(dim, y, x, ny, nx, eps, stream=None):
+    torch = _torch()
+    n = y.shape[0]
+    out = torch.zeros((n, 3 + 2 * dim), dtype=torch.float32, device=y.device)
+    check(lib().wob_eval_kernels(dim, n, _ptr(y), _ptr(x), _ptr(ny), _ptr(nx), float(eps), _ptr(out),
+                                 _stream_ptr(stream)))
+    return out
+
+
+def eval_weights(dim, bie_model, domain, eps, fkesc, x, y, inv_pdf, xdomain, u_split, stream=None):
+    torch = _torch()
+    n = x.shape[0]
+    out = torch.zeros((n, 5 + dim), dtype=torch.float32, device=x.device)
+    check(lib().wob_eval_weights(dim, _enum(bie_model, string_to_bie_model_type), _enum(domain, string_to_domain_type),
+                                 float(eps), float(fkesc), n, _ptr(x), _ptr(y), _ptr(inv_pdf), _ptr(xdomain),
+                                 _ptr(u_split), _ptr(out), _stream_ptr(stream)))
+    return out
+
+
+def philox4x32_10(ctr, key, stream=None):
+    torch = _torch()
+    n = ctr.shape[0]
+    out = torch.zeros((n, 4), dtype=torch.int32, device=ctr.device)
+    check(lib().wob_philox4x32_10(n, _ptr(ctr), _ptr(key), _ptr(out), _stream_ptr(stream)))
+    return out

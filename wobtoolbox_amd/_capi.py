@@ -1,0 +1,97 @@
+"""ctypes binding of libwob_hip.so (include/wob_c.h).
+
+The product path: there is no CPU fallback.  If the library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libwob_hip.so")
+
+WOB_OK = 0
+STATUS_NAMES = {0: "WOB_OK", 1: "WOB_INVALID_ARG", 2: "WOB_INVALID_ENUM", 3: "WOB_UNSUPPORTED_COMBO", 4: "WOB_OOM",
+                5: "WOB_DEVICE", 6: "WOB_CANCELLED"}
+
+# every symbol declared in include/wob_c.h
+EXPORTS = ["wob_scene_create", "wob_scene_destroy", "wob_scene_get_info", "wob_scene_set_volume_grid",
+           "wob_estimate_domain", "wob_estimate_boundary", "wob_intersect_lines", "wob_eval_kernels",
+           "wob_eval_weights", "wob_philox4x32_10", "wob_string_to_enum", "wob_enum_to_string",
+           "wob_last_error_message", "wob_version"]
+
+
+class WobError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("dim", C.c_uint32), ("n_elements", C.c_uint32), ("vertices", C.c_void_p), ("normals", C.c_void_p),
+                ("boundary_values", C.c_void_p), ("boundary_types", C.c_void_p), ("robin_alphas", C.c_void_p)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("dim", C.c_uint32), ("n_elements", C.c_uint32), ("n_nodes", C.c_uint32), ("depth", C.c_uint32),
+                ("total_area", C.c_float), ("total_boundary_value", C.c_float), ("device_bytes", C.c_uint64)]
+
+
+class EstimatorDesc(C.Structure):
+    _fields_ = [("equation_kernel", C.c_int32), ("bie_model", C.c_int32), ("domain", C.c_int32),
+                ("estimator", C.c_int32), ("problem", C.c_int32), ("path_length", C.c_uint32),
+                ("num_resampling_candidates", C.c_uint32), ("num_volume_samples", C.c_uint32), ("eps", C.c_float),
+                ("first_kind_equation_scaling_constant", C.c_float)]
+
+
+class LaunchConfig(C.Structure):
+    _fields_ = [("paths_per_lane", C.c_uint32), ("block_size", C.c_uint32), ("flags", C.c_uint32),
+                ("walk_ms", C.c_float), ("used_paths_per_lane", C.c_uint32), ("used_work_items", C.c_uint32)]
+
+
+LAUNCH_TIME_WALK = 1
+
+
+_lib = None
+
+
+def lib():
+    """Load libwob_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m wobtoolbox_amd.build` "
+                           "(the walk-on-boundary engine has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    P, U32, U64, I32, F = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32, C.c_float
+    sig = {
+        "wob_scene_create": ([C.POINTER(SceneDesc), C.POINTER(P)], C.c_int),
+        "wob_scene_destroy": ([P], C.c_int),
+        "wob_scene_get_info": ([P, C.POINTER(SceneInfo)], C.c_int),
+        "wob_scene_set_volume_grid": ([P, U32, F, P], C.c_int),
+        "wob_estimate_domain": ([P, C.POINTER(EstimatorDesc), P, U32, U32, U32, U64, U64, U32, P, P,
+                                 C.POINTER(LaunchConfig), P], C.c_int),
+        "wob_estimate_boundary": ([P, C.POINTER(EstimatorDesc), P, U32, U32, U32, U64, U64, U32, P, P,
+                                   C.POINTER(LaunchConfig), P], C.c_int),
+        "wob_intersect_lines": ([P, P, P, U32, U32, P, P, P, P], C.c_int),
+        "wob_eval_kernels": ([U32, U32, P, P, P, P, F, P, P], C.c_int),
+        "wob_eval_weights": ([U32, I32, I32, F, F, U32, P, P, P, P, P, P, P], C.c_int),
+        "wob_philox4x32_10": ([U32, P, P, P, P], C.c_int),
+        "wob_string_to_enum": ([C.c_char_p], I32),
+        "wob_enum_to_string": ([C.c_char_p, I32], C.c_char_p),
+        "wob_last_error_message": ([], C.c_char_p),
+        "wob_version": ([], C.c_char_p),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def check(status):
+    if status != WOB_OK:
+        msg = lib().wob_last_error_message().decode(errors="replace")
+        raise WobError(status, msg)

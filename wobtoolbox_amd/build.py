@@ -1,0 +1,73 @@
+"""Build libwob_hip.so (gfx950 HIP kernels + host scene builder + C ABI) in-tree.
+
+Every HIP translation unit is compiled with packed-FP32 instruction selection disabled
+(-Xclang -target-feature -Xclang -packed-fp32-ops); the host scene builder is compiled
+with FP contraction off so its FP32 areas and Kahan sums keep their written order.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(PKG, "csrc")
+OBJ = os.path.join(PKG, "_build")
+LIBDIR = os.path.join(PKG, "lib")
+LIB = os.path.join(LIBDIR, "libwob_hip.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+ARCH = "gfx950"
+
+HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off"]
+
+SOURCES = [
+    ("wob_kernels.hip", "hip"),
+    ("wob_capi.cpp", "hip"),
+    ("scene_build.cpp", "host"),
+]
+
+
+def _cmd(src, kind, out):
+    path = os.path.join(CSRC, src)
+    if kind == "hip":
+        return [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", path, "-o", out]
+    if kind == "hip-host":
+        return [HIPCC, "-O2", "-std=c++17", "-fPIC", "-Wall", "-D__HIP_PLATFORM_AMD__", "-c", "-x", "c++", path, "-o", out]
+    return ["g++", *HOST_FLAGS, "-c", path, "-o", out]
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    procs = []
+    objs = []
+    for src, kind in SOURCES:
+        out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+        objs.append(out)
+        cmd = _cmd(src, kind, out)
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+    errors = []
+    for src, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            errors.append(f"--- {src} ---\n{out.decode(errors='replace')}")
+        elif verbose and out:
+            print(out.decode(errors="replace"))
+    if errors:
+        raise RuntimeError("libwob_hip build failed:\n" + "\n".join(errors))
+    tmp = LIB + ".tmp"
+    link = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs]
+    r = subprocess.run(link, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("libwob_hip link failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,51 @@
+// Host-side scene construction for the MI355X walk-on-boundary engine.
+//
+// Replaces SceneHost's constructor (wob/wob/scene.cuh:264-299), which builds an lbvh binary
+// BVH over the elements and the area / boundary-value CDFs in BVH leaf order.  Here:
+//   * a binned-SAH binary BVH is built top-down and collapsed into a 4-wide BVH whose nodes
+//     hold their four child boxes as structure-of-arrays (one 128-byte line per 3D node);
+//   * primitive geometry is re-laid out in leaf order in the form the intersection test
+//     consumes (2D: a, a-b; 3D: a, a-b, a-c, (a-b)x(a-c));
+//   * element attributes stay in the caller's order, as do the CDFs (FP32 Kahan running
+//     sums, scene.cuh:271-298), each with a guide table for O(1) inverse-CDF sampling that
+//     returns exactly what binary_search (scene.cuh:30-43) returns.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/wob_c.h"
+
+namespace wobhost {
+
+// Node layouts (float words).  Child boxes are SoA; `child` words are int32: >= 0 internal
+// node index, < 0 leaf ~((first_prim << 4) | (count - 1)); `meta[0]` is the valid-child mask.
+constexpr int kNodeWords2 = 24;   // lox[4] loy[4] hix[4] hiy[4] child[4] meta[4]       (96 B)
+constexpr int kNodeWords3 = 32;   // lox[4] loy[4] loz[4] hix[4] hiy[4] hiz[4] child[4] meta[4] (128 B)
+constexpr int kPrimWords2 = 4;    // a.x a.y e.x e.y                 (e = a - b)
+constexpr int kPrimWords3 = 12;   // a.xyz e1.xyz e2.xyz N.xyz       (e1 = a - b, e2 = a - c, N = e1 x e2)
+constexpr int kAttrWords2 = 12;   // a.xy b.xy | n.xy bv0 bv1 | al0 al1 type area
+constexpr int kAttrWords3 = 20;   // a.xyz b.x | b.yz c.xy | c.z n.xyz | bv0 bv1 bv2 type | al0 al1 al2 area
+constexpr int kMaxLeaf = 8;
+
+struct HostScene {
+    uint32_t dim = 0, n = 0;
+    std::vector<float> nodes;          // n_nodes * kNodeWords
+    std::vector<float> prim_geom;      // n * kPrimWords (leaf order)
+    std::vector<uint32_t> prim_id;     // leaf order -> element id
+    std::vector<float> elem_attr;      // n * kAttrWords (element order)
+    std::vector<float> area_cdf, bv_cdf;
+    std::vector<uint32_t> area_guide, bv_guide;
+    float total_area = 0, total_bv = 0;
+    uint32_t n_nodes = 0, depth = 0, stack_need = 0;
+};
+
+// Returns an empty string on success, otherwise an error message.
+std::string build_scene(const wob_scene_desc &desc, HostScene &out);
+
+// FP32 Kahan CDF (scene.cuh:271-298) and its guide table.
+void build_cdf(const std::vector<float> &weights, std::vector<float> &cdf, float &total,
+               std::vector<uint32_t> &guide);
+
+}  // namespace wobhost

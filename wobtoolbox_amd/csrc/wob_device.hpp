@@ -1,0 +1,813 @@
+// Device code of the MI355X walk-on-boundary engine (gfx950, wave64).
+//
+// Mirrors, in FP32, the reference's hot path:
+//   equation_kernel.cuh:67-145   Laplace kernels            -> Kern<D>
+//   bie_models.cuh:53-588        BIE weights                -> Model<D, BIE>
+//   scene.cuh:441-622, 734-918   boundary samplers          -> sample_uniform / sample_bv / line_sample / ris
+//   scene.cuh:414-436, 707-730   line-element intersection  -> prim_test
+//   lbvh query_line_intersect    all-hit traversal          -> line_query (4-wide BVH, LDS stack)
+//   estimator.cuh:113-504        backward / forward walk    -> walk_backward, domain_path, boundary_path
+// Randomness is a Philox4x32-10 stream per path (key = seed, counter = (block, path, point,
+// stream)); the hit taken by a line sample is the one with the smallest
+// triple32(pick ^ id * 0x9E3779B9), which is uniform over the hits and independent of the
+// traversal order (the reference takes buf[floor(N u)] in lbvh order, scene.cuh:500-502).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wobdev {
+
+constexpr float kFltEps = 1.1920928955078125e-07f;
+constexpr float kPi = 3.14159274101257324f;   // (float)M_PI
+
+enum { kDirichlet = 0, kNeumann = 1, kRobin = 2 };
+enum { kSL = 0, kDL = 1, kDirect = 2 };
+enum { kBackward = 0, kForward = 1 };
+// sampling flavour selected from (model, problem, estimator) exactly as estimator.cuh does
+enum { FL_LINE_LINE = 0,   // backward, double layer, Dirichlet problem   (estimator.cuh:158-161, 255-263)
+       FL_LINE_BV = 1,     // backward, direct, Neumann problem           (estimator.cuh:162-164, 264-275)
+       FL_UNIFORM = 2,     // backward, everything else (SL uses RIS when nrc > 1: :276-347)
+       FL_FWD_LINE = 3,    // forward, single layer, Neumann problem      (estimator.cuh:423-424, 445-448)
+       FL_FWD_UNIFORM = 4  // forward, everything else
+};
+
+struct DevScene {
+    const float4 *nodes;
+    const float4 *prims;
+    const uint32_t *prim_id;
+    const float4 *attr;
+    const float *area_cdf;
+    const uint32_t *area_guide;
+    const float *bv_cdf;
+    const uint32_t *bv_guide;
+    uint32_t n;
+    float total_area;
+    float total_bv;
+    uint32_t pad;
+};
+
+// ----------------------------------------------------------------------------------------
+// RNG
+// ----------------------------------------------------------------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32_t k0,
+                                              uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+__device__ __forceinline__ uint32_t triple32(uint32_t x) {
+    x ^= x >> 17; x *= 0xED5AD4BBu;
+    x ^= x >> 11; x *= 0xAC4C1B51u;
+    x ^= x >> 15; x *= 0x31848BABu;
+    x ^= x >> 14;
+    return x;
+}
+
+struct Rng {
+    uint32_t k0, k1, block, path, point, stream;
+    uint32_t b0, b1, b2, b3;
+    int left;
+    __device__ __forceinline__ void init(uint64_t seed, uint32_t path_, uint32_t point_, uint32_t stream_) {
+        k0 = (uint32_t)seed; k1 = (uint32_t)(seed >> 32);
+        block = 0; path = path_; point = point_; stream = stream_;
+        left = 0;
+    }
+    __device__ __forceinline__ uint32_t next_u32() {
+        if (left == 0) {
+            uint32_t c0 = block, c1 = path, c2 = point, c3 = stream;
+            philox4x32_10(c0, c1, c2, c3, k0, k1);
+            b0 = c0; b1 = c1; b2 = c2; b3 = c3;
+            block += 1;
+            left = 4;
+        }
+        const uint32_t r = b0;
+        b0 = b1; b1 = b2; b2 = b3;
+        left -= 1;
+        return r;
+    }
+    // (0, 1] with 24 bits, like curand_uniform's range (utils.cuh:35-37)
+    __device__ __forceinline__ float uniform() { return (float)((next_u32() >> 8) + 1u) * (1.0f / 16777216.0f); }
+};
+
+// ----------------------------------------------------------------------------------------
+// small vector helpers (Eigen evaluation order: left-to-right dot, v / norm normalisation)
+// ----------------------------------------------------------------------------------------
+template <int D> __device__ __forceinline__ float dotv(const float *a, const float *b) {
+    float s = a[0] * b[0];
+#pragma unroll
+    for (int i = 1; i < D; ++i) s = s + a[i] * b[i];
+    return s;
+}
+
+template <int D> struct BP {
+    float p[D], n[D];
+    float bv;
+    float alpha;
+    int type;
+};
+
+template <int D> __device__ __forceinline__ BP<D> bp_zero() {
+    BP<D> b;
+#pragma unroll
+    for (int k = 0; k < D; ++k) { b.p[k] = 0.f; b.n[k] = 0.f; }
+    b.bv = 0.f; b.alpha = 0.f; b.type = kDirichlet;
+    return b;
+}
+
+// ----------------------------------------------------------------------------------------
+// Laplace kernels (equation_kernel.cuh:67-145).  r = y - x; rhat = r / |r| when |r| > 0.
+// ----------------------------------------------------------------------------------------
+template <int D> struct Kern;
+
+template <> struct Kern<2> {
+    float eps;
+    __device__ __forceinline__ float G(const float *y, const float *x) const {
+        const float r0 = y[0] - x[0], r1 = y[1] - x[1];
+        return -1.0f / (2.0f * kPi) * logf(fmaxf(sqrtf(r0 * r0 + r1 * r1), eps));
+    }
+    __device__ __forceinline__ void dG_dx(const float *y, const float *x, float *out) const {
+        const float r[2] = {y[0] - x[0], y[1] - x[1]};
+        const float z = dotv<2>(r, r), nr = sqrtf(z);
+        const float m = fmaxf(nr, eps);
+        const float c = 1.0f / (2.0f * kPi);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) out[k] = c * (z > 0.f ? r[k] / nr : r[k]) / m;
+    }
+    __device__ __forceinline__ float dG_dny(const float *y, const float *x, const float *ny) const {
+        const float r[2] = {y[0] - x[0], y[1] - x[1]};
+        const float z = dotv<2>(r, r), nr = sqrtf(z);
+        float rh[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) rh[k] = z > 0.f ? r[k] / nr : r[k];
+        return -1.0f / (2.0f * kPi) * dotv<2>(rh, ny) / fmaxf(nr, eps);
+    }
+    __device__ __forceinline__ void d2G_dxdny(const float *y, const float *x, const float *ny, float *out) const {
+        const float r[2] = {y[0] - x[0], y[1] - x[1]};
+        const float z = dotv<2>(r, r), nr = sqrtf(z);
+        float rh[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) rh[k] = z > 0.f ? r[k] / nr : r[k];
+        const float s = 1.0f / (2.0f * kPi * fmaxf(z, eps));
+        const float t = 2.0f * dotv<2>(rh, ny);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) out[k] = s * (ny[k] - t * rh[k]);
+    }
+    __device__ __forceinline__ float d2G_dnxdny(const float *y, const float *x, const float *ny, const float *nx) const {
+        const float r[2] = {y[0] - x[0], y[1] - x[1]};
+        const float z = dotv<2>(r, r), nr = sqrtf(z);
+        float rh[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) rh[k] = z > 0.f ? r[k] / nr : r[k];
+        return 1.0f / (2.0f * kPi * fmaxf(z, eps)) * (dotv<2>(nx, ny) - 2.0f * dotv<2>(rh, nx) * dotv<2>(rh, ny));
+    }
+};
+
+template <> struct Kern<3> {
+    float eps;
+    __device__ __forceinline__ float G(const float *y, const float *x) const {
+        const float r0 = y[0] - x[0], r1 = y[1] - x[1], r2 = y[2] - x[2];
+        return 1.0f / (4.0f * kPi * fmaxf(sqrtf(r0 * r0 + r1 * r1 + r2 * r2), eps));
+    }
+    __device__ __forceinline__ void dG_dx(const float *y, const float *x, float *out) const {
+        const float r[3] = {y[0] - x[0], y[1] - x[1], y[2] - x[2]};
+        const float z = dotv<3>(r, r), nr = sqrtf(z);
+        const float m = fmaxf(z, eps);
+        const float c = 1.0f / (4.0f * kPi);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) out[k] = c * (z > 0.f ? r[k] / nr : r[k]) / m;
+    }
+    __device__ __forceinline__ float dG_dny(const float *y, const float *x, const float *ny) const {
+        const float r[3] = {y[0] - x[0], y[1] - x[1], y[2] - x[2]};
+        const float z = dotv<3>(r, r), nr = sqrtf(z);
+        float rh[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) rh[k] = z > 0.f ? r[k] / nr : r[k];
+        return -1.0f / (4.0f * kPi) * dotv<3>(rh, ny) / fmaxf(z, eps);
+    }
+    __device__ __forceinline__ void d2G_dxdny(const float *y, const float *x, const float *ny, float *out) const {
+        const float r[3] = {y[0] - x[0], y[1] - x[1], y[2] - x[2]};
+        const float z = dotv<3>(r, r), nr = sqrtf(z);
+        float rh[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) rh[k] = z > 0.f ? r[k] / nr : r[k];
+        const float s = 1.0f / (4.0f * kPi * fmaxf(nr * z, eps));
+        const float t = 3.0f * dotv<3>(rh, ny);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) out[k] = s * (ny[k] - t * rh[k]);
+    }
+    __device__ __forceinline__ float d2G_dnxdny(const float *y, const float *x, const float *ny, const float *nx) const {
+        const float r[3] = {y[0] - x[0], y[1] - x[1], y[2] - x[2]};
+        const float z = dotv<3>(r, r), nr = sqrtf(z);
+        float rh[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) rh[k] = z > 0.f ? r[k] / nr : r[k];
+        return 1.0f / (4.0f * kPi * fmaxf(nr * z, eps)) * (dotv<3>(nx, ny) - 3.0f * dotv<3>(rh, nx) * dotv<3>(rh, ny));
+    }
+};
+
+// ----------------------------------------------------------------------------------------
+// element attributes and samplers
+// ----------------------------------------------------------------------------------------
+// Inverse CDF with a guide table: the first i with u <= cdf[i], clamped to n - 1, which is
+// what binary_search (scene.cuh:30-43) returns.
+__device__ __forceinline__ uint32_t cdf_lookup(const float *__restrict__ cdf, const uint32_t *__restrict__ guide,
+                                               uint32_t n, float u) {
+    uint32_t k = (uint32_t)(u * (float)n);
+    k = k < n ? k : n - 1;
+    uint32_t i = guide[k];
+    while (i + 1 < n && u > cdf[i]) ++i;
+    while (i > 0 && u <= cdf[i - 1]) --i;
+    return i;
+}
+
+// point on element e at parameter(s) from the sampler; fills everything but p
+template <int D> __device__ __forceinline__ void load_attr(const DevScene &S, uint32_t e, float *v /* D*D */, BP<D> &b,
+                                                          float *bvv, float *alv, float &area) {
+    if constexpr (D == 2) {
+        const float4 f0 = S.attr[3 * e], f1 = S.attr[3 * e + 1], f2 = S.attr[3 * e + 2];
+        v[0] = f0.x; v[1] = f0.y; v[2] = f0.z; v[3] = f0.w;
+        b.n[0] = f1.x; b.n[1] = f1.y;
+        bvv[0] = f1.z; bvv[1] = f1.w;
+        alv[0] = f2.x; alv[1] = f2.y;
+        b.type = (int)f2.z;
+        area = f2.w;
+    } else {
+        const float4 f0 = S.attr[5 * e], f1 = S.attr[5 * e + 1], f2 = S.attr[5 * e + 2], f3 = S.attr[5 * e + 3],
+                     f4 = S.attr[5 * e + 4];
+        v[0] = f0.x; v[1] = f0.y; v[2] = f0.z; v[3] = f0.w; v[4] = f1.x; v[5] = f1.y; v[6] = f1.z; v[7] = f1.w; v[8] = f2.x;
+        b.n[0] = f2.y; b.n[1] = f2.z; b.n[2] = f2.w;
+        bvv[0] = f3.x; bvv[1] = f3.y; bvv[2] = f3.z;
+        b.type = (int)f3.w;
+        alv[0] = f4.x; alv[1] = f4.y; alv[2] = f4.z;
+        area = f4.w;
+    }
+}
+
+template <int D> struct Sample {
+    BP<D> y;
+    float inv_pdf;
+};
+
+template <int D> __device__ __forceinline__ void uniform_in_element(const float *v, const float *bvv, const float *alv,
+                                                                   BP<D> &b, Rng &rng) {
+    if constexpr (D == 2) {
+        const float u2 = rng.uniform();
+#pragma unroll
+        for (int k = 0; k < 2; ++k) b.p[k] = v[k] + u2 * (v[2 + k] - v[k]);          // interpolate (scene.cuh:394-396)
+        b.bv = bvv[0] + u2 * (bvv[1] - bvv[0]);
+        b.alpha = alv[0] + u2 * (alv[1] - alv[0]);
+    } else {
+        const float u2 = rng.uniform();
+        const float u3 = rng.uniform();
+        const float s = sqrtf(u2);                                                     // scene.cuh:667-671
+        const float w0 = 1.0f - s, w1 = s * (1.0f - u3), w2 = u3 * s;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) b.p[k] = w0 * v[k] + w1 * v[3 + k] + w2 * v[6 + k];
+        b.bv = w0 * bvv[0] + w1 * bvv[1] + w2 * bvv[2];
+        b.alpha = w0 * alv[0] + w1 * alv[1] + w2 * alv[2];
+    }
+}
+
+// scene.cuh:441-462 / 734-758
+template <int D> __device__ __forceinline__ Sample<D> sample_uniform(const DevScene &S, Rng &rng) {
+    const float u1 = rng.uniform();
+    const uint32_t e = cdf_lookup(S.area_cdf, S.area_guide, S.n, u1);
+    float v[D * D], bvv[D], alv[D], area;
+    Sample<D> r;
+    load_attr<D>(S, e, v, r.y, bvv, alv, area);
+    uniform_in_element<D>(v, bvv, alv, r.y, rng);
+    r.inv_pdf = S.total_area;
+    return r;
+}
+
+// scene.cuh:586-622 / 891-918
+template <int D> __device__ __forceinline__ Sample<D> sample_bv(const DevScene &S, Rng &rng) {
+    const float u1 = rng.uniform();
+    const uint32_t e = cdf_lookup(S.bv_cdf, S.bv_guide, S.n, u1);
+    float v[D * D], bvv[D], alv[D], area;
+    Sample<D> r;
+    load_attr<D>(S, e, v, r.y, bvv, alv, area);
+    if constexpr (D == 2) {
+        const float u2 = rng.uniform();
+        const float ba = fabsf(bvv[0]), bb = fabsf(bvv[1]);
+        float interp;
+        if (fabsf(ba - bb) < 10 * kFltEps) interp = u2;
+        else interp = (-ba + sqrtf(ba * ba + u2 * (bb * bb - ba * ba))) / (bb - ba);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) r.y.p[k] = v[k] + interp * (v[2 + k] - v[k]);
+        r.y.bv = bvv[0] + interp * (bvv[1] - bvv[0]);
+        r.y.alpha = alv[0] + interp * (alv[1] - alv[0]);
+        r.inv_pdf = S.total_bv / (ba + interp * (bb - ba));
+    } else {
+        const float pmf = e == 0 ? S.bv_cdf[e] : S.bv_cdf[e] - S.bv_cdf[e - 1];
+        uniform_in_element<D>(v, bvv, alv, r.y, rng);
+        r.inv_pdf = area / pmf;
+    }
+    return r;
+}
+
+// ----------------------------------------------------------------------------------------
+// 4-wide BVH all-hit line query
+// ----------------------------------------------------------------------------------------
+template <int D> struct LineHit {
+    uint32_t count;
+    uint32_t id;      // element id of the picked hit
+    float t, a, b;    // line parameter and element parameters (2D: a; 3D: barycentrics of b, c)
+};
+
+// line o + t d against primitive `prim` (leaf order).  Reference predicate (scene.cuh:431-432,
+// 725-727): parameters in the closed element, |t| > 10 FLT_EPSILON, no NaN.
+template <int D> __device__ __forceinline__ bool prim_test(const DevScene &S, uint32_t prim, const float *o,
+                                                           const float *d, float &t, float &a, float &b) {
+    constexpr float eps = 10 * kFltEps;
+    if constexpr (D == 2) {
+        const float4 g = S.prims[prim];
+        const float sx = g.x - o[0], sy = g.y - o[1];
+        const float det = d[0] * g.w - d[1] * g.z;
+        const float inv = 1.0f / det;
+        t = (sx * g.w - sy * g.z) * inv;
+        a = (d[0] * sy - d[1] * sx) * inv;
+        b = 0.f;
+        return !(a < 0.0f || a > 1.0f || fabsf(t) <= eps || __builtin_isnan(t) || __builtin_isnan(a));
+    } else {
+        const float4 g0 = S.prims[3 * prim], g1 = S.prims[3 * prim + 1], g2 = S.prims[3 * prim + 2];
+        const float s0 = g0.x - o[0], s1 = g0.y - o[1], s2 = g0.z - o[2];
+        // q = d x s
+        const float q0 = d[1] * s2 - d[2] * s1, q1 = d[2] * s0 - d[0] * s2, q2 = d[0] * s1 - d[1] * s0;
+        const float det = d[0] * g2.y + d[1] * g2.z + d[2] * g2.w;       // d . (e1 x e2)
+        const float inv = 1.0f / det;
+        t = (s0 * g2.y + s1 * g2.z + s2 * g2.w) * inv;                    // s . N / det
+        a = (q0 * g1.z + q1 * g1.w + q2 * g2.x) * inv;                    // (d x s) . e2 / det
+        b = -(q0 * g0.w + q1 * g1.x + q2 * g1.y) * inv;                   // -(d x s) . e1 / det
+        return !(a < 0.0f || b < 0.0f || a + b > 1.0f || fabsf(t) <= eps || __builtin_isnan(t) || __builtin_isnan(a) ||
+                 __builtin_isnan(b));
+    }
+}
+
+__device__ __forceinline__ bool slab_hit(float tmin, float tmax) {
+    return tmin <= tmax + 1.0e-6f * (fabsf(tmax) + fabsf(tmin));
+}
+
+// Depth-first traversal of the 4-wide BVH for the infinite line o + t d.  Calls
+// on_hit(prim, t, a, b) for every primitive the reference predicate accepts.  Internal
+// children: continue with the first hit child, push the others on this lane's LDS stack
+// (`stack`, entries `stride` words apart; the builder bounds the depth needed).
+template <int D, int STACK, typename OnHit>
+__device__ __forceinline__ void traverse(const DevScene &S, const float *o, const float *d, uint32_t *stack,
+                                         uint32_t stride, OnHit &&on_hit) {
+    float inv[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) inv[k] = 1.0f / d[k];
+    int node = 0;
+    int sp = 0;
+    constexpr int NQ = D == 2 ? 6 : 8;   // float4 per node
+    while (true) {
+        const float4 *np = S.nodes + (size_t)node * NQ;
+        float4 lo[D], hi[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) { lo[k] = np[k]; hi[k] = np[D + k]; }
+        const float4 chf = np[2 * D];
+        const float4 meta = np[2 * D + 1];
+        const uint32_t valid = __float_as_uint(meta.x);
+        const int ch[4] = {(int)__float_as_uint(chf.x), (int)__float_as_uint(chf.y), (int)__float_as_uint(chf.z),
+                           (int)__float_as_uint(chf.w)};
+        uint32_t mask = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float tmin = -INFINITY, tmax = INFINITY;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const float l = c == 0 ? lo[k].x : c == 1 ? lo[k].y : c == 2 ? lo[k].z : lo[k].w;
+                const float u = c == 0 ? hi[k].x : c == 1 ? hi[k].y : c == 2 ? hi[k].z : hi[k].w;
+                const float t1 = (l - o[k]) * inv[k], t2 = (u - o[k]) * inv[k];
+                tmin = fmaxf(tmin, fminf(t1, t2));
+                tmax = fminf(tmax, fmaxf(t1, t2));
+            }
+            mask |= slab_hit(tmin, tmax) ? (1u << c) : 0u;
+        }
+        mask &= valid;
+        int next = -1;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (mask & (1u << c)) {
+                const int cc = ch[c];
+                if (cc < 0) {
+                    const uint32_t v = ~(uint32_t)cc;
+                    const uint32_t first = v >> 4, cnt = (v & 15u) + 1u;
+                    for (uint32_t i = 0; i < cnt; ++i) {
+                        float t, a, b;
+                        if (prim_test<D>(S, first + i, o, d, t, a, b)) on_hit(first + i, t, a, b);
+                    }
+                } else if (next < 0) {
+                    next = cc;
+                } else if (sp < STACK) {
+                    stack[(uint32_t)sp * stride] = (uint32_t)cc;
+                    ++sp;
+                }
+            }
+        }
+        if (next >= 0) { node = next; continue; }
+        if (sp == 0) break;
+        --sp;
+        node = (int)stack[(uint32_t)sp * stride];
+    }
+}
+
+// All hits of the line; returns the count and the hit with the smallest key
+// triple32(pick ^ id * 0x9E3779B9): a uniform choice among the hits for a uniform pick.
+template <int D, int STACK>
+__device__ __forceinline__ LineHit<D> line_query(const DevScene &S, const float *o, const float *d, uint32_t pick,
+                                                 uint32_t *stack, uint32_t stride) {
+    LineHit<D> h;
+    h.count = 0; h.id = 0xFFFFFFFFu; h.t = 0.f; h.a = 0.f; h.b = 0.f;
+    uint32_t best = 0xFFFFFFFFu;
+    traverse<D, STACK>(S, o, d, stack, stride, [&](uint32_t prim, float t, float a, float b) {
+        const uint32_t id = S.prim_id[prim];
+        const uint32_t key = triple32(pick ^ (id * 0x9E3779B9u));
+        h.count += 1;
+        if (key < best || (key == best && id < h.id)) {
+            best = key; h.id = id; h.t = t; h.a = a; h.b = b;
+        }
+    });
+    return h;
+}
+
+// scene.cuh:464-535 (2D), 760-840 (3D): a uniformly oriented line through x, all hits, one
+// hit taken uniformly, inv_pdf = N pi |r| / |rhat.n| (2D) or N 2pi |r|^2 / |rhat.n| (3D).
+template <int D, int STACK>
+__device__ __forceinline__ Sample<D> line_sample(const DevScene &S, const float *x, Rng &rng, uint32_t *stack,
+                                                 uint32_t stride) {
+    float d[D];
+    if constexpr (D == 2) {
+        const float theta = rng.uniform();
+        float s, c;
+        sincospif(theta, &s, &c);
+        d[0] = c; d[1] = s;
+    } else {
+        d[2] = rng.uniform();
+        const float phi = 2.f * rng.uniform();
+        float s, c;
+        sincospif(phi, &s, &c);
+        const float r = sqrtf(1.f - d[2] * d[2]);
+        d[0] = c * r; d[1] = s * r;
+    }
+    const uint32_t pick = rng.next_u32();
+    const LineHit<D> h = line_query<D, STACK>(S, x, d, pick, stack, stride);
+    Sample<D> r;
+    if (h.count == 0) {                                                                 // scene.cuh:488-495
+        r.y = bp_zero<D>();
+        r.inv_pdf = 0.f;
+        return r;
+    }
+    float v[D * D], bvv[D], alv[D], area;
+    load_attr<D>(S, h.id, v, r.y, bvv, alv, area);
+#pragma unroll
+    for (int k = 0; k < D; ++k) r.y.p[k] = x[k] + h.t * d[k];
+    if constexpr (D == 2) {
+        const float dx = r.y.p[0] - v[0], dy = r.y.p[1] - v[1];
+        const float interp = sqrtf(dx * dx + dy * dy) / area;                           // scene.cuh:505
+        r.y.bv = bvv[0] + interp * (bvv[1] - bvv[0]);
+        r.y.alpha = alv[0] + interp * (alv[1] - alv[0]);
+    } else {
+        const float w0 = 1.0f - (h.a + h.b);
+        r.y.bv = w0 * bvv[0] + h.a * bvv[1] + h.b * bvv[2];
+        r.y.alpha = w0 * alv[0] + h.a * alv[1] + h.b * alv[2];
+    }
+    constexpr float eps = 10 * kFltEps;
+    float rv[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) rv[k] = r.y.p[k] - x[k];
+    const float z = dotv<D>(rv, rv), nr = sqrtf(z);
+    float rh[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) rh[k] = z > 0.f ? rv[k] / nr : rv[k];
+    const float cosv = fmaxf(fabsf(dotv<D>(rh, r.y.n)), eps);
+    if constexpr (D == 2) r.inv_pdf = (float)h.count * kPi * nr / cosv;
+    else r.inv_pdf = (float)h.count * 2.f * kPi * z / cosv;
+    return r;
+}
+
+// ----------------------------------------------------------------------------------------
+// BIE models (bie_models.cuh).  interior_angle is 0.5 for every sample the scene produces
+// (scene.cuh:455, 510, 615, 750, 815, 910), so integral_free_term is 0.5 for both domains.
+// ----------------------------------------------------------------------------------------
+template <int D, int BIE> struct Model {
+    Kern<D> k;
+    float dsign;   // domain_type_coefficient (+1 interior, -1 exterior)
+    float c;       // first_kind_equation_scaling_constant
+    static constexpr float ift = 0.5f;
+
+    __device__ __forceinline__ float solution_domain_unknown(const float *x, const BP<D> &y, float inv) const {
+        if constexpr (BIE == kSL) return k.G(y.p, x) * inv;                                       // :228-231
+        else if constexpr (BIE == kDL) return -k.dG_dny(y.p, x, y.n) * inv;                       // :341-344
+        else {
+            if (y.type == kDirichlet) return dsign * inv * k.G(y.p, x);                           // :429-435
+            if (y.type == kNeumann) return -dsign * inv * k.dG_dny(y.p, x, y.n);
+            return -dsign * inv * (k.dG_dny(y.p, x, y.n) + y.alpha * k.G(y.p, x));
+        }
+    }
+    __device__ __forceinline__ void gradient_domain_unknown(const float *x, const BP<D> &y, float inv, float *g) const {
+        if constexpr (BIE == kSL) {                                                               // :240-243
+            k.dG_dx(y.p, x, g);
+#pragma unroll
+            for (int i = 0; i < D; ++i) g[i] = g[i] * inv;
+        } else if constexpr (BIE == kDL) {                                                        // :353-356
+            k.d2G_dxdny(y.p, x, y.n, g);
+#pragma unroll
+            for (int i = 0; i < D; ++i) g[i] = -(g[i] * inv);
+        } else {                                                                                  // :452-462
+            if (y.type == kDirichlet) {
+                k.dG_dx(y.p, x, g);
+#pragma unroll
+                for (int i = 0; i < D; ++i) g[i] = (dsign * inv) * g[i];
+            } else if (y.type == kNeumann) {
+                k.d2G_dxdny(y.p, x, y.n, g);
+#pragma unroll
+                for (int i = 0; i < D; ++i) g[i] = (-dsign * inv) * g[i];
+            } else {
+                float h[D];
+                k.d2G_dxdny(y.p, x, y.n, g);
+                k.dG_dx(y.p, x, h);
+#pragma unroll
+                for (int i = 0; i < D; ++i) g[i] = (-dsign * inv) * (g[i] + y.alpha * h[i]);
+            }
+        }
+    }
+    // known terms without a volume source
+    __device__ __forceinline__ float solution_domain_known(const float *x, const BP<D> &y, float inv) const {
+        if constexpr (BIE != kDirect) return 0.f;                                                 // :233-238, :346-351
+        else {
+            if (y.type == kDirichlet) return 0.f + -dsign * inv * k.dG_dny(y.p, x, y.n) * (y.bv + 0.f);   // :443-445
+            return 0.f + dsign * inv * k.G(y.p, x) * (y.bv + 0.f);                                // :446-448
+        }
+    }
+    __device__ __forceinline__ void gradient_domain_known(const float *x, const BP<D> &y, float inv, float *g) const {
+        if constexpr (BIE != kDirect) {
+#pragma unroll
+            for (int i = 0; i < D; ++i) g[i] = 0.f;
+        } else {                                                                                  // :468-477
+            if (y.type == kDirichlet) {
+                k.d2G_dxdny(y.p, x, y.n, g);
+#pragma unroll
+                for (int i = 0; i < D; ++i) g[i] = 0.f + ((-dsign * inv) * g[i]) * (y.bv + 0.f);
+            } else {
+                k.dG_dx(y.p, x, g);
+#pragma unroll
+                for (int i = 0; i < D; ++i) g[i] = 0.f + ((dsign * inv) * g[i]) * (y.bv + 0.f);
+            }
+        }
+    }
+    __device__ __forceinline__ float solution_boundary_unknown(const BP<D> &x, const BP<D> &y, float inv) const {
+        if constexpr (BIE == kSL) {                                                               // :252-257
+            if (x.type == kDirichlet) return 0.f;
+            return k.G(y.p, x.p) * inv;
+        } else if constexpr (BIE == kDL) {                                                        // :365-369
+            return 0.f;
+        } else {                                                                                  // :480-493
+            if (x.type == kDirichlet) return 0.f;
+            if (y.type == kDirichlet) return dsign / ift * inv * k.G(y.p, x.p);
+            if (y.type == kNeumann) return -dsign / ift * inv * k.dG_dny(y.p, x.p, y.n);
+            return -dsign / ift * inv * (k.dG_dny(y.p, x.p, y.n) + y.alpha * k.G(y.p, x.p));
+        }
+    }
+    __device__ __forceinline__ float solution_boundary_known(const BP<D> &x, const BP<D> &y, float inv) const {
+        if constexpr (BIE == kSL) {                                                               // :259-266
+            if (x.type == kDirichlet) return x.bv;
+            return 0.f;
+        } else if constexpr (BIE == kDL) {                                                        // :371-377
+            return x.bv;
+        } else {                                                                                  // :495-511
+            if (x.type == kDirichlet) return x.bv;
+            if (y.type == kDirichlet) return 0.f + -dsign / ift * inv * k.dG_dny(y.p, x.p, y.n) * (y.bv + 0.f);
+            return 0.f + dsign / ift * inv * k.G(y.p, x.p) * (y.bv + 0.f);
+        }
+    }
+    // Fredholm unknown after the first-kind splitting has been resolved (split not taken)
+    __device__ __forceinline__ float fredholm_unknown_kernel(const BP<D> &x, const BP<D> &y, float inv) const {
+        if constexpr (BIE == kDL) {                                                               // :386-387
+            return dsign / (1.f - ift) * inv * k.dG_dny(y.p, x.p, y.n);
+        } else if constexpr (BIE == kSL) {                                                        // :290-299
+            if (x.type == kDirichlet) return -2.f * c * inv * k.G(y.p, x.p);
+            if (x.type == kNeumann) return -dsign / ift * inv * k.dG_dny(x.p, y.p, x.n);
+            return -dsign / ift * inv * (k.dG_dny(x.p, y.p, x.n) + x.alpha * k.G(y.p, x.p));
+        } else {                                                                                  // :535-555
+            if (x.type == kDirichlet) {
+                if (y.type == kDirichlet) return -c / ift * inv * k.G(y.p, x.p);
+                if (y.type == kNeumann) return c / ift * inv * k.dG_dny(y.p, x.p, y.n);
+                return c / ift * inv * (k.dG_dny(y.p, x.p, y.n) + y.alpha * k.G(y.p, x.p));
+            }
+            if (y.type == kDirichlet) return dsign / ift * inv * k.G(y.p, x.p);
+            if (y.type == kNeumann) return -dsign / ift * inv * k.dG_dny(y.p, x.p, y.n);
+            return -dsign / ift * inv * (k.dG_dny(y.p, x.p, y.n) + y.alpha * k.G(y.p, x.p));
+        }
+    }
+    // bie_models.cuh:268-300 (SL), 379-388 (DL), 513-556 (Direct).  Backward form: may set y = x.
+    __device__ __forceinline__ float fredholm_unknown_backward(const BP<D> &x, BP<D> &y, float inv, Rng &rng) const {
+        if constexpr (BIE != kDL) {
+            if (x.type == kDirichlet) {
+                if (rng.uniform() * (1.0f + c) < 1.0f) { y = x; return 1.0f + c; }
+                inv *= (1.0f + c) / c;
+            }
+        }
+        return fredholm_unknown_kernel(x, y, inv);
+    }
+    // forward form (is_backward_sampling = false): the split looks at y and may set x = y
+    __device__ __forceinline__ float fredholm_unknown_forward(BP<D> &x, const BP<D> &y, float inv, Rng &rng) const {
+        if constexpr (BIE != kDL) {
+            if (y.type == kDirichlet) {
+                if (rng.uniform() * (1.0f + c) < 1.0f) { x = y; return 1.0f + c; }
+                inv *= (1.0f + c) / c;
+            }
+        }
+        return fredholm_unknown_kernel(x, y, inv);
+    }
+    // bie_models.cuh:301-316 (SL), 389-399 (DL), 557-587 (Direct), no volume source
+    __device__ __forceinline__ float fredholm_known(const BP<D> &x, const BP<D> &y, float inv) const {
+        if constexpr (BIE == kSL) {
+            if (x.type == kDirichlet) return 2.f * c * (x.bv + 0.f);
+            return dsign / ift * (x.bv + 0.f);
+        } else if constexpr (BIE == kDL) {
+            return dsign * 1.f / (1.f - ift) * (x.bv + 0.f);
+        } else {
+            if (x.type == kDirichlet) {
+                if (y.type == kDirichlet)
+                    return dsign * c * ((x.bv + 0.f) + dsign / ift * inv * k.dG_dny(y.p, x.p, y.n) * (y.bv + 0.f));
+                return dsign * c * ((x.bv + 0.f) - dsign / ift * inv * k.G(y.p, x.p) * (y.bv + 0.f));
+            }
+            if (y.type == kDirichlet) return -dsign / ift * inv * k.dG_dny(y.p, x.p, y.n) * (y.bv + 0.f);
+            return dsign / ift * inv * k.G(y.p, x.p) * (y.bv + 0.f);
+        }
+    }
+};
+
+// ----------------------------------------------------------------------------------------
+// Estimators (estimator.cuh)
+// ----------------------------------------------------------------------------------------
+struct WalkParams {
+    uint32_t path_length;
+    uint32_t nrc;
+};
+
+// resampled importance sampling, scene.cuh:542-584 (target by the type of y_prev,
+// estimator.cuh:286-328); candidates are uniform samples
+template <int D, int BIE>
+__device__ __forceinline__ Sample<D> ris(const DevScene &S, const Model<D, BIE> &m, const BP<D> &yp, uint32_t nrc,
+                                         Rng &rng) {
+    Sample<D> r;
+    r.y = bp_zero<D>();
+    float w_sum = 0.f;
+    const int tp = yp.type;
+    auto target = [&](const BP<D> &y) -> float {
+        if (tp == kDirichlet) return fabsf(m.k.G(y.p, yp.p));
+        if (tp == kNeumann) return fabsf(m.k.dG_dny(yp.p, y.p, yp.n));
+        return fabsf(m.k.dG_dny(yp.p, y.p, yp.n) + yp.alpha * m.k.G(y.p, yp.p));
+    };
+    for (uint32_t i = 0; i < nrc; ++i) {
+        const Sample<D> xi = sample_uniform<D>(S, rng);
+        const float w = target(xi.y) * xi.inv_pdf;
+        w_sum += w;
+        if (rng.uniform() <= w / w_sum) r.y = xi.y;
+    }
+    r.inv_pdf = w_sum / fmaxf(target(r.y) * (float)nrc, 10 * kFltEps);
+    return r;
+}
+
+// estimator.cuh:232-357
+template <int D, int BIE, int FL, int STACK>
+__device__ __forceinline__ float walk_backward(const DevScene &S, const Model<D, BIE> &m, const WalkParams &wp,
+                                               const BP<D> &x0, Rng &rng, uint32_t *stack, uint32_t stride) {
+    float spc = 0.f, W = 1.f;
+    BP<D> yp = x0;
+    for (uint32_t it = 0; it < wp.path_length; ++it) {
+        if (it == wp.path_length - 1) W = 0.5f * W;
+        if constexpr (FL == FL_LINE_LINE) {
+            Sample<D> q = line_sample<D, STACK>(S, yp.p, rng, stack, stride);
+            spc += W * m.fredholm_known(yp, q.y, q.inv_pdf);
+            W *= m.fredholm_unknown_backward(yp, q.y, q.inv_pdf, rng);
+            yp = q.y;
+        } else if constexpr (FL == FL_LINE_BV) {
+            Sample<D> q = sample_bv<D>(S, rng);
+            spc += W * m.fredholm_known(yp, q.y, q.inv_pdf);
+            q = line_sample<D, STACK>(S, yp.p, rng, stack, stride);
+            W = W * m.fredholm_unknown_backward(yp, q.y, q.inv_pdf, rng);
+            yp = q.y;
+        } else if constexpr (BIE == kSL) {
+            spc += W * m.fredholm_known(yp, yp, 0.f);   // SL ignores y here (estimator.cuh:278-281)
+            Sample<D> q = wp.nrc == 1 ? sample_uniform<D>(S, rng) : ris<D, BIE>(S, m, yp, wp.nrc, rng);
+            W = W * m.fredholm_unknown_backward(yp, q.y, q.inv_pdf, rng);
+            yp = q.y;
+        } else {
+            Sample<D> q = sample_uniform<D>(S, rng);
+            spc += W * m.fredholm_known(yp, q.y, q.inv_pdf);
+            q = sample_uniform<D>(S, rng);
+            W = W * m.fredholm_unknown_backward(yp, q.y, q.inv_pdf, rng);
+            yp = q.y;
+        }
+        if (W == 0.0f) break;
+    }
+    return spc;
+}
+
+// first samples of the backward estimators (estimator.cuh:158-168, 199-209)
+template <int D, int FL, int STACK>
+__device__ __forceinline__ void first_samples(const DevScene &S, const float *x, Sample<D> &y, Sample<D> &y1, Rng &rng,
+                                              uint32_t *stack, uint32_t stride) {
+    if constexpr (FL == FL_LINE_LINE) {
+        y = line_sample<D, STACK>(S, x, rng, stack, stride);
+        y1 = y;
+    } else if constexpr (FL == FL_LINE_BV) {
+        y = line_sample<D, STACK>(S, x, rng, stack, stride);
+        y1 = sample_bv<D>(S, rng);
+    } else {
+        y = sample_uniform<D>(S, rng);
+        y1 = sample_uniform<D>(S, rng);
+    }
+}
+
+// one domain path: out = [u, grad u]
+template <int D, int BIE, int EST, int FL, int STACK>
+__device__ __forceinline__ void domain_path(const DevScene &S, const Model<D, BIE> &m, const WalkParams &wp,
+                                            const float *x, Rng &rng, uint32_t *stack, uint32_t stride, float *out) {
+    if constexpr (EST == kBackward) {                                                          // estimator.cuh:154-191
+        Sample<D> y, y1;
+        first_samples<D, FL, STACK>(S, x, y, y1, rng, stack, stride);
+        const float known = m.solution_domain_known(x, y1.y, y1.inv_pdf);
+        float gk[D], gu[D];
+        m.gradient_domain_known(x, y1.y, y1.inv_pdf, gk);
+        const float unk = m.solution_domain_unknown(x, y.y, y.inv_pdf);
+        m.gradient_domain_unknown(x, y.y, y.inv_pdf, gu);
+        const float spc = walk_backward<D, BIE, FL, STACK>(S, m, wp, y.y, rng, stack, stride);
+        out[0] = known + spc * unk;
+#pragma unroll
+        for (int k = 0; k < D; ++k) out[1 + k] = gk[k] + spc * gu[k];
+    } else {                                                                                   // estimator.cuh:405-454
+        Sample<D> y = sample_uniform<D>(S, rng);
+        out[0] = m.solution_domain_known(x, y.y, y.inv_pdf);
+        float g[D], gu[D];
+        m.gradient_domain_known(x, y.y, y.inv_pdf, g);
+        if constexpr (FL == FL_FWD_LINE) y = sample_bv<D>(S, rng);
+        else y = sample_uniform<D>(S, rng);
+        const Sample<D> y2 = sample_uniform<D>(S, rng);
+        float W = y.inv_pdf * m.fredholm_known(y.y, y2.y, y2.inv_pdf);
+        BP<D> yc = y.y;
+        for (uint32_t it = 0; it < wp.path_length; ++it) {
+            if (it == wp.path_length - 1) W *= 0.5f;
+            out[0] += W * m.solution_domain_unknown(x, yc, 1.0f);
+            m.gradient_domain_unknown(x, yc, 1.0f, gu);
+#pragma unroll
+            for (int k = 0; k < D; ++k) g[k] += W * gu[k];
+            BP<D> yp = yc;
+            Sample<D> q;
+            if constexpr (FL == FL_FWD_LINE) q = line_sample<D, STACK>(S, yp.p, rng, stack, stride);
+            else q = sample_uniform<D>(S, rng);
+            yc = q.y;
+            W = m.fredholm_unknown_forward(yc, yp, q.inv_pdf, rng) * W;
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) out[1 + k] = g[k];
+    }
+}
+
+// one boundary path (estimator.cuh:193-222, forward :456-496)
+template <int D, int BIE, int EST, int FL, int STACK>
+__device__ __forceinline__ float boundary_path(const DevScene &S, const Model<D, BIE> &m, const WalkParams &wp,
+                                               const BP<D> &x, Rng &rng, uint32_t *stack, uint32_t stride) {
+    if constexpr (EST == kBackward) {
+        Sample<D> y, y1;
+        first_samples<D, FL, STACK>(S, x.p, y, y1, rng, stack, stride);
+        float res = m.solution_boundary_known(x, y1.y, y1.inv_pdf);
+        const float unk = m.solution_boundary_unknown(x, y.y, y.inv_pdf);
+        const float spc = walk_backward<D, BIE, FL, STACK>(S, m, wp, y.y, rng, stack, stride);
+        return res + spc * unk;
+    } else {
+        Sample<D> y = sample_uniform<D>(S, rng);
+        float res = m.solution_boundary_known(x, y.y, y.inv_pdf);
+        if constexpr (FL == FL_FWD_LINE) y = sample_bv<D>(S, rng);
+        else y = sample_uniform<D>(S, rng);
+        const Sample<D> y2 = sample_uniform<D>(S, rng);
+        float W = y.inv_pdf * m.fredholm_known(y.y, y2.y, y2.inv_pdf);
+        BP<D> yc = y.y;
+        for (uint32_t it = 0; it < wp.path_length; ++it) {
+            if (it == wp.path_length - 1) W *= 0.5f;
+            res += m.solution_boundary_unknown(x, yc, 1.0f) * W;
+            BP<D> yp = yc;
+            Sample<D> q;
+            if constexpr (FL == FL_FWD_LINE) q = line_sample<D, STACK>(S, yp.p, rng, stack, stride);
+            else q = sample_uniform<D>(S, rng);
+            yc = q.y;
+            W = m.fredholm_unknown_forward(yc, yp, q.inv_pdf, rng) * W;
+        }
+        return res;
+    }
+}
+
+}  // namespace wobdev

@@ -1,0 +1,46 @@
+"""Algorithmic work per walk path (a fixed model, independent of the implementation).
+
+Counts are FP32 operations (add, mul, compare, min/max = 1; FMA = 2; sqrt, rcp, log,
+sin/cos = 1) and compulsory record bytes for an ideal binary BVH descent:
+  per line query with k hits: 2 L k node tests (F_node ops, B_node bytes each) and 2 k
+  primitive tests (F_prim, B_prim), plus F_q of per-query work (direction, pick, inv_pdf);
+  per bounce F_w of weight / kernel evaluation; per CDF sample L + 15 compares and
+  interpolation, 4 L + B_prim bytes.
+L = ceil(log2 N_elements).  The mean hit counts k were measured once with the CPU oracle on
+20,000 random lines per origin type (seed 0) and are frozen here:
+  circle_dirichlet_2d    k = 2.000 from interior points, 1.008 from boundary points
+  icosphere_neumann_3d   k = 2.012 from interior points, 1.116 from boundary points
+"""
+import math
+
+MEASURED_K = {
+    "circle_dirichlet_2d": (2.000, 1.008),
+    "icosphere_neumann_3d": (2.012, 1.116),
+}
+
+PARAMS = {2: dict(F_node=14, B_node=32, F_prim=24, B_prim=44, F_q=20, F_w=30),
+          3: dict(F_node=20, B_node=32, F_prim=60, B_prim=76, F_q=50, F_w=35)}
+
+
+def per_path(dim, n_elements, path_length, k_interior, k_boundary, cdf_samples_per_bounce=0):
+    """(flop, bytes) per path for line-sampled walks: one domain query + path_length bounce queries."""
+    p = PARAMS[dim]
+    L = math.ceil(math.log2(n_elements))
+    sum_k = k_interior + path_length * k_boundary
+    queries = 1 + path_length
+    samples = queries * cdf_samples_per_bounce
+    flop = (2 * L * sum_k * p["F_node"] + 2 * sum_k * p["F_prim"] + queries * p["F_q"] + queries * p["F_w"]
+            + samples * (L + 15))
+    byts = 2 * L * sum_k * p["B_node"] + 2 * sum_k * p["B_prim"] + samples * (4 * L + p["B_prim"])
+    return flop, byts
+
+
+def workload_model(name, n_elements, path_length):
+    ki, kb = MEASURED_K[name]
+    if name == "icosphere_neumann_3d":
+        # direct Neumann: one boundary-value CDF sample per query (y1 and one per bounce)
+        return per_path(3, n_elements, path_length, ki, kb, cdf_samples_per_bounce=1)
+    return per_path(2, n_elements, path_length, ki, kb)
+
+
+MI355X_PEAKS = {"hbm_GBps": 8000.0, "fp32_TFLOPs": 157.3, "bf16_TFLOPs": 2500.0}
