@@ -103,7 +103,16 @@ typedef struct wob_estimator_desc {
     uint32_t num_volume_samples;
     float eps;                            /* kernel clamp, default FLT_EPSILON */
     float first_kind_equation_scaling_constant;  /* default 1 */
+    uint32_t flags;                       /* WOB_EST_* options, 0 = the reference's behaviour */
 } wob_estimator_desc;
+
+/* Estimator options (no counterpart in the reference).
+ * WOB_EST_EXCLUDE_ORIGIN_ELEMENT: a line sampled from a point on the boundary ignores the
+ *   element that point lies on.  The reference only rejects hits with |t| <= 10 FLT_EPSILON
+ *   (scene.cuh:431, 726); an FP32 point sits ~1e-7 off its own element, so a line within a
+ *   few degrees of that element re-hits it at a larger |t| and adds a spurious hit (a few
+ *   percent of bounces), which inflates the variance and makes paths sensitive to rounding. */
+enum { WOB_EST_EXCLUDE_ORIGIN_ELEMENT = 1 };
 
 /* Launch tuning and timing (in/out; may be NULL).  paths_per_lane: paths one lane
  * accumulates before it writes its partial sums; block_size: threads per workgroup
@@ -144,6 +153,44 @@ wob_status wob_estimate_boundary(const wob_scene *scene, const wob_estimator_des
                                  uint32_t n_points, uint32_t point_base, uint32_t paths_per_point, uint64_t seed,
                                  uint64_t path_base, uint32_t stream_id, double *d_sum, double *d_sumsq,
                                  wob_launch_config *launch, void *stream);
+
+/* Batched launches (no counterpart in the reference, whose apps run one scene per
+ * thrust::for_each): many (scene, estimator, point set) entries of uneven sizes run in ONE
+ * walk launch and ONE reduction launch per pass.  All entries must share the dimension and
+ * the kernel flavour (model, estimator and sampling strategy; problem types may differ when
+ * they select the same strategy, e.g. single-layer Neumann / Robin / mixed scenes with the
+ * backward estimator).  Pass p of entry e walks paths [path_base + p * paths_per_point, +
+ * paths_per_point) of every point and accumulates into its d_sum / d_sumsq.  With use_graph
+ * the pass (walk, reduction, counter update) is captured once in a hipGraph and replayed.  */
+typedef struct wob_batch wob_batch;
+typedef struct wob_batch_entry {
+    const wob_scene *scene;
+    wob_estimator_desc est;
+    const float *d_points;      /* domain points, or BoundaryPoint records when boundary != 0 */
+    uint32_t n_points;
+    uint32_t paths_per_point;
+    uint32_t point_base;
+    uint32_t stream_id;
+    uint64_t path_base;
+    double *d_sum, *d_sumsq;    /* n_points * (dim + 1) doubles each (1 for boundary batches) */
+} wob_batch_entry;
+
+typedef struct wob_batch_config {
+    uint32_t boundary;          /* 0: domain estimates, 1: on-boundary estimates */
+    uint32_t paths_per_lane;    /* 0 = automatic */
+    uint32_t block_size;        /* 0 = automatic (128) */
+    uint32_t persistent;        /* 1: resident grid, waves pull 64-item blocks from a counter */
+    uint32_t use_graph;         /* 1: capture a pass in a hipGraph */
+} wob_batch_config;
+
+wob_status wob_batch_create(const wob_batch_entry *entries, uint32_t n_entries, const wob_batch_config *cfg,
+                            wob_batch **out_batch);
+wob_status wob_batch_run(wob_batch *batch, uint64_t seed, uint32_t n_passes, void *stream);
+/* Autotuner: times one pass for each candidate (paths per lane x block size x persistent)
+ * into scratch outputs (the entries' outputs are not touched), keeps the fastest and
+ * reports it in *best (may be NULL) with its pass time in *best_ms.                          */
+wob_status wob_batch_tune(wob_batch *batch, uint64_t seed, wob_batch_config *best, float *best_ms, void *stream);
+wob_status wob_batch_destroy(wob_batch *batch);
 
 /* All-hit line query (the lbvh query_line_intersect call of scene.cuh:481-486 / 787-790,
  * without its 64-hit cap).  For each line i (origin, direction; device arrays of n*dim

@@ -164,7 +164,8 @@ template <int D> struct BPoint {
     float boundary_value;
     int boundary_type;
     float robin_alpha;
-    static BPoint zero() { BPoint b; b.p = Vec<D>::zero(); b.n = Vec<D>::zero(); b.interior_angle = 0; b.boundary_value = 0; b.boundary_type = Dirichlet; b.robin_alpha = 0; return b; }
+    int elem = -1;   // element the point lies on (-1: none / unknown); used only by exclude_origin
+    static BPoint zero() { BPoint b; b.p = Vec<D>::zero(); b.n = Vec<D>::zero(); b.interior_angle = 0; b.boundary_value = 0; b.boundary_type = Dirichlet; b.robin_alpha = 0; b.elem = -1; return b; }
 };
 
 // ------------------------------------------------------------------------------------
@@ -476,9 +477,10 @@ template <int D> struct Hit { int obj; Vec<D> p; float t; };
 // All hits of the infinite line o + t d.  use_bvh=0: brute force over every element.
 template <int D>
 static unsigned collect_hits(const Scene<D> &s, const Vec<D> &o, const Vec<D> &d, bool use_bvh, Hit<D> *buf,
-                             unsigned cap) {
+                             unsigned cap, int skip = -1) {
     unsigned count = 0;
     auto test = [&](int obj) {
+        if (obj == skip) return;
         Vec<D> p; float t;
         if (intersect(o, d, s.elems[obj], p, t)) {
             if (count < cap) buf[count] = {obj, p, t};
@@ -517,10 +519,12 @@ template <int D> struct Sampled { BPoint<D> y; float inv_pdf; };
 struct Options {
     int pick_mode = 1;      // 0: reference buffer pick, 1: matched min-hash pick
     bool use_bvh = true;
+    bool exclude_origin = false;   // ignore the element the line's origin lies on
 };
 
 template <int D> static BPoint<D> point_on_element_uniform(const Element<D> &e, Rng &rng) {
     BPoint<D> r;
+    r.elem = -1;   // the callers set the element index
     r.interior_angle = 0.5f;
     r.boundary_type = e.type;
     r.n = e.n;
@@ -544,7 +548,9 @@ template <int D> static BPoint<D> point_on_element_uniform(const Element<D> &e, 
 template <int D> static Sampled<D> sample_uniform(const Scene<D> &s, Rng &rng) {
     const float u1 = rng.uniform();
     const unsigned idx = binary_search(s.area_cdf.data(), (unsigned)s.elems.size(), u1);
-    return {point_on_element_uniform(s.elems[idx], rng), s.total_area};
+    Sampled<D> r{point_on_element_uniform(s.elems[idx], rng), s.total_area};
+    r.y.elem = (int)idx;
+    return r;
 }
 
 template <int D> static Sampled<D> sample_boundary_value(const Scene<D> &s, Rng &rng) {
@@ -564,17 +570,21 @@ template <int D> static Sampled<D> sample_boundary_value(const Scene<D> &s, Rng 
         r.boundary_value = e.bv[0] + interp * (e.bv[1] - e.bv[0]);
         r.boundary_type = e.type;
         r.robin_alpha = e.alpha[0] + interp * (e.alpha[1] - e.alpha[0]);
+        r.elem = (int)idx;
         return {r, s.total_bv / (ba + interp * (bb - ba))};
     } else {
         // scene.cuh:896-917
         const float pmf = idx == 0 ? s.bv_cdf[idx] : s.bv_cdf[idx] - s.bv_cdf[idx - 1];
         BPoint<3> r = point_on_element_uniform(e, rng);
+        r.elem = (int)idx;
         return {r, e.area() / pmf};
     }
 }
 
 // scene.cuh:464-535 (2D) and 760-840 (3D)
-template <int D> static Sampled<D> sample_line_intersection(const Scene<D> &s, const Vec<D> &x, Rng &rng, const Options &opt) {
+template <int D> static Sampled<D> sample_line_intersection(const Scene<D> &s, const Vec<D> &x, Rng &rng, const Options &opt,
+                                                            int origin_elem = -1) {
+    const int skip = opt.exclude_origin ? origin_elem : -1;
     Vec<D> dir;
     if constexpr (D == 2) {
         const float theta = rng.uniform();
@@ -599,14 +609,14 @@ template <int D> static Sampled<D> sample_line_intersection(const Scene<D> &s, c
     if (opt.pick_mode == 1) {
         // order-independent uniform pick: smallest hash of the element id
         Hit<D> all[1024];
-        count = collect_hits(s, x, dir, opt.use_bvh, all, 1024);
+        count = collect_hits(s, x, dir, opt.use_bvh, all, 1024, skip);
         uint32_t best = 0xFFFFFFFFu;
         for (unsigned i = 0; i < std::min(count, 1024u); ++i) {
             const uint32_t h = triple32(pick_bits ^ ((uint32_t)all[i].obj * 0x9E3779B9u));
             if (obj < 0 || h < best || (h == best && all[i].obj < obj)) { best = h; obj = all[i].obj; p = all[i].p; }
         }
     } else {
-        count = collect_hits(s, x, dir, opt.use_bvh, buf, kBuf);
+        count = collect_hits(s, x, dir, opt.use_bvh, buf, kBuf, skip);
         if (count > 0) {
             count = std::min(count, kBuf);
             const unsigned k = std::min(count - 1, (unsigned)(count * rng.uniform()));
@@ -623,7 +633,7 @@ template <int D> static Sampled<D> sample_line_intersection(const Scene<D> &s, c
     }
     const Element<D> &e = s.elems[obj];
     BPoint<D> r;
-    r.p = p; r.n = e.n; r.interior_angle = 0.5f; r.boundary_type = e.type;
+    r.p = p; r.n = e.n; r.interior_angle = 0.5f; r.boundary_type = e.type; r.elem = obj;
     if constexpr (D == 2) {
         const float interp = norm(p - e.v[0]) / e.area();                        // scene.cuh:505
         r.boundary_value = e.bv[0] + interp * (e.bv[1] - e.bv[0]);
@@ -896,7 +906,7 @@ template <int D> struct Estimator {
         for (unsigned it = 0; it < L; ++it) {
             if (it == L - 1) W = 0.5f * W;
             if (dl_dirichlet()) {
-                Sampled<D> q = sample_line_intersection(*s, yp.p, rng, opt);
+                Sampled<D> q = sample_line_intersection(*s, yp.p, rng, opt, yp.elem);
                 y = q.y; inv = q.inv_pdf;
                 spc += W * m.fredholm_known(yp, y, inv, rng);
                 W *= m.fredholm_unknown(yp, y, inv, rng, true);
@@ -905,7 +915,7 @@ template <int D> struct Estimator {
                 Sampled<D> q = sample_boundary_value(*s, rng);
                 y = q.y; inv = q.inv_pdf;
                 spc += W * m.fredholm_known(yp, y, inv, rng);
-                q = sample_line_intersection(*s, yp.p, rng, opt);
+                q = sample_line_intersection(*s, yp.p, rng, opt, yp.elem);
                 y = q.y; inv = q.inv_pdf;
                 W = W * m.fredholm_unknown(yp, y, inv, rng, true);
                 yp = y;
@@ -929,9 +939,9 @@ template <int D> struct Estimator {
         return spc;
     }
 
-    void first_samples(const Vec<D> &x, Sampled<D> &y, Sampled<D> &y1, Rng &rng) const {
-        if (dl_dirichlet()) { y = sample_line_intersection(*s, x, rng, opt); y1 = y; }
-        else if (direct_neumann()) { y = sample_line_intersection(*s, x, rng, opt); y1 = sample_boundary_value(*s, rng); }
+    void first_samples(const Vec<D> &x, Sampled<D> &y, Sampled<D> &y1, Rng &rng, int origin_elem = -1) const {
+        if (dl_dirichlet()) { y = sample_line_intersection(*s, x, rng, opt, origin_elem); y1 = y; }
+        else if (direct_neumann()) { y = sample_line_intersection(*s, x, rng, opt, origin_elem); y1 = sample_boundary_value(*s, rng); }
         else { y = sample_uniform(*s, rng); y1 = sample_uniform(*s, rng); }
     }
 
@@ -963,7 +973,7 @@ template <int D> struct Estimator {
             const Vec<D> gu = m.gradient_domain_unknown(x, yc, 1.0f);
             for (int k = 0; k < D; ++k) g[k] += W * gu[k];
             BPoint<D> yp = yc;
-            Sampled<D> q = sl_neumann ? sample_line_intersection(*s, yp.p, rng, opt) : sample_uniform(*s, rng);
+            Sampled<D> q = sl_neumann ? sample_line_intersection(*s, yp.p, rng, opt, yp.elem) : sample_uniform(*s, rng);
             yc = q.y;
             W = m.fredholm_unknown(yc, yp, q.inv_pdf, rng, false) * W;
         }
@@ -973,7 +983,7 @@ template <int D> struct Estimator {
     float boundary_path(const BPoint<D> &x, Rng &rng) const {
         if (est == BackwardEstimator) {                                                             // :193-222
             Sampled<D> y, y1;
-            first_samples(x.p, y, y1, rng);
+            first_samples(x.p, y, y1, rng, x.elem);
             float res = m.solution_boundary_known(x, y1.y, y1.inv_pdf, rng);
             const float spc = walk(y.y, rng);
             res += spc * m.solution_boundary_unknown(x, y.y, y.inv_pdf);
@@ -991,7 +1001,7 @@ template <int D> struct Estimator {
             if (it == L - 1) W *= 0.5f;
             res += m.solution_boundary_unknown(x, yc, 1.0f) * W;
             BPoint<D> yp = yc;
-            Sampled<D> q = sl_neumann ? sample_line_intersection(*s, yp.p, rng, opt) : sample_uniform(*s, rng);
+            Sampled<D> q = sl_neumann ? sample_line_intersection(*s, yp.p, rng, opt, yp.elem) : sample_uniform(*s, rng);
             yc = q.y;
             W = m.fredholm_unknown(yc, yp, q.inv_pdf, rng, false) * W;
         }
@@ -1016,7 +1026,7 @@ struct orc_est_desc {
     int32_t bie_model, domain, estimator, problem;
     uint32_t path_length, num_resampling_candidates, num_volume_samples;
     float eps, first_kind_equation_scaling_constant;
-    int32_t pick_mode, rng_mode, use_bvh;
+    int32_t pick_mode, rng_mode, use_bvh, exclude_origin;
 };
 
 extern "C" {
@@ -1172,6 +1182,7 @@ template <int D> static Estimator<D> make_est(const Scene<D> &s, const orc_est_d
     e.nrc = d->num_resampling_candidates;
     e.opt.pick_mode = d->pick_mode;
     e.opt.use_bvh = d->use_bvh != 0;
+    e.opt.exclude_origin = d->exclude_origin != 0;
     return e;
 }
 

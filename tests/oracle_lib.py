@@ -27,7 +27,7 @@ class EstDesc(C.Structure):
                 ("path_length", C.c_uint32), ("num_resampling_candidates", C.c_uint32),
                 ("num_volume_samples", C.c_uint32), ("eps", C.c_float),
                 ("first_kind_equation_scaling_constant", C.c_float), ("pick_mode", C.c_int32),
-                ("rng_mode", C.c_int32), ("use_bvh", C.c_int32)]
+                ("rng_mode", C.c_int32), ("use_bvh", C.c_int32), ("exclude_origin", C.c_int32)]
 
 
 def build():
@@ -73,10 +73,11 @@ def _p(a):
 
 
 def est_desc(solver: dict, pick_mode=1, rng_mode=0, use_bvh=1, num_volume_samples=0) -> EstDesc:
+    ex = int(solver.get("flags", 0)) & 1
     return EstDesc(ENUMS[solver["bie_model"]], ENUMS[solver["domain"]], ENUMS[solver["estimator"]],
                    ENUMS[solver["problem"]], solver["path_length"], solver.get("num_resampling_candidates", 1),
                    num_volume_samples, solver.get("eps", float(np.finfo(np.float32).eps)),
-                   solver.get("first_kind_equation_scaling_constant", 1.0), pick_mode, rng_mode, use_bvh)
+                   solver.get("first_kind_equation_scaling_constant", 1.0), pick_mode, rng_mode, use_bvh, ex)
 
 
 class OracleScene:

@@ -58,11 +58,13 @@ def test_kernels_device_vs_oracle(dim, gpu, oracle):
     sc = kernel_scales(dim, y, x)
     ulp_vs_oracle = np.abs(dev.astype(np.float64) - ref) / (FLT_EPS * sc)
     ulp_vs_exact = np.abs(dev.astype(np.float64) - ref64) / (FLT_EPS * sc)
-    _say(f"dim {dim}: device vs oracle, units of FLT_EPS x term magnitude: max {ulp_vs_oracle.max():.2f} "
-         f"q99.9 {np.quantile(ulp_vs_oracle, 0.999):.2f}; vs FP64 max {ulp_vs_exact.max():.2f}")
-    assert np.quantile(ulp_vs_oracle, 0.999) <= 2.0
-    assert ulp_vs_oracle.max() <= 8.0
-    assert ulp_vs_exact.max() <= 8.0
+    cols = ["G"] + [f"dG_dx{k}" for k in range(dim)] + ["dG_dny"] + [f"d2G_dxdny{k}" for k in range(dim)] + ["d2G_dnxdny"]
+    for j, c in enumerate(cols):
+        _say(f"dim {dim} {c}: vs oracle max {ulp_vs_oracle[:, j].max():.2f} bit-identical "
+             f"{(dev[:, j] == ref[:, j]).mean():.4f}; vs FP64 max {ulp_vs_exact[:, j].max():.2f}")
+    # tolerance: 2 units of FLT_EPS x term magnitude (the FP32 "ulp" of the terms involved)
+    assert ulp_vs_oracle.max() <= 2.0
+    assert ulp_vs_exact.max() <= 4.0
 
 
 def _random_records(dim, n, rng, types=(0, 1, 2)):
@@ -132,7 +134,7 @@ def test_intersections_device_vs_brute_force(dim, gpu, oracle):
     assert mism <= max(2, n // 500)
 
 
-def _compare_walk(gpu, oracle, el, solver, pts, paths, seed=3, point_index=None, rel=1e-3):
+def _compare_walk(gpu, oracle, el, solver, pts, paths, seed=3, point_index=None, rel=1e-4):
     import wobtoolbox_amd as w
     scene = w.SceneHost(el)
     est = w.Estimator.from_config(scene, solver)
@@ -151,22 +153,41 @@ def _compare_walk(gpu, oracle, el, solver, pts, paths, seed=3, point_index=None,
     return m, se, mo, seo, frac, z
 
 
+# Line-sampled walks are compared path for path with the origin element excluded (the
+# reference's self re-hit depends on the last bit of the FP32 point, so with it the two
+# sides pick different hits in a few percent of bounces); with the reference behaviour
+# they are compared statistically.
 def test_walk_matched_circle_dirichlet(gpu, oracle):
     w0 = G.circle_dirichlet_2d(grid_stride=201)
-    m, se, mo, seo, frac, z = _compare_walk(gpu, oracle, w0.elements, w0.solver, w0.points, 256)
-    assert frac < 0.05 and z.max() < 3.0
+    solver = dict(w0.solver, flags=1)
+    m, se, mo, seo, frac, z = _compare_walk(gpu, oracle, w0.elements, solver, w0.points, 256)
+    # a few paths per point may still split at segment vertices (hit sets decided by the last bit)
+    assert frac < 0.2 and z.max() < 0.5
+
+
+def test_walk_reference_mode_circle_dirichlet(gpu, oracle):
+    w0 = G.circle_dirichlet_2d(grid_stride=201)
+    m, se, mo, seo, frac, z = _compare_walk(gpu, oracle, w0.elements, w0.solver, w0.points, 1024)
+    assert z.max() < 4.5
 
 
 def test_walk_matched_icosphere_neumann(gpu, oracle):
     w1 = G.icosphere_neumann_3d(n_points=128)
-    m, se, mo, seo, frac, z = _compare_walk(gpu, oracle, w1.elements, w1.solver, w1.points, 64)
-    assert frac < 0.10 and z.max() < 3.0
+    solver = dict(w1.solver, flags=1)
+    m, se, mo, seo, frac, z = _compare_walk(gpu, oracle, w1.elements, solver, w1.points, 64)
+    assert frac < 0.05 and z.max() < 0.5
+
+
+def test_walk_reference_mode_icosphere_neumann(gpu, oracle):
+    w1 = G.icosphere_neumann_3d(n_points=128)
+    m, se, mo, seo, frac, z = _compare_walk(gpu, oracle, w1.elements, w1.solver, w1.points, 256)
+    assert z.max() < 4.5
 
 
 def test_walk_matched_single_layer_exterior_ris(gpu, oracle):
     ws = star_scene(seed=4, n_segments=300, kind=1)
     m, se, mo, seo, frac, z = _compare_walk(gpu, oracle, ws.elements, ws.solver, ws.points[:64], 128)
-    assert frac < 0.10 and z.max() < 3.0
+    assert frac < 0.05 and z.max() < 0.5
 
 
 def test_walk_matched_forward_single_layer_neumann(gpu, oracle):
@@ -174,10 +195,10 @@ def test_walk_matched_forward_single_layer_neumann(gpu, oracle):
     v = np.stack([np.cos(th), np.sin(th)], 1)
     el = G.polygon_elements(v, G.NEUMANN, lambda p, nr, ty: (G.exact_exp_cos_2d(p)[:, 1:] * nr).sum(1))
     solver = dict(bie_model="IndirectSingleLayer", domain="InteriorDomain", estimator="ForwardEstimator",
-                  problem="NeumannProblem", path_length=7)
+                  problem="NeumannProblem", path_length=7, flags=1)
     pts = np.random.default_rng(2).uniform(-0.5, 0.5, (64, 2)).astype(np.float32)
     m, se, mo, seo, frac, z = _compare_walk(gpu, oracle, el, solver, pts, 256)
-    assert frac < 0.10 and z.max() < 3.0
+    assert frac < 0.05 and z.max() < 0.5
 
 
 @pytest.mark.parametrize("bie,problem", [("IndirectDoubleLayer", "DirichletProblem"), ("Direct", "MixedProblem"),
@@ -185,13 +206,14 @@ def test_walk_matched_forward_single_layer_neumann(gpu, oracle):
 def test_walk_matched_uniform_sampling_3d(bie, problem, gpu, oracle):
     el, V, F = small_icosphere(3, kind="mixed")
     solver = dict(bie_model=bie, domain="InteriorDomain", estimator="BackwardEstimator", problem=problem,
-                  path_length=4, num_resampling_candidates=1, eps=1e-5, first_kind_equation_scaling_constant=2.0)
+                  path_length=4, num_resampling_candidates=1, eps=1e-5, first_kind_equation_scaling_constant=2.0,
+                  flags=1)
     if bie == "IndirectDoubleLayer":
         el, V, F = small_icosphere(3, kind="mixed")
         el.boundary_types[:] = 0
     pts = G.interior_points_star(48, np.random.default_rng(3), margin=0.05).astype(np.float32)
     m, se, mo, seo, frac, z = _compare_walk(gpu, oracle, el, solver, pts, 64)
-    assert frac < 0.10 and z.max() < 3.0
+    assert frac < 0.05 and z.max() < 0.5
 
 
 def test_boundary_estimator_matched(gpu, oracle):
@@ -208,27 +230,38 @@ def test_boundary_estimator_matched(gpu, oracle):
     m, se = mean_se(s.cpu().numpy(), q.cpu().numpy(), 128)
     mo, seo = mean_se(so, qo, 128)
     z = np.abs(m - mo) / np.sqrt(se ** 2 + seo ** 2 + 1e-30)
-    frac = (np.abs(m - mo) > 1e-3 * (np.abs(mo) + se + 1e-3)).mean()
+    frac = (np.abs(m - mo) > 1e-4 * (np.abs(mo) + se + 1e-3)).mean()
     _say(f"boundary estimator: frac differing {frac:.4f} max|z| {z.max():.3f}")
-    assert frac < 0.10 and z.max() < 3.0
+    assert frac < 0.05 and z.max() < 0.5
     dirichlet = rec[:, 8] == 0
     np.testing.assert_allclose(m[dirichlet], rec[dirichlet, 7], rtol=1e-5)
 
 
-def test_circle_dirichlet_full_grid_analytic(gpu):
+@pytest.mark.parametrize("flags", [0, 1])
+def test_circle_dirichlet_full_grid_analytic(flags, gpu):
+    # 256^2 grid clipped to the unit-circle 1024-gon (51,468 points) x 1024 paths, g = exp(x) cos(y)
     import torch
     import wobtoolbox_amd as w
     w0 = G.circle_dirichlet_2d()
     scene = w.SceneHost(w0.elements)
-    est = w.Estimator.from_config(scene, w0.solver)
+    est = w.Estimator.from_config(scene, w0.solver, flags=flags)
     s, q = est.compute_domain(_dev(w0.points, gpu), 1024, seed=11)
     torch.cuda.synchronize()
     m, se = mean_se(s.cpu().numpy(), q.cpu().numpy(), 1024)
-    z = (m - w0.exact(w0.points)) / se
-    rmse = np.sqrt(((m[:, 0] - w0.exact(w0.points)[:, 0]) ** 2).mean())
-    _say(f"C0 full grid: {len(m)} points, rmse u {rmse:.4e}, mean se {se[:, 0].mean():.4e}, max|z| {np.abs(z).max():.2f}")
-    assert np.abs(z).max() < 6.0
-    assert abs(rmse / np.sqrt((se[:, 0] ** 2).mean()) - 1) < 0.1
+    ex = w0.exact(w0.points)
+    z = (m - ex) / se
+    rmse = np.sqrt(((m[:, 0] - ex[:, 0]) ** 2).mean())
+    ratio = rmse / np.sqrt((se[:, 0] ** 2).mean())
+    _say(f"C0 full grid flags={flags}: {len(m)} points, rmse u {rmse:.4e}, rms se {np.sqrt((se[:, 0] ** 2).mean()):.4e}, "
+         f"max|z| {np.abs(z).max():.2f}, mean z {z.mean(0)}")
+    assert abs(ratio - 1) < 0.1
+    assert np.all(np.abs(z.mean(0)) < 5 / np.sqrt(len(z)))      # no bias at the SE / sqrt(N) level
+    # The gradient estimator is heavy-tailed next to the boundary (its double-layer weight
+    # grows like 1/r^2; the oracle shows gradient kurtosis ~10 for r > 0.985 at 1024 paths),
+    # so the max |z| bound applies to u everywhere and to grad u away from the boundary.
+    r = np.linalg.norm(w0.points, axis=1)
+    assert np.abs(z[:, 0]).max() < 6.0
+    assert np.abs(z[r < 0.9, 1:]).max() < 6.0
 
 
 def test_determinism_and_progressive_passes(gpu):
@@ -256,3 +289,54 @@ def test_error_paths(gpu):
     est = w.Estimator(scene, bie_model="IndirectDoubleLayer", path_length=0)
     with pytest.raises(w.WobError):
         est.compute_domain(_dev(np.zeros((4, 2), np.float32), gpu), 4)
+
+
+def _batch_scenes():
+    ws = [star_scene(seed=s, n_segments=n, kind=k) for s, n, k in [(1, 300, 0), (2, 900, 1), (3, 2000, 2)]]
+    return ws
+
+
+def test_batch_equals_single_calls(gpu):
+    import torch
+    import wobtoolbox_amd as w
+    ws = _batch_scenes()
+    scenes = [w.SceneHost(x.elements) for x in ws]
+    ests = [w.Estimator.from_config(sc, x.solver, paths_per_lane=8) for sc, x in zip(scenes, ws)]
+    pts = [_dev(x.points[: 40 + 17 * i], gpu) for i, x in enumerate(ws)]
+    single = [e.compute_domain(p, 64, seed=5, stream_id=i) for i, (e, p) in enumerate(zip(ests, pts))]
+    for persistent in (0, 1):
+        b = w.Batch([(e, p, 64, i) for i, (e, p) in enumerate(zip(ests, pts))], paths_per_lane=8,
+                    persistent=bool(persistent))
+        b.run(1, seed=5)
+        torch.cuda.synchronize()
+        for (s1, q1), (s2, q2) in zip(single, b.outputs):
+            assert torch.equal(s1, s2) and torch.equal(q1, q2)
+        _say(f"batch persistent={persistent}: bit-identical to single calls")
+        b.close()
+
+
+def test_batch_graph_replays_equal_progressive_calls(gpu):
+    import torch
+    import wobtoolbox_amd as w
+    ws = _batch_scenes()
+    scenes = [w.SceneHost(x.elements) for x in ws]
+    ests = [w.Estimator.from_config(sc, x.solver, paths_per_lane=16) for sc, x in zip(scenes, ws)]
+    pts = [_dev(x.points[:50], gpu) for x in ws]
+    ref = []
+    for i, (e, p) in enumerate(zip(ests, pts)):
+        out = None
+        for k in range(3):
+            out = e.compute_domain(p, 32, seed=9, stream_id=i, path_base=32 * k, out=out)
+        ref.append(out)
+    b = w.Batch([(e, p, 32, i) for i, (e, p) in enumerate(zip(ests, pts))], paths_per_lane=16, use_graph=True)
+    b.run(1, seed=9)
+    torch.cuda.synchronize()
+    b.run(2, seed=9)   # replays with eager work and a synchronisation in between
+    torch.cuda.synchronize()
+    for (s1, q1), (s2, q2) in zip(ref, b.outputs):
+        np.testing.assert_allclose(s2.cpu().numpy(), s1.cpu().numpy(), rtol=1e-12, atol=1e-12)
+    best, ms = b.tune(seed=1)
+    _say(f"autotuned batch config {best}, pass {ms:.3f} ms")
+    for (s1, q1), (s2, q2) in zip(ref, b.outputs):          # tuning leaves the outputs alone
+        np.testing.assert_allclose(s2.cpu().numpy(), s1.cpu().numpy(), rtol=1e-12, atol=1e-12)
+    b.close()

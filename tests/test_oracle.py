@@ -129,6 +129,18 @@ def test_double_layer_constant_data_every_path_exact(oracle):
             assert exact.all()   # no bounce: only the two-hit domain sample
 
 
+def test_exclude_origin_element_makes_constant_data_exact(oracle):
+    # With the origin element excluded, the self re-hit disappears and K1 holds on every path.
+    el = small_circle(96, value=2.5)
+    sc = oracle.OracleScene(el)
+    pts = np.array([[0.1, 0.2], [-0.5, 0.3], [0.7, -0.1]], np.float32)
+    for L in (2, 8):
+        solver = dict(bie_model="IndirectDoubleLayer", domain="InteriorDomain", estimator="BackwardEstimator",
+                      problem="DirichletProblem", path_length=L, flags=1)
+        s, q, paths = sc.estimate_domain(solver, pts, 2048, seed=7, return_paths=True)
+        np.testing.assert_allclose(paths[:, :, 0], 2.5, rtol=2e-5)
+
+
 def test_double_layer_weights_known_answer(oracle):
     # K1 weights: from a boundary point of a convex polygon the bounce weight is exactly -1;
     # from an interior point the domain weight is +1 (estimator.cuh:253-263, bie_models.cuh:341-344, 386-387).

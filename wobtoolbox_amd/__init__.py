@@ -21,7 +21,7 @@ from . import _capi
 from ._capi import WobError, check, lib
 from .geometry import Elements
 
-__all__ = ["SceneHost", "Estimator", "WobError", "Elements", "string_to_bie_model_type", "string_to_domain_type",
+__all__ = ["SceneHost", "Estimator", "Batch", "WobError", "Elements", "string_to_bie_model_type", "string_to_domain_type",
            "string_to_estimator_type", "string_to_problem_type", "string_to_equation_kernel_type",
            "bie_model_type_to_string", "domain_type_to_string", "estimator_type_to_string",
            "problem_type_to_string", "Dirichlet", "Neumann", "Robin"]
@@ -134,13 +134,14 @@ class Estimator:
     eps: float = float(np.finfo(np.float32).eps)
     first_kind_equation_scaling_constant: float = 1.0
     num_volume_samples: int = 0
+    flags: int = 0
     paths_per_lane: int = 0
     block_size: int = 0
 
     @classmethod
     def from_config(cls, scene, cfg: dict, **overrides):
         keys = ["bie_model", "domain", "estimator", "problem", "path_length", "num_resampling_candidates", "eps",
-                "first_kind_equation_scaling_constant", "num_volume_samples"]
+                "first_kind_equation_scaling_constant", "num_volume_samples", "flags"]
         kw = {k: cfg[k] for k in keys if k in cfg}
         kw.update(overrides)
         return cls(scene, **kw)
@@ -150,7 +151,7 @@ class Estimator:
             PoissonKernel, _enum(self.bie_model, string_to_bie_model_type), _enum(self.domain, string_to_domain_type),
             _enum(self.estimator, string_to_estimator_type), _enum(self.problem, string_to_problem_type),
             int(self.path_length), int(self.num_resampling_candidates), int(self.num_volume_samples), float(self.eps),
-            float(self.first_kind_equation_scaling_constant))
+            float(self.first_kind_equation_scaling_constant), int(self.flags))
 
     time_walk: bool = False
     last_launch: object = None
@@ -198,6 +199,57 @@ class Estimator:
                                           seed, path_base, stream_id, _ptr(s), _ptr(q), C.byref(lc),
                                           _stream_ptr(stream)))
         return s, q
+
+
+class Batch:
+    """Many (scene, estimator, point set) entries in one launch per pass (wob_batch_*)."""
+
+    def __init__(self, entries, boundary=False, paths_per_lane=0, block_size=0, persistent=False, use_graph=False):
+        """entries: list of (Estimator, device points, paths_per_point[, stream_id]).
+
+        Each entry gets FP64 (sum, sumsq) device buffers, accumulated over passes."""
+        torch = _torch()
+        self.outputs = []
+        arr = (_capi.BatchEntry * len(entries))()
+        self._keep = []
+        for i, ent in enumerate(entries):
+            est, pts, paths = ent[0], ent[1].contiguous(), int(ent[2])
+            sid = int(ent[3]) if len(ent) > 3 else i
+            V = 1 if boundary else est.scene.dim + 1
+            shape = (pts.shape[0],) if boundary else (pts.shape[0], V)
+            s = torch.zeros(shape, dtype=torch.float64, device=pts.device)
+            q = torch.zeros_like(s)
+            self.outputs.append((s, q))
+            self._keep += [pts, est]
+            arr[i] = _capi.BatchEntry(est.scene.handle, est.desc(), pts.data_ptr(), pts.shape[0], paths, 0, sid, 0,
+                                      s.data_ptr(), q.data_ptr())
+        cfg = _capi.BatchConfig(int(boundary), paths_per_lane, block_size, int(persistent), int(use_graph))
+        h = C.c_void_p()
+        check(lib().wob_batch_create(arr, len(entries), C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.paths = [int(e[2]) for e in entries]
+        self.passes = 0
+
+    def run(self, n_passes=1, seed=0, stream=None):
+        check(lib().wob_batch_run(self._h, seed, n_passes, _stream_ptr(stream)))
+        self.passes += n_passes
+
+    def tune(self, seed=0, stream=None):
+        best = _capi.BatchConfig()
+        ms = C.c_float()
+        check(lib().wob_batch_tune(self._h, seed, C.byref(best), C.byref(ms), _stream_ptr(stream)))
+        return {f: getattr(best, f) for f, _ in best._fields_}, ms.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().wob_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def mean_and_se(s, q, n_paths):

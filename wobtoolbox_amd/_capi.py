@@ -18,7 +18,8 @@ STATUS_NAMES = {0: "WOB_OK", 1: "WOB_INVALID_ARG", 2: "WOB_INVALID_ENUM", 3: "WO
 EXPORTS = ["wob_scene_create", "wob_scene_destroy", "wob_scene_get_info", "wob_scene_set_volume_grid",
            "wob_estimate_domain", "wob_estimate_boundary", "wob_intersect_lines", "wob_eval_kernels",
            "wob_eval_weights", "wob_philox4x32_10", "wob_string_to_enum", "wob_enum_to_string",
-           "wob_last_error_message", "wob_version"]
+           "wob_last_error_message", "wob_version", "wob_batch_create", "wob_batch_run", "wob_batch_tune",
+           "wob_batch_destroy"]
 
 
 class WobError(RuntimeError):
@@ -41,7 +42,21 @@ class EstimatorDesc(C.Structure):
     _fields_ = [("equation_kernel", C.c_int32), ("bie_model", C.c_int32), ("domain", C.c_int32),
                 ("estimator", C.c_int32), ("problem", C.c_int32), ("path_length", C.c_uint32),
                 ("num_resampling_candidates", C.c_uint32), ("num_volume_samples", C.c_uint32), ("eps", C.c_float),
-                ("first_kind_equation_scaling_constant", C.c_float)]
+                ("first_kind_equation_scaling_constant", C.c_float), ("flags", C.c_uint32)]
+
+
+EST_EXCLUDE_ORIGIN_ELEMENT = 1
+
+
+class BatchEntry(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("est", EstimatorDesc), ("d_points", C.c_void_p), ("n_points", C.c_uint32),
+                ("paths_per_point", C.c_uint32), ("point_base", C.c_uint32), ("stream_id", C.c_uint32),
+                ("path_base", C.c_uint64), ("d_sum", C.c_void_p), ("d_sumsq", C.c_void_p)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("boundary", C.c_uint32), ("paths_per_lane", C.c_uint32), ("block_size", C.c_uint32),
+                ("persistent", C.c_uint32), ("use_graph", C.c_uint32)]
 
 
 class LaunchConfig(C.Structure):
@@ -82,6 +97,10 @@ def lib():
         "wob_enum_to_string": ([C.c_char_p, I32], C.c_char_p),
         "wob_last_error_message": ([], C.c_char_p),
         "wob_version": ([], C.c_char_p),
+        "wob_batch_create": ([C.POINTER(BatchEntry), U32, C.POINTER(BatchConfig), C.POINTER(P)], C.c_int),
+        "wob_batch_run": ([P, U64, U32, P], C.c_int),
+        "wob_batch_tune": ([P, U64, C.POINTER(BatchConfig), C.POINTER(C.c_float), P], C.c_int),
+        "wob_batch_destroy": ([P], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/wob_c.h"
 #include "scene_build.hpp"
@@ -90,6 +91,7 @@ wob_status check_est(const wob_scene *scene, const wob_estimator_desc *est, int 
     if (est->estimator < 0 || est->estimator > 1) return fail(WOB_INVALID_ENUM, "estimator out of range");
     if (est->problem < 0 || est->problem > 3) return fail(WOB_INVALID_ENUM, "problem out of range");
     if (est->path_length == 0) return fail(WOB_INVALID_ARG, "path_length must be >= 1");
+    if (est->flags & ~(uint32_t)WOB_EST_EXCLUDE_ORIGIN_ELEMENT) return fail(WOB_INVALID_ARG, "unknown estimator flags");
     if (est->num_resampling_candidates == 0) return fail(WOB_INVALID_ARG, "num_resampling_candidates must be >= 1");
     if (est->num_volume_samples != 0)
         return fail(WOB_UNSUPPORTED_COMBO, "volume sources are not enabled in this build (num_volume_samples must be 0)");
@@ -115,6 +117,47 @@ wob_status check_est(const wob_scene *scene, const wob_estimator_desc *est, int 
     return WOB_OK;
 }
 
+uint32_t auto_paths_per_lane(double total_paths) {
+    // about a million work items keeps every CU busy with room for load balancing
+    return (uint32_t)std::max(1.0, std::ceil(total_paths / (1u << 20)));
+}
+
+// chunking of one entry: paths per chunk (balanced) and chunks per point
+void chunking(uint32_t paths, uint32_t ppl, uint32_t &paths_per_chunk, uint32_t &chunks) {
+    ppl = std::max(1u, std::min(ppl, paths));
+    chunks = (paths + ppl - 1) / ppl;
+    paths_per_chunk = (paths + chunks - 1) / chunks;
+    chunks = (paths + paths_per_chunk - 1) / paths_per_chunk;
+}
+
+wobl::WalkEntry make_entry(const wob_scene *scene, const wob_estimator_desc *est, const float *points,
+                           uint32_t n_points, uint32_t paths, uint32_t ppl, uint32_t point_base, uint32_t stream_id,
+                           uint64_t path_base, double *sum, double *sumsq) {
+    wobl::WalkEntry e{};
+    e.scene = scene->dev;
+    e.points = points;
+    e.sum = sum;
+    e.sumsq = sumsq;
+    e.n_points = n_points;
+    chunking(paths, ppl, e.paths_per_chunk, e.chunks);
+    e.paths_per_point = paths;
+    e.point_base = point_base;
+    e.stream = stream_id;
+    e.path_length = est->path_length;
+    e.nrc = est->num_resampling_candidates;
+    e.dsign = est->domain == WOB_INTERIOR ? 1.0f : -1.0f;
+    e.eps = est->eps;
+    e.fkesc = est->first_kind_equation_scaling_constant;
+    e.path_base = path_base;
+    e.flags = est->flags;
+    return e;
+}
+
+wob_status check_block(uint32_t block) {
+    if (block < 64 || block > 256 || block % 64 != 0) return fail(WOB_INVALID_ARG, "block_size must be 64, 128, 192 or 256");
+    return WOB_OK;
+}
+
 wob_status run_walk(const wob_scene *cscene, const wob_estimator_desc *est, const float *d_points, uint32_t n_points,
                     uint32_t point_base, uint32_t paths, uint64_t seed, uint64_t path_base, uint32_t stream_id,
                     double *d_sum, double *d_sumsq, wob_launch_config *launch, void *stream, bool boundary) {
@@ -127,40 +170,27 @@ wob_status run_walk(const wob_scene *cscene, const wob_estimator_desc *est, cons
     const int D = (int)scene->host.dim;
     const uint32_t V = boundary ? 1u : (uint32_t)D + 1u;
 
-    uint32_t block = launch && launch->block_size ? launch->block_size : 128u;
-    if (block < 64 || block > 256 || block % 64 != 0) return fail(WOB_INVALID_ARG, "block_size must be 64, 128, 192 or 256");
+    const uint32_t block = launch && launch->block_size ? launch->block_size : 128u;
+    if ((st = check_block(block)) != WOB_OK) return st;
     uint32_t ppl = launch && launch->paths_per_lane ? launch->paths_per_lane : 0u;
-    if (ppl == 0) {
-        // about a million work items keeps every CU busy with room for load balancing
-        const double total = (double)n_points * paths;
-        ppl = (uint32_t)std::max(1.0, std::ceil(total / (1u << 20)));
-    }
-    ppl = std::min(ppl, paths);
-    uint32_t chunks = (paths + ppl - 1) / ppl;
-    ppl = (paths + chunks - 1) / chunks;     // balance the chunks
-    chunks = (paths + ppl - 1) / ppl;
-    const uint64_t n_items = (uint64_t)n_points * chunks;
+    if (ppl == 0) ppl = auto_paths_per_lane((double)n_points * paths);
+
+    wobl::WalkLaunch L{};
+    L.e0 = make_entry(scene, est, d_points, n_points, paths, ppl, point_base, stream_id, path_base, d_sum, d_sumsq);
+    const uint64_t n_items = (uint64_t)n_points * L.e0.chunks;
     if (n_items >= (1ull << 31)) return fail(WOB_INVALID_ARG, "too many work items; split the point set");
     st = ensure_scratch(scene, (size_t)2 * V * n_items * sizeof(float));
     if (st != WOB_OK) return st;
+    L.table = nullptr;
+    L.n_entries = 1;
+    L.total_items = (uint32_t)n_items;
+    L.total_points = n_points;
+    L.persistent = 0;
+    L.work_counter = nullptr;
+    L.pass_counter = nullptr;
+    L.seed = seed;
+    L.partials = scene->scratch;
 
-    wobl::WalkArgs a;
-    a.scene = scene->dev;
-    a.points = d_points;
-    a.partials = scene->scratch;
-    a.n_points = n_points;
-    a.chunks = chunks;
-    a.paths_per_point = paths;
-    a.paths_per_chunk = ppl;
-    a.point_base = point_base;
-    a.stream = stream_id;
-    a.seed = seed;
-    a.path_base = path_base;
-    a.dsign = est->domain == WOB_INTERIOR ? 1.0f : -1.0f;
-    a.eps = est->eps;
-    a.fkesc = est->first_kind_equation_scaling_constant;
-    a.path_length = est->path_length;
-    a.nrc = est->num_resampling_candidates;
     hipStream_t hs = (hipStream_t)stream;
     const bool timed = launch && (launch->flags & WOB_LAUNCH_TIME_WALK);
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -169,11 +199,11 @@ wob_status run_walk(const wob_scene *cscene, const wob_estimator_desc *est, cons
         WOB_HIP(hipEventCreate(&ev[1]), "hipEventCreate");
         WOB_HIP(hipEventRecord(ev[0], hs), "hipEventRecord");
     }
-    WOB_HIP(wobl::launch_walk(D, boundary, a, est->bie_model, est->estimator, flavour, block, hs), "walk launch");
+    WOB_HIP(wobl::launch_walk(D, boundary, L, est->bie_model, est->estimator, flavour, block, 0, hs), "walk launch");
     if (timed) WOB_HIP(hipEventRecord(ev[1], hs), "hipEventRecord");
-    WOB_HIP(wobl::launch_reduce(scene->scratch, n_points, chunks, V, d_sum, d_sumsq, hs), "reduce launch");
+    WOB_HIP(wobl::launch_reduce(L, V, hs), "reduce launch");
     if (launch) {
-        launch->used_paths_per_lane = ppl;
+        launch->used_paths_per_lane = L.e0.paths_per_chunk;
         launch->used_work_items = (uint32_t)n_items;
     }
     if (timed) {
@@ -184,6 +214,111 @@ wob_status run_walk(const wob_scene *cscene, const wob_estimator_desc *est, cons
         (void)hipEventDestroy(ev[0]);
         (void)hipEventDestroy(ev[1]);
     }
+    return WOB_OK;
+}
+
+}  // namespace
+
+struct wob_batch {
+    std::vector<wob_batch_entry> entries;
+    wob_batch_config cfg{};
+    int dim = 0, bie = 0, est = 0, flavour = 0;
+    uint32_t V = 1;
+    wobl::WalkLaunch L{};
+    wobl::WalkEntry *d_table = nullptr;     // device entry table
+    float *partials = nullptr;
+    size_t partial_bytes = 0;
+    uint32_t *counters = nullptr;           // [0] work counter, [1] pass counter
+    uint32_t grid = 0;                      // 0: one lane per item
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t capture_stream = nullptr;
+    double *tune_out = nullptr;             // scratch outputs for the autotuner
+};
+
+namespace {
+
+void batch_release_graph(wob_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+void batch_free(wob_batch *b) {
+    batch_release_graph(b);
+    if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+    if (b->d_table) (void)hipFree(b->d_table);
+    if (b->partials) (void)hipFree(b->partials);
+    if (b->counters) (void)hipFree(b->counters);
+    if (b->tune_out) (void)hipFree(b->tune_out);
+}
+
+// (Re)build the device entry table and partial buffer for the batch's launch configuration.
+// `redirect`: write every entry's outputs into the tuning scratch instead.
+wob_status batch_configure(wob_batch *b, bool redirect) {
+    const uint32_t n = (uint32_t)b->entries.size();
+    std::vector<wobl::WalkEntry> tab(n);
+    uint64_t items = 0, points = 0, partial = 0, out_off = 0;
+    uint32_t ppl = b->cfg.paths_per_lane;
+    if (ppl == 0) {
+        double total = 0;
+        for (const auto &e : b->entries) total += (double)e.n_points * e.paths_per_point;
+        ppl = auto_paths_per_lane(total);
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        const auto &be = b->entries[i];
+        double *s = be.d_sum, *q = be.d_sumsq;
+        if (redirect) {
+            s = b->tune_out + out_off;
+            q = b->tune_out + out_off;
+            out_off += (uint64_t)be.n_points * b->V;
+        }
+        tab[i] = make_entry(be.scene, &be.est, be.d_points, be.n_points, be.paths_per_point, ppl, be.point_base,
+                            be.stream_id, be.path_base, s, q);
+        tab[i].item_begin = (uint32_t)items;
+        tab[i].point_begin = (uint32_t)points;
+        tab[i].partial_base = (uint32_t)partial;
+        const uint64_t ni = (uint64_t)be.n_points * tab[i].chunks;
+        items += ni;
+        points += be.n_points;
+        partial += 2ull * b->V * ni;
+        if (items >= (1ull << 31) || partial >= (1ull << 32)) return fail(WOB_INVALID_ARG, "batch too large for one launch");
+    }
+    const size_t pbytes = (size_t)partial * sizeof(float);
+    if (pbytes > b->partial_bytes) {
+        if (b->partials) WOB_HIP(hipFree(b->partials), "hipFree");
+        b->partials = nullptr;
+        b->partial_bytes = 0;
+        if (hipMalloc(&b->partials, pbytes) != hipSuccess) return fail(WOB_OOM, "hipMalloc batch partials");
+        b->partial_bytes = pbytes;
+    }
+    WOB_HIP(hipMemcpy(b->d_table, tab.data(), n * sizeof(wobl::WalkEntry), hipMemcpyHostToDevice), "hipMemcpy table");
+    wobl::WalkLaunch &L = b->L;
+    L = wobl::WalkLaunch{};
+    L.e0 = tab[0];
+    L.table = b->d_table;
+    L.n_entries = n;
+    L.total_items = (uint32_t)items;
+    L.total_points = (uint32_t)points;
+    L.persistent = b->cfg.persistent ? 1u : 0u;
+    L.work_counter = b->counters;
+    L.pass_counter = b->counters + 1;
+    L.partials = b->partials;
+    b->grid = L.persistent ? wobl::persistent_grid(b->dim, b->cfg.boundary != 0, b->bie, b->est, b->flavour,
+                                                   b->cfg.block_size ? b->cfg.block_size : 128u)
+                           : 0u;
+    batch_release_graph(b);
+    return WOB_OK;
+}
+
+wob_status batch_pass(wob_batch *b, uint64_t seed, hipStream_t hs) {
+    b->L.seed = seed;
+    const uint32_t block = b->cfg.block_size ? b->cfg.block_size : 128u;
+    WOB_HIP(wobl::launch_walk(b->dim, b->cfg.boundary != 0, b->L, b->bie, b->est, b->flavour, block, b->grid, hs),
+            "batch walk launch");
+    WOB_HIP(wobl::launch_reduce(b->L, b->V, hs), "batch reduce launch");
+    WOB_HIP(wobl::launch_finalize(b->counters, b->counters + 1, hs), "batch finalize launch");
     return WOB_OK;
 }
 
@@ -351,6 +486,161 @@ const char *wob_enum_to_string(const char *kind, int32_t v) {
         return v >= 0 && v < 4 ? n[v] : "InvalidProblemType";
     }
     return "Invalid";
+}
+
+wob_status wob_batch_create(const wob_batch_entry *entries, uint32_t n, const wob_batch_config *cfg,
+                            wob_batch **out) {
+    if (!entries || !out || n == 0) return fail(WOB_INVALID_ARG, "null or empty batch");
+    *out = nullptr;
+    wob_batch_config c{};
+    if (cfg) c = *cfg;
+    if (c.block_size && check_block(c.block_size) != WOB_OK) return WOB_INVALID_ARG;
+    auto *b = new (std::nothrow) wob_batch();
+    if (!b) return fail(WOB_OOM, "host allocation failed");
+    b->entries.assign(entries, entries + n);
+    b->cfg = c;
+    for (uint32_t i = 0; i < n; ++i) {
+        const auto &e = entries[i];
+        int fl = 0;
+        wob_status st = check_est(e.scene, &e.est, fl);
+        if (st != WOB_OK) { delete b; return fail(st, "entry " + std::to_string(i) + ": " + g_last_error); }
+        if (e.n_points == 0 || e.paths_per_point == 0 || !e.d_points || !e.d_sum || !e.d_sumsq) {
+            delete b;
+            return fail(WOB_INVALID_ARG, "entry " + std::to_string(i) + ": empty point set or null buffer");
+        }
+        const int dim = (int)e.scene->host.dim;
+        if (i == 0) { b->dim = dim; b->bie = e.est.bie_model; b->est = e.est.estimator; b->flavour = fl; }
+        else if (dim != b->dim || e.est.bie_model != b->bie || e.est.estimator != b->est || fl != b->flavour) {
+            delete b;
+            return fail(WOB_UNSUPPORTED_COMBO, "entry " + std::to_string(i) +
+                                                   ": all batch entries must share dimension, model, estimator and "
+                                                   "sampling strategy (split the batch)");
+        }
+    }
+    b->V = c.boundary ? 1u : (uint32_t)b->dim + 1u;
+    uint64_t out_elems = 0;
+    for (const auto &e : b->entries) out_elems += (uint64_t)e.n_points * b->V;
+    if (hipMalloc(&b->d_table, n * sizeof(wobl::WalkEntry)) != hipSuccess || hipMalloc(&b->counters, 64) != hipSuccess ||
+        hipMalloc(&b->tune_out, out_elems * sizeof(double)) != hipSuccess) {
+        batch_free(b);
+        delete b;
+        return fail(WOB_OOM, "hipMalloc batch buffers");
+    }
+    if (hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking) != hipSuccess) {
+        batch_free(b);
+        delete b;
+        return fail(WOB_DEVICE, "hipStreamCreate");
+    }
+    wob_status st = batch_configure(b, false);
+    if (st == WOB_OK) {
+        hipError_t e = wobl::launch_fill_u32(b->counters, 16, 0u, b->capture_stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->capture_stream);
+        if (e != hipSuccess) st = hip_fail(e, "counter init");
+    }
+    if (st != WOB_OK) {
+        batch_free(b);
+        delete b;
+        return st;
+    }
+    *out = b;
+    return WOB_OK;
+}
+
+wob_status wob_batch_run(wob_batch *b, uint64_t seed, uint32_t n_passes, void *stream) {
+    if (!b) return fail(WOB_INVALID_ARG, "null batch");
+    hipStream_t hs = (hipStream_t)stream;
+    if (!b->cfg.use_graph) {
+        for (uint32_t p = 0; p < n_passes; ++p) {
+            wob_status st = batch_pass(b, seed, hs);
+            if (st != WOB_OK) return st;
+        }
+        return WOB_OK;
+    }
+    if (b->exec && b->L.seed != seed) batch_release_graph(b);
+    if (!b->exec) {
+        // capture one pass (walk, reduction, counter update) on the batch's own stream
+        WOB_HIP(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        wob_status st = batch_pass(b, seed, b->capture_stream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->capture_stream, &g);
+        if (st != WOB_OK) { if (g) (void)hipGraphDestroy(g); return st; }
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        b->graph = g;
+        WOB_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    }
+    for (uint32_t p = 0; p < n_passes; ++p) WOB_HIP(hipGraphLaunch(b->exec, hs), "hipGraphLaunch");
+    return WOB_OK;
+}
+
+wob_status wob_batch_tune(wob_batch *b, uint64_t seed, wob_batch_config *best, float *best_ms, void *stream) {
+    if (!b) return fail(WOB_INVALID_ARG, "null batch");
+    hipStream_t hs = (hipStream_t)stream;
+    const wob_batch_config orig = b->cfg;
+    // the pass counter is not advanced by tuning passes
+    uint32_t saved_pass = 0;
+    WOB_HIP(hipStreamSynchronize(hs), "hipStreamSynchronize");
+    WOB_HIP(hipMemcpy(&saved_pass, b->counters + 1, 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    double total = 0;
+    uint32_t min_paths = 0xFFFFFFFFu;
+    for (const auto &e : b->entries) {
+        total += (double)e.n_points * e.paths_per_point;
+        min_paths = std::min(min_paths, e.paths_per_point);
+    }
+    const uint32_t base = auto_paths_per_lane(total);
+    std::vector<uint32_t> ppls;
+    for (uint32_t f : {1u, 2u, 4u, 8u, 16u, 32u, 64u}) ppls.push_back(f);
+    ppls.push_back(base);
+    std::sort(ppls.begin(), ppls.end());
+    ppls.erase(std::unique(ppls.begin(), ppls.end()), ppls.end());
+    hipEvent_t e0, e1;
+    WOB_HIP(hipEventCreate(&e0), "hipEventCreate");
+    WOB_HIP(hipEventCreate(&e1), "hipEventCreate");
+    float best_t = 1e30f;
+    wob_batch_config best_c = orig;
+    for (uint32_t ppl : ppls) {
+        if (ppl > std::max(1u, min_paths) && ppl != base) continue;
+        if ((double)total / ppl < 65536.0 && ppl != base) continue;   // too few items to fill the GPU
+        for (uint32_t block : {64u, 128u, 256u}) {
+            for (uint32_t pers : {0u, 1u}) {
+                b->cfg = orig;
+                b->cfg.paths_per_lane = ppl;
+                b->cfg.block_size = block;
+                b->cfg.persistent = pers;
+                b->cfg.use_graph = 0;
+                wob_status st = batch_configure(b, true);
+                if (st != WOB_OK) { b->cfg = orig; (void)batch_configure(b, false); return st; }
+                st = batch_pass(b, seed, hs);                       // warm-up
+                if (st != WOB_OK) { b->cfg = orig; (void)batch_configure(b, false); return st; }
+                WOB_HIP(hipEventRecord(e0, hs), "hipEventRecord");
+                st = batch_pass(b, seed, hs);
+                if (st != WOB_OK) { b->cfg = orig; (void)batch_configure(b, false); return st; }
+                WOB_HIP(hipEventRecord(e1, hs), "hipEventRecord");
+                WOB_HIP(hipEventSynchronize(e1), "hipEventSynchronize");
+                float ms = 0.f;
+                WOB_HIP(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                if (ms < best_t) { best_t = ms; best_c = b->cfg; }
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->cfg = best_c;
+    b->cfg.use_graph = orig.use_graph;
+    wob_status st = batch_configure(b, false);
+    if (st != WOB_OK) return st;
+    WOB_HIP(wobl::launch_fill_u32(b->counters, 1, 0u, hs), "counter reset");
+    WOB_HIP(wobl::launch_fill_u32(b->counters + 1, 1, saved_pass, hs), "counter restore");
+    WOB_HIP(hipStreamSynchronize(hs), "hipStreamSynchronize");
+    if (best) *best = b->cfg;
+    if (best_ms) *best_ms = best_t;
+    return WOB_OK;
+}
+
+wob_status wob_batch_destroy(wob_batch *b) {
+    if (!b) return WOB_OK;
+    batch_free(b);
+    delete b;
+    return WOB_OK;
 }
 
 const char *wob_last_error_message(void) { return g_last_error.c_str(); }

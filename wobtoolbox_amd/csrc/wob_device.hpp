@@ -16,6 +16,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// FP32 expressions are evaluated as written (no fused multiply-add contraction), the
+// operation order of the reference's formulas; transcendental calls aside, this keeps the
+// kernel and weight values bit-identical to a non-contracted CPU evaluation.
+#pragma clang fp contract(off)
+
 namespace wobdev {
 
 constexpr float kFltEps = 1.1920928955078125e-07f;
@@ -113,13 +118,14 @@ template <int D> struct BP {
     float bv;
     float alpha;
     int type;
+    uint32_t elem;   // element the point lies on (0xFFFFFFFF: none / unknown)
 };
 
 template <int D> __device__ __forceinline__ BP<D> bp_zero() {
     BP<D> b;
 #pragma unroll
     for (int k = 0; k < D; ++k) { b.p[k] = 0.f; b.n[k] = 0.f; }
-    b.bv = 0.f; b.alpha = 0.f; b.type = kDirichlet;
+    b.bv = 0.f; b.alpha = 0.f; b.type = kDirichlet; b.elem = 0xFFFFFFFFu;
     return b;
 }
 
@@ -232,6 +238,7 @@ __device__ __forceinline__ uint32_t cdf_lookup(const float *__restrict__ cdf, co
 // point on element e at parameter(s) from the sampler; fills everything but p
 template <int D> __device__ __forceinline__ void load_attr(const DevScene &S, uint32_t e, float *v /* D*D */, BP<D> &b,
                                                           float *bvv, float *alv, float &area) {
+    b.elem = e;
     if constexpr (D == 2) {
         const float4 f0 = S.attr[3 * e], f1 = S.attr[3 * e + 1], f2 = S.attr[3 * e + 2];
         v[0] = f0.x; v[1] = f0.y; v[2] = f0.z; v[3] = f0.w;
@@ -426,12 +433,13 @@ __device__ __forceinline__ void traverse(const DevScene &S, const float *o, cons
 // triple32(pick ^ id * 0x9E3779B9): a uniform choice among the hits for a uniform pick.
 template <int D, int STACK>
 __device__ __forceinline__ LineHit<D> line_query(const DevScene &S, const float *o, const float *d, uint32_t pick,
-                                                 uint32_t *stack, uint32_t stride) {
+                                                 uint32_t skip, uint32_t *stack, uint32_t stride) {
     LineHit<D> h;
     h.count = 0; h.id = 0xFFFFFFFFu; h.t = 0.f; h.a = 0.f; h.b = 0.f;
     uint32_t best = 0xFFFFFFFFu;
     traverse<D, STACK>(S, o, d, stack, stride, [&](uint32_t prim, float t, float a, float b) {
         const uint32_t id = S.prim_id[prim];
+        if (id == skip) return;
         const uint32_t key = triple32(pick ^ (id * 0x9E3779B9u));
         h.count += 1;
         if (key < best || (key == best && id < h.id)) {
@@ -444,8 +452,8 @@ __device__ __forceinline__ LineHit<D> line_query(const DevScene &S, const float 
 // scene.cuh:464-535 (2D), 760-840 (3D): a uniformly oriented line through x, all hits, one
 // hit taken uniformly, inv_pdf = N pi |r| / |rhat.n| (2D) or N 2pi |r|^2 / |rhat.n| (3D).
 template <int D, int STACK>
-__device__ __forceinline__ Sample<D> line_sample(const DevScene &S, const float *x, Rng &rng, uint32_t *stack,
-                                                 uint32_t stride) {
+__device__ __forceinline__ Sample<D> line_sample(const DevScene &S, const float *x, uint32_t skip, Rng &rng,
+                                                 uint32_t *stack, uint32_t stride) {
     float d[D];
     if constexpr (D == 2) {
         const float theta = rng.uniform();
@@ -461,7 +469,7 @@ __device__ __forceinline__ Sample<D> line_sample(const DevScene &S, const float 
         d[0] = c * r; d[1] = s * r;
     }
     const uint32_t pick = rng.next_u32();
-    const LineHit<D> h = line_query<D, STACK>(S, x, d, pick, stack, stride);
+    const LineHit<D> h = line_query<D, STACK>(S, x, d, pick, skip, stack, stride);
     Sample<D> r;
     if (h.count == 0) {                                                                 // scene.cuh:488-495
         r.y = bp_zero<D>();
@@ -655,6 +663,8 @@ template <int D, int BIE> struct Model {
 struct WalkParams {
     uint32_t path_length;
     uint32_t nrc;
+    uint32_t exclude_origin;   // WOB_EST_EXCLUDE_ORIGIN_ELEMENT
+    __device__ __forceinline__ uint32_t skip(uint32_t elem) const { return exclude_origin ? elem : 0xFFFFFFFFu; }
 };
 
 // resampled importance sampling, scene.cuh:542-584 (target by the type of y_prev,
@@ -690,14 +700,14 @@ __device__ __forceinline__ float walk_backward(const DevScene &S, const Model<D,
     for (uint32_t it = 0; it < wp.path_length; ++it) {
         if (it == wp.path_length - 1) W = 0.5f * W;
         if constexpr (FL == FL_LINE_LINE) {
-            Sample<D> q = line_sample<D, STACK>(S, yp.p, rng, stack, stride);
+            Sample<D> q = line_sample<D, STACK>(S, yp.p, wp.skip(yp.elem), rng, stack, stride);
             spc += W * m.fredholm_known(yp, q.y, q.inv_pdf);
             W *= m.fredholm_unknown_backward(yp, q.y, q.inv_pdf, rng);
             yp = q.y;
         } else if constexpr (FL == FL_LINE_BV) {
             Sample<D> q = sample_bv<D>(S, rng);
             spc += W * m.fredholm_known(yp, q.y, q.inv_pdf);
-            q = line_sample<D, STACK>(S, yp.p, rng, stack, stride);
+            q = line_sample<D, STACK>(S, yp.p, wp.skip(yp.elem), rng, stack, stride);
             W = W * m.fredholm_unknown_backward(yp, q.y, q.inv_pdf, rng);
             yp = q.y;
         } else if constexpr (BIE == kSL) {
@@ -719,13 +729,13 @@ __device__ __forceinline__ float walk_backward(const DevScene &S, const Model<D,
 
 // first samples of the backward estimators (estimator.cuh:158-168, 199-209)
 template <int D, int FL, int STACK>
-__device__ __forceinline__ void first_samples(const DevScene &S, const float *x, Sample<D> &y, Sample<D> &y1, Rng &rng,
-                                              uint32_t *stack, uint32_t stride) {
+__device__ __forceinline__ void first_samples(const DevScene &S, const float *x, uint32_t skip, Sample<D> &y,
+                                              Sample<D> &y1, Rng &rng, uint32_t *stack, uint32_t stride) {
     if constexpr (FL == FL_LINE_LINE) {
-        y = line_sample<D, STACK>(S, x, rng, stack, stride);
+        y = line_sample<D, STACK>(S, x, skip, rng, stack, stride);
         y1 = y;
     } else if constexpr (FL == FL_LINE_BV) {
-        y = line_sample<D, STACK>(S, x, rng, stack, stride);
+        y = line_sample<D, STACK>(S, x, skip, rng, stack, stride);
         y1 = sample_bv<D>(S, rng);
     } else {
         y = sample_uniform<D>(S, rng);
@@ -739,7 +749,7 @@ __device__ __forceinline__ void domain_path(const DevScene &S, const Model<D, BI
                                             const float *x, Rng &rng, uint32_t *stack, uint32_t stride, float *out) {
     if constexpr (EST == kBackward) {                                                          // estimator.cuh:154-191
         Sample<D> y, y1;
-        first_samples<D, FL, STACK>(S, x, y, y1, rng, stack, stride);
+        first_samples<D, FL, STACK>(S, x, 0xFFFFFFFFu, y, y1, rng, stack, stride);
         const float known = m.solution_domain_known(x, y1.y, y1.inv_pdf);
         float gk[D], gu[D];
         m.gradient_domain_known(x, y1.y, y1.inv_pdf, gk);
@@ -767,7 +777,7 @@ __device__ __forceinline__ void domain_path(const DevScene &S, const Model<D, BI
             for (int k = 0; k < D; ++k) g[k] += W * gu[k];
             BP<D> yp = yc;
             Sample<D> q;
-            if constexpr (FL == FL_FWD_LINE) q = line_sample<D, STACK>(S, yp.p, rng, stack, stride);
+            if constexpr (FL == FL_FWD_LINE) q = line_sample<D, STACK>(S, yp.p, wp.skip(yp.elem), rng, stack, stride);
             else q = sample_uniform<D>(S, rng);
             yc = q.y;
             W = m.fredholm_unknown_forward(yc, yp, q.inv_pdf, rng) * W;
@@ -783,7 +793,7 @@ __device__ __forceinline__ float boundary_path(const DevScene &S, const Model<D,
                                                const BP<D> &x, Rng &rng, uint32_t *stack, uint32_t stride) {
     if constexpr (EST == kBackward) {
         Sample<D> y, y1;
-        first_samples<D, FL, STACK>(S, x.p, y, y1, rng, stack, stride);
+        first_samples<D, FL, STACK>(S, x.p, wp.skip(x.elem), y, y1, rng, stack, stride);
         float res = m.solution_boundary_known(x, y1.y, y1.inv_pdf);
         const float unk = m.solution_boundary_unknown(x, y.y, y.inv_pdf);
         const float spc = walk_backward<D, BIE, FL, STACK>(S, m, wp, y.y, rng, stack, stride);
@@ -801,7 +811,7 @@ __device__ __forceinline__ float boundary_path(const DevScene &S, const Model<D,
             res += m.solution_boundary_unknown(x, yc, 1.0f) * W;
             BP<D> yp = yc;
             Sample<D> q;
-            if constexpr (FL == FL_FWD_LINE) q = line_sample<D, STACK>(S, yp.p, rng, stack, stride);
+            if constexpr (FL == FL_FWD_LINE) q = line_sample<D, STACK>(S, yp.p, wp.skip(yp.elem), rng, stack, stride);
             else q = sample_uniform<D>(S, rng);
             yc = q.y;
             W = m.fredholm_unknown_forward(yc, yp, q.inv_pdf, rng) * W;

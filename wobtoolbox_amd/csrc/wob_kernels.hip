@@ -11,6 +11,8 @@
 // device functions of the walk.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "wob_device.hpp"
 #include "wob_launch.hpp"
 
@@ -18,7 +20,8 @@ using namespace wobdev;
 
 namespace wobk {
 
-using wobl::WalkArgs;
+using wobl::WalkEntry;
+using wobl::WalkLaunch;
 
 struct Kahan {
     float s, c;
@@ -31,79 +34,145 @@ struct Kahan {
     }
 };
 
+// entry of a global work item / point (binary search over the entry table)
+__device__ __forceinline__ const WalkEntry &entry_of_item(const WalkLaunch &L, uint32_t item) {
+    if (L.table == nullptr) return L.e0;
+    uint32_t lo = 0, hi = L.n_entries - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (L.table[mid].item_begin <= item) lo = mid; else hi = mid - 1;
+    }
+    return L.table[lo];
+}
+
+__device__ __forceinline__ const WalkEntry &entry_of_point(const WalkLaunch &L, uint32_t gp) {
+    if (L.table == nullptr) return L.e0;
+    uint32_t lo = 0, hi = L.n_entries - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (L.table[mid].point_begin <= gp) lo = mid; else hi = mid - 1;
+    }
+    return L.table[lo];
+}
+
+// One work item: `paths_per_chunk` consecutive paths of one point, back to back.
 template <int D, int BIE, int EST, int FL, int STACK, bool BOUNDARY>
-__global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
-    extern __shared__ uint32_t lds_stack[];
+__device__ __forceinline__ void walk_item(const WalkLaunch &L, uint32_t item, uint32_t pass, uint32_t *stack,
+                                          uint32_t stride) {
     constexpr int V = BOUNDARY ? 1 : D + 1;
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n_items = a.n_points * a.chunks;
-    if (item >= n_items) return;
-    const uint32_t chunk = item / a.n_points;
-    const uint32_t pt = item - chunk * a.n_points;
-    const uint32_t j0 = chunk * a.paths_per_chunk;
-    const uint32_t j1 = min(a.paths_per_point, j0 + a.paths_per_chunk);
+    const WalkEntry &e = entry_of_item(L, item);
+    const uint32_t local = item - e.item_begin;
+    const uint32_t chunk = local / e.n_points;
+    const uint32_t pt = local - chunk * e.n_points;
+    const uint32_t j0 = chunk * e.paths_per_chunk;
+    const uint32_t j1 = min(e.paths_per_point, j0 + e.paths_per_chunk);
+    const uint64_t path_base = e.path_base + (uint64_t)pass * e.paths_per_point;
+    const DevScene S = e.scene;
 
     Model<D, BIE> m;
-    m.k.eps = a.eps;
-    m.dsign = a.dsign;
-    m.c = a.fkesc;
-    const WalkParams wp{a.path_length, a.nrc};
-    uint32_t *stack = lds_stack + threadIdx.x;
-    const uint32_t stride = blockDim.x;
+    m.k.eps = e.eps;
+    m.dsign = e.dsign;
+    m.c = e.fkesc;
+    const WalkParams wp{e.path_length, e.nrc, e.flags & 1u};
 
     Kahan ks[V], kq[V];
 #pragma unroll
     for (int k = 0; k < V; ++k) { ks[k].init(); kq[k].init(); }
 
     if constexpr (BOUNDARY) {
-        const float *r = a.points + (size_t)pt * (2 * D + 4);
+        const float *r = e.points + (size_t)pt * (2 * D + 4);
         BP<D> x;
 #pragma unroll
         for (int k = 0; k < D; ++k) { x.p[k] = r[k]; x.n[k] = r[D + k]; }
         x.bv = r[2 * D + 1];
         x.type = (int)r[2 * D + 2];
         x.alpha = r[2 * D + 3];
+        x.elem = 0xFFFFFFFFu;
         for (uint32_t j = j0; j < j1; ++j) {
             Rng rng;
-            rng.init(a.seed, (uint32_t)(a.path_base + j), a.point_base + pt, a.stream);
-            const float v = boundary_path<D, BIE, EST, FL, STACK>(a.scene, m, wp, x, rng, stack, stride);
+            rng.init(L.seed, (uint32_t)(path_base + j), e.point_base + pt, e.stream);
+            const float v = boundary_path<D, BIE, EST, FL, STACK>(S, m, wp, x, rng, stack, stride);
             ks[0].add(v);
             kq[0].add(v * v);
         }
     } else {
         float x[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) x[k] = a.points[(size_t)pt * D + k];
+        for (int k = 0; k < D; ++k) x[k] = e.points[(size_t)pt * D + k];
         for (uint32_t j = j0; j < j1; ++j) {
             Rng rng;
-            rng.init(a.seed, (uint32_t)(a.path_base + j), a.point_base + pt, a.stream);
+            rng.init(L.seed, (uint32_t)(path_base + j), e.point_base + pt, e.stream);
             float out[D + 1];
-            domain_path<D, BIE, EST, FL, STACK>(a.scene, m, wp, x, rng, stack, stride, out);
+            domain_path<D, BIE, EST, FL, STACK>(S, m, wp, x, rng, stack, stride, out);
 #pragma unroll
             for (int k = 0; k < V; ++k) { ks[k].add(out[k]); kq[k].add(out[k] * out[k]); }
         }
     }
+    const size_t n_items = (size_t)e.n_points * e.chunks;
+    float *part = L.partials + e.partial_base;
 #pragma unroll
     for (int k = 0; k < V; ++k) {
-        a.partials[(size_t)k * n_items + item] = ks[k].s;
-        a.partials[(size_t)(V + k) * n_items + item] = kq[k].s;
+        part[(size_t)k * n_items + local] = ks[k].s;
+        part[(size_t)(V + k) * n_items + local] = kq[k].s;
     }
 }
 
-__global__ void reduce_kernel(const float *__restrict__ partials, uint32_t n_points, uint32_t chunks, uint32_t V,
-                              double *__restrict__ sum, double *__restrict__ sumsq) {
-    const uint32_t pt = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pt >= n_points) return;
-    const size_t n_items = (size_t)n_points * chunks;
+// Work items are numbered entry by entry, chunk-major inside an entry so that neighbouring
+// lanes walk from neighbouring points.  Persistent mode: each wave takes 64 items at a time
+// from a device counter until the items run out, so lanes of a finished wave move on to new
+// (point, chunk) work at once.
+template <int D, int BIE, int EST, int FL, int STACK, bool BOUNDARY>
+__global__ __launch_bounds__(256) void walk_kernel(WalkLaunch L) {
+    extern __shared__ uint32_t lds_stack[];
+    uint32_t *stack = lds_stack + threadIdx.x;
+    const uint32_t stride = blockDim.x;
+    const uint32_t pass = L.pass_counter ? *L.pass_counter : 0u;
+    if (!L.persistent) {
+        const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+        if (item < L.total_items) walk_item<D, BIE, EST, FL, STACK, BOUNDARY>(L, item, pass, stack, stride);
+        return;
+    }
+    const uint32_t lane = threadIdx.x & 63u;
+    while (true) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(L.work_counter, 64u);
+        base = __shfl(base, 0, 64);
+        if (base >= L.total_items) break;
+        const uint32_t item = base + lane;
+        if (item < L.total_items) walk_item<D, BIE, EST, FL, STACK, BOUNDARY>(L, item, pass, stack, stride);
+    }
+}
+
+// Per point: FP64 sum of its chunk partials in chunk order, added into the entry's outputs.
+__global__ void reduce_kernel(WalkLaunch L, uint32_t V) {
+    const uint32_t gp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gp >= L.total_points) return;
+    const WalkEntry &e = entry_of_point(L, gp);
+    const uint32_t pt = gp - e.point_begin;
+    const size_t n_items = (size_t)e.n_points * e.chunks;
+    const float *part = L.partials + e.partial_base;
     for (uint32_t k = 0; k < V; ++k) {
         double s = 0.0, q = 0.0;
-        for (uint32_t c = 0; c < chunks; ++c) {
-            s += (double)partials[k * n_items + (size_t)c * n_points + pt];
-            q += (double)partials[(V + k) * n_items + (size_t)c * n_points + pt];
+        for (uint32_t c = 0; c < e.chunks; ++c) {
+            s += (double)part[k * n_items + (size_t)c * e.n_points + pt];
+            q += (double)part[(V + k) * n_items + (size_t)c * e.n_points + pt];
         }
-        sum[(size_t)pt * V + k] += s;
-        sumsq[(size_t)pt * V + k] += q;
+        e.sum[(size_t)pt * V + k] += s;
+        e.sumsq[(size_t)pt * V + k] += q;
     }
+}
+
+// end of a pass: clear the persistent work counter, advance the pass counter
+__global__ void finalize_kernel(uint32_t *work_counter, uint32_t *pass_counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (work_counter) *work_counter = 0u;
+        if (pass_counter) *pass_counter += 1u;
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t n, uint32_t v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -138,6 +207,7 @@ template <int D> __device__ __forceinline__ BP<D> load_record(const float *r) {
     b.bv = r[2 * D + 1];
     b.type = (int)r[2 * D + 2];
     b.alpha = r[2 * D + 3];
+    b.elem = 0xFFFFFFFFu;
     return b;
 }
 
@@ -214,43 +284,69 @@ namespace wobl {
 
 using namespace wobk;
 
-template <int D, int BIE, int EST, int FL, bool BOUNDARY>
-static hipError_t launch_walk_t(const WalkArgs &a, uint32_t block, hipStream_t st) {
-    const uint32_t n_items = a.n_points * a.chunks;
-    const uint32_t grid = (n_items + block - 1) / block;
-    constexpr bool lines = FL == FL_LINE_LINE || FL == FL_LINE_BV || FL == FL_FWD_LINE;
-    constexpr int STACK = lines ? kStackEntries : 0;
-    const size_t lds = lines ? (size_t)STACK * block * sizeof(uint32_t) : 0;
-    hipLaunchKernelGGL((walk_kernel<D, BIE, EST, FL, STACK, BOUNDARY>), dim3(grid), dim3(block), lds, st, a);
+template <int D, int BIE, int EST, int FL, bool BOUNDARY> struct WalkKernel {
+    static constexpr bool lines = FL == FL_LINE_LINE || FL == FL_LINE_BV || FL == FL_FWD_LINE;
+    static constexpr int STACK = lines ? kStackEntries : 0;
+    static size_t lds(uint32_t block) { return lines ? (size_t)STACK * block * sizeof(uint32_t) : 0; }
+    static const void *fn() { return (const void *)&walk_kernel<D, BIE, EST, FL, STACK, BOUNDARY>; }
+    static hipError_t launch(const WalkLaunch &a, uint32_t block, uint32_t grid, hipStream_t st) {
+        if (grid == 0) grid = (a.total_items + block - 1) / block;
+        hipLaunchKernelGGL((walk_kernel<D, BIE, EST, FL, STACK, BOUNDARY>), dim3(grid), dim3(block), lds(block), st, a);
+        return hipGetLastError();
+    }
+    static uint32_t resident_grid(uint32_t block) {
+        int per_cu = 0, dev = 0, cus = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn(), (int)block, lds(block)) != hipSuccess)
+            per_cu = 1;
+        return (uint32_t)std::max(1, per_cu) * (uint32_t)std::max(1, cus);
+    }
+};
+
+// dispatch (model, estimator, flavour) -> instantiation; F(kernel-type) is called with the chosen one
+template <int D, bool BOUNDARY, typename F> static auto dispatch(int bie, int est, int fl, F &&f) {
+    if (est == kBackward) {
+        if (bie == kDL && fl == FL_LINE_LINE) return f(WalkKernel<D, kDL, kBackward, FL_LINE_LINE, BOUNDARY>{});
+        if (bie == kDL) return f(WalkKernel<D, kDL, kBackward, FL_UNIFORM, BOUNDARY>{});
+        if (bie == kDirect && fl == FL_LINE_BV) return f(WalkKernel<D, kDirect, kBackward, FL_LINE_BV, BOUNDARY>{});
+        if (bie == kDirect) return f(WalkKernel<D, kDirect, kBackward, FL_UNIFORM, BOUNDARY>{});
+        return f(WalkKernel<D, kSL, kBackward, FL_UNIFORM, BOUNDARY>{});
+    }
+    if (bie == kSL && fl == FL_FWD_LINE) return f(WalkKernel<D, kSL, kForward, FL_FWD_LINE, BOUNDARY>{});
+    if (bie == kSL) return f(WalkKernel<D, kSL, kForward, FL_FWD_UNIFORM, BOUNDARY>{});
+    if (bie == kDL) return f(WalkKernel<D, kDL, kForward, FL_FWD_UNIFORM, BOUNDARY>{});
+    return f(WalkKernel<D, kDirect, kForward, FL_FWD_UNIFORM, BOUNDARY>{});
+}
+
+template <typename F> static auto dispatch_all(int dim, bool boundary, int bie, int est, int fl, F &&f) {
+    if (dim == 2) return boundary ? dispatch<2, true>(bie, est, fl, f) : dispatch<2, false>(bie, est, fl, f);
+    return boundary ? dispatch<3, true>(bie, est, fl, f) : dispatch<3, false>(bie, est, fl, f);
+}
+
+hipError_t launch_walk(int dim, bool boundary, const WalkLaunch &a, int bie, int est, int fl, uint32_t block,
+                       uint32_t grid, hipStream_t st) {
+    return dispatch_all(dim, boundary, bie, est, fl, [&](auto k) { return decltype(k)::launch(a, block, grid, st); });
+}
+
+uint32_t persistent_grid(int dim, bool boundary, int bie, int est, int fl, uint32_t block) {
+    return dispatch_all(dim, boundary, bie, est, fl, [&](auto k) { return decltype(k)::resident_grid(block); });
+}
+
+hipError_t launch_reduce(const WalkLaunch &a, uint32_t V, hipStream_t st) {
+    const uint32_t block = 256;
+    hipLaunchKernelGGL(reduce_kernel, dim3((a.total_points + block - 1) / block), dim3(block), 0, st, a, V);
     return hipGetLastError();
 }
 
-template <int D, bool BOUNDARY>
-static hipError_t launch_walk_d(const WalkArgs &a, int bie, int est, int fl, uint32_t block, hipStream_t st) {
-    if (est == kBackward) {
-        if (bie == kDL && fl == FL_LINE_LINE) return launch_walk_t<D, kDL, kBackward, FL_LINE_LINE, BOUNDARY>(a, block, st);
-        if (bie == kDL) return launch_walk_t<D, kDL, kBackward, FL_UNIFORM, BOUNDARY>(a, block, st);
-        if (bie == kDirect && fl == FL_LINE_BV) return launch_walk_t<D, kDirect, kBackward, FL_LINE_BV, BOUNDARY>(a, block, st);
-        if (bie == kDirect) return launch_walk_t<D, kDirect, kBackward, FL_UNIFORM, BOUNDARY>(a, block, st);
-        return launch_walk_t<D, kSL, kBackward, FL_UNIFORM, BOUNDARY>(a, block, st);
-    }
-    if (bie == kSL && fl == FL_FWD_LINE) return launch_walk_t<D, kSL, kForward, FL_FWD_LINE, BOUNDARY>(a, block, st);
-    if (bie == kSL) return launch_walk_t<D, kSL, kForward, FL_FWD_UNIFORM, BOUNDARY>(a, block, st);
-    if (bie == kDL) return launch_walk_t<D, kDL, kForward, FL_FWD_UNIFORM, BOUNDARY>(a, block, st);
-    return launch_walk_t<D, kDirect, kForward, FL_FWD_UNIFORM, BOUNDARY>(a, block, st);
+hipError_t launch_finalize(uint32_t *work_counter, uint32_t *pass_counter, hipStream_t st) {
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(64), 0, st, work_counter, pass_counter);
+    return hipGetLastError();
 }
 
-hipError_t launch_walk(int dim, bool boundary, const WalkArgs &a, int bie, int est, int fl, uint32_t block,
-                       hipStream_t st) {
-    if (dim == 2) return boundary ? launch_walk_d<2, true>(a, bie, est, fl, block, st) : launch_walk_d<2, false>(a, bie, est, fl, block, st);
-    return boundary ? launch_walk_d<3, true>(a, bie, est, fl, block, st) : launch_walk_d<3, false>(a, bie, est, fl, block, st);
-}
-
-hipError_t launch_reduce(const float *partials, uint32_t n_points, uint32_t chunks, uint32_t V, double *sum,
-                         double *sumsq, hipStream_t st) {
+hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st) {
     const uint32_t block = 256;
-    hipLaunchKernelGGL(reduce_kernel, dim3((n_points + block - 1) / block), dim3(block), 0, st, partials, n_points,
-                       chunks, V, sum, sumsq);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((n + block - 1) / block), dim3(block), 0, st, p, n, v);
     return hipGetLastError();
 }
 

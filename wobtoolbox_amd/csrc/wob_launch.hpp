@@ -13,26 +13,46 @@ namespace wobl {
 // depth-first stack bound exceeds this.
 constexpr int kStackEntries = 32;
 
-struct WalkArgs {
+// One (scene, estimator, point set) of a launch.  A single estimate call is a one-entry
+// launch; a batch packs many entries (scenes of different sizes, uneven point sets) into one
+// launch whose work items are numbered entry by entry.
+struct WalkEntry {
     wobdev::DevScene scene;
     const float *points;      // domain points (n * D) or boundary records (n * (2D + 4))
-    float *partials;          // 2 * V * n_points * chunks floats (structure of arrays)
+    double *sum, *sumsq;      // n_points * V each, accumulated into
     uint32_t n_points;
     uint32_t chunks;          // work items per point
     uint32_t paths_per_point;
     uint32_t paths_per_chunk;
-    uint32_t point_base;
-    uint32_t stream;
-    uint64_t seed;
-    uint64_t path_base;
-    float dsign, eps, fkesc;
+    uint32_t point_base;      // RNG counter word of points[0]
+    uint32_t stream;          // RNG stream word
     uint32_t path_length, nrc;
+    float dsign, eps, fkesc;
+    uint32_t item_begin;      // first global work item of this entry
+    uint32_t point_begin;     // first global point of this entry (reduction)
+    uint32_t partial_base;    // offset of this entry's partials (floats)
+    uint32_t flags;           // WOB_EST_* options
+    uint64_t path_base;       // first path index of pass 0
 };
 
-hipError_t launch_walk(int dim, bool boundary, const WalkArgs &a, int bie, int est, int flavour, uint32_t block,
-                       hipStream_t st);
-hipError_t launch_reduce(const float *partials, uint32_t n_points, uint32_t chunks, uint32_t V, double *sum,
-                         double *sumsq, hipStream_t st);
+struct WalkLaunch {
+    WalkEntry e0;                  // used when table == nullptr
+    const WalkEntry *table;        // device table of n_entries (batches)
+    uint32_t n_entries;
+    uint32_t total_items;
+    uint32_t total_points;
+    uint32_t persistent;           // 1: waves pull 64-item blocks from *work_counter
+    uint32_t *work_counter;        // device counter (persistent mode)
+    const uint32_t *pass_counter;  // device pass index (graph replays), may be null
+    uint64_t seed;
+    float *partials;
+};
+
+hipError_t launch_walk(int dim, bool boundary, const WalkLaunch &a, int bie, int est, int flavour, uint32_t block,
+                       uint32_t grid, hipStream_t st);
+hipError_t launch_reduce(const WalkLaunch &a, uint32_t V, hipStream_t st);
+hipError_t launch_finalize(uint32_t *work_counter, uint32_t *pass_counter, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
 hipError_t launch_eval_kernels(int dim, uint32_t n, const float *y, const float *x, const float *ny, const float *nx,
                                float eps, float *out, hipStream_t st);
 hipError_t launch_eval_weights(int dim, int bie, float dsign, float eps, float fkesc, uint32_t n, const float *xr,
@@ -41,5 +61,8 @@ hipError_t launch_eval_weights(int dim, int bie, float dsign, float eps, float f
 hipError_t launch_intersect(int dim, const wobdev::DevScene &S, uint32_t n, const float *o, const float *d,
                             uint32_t max_hits, uint32_t *counts, uint32_t *ids, float *ts, hipStream_t st);
 hipError_t launch_philox(uint32_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out, hipStream_t st);
+
+// Occupancy-derived grid for persistent launches (waves that fit at once on the device).
+uint32_t persistent_grid(int dim, bool boundary, int bie, int est, int flavour, uint32_t block);
 
 }  // namespace wobl
