@@ -63,6 +63,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--paths-per-lane", type=int, default=0)
     ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--profile-copy-mib", type=int, default=0,
+                    help="after the timed steps, copy this many MiB once (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested). "
@@ -101,6 +103,13 @@ def main():
     elapsed = time.perf_counter() - t0
     total_paths = args.steps * n * paths
     value = total_paths / elapsed
+
+    if args.profile_copy_mib:
+        src = torch.ones(args.profile_copy_mib * (1 << 18), dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copied {args.profile_copy_mib} MiB", file=sys.stderr, flush=True)
 
     # walk-kernel duration per launch, HIP events on the launch stream (inside the library)
     est.time_walk = True

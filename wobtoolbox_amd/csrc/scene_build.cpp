@@ -358,21 +358,21 @@ std::string build_scene(const wob_scene_desc &d, HostScene &S) {
             std::memcpy(&nd[NW - 4], &mask, 4);
         }
         if (next_prim != n) return "internal error: primitive count mismatch";
-        // stack bound for DFS "continue with one child, push the others": need(w) =
-        // (#internal children - 1) + max need(child); computed bottom-up (BFS order reversed).
+        // stack bound for DFS "continue with one hit child, push the others" (leaves are
+        // pushed too): need(w) = (#children - 1) + max need(internal child); bottom-up.
         for (size_t w = wides.size(); w-- > 0;) {
-            int k = 0, m = 0;
+            int m = 0;
             for (int c = 0; c < wides[w].nchild; ++c) {
                 const int bn = wides[w].child_bnode[c];
-                if (!B.nodes[bn].leaf()) { ++k; m = std::max(m, need[widx[bn]]); }
+                if (!B.nodes[bn].leaf()) m = std::max(m, need[widx[bn]]);
             }
-            need[w] = std::max(0, k - 1) + m;
+            need[w] = std::max(0, wides[w].nchild - 1) + m;
         }
         S.n_nodes = (uint32_t)wides.size();
         S.depth = maxdepth + 1;
-        S.stack_need = (uint32_t)need[0] + 4;
-        if (S.stack_need <= 64) break;
-        if (attempt == 1) return "BVH stack bound exceeds 64 entries";
+        S.stack_need = (uint32_t)need[0] + 1;
+        if (S.stack_need <= 48) break;
+        if (attempt == 1) return "BVH stack bound exceeds 48 entries";
     }
     return "";
 }

@@ -335,12 +335,13 @@ template <int D> struct LineHit {
 // 725-727): parameters in the closed element, |t| > 10 FLT_EPSILON, no NaN.
 template <int D> __device__ __forceinline__ bool prim_test(const DevScene &S, uint32_t prim, const float *o,
                                                            const float *d, float &t, float &a, float &b) {
+#pragma clang fp contract(fast)
     constexpr float eps = 10 * kFltEps;
     if constexpr (D == 2) {
         const float4 g = S.prims[prim];
         const float sx = g.x - o[0], sy = g.y - o[1];
         const float det = d[0] * g.w - d[1] * g.z;
-        const float inv = 1.0f / det;
+        const float inv = __builtin_amdgcn_rcpf(det);
         t = (sx * g.w - sy * g.z) * inv;
         a = (d[0] * sy - d[1] * sx) * inv;
         b = 0.f;
@@ -351,7 +352,7 @@ template <int D> __device__ __forceinline__ bool prim_test(const DevScene &S, ui
         // q = d x s
         const float q0 = d[1] * s2 - d[2] * s1, q1 = d[2] * s0 - d[0] * s2, q2 = d[0] * s1 - d[1] * s0;
         const float det = d[0] * g2.y + d[1] * g2.z + d[2] * g2.w;       // d . (e1 x e2)
-        const float inv = 1.0f / det;
+        const float inv = __builtin_amdgcn_rcpf(det);                     // hardware reciprocal (1 ulp)
         t = (s0 * g2.y + s1 * g2.z + s2 * g2.w) * inv;                    // s . N / det
         a = (q0 * g1.z + q1 * g1.w + q2 * g2.x) * inv;                    // (d x s) . e2 / det
         b = -(q0 * g0.w + q1 * g1.x + q2 * g1.y) * inv;                   // -(d x s) . e1 / det
@@ -360,72 +361,83 @@ template <int D> __device__ __forceinline__ bool prim_test(const DevScene &S, ui
     }
 }
 
-__device__ __forceinline__ bool slab_hit(float tmin, float tmax) {
-    return tmin <= tmax + 1.0e-6f * (fabsf(tmax) + fabsf(tmin));
-}
+// Child boxes are inflated at build time by 4e-6 x |coordinate| (scene_build.cpp), which is
+// about 30 times the rounding of the FMA slab parameters below; a line meeting an element
+// therefore always meets every box on the element's path to the root, and the plain
+// comparison tmin <= tmax decides.
 
-// Depth-first traversal of the 4-wide BVH for the infinite line o + t d.  Calls
-// on_hit(prim, t, a, b) for every primitive the reference predicate accepts.  Internal
-// children: continue with the first hit child, push the others on this lane's LDS stack
-// (`stack`, entries `stride` words apart; the builder bounds the depth needed).
+// Depth-first traversal of the 4-wide BVH for the infinite line o + t d, calling
+// on_hit(prim, t, a, b) for every primitive the reference predicate accepts.
+//
+// Wave-scheduled: every lane holds one current entry (internal node or leaf) and a range of
+// pending primitives; hit children beyond the first are pushed on the lane's LDS stack
+// (`stack`, entries `stride` words apart; the builder bounds the depth), leaves included.
+// Each iteration the wave runs ONE kind of step, chosen by ballot: a primitive test for the
+// lanes with pending primitives, or a node step (four slab tests) for the lanes that are
+// ready to descend -- whichever more lanes want.  Lanes therefore test primitives together
+// instead of one at a time inside divergent per-child branches.
 template <int D, int STACK, typename OnHit>
 __device__ __forceinline__ void traverse(const DevScene &S, const float *o, const float *d, uint32_t *stack,
                                          uint32_t stride, OnHit &&on_hit) {
-    float inv[D];
+#pragma clang fp contract(fast)
+    constexpr int kEmpty = (int)0x80000000;
+    float inv[D], oi[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) inv[k] = 1.0f / d[k];
-    int node = 0;
+    for (int k = 0; k < D; ++k) {
+        const float dk = fabsf(d[k]) < 1e-30f ? copysignf(1e-30f, d[k]) : d[k];   // finite slab parameters
+        inv[k] = 1.0f / dk;
+        oi[k] = o[k] * inv[k];
+    }
+    int cur = 0;
     int sp = 0;
+    uint32_t lp = 0, le = 0;
     constexpr int NQ = D == 2 ? 6 : 8;   // float4 per node
     while (true) {
-        const float4 *np = S.nodes + (size_t)node * NQ;
-        float4 lo[D], hi[D];
-#pragma unroll
-        for (int k = 0; k < D; ++k) { lo[k] = np[k]; hi[k] = np[D + k]; }
-        const float4 chf = np[2 * D];
-        const float4 meta = np[2 * D + 1];
-        const uint32_t valid = __float_as_uint(meta.x);
-        const int ch[4] = {(int)__float_as_uint(chf.x), (int)__float_as_uint(chf.y), (int)__float_as_uint(chf.z),
-                           (int)__float_as_uint(chf.w)};
-        uint32_t mask = 0;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            float tmin = -INFINITY, tmax = INFINITY;
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-                const float l = c == 0 ? lo[k].x : c == 1 ? lo[k].y : c == 2 ? lo[k].z : lo[k].w;
-                const float u = c == 0 ? hi[k].x : c == 1 ? hi[k].y : c == 2 ? hi[k].z : hi[k].w;
-                const float t1 = (l - o[k]) * inv[k], t2 = (u - o[k]) * inv[k];
-                tmin = fmaxf(tmin, fminf(t1, t2));
-                tmax = fminf(tmax, fmaxf(t1, t2));
-            }
-            mask |= slab_hit(tmin, tmax) ? (1u << c) : 0u;
+        if (lp >= le && cur < 0 && cur != kEmpty) {          // leaf entry -> pending primitives
+            const uint32_t v = ~(uint32_t)cur;
+            lp = v >> 4;
+            le = lp + (v & 15u) + 1u;
+            cur = sp > 0 ? (int)stack[(uint32_t)(--sp) * stride] : kEmpty;
         }
-        mask &= valid;
-        int next = -1;
+        const bool want_leaf = lp < le;
+        const bool want_node = !want_leaf && cur >= 0;
+        const uint64_t bl = __ballot(want_leaf);
+        const uint64_t bn = __ballot(want_node);
+        if ((bl | bn) == 0) break;
+        if (__popcll(bl) >= __popcll(bn)) {
+            if (want_leaf) {
+                float t, a, b;
+                if (prim_test<D>(S, lp, o, d, t, a, b)) on_hit(lp, t, a, b);
+                ++lp;
+            }
+        } else if (want_node) {
+            const float4 *np = S.nodes + (size_t)cur * NQ;
+            float4 lo[D], hi[D];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            if (mask & (1u << c)) {
-                const int cc = ch[c];
-                if (cc < 0) {
-                    const uint32_t v = ~(uint32_t)cc;
-                    const uint32_t first = v >> 4, cnt = (v & 15u) + 1u;
-                    for (uint32_t i = 0; i < cnt; ++i) {
-                        float t, a, b;
-                        if (prim_test<D>(S, first + i, o, d, t, a, b)) on_hit(first + i, t, a, b);
-                    }
-                } else if (next < 0) {
-                    next = cc;
-                } else if (sp < STACK) {
-                    stack[(uint32_t)sp * stride] = (uint32_t)cc;
-                    ++sp;
+            for (int k = 0; k < D; ++k) { lo[k] = np[k]; hi[k] = np[D + k]; }
+            const float4 chf = np[2 * D];
+            const uint32_t valid = __float_as_uint(np[2 * D + 1].x);
+            const int ch[4] = {(int)__float_as_uint(chf.x), (int)__float_as_uint(chf.y), (int)__float_as_uint(chf.z),
+                               (int)__float_as_uint(chf.w)};
+            int next = kEmpty;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                float tmin = -INFINITY, tmax = INFINITY;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const float l = c == 0 ? lo[k].x : c == 1 ? lo[k].y : c == 2 ? lo[k].z : lo[k].w;
+                    const float u = c == 0 ? hi[k].x : c == 1 ? hi[k].y : c == 2 ? hi[k].z : hi[k].w;
+                    const float t1 = __builtin_fmaf(l, inv[k], -oi[k]), t2 = __builtin_fmaf(u, inv[k], -oi[k]);
+                    tmin = fmaxf(tmin, fminf(t1, t2));
+                    tmax = fminf(tmax, fmaxf(t1, t2));
+                }
+                if (((valid >> c) & 1u) && tmin <= tmax) {
+                    if (next == kEmpty) next = ch[c];
+                    else if (sp < STACK) { stack[(uint32_t)sp * stride] = (uint32_t)ch[c]; ++sp; }
                 }
             }
+            cur = next != kEmpty ? next : (sp > 0 ? (int)stack[(uint32_t)(--sp) * stride] : kEmpty);
         }
-        if (next >= 0) { node = next; continue; }
-        if (sp == 0) break;
-        --sp;
-        node = (int)stack[(uint32_t)sp * stride];
     }
 }
 

@@ -11,7 +11,7 @@ namespace wobl {
 
 // Per-lane LDS traversal stack (uint32 entries).  The scene builder rejects BVHs whose
 // depth-first stack bound exceeds this.
-constexpr int kStackEntries = 32;
+constexpr int kStackEntries = 48;
 
 // One (scene, estimator, point set) of a launch.  A single estimate call is a one-entry
 // launch; a batch packs many entries (scenes of different sizes, uneven point sets) into one
