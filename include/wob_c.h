@@ -104,6 +104,8 @@ typedef struct wob_estimator_desc {
     float eps;                            /* kernel clamp, default FLT_EPSILON */
     float first_kind_equation_scaling_constant;  /* default 1 */
     uint32_t flags;                       /* WOB_EST_* options, 0 = the reference's behaviour */
+    uint32_t cv_samples;                  /* WOB_EST_CONTROL_VARIATE: samples of the domain-level
+                                             control-variate integral (0 = 64) */
 } wob_estimator_desc;
 
 /* Estimator options (no counterpart in the reference).
@@ -113,6 +115,15 @@ typedef struct wob_estimator_desc {
  *   few degrees of that element re-hits it at a larger |t| and adds a spurious hit (a few
  *   percent of bounces), which inflates the variance and makes paths sensitive to rounding. */
 enum { WOB_EST_EXCLUDE_ORIGIN_ELEMENT = 1 };
+/* WOB_EST_CONTROL_VARIATE (single-layer model, backward estimator): use the value table bound
+ *   with wob_scene_set_control_variate as a control variate.  Every added term has zero mean
+ *   for any table, so the estimator's expectation is exactly that of plain WoB; a table close
+ *   to the walk's value functions removes most of the path-choice variance.
+ * WOB_EST_BOUNDARY_DENSITY (backward estimator, boundary entry points): return the walk's
+ *   boundary-density estimate at each record (compute_sample_unknown_boundary,
+ *   estimator.cuh:232-357) instead of the solution; this generates control-variate training
+ *   data.                                                                                    */
+enum { WOB_EST_CONTROL_VARIATE = 2, WOB_EST_BOUNDARY_DENSITY = 4 };
 
 /* Launch tuning and timing (in/out; may be NULL).  paths_per_lane: paths one lane
  * accumulates before it writes its partial sums; block_size: threads per workgroup
@@ -191,6 +202,41 @@ wob_status wob_batch_run(wob_batch *batch, uint64_t seed, uint32_t n_passes, voi
  * reports it in *best (may be NULL) with its pass time in *best_ms.                          */
 wob_status wob_batch_tune(wob_batch *batch, uint64_t seed, wob_batch_config *best, float *best_ms, void *stream);
 wob_status wob_batch_destroy(wob_batch *batch);
+
+/* Learned control variate (no counterpart in the reference): a Fourier-feature MLP
+ * (64 Gaussian frequencies of scale `sigma`, 4 hidden layers of width 128, ReLU,
+ * n_outputs <= 16 outputs) trained and evaluated by BF16 MFMA kernels with FP32
+ * accumulation and FP32 Adam master weights.  Output r - 1 models V_r, the expected value of
+ * a single-layer walk of r bounces from a boundary point.                                   */
+typedef struct wob_cv wob_cv;
+wob_status wob_cv_create(uint32_t dim, uint32_t n_outputs, float sigma, uint64_t seed, wob_cv **out_cv);
+wob_status wob_cv_destroy(wob_cv *cv);
+/* n_params: FP32 parameter count (layer l: W_l (out x in) then b_l).                        */
+wob_status wob_cv_get_info(const wob_cv *cv, uint32_t *n_params, uint32_t *n_outputs);
+/* host copies of the FP32 master parameters and the 64 x 3 Fourier frequencies             */
+wob_status wob_cv_get_params(const wob_cv *cv, float *params, float *freq);
+wob_status wob_cv_set_params(wob_cv *cv, const float *params, const float *freq);
+/* Training: `steps` Adam steps over consecutive mini-batches of `batch` samples (wrapping
+ * around n); sample i has position d_pos[i] (dim floats), output index d_out_index[i] and
+ * target d_target[i] (masked mean squared error).  losses (host, may be NULL) receives the
+ * mean loss of every step.  lr decays by a cosine schedule to lr / 20.                      */
+wob_status wob_cv_train(wob_cv *cv, const float *d_pos, const int32_t *d_out_index, const float *d_target, uint32_t n,
+                        uint32_t batch, uint32_t steps, float lr, float *losses, void *stream);
+/* Gradient of the batch loss (no update): grad (host, n_params floats), loss (host).        */
+wob_status wob_cv_gradient(wob_cv *cv, const float *d_pos, const int32_t *d_out_index, const float *d_target,
+                           uint32_t n, float *grad, float *loss, void *stream);
+wob_status wob_cv_infer(wob_cv *cv, const float *d_pos, uint32_t n, float *d_out, void *stream);
+/* Evaluate the network at every element vertex and bind the table (values r = 1..n_values)
+ * to the scene; cv == NULL with d_table != NULL binds a caller table [n_values][n][dim]
+ * (device) instead; both NULL unbinds.                                                      */
+wob_status wob_scene_set_control_variate(wob_scene *scene, wob_cv *cv, const float *d_table, uint32_t n_values,
+                                         void *stream);
+
+/* Uniform boundary samples as BoundaryPoint records (Scene::sample_boundary_uniform,
+ * scene.cuh:441-462 / 734-758): record i uses the Philox stream (seed, path i, point 0,
+ * stream_id).  d_records: n * (2*dim + 4) floats (device).                                  */
+wob_status wob_sample_boundary(const wob_scene *scene, uint32_t n, uint64_t seed, uint32_t stream_id, float *d_records,
+                               void *stream);
 
 /* All-hit line query (the lbvh query_line_intersect call of scene.cuh:481-486 / 787-790,
  * without its 64-hit cap).  For each line i (origin, direction; device arrays of n*dim

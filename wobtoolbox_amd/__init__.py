@@ -135,6 +135,7 @@ class Estimator:
     first_kind_equation_scaling_constant: float = 1.0
     num_volume_samples: int = 0
     flags: int = 0
+    cv_samples: int = 0
     paths_per_lane: int = 0
     block_size: int = 0
 
@@ -151,7 +152,7 @@ class Estimator:
             PoissonKernel, _enum(self.bie_model, string_to_bie_model_type), _enum(self.domain, string_to_domain_type),
             _enum(self.estimator, string_to_estimator_type), _enum(self.problem, string_to_problem_type),
             int(self.path_length), int(self.num_resampling_candidates), int(self.num_volume_samples), float(self.eps),
-            float(self.first_kind_equation_scaling_constant), int(self.flags))
+            float(self.first_kind_equation_scaling_constant), int(self.flags), int(self.cv_samples))
 
     time_walk: bool = False
     last_launch: object = None

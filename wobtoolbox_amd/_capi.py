@@ -19,7 +19,9 @@ EXPORTS = ["wob_scene_create", "wob_scene_destroy", "wob_scene_get_info", "wob_s
            "wob_estimate_domain", "wob_estimate_boundary", "wob_intersect_lines", "wob_eval_kernels",
            "wob_eval_weights", "wob_philox4x32_10", "wob_string_to_enum", "wob_enum_to_string",
            "wob_last_error_message", "wob_version", "wob_batch_create", "wob_batch_run", "wob_batch_tune",
-           "wob_batch_destroy"]
+           "wob_batch_destroy", "wob_cv_create", "wob_cv_destroy", "wob_cv_get_info", "wob_cv_get_params",
+           "wob_cv_set_params", "wob_cv_train", "wob_cv_gradient", "wob_cv_infer", "wob_scene_set_control_variate",
+           "wob_sample_boundary"]
 
 
 class WobError(RuntimeError):
@@ -42,10 +44,13 @@ class EstimatorDesc(C.Structure):
     _fields_ = [("equation_kernel", C.c_int32), ("bie_model", C.c_int32), ("domain", C.c_int32),
                 ("estimator", C.c_int32), ("problem", C.c_int32), ("path_length", C.c_uint32),
                 ("num_resampling_candidates", C.c_uint32), ("num_volume_samples", C.c_uint32), ("eps", C.c_float),
-                ("first_kind_equation_scaling_constant", C.c_float), ("flags", C.c_uint32)]
+                ("first_kind_equation_scaling_constant", C.c_float), ("flags", C.c_uint32),
+                ("cv_samples", C.c_uint32)]
 
 
 EST_EXCLUDE_ORIGIN_ELEMENT = 1
+EST_CONTROL_VARIATE = 2
+EST_BOUNDARY_DENSITY = 4
 
 
 class BatchEntry(C.Structure):
@@ -101,6 +106,16 @@ def lib():
         "wob_batch_run": ([P, U64, U32, P], C.c_int),
         "wob_batch_tune": ([P, U64, C.POINTER(BatchConfig), C.POINTER(C.c_float), P], C.c_int),
         "wob_batch_destroy": ([P], C.c_int),
+        "wob_cv_create": ([U32, U32, F, U64, C.POINTER(P)], C.c_int),
+        "wob_cv_destroy": ([P], C.c_int),
+        "wob_cv_get_info": ([P, C.POINTER(U32), C.POINTER(U32)], C.c_int),
+        "wob_cv_get_params": ([P, P, P], C.c_int),
+        "wob_cv_set_params": ([P, P, P], C.c_int),
+        "wob_cv_train": ([P, P, P, P, U32, U32, U32, F, P, P], C.c_int),
+        "wob_cv_gradient": ([P, P, P, P, U32, P, C.POINTER(F), P], C.c_int),
+        "wob_cv_infer": ([P, P, U32, P, P], C.c_int),
+        "wob_scene_set_control_variate": ([P, P, P, U32, P], C.c_int),
+        "wob_sample_boundary": ([P, U32, U64, U32, P, P], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

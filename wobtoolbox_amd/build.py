@@ -25,7 +25,9 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off"]
 
 SOURCES = [
     ("wob_kernels.hip", "hip"),
+    ("mlp_kernels.hip", "hip"),
     ("wob_capi.cpp", "hip"),
+    ("wob_cv.cpp", "hip"),
     ("scene_build.cpp", "host"),
 ]
 

@@ -16,27 +16,25 @@
 
 #include "../../include/wob_c.h"
 #include "scene_build.hpp"
+#include "wob_capi_internal.hpp"
 #include "wob_launch.hpp"
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-wob_status fail(wob_status s, const std::string &msg) {
+wob_status wob_fail(wob_status s, const std::string &msg) {
     g_last_error = msg;
     return s;
 }
 
-wob_status hip_fail(hipError_t e, const char *what) {
-    return fail(WOB_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+wob_status wob_hip_fail(hipError_t e, const char *what) {
+    return wob_fail(WOB_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define WOB_HIP(call, what)                               \
-    do {                                                  \
-        hipError_t e_ = (call);                           \
-        if (e_ != hipSuccess) return hip_fail(e_, what);  \
-    } while (0)
-
+namespace {
+inline wob_status fail(wob_status s, const std::string &msg) { return wob_fail(s, msg); }
+inline wob_status hip_fail(hipError_t e, const char *what) { return wob_hip_fail(e, what); }
 }  // namespace
 
 struct wob_scene {
@@ -47,7 +45,18 @@ struct wob_scene {
     size_t scratch_bytes = 0;
     uint64_t device_bytes = 0;
     int device = 0;
+    float *cv_table = nullptr;
 };
+
+const wobhost::HostScene &wob_scene_host(const wob_scene *scene) { return scene->host; }
+
+wob_status wob_scene_bind_cv_table(wob_scene *scene, float *d_table, uint32_t R) {
+    if (scene->cv_table) (void)hipFree(scene->cv_table);
+    scene->cv_table = d_table;
+    scene->dev.cv_table = d_table;
+    scene->dev.cv_R = d_table ? R : 0u;
+    return WOB_OK;
+}
 
 namespace {
 
@@ -68,6 +77,8 @@ void free_scene(wob_scene *s) {
         if (b) { (void)hipFree(b); b = nullptr; }
     if (s->scratch) (void)hipFree(s->scratch);
     s->scratch = nullptr;
+    if (s->cv_table) (void)hipFree(s->cv_table);
+    s->cv_table = nullptr;
 }
 
 wob_status ensure_scratch(wob_scene *s, size_t bytes) {
@@ -91,7 +102,16 @@ wob_status check_est(const wob_scene *scene, const wob_estimator_desc *est, int 
     if (est->estimator < 0 || est->estimator > 1) return fail(WOB_INVALID_ENUM, "estimator out of range");
     if (est->problem < 0 || est->problem > 3) return fail(WOB_INVALID_ENUM, "problem out of range");
     if (est->path_length == 0) return fail(WOB_INVALID_ARG, "path_length must be >= 1");
-    if (est->flags & ~(uint32_t)WOB_EST_EXCLUDE_ORIGIN_ELEMENT) return fail(WOB_INVALID_ARG, "unknown estimator flags");
+    if (est->flags & ~(uint32_t)(WOB_EST_EXCLUDE_ORIGIN_ELEMENT | WOB_EST_CONTROL_VARIATE | WOB_EST_BOUNDARY_DENSITY))
+        return fail(WOB_INVALID_ARG, "unknown estimator flags");
+    if (est->flags & WOB_EST_CONTROL_VARIATE) {
+        if (est->bie_model != WOB_SINGLE_LAYER || est->estimator != WOB_BACKWARD)
+            return fail(WOB_UNSUPPORTED_COMBO, "the control variate needs the single-layer model and the backward estimator");
+        if (scene->dev.cv_R < est->path_length)
+            return fail(WOB_INVALID_ARG, "no control-variate table with path_length value functions is bound to the scene");
+    }
+    if ((est->flags & WOB_EST_BOUNDARY_DENSITY) && est->estimator != WOB_BACKWARD)
+        return fail(WOB_UNSUPPORTED_COMBO, "boundary density estimates need the backward estimator");
     if (est->num_resampling_candidates == 0) return fail(WOB_INVALID_ARG, "num_resampling_candidates must be >= 1");
     if (est->num_volume_samples != 0)
         return fail(WOB_UNSUPPORTED_COMBO, "volume sources are not enabled in this build (num_volume_samples must be 0)");
@@ -150,6 +170,7 @@ wobl::WalkEntry make_entry(const wob_scene *scene, const wob_estimator_desc *est
     e.fkesc = est->first_kind_equation_scaling_constant;
     e.path_base = path_base;
     e.flags = est->flags;
+    e.cv_samples = est->cv_samples ? est->cv_samples : 64u;
     return e;
 }
 
@@ -449,6 +470,17 @@ wob_status wob_eval_weights(uint32_t dim, int32_t bie_model, int32_t domain, flo
     WOB_HIP(wobl::launch_eval_weights((int)dim, bie_model, domain == WOB_INTERIOR ? 1.f : -1.f, eps, fkesc, n, d_x, d_y,
                                       d_inv_pdf, d_xdomain, d_u_split, d_out, (hipStream_t)stream),
             "eval_weights launch");
+    return WOB_OK;
+}
+
+wob_status wob_sample_boundary(const wob_scene *scene, uint32_t n, uint64_t seed, uint32_t stream_id, float *d_records,
+                               void *stream) {
+    if (!scene) return fail(WOB_INVALID_ARG, "null scene");
+    if (n == 0) return WOB_OK;
+    if (!d_records) return fail(WOB_INVALID_ARG, "null device buffer");
+    WOB_HIP(wobl::launch_sample_boundary((int)scene->host.dim, scene->dev, n, seed, stream_id, d_records,
+                                         (hipStream_t)stream),
+            "sample_boundary launch");
     return WOB_OK;
 }
 

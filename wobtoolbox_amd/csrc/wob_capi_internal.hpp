@@ -1,0 +1,23 @@
+// Helpers shared by the C-ABI translation units (wob_capi.cpp, wob_cv.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/wob_c.h"
+#include "scene_build.hpp"
+
+// record a message for wob_last_error_message() and return the status
+wob_status wob_fail(wob_status s, const std::string &msg);
+wob_status wob_hip_fail(hipError_t e, const char *what);
+
+#define WOB_HIP(call, what)                                   \
+    do {                                                      \
+        hipError_t e_ = (call);                               \
+        if (e_ != hipSuccess) return wob_hip_fail(e_, what);  \
+    } while (0)
+
+const wobhost::HostScene &wob_scene_host(const wob_scene *scene);
+// takes ownership of d_table (device, [R][n][dim] floats); frees the previous table
+wob_status wob_scene_bind_cv_table(wob_scene *scene, float *d_table, uint32_t R);

@@ -49,7 +49,8 @@ struct DevScene {
     uint32_t n;
     float total_area;
     float total_bv;
-    uint32_t pad;
+    uint32_t cv_R;            // number of value functions in cv_table (0: none)
+    const float *cv_table;    // control variate values per (r - 1, element, vertex): [cv_R][n][D]
 };
 
 // ----------------------------------------------------------------------------------------
@@ -119,6 +120,7 @@ template <int D> struct BP {
     float alpha;
     int type;
     uint32_t elem;   // element the point lies on (0xFFFFFFFF: none / unknown)
+    float bw[D];     // weights of the element's vertices at p (control-variate lookups)
 };
 
 template <int D> __device__ __forceinline__ BP<D> bp_zero() {
@@ -126,6 +128,8 @@ template <int D> __device__ __forceinline__ BP<D> bp_zero() {
 #pragma unroll
     for (int k = 0; k < D; ++k) { b.p[k] = 0.f; b.n[k] = 0.f; }
     b.bv = 0.f; b.alpha = 0.f; b.type = kDirichlet; b.elem = 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < D; ++k) b.bw[k] = 0.f;
     return b;
 }
 
@@ -272,6 +276,7 @@ template <int D> __device__ __forceinline__ void uniform_in_element(const float 
         for (int k = 0; k < 2; ++k) b.p[k] = v[k] + u2 * (v[2 + k] - v[k]);          // interpolate (scene.cuh:394-396)
         b.bv = bvv[0] + u2 * (bvv[1] - bvv[0]);
         b.alpha = alv[0] + u2 * (alv[1] - alv[0]);
+        b.bw[0] = 1.0f - u2; b.bw[1] = u2;
     } else {
         const float u2 = rng.uniform();
         const float u3 = rng.uniform();
@@ -281,6 +286,7 @@ template <int D> __device__ __forceinline__ void uniform_in_element(const float 
         for (int k = 0; k < 3; ++k) b.p[k] = w0 * v[k] + w1 * v[3 + k] + w2 * v[6 + k];
         b.bv = w0 * bvv[0] + w1 * bvv[1] + w2 * bvv[2];
         b.alpha = w0 * alv[0] + w1 * alv[1] + w2 * alv[2];
+        b.bw[0] = w0; b.bw[1] = w1; b.bw[2] = w2;
     }
 }
 
@@ -313,6 +319,7 @@ template <int D> __device__ __forceinline__ Sample<D> sample_bv(const DevScene &
         for (int k = 0; k < 2; ++k) r.y.p[k] = v[k] + interp * (v[2 + k] - v[k]);
         r.y.bv = bvv[0] + interp * (bvv[1] - bvv[0]);
         r.y.alpha = alv[0] + interp * (alv[1] - alv[0]);
+        r.y.bw[0] = 1.0f - interp; r.y.bw[1] = interp;
         r.inv_pdf = S.total_bv / (ba + interp * (bb - ba));
     } else {
         const float pmf = e == 0 ? S.bv_cdf[e] : S.bv_cdf[e] - S.bv_cdf[e - 1];
@@ -497,10 +504,12 @@ __device__ __forceinline__ Sample<D> line_sample(const DevScene &S, const float 
         const float interp = sqrtf(dx * dx + dy * dy) / area;                           // scene.cuh:505
         r.y.bv = bvv[0] + interp * (bvv[1] - bvv[0]);
         r.y.alpha = alv[0] + interp * (alv[1] - alv[0]);
+        r.y.bw[0] = 1.0f - interp; r.y.bw[1] = interp;
     } else {
         const float w0 = 1.0f - (h.a + h.b);
         r.y.bv = w0 * bvv[0] + h.a * bvv[1] + h.b * bvv[2];
         r.y.alpha = w0 * alv[0] + h.a * alv[1] + h.b * alv[2];
+        r.y.bw[0] = w0; r.y.bw[1] = h.a; r.y.bw[2] = h.b;
     }
     constexpr float eps = 10 * kFltEps;
     float rv[D];
@@ -676,34 +685,59 @@ struct WalkParams {
     uint32_t path_length;
     uint32_t nrc;
     uint32_t exclude_origin;   // WOB_EST_EXCLUDE_ORIGIN_ELEMENT
+    uint32_t cv;               // WOB_EST_CONTROL_VARIATE (single-layer backward walks)
+    uint32_t cv_samples;       // samples of the domain-level control-variate integral
+    uint32_t density;          // WOB_EST_BOUNDARY_DENSITY: boundary estimates return the walk value
     __device__ __forceinline__ uint32_t skip(uint32_t elem) const { return exclude_origin ? elem : 0xFFFFFFFFu; }
 };
 
+// Control-variate value V_r at a sampled point: linear interpolation of the per-vertex table
+// of value function r (remaining path length r = 1 .. cv_R).
+template <int D> __device__ __forceinline__ float cv_value(const DevScene &S, uint32_t r, const BP<D> &b) {
+    if (b.elem == 0xFFFFFFFFu || r == 0 || r > S.cv_R) return 0.f;
+    const float *t = S.cv_table + ((size_t)(r - 1) * S.n + b.elem) * D;
+    float v = b.bw[0] * t[0];
+#pragma unroll
+    for (int k = 1; k < D; ++k) v += b.bw[k] * t[k];
+    return v;
+}
+
 // resampled importance sampling, scene.cuh:542-584 (target by the type of y_prev,
-// estimator.cuh:286-328); candidates are uniform samples
+// estimator.cuh:286-328); candidates are uniform samples.  With cv_r > 0 it also returns
+// ksum = sum_i s_i V_{cv_r}(z_i), s_i the signed kernel whose magnitude is the target.
 template <int D, int BIE>
 __device__ __forceinline__ Sample<D> ris(const DevScene &S, const Model<D, BIE> &m, const BP<D> &yp, uint32_t nrc,
-                                         Rng &rng) {
+                                         Rng &rng, uint32_t cv_r, float &ksum) {
     Sample<D> r;
     r.y = bp_zero<D>();
     float w_sum = 0.f;
+    ksum = 0.f;
     const int tp = yp.type;
-    auto target = [&](const BP<D> &y) -> float {
-        if (tp == kDirichlet) return fabsf(m.k.G(y.p, yp.p));
-        if (tp == kNeumann) return fabsf(m.k.dG_dny(yp.p, y.p, yp.n));
-        return fabsf(m.k.dG_dny(yp.p, y.p, yp.n) + yp.alpha * m.k.G(y.p, yp.p));
+    auto signed_kernel = [&](const BP<D> &y) -> float {
+        if (tp == kDirichlet) return m.k.G(y.p, yp.p);
+        if (tp == kNeumann) return m.k.dG_dny(yp.p, y.p, yp.n);
+        return m.k.dG_dny(yp.p, y.p, yp.n) + yp.alpha * m.k.G(y.p, yp.p);
     };
     for (uint32_t i = 0; i < nrc; ++i) {
         const Sample<D> xi = sample_uniform<D>(S, rng);
-        const float w = target(xi.y) * xi.inv_pdf;
+        const float sk = signed_kernel(xi.y);
+        const float w = fabsf(sk) * xi.inv_pdf;
+        if (cv_r) ksum += sk * cv_value<D>(S, cv_r, xi.y);
         w_sum += w;
         if (rng.uniform() <= w / w_sum) r.y = xi.y;
     }
-    r.inv_pdf = w_sum / fmaxf(target(r.y) * (float)nrc, 10 * kFltEps);
+    r.inv_pdf = w_sum / fmaxf(fabsf(signed_kernel(r.y)) * (float)nrc, 10 * kFltEps);
     return r;
 }
 
 // estimator.cuh:232-357
+// With the control variate (single-layer walks), zero-mean terms
+//   D_m = W_m E[fu_m V_{r}(y_{m+1}) | y_m, candidates] - W_{m+1} V_{r}(y_{m+1}),  r = L - m - 1,
+// are added for m <= L - 2, where the conditional mean is exact given the RIS candidates:
+//   Dirichlet y_m (first-kind split): V_r(y_m) - 2c |G| / M sum_i G(z_i, y_m) V_r(z_i)
+//   Neumann / Robin y_m:              -(dsign / ift) |G| / M sum_i s_i V_r(z_i).
+// The walk's expectation is unchanged for any table V; with V equal to the value functions
+// of the walk the path-choice randomness cancels.
 template <int D, int BIE, int FL, int STACK>
 __device__ __forceinline__ float walk_backward(const DevScene &S, const Model<D, BIE> &m, const WalkParams &wp,
                                                const BP<D> &x0, Rng &rng, uint32_t *stack, uint32_t stride) {
@@ -724,8 +758,28 @@ __device__ __forceinline__ float walk_backward(const DevScene &S, const Model<D,
             yp = q.y;
         } else if constexpr (BIE == kSL) {
             spc += W * m.fredholm_known(yp, yp, 0.f);   // SL ignores y here (estimator.cuh:278-281)
-            Sample<D> q = wp.nrc == 1 ? sample_uniform<D>(S, rng) : ris<D, BIE>(S, m, yp, wp.nrc, rng);
-            W = W * m.fredholm_unknown_backward(yp, q.y, q.inv_pdf, rng);
+            const uint32_t cv_r = (wp.cv && it + 1 < wp.path_length) ? wp.path_length - it - 1 : 0u;
+            float ksum = 0.f;
+            Sample<D> q;
+            if (wp.nrc == 1) {
+                q = sample_uniform<D>(S, rng);
+                if (cv_r) {
+                    const float sk = yp.type == kDirichlet ? m.k.G(q.y.p, yp.p)
+                                     : yp.type == kNeumann ? m.k.dG_dny(yp.p, q.y.p, yp.n)
+                                                           : m.k.dG_dny(yp.p, q.y.p, yp.n) + yp.alpha * m.k.G(q.y.p, yp.p);
+                    ksum = sk * cv_value<D>(S, cv_r, q.y);
+                }
+            } else {
+                q = ris<D, BIE>(S, m, yp, wp.nrc, rng, cv_r, ksum);
+            }
+            const float fu = m.fredholm_unknown_backward(yp, q.y, q.inv_pdf, rng);
+            if (cv_r) {
+                const float scale = S.total_area / (float)wp.nrc;
+                const float e = yp.type == kDirichlet ? cv_value<D>(S, cv_r, yp) - 2.f * m.c * scale * ksum
+                                                      : -(m.dsign / m.ift) * scale * ksum;
+                spc += W * e - (W * fu) * cv_value<D>(S, cv_r, q.y);
+            }
+            W = W * fu;
             yp = q.y;
         } else {
             Sample<D> q = sample_uniform<D>(S, rng);
@@ -771,6 +825,31 @@ __device__ __forceinline__ void domain_path(const DevScene &S, const Model<D, BI
         out[0] = known + spc * unk;
 #pragma unroll
         for (int k = 0; k < D; ++k) out[1 + k] = gk[k] + spc * gu[k];
+        if constexpr (BIE == kSL && FL == FL_UNIFORM) {
+            if (wp.cv) {
+                // I(x) - unk V_L(y0): I(x) = |G| / M0 sum_j G(z_j, x) V_L(z_j) from an independent
+                // Philox stream, so a zero table leaves every other draw and the result unchanged
+                const float v0 = cv_value<D>(S, wp.path_length, y.y);
+                Rng r2 = rng;
+                r2.block = 0; r2.left = 0; r2.stream = rng.stream ^ 0x80000000u;
+                float iu = 0.f, ig[D];
+#pragma unroll
+                for (int k = 0; k < D; ++k) ig[k] = 0.f;
+                for (uint32_t j = 0; j < wp.cv_samples; ++j) {
+                    const Sample<D> z = sample_uniform<D>(S, r2);
+                    const float v = cv_value<D>(S, wp.path_length, z.y);
+                    iu += m.k.G(z.y.p, x) * v;
+                    float g[D];
+                    m.k.dG_dx(z.y.p, x, g);
+#pragma unroll
+                    for (int k = 0; k < D; ++k) ig[k] += g[k] * v;
+                }
+                const float sc = wp.cv_samples ? S.total_area / (float)wp.cv_samples : 0.f;
+                out[0] += sc * iu - unk * v0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) out[1 + k] += sc * ig[k] - gu[k] * v0;
+            }
+        }
     } else {                                                                                   // estimator.cuh:405-454
         Sample<D> y = sample_uniform<D>(S, rng);
         out[0] = m.solution_domain_known(x, y.y, y.inv_pdf);
@@ -804,6 +883,7 @@ template <int D, int BIE, int EST, int FL, int STACK>
 __device__ __forceinline__ float boundary_path(const DevScene &S, const Model<D, BIE> &m, const WalkParams &wp,
                                                const BP<D> &x, Rng &rng, uint32_t *stack, uint32_t stride) {
     if constexpr (EST == kBackward) {
+        if (wp.density) return walk_backward<D, BIE, FL, STACK>(S, m, wp, x, rng, stack, stride);
         Sample<D> y, y1;
         first_samples<D, FL, STACK>(S, x.p, wp.skip(x.elem), y, y1, rng, stack, stride);
         float res = m.solution_boundary_known(x, y1.y, y1.inv_pdf);

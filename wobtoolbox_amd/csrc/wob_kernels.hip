@@ -73,7 +73,7 @@ __device__ __forceinline__ void walk_item(const WalkLaunch &L, uint32_t item, ui
     m.k.eps = e.eps;
     m.dsign = e.dsign;
     m.c = e.fkesc;
-    const WalkParams wp{e.path_length, e.nrc, e.flags & 1u};
+    const WalkParams wp{e.path_length, e.nrc, e.flags & 1u, (e.flags >> 1) & 1u, e.cv_samples, (e.flags >> 2) & 1u};
 
     Kahan ks[V], kq[V];
 #pragma unroll
@@ -88,6 +88,8 @@ __device__ __forceinline__ void walk_item(const WalkLaunch &L, uint32_t item, ui
         x.type = (int)r[2 * D + 2];
         x.alpha = r[2 * D + 3];
         x.elem = 0xFFFFFFFFu;
+#pragma unroll
+        for (int k = 0; k < D; ++k) x.bw[k] = 0.f;
         for (uint32_t j = j0; j < j1; ++j) {
             Rng rng;
             rng.init(L.seed, (uint32_t)(path_base + j), e.point_base + pt, e.stream);
@@ -208,6 +210,8 @@ template <int D> __device__ __forceinline__ BP<D> load_record(const float *r) {
     b.type = (int)r[2 * D + 2];
     b.alpha = r[2 * D + 3];
     b.elem = 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < D; ++k) b.bw[k] = 0.f;
     return b;
 }
 
@@ -265,6 +269,22 @@ __global__ __launch_bounds__(256) void intersect_kernel(DevScene S, uint32_t n, 
         ++c;
     });
     counts[i] = c;
+}
+
+template <int D>
+__global__ void sample_boundary_kernel(DevScene S, uint32_t n, uint64_t seed, uint32_t stream, float *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Rng rng;
+    rng.init(seed, i, 0u, stream);
+    const Sample<D> z = sample_uniform<D>(S, rng);
+    float *r = out + (size_t)i * (2 * D + 4);
+#pragma unroll
+    for (int k = 0; k < D; ++k) { r[k] = z.y.p[k]; r[D + k] = z.y.n[k]; }
+    r[2 * D] = 0.5f;
+    r[2 * D + 1] = z.y.bv;
+    r[2 * D + 2] = (float)z.y.type;
+    r[2 * D + 3] = z.y.alpha;
 }
 
 __global__ void philox_kernel(uint32_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
@@ -380,6 +400,14 @@ hipError_t launch_intersect(int dim, const DevScene &S, uint32_t n, const float 
         hipLaunchKernelGGL((intersect_kernel<2, kStackEntries>), dim3(grid), dim3(block), lds, st, S, n, o, d, max_hits, counts, ids, ts);
     else
         hipLaunchKernelGGL((intersect_kernel<3, kStackEntries>), dim3(grid), dim3(block), lds, st, S, n, o, d, max_hits, counts, ids, ts);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample_boundary(int dim, const DevScene &S, uint32_t n, uint64_t seed, uint32_t stream, float *out,
+                                  hipStream_t st) {
+    const uint32_t block = 256, grid = (n + block - 1) / block;
+    if (dim == 2) hipLaunchKernelGGL(sample_boundary_kernel<2>, dim3(grid), dim3(block), 0, st, S, n, seed, stream, out);
+    else hipLaunchKernelGGL(sample_boundary_kernel<3>, dim3(grid), dim3(block), 0, st, S, n, seed, stream, out);
     return hipGetLastError();
 }
 

@@ -32,6 +32,7 @@ struct WalkEntry {
     uint32_t point_begin;     // first global point of this entry (reduction)
     uint32_t partial_base;    // offset of this entry's partials (floats)
     uint32_t flags;           // WOB_EST_* options
+    uint32_t cv_samples;      // domain-level control-variate samples
     uint64_t path_base;       // first path index of pass 0
 };
 
@@ -60,6 +61,8 @@ hipError_t launch_eval_weights(int dim, int bie, float dsign, float eps, float f
                                hipStream_t st);
 hipError_t launch_intersect(int dim, const wobdev::DevScene &S, uint32_t n, const float *o, const float *d,
                             uint32_t max_hits, uint32_t *counts, uint32_t *ids, float *ts, hipStream_t st);
+hipError_t launch_sample_boundary(int dim, const wobdev::DevScene &S, uint32_t n, uint64_t seed, uint32_t stream,
+                                  float *out, hipStream_t st);
 hipError_t launch_philox(uint32_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out, hipStream_t st);
 
 // Occupancy-derived grid for persistent launches (waves that fit at once on the device).
