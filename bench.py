@@ -54,6 +54,53 @@ def cpu_baseline(workload, cores, target_s=10.0):
                       f"OpenMP static schedule over points"}
 
 
+def time_to_rmse_leg(args, torch, wob, G):
+    """Plain WoB vs learned control variate on the 200k-triangle geodesic mixed workload.
+
+    Both arms run progressive passes for the same walk-time budget; the control-variate arm's
+    clock starts before its training data are generated and includes training."""
+    from wobtoolbox_amd import cv as CV
+    from wobtoolbox_amd import progressive
+    print("[bench] time-to-RMSE: building geodesic_mixed_3d", file=sys.stderr, flush=True)
+    w = G.geodesic_mixed_3d(n_points=args.cv_points)
+    dev = torch.device("cuda:0")
+    scene = wob.SceneHost(w.elements)
+    pts = torch.as_tensor(w.points).to(dev)
+    exact_u = torch.as_tensor(w.exact(w.points)[:, 0]).to(dev)
+    plain = wob.Estimator.from_config(scene, w.solver)
+    L = w.solver["path_length"]
+    print("[bench] time-to-RMSE: plain arm", file=sys.stderr, flush=True)
+    curve_plain = progressive.run(plain, pts, exact_u, args.cv_paths_per_pass, args.cv_budget, seed=1)
+    print("[bench] time-to-RMSE: control-variate arm", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pos, idx, tgt = CV.training_data(plain, args.cv_train_points, seed=2, paths_per_sample=args.cv_train_paths)
+    net = CV.ControlVariate(3, L, sigma=1.0, seed=3)
+    net.train(pos, idx, tgt, batch=16384, steps=args.cv_train_steps, lr=2e-3)
+    net.bind(scene, L)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    cvest = wob.Estimator.from_config(scene, w.solver, flags=2)
+    curve_cv = progressive.run(cvest, pts, exact_u, args.cv_paths_per_pass, t_setup + args.cv_budget, t0=t0, seed=1)
+    target = 1e-3
+    tp, reached_p = progressive.time_to_rmse(curve_plain, target)
+    tc, reached_c = progressive.time_to_rmse(curve_cv, target, t_setup)
+    # variance efficiency: rmse^2 x walk seconds at the last checkpoint
+    eff_p = curve_plain[-1][2] ** 2 * curve_plain[-1][0]
+    eff_c = curve_cv[-1][2] ** 2 * (curve_cv[-1][0] - t_setup)
+    return {"target_rmse": target, "workload": w.name, "elements": w.elements.n, "points": len(w.points),
+            "solver": {k: w.solver[k] for k in ("bie_model", "problem", "path_length", "num_resampling_candidates",
+                                                 "first_kind_equation_scaling_constant")},
+            "plain_s": tp, "plain_reached": reached_p, "cv_s": tc, "cv_reached": reached_c, "speedup": tp / tc,
+            "cv_setup_s": t_setup, "cv_training_walks": args.cv_train_points * L * args.cv_train_paths,
+            "efficiency_ratio": eff_p / eff_c,
+            "note": "not reached: extrapolated as setup + rmse^2 x walk time / target^2 (both arms estimate the "
+                    "same expectation; truncation bias of the path_length-4 series is not resolved at these "
+                    "RMSE levels)",
+            "plain_curve": [[round(t, 3), k, r] for t, k, r in curve_plain],
+            "cv_curve": [[round(t, 3), k, r] for t, k, r in curve_cv]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +110,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--paths-per-lane", type=int, default=0)
     ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--no-time-to-rmse", action="store_true")
+    ap.add_argument("--cv-points", type=int, default=1_000_000)
+    ap.add_argument("--cv-budget", type=float, default=15.0, help="walk seconds per arm")
+    ap.add_argument("--cv-paths-per-pass", type=int, default=4)
+    ap.add_argument("--cv-train-points", type=int, default=262144, help="boundary points per value function")
+    ap.add_argument("--cv-train-paths", type=int, default=4, help="walks averaged per training point")
+    ap.add_argument("--cv-train-steps", type=int, default=1500)
     ap.add_argument("--profile-copy-mib", type=int, default=0,
                     help="after the timed steps, copy this many MiB once (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -133,6 +187,10 @@ def main():
         d = d - d.mean()
     rmse = float(np.sqrt((d ** 2).mean()))
 
+    ttr = None
+    if not args.no_time_to_rmse:
+        ttr = time_to_rmse_leg(args, torch, wob, G)
+
     cores = min(16, os.cpu_count() or 1)
     cpu = None
     if not args.no_cpu_baseline:
@@ -155,6 +213,7 @@ def main():
                      "fp32_valu": {"achieved": tflops, "peak": MI355X_PEAKS["fp32_TFLOPs"], "unit": "TFLOP/s",
                                    "frac": tflops / MI355X_PEAKS["fp32_TFLOPs"]}},
         "cpu_baseline": cpu,
+        "time_to_rmse": ttr,
         "accuracy": {"rmse_u": rmse, "paths_per_point_total": step * paths,
                      "note": "Neumann problem: u compared after removing the mean offset"
                      if w.solver["problem"] == "NeumannProblem" else "u compared directly"},

@@ -98,10 +98,11 @@ def sample_boundary(scene, n, seed=0, stream_id=0x40000000, stream=None):
     return rec
 
 
-def training_data(estimator, n_per_value, seed=0, stream=None):
+def training_data(estimator, n_per_value, seed=0, paths_per_sample=1, stream=None):
     """Boundary walks of r = 1..path_length bounces from uniform boundary points.
 
-    Returns (positions, output index r - 1, walk value) as device tensors, shuffled."""
+    Each sample's target is the mean of `paths_per_sample` walks from its point.
+    Returns (positions, output index r - 1, target) as device tensors, shuffled."""
     torch = _torch()
     from . import Estimator
     L = estimator.path_length
@@ -111,10 +112,11 @@ def training_data(estimator, n_per_value, seed=0, stream=None):
         e = Estimator(**{**estimator.__dict__, "path_length": r,
                          "flags": (estimator.flags | _capi.EST_BOUNDARY_DENSITY) & ~_capi.EST_CONTROL_VARIATE})
         e.last_launch = None
-        s, q = e.compute_boundary(rec, 1, seed=seed + 104729 * r, stream_id=0x20000000 + r, stream=stream)
+        s, q = e.compute_boundary(rec, paths_per_sample, seed=seed + 104729 * r, stream_id=0x20000000 + r,
+                                  stream=stream)
         pos.append(rec[:, :estimator.scene.dim])
         idx.append(torch.full((n_per_value,), r - 1, dtype=torch.int32, device=rec.device))
-        val.append(s.float())
+        val.append((s / paths_per_sample).float())
     pos, idx, val = torch.cat(pos), torch.cat(idx), torch.cat(val)
     g = torch.Generator(device="cpu").manual_seed(seed)
     perm = torch.randperm(pos.shape[0], generator=g).to(pos.device)

@@ -1,0 +1,63 @@
+"""Progressive estimation with RMSE tracking (the reference apps' main loop).
+
+The reference apps run passes of num_sample_paths per point until max_time, normalise, and
+log "samples, seconds, rmse_u, rmse_grad..." to rmse_list.csv (apps/general_2d.cu:318-367,
+237, 358-359).  Here every pass adds fresh Philox paths (path_base advances) into FP64
+accumulators; the RMSE of u against an exact solution is computed on the GPU at geometric
+checkpoints, and the seconds are the wall time of the estimation (walk launches included).
+"""
+from __future__ import annotations
+
+import math
+import time
+
+
+def run(estimator, points, exact_u, paths_per_pass, time_budget_s, t0=None, seed=0, neumann_shift=False,
+        csv_path=None, max_passes=1_000_000):
+    """Returns a list of (seconds since t0, paths per point, rmse_u) checkpoints.
+
+    points: device tensor (n, dim); exact_u: device FP64 tensor (n,) of exact values.
+    t0: time origin (perf_counter); setup work done before the call (training) counts.
+    """
+    import torch
+    n = points.shape[0]
+    dim = points.shape[1]
+    s = torch.zeros((n, dim + 1), dtype=torch.float64, device=points.device)
+    q = torch.zeros_like(s)
+    if t0 is None:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+    curve = []
+    passes = 0
+    next_check = 1
+    while passes < max_passes:
+        estimator.compute_domain(points, paths_per_pass, seed=seed, path_base=passes * paths_per_pass, out=(s, q))
+        passes += 1
+        if passes >= next_check:
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            m = s[:, 0] / (passes * paths_per_pass)
+            err = m - exact_u
+            if neumann_shift:
+                err = err - err.mean()
+            rmse = float(torch.sqrt((err * err).mean()))
+            curve.append((t, passes * paths_per_pass, rmse))
+            if t >= time_budget_s:
+                break
+            next_check = max(passes + 1, int(math.ceil(passes * 1.5)))
+    if csv_path:
+        with open(csv_path, "w") as f:
+            for t, k, r in curve:
+                f.write(f"{k},{t},{r}\n")
+    return curve
+
+
+def time_to_rmse(curve, target, t_setup=0.0):
+    """Seconds to reach `target`: measured if reached, else extrapolated with rmse^2 (t - t_setup) = const
+    fitted on the last checkpoint (variance-limited estimator, same expectation for both arms)."""
+    for t, k, r in curve:
+        if r <= target:
+            return t, True
+    t, k, r = curve[-1]
+    eff = r * r * max(t - t_setup, 1e-9)
+    return t_setup + eff / (target * target), False
