@@ -29,29 +29,32 @@ METRIC = "WoB paths/s and time-to-RMSE 1e-3 (learned CV vs plain WoB), 1 MI355X"
 
 
 def cpu_baseline(workload, cores, target_s=10.0):
-    """Reference-mode OpenMP oracle on a deterministic subset (every 64th point, full paths)."""
+    """Reference-mode OpenMP oracle on a deterministic subset (every k-th point, full paths).
+
+    k is chosen from a short calibration run so that the timed sample takes about target_s."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     sc = oracle_lib.OracleScene(workload.elements)
     paths = workload.solver["num_sample_paths"]
-    idx = np.arange(0, len(workload.points), 64, dtype=np.uint32)
-    # calibrate on a slice, then size the run to about target_s seconds
-    cal = idx[: max(cores * 2, 32)]
+    n_all = len(workload.points)
+    cal = np.arange(0, n_all, max(1, n_all // max(cores * 4, 64)), dtype=np.uint32)[: max(cores * 4, 64)]
     t0 = time.perf_counter()
     sc.estimate_domain(workload.solver, workload.points[cal], paths, seed=0, threads=cores, pick_mode=0, rng_mode=1,
                        point_index=cal)
     dt = time.perf_counter() - t0
     rate = len(cal) * paths / dt
-    n = int(min(len(idx), max(cores, target_s * rate / paths)))
-    sel = idx[:n]
+    want = max(cores, int(target_s * rate / paths))
+    stride = max(1, n_all // want)
+    sel = np.arange(0, n_all, stride, dtype=np.uint32)[:want]
+    n = len(sel)
     t0 = time.perf_counter()
     sc.estimate_domain(workload.solver, workload.points[sel], paths, seed=0, threads=cores, pick_mode=0, rng_mode=1,
                        point_index=sel)
     dt = time.perf_counter() - t0
     return {"value": n * paths / dt, "unit": "paths/s", "cores": cores, "kind": "port",
-            "sample": f"{n} points (every 64th evaluation point) x {paths} paths = {n * paths} paths in {dt:.1f} s; "
-                      f"reference hit pick (64-entry buffer), per-point mt19937 seeded as runtime_utils.cuh:89-103, "
-                      f"OpenMP static schedule over points"}
+            "sample": f"{n} points (every {stride}th evaluation point) x {paths} paths = {n * paths} paths in "
+                      f"{dt:.1f} s; reference hit pick (64-entry buffer), per-point mt19937 seeded as "
+                      f"runtime_utils.cuh:89-103, OpenMP static schedule over points"}
 
 
 def time_to_rmse_leg(args, torch, wob, G):
@@ -101,6 +104,47 @@ def time_to_rmse_leg(args, torch, wob, G):
             "cv_curve": [[round(t, 3), k, r] for t, k, r in curve_cv]}
 
 
+def batch_leg(args, torch, wob, G):
+    """64 exterior star polygons (Neumann / Robin / mixed, single layer, RIS 64) in one launch per pass.
+
+    The autotuner picks paths per lane, block size and persistent mode from timed passes;
+    the timed passes replay the captured hipGraph.  Reports paths/s and the RMSE of u against
+    the exact log-source solution per problem kind."""
+    print("[bench] batch leg: building 64 star scenes", file=sys.stderr, flush=True)
+    ws = G.star_exterior_batch_2d()
+    dev = torch.device("cuda:0")
+    scenes = [wob.SceneHost(w.elements) for w in ws]
+    pts = [torch.as_tensor(w.points).to(dev) for w in ws]
+    paths = args.batch_paths
+    entries = [(wob.Estimator.from_config(sc, w.solver), p, paths) for sc, w, p in zip(scenes, ws, pts)]
+    b = wob.Batch(entries, persistent=True, use_graph=True)
+    t0 = time.perf_counter()
+    best, best_ms = b.tune(seed=1)
+    tune_s = time.perf_counter() - t0
+    print(f"[bench] batch tuned in {tune_s:.1f} s: {best} ({best_ms:.1f} ms/pass)", file=sys.stderr, flush=True)
+    b.run(1, seed=0)   # warm-up pass (graph capture)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run(args.batch_passes, seed=0)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    n_points = sum(len(w.points) for w in ws)
+    total = n_points * paths * args.batch_passes
+    per_kind = {}
+    for w, (s, q) in zip(ws, b.outputs):
+        k = b.passes * paths
+        m = (s[:, 0] / k).cpu().numpy()
+        err = m - w.exact(w.points.astype(np.float64))[:, 0]
+        per_kind.setdefault(w.solver["problem"], []).append(err)
+    rmse = {k: float(np.sqrt(np.mean(np.concatenate(v) ** 2))) for k, v in per_kind.items()}
+    return {"workload": "star_exterior_batch_2d", "scenes": len(ws), "segments": int(sum(w.elements.n for w in ws)),
+            "points": n_points, "paths_per_point_per_pass": paths, "passes": args.batch_passes,
+            "paths_per_s": total / el, "ms_per_pass": el / args.batch_passes * 1e3, "tuned": best,
+            "tune_s": tune_s, "rmse_u_by_problem": rmse, "paths_per_point_total": b.passes * paths,
+            "solver": "IndirectSingleLayer, ExteriorDomain, BackwardEstimator, RIS 64, eps 1e-5; path_length 7/4/6 "
+                      "and fkesc 1/1/2 for Neumann/Robin/mixed scenes"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +155,9 @@ def main():
     ap.add_argument("--paths-per-lane", type=int, default=0)
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--no-time-to-rmse", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-paths", type=int, default=16, help="paths per point per batch pass")
+    ap.add_argument("--batch-passes", type=int, default=4)
     ap.add_argument("--cv-points", type=int, default=1_000_000)
     ap.add_argument("--cv-budget", type=float, default=15.0, help="walk seconds per arm")
     ap.add_argument("--cv-paths-per-pass", type=int, default=4)
@@ -187,6 +234,10 @@ def main():
         d = d - d.mean()
     rmse = float(np.sqrt((d ** 2).mean()))
 
+    batch = None
+    if not args.no_batch:
+        batch = batch_leg(args, torch, wob, G)
+
     ttr = None
     if not args.no_time_to_rmse:
         ttr = time_to_rmse_leg(args, torch, wob, G)
@@ -214,6 +265,7 @@ def main():
                                    "frac": tflops / MI355X_PEAKS["fp32_TFLOPs"]}},
         "cpu_baseline": cpu,
         "time_to_rmse": ttr,
+        "batch": batch,
         "accuracy": {"rmse_u": rmse, "paths_per_point_total": step * paths,
                      "note": "Neumann problem: u compared after removing the mean offset"
                      if w.solver["problem"] == "NeumannProblem" else "u compared directly"},

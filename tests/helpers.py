@@ -68,7 +68,7 @@ def small_icosphere(level=3, kind="neumann"):
 def star_scene(seed=0, n_segments=400, kind=1):
     """One exterior star polygon scene (kind 0 Neumann, 1 Robin, 2 mixed) with few points."""
     ws = G.star_exterior_batch_2d(n_scenes=1, seed=seed, min_segments=n_segments, max_segments=n_segments,
-                                  min_points=64, max_points=64)
+                                  min_points=64, max_points=64, kinds=[kind])
     return ws[0]
 
 

@@ -134,14 +134,17 @@ def test_intersections_device_vs_brute_force(dim, gpu, oracle):
     assert mism <= max(2, n // 500)
 
 
-def _compare_walk(gpu, oracle, el, solver, pts, paths, seed=3, point_index=None, rel=1e-4):
+def _compare_walk(gpu, oracle, el, solver, pts, paths, seed=3, point_index=None, rel=1e-4, reference_rng=False):
     import wobtoolbox_amd as w
     scene = w.SceneHost(el)
     est = w.Estimator.from_config(scene, solver)
     s, q = est.compute_domain(_dev(pts, gpu), paths, seed=seed)
     s, q = s.cpu().numpy(), q.cpu().numpy()
     sc = oracle.OracleScene(el)
-    so, qo = sc.estimate_domain(solver, pts, paths, seed=seed, pick_mode=1)
+    if reference_rng:   # the reference's own sampling: 64-entry hit buffer, per-point mt19937
+        so, qo = sc.estimate_domain(solver, pts, paths, seed=seed, pick_mode=0, rng_mode=1)
+    else:
+        so, qo = sc.estimate_domain(solver, pts, paths, seed=seed, pick_mode=1)
     m, se = mean_se(s, q, paths)
     mo, seo = mean_se(so, qo, paths)
     diff = np.abs(m - mo)
@@ -188,6 +191,38 @@ def test_walk_matched_single_layer_exterior_ris(gpu, oracle):
     ws = star_scene(seed=4, n_segments=300, kind=1)
     m, se, mo, seo, frac, z = _compare_walk(gpu, oracle, ws.elements, ws.solver, ws.points[:64], 128)
     assert frac < 0.05 and z.max() < 0.5
+
+
+def _aggregate_z(m, se, mo, seo):
+    """z of the mean difference over all points (robust to a few heavy-tailed points)."""
+    d = (m - mo).mean(0)
+    return np.abs(d) / np.sqrt(((se ** 2 + seo ** 2).sum(0)) / len(m) ** 2 + 1e-30)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_walk_reference_rng_star_exterior(kind, gpu, oracle):
+    """Single layer, exterior, RIS 64 (Neumann / Robin / mixed star scenes) against the oracle
+    run with the reference's own sampling (hit buffer order, per-point mt19937 streams)."""
+    ws = star_scene(seed=10 + kind, n_segments=500, kind=kind)
+    m, se, mo, seo, frac, z = _compare_walk(gpu, oracle, ws.elements, ws.solver, ws.points[:64], 512,
+                                            reference_rng=True)
+    za = _aggregate_z(m, se, mo, seo)
+    _say(f"star kind {kind}: aggregate |z| {za}")
+    assert z.max() < 5.0 and za.max() < 4.0
+
+
+def test_walk_reference_rng_geodesic_mixed(gpu, oracle):
+    """Single layer, interior mixed Dirichlet / Neumann on a small geodesic sphere (the
+    learned-control-variate workload's solver), reference sampling on the oracle side."""
+    el, V, F = small_icosphere(level=3, kind="mixed")
+    solver = dict(bie_model="IndirectSingleLayer", domain="InteriorDomain", estimator="BackwardEstimator",
+                  problem="MixedProblem", path_length=4, num_resampling_candidates=64, eps=1e-5,
+                  first_kind_equation_scaling_constant=2.0)
+    pts = G.interior_points_star(48, np.random.default_rng(4)).astype(np.float32)
+    m, se, mo, seo, frac, z = _compare_walk(gpu, oracle, el, solver, pts, 256, reference_rng=True)
+    za = _aggregate_z(m, se, mo, seo)
+    _say(f"geodesic mixed: aggregate |z| {za}")
+    assert z.max() < 5.0 and za.max() < 4.0
 
 
 def test_walk_matched_forward_single_layer_neumann(gpu, oracle):

@@ -68,7 +68,24 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError("libwob_hip link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    _build_shim_check()
     return LIB
+
+
+SHIM_CHECK = os.path.join(LIBDIR, "wob_shim_check")
+
+
+def _build_shim_check():
+    """The C++ host-interface check program (tests/cpp/shim_check.cpp), linked to the library."""
+    src = os.path.join(os.path.dirname(PKG), "tests", "cpp", "shim_check.cpp")
+    if not os.path.exists(src):
+        return
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"), src,
+           "-o", SHIM_CHECK, "-L" + LIBDIR, "-lwob_hip", "-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("shim_check build failed:\n" + r.stdout + r.stderr)
 
 
 if __name__ == "__main__":

@@ -341,12 +341,13 @@ def geodesic_mixed_3d(n_points: int = 1_000_000, freq: int = 100, seed: int = 30
 # ----------------------------------------------------------------------------------------
 def star_exterior_batch_2d(n_scenes: int = 64, seed: int = 2001, min_segments: int = 256,
                            max_segments: int = 16384, min_points: int = 1000, max_points: int = 64000,
-                           point_scale: float = 1.0):
+                           point_scale: float = 1.0, kinds=None):
     """64 star polygons r = 1 + sum_k a_k cos(k th + phi_k); exterior data from log sources.
 
     Returns a list of Workloads (one per scene).  Each scene's problem type is drawn from
     {Neumann, Robin, Mixed}; mixed scenes draw a type per segment (general_2d.cu:242-253
-    draws them from std::random_device; here they are seeded).
+    draws them from std::random_device; here they are seeded).  kinds (optional, one per
+    scene: 0 Neumann, 1 Robin, 2 mixed) overrides the drawn kind.
     """
     rng = np.random.default_rng(seed)
     scenes = []
@@ -366,6 +367,8 @@ def star_exterior_batch_2d(n_scenes: int = 64, seed: int = 2001, min_segments: i
         q -= q.mean()
         exact = make_log_sources_2d(np.array(src), q)
         kind = int(rng.integers(0, 3))  # 0 Neumann, 1 Robin, 2 Mixed
+        if kinds is not None:
+            kind = int(kinds[s])
         if kind == 0:
             types = np.full(ns, NEUMANN, np.int32)
         elif kind == 1:
