@@ -219,17 +219,24 @@ def test_direct_neumann_unbiased(oracle):
     assert np.abs((m[:, 1:] - ex[:, 1:]) / se[:, 1:]).max() < 5.0
 
 
-def test_single_layer_exterior_ris(oracle):
-    w = star_scene(seed=4, n_segments=300, kind=1)
+@pytest.mark.parametrize("kind,path_length", [(0, None), (2, None), (1, 8)])
+def test_single_layer_exterior_ris(kind, path_length, oracle):
+    """Single layer, exterior, RIS 64 against the exact log-source solution.
+
+    Neumann (path length 7) and mixed (6) scenes run at their configured path length.  The
+    Robin configuration's path length 4 leaves a truncation bias of up to ~13 SE at 4096
+    paths at some points (the single-layer Robin series converges slowly), so Robin is
+    checked at path length 8, where the truncation bias is below the noise."""
+    w = star_scene(seed=4, n_segments=300, kind=kind)
+    solver = dict(w.solver) if path_length is None else dict(w.solver, path_length=path_length)
     sc = oracle.OracleScene(w.elements)
     pts = w.points[:24]
     n = 4096
-    s, q = sc.estimate_domain(w.solver, pts, n, seed=4)
+    s, q = sc.estimate_domain(solver, pts, n, seed=4)
     m, se = mean_se(s, q, n)
     ex = w.exact(pts)
     z = (m - ex) / se
-    # truncated series (path_length 4-7) is slightly biased; the bias stays within a few SE at this n
-    assert np.abs(z[:, 0]).max() < 5.5, np.abs(z).max()
+    assert np.abs(z[:, 0]).max() < 5.0, np.abs(z).max()
 
 
 def test_forward_single_layer_neumann(oracle):
