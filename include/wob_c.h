@@ -86,9 +86,24 @@ typedef struct wob_scene_info {
 wob_status wob_scene_get_info(const wob_scene *scene, wob_scene_info *info);
 
 /* Replaces SceneHost::set_volume_cache(grid_res, domain_size, cache) (scene.cuh:301-306,
- * 206-240): a grid source over (-domain_size/2, domain_size/2)^dim sampled proportionally
- * to |cache|.  cache: grid_res^dim host floats.                                            */
+ * 206-240): a grid source over (-domain_size/2, domain_size/2)^dim; estimators with
+ * num_volume_samples > 0 sample a cell with probability |cache| / sum |cache| and a uniform
+ * point in it (scene.cuh:100-120).  As in the reference there is no inside test: cells
+ * outside the domain must hold 0.  cache: grid_res^dim host floats, x fastest
+ * (utils.cuh:92-99).  grid_res == 0 or cache == NULL removes the source.  Replaces any
+ * earlier source of the scene; waits for the device to be idle.                             */
 wob_status wob_scene_set_volume_grid(wob_scene *scene, uint32_t grid_res, float domain_size, const float *cache);
+
+/* Gaussian-mixture source f(x) = sum_j A_j exp(-|x - c_j|^2 / (2 w_j^2)) (no counterpart in
+ * the reference, whose only source is the grid cache).  Volume samples come from the mixture
+ * with weights |A_j| w_j^dim (inv_pdf = 1 / mixture density) and, unless
+ * WOB_VOLUME_NO_MASK is given, a sample on the far side of the boundary (outside an interior
+ * domain, inside the boundary for an exterior one; parity of the crossings of a fixed ray)
+ * contributes 0, so the source is restricted to the domain.  centers: n*dim floats, widths
+ * and amplitudes: n floats (host).  n == 0 removes the source.                              */
+enum { WOB_VOLUME_NO_MASK = 1 };
+wob_status wob_scene_set_volume_gaussians(wob_scene *scene, uint32_t n, const float *centers, const float *widths,
+                                          const float *amplitudes, uint32_t flags);
 
 /* Estimator configuration: the reference's Estimator template and constructor arguments
  * (estimator.cuh:118-152) plus the JSON keys of the apps (general_2d.cu:174-195).           */
@@ -100,7 +115,7 @@ typedef struct wob_estimator_desc {
     int32_t problem;              /* WOB_*_PROBLEM */
     uint32_t path_length;
     uint32_t num_resampling_candidates;   /* 1 = plain uniform sampling */
-    uint32_t num_volume_samples;
+    uint32_t num_volume_samples;          /* > 0: add the scene's volume source term (Poisson) */
     float eps;                            /* kernel clamp, default FLT_EPSILON */
     float first_kind_equation_scaling_constant;  /* default 1 */
     uint32_t flags;                       /* WOB_EST_* options, 0 = the reference's behaviour */
@@ -124,6 +139,10 @@ enum { WOB_EST_EXCLUDE_ORIGIN_ELEMENT = 1 };
  *   estimator.cuh:232-357) instead of the solution; this generates control-variate training
  *   data.                                                                                    */
 enum { WOB_EST_CONTROL_VARIATE = 2, WOB_EST_BOUNDARY_DENSITY = 4 };
+/* WOB_EST_INDEPENDENT_VOLUME_SAMPLES: each of the num_volume_samples addends of a volume term
+ *   uses its own sample.  The reference draws one sample per term and adds it
+ *   num_volume_samples times (bie_models.cuh:85-101), which leaves the variance unchanged.   */
+enum { WOB_EST_INDEPENDENT_VOLUME_SAMPLES = 8 };
 
 /* Launch tuning and timing (in/out; may be NULL).  paths_per_lane: paths one lane
  * accumulates before it writes its partial sums; block_size: threads per workgroup

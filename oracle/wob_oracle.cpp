@@ -269,6 +269,16 @@ struct VolumeGrid {
     std::vector<float> cache, cdf;
 };
 
+// Source f(x) = sum_j A_j exp(-|x - c_j|^2 / (2 w_j^2)), sampled from the mixture with weights
+// proportional to |A_j| w_j^D (no counterpart in the reference, whose only source is the grid).
+// With `mask`, samples on the far side of the boundary (outside the domain) contribute 0.
+struct VolumeGauss {
+    bool present = false;
+    bool mask = true;
+    unsigned n = 0;
+    std::vector<float> c, amp, inv2w2, pn, cdf, w;   // c: n * 3; pn: mixture density coefficient
+};
+
 template <int D> struct Scene {
     std::vector<Element<D>> elems;
     std::vector<float> area_cdf, bv_cdf;
@@ -277,6 +287,7 @@ template <int D> struct Scene {
     struct Node { int left, right, obj; AABB<D> box; };
     std::vector<Node> nodes;
     VolumeGrid vol;
+    VolumeGauss gauss;
 };
 
 // utils.cuh:148-166
@@ -676,25 +687,91 @@ static inline void idx_to_domain_point(unsigned idx, unsigned grid_res, float do
     for (int k = 0; k < D; ++k) out[k] = ((float)u[k] + 0.5f) * dx - (domain_size / 2);
 }
 
-template <int D> static VolSample sample_volume(const Scene<D> &s, Rng &rng) {
-    // scene.cuh:100-120
-    VolSample r{false, {0, 0, 0}, 0, 0};
-    if (!s.vol.present) return r;
-    const auto &v = s.vol;
-    unsigned cells = 1;
-    for (int k = 0; k < D; ++k) cells *= v.grid_res;
-    const float unif = rng.uniform();
-    const unsigned idx = binary_search(v.cdf.data(), cells, unif);
-    idx_to_domain_point(idx, v.grid_res, v.domain_size, D, r.p);
-    for (int k = 0; k < D; ++k) {
-        const float u = rng.uniform();
-        r.p[k] += (u - 0.5f) * v.domain_size / v.grid_res;
+// Fixed direction of the inside/outside ray (parity of the crossings with t > 0).
+static const float kRay2[2] = {0.58778525f, 0.80901699f};
+static const float kRay3[3] = {0.48218104f, 0.66748405f, 0.56749094f};
+
+template <int D> static unsigned ray_crossings(const Scene<D> &s, const Vec<D> &o) {
+    Vec<D> d;
+    for (int k = 0; k < D; ++k) d[k] = D == 2 ? kRay2[k] : kRay3[k];
+    unsigned count = 0;
+    auto test = [&](int obj) {
+        Vec<D> p; float t;
+        if (intersect(o, d, s.elems[obj], p, t) && t > 0.f) ++count;
+    };
+    const int n = (int)s.elems.size();
+    if (n == 1) { test(0); return count; }
+    int stack[64];
+    int sp = 0;
+    stack[sp++] = 0;
+    const int leaf0 = n - 1;
+    while (sp > 0) {
+        const auto &N = s.nodes[stack[--sp]];
+        const int ch[2] = {N.left, N.right};
+        for (int c = 0; c < 2; ++c) {
+            const int u = ch[c];
+            if (!line_hits_box(s.nodes[u].box, o, d)) continue;
+            if (u >= leaf0) test(s.nodes[u].obj);
+            else if (sp < 64) stack[sp++] = u;
+        }
     }
-    r.value = v.cache[idx];
-    const float pdf = idx == 0 ? v.cdf[idx] : v.cdf[idx] - v.cdf[idx - 1];
-    float cellv = 1.f;
-    for (int k = 0; k < D; ++k) cellv *= v.domain_size / v.grid_res;
-    r.inv_pdf = cellv / pdf;
+    return count;
+}
+
+template <int D> static VolSample sample_volume(const Scene<D> &s, Rng &rng, bool interior) {
+    VolSample r{false, {0, 0, 0}, 0, 0};
+    if (s.vol.present) {
+        // scene.cuh:100-120 (sample_volme_cache_cache_value_sampling)
+        const auto &v = s.vol;
+        unsigned cells = 1;
+        for (int k = 0; k < D; ++k) cells *= v.grid_res;
+        const float unif = rng.uniform();
+        const unsigned idx = binary_search(v.cdf.data(), cells, unif);
+        idx_to_domain_point(idx, v.grid_res, v.domain_size, D, r.p);
+        for (int k = 0; k < D; ++k) {
+            const float u = rng.uniform();
+            r.p[k] += (u - 0.5f) * v.domain_size / v.grid_res;
+        }
+        r.value = v.cache[idx];
+        const float pdf = idx == 0 ? v.cdf[idx] : v.cdf[idx] - v.cdf[idx - 1];
+        float cellv = 1.f;
+        for (int k = 0; k < D; ++k) cellv *= v.domain_size / v.grid_res;
+        r.inv_pdf = cellv / pdf;
+        r.valid = true;
+        return r;
+    }
+    if (!s.gauss.present) return r;
+    const auto &g = s.gauss;
+    const float u = rng.uniform();
+    unsigned j = 0;
+    while (j + 1 < g.n && u > g.cdf[j]) ++j;
+    // Box-Muller normals from (0, 1] uniforms
+    const float u1 = rng.uniform(), u2 = rng.uniform();
+    const float r1 = std::sqrt(-2.f * std::log(u1));
+    float z[3];
+    z[0] = r1 * (float)std::cos(2.0 * M_PI * (double)u2);
+    z[1] = r1 * (float)std::sin(2.0 * M_PI * (double)u2);
+    if (D == 3) {
+        const float u3 = rng.uniform(), u4 = rng.uniform();
+        z[2] = std::sqrt(-2.f * std::log(u3)) * (float)std::cos(2.0 * M_PI * (double)u4);
+    }
+    for (int k = 0; k < D; ++k) r.p[k] = g.c[3 * j + k] + g.w[j] * z[k];
+    float f = 0.f, pdf = 0.f;
+    for (unsigned i = 0; i < g.n; ++i) {
+        float d2 = 0.f;
+        for (int k = 0; k < D; ++k) { const float dk = r.p[k] - g.c[3 * i + k]; d2 += dk * dk; }
+        const float e = std::exp(-d2 * g.inv2w2[i]);
+        f += g.amp[i] * e;
+        pdf += g.pn[i] * e;
+    }
+    r.value = f;
+    r.inv_pdf = 1.0f / pdf;
+    if (g.mask) {
+        Vec<D> o;
+        for (int k = 0; k < D; ++k) o[k] = r.p[k];
+        const bool inside = (ray_crossings(s, o) & 1u) != 0;
+        if (inside != interior) r.value = 0.f;
+    }
     r.valid = true;
     return r;
 }
@@ -709,6 +786,7 @@ template <int D> struct Model {
     Kernel<D> k;
     float c;            // first_kind_equation_scaling_constant
     unsigned nvs;       // num_volume_samples
+    bool indep = false; // draw num_volume_samples independent samples (the reference reuses one)
     const Scene<D> *scene;
 
     float ift(float ia) const { return interior ? ia : 1.f - ia; }   // :73-78
@@ -718,10 +796,11 @@ template <int D> struct Model {
     // :79-103 (one sample reused num_volume_samples times)
     float vol_term(const BPoint<D> &bp, Rng &rng) const {
         if (nvs == 0) return 0.f;
-        VolSample v = sample_volume(*scene, rng);
+        VolSample v = sample_volume(*scene, rng, interior);
         float res = 0.f;
-        const Vec<D> q = vp(v);
+        Vec<D> q = vp(v);
         for (unsigned i = 0; i < nvs; ++i) {
+            if (indep && i > 0) { v = sample_volume(*scene, rng, interior); q = vp(v); }
             if (bp.boundary_type == Dirichlet) res += k.G(bp.p, q) * v.value * v.inv_pdf;
             else if (bp.boundary_type == Neumann) res += k.dG_dny(bp.p, q, bp.n) * v.value * v.inv_pdf;
             else res += (k.dG_dny(bp.p, q, bp.n) + bp.robin_alpha * k.G(q, bp.p)) * v.value * v.inv_pdf;
@@ -731,10 +810,11 @@ template <int D> struct Model {
     // :105-119 and :121-135
     float vol_init(const Vec<D> &x, bool boundary_form, Rng &rng) const {
         if (nvs == 0) return 0.f;
-        VolSample v = sample_volume(*scene, rng);
+        VolSample v = sample_volume(*scene, rng, interior);
         float res = 0.f;
-        const Vec<D> q = vp(v);
+        Vec<D> q = vp(v);
         for (unsigned i = 0; i < nvs; ++i) {
+            if (indep && i > 0) { v = sample_volume(*scene, rng, interior); q = vp(v); }
             if (boundary_form) res -= k.G(x, q) * v.value * v.inv_pdf;
             else res -= k.G(q, x) * v.value * v.inv_pdf;
         }
@@ -744,9 +824,12 @@ template <int D> struct Model {
     Vec<D> vol_init_derivative(const Vec<D> &x, Rng &rng) const {
         Vec<D> res = Vec<D>::zero();
         if (nvs == 0) return res;
-        VolSample v = sample_volume(*scene, rng);
-        const Vec<D> q = vp(v);
-        for (unsigned i = 0; i < nvs; ++i) res = res - (v.value * v.inv_pdf) * k.dG_dx(q, x);
+        VolSample v = sample_volume(*scene, rng, interior);
+        Vec<D> q = vp(v);
+        for (unsigned i = 0; i < nvs; ++i) {
+            if (indep && i > 0) { v = sample_volume(*scene, rng, interior); q = vp(v); }
+            res = res - (v.value * v.inv_pdf) * k.dG_dx(q, x);
+        }
         return res / (float)nvs;
     }
 
@@ -1027,6 +1110,7 @@ struct orc_est_desc {
     uint32_t path_length, num_resampling_candidates, num_volume_samples;
     float eps, first_kind_equation_scaling_constant;
     int32_t pick_mode, rng_mode, use_bvh, exclude_origin;
+    int32_t independent_volume;
 };
 
 extern "C" {
@@ -1146,6 +1230,45 @@ int orc_scene_set_volume_grid(void *p, unsigned grid_res, float domain_size, con
     return 0;
 }
 
+// Gaussian-mixture volume source (see VolumeGauss); n = 0 removes it.  The host-side
+// constants are computed in double and rounded once.
+int orc_scene_set_volume_gaussians(void *p, unsigned n, const float *centers, const float *widths, const float *amps,
+                                   int mask) {
+    auto *S = (OrcScene *)p;
+    VolumeGauss &g = S->dim == 2 ? S->s2.gauss : S->s3.gauss;
+    const int D = S->dim;
+    g = VolumeGauss();
+    if (n == 0) return 0;
+    g.present = true; g.mask = mask != 0; g.n = n;
+    g.c.assign((size_t)n * 3, 0.f);
+    std::vector<double> wt(n);
+    double tot = 0;
+    for (unsigned j = 0; j < n; ++j) {
+        for (int k = 0; k < D; ++k) g.c[3 * j + k] = centers[(size_t)j * D + k];
+        wt[j] = std::abs((double)amps[j]) * std::pow((double)widths[j], D);
+        tot += wt[j];
+    }
+    double acc = 0;
+    for (unsigned j = 0; j < n; ++j) {
+        const double w = widths[j], pj = wt[j] / tot;
+        acc += pj;
+        g.w.push_back(widths[j]);
+        g.amp.push_back(amps[j]);
+        g.inv2w2.push_back((float)(1.0 / (2.0 * w * w)));
+        g.pn.push_back((float)(pj * std::pow(2.0 * M_PI * w * w, -0.5 * D)));
+        g.cdf.push_back(j + 1 == n ? 1.0f : (float)acc);
+    }
+    return 0;
+}
+
+// Inside/outside parity of a point (ray crossings with t > 0 along the fixed direction).
+int orc_inside(void *p, const float *pt) {
+    auto *S = (OrcScene *)p;
+    if (S->dim == 2) { Vec<2> o{{pt[0], pt[1]}}; return (int)(ray_crossings(S->s2, o) & 1u); }
+    Vec<3> o{{pt[0], pt[1], pt[2]}};
+    return (int)(ray_crossings(S->s3, o) & 1u);
+}
+
 // All-hit line query; returns the hit count, ids / t of the first `cap` hits (unsorted).
 int orc_intersect_line(void *p, const float *origin, const float *dir, int use_bvh, int cap, int32_t *ids, float *ts) {
     auto *S = (OrcScene *)p;
@@ -1175,6 +1298,7 @@ template <int D> static Estimator<D> make_est(const Scene<D> &s, const orc_est_d
     e.m.k.eps = d->eps;
     e.m.c = d->first_kind_equation_scaling_constant;
     e.m.nvs = d->num_volume_samples;
+    e.m.indep = d->independent_volume != 0;
     e.m.scene = &s;
     e.est = d->estimator;
     e.problem = d->problem;

@@ -27,7 +27,8 @@ class EstDesc(C.Structure):
                 ("path_length", C.c_uint32), ("num_resampling_candidates", C.c_uint32),
                 ("num_volume_samples", C.c_uint32), ("eps", C.c_float),
                 ("first_kind_equation_scaling_constant", C.c_float), ("pick_mode", C.c_int32),
-                ("rng_mode", C.c_int32), ("use_bvh", C.c_int32), ("exclude_origin", C.c_int32)]
+                ("rng_mode", C.c_int32), ("use_bvh", C.c_int32), ("exclude_origin", C.c_int32),
+                ("independent_volume", C.c_int32)]
 
 
 def build():
@@ -50,6 +51,9 @@ def lib():
         L.orc_scene_totals.argtypes = [P, P]
         L.orc_scene_cdfs.argtypes = [P, P, P]
         L.orc_scene_set_volume_grid.argtypes = [P, C.c_uint, C.c_float, P]
+        L.orc_scene_set_volume_gaussians.argtypes = [P, C.c_uint, P, P, P, C.c_int]
+        L.orc_inside.argtypes = [P, P]
+        L.orc_inside.restype = C.c_int
         L.orc_intersect_line.argtypes = [P, P, P, C.c_int, C.c_int, P, P]
         L.orc_intersect_line.restype = C.c_int
         L.orc_estimate_domain.argtypes = [P, C.POINTER(EstDesc), P, P, C.c_int, C.c_uint, C.c_uint64, C.c_uint64,
@@ -73,11 +77,16 @@ def _p(a):
 
 
 def est_desc(solver: dict, pick_mode=1, rng_mode=0, use_bvh=1, num_volume_samples=0) -> EstDesc:
-    ex = int(solver.get("flags", 0)) & 1
+    """Solver dict (reference JSON keys + the device estimator flags) -> oracle descriptor.
+    flags bit 0: exclude the origin element of boundary line samples; bit 3: independent
+    volume samples."""
+    fl = int(solver.get("flags", 0))
+    nvs = num_volume_samples or int(solver.get("num_volume_samples", 0))
     return EstDesc(ENUMS[solver["bie_model"]], ENUMS[solver["domain"]], ENUMS[solver["estimator"]],
                    ENUMS[solver["problem"]], solver["path_length"], solver.get("num_resampling_candidates", 1),
-                   num_volume_samples, solver.get("eps", float(np.finfo(np.float32).eps)),
-                   solver.get("first_kind_equation_scaling_constant", 1.0), pick_mode, rng_mode, use_bvh, ex)
+                   nvs, solver.get("eps", float(np.finfo(np.float32).eps)),
+                   solver.get("first_kind_equation_scaling_constant", 1.0), pick_mode, rng_mode, use_bvh, fl & 1,
+                   (fl >> 3) & 1)
 
 
 class OracleScene:
@@ -107,6 +116,16 @@ class OracleScene:
     def set_volume_grid(self, grid_res, domain_size, cache):
         cache = np.ascontiguousarray(cache, np.float32)
         lib().orc_scene_set_volume_grid(self.h, grid_res, domain_size, _p(cache))
+
+    def set_volume_gaussians(self, centers, widths, amplitudes, mask=True):
+        c = np.ascontiguousarray(centers, np.float32)
+        w = np.ascontiguousarray(widths, np.float32)
+        a = np.ascontiguousarray(amplitudes, np.float32)
+        lib().orc_scene_set_volume_gaussians(self.h, len(w), _p(c), _p(w), _p(a), int(mask))
+
+    def inside(self, points):
+        pts = np.ascontiguousarray(points, np.float32)
+        return np.array([lib().orc_inside(self.h, _p(pts[i])) for i in range(len(pts))], bool)
 
     def intersect(self, origin, direction, use_bvh=True, cap=256):
         o = np.ascontiguousarray(origin, np.float32)
@@ -138,7 +157,7 @@ class OracleScene:
         s = np.zeros(n, np.float64)
         q = np.zeros(n, np.float64)
         po = np.zeros((n, paths), np.float32) if return_paths else None
-        d = est_desc(solver, pick_mode, 0, 1)
+        d = est_desc(solver, pick_mode, 0, 1, 0)
         lib().orc_estimate_boundary(self.h, C.byref(d), _p(r), n, paths, seed, path_base, stream, threads, _p(s),
                                     _p(q), _p(po))
         return (s, q, po) if return_paths else (s, q)

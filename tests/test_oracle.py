@@ -268,3 +268,101 @@ def test_boundary_estimator_dirichlet_returns_data(oracle):
                   first_kind_equation_scaling_constant=2.0, eps=1e-5)
     s, q = sc.estimate_boundary(solver, rec, 16, seed=1)
     np.testing.assert_allclose(s / 16, rec[:, 7], rtol=1e-5)
+
+
+# ---------------------------------------------------------------------------------------
+# Poisson volume terms (bie_models.cuh:79-154, scene.cuh:100-120)
+# ---------------------------------------------------------------------------------------
+def test_poisson_grid_source_square(oracle):
+    """The reference app's setting: square, constant source on a grid cache.  Against the
+    closed form u = 5 r^2 / 4 + exp(x) cos(y)."""
+    w = G.square_poisson_2d(n_points=24)
+    sc = oracle.OracleScene(w.elements)
+    sc.set_volume_grid(w.extra["grid_res"], w.extra["domain_size"], w.extra["cache"])
+    n = 8192
+    s, q = sc.estimate_domain(w.solver, w.points, n, seed=1)
+    m, se = mean_se(s, q, n)
+    z = (m - w.exact(w.points)) / se
+    assert np.abs(z).max() < 4.5, np.abs(z).max()
+    # without the volume term the estimate solves Laplace's equation with the same data and
+    # misses u by the source's contribution (up to 5/4 at the centre)
+    s0, q0 = sc.estimate_domain(dict(w.solver, num_volume_samples=0), w.points, n, seed=1)
+    m0, se0 = mean_se(s0, q0, n)
+    assert np.abs((m0[:, 0] - w.exact(w.points)[:, 0]) / se0[:, 0]).max() > 10
+
+
+def test_newtonian_potential_closed_form():
+    """The exact Poisson solution's Gaussian part: finite-difference Laplacian = f and
+    gradient = finite differences of N."""
+    c, wd, a = G.gaussian_sources_3d()
+    fn = G.newtonian_gaussians_3d(c, wd, a)
+    p = np.random.default_rng(0).uniform(-0.6, 0.6, (8, 3))
+    h = 1e-3
+    lap = sum((fn(p + h * e)[:, 0] - 2 * fn(p)[:, 0] + fn(p - h * e)[:, 0]) / h ** 2 for e in np.eye(3))
+    f = (a[None, :] * np.exp(-((p[:, None, :] - c[None]) ** 2).sum(-1) / (2 * wd[None] ** 2))).sum(1)
+    np.testing.assert_allclose(lap, f, rtol=1e-3, atol=1e-5)
+    g = np.stack([(fn(p + 1e-6 * e)[:, 0] - fn(p - 1e-6 * e)[:, 0]) / 2e-6 for e in np.eye(3)], 1)
+    np.testing.assert_allclose(fn(p)[:, 1:], g, rtol=1e-6, atol=1e-8)
+    # a single Gaussian far away looks like a point charge of mass A (2 pi w^2)^(3/2)
+    one = G.newtonian_gaussians_3d(np.zeros((1, 3)), [0.1], [2.0])
+    far = np.array([[5.0, 0.0, 0.0]])
+    np.testing.assert_allclose(one(far)[0, 0], -2.0 * (2 * np.pi * 0.01) ** 1.5 / (4 * np.pi * 5.0), rtol=1e-9)
+
+
+def test_inside_parity_matches_star_shape(oracle):
+    el, V, F = small_icosphere(level=4, kind="neumann")
+    sc = oracle.OracleScene(el)
+    rng = np.random.default_rng(3)
+    p = rng.uniform(-1.3, 1.3, (3000, 3))
+    r = np.linalg.norm(p, axis=1)
+    rs = G.smooth_radius(p / r[:, None])
+    keep = np.abs(r - rs) > 0.03          # away from the surface (the mesh is a chordal approximation)
+    ins = sc.inside(p[keep].astype(np.float32))
+    assert np.array_equal(ins, (r < rs)[keep])
+
+
+def _poisson_mesh_workload(level=3):
+    V, F = G.icosphere(level)
+    V = G.displace_radial(V)
+    return G.poisson_gaussian_3d(point_sets=[8], V=V, F=F)
+
+
+def test_poisson_gaussian_source_mesh(oracle):
+    """Gaussian-mixture source on a displaced icosphere (double layer, path length 4) against
+    u = N_f + H in closed form; points at least 0.1 inside the smooth surface."""
+    w = _poisson_mesh_workload()
+    sc = oracle.OracleScene(w.elements)
+    sc.set_volume_gaussians(w.extra["centers"], w.extra["widths"], w.extra["amplitudes"])
+    pts = G.interior_points_star(24, np.random.default_rng(5), margin=0.1).astype(np.float32)
+    n = 4096
+    s, q = sc.estimate_domain(w.solver, pts, n, seed=2)
+    m, se = mean_se(s, q, n)
+    z = (m - w.exact(pts)) / se
+    assert np.abs(z[:, 0]).max() < 4.5, np.abs(z[:, 0]).max()
+    assert np.abs(z[:, 1:]).max() < 5.5, np.abs(z[:, 1:]).max()
+
+
+def test_independent_volume_samples(oracle):
+    """The reference reuses one volume sample num_volume_samples times (its variance does
+    not fall); independent samples keep the mean and cut the variance.  Data = the
+    Newtonian potential alone (H = 0), so the volume terms carry the variance."""
+    V, F = G.icosphere(3)
+    V = G.displace_radial(V)
+    c, wd, a = G.gaussian_sources_3d()
+    nf = G.newtonian_gaussians_3d(c, wd, a)
+    el = G.mesh_elements(V, F, G.DIRICHLET, lambda p, nr, ty: nf(p)[:, 0])
+    sc = oracle.OracleScene(el)
+    sc.set_volume_gaussians(c, wd, a)
+    pts = G.interior_points_star(16, np.random.default_rng(6), margin=0.1).astype(np.float32)
+    solver = dict(bie_model="IndirectDoubleLayer", domain="InteriorDomain", estimator="BackwardEstimator",
+                  problem="DirichletProblem", path_length=4, eps=1e-4)
+    n = 4096
+    res = {}
+    for nvs, fl in ((1, 0), (8, 0), (8, 8)):
+        s, q = sc.estimate_domain(dict(solver, num_volume_samples=nvs, flags=fl), pts, n, seed=3)
+        res[(nvs, fl)] = mean_se(s, q, n)
+    np.testing.assert_allclose(res[(8, 0)][0][:, 0], res[(1, 0)][0][:, 0], rtol=1e-5, atol=1e-7)   # one sample x 8
+    ratio = res[(1, 0)][1][:, 0] / res[(8, 8)][1][:, 0]
+    assert np.median(ratio) > 2.0, ratio           # about sqrt(8) with 8 independent samples
+    m, se = res[(8, 8)]
+    assert np.abs((m[:, 0] - nf(pts)[:, 0]) / se[:, 0]).max() < 4.5

@@ -97,6 +97,28 @@ class SceneHost:
         check(lib().wob_scene_get_info(self._h, C.byref(i)))
         return {f: getattr(i, f) for f, _ in i._fields_}
 
+    def set_volume_cache(self, grid_res, domain_size, cache):
+        """Grid source over (-domain_size/2, domain_size/2)^dim (SceneHost::set_volume_cache,
+        scene.cuh:301-306); cache: grid_res^dim values, x fastest.  None removes the source."""
+        if cache is None:
+            check(lib().wob_scene_set_volume_grid(self._h, 0, 0.0, None))
+            return
+        c = np.ascontiguousarray(cache, np.float32).reshape(-1)
+        if c.size != grid_res ** self.dim:
+            raise ValueError(f"cache has {c.size} values, expected grid_res^dim = {grid_res ** self.dim}")
+        check(lib().wob_scene_set_volume_grid(self._h, int(grid_res), float(domain_size), c.ctypes.data))
+
+    def set_volume_gaussians(self, centers, widths, amplitudes, mask=True):
+        """Gaussian-mixture source sum_j A_j exp(-|x - c_j|^2 / (2 w_j^2)); with mask, samples
+        outside the domain contribute 0."""
+        c = np.ascontiguousarray(centers, np.float32).reshape(-1, self.dim)
+        w = np.ascontiguousarray(widths, np.float32).reshape(-1)
+        a = np.ascontiguousarray(amplitudes, np.float32).reshape(-1)
+        if not (len(c) == len(w) == len(a)):
+            raise ValueError("centers, widths and amplitudes must have one entry per Gaussian")
+        check(lib().wob_scene_set_volume_gaussians(self._h, len(w), c.ctypes.data, w.ctypes.data, a.ctypes.data,
+                                                   0 if mask else _capi.VOLUME_NO_MASK))
+
     def intersect_lines(self, origins, dirs, max_hits=64, stream=None):
         """All-hit line query: (counts, ids, t) for device tensors of lines."""
         torch = _torch()
@@ -142,7 +164,7 @@ class Estimator:
     @classmethod
     def from_config(cls, scene, cfg: dict, **overrides):
         keys = ["bie_model", "domain", "estimator", "problem", "path_length", "num_resampling_candidates", "eps",
-                "first_kind_equation_scaling_constant", "num_volume_samples", "flags"]
+                "first_kind_equation_scaling_constant", "num_volume_samples", "flags", "cv_samples"]
         kw = {k: cfg[k] for k in keys if k in cfg}
         kw.update(overrides)
         return cls(scene, **kw)

@@ -16,6 +16,7 @@ STATUS_NAMES = {0: "WOB_OK", 1: "WOB_INVALID_ARG", 2: "WOB_INVALID_ENUM", 3: "WO
 
 # every symbol declared in include/wob_c.h
 EXPORTS = ["wob_scene_create", "wob_scene_destroy", "wob_scene_get_info", "wob_scene_set_volume_grid",
+           "wob_scene_set_volume_gaussians",
            "wob_estimate_domain", "wob_estimate_boundary", "wob_intersect_lines", "wob_eval_kernels",
            "wob_eval_weights", "wob_philox4x32_10", "wob_string_to_enum", "wob_enum_to_string",
            "wob_last_error_message", "wob_version", "wob_batch_create", "wob_batch_run", "wob_batch_tune",
@@ -51,6 +52,8 @@ class EstimatorDesc(C.Structure):
 EST_EXCLUDE_ORIGIN_ELEMENT = 1
 EST_CONTROL_VARIATE = 2
 EST_BOUNDARY_DENSITY = 4
+EST_INDEPENDENT_VOLUME_SAMPLES = 8
+VOLUME_NO_MASK = 1
 
 
 class BatchEntry(C.Structure):
@@ -90,6 +93,7 @@ def lib():
         "wob_scene_destroy": ([P], C.c_int),
         "wob_scene_get_info": ([P, C.POINTER(SceneInfo)], C.c_int),
         "wob_scene_set_volume_grid": ([P, U32, F, P], C.c_int),
+        "wob_scene_set_volume_gaussians": ([P, U32, P, P, P, U32], C.c_int),
         "wob_estimate_domain": ([P, C.POINTER(EstimatorDesc), P, U32, U32, U32, U64, U64, U32, P, P,
                                  C.POINTER(LaunchConfig), P], C.c_int),
         "wob_estimate_boundary": ([P, C.POINTER(EstimatorDesc), P, U32, U32, U32, U64, U64, U32, P, P,

@@ -24,6 +24,10 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off"]
 
 SOURCES = [
+    ("wob_walk.hip", "hip", ["-DWOB_WALK_DIM=2", "-DWOB_WALK_VOL=0"], "wob_walk_20.o"),
+    ("wob_walk.hip", "hip", ["-DWOB_WALK_DIM=2", "-DWOB_WALK_VOL=1"], "wob_walk_21.o"),
+    ("wob_walk.hip", "hip", ["-DWOB_WALK_DIM=3", "-DWOB_WALK_VOL=0"], "wob_walk_30.o"),
+    ("wob_walk.hip", "hip", ["-DWOB_WALK_DIM=3", "-DWOB_WALK_VOL=1"], "wob_walk_31.o"),
     ("wob_kernels.hip", "hip"),
     ("mlp_kernels.hip", "hip"),
     ("wob_capi.cpp", "hip"),
@@ -32,10 +36,10 @@ SOURCES = [
 ]
 
 
-def _cmd(src, kind, out):
+def _cmd(src, kind, out, defines=()):
     path = os.path.join(CSRC, src)
     if kind == "hip":
-        return [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", path, "-o", out]
+        return [HIPCC, *HIP_FLAGS, *defines, "-x", "hip", "-c", path, "-o", out]
     if kind == "hip-host":
         return [HIPCC, "-O2", "-std=c++17", "-fPIC", "-Wall", "-D__HIP_PLATFORM_AMD__", "-c", "-x", "c++", path, "-o", out]
     return ["g++", *HOST_FLAGS, "-c", path, "-o", out]
@@ -46,10 +50,13 @@ def build(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     procs = []
     objs = []
-    for src, kind in SOURCES:
-        out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    for entry in SOURCES:
+        src, kind = entry[0], entry[1]
+        defines = entry[2] if len(entry) > 2 else ()
+        obj = entry[3] if len(entry) > 3 else os.path.splitext(src)[0] + ".o"
+        out = os.path.join(OBJ, obj)
         objs.append(out)
-        cmd = _cmd(src, kind, out)
+        cmd = _cmd(src, kind, out, defines)
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

@@ -149,8 +149,13 @@ void build_cdf(const std::vector<float> &w, std::vector<float> &cdf, float &tota
     }
     total = sum;
     for (size_t i = 0; i < n; ++i) cdf[i] /= total;
+    build_guide(cdf, guide);
+}
+
+void build_guide(const std::vector<float> &cdf, std::vector<uint32_t> &guide) {
     // guide[k] = first i with cdf[i] >= k / n; the device search walks forward / backward
     // from there, so it returns the first i with u <= cdf[i] (clamped to n-1) for any u.
+    const size_t n = cdf.size();
     guide.resize(n);
     size_t i = 0;
     for (size_t k = 0; k < n; ++k) {

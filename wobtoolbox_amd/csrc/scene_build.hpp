@@ -48,4 +48,7 @@ std::string build_scene(const wob_scene_desc &desc, HostScene &out);
 void build_cdf(const std::vector<float> &weights, std::vector<float> &cdf, float &total,
                std::vector<uint32_t> &guide);
 
+// Guide table of a normalised CDF: guide[k] = first i with cdf[i] >= k / n.
+void build_guide(const std::vector<float> &cdf, std::vector<uint32_t> &guide);
+
 }  // namespace wobhost

@@ -46,6 +46,8 @@ struct wob_scene {
     uint64_t device_bytes = 0;
     int device = 0;
     float *cv_table = nullptr;
+    void *vol_bufs[3] = {nullptr, nullptr, nullptr};
+    uint64_t version = 0;   // bumped whenever device data referenced by launches changes
 };
 
 const wobhost::HostScene &wob_scene_host(const wob_scene *scene) { return scene->host; }
@@ -55,6 +57,7 @@ wob_status wob_scene_bind_cv_table(wob_scene *scene, float *d_table, uint32_t R)
     scene->cv_table = d_table;
     scene->dev.cv_table = d_table;
     scene->dev.cv_R = d_table ? R : 0u;
+    scene->version += 1;
     return WOB_OK;
 }
 
@@ -72,7 +75,15 @@ template <typename T> wob_status upload(wob_scene *s, int slot, const std::vecto
     return WOB_OK;
 }
 
+void free_volume(wob_scene *s) {
+    for (auto &b : s->vol_bufs)
+        if (b) { (void)hipFree(b); b = nullptr; }
+    s->dev.vol = wobdev::VolSrc{};
+    s->version += 1;
+}
+
 void free_scene(wob_scene *s) {
+    free_volume(s);
     for (auto &b : s->bufs)
         if (b) { (void)hipFree(b); b = nullptr; }
     if (s->scratch) (void)hipFree(s->scratch);
@@ -102,7 +113,8 @@ wob_status check_est(const wob_scene *scene, const wob_estimator_desc *est, int 
     if (est->estimator < 0 || est->estimator > 1) return fail(WOB_INVALID_ENUM, "estimator out of range");
     if (est->problem < 0 || est->problem > 3) return fail(WOB_INVALID_ENUM, "problem out of range");
     if (est->path_length == 0) return fail(WOB_INVALID_ARG, "path_length must be >= 1");
-    if (est->flags & ~(uint32_t)(WOB_EST_EXCLUDE_ORIGIN_ELEMENT | WOB_EST_CONTROL_VARIATE | WOB_EST_BOUNDARY_DENSITY))
+    if (est->flags & ~(uint32_t)(WOB_EST_EXCLUDE_ORIGIN_ELEMENT | WOB_EST_CONTROL_VARIATE | WOB_EST_BOUNDARY_DENSITY |
+                                 WOB_EST_INDEPENDENT_VOLUME_SAMPLES))
         return fail(WOB_INVALID_ARG, "unknown estimator flags");
     if (est->flags & WOB_EST_CONTROL_VARIATE) {
         if (est->bie_model != WOB_SINGLE_LAYER || est->estimator != WOB_BACKWARD)
@@ -113,8 +125,9 @@ wob_status check_est(const wob_scene *scene, const wob_estimator_desc *est, int 
     if ((est->flags & WOB_EST_BOUNDARY_DENSITY) && est->estimator != WOB_BACKWARD)
         return fail(WOB_UNSUPPORTED_COMBO, "boundary density estimates need the backward estimator");
     if (est->num_resampling_candidates == 0) return fail(WOB_INVALID_ARG, "num_resampling_candidates must be >= 1");
-    if (est->num_volume_samples != 0)
-        return fail(WOB_UNSUPPORTED_COMBO, "volume sources are not enabled in this build (num_volume_samples must be 0)");
+    if (est->num_volume_samples != 0 && scene->dev.vol.kind == wobdev::kVolNone)
+        return fail(WOB_INVALID_ARG, "num_volume_samples > 0 needs a volume source on the scene "
+                                     "(wob_scene_set_volume_grid / wob_scene_set_volume_gaussians)");
     if (est->bie_model == WOB_DOUBLE_LAYER && est->problem != WOB_DIRICHLET_PROBLEM)
         return fail(WOB_UNSUPPORTED_COMBO,
                     "the double-layer model supports Dirichlet problems only (bie_models.cuh:383, 394)");
@@ -171,6 +184,7 @@ wobl::WalkEntry make_entry(const wob_scene *scene, const wob_estimator_desc *est
     e.path_base = path_base;
     e.flags = est->flags;
     e.cv_samples = est->cv_samples ? est->cv_samples : 64u;
+    e.nvs = est->num_volume_samples;
     return e;
 }
 
@@ -220,7 +234,9 @@ wob_status run_walk(const wob_scene *cscene, const wob_estimator_desc *est, cons
         WOB_HIP(hipEventCreate(&ev[1]), "hipEventCreate");
         WOB_HIP(hipEventRecord(ev[0], hs), "hipEventRecord");
     }
-    WOB_HIP(wobl::launch_walk(D, boundary, L, est->bie_model, est->estimator, flavour, block, 0, hs), "walk launch");
+    WOB_HIP(wobl::launch_walk(D, boundary, est->num_volume_samples > 0, L, est->bie_model, est->estimator, flavour, block,
+                              0, hs),
+            "walk launch");
     if (timed) WOB_HIP(hipEventRecord(ev[1], hs), "hipEventRecord");
     WOB_HIP(wobl::launch_reduce(L, V, hs), "reduce launch");
     if (launch) {
@@ -244,7 +260,9 @@ struct wob_batch {
     std::vector<wob_batch_entry> entries;
     wob_batch_config cfg{};
     int dim = 0, bie = 0, est = 0, flavour = 0;
+    bool volume = false;
     uint32_t V = 1;
+    std::vector<uint64_t> versions;         // scene versions the entry table was built from
     wobl::WalkLaunch L{};
     wobl::WalkEntry *d_table = nullptr;     // device entry table
     float *partials = nullptr;
@@ -326,9 +344,11 @@ wob_status batch_configure(wob_batch *b, bool redirect) {
     L.work_counter = b->counters;
     L.pass_counter = b->counters + 1;
     L.partials = b->partials;
-    b->grid = L.persistent ? wobl::persistent_grid(b->dim, b->cfg.boundary != 0, b->bie, b->est, b->flavour,
+    b->grid = L.persistent ? wobl::persistent_grid(b->dim, b->cfg.boundary != 0, b->volume, b->bie, b->est, b->flavour,
                                                    b->cfg.block_size ? b->cfg.block_size : 128u)
                            : 0u;
+    b->versions.resize(n);
+    for (uint32_t i = 0; i < n; ++i) b->versions[i] = b->entries[i].scene->version;
     batch_release_graph(b);
     return WOB_OK;
 }
@@ -336,7 +356,8 @@ wob_status batch_configure(wob_batch *b, bool redirect) {
 wob_status batch_pass(wob_batch *b, uint64_t seed, hipStream_t hs) {
     b->L.seed = seed;
     const uint32_t block = b->cfg.block_size ? b->cfg.block_size : 128u;
-    WOB_HIP(wobl::launch_walk(b->dim, b->cfg.boundary != 0, b->L, b->bie, b->est, b->flavour, block, b->grid, hs),
+    WOB_HIP(wobl::launch_walk(b->dim, b->cfg.boundary != 0, b->volume, b->L, b->bie, b->est, b->flavour, block, b->grid,
+                              hs),
             "batch walk launch");
     WOB_HIP(wobl::launch_reduce(b->L, b->V, hs), "batch reduce launch");
     WOB_HIP(wobl::launch_finalize(b->counters, b->counters + 1, hs), "batch finalize launch");
@@ -416,9 +437,96 @@ wob_status wob_scene_get_info(const wob_scene *scene, wob_scene_info *info) {
     return WOB_OK;
 }
 
-wob_status wob_scene_set_volume_grid(wob_scene *scene, uint32_t, float, const float *) {
+wob_status wob_scene_set_volume_grid(wob_scene *scene, uint32_t grid_res, float domain_size, const float *cache) {
     if (!scene) return fail(WOB_INVALID_ARG, "null scene");
-    return fail(WOB_UNSUPPORTED_COMBO, "grid volume sources are not enabled in this build");
+    WOB_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    free_volume(scene);
+    if (grid_res == 0 || !cache) return WOB_OK;
+    if (!(domain_size > 0.f)) return fail(WOB_INVALID_ARG, "domain_size must be > 0");
+    const uint32_t D = scene->host.dim;
+    const uint64_t cells = (uint64_t)grid_res * grid_res * (D == 3 ? grid_res : 1u);
+    if (cells >= (1ull << 31)) return fail(WOB_INVALID_ARG, "grid too large");
+    // scene.cuh:215-228: |cache| normalised by its sum, inclusive scan (here sequential FP32)
+    std::vector<float> cdf(cells);
+    float total = 0.f;
+    for (uint64_t i = 0; i < cells; ++i) { cdf[i] = std::fabs(cache[i]); total += cdf[i]; }
+    if (total == 0.f) for (auto &c : cdf) c = 1.f / (float)cells;
+    else for (auto &c : cdf) c /= total;
+    for (uint64_t i = 1; i < cells; ++i) cdf[i] += cdf[i - 1];
+    std::vector<uint32_t> guide;
+    wobhost::build_guide(cdf, guide);
+    const std::vector<float> vals(cache, cache + cells);
+    void *p[3] = {nullptr, nullptr, nullptr};
+    const size_t bytes[3] = {cells * 4, cells * 4, guide.size() * 4};
+    for (int i = 0; i < 3; ++i) {
+        if (hipMalloc(&p[i], bytes[i]) != hipSuccess) {
+            for (int j = 0; j < i; ++j) (void)hipFree(p[j]);
+            return fail(WOB_OOM, "hipMalloc volume grid");
+        }
+        scene->vol_bufs[i] = p[i];
+    }
+    WOB_HIP(hipMemcpy(p[0], vals.data(), bytes[0], hipMemcpyHostToDevice), "hipMemcpy volume");
+    WOB_HIP(hipMemcpy(p[1], cdf.data(), bytes[1], hipMemcpyHostToDevice), "hipMemcpy volume");
+    WOB_HIP(hipMemcpy(p[2], guide.data(), bytes[2], hipMemcpyHostToDevice), "hipMemcpy volume");
+    wobdev::VolSrc &v = scene->dev.vol;
+    v.kind = wobdev::kVolGrid;
+    v.n = (uint32_t)cells;
+    v.grid_res = grid_res;
+    v.domain_size = domain_size;
+    v.cache = (const float *)p[0];
+    v.cdf = (const float *)p[1];
+    v.guide = (const uint32_t *)p[2];
+    scene->version += 1;
+    return WOB_OK;
+}
+
+wob_status wob_scene_set_volume_gaussians(wob_scene *scene, uint32_t n, const float *centers, const float *widths,
+                                          const float *amplitudes, uint32_t flags) {
+    if (!scene) return fail(WOB_INVALID_ARG, "null scene");
+    WOB_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    free_volume(scene);
+    if (n == 0) return WOB_OK;
+    if (!centers || !widths || !amplitudes) return fail(WOB_INVALID_ARG, "null source arrays");
+    if (flags & ~(uint32_t)WOB_VOLUME_NO_MASK) return fail(WOB_INVALID_ARG, "unknown volume flags");
+    const int D = (int)scene->host.dim;
+    // mixture weights |A_j| w_j^D; constants in double, rounded once
+    std::vector<double> wt(n);
+    double tot = 0.0;
+    for (uint32_t j = 0; j < n; ++j) {
+        if (!(widths[j] > 0.f)) return fail(WOB_INVALID_ARG, "Gaussian widths must be > 0");
+        wt[j] = std::fabs((double)amplitudes[j]) * std::pow((double)widths[j], D);
+        tot += wt[j];
+    }
+    if (!(tot > 0.0)) return fail(WOB_INVALID_ARG, "all Gaussian amplitudes are zero");
+    std::vector<float> comp((size_t)8 * n), cdf(n);
+    double acc = 0.0;
+    for (uint32_t j = 0; j < n; ++j) {
+        const double w = widths[j], pj = wt[j] / tot;
+        acc += pj;
+        float *c = &comp[(size_t)8 * j];
+        for (int k = 0; k < 3; ++k) c[k] = k < D ? centers[(size_t)j * D + k] : 0.f;
+        c[3] = widths[j];
+        c[4] = amplitudes[j];
+        c[5] = (float)(pj * std::pow(2.0 * M_PI * w * w, -0.5 * D));
+        c[6] = (float)(1.0 / (2.0 * w * w));
+        c[7] = 0.f;
+        cdf[j] = j + 1 == n ? 1.0f : (float)acc;
+    }
+    void *p[2] = {nullptr, nullptr};
+    if (hipMalloc(&p[0], comp.size() * 4) != hipSuccess) return fail(WOB_OOM, "hipMalloc volume source");
+    scene->vol_bufs[0] = p[0];
+    if (hipMalloc(&p[1], cdf.size() * 4) != hipSuccess) return fail(WOB_OOM, "hipMalloc volume source");
+    scene->vol_bufs[1] = p[1];
+    WOB_HIP(hipMemcpy(p[0], comp.data(), comp.size() * 4, hipMemcpyHostToDevice), "hipMemcpy volume");
+    WOB_HIP(hipMemcpy(p[1], cdf.data(), cdf.size() * 4, hipMemcpyHostToDevice), "hipMemcpy volume");
+    wobdev::VolSrc &v = scene->dev.vol;
+    v.kind = wobdev::kVolGauss;
+    v.n = n;
+    v.mask = (flags & WOB_VOLUME_NO_MASK) ? 0u : 1u;
+    v.comp = (const float4 *)p[0];
+    v.cdf = (const float *)p[1];
+    scene->version += 1;
+    return WOB_OK;
 }
 
 wob_status wob_estimate_domain(const wob_scene *scene, const wob_estimator_desc *est, const float *d_points,
@@ -541,12 +649,14 @@ wob_status wob_batch_create(const wob_batch_entry *entries, uint32_t n, const wo
             return fail(WOB_INVALID_ARG, "entry " + std::to_string(i) + ": empty point set or null buffer");
         }
         const int dim = (int)e.scene->host.dim;
-        if (i == 0) { b->dim = dim; b->bie = e.est.bie_model; b->est = e.est.estimator; b->flavour = fl; }
-        else if (dim != b->dim || e.est.bie_model != b->bie || e.est.estimator != b->est || fl != b->flavour) {
+        const bool vol = e.est.num_volume_samples > 0;
+        if (i == 0) { b->dim = dim; b->bie = e.est.bie_model; b->est = e.est.estimator; b->flavour = fl; b->volume = vol; }
+        else if (dim != b->dim || e.est.bie_model != b->bie || e.est.estimator != b->est || fl != b->flavour ||
+                 vol != b->volume) {
             delete b;
             return fail(WOB_UNSUPPORTED_COMBO, "entry " + std::to_string(i) +
-                                                   ": all batch entries must share dimension, model, estimator and "
-                                                   "sampling strategy (split the batch)");
+                                                   ": all batch entries must share dimension, model, estimator, "
+                                                   "sampling strategy and the use of a volume term (split the batch)");
         }
     }
     b->V = c.boundary ? 1u : (uint32_t)b->dim + 1u;
@@ -581,6 +691,16 @@ wob_status wob_batch_create(const wob_batch_entry *entries, uint32_t n, const wo
 wob_status wob_batch_run(wob_batch *b, uint64_t seed, uint32_t n_passes, void *stream) {
     if (!b) return fail(WOB_INVALID_ARG, "null batch");
     hipStream_t hs = (hipStream_t)stream;
+    // a scene whose device data changed since the entry table was built (volume source, control
+    // variate) invalidates the table and any captured graph
+    for (size_t i = 0; i < b->entries.size(); ++i) {
+        if (b->entries[i].scene->version != b->versions[i]) {
+            WOB_HIP(hipStreamSynchronize(hs), "hipStreamSynchronize");
+            wob_status st = batch_configure(b, false);
+            if (st != WOB_OK) return st;
+            break;
+        }
+    }
     if (!b->cfg.use_graph) {
         for (uint32_t p = 0; p < n_passes; ++p) {
             wob_status st = batch_pass(b, seed, hs);

@@ -37,6 +37,22 @@ enum { FL_LINE_LINE = 0,   // backward, double layer, Dirichlet problem   (estim
        FL_FWD_UNIFORM = 4  // forward, everything else
 };
 
+// Volume source of a Poisson problem: the reference's grid cache (scene.cuh:78-120,
+// value-proportional cell sampling) or a Gaussian mixture f(x) = sum_j A_j exp(-|x - c_j|^2 /
+// (2 w_j^2)) sampled from the mixture with weights proportional to |A_j| w_j^D.
+enum { kVolNone = 0, kVolGrid = 1, kVolGauss = 2 };
+struct VolSrc {
+    uint32_t kind;
+    uint32_t n;              // grid: cells; mixture: components
+    uint32_t grid_res;
+    uint32_t mask;           // mixture: samples on the far side of the boundary contribute 0
+    float domain_size;       // grid covers (-domain_size / 2, domain_size / 2)^D
+    const float *cache;      // grid values
+    const float *cdf;        // grid cell CDF (n) or mixture CDF (n)
+    const uint32_t *guide;   // grid guide table
+    const float4 *comp;      // mixture: [cx cy cz w] [A pn inv2w2 0] per component
+};
+
 struct DevScene {
     const float4 *nodes;
     const float4 *prims;
@@ -51,6 +67,7 @@ struct DevScene {
     float total_bv;
     uint32_t cv_R;            // number of value functions in cv_table (0: none)
     const float *cv_table;    // control variate values per (r - 1, element, vertex): [cv_R][n][D]
+    VolSrc vol;
 };
 
 // ----------------------------------------------------------------------------------------
@@ -526,6 +543,196 @@ __device__ __forceinline__ Sample<D> line_sample(const DevScene &S, const float 
 }
 
 // ----------------------------------------------------------------------------------------
+// Volume sources
+// ----------------------------------------------------------------------------------------
+template <int D> struct VolSample {
+    float p[D];
+    float value;
+    float inv_pdf;
+};
+
+// Crossings of the ray o + t r, t > 0, with the boundary along a fixed direction r; odd
+// means inside a closed boundary.  Kept out of line: it is the only traversal of a volume
+// sample, and inlining it at every volume-term call site multiplies the kernels' code.
+template <int D, int STACK>
+__device__ __noinline__ uint32_t ray_crossings(const float4 *nodes, const float4 *prims, float o0, float o1, float o2,
+                                               uint32_t *stack, uint32_t stride) {
+    DevScene S{};
+    S.nodes = nodes;
+    S.prims = prims;
+    const float o[3] = {o0, o1, o2};
+    float d[D];
+    if constexpr (D == 2) { d[0] = 0.58778525f; d[1] = 0.80901699f; }
+    else { d[0] = 0.48218104f; d[1] = 0.66748405f; d[2] = 0.56749094f; }
+    uint32_t c = 0;
+    traverse<D, STACK>(S, o, d, stack, stride, [&](uint32_t, float t, float, float) {
+        if (t > 0.f) ++c;
+    });
+    return c;
+}
+
+// scene.cuh:100-120: a cell with probability |cache| / sum |cache|, a uniform point in it,
+// inv_pdf = cell volume / cell probability.
+template <int D> __device__ __forceinline__ VolSample<D> sample_grid(const VolSrc &v, Rng &rng) {
+    VolSample<D> r;
+    const float unif = rng.uniform();
+    const uint32_t idx = cdf_lookup(v.cdf, v.guide, v.n, unif);
+    const float dx = v.domain_size / (float)v.grid_res;
+    uint32_t u[3] = {idx % v.grid_res, (idx / v.grid_res) % v.grid_res, idx / (v.grid_res * v.grid_res)};
+    if constexpr (D == 2) u[1] = idx / v.grid_res;
+#pragma unroll
+    for (int k = 0; k < D; ++k) r.p[k] = ((float)u[k] + 0.5f) * dx - (v.domain_size / 2);   // utils.cuh:92-99
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const float uu = rng.uniform();
+        r.p[k] += (uu - 0.5f) * v.domain_size / (float)v.grid_res;
+    }
+    r.value = v.cache[idx];
+    const float pdf = idx == 0 ? v.cdf[idx] : v.cdf[idx] - v.cdf[idx - 1];
+    float cellv = 1.f;
+#pragma unroll
+    for (int k = 0; k < D; ++k) cellv *= v.domain_size / (float)v.grid_res;
+    r.inv_pdf = cellv / pdf;
+    return r;
+}
+
+// Gaussian mixture: component j from the mixture CDF, Box-Muller normals from (0, 1]
+// uniforms, value f(p), inv_pdf = 1 / mixture density(p); with the mask, value 0 on the far
+// side of the boundary.
+template <int D, int STACK>
+__device__ __forceinline__ VolSample<D> sample_gauss(const DevScene &S, Rng &rng, bool interior, uint32_t *stack,
+                                                     uint32_t stride) {
+    const VolSrc &v = S.vol;
+    VolSample<D> r;
+    const float u = rng.uniform();
+    uint32_t j = 0;
+    while (j + 1 < v.n && u > v.cdf[j]) ++j;
+    const float u1 = rng.uniform(), u2 = rng.uniform();
+    const float r1 = sqrtf(-2.f * logf(u1));
+    float sn, cs;
+    sincospif(2.f * u2, &sn, &cs);
+    float z[3];
+    z[0] = r1 * cs;
+    z[1] = r1 * sn;
+    if constexpr (D == 3) {
+        const float u3 = rng.uniform(), u4 = rng.uniform();
+        float s2, c2;
+        sincospif(2.f * u4, &s2, &c2);
+        z[2] = sqrtf(-2.f * logf(u3)) * c2;
+    }
+    const float4 cj = v.comp[2 * j];
+    const float cjv[3] = {cj.x, cj.y, cj.z};
+#pragma unroll
+    for (int k = 0; k < D; ++k) r.p[k] = cjv[k] + cj.w * z[k];
+    float f = 0.f, pdf = 0.f;
+    for (uint32_t i = 0; i < v.n; ++i) {
+        const float4 a = v.comp[2 * i], b = v.comp[2 * i + 1];
+        const float av[3] = {a.x, a.y, a.z};
+        float d2 = 0.f;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const float dk = r.p[k] - av[k];
+            d2 += dk * dk;
+        }
+        const float e = expf(-d2 * b.z);
+        f += b.x * e;
+        pdf += b.y * e;
+    }
+    r.value = f;
+    r.inv_pdf = 1.0f / pdf;
+    if (v.mask) {
+        const uint32_t c = ray_crossings<D, STACK>(S.nodes, S.prims, r.p[0], r.p[1], D == 3 ? r.p[D - 1] : 0.f, stack,
+                                                   stride);
+        if (((c & 1u) != 0u) != interior) r.value = 0.f;
+    }
+    return r;
+}
+
+// No volume source: every volume term is zero and draws nothing.
+struct NoVol {
+    template <int D> __device__ __forceinline__ float term(const BP<D> &, const Kern<D> &, Rng &) const { return 0.f; }
+    template <int D> __device__ __forceinline__ float init_domain(const float *, const Kern<D> &, Rng &) const {
+        return 0.f;
+    }
+    template <int D> __device__ __forceinline__ float init_boundary(const float *, const Kern<D> &, Rng &) const {
+        return 0.f;
+    }
+    template <int D>
+    __device__ __forceinline__ void init_derivative(const float *, const Kern<D> &, Rng &, float *g) const {
+#pragma unroll
+        for (int i = 0; i < D; ++i) g[i] = 0.f;
+    }
+};
+
+// Volume terms of bie_models.cuh:79-154.  The reference draws ONE sample per call and adds its
+// contribution num_volume_samples times; with `indep` every addend has its own sample.
+template <int D, int STACK> struct VolCtx {
+    const DevScene *S;
+    uint32_t nvs;
+    uint32_t indep;
+    bool interior;
+    uint32_t *stack;
+    uint32_t stride;
+
+    __device__ __forceinline__ VolSample<D> sample(Rng &rng) const {
+        if (S->vol.kind == kVolGrid) return sample_grid<D>(S->vol, rng);
+        return sample_gauss<D, STACK>(*S, rng, interior, stack, stride);
+    }
+    // :79-103
+    __device__ __forceinline__ float term(const BP<D> &bp, const Kern<D> &k, Rng &rng) const {
+        if (nvs == 0) return 0.f;
+        VolSample<D> v;
+        float res = 0.f;
+        for (uint32_t i = 0; i < nvs; ++i) {
+            if (i == 0 || indep) v = sample(rng);
+            if (bp.type == kDirichlet) res += k.G(bp.p, v.p) * v.value * v.inv_pdf;
+            else if (bp.type == kNeumann) res += k.dG_dny(bp.p, v.p, bp.n) * v.value * v.inv_pdf;
+            else res += (k.dG_dny(bp.p, v.p, bp.n) + bp.alpha * k.G(v.p, bp.p)) * v.value * v.inv_pdf;
+        }
+        return res / (float)nvs;
+    }
+    // :121-135 (domain point)
+    __device__ __forceinline__ float init_domain(const float *x, const Kern<D> &k, Rng &rng) const {
+        if (nvs == 0) return 0.f;
+        VolSample<D> v;
+        float res = 0.f;
+        for (uint32_t i = 0; i < nvs; ++i) {
+            if (i == 0 || indep) v = sample(rng);
+            res -= k.G(v.p, x) * v.value * v.inv_pdf;
+        }
+        return res / (float)nvs;
+    }
+    // :105-119 (boundary point)
+    __device__ __forceinline__ float init_boundary(const float *x, const Kern<D> &k, Rng &rng) const {
+        if (nvs == 0) return 0.f;
+        VolSample<D> v;
+        float res = 0.f;
+        for (uint32_t i = 0; i < nvs; ++i) {
+            if (i == 0 || indep) v = sample(rng);
+            res -= k.G(x, v.p) * v.value * v.inv_pdf;
+        }
+        return res / (float)nvs;
+    }
+    // :137-154
+    __device__ __forceinline__ void init_derivative(const float *x, const Kern<D> &k, Rng &rng, float *g) const {
+#pragma unroll
+        for (int i = 0; i < D; ++i) g[i] = 0.f;
+        if (nvs == 0) return;
+        VolSample<D> v;
+        for (uint32_t i = 0; i < nvs; ++i) {
+            if (i == 0 || indep) v = sample(rng);
+            float dg[D];
+            k.dG_dx(v.p, x, dg);
+            const float w = v.value * v.inv_pdf;
+#pragma unroll
+            for (int c = 0; c < D; ++c) g[c] = g[c] - w * dg[c];
+        }
+#pragma unroll
+        for (int c = 0; c < D; ++c) g[c] = g[c] / (float)nvs;
+    }
+};
+
+// ----------------------------------------------------------------------------------------
 // BIE models (bie_models.cuh).  interior_angle is 0.5 for every sample the scene produces
 // (scene.cuh:455, 510, 615, 750, 815, 910), so integral_free_term is 0.5 for both domains.
 // ----------------------------------------------------------------------------------------
@@ -571,27 +778,37 @@ template <int D, int BIE> struct Model {
             }
         }
     }
-    // known terms without a volume source
-    __device__ __forceinline__ float solution_domain_known(const float *x, const BP<D> &y, float inv) const {
-        if constexpr (BIE != kDirect) return 0.f;                                                 // :233-238, :346-351
+    // known terms; `vt` supplies the volume terms (NoVol: none) and draws their samples in the
+    // reference's order (the domain-point init term first, then the term at y)
+    template <typename VT>
+    __device__ __forceinline__ float solution_domain_known(const float *x, const BP<D> &y, float inv, const VT &vt,
+                                                           Rng &rng) const {
+        const float vi = vt.init_domain(x, k, rng);                                               // :233-238, :346-351, :442
+        if constexpr (BIE != kDirect) return vi;
         else {
-            if (y.type == kDirichlet) return 0.f + -dsign * inv * k.dG_dny(y.p, x, y.n) * (y.bv + 0.f);   // :443-445
-            return 0.f + dsign * inv * k.G(y.p, x) * (y.bv + 0.f);                                // :446-448
+            const float b = y.bv + vt.term(y, k, rng);
+            if (y.type == kDirichlet) return vi + -dsign * inv * k.dG_dny(y.p, x, y.n) * b;     // :443-445
+            return vi + dsign * inv * k.G(y.p, x) * b;                                           // :446-448
         }
     }
-    __device__ __forceinline__ void gradient_domain_known(const float *x, const BP<D> &y, float inv, float *g) const {
+    template <typename VT>
+    __device__ __forceinline__ void gradient_domain_known(const float *x, const BP<D> &y, float inv, const VT &vt,
+                                                          Rng &rng, float *g) const {
+        float gv[D];
+        vt.init_derivative(x, k, rng, gv);                                                        // :245-250, :358-363, :468
         if constexpr (BIE != kDirect) {
 #pragma unroll
-            for (int i = 0; i < D; ++i) g[i] = 0.f;
-        } else {                                                                                  // :468-477
+            for (int i = 0; i < D; ++i) g[i] = gv[i];
+        } else {                                                                                  // :469-477
+            const float b = y.bv + vt.term(y, k, rng);
             if (y.type == kDirichlet) {
                 k.d2G_dxdny(y.p, x, y.n, g);
 #pragma unroll
-                for (int i = 0; i < D; ++i) g[i] = 0.f + ((-dsign * inv) * g[i]) * (y.bv + 0.f);
+                for (int i = 0; i < D; ++i) g[i] = gv[i] + ((-dsign * inv) * g[i]) * b;
             } else {
                 k.dG_dx(y.p, x, g);
 #pragma unroll
-                for (int i = 0; i < D; ++i) g[i] = 0.f + ((dsign * inv) * g[i]) * (y.bv + 0.f);
+                for (int i = 0; i < D; ++i) g[i] = gv[i] + ((dsign * inv) * g[i]) * b;
             }
         }
     }
@@ -608,16 +825,20 @@ template <int D, int BIE> struct Model {
             return -dsign / ift * inv * (k.dG_dny(y.p, x.p, y.n) + y.alpha * k.G(y.p, x.p));
         }
     }
-    __device__ __forceinline__ float solution_boundary_known(const BP<D> &x, const BP<D> &y, float inv) const {
+    template <typename VT>
+    __device__ __forceinline__ float solution_boundary_known(const BP<D> &x, const BP<D> &y, float inv, const VT &vt,
+                                                             Rng &rng) const {
         if constexpr (BIE == kSL) {                                                               // :259-266
             if (x.type == kDirichlet) return x.bv;
-            return 0.f;
+            return vt.init_boundary(x.p, k, rng);
         } else if constexpr (BIE == kDL) {                                                        // :371-377
             return x.bv;
         } else {                                                                                  // :495-511
             if (x.type == kDirichlet) return x.bv;
-            if (y.type == kDirichlet) return 0.f + -dsign / ift * inv * k.dG_dny(y.p, x.p, y.n) * (y.bv + 0.f);
-            return 0.f + dsign / ift * inv * k.G(y.p, x.p) * (y.bv + 0.f);
+            const float vi = vt.init_boundary(x.p, k, rng);
+            const float b = y.bv + vt.term(y, k, rng);
+            if (y.type == kDirichlet) return vi + -dsign / ift * inv * k.dG_dny(y.p, x.p, y.n) * b;
+            return vi + dsign / ift * inv * k.G(y.p, x.p) * b;
         }
     }
     // Fredholm unknown after the first-kind splitting has been resolved (split not taken)
@@ -659,22 +880,33 @@ template <int D, int BIE> struct Model {
         }
         return fredholm_unknown_kernel(x, y, inv);
     }
-    // bie_models.cuh:301-316 (SL), 389-399 (DL), 557-587 (Direct), no volume source
-    __device__ __forceinline__ float fredholm_known(const BP<D> &x, const BP<D> &y, float inv) const {
+    // bie_models.cuh:301-316 (SL), 389-399 (DL), 557-587 (Direct); the term at x is drawn
+    // before the term at y
+    template <typename VT>
+    __device__ __forceinline__ float fredholm_known(const BP<D> &x, const BP<D> &y, float inv, const VT &vt,
+                                                    Rng &rng) const {
         if constexpr (BIE == kSL) {
-            if (x.type == kDirichlet) return 2.f * c * (x.bv + 0.f);
-            return dsign / ift * (x.bv + 0.f);
+            const float a = x.bv + vt.term(x, k, rng);
+            if (x.type == kDirichlet) return 2.f * c * a;
+            return dsign / ift * a;
         } else if constexpr (BIE == kDL) {
-            return dsign * 1.f / (1.f - ift) * (x.bv + 0.f);
+            return dsign * 1.f / (1.f - ift) * (x.bv + vt.term(x, k, rng));
         } else {
-            if (x.type == kDirichlet) {
-                if (y.type == kDirichlet)
-                    return dsign * c * ((x.bv + 0.f) + dsign / ift * inv * k.dG_dny(y.p, x.p, y.n) * (y.bv + 0.f));
-                return dsign * c * ((x.bv + 0.f) - dsign / ift * inv * k.G(y.p, x.p) * (y.bv + 0.f));
+            const bool xd = x.type == kDirichlet;
+            const float a = xd ? x.bv + vt.term(x, k, rng) : 0.f;
+            const float b = y.bv + vt.term(y, k, rng);
+            if (xd) {
+                if (y.type == kDirichlet) return dsign * c * (a + dsign / ift * inv * k.dG_dny(y.p, x.p, y.n) * b);
+                return dsign * c * (a - dsign / ift * inv * k.G(y.p, x.p) * b);
             }
-            if (y.type == kDirichlet) return -dsign / ift * inv * k.dG_dny(y.p, x.p, y.n) * (y.bv + 0.f);
-            return dsign / ift * inv * k.G(y.p, x.p) * (y.bv + 0.f);
+            if (y.type == kDirichlet) return -dsign / ift * inv * k.dG_dny(y.p, x.p, y.n) * b;
+            return dsign / ift * inv * k.G(y.p, x.p) * b;
         }
+    }
+    // without a volume source (deterministic parity kernels)
+    __device__ __forceinline__ float fredholm_known(const BP<D> &x, const BP<D> &y, float inv) const {
+        Rng unused;
+        return fredholm_known(x, y, inv, NoVol{}, unused);
     }
 };
 
@@ -738,26 +970,27 @@ __device__ __forceinline__ Sample<D> ris(const DevScene &S, const Model<D, BIE> 
 //   Neumann / Robin y_m:              -(dsign / ift) |G| / M sum_i s_i V_r(z_i).
 // The walk's expectation is unchanged for any table V; with V equal to the value functions
 // of the walk the path-choice randomness cancels.
-template <int D, int BIE, int FL, int STACK>
+template <int D, int BIE, int FL, int STACK, typename VT>
 __device__ __forceinline__ float walk_backward(const DevScene &S, const Model<D, BIE> &m, const WalkParams &wp,
-                                               const BP<D> &x0, Rng &rng, uint32_t *stack, uint32_t stride) {
+                                               const VT &vt, const BP<D> &x0, Rng &rng, uint32_t *stack,
+                                               uint32_t stride) {
     float spc = 0.f, W = 1.f;
     BP<D> yp = x0;
     for (uint32_t it = 0; it < wp.path_length; ++it) {
         if (it == wp.path_length - 1) W = 0.5f * W;
         if constexpr (FL == FL_LINE_LINE) {
             Sample<D> q = line_sample<D, STACK>(S, yp.p, wp.skip(yp.elem), rng, stack, stride);
-            spc += W * m.fredholm_known(yp, q.y, q.inv_pdf);
+            spc += W * m.fredholm_known(yp, q.y, q.inv_pdf, vt, rng);
             W *= m.fredholm_unknown_backward(yp, q.y, q.inv_pdf, rng);
             yp = q.y;
         } else if constexpr (FL == FL_LINE_BV) {
             Sample<D> q = sample_bv<D>(S, rng);
-            spc += W * m.fredholm_known(yp, q.y, q.inv_pdf);
+            spc += W * m.fredholm_known(yp, q.y, q.inv_pdf, vt, rng);
             q = line_sample<D, STACK>(S, yp.p, wp.skip(yp.elem), rng, stack, stride);
             W = W * m.fredholm_unknown_backward(yp, q.y, q.inv_pdf, rng);
             yp = q.y;
         } else if constexpr (BIE == kSL) {
-            spc += W * m.fredholm_known(yp, yp, 0.f);   // SL ignores y here (estimator.cuh:278-281)
+            spc += W * m.fredholm_known(yp, yp, 0.f, vt, rng);   // SL ignores y here (estimator.cuh:278-281)
             const uint32_t cv_r = (wp.cv && it + 1 < wp.path_length) ? wp.path_length - it - 1 : 0u;
             float ksum = 0.f;
             Sample<D> q;
@@ -783,7 +1016,7 @@ __device__ __forceinline__ float walk_backward(const DevScene &S, const Model<D,
             yp = q.y;
         } else {
             Sample<D> q = sample_uniform<D>(S, rng);
-            spc += W * m.fredholm_known(yp, q.y, q.inv_pdf);
+            spc += W * m.fredholm_known(yp, q.y, q.inv_pdf, vt, rng);
             q = sample_uniform<D>(S, rng);
             W = W * m.fredholm_unknown_backward(yp, q.y, q.inv_pdf, rng);
             yp = q.y;
@@ -810,18 +1043,19 @@ __device__ __forceinline__ void first_samples(const DevScene &S, const float *x,
 }
 
 // one domain path: out = [u, grad u]
-template <int D, int BIE, int EST, int FL, int STACK>
+template <int D, int BIE, int EST, int FL, int STACK, typename VT>
 __device__ __forceinline__ void domain_path(const DevScene &S, const Model<D, BIE> &m, const WalkParams &wp,
-                                            const float *x, Rng &rng, uint32_t *stack, uint32_t stride, float *out) {
+                                            const VT &vt, const float *x, Rng &rng, uint32_t *stack, uint32_t stride,
+                                            float *out) {
     if constexpr (EST == kBackward) {                                                          // estimator.cuh:154-191
         Sample<D> y, y1;
         first_samples<D, FL, STACK>(S, x, 0xFFFFFFFFu, y, y1, rng, stack, stride);
-        const float known = m.solution_domain_known(x, y1.y, y1.inv_pdf);
+        const float known = m.solution_domain_known(x, y1.y, y1.inv_pdf, vt, rng);
         float gk[D], gu[D];
-        m.gradient_domain_known(x, y1.y, y1.inv_pdf, gk);
+        m.gradient_domain_known(x, y1.y, y1.inv_pdf, vt, rng, gk);
         const float unk = m.solution_domain_unknown(x, y.y, y.inv_pdf);
         m.gradient_domain_unknown(x, y.y, y.inv_pdf, gu);
-        const float spc = walk_backward<D, BIE, FL, STACK>(S, m, wp, y.y, rng, stack, stride);
+        const float spc = walk_backward<D, BIE, FL, STACK>(S, m, wp, vt, y.y, rng, stack, stride);
         out[0] = known + spc * unk;
 #pragma unroll
         for (int k = 0; k < D; ++k) out[1 + k] = gk[k] + spc * gu[k];
@@ -852,13 +1086,13 @@ __device__ __forceinline__ void domain_path(const DevScene &S, const Model<D, BI
         }
     } else {                                                                                   // estimator.cuh:405-454
         Sample<D> y = sample_uniform<D>(S, rng);
-        out[0] = m.solution_domain_known(x, y.y, y.inv_pdf);
+        out[0] = m.solution_domain_known(x, y.y, y.inv_pdf, vt, rng);
         float g[D], gu[D];
-        m.gradient_domain_known(x, y.y, y.inv_pdf, g);
+        m.gradient_domain_known(x, y.y, y.inv_pdf, vt, rng, g);
         if constexpr (FL == FL_FWD_LINE) y = sample_bv<D>(S, rng);
         else y = sample_uniform<D>(S, rng);
         const Sample<D> y2 = sample_uniform<D>(S, rng);
-        float W = y.inv_pdf * m.fredholm_known(y.y, y2.y, y2.inv_pdf);
+        float W = y.inv_pdf * m.fredholm_known(y.y, y2.y, y2.inv_pdf, vt, rng);
         BP<D> yc = y.y;
         for (uint32_t it = 0; it < wp.path_length; ++it) {
             if (it == wp.path_length - 1) W *= 0.5f;
@@ -879,24 +1113,25 @@ __device__ __forceinline__ void domain_path(const DevScene &S, const Model<D, BI
 }
 
 // one boundary path (estimator.cuh:193-222, forward :456-496)
-template <int D, int BIE, int EST, int FL, int STACK>
+template <int D, int BIE, int EST, int FL, int STACK, typename VT>
 __device__ __forceinline__ float boundary_path(const DevScene &S, const Model<D, BIE> &m, const WalkParams &wp,
-                                               const BP<D> &x, Rng &rng, uint32_t *stack, uint32_t stride) {
+                                               const VT &vt, const BP<D> &x, Rng &rng, uint32_t *stack,
+                                               uint32_t stride) {
     if constexpr (EST == kBackward) {
-        if (wp.density) return walk_backward<D, BIE, FL, STACK>(S, m, wp, x, rng, stack, stride);
+        if (wp.density) return walk_backward<D, BIE, FL, STACK>(S, m, wp, vt, x, rng, stack, stride);
         Sample<D> y, y1;
         first_samples<D, FL, STACK>(S, x.p, wp.skip(x.elem), y, y1, rng, stack, stride);
-        float res = m.solution_boundary_known(x, y1.y, y1.inv_pdf);
+        float res = m.solution_boundary_known(x, y1.y, y1.inv_pdf, vt, rng);
         const float unk = m.solution_boundary_unknown(x, y.y, y.inv_pdf);
-        const float spc = walk_backward<D, BIE, FL, STACK>(S, m, wp, y.y, rng, stack, stride);
+        const float spc = walk_backward<D, BIE, FL, STACK>(S, m, wp, vt, y.y, rng, stack, stride);
         return res + spc * unk;
     } else {
         Sample<D> y = sample_uniform<D>(S, rng);
-        float res = m.solution_boundary_known(x, y.y, y.inv_pdf);
+        float res = m.solution_boundary_known(x, y.y, y.inv_pdf, vt, rng);
         if constexpr (FL == FL_FWD_LINE) y = sample_bv<D>(S, rng);
         else y = sample_uniform<D>(S, rng);
         const Sample<D> y2 = sample_uniform<D>(S, rng);
-        float W = y.inv_pdf * m.fredholm_known(y.y, y2.y, y2.inv_pdf);
+        float W = y.inv_pdf * m.fredholm_known(y.y, y2.y, y2.inv_pdf, vt, rng);
         BP<D> yc = y.y;
         for (uint32_t it = 0; it < wp.path_length; ++it) {
             if (it == wp.path_length - 1) W *= 0.5f;

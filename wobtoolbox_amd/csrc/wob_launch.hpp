@@ -33,6 +33,8 @@ struct WalkEntry {
     uint32_t partial_base;    // offset of this entry's partials (floats)
     uint32_t flags;           // WOB_EST_* options
     uint32_t cv_samples;      // domain-level control-variate samples
+    uint32_t nvs;             // num_volume_samples (volume kernels only)
+    uint32_t pad_;
     uint64_t path_base;       // first path index of pass 0
 };
 
@@ -49,8 +51,8 @@ struct WalkLaunch {
     float *partials;
 };
 
-hipError_t launch_walk(int dim, bool boundary, const WalkLaunch &a, int bie, int est, int flavour, uint32_t block,
-                       uint32_t grid, hipStream_t st);
+hipError_t launch_walk(int dim, bool boundary, bool volume, const WalkLaunch &a, int bie, int est, int flavour,
+                       uint32_t block, uint32_t grid, hipStream_t st);
 hipError_t launch_reduce(const WalkLaunch &a, uint32_t V, hipStream_t st);
 hipError_t launch_finalize(uint32_t *work_counter, uint32_t *pass_counter, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
@@ -65,7 +67,20 @@ hipError_t launch_sample_boundary(int dim, const wobdev::DevScene &S, uint32_t n
                                   float *out, hipStream_t st);
 hipError_t launch_philox(uint32_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out, hipStream_t st);
 
+// Per (dimension, volume term) entry points of the walk translation units (wob_walk.hip).
+#define WOB_DECLARE_WALK(d, v)                                                                                  \
+    hipError_t launch_walk_##d##v(bool boundary, const WalkLaunch &a, int bie, int est, int flavour, uint32_t block, \
+                                  uint32_t grid, hipStream_t st);                                               \
+    uint32_t persistent_grid_##d##v(bool boundary, int bie, int est, int flavour, uint32_t block);
+WOB_DECLARE_WALK(2, 0)
+WOB_DECLARE_WALK(2, 1)
+WOB_DECLARE_WALK(3, 0)
+WOB_DECLARE_WALK(3, 1)
+#undef WOB_DECLARE_WALK
+
 // Occupancy-derived grid for persistent launches (waves that fit at once on the device).
-uint32_t persistent_grid(int dim, bool boundary, int bie, int est, int flavour, uint32_t block);
+uint32_t persistent_grid(int dim, bool boundary, bool volume, int bie, int est, int flavour, uint32_t block);
+// LDS bytes of a walk launch (per-lane traversal stacks)
+size_t walk_lds_bytes(int flavour, bool volume, uint32_t block);
 
 }  // namespace wobl

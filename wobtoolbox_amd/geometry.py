@@ -337,6 +337,103 @@ def geodesic_mixed_3d(n_points: int = 1_000_000, freq: int = 100, seed: int = 30
 
 
 # ----------------------------------------------------------------------------------------
+# Poisson: the reference app's square with a constant grid source
+# ----------------------------------------------------------------------------------------
+def square_poisson_2d(source: float = 5.0, per_side: int = 128, grid_res: int = 64, n_points: int = 1024,
+                      seed: int = 5001) -> Workload:
+    """Square (-1, 1)^2 with a constant source on a grid cache covering it, as in
+    apps/poisson_equation.cu:105-135 (there: grid 256, source 5, data 1).  Exact solution
+    u = source r^2 / 4 + exp(x) cos(y) (Laplacian(u) = source); each side is split into
+    `per_side` segments so the linearly interpolated data follow u closely."""
+    t = np.linspace(-1.0, 1.0, per_side + 1)[:-1]
+    one = np.ones(per_side)
+    v = np.concatenate([np.stack([t, -one], 1), np.stack([one, t], 1), np.stack([-t, one], 1), np.stack([-one, -t], 1)])
+
+    def exact(p):
+        p = np.asarray(p, np.float64)
+        x, y = p[:, 0], p[:, 1]
+        e = np.exp(x)
+        return np.stack([source * (x * x + y * y) / 4 + e * np.cos(y), source * x / 2 + e * np.cos(y),
+                         source * y / 2 - e * np.sin(y)], 1)
+
+    el = polygon_elements(v, DIRICHLET, lambda p, nr, ty: exact(p)[:, 0])
+    pts = np.random.default_rng(seed).uniform(-0.9, 0.9, (n_points, 2)).astype(np.float32)
+    solver = dict(bie_model="IndirectDoubleLayer", domain="InteriorDomain", estimator="BackwardEstimator",
+                  problem="DirichletProblem", path_length=4, eps=1e-4, num_volume_samples=16, num_sample_paths=1000)
+    return Workload("square_poisson_2d", el, pts, exact, solver,
+                    extra=dict(grid_res=grid_res, domain_size=2.0,
+                               cache=np.full(grid_res * grid_res, source, np.float32)))
+
+
+# ----------------------------------------------------------------------------------------
+# Poisson: Gaussian-mixture source on the geodesic sphere
+# ----------------------------------------------------------------------------------------
+def gaussian_sources_3d(n: int = 16, seed: int = 4001):
+    """16 isotropic Gaussians: centres uniform in the ball r < 0.7, widths U(0.05, 0.2),
+    amplitudes U(-5, 5)."""
+    rng = np.random.default_rng(seed)
+    c = []
+    while len(c) < n:
+        q = rng.uniform(-0.7, 0.7, 3)
+        if np.linalg.norm(q) < 0.7:
+            c.append(q)
+    return np.array(c), rng.uniform(0.05, 0.2, n), rng.uniform(-5.0, 5.0, n)
+
+
+def newtonian_gaussians_3d(centers, widths, amps):
+    """N with Laplacian(N) = sum_j A_j exp(-|x - c_j|^2 / (2 w_j^2)) in R^3 (the potential of a
+    Gaussian charge): N_j(r) = -A_j w_j^3 sqrt(pi/2) erf(r / (sqrt(2) w_j)) / r.  Returns a
+    function p -> [N, grad N] (FP64)."""
+    from scipy.special import erf
+    c = np.asarray(centers, np.float64)
+    w = np.asarray(widths, np.float64)
+    a = np.asarray(amps, np.float64)
+
+    def fn(p):
+        p = np.asarray(p, np.float64)
+        d = p[:, None, :] - c[None, :, :]
+        r = np.sqrt((d ** 2).sum(-1))
+        s = np.sqrt(2.0) * w[None, :]
+        K = a[None, :] * w[None, :] ** 3 * math.sqrt(math.pi / 2.0)
+        small = r < 1e-6
+        rs = np.where(small, 1.0, r)
+        er = erf(rs / s)
+        N = np.where(small, -K * 2.0 / (math.sqrt(math.pi) * s), -K * er / rs)
+        dN = -K * (2.0 / math.sqrt(math.pi) * np.exp(-(rs / s) ** 2) / (s * rs) - er / rs ** 2)
+        dN = np.where(small, 0.0, dN)
+        g = (dN / rs)[:, :, None] * d
+        return np.concatenate([N.sum(1)[:, None], g.sum(1)], 1)
+
+    return fn
+
+
+def poisson_exact_3d(centers, widths, amps):
+    """u = N_f + H with H the harmonic exp(0.6x)cos(0.6y) + xz, so Laplacian(u) = f."""
+    nf = newtonian_gaussians_3d(centers, widths, amps)
+    return lambda p: nf(p) + exact_expcos_xz_3d(p)
+
+
+def poisson_gaussian_3d(freq: int = 100, seed: int = 4001, point_sets=None, V=None, F=None) -> Workload:
+    """Interior Dirichlet Poisson problem on the displaced geodesic sphere: f = 16 Gaussians,
+    g = u restricted to the boundary.  Double layer, backward estimator, path length 4 with
+    the volume term (data/config_poisson_equation.json); evaluation point sets of 16,384 x
+    32^(j/7) points, j = 0..7, drawn inside the mesh."""
+    if V is None:
+        V, F = geodesic_sphere(freq)
+        V = displace_radial(V)
+    cen, wid, amp = gaussian_sources_3d(seed=seed)
+    exact = poisson_exact_3d(cen, wid, amp)
+    el = mesh_elements(V, F, DIRICHLET, lambda p, nr, ty: exact(p)[:, 0])
+    rng = np.random.default_rng(seed + 1)
+    sizes = point_sets if point_sets is not None else [int(round(16384 * 32 ** (j / 7))) for j in range(8)]
+    sets = [interior_points_star(n, rng).astype(np.float32) for n in sizes]
+    solver = dict(bie_model="IndirectDoubleLayer", domain="InteriorDomain", estimator="BackwardEstimator",
+                  problem="DirichletProblem", path_length=4, eps=1e-4, num_volume_samples=1, num_sample_paths=256)
+    return Workload("poisson_gaussian_3d", el, np.concatenate(sets), exact, solver,
+                    extra=dict(vertices=V, faces=F, point_sets=sets, centers=cen, widths=wid, amplitudes=amp))
+
+
+# ----------------------------------------------------------------------------------------
 # batched exterior star polygons
 # ----------------------------------------------------------------------------------------
 def star_exterior_batch_2d(n_scenes: int = 64, seed: int = 2001, min_segments: int = 256,
