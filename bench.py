@@ -145,6 +145,48 @@ def batch_leg(args, torch, wob, G):
                       "and fkesc 1/1/2 for Neumann/Robin/mixed scenes"}
 
 
+def poisson_leg(args, torch, wob, G):
+    """Poisson problem (16 Gaussians, interior Dirichlet, double layer, path length 4) on the
+    200k-triangle geodesic sphere: 8 point sets of 16,384 .. 524,288 points as 8 entries of one
+    batched launch per pass (autotuned, hipGraph replays).  Reports paths/s and the RMSE of u
+    against u = N_f + H."""
+    print("[bench] poisson leg: building the 200k-triangle scene", file=sys.stderr, flush=True)
+    w = G.poisson_gaussian_3d()
+    dev = torch.device("cuda:0")
+    scene = wob.SceneHost(w.elements)
+    scene.set_volume_gaussians(w.extra["centers"], w.extra["widths"], w.extra["amplitudes"])
+    est = wob.Estimator.from_config(scene, w.solver)
+    sets = w.extra["point_sets"]
+    pts = [torch.as_tensor(p).to(dev) for p in sets]
+    paths = args.poisson_paths
+    b = wob.Batch([(est, p, paths, j) for j, p in enumerate(pts)], persistent=True, use_graph=True)
+    t0 = time.perf_counter()
+    best, best_ms = b.tune(seed=1)
+    tune_s = time.perf_counter() - t0
+    print(f"[bench] poisson tuned in {tune_s:.1f} s: {best} ({best_ms:.1f} ms/pass)", file=sys.stderr, flush=True)
+    b.run(1, seed=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run(args.poisson_passes, seed=0)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    n_points = sum(len(p) for p in sets)
+    k = b.passes * paths
+    rmse, se_mean = [], []
+    for p, (s, q) in zip(sets, b.outputs):
+        m = (s[:, 0] / k).cpu().numpy()
+        var = np.maximum((q[:, 0] / k).cpu().numpy() - m * m, 0)
+        rmse.append(float(np.sqrt(np.mean((m - w.exact(p)[:, 0]) ** 2))))
+        se_mean.append(float(np.mean(np.sqrt(var / k))))
+    return {"workload": w.name, "elements": w.elements.n, "point_sets": [len(p) for p in sets], "points": n_points,
+            "paths_per_point_per_pass": paths, "passes": args.poisson_passes,
+            "paths_per_s": n_points * paths * args.poisson_passes / el,
+            "ms_per_pass": el / args.poisson_passes * 1e3, "tuned": best, "tune_s": tune_s,
+            "rmse_u_by_set": rmse, "mean_se_u_by_set": se_mean, "paths_per_point_total": k,
+            "solver": "IndirectDoubleLayer, InteriorDomain, BackwardEstimator, DirichletProblem, path_length 4, "
+                      "eps 1e-4, num_volume_samples 1 (Gaussian-mixture source, inside test per sample)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +200,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-paths", type=int, default=16, help="paths per point per batch pass")
     ap.add_argument("--batch-passes", type=int, default=4)
+    ap.add_argument("--no-poisson", action="store_true")
+    ap.add_argument("--poisson-paths", type=int, default=16, help="paths per point per Poisson pass")
+    ap.add_argument("--poisson-passes", type=int, default=4)
     ap.add_argument("--cv-points", type=int, default=1_000_000)
     ap.add_argument("--cv-budget", type=float, default=15.0, help="walk seconds per arm")
     ap.add_argument("--cv-paths-per-pass", type=int, default=4)
@@ -238,6 +283,10 @@ def main():
     if not args.no_batch:
         batch = batch_leg(args, torch, wob, G)
 
+    poisson = None
+    if not args.no_poisson:
+        poisson = poisson_leg(args, torch, wob, G)
+
     ttr = None
     if not args.no_time_to_rmse:
         ttr = time_to_rmse_leg(args, torch, wob, G)
@@ -266,6 +315,7 @@ def main():
         "cpu_baseline": cpu,
         "time_to_rmse": ttr,
         "batch": batch,
+        "poisson": poisson,
         "accuracy": {"rmse_u": rmse, "paths_per_point_total": step * paths,
                      "note": "Neumann problem: u compared after removing the mean offset"
                      if w.solver["problem"] == "NeumannProblem" else "u compared directly"},

@@ -148,8 +148,15 @@ enum { WOB_EST_INDEPENDENT_VOLUME_SAMPLES = 8 };
  * accumulates before it writes its partial sums; block_size: threads per workgroup
  * (64..256); 0 = automatic.  With WOB_LAUNCH_TIME_WALK in flags the call brackets the walk
  * kernel with HIP events on the launch stream, waits for the end event and writes the
- * kernel's duration to walk_ms (and the work-item geometry it chose to the other outputs). */
+ * kernel's duration to walk_ms (and the work-item geometry it chose to the other outputs).
+ * schedule: WOB_SCHEDULE_AUTO picks the wavefront walk for double-layer line walks (where it
+ * is faster) and the per-lane walk otherwise; WOB_SCHEDULE_LANE runs
+ * every lane's queries to their end; WOB_SCHEDULE_WAVEFRONT lets a lane whose query ended run
+ * its walk to the next query while the others traverse (an error where unsupported).  The
+ * results are bit-identical either way.  regen_threshold: lanes that must have finished their
+ * query before a wave advances them (wavefront walk; 0 = default 32).                       */
 enum { WOB_LAUNCH_TIME_WALK = 1 };
+enum { WOB_SCHEDULE_AUTO = 0, WOB_SCHEDULE_LANE = 1, WOB_SCHEDULE_WAVEFRONT = 2 };
 typedef struct wob_launch_config {
     uint32_t paths_per_lane;
     uint32_t block_size;
@@ -157,6 +164,9 @@ typedef struct wob_launch_config {
     float walk_ms;              /* out */
     uint32_t used_paths_per_lane; /* out */
     uint32_t used_work_items;     /* out */
+    uint32_t schedule;
+    uint32_t regen_threshold;
+    uint32_t used_schedule;       /* out */
 } wob_launch_config;
 
 /* Domain estimates: replaces utils::run<DeviceFunctor> with an app functor that calls
@@ -211,6 +221,9 @@ typedef struct wob_batch_config {
     uint32_t block_size;        /* 0 = automatic (128) */
     uint32_t persistent;        /* 1: resident grid, waves pull 64-item blocks from a counter */
     uint32_t use_graph;         /* 1: capture a pass in a hipGraph */
+    uint32_t schedule;          /* WOB_SCHEDULE_* (the wavefront walk is always resident); the
+                                   autotuner tries both where the wavefront walk exists */
+    uint32_t regen_threshold;   /* wavefront walk; 0 = default */
 } wob_batch_config;
 
 wob_status wob_batch_create(const wob_batch_entry *entries, uint32_t n_entries, const wob_batch_config *cfg,

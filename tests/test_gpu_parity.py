@@ -380,3 +380,46 @@ def test_batch_graph_replays_equal_progressive_calls(gpu):
     for (s1, q1), (s2, q2) in zip(ref, b.outputs):          # tuning leaves the outputs alone
         np.testing.assert_allclose(s2.cpu().numpy(), s1.cpu().numpy(), rtol=1e-12, atol=1e-12)
     b.close()
+
+
+@pytest.mark.parametrize("case", ["circle_dl", "circle_dl_exclude", "icosphere_direct_neumann"])
+def test_wavefront_schedule_bit_identical(case, gpu):
+    """The wavefront walk (lanes advance their own walk between queries) against the per-lane
+    walk: same Philox streams and operation order, so the sums must be bit-identical, for
+    single launches and for multi-scene batches, at several regeneration thresholds."""
+    import torch
+    import wobtoolbox_amd as w
+    from wobtoolbox_amd import _capi
+    if case.startswith("circle"):
+        x = G.circle_dirichlet_2d(grid_stride=37)
+        solver = dict(x.solver, flags=1 if case.endswith("exclude") else 0)
+    else:
+        x = G.icosphere_neumann_3d(n_points=3000, level=4)
+        solver = dict(x.solver)
+    scene = w.SceneHost(x.elements)
+    pts = _dev(x.points, gpu)
+    lane = w.Estimator.from_config(scene, solver, schedule=_capi.SCHEDULE_LANE, paths_per_lane=8)
+    s0, q0 = lane.compute_domain(pts, 64, seed=21, path_base=64)
+    assert lane.last_launch.used_schedule == _capi.SCHEDULE_LANE
+    for regen in (1, 32, 64):
+        wf = w.Estimator.from_config(scene, solver, schedule=_capi.SCHEDULE_WAVEFRONT, paths_per_lane=8,
+                                     regen_threshold=regen)
+        s1, q1 = wf.compute_domain(pts, 64, seed=21, path_base=64)
+        assert wf.last_launch.used_schedule == _capi.SCHEDULE_WAVEFRONT
+        torch.cuda.synchronize()
+        assert torch.equal(s0, s1) and torch.equal(q0, q1), f"regen {regen}"
+    # batches of uneven entries on two scenes (the wavefront kernel reads each lane's entry)
+    scene2 = w.SceneHost(x.elements)
+    n = pts.shape[0]
+    sets = [pts[: n // 6], pts[n // 6: n // 2], pts[n // 2:]]
+    ests = [w.Estimator.from_config(sc, solver, paths_per_lane=8) for sc in (scene, scene2, scene)]
+    single = [e.compute_domain(p, 32, seed=4, stream_id=i) for i, (e, p) in enumerate(zip(ests, sets))]
+    for sched in (_capi.SCHEDULE_LANE, _capi.SCHEDULE_WAVEFRONT):
+        b = w.Batch([(e, p, 32, i) for i, (e, p) in enumerate(zip(ests, sets))], paths_per_lane=8,
+                    schedule=sched, use_graph=True)
+        b.run(1, seed=4)
+        torch.cuda.synchronize()
+        for (sa, qa), (sb, qb) in zip(single, b.outputs):
+            assert torch.equal(sa, sb) and torch.equal(qa, qb), f"batch schedule {sched}"
+        b.close()
+    _say(f"{case}: wavefront walk bit-identical to the per-lane walk (single and batched)")

@@ -160,6 +160,8 @@ class Estimator:
     cv_samples: int = 0
     paths_per_lane: int = 0
     block_size: int = 0
+    schedule: int = 0            # _capi.SCHEDULE_AUTO / SCHEDULE_LANE / SCHEDULE_WAVEFRONT
+    regen_threshold: int = 0
 
     @classmethod
     def from_config(cls, scene, cfg: dict, **overrides):
@@ -181,7 +183,8 @@ class Estimator:
 
     def _launch(self):
         lc = _capi.LaunchConfig(int(self.paths_per_lane), int(self.block_size),
-                                _capi.LAUNCH_TIME_WALK if self.time_walk else 0, 0.0, 0, 0)
+                                _capi.LAUNCH_TIME_WALK if self.time_walk else 0, 0.0, 0, 0, int(self.schedule),
+                                int(self.regen_threshold), 0)
         self.last_launch = lc
         return lc
 
@@ -227,7 +230,8 @@ class Estimator:
 class Batch:
     """Many (scene, estimator, point set) entries in one launch per pass (wob_batch_*)."""
 
-    def __init__(self, entries, boundary=False, paths_per_lane=0, block_size=0, persistent=False, use_graph=False):
+    def __init__(self, entries, boundary=False, paths_per_lane=0, block_size=0, persistent=False, use_graph=False,
+                 schedule=0, regen_threshold=0):
         """entries: list of (Estimator, device points, paths_per_point[, stream_id]).
 
         Each entry gets FP64 (sum, sumsq) device buffers, accumulated over passes."""
@@ -246,7 +250,8 @@ class Batch:
             self._keep += [pts, est]
             arr[i] = _capi.BatchEntry(est.scene.handle, est.desc(), pts.data_ptr(), pts.shape[0], paths, 0, sid, 0,
                                       s.data_ptr(), q.data_ptr())
-        cfg = _capi.BatchConfig(int(boundary), paths_per_lane, block_size, int(persistent), int(use_graph))
+        cfg = _capi.BatchConfig(int(boundary), paths_per_lane, block_size, int(persistent), int(use_graph),
+                                int(schedule), int(regen_threshold))
         h = C.c_void_p()
         check(lib().wob_batch_create(arr, len(entries), C.byref(cfg), C.byref(h)))
         self._h = h

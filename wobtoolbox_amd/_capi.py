@@ -64,15 +64,18 @@ class BatchEntry(C.Structure):
 
 class BatchConfig(C.Structure):
     _fields_ = [("boundary", C.c_uint32), ("paths_per_lane", C.c_uint32), ("block_size", C.c_uint32),
-                ("persistent", C.c_uint32), ("use_graph", C.c_uint32)]
+                ("persistent", C.c_uint32), ("use_graph", C.c_uint32), ("schedule", C.c_uint32),
+                ("regen_threshold", C.c_uint32)]
 
 
 class LaunchConfig(C.Structure):
     _fields_ = [("paths_per_lane", C.c_uint32), ("block_size", C.c_uint32), ("flags", C.c_uint32),
-                ("walk_ms", C.c_float), ("used_paths_per_lane", C.c_uint32), ("used_work_items", C.c_uint32)]
+                ("walk_ms", C.c_float), ("used_paths_per_lane", C.c_uint32), ("used_work_items", C.c_uint32),
+                ("schedule", C.c_uint32), ("regen_threshold", C.c_uint32), ("used_schedule", C.c_uint32)]
 
 
 LAUNCH_TIME_WALK = 1
+SCHEDULE_AUTO, SCHEDULE_LANE, SCHEDULE_WAVEFRONT = 0, 1, 2
 
 
 _lib = None

@@ -43,6 +43,7 @@ struct wob_scene {
     void *bufs[8] = {nullptr};
     float *scratch = nullptr;
     size_t scratch_bytes = 0;
+    uint32_t *counter = nullptr;   // work counter of wavefront launches
     uint64_t device_bytes = 0;
     int device = 0;
     float *cv_table = nullptr;
@@ -88,6 +89,8 @@ void free_scene(wob_scene *s) {
         if (b) { (void)hipFree(b); b = nullptr; }
     if (s->scratch) (void)hipFree(s->scratch);
     s->scratch = nullptr;
+    if (s->counter) (void)hipFree(s->counter);
+    s->counter = nullptr;
     if (s->cv_table) (void)hipFree(s->cv_table);
     s->cv_table = nullptr;
 }
@@ -188,6 +191,23 @@ wobl::WalkEntry make_entry(const wob_scene *scene, const wob_estimator_desc *est
     return e;
 }
 
+constexpr uint32_t kDefaultRegen = 32;
+
+// Resolve a requested schedule: true = wavefront walk.  Automatic: the wavefront walk where it
+// measured faster on MI355X, the double-layer line walk (2D circle, 1,024 segments: 501 vs 452
+// M paths/s); the direct Neumann line walk stays per lane (icosphere, 20,480 triangles: 163 vs
+// 200 M paths/s -- its longer walk step between queries costs more in partial waves than the
+// traversal it saves).
+wob_status resolve_schedule(uint32_t schedule, bool supported, int flavour, bool &wavefront) {
+    if (schedule > WOB_SCHEDULE_WAVEFRONT) return fail(WOB_INVALID_ARG, "unknown schedule");
+    if (schedule == WOB_SCHEDULE_WAVEFRONT && !supported)
+        return fail(WOB_UNSUPPORTED_COMBO, "the wavefront walk covers line-sampled backward domain walks without a "
+                                           "volume term only");
+    if (schedule == WOB_SCHEDULE_AUTO) wavefront = supported && flavour == wobdev::FL_LINE_LINE;
+    else wavefront = supported && schedule == WOB_SCHEDULE_WAVEFRONT;
+    return WOB_OK;
+}
+
 wob_status check_block(uint32_t block) {
     if (block < 64 || block > 256 || block % 64 != 0) return fail(WOB_INVALID_ARG, "block_size must be 64, 128, 192 or 256");
     return WOB_OK;
@@ -225,8 +245,26 @@ wob_status run_walk(const wob_scene *cscene, const wob_estimator_desc *est, cons
     L.pass_counter = nullptr;
     L.seed = seed;
     L.partials = scene->scratch;
-
+    const bool volume = est->num_volume_samples > 0;
+    bool wavefront = false;
+    st = resolve_schedule(launch ? launch->schedule : 0u,
+                          wobl::supports_wavefront(D, boundary, volume, est->bie_model, est->estimator, flavour),
+                          flavour, wavefront);
+    if (st != WOB_OK) return st;
+    uint32_t grid = 0;
     hipStream_t hs = (hipStream_t)stream;
+    if (wavefront) {
+        if (!scene->counter) {
+            void *p = nullptr;
+            if (hipMalloc(&p, 256) != hipSuccess) return fail(WOB_OOM, "hipMalloc work counter");
+            scene->counter = (uint32_t *)p;
+        }
+        WOB_HIP(wobl::launch_fill_u32(scene->counter, 1, 0u, hs), "counter reset");
+        L.work_counter = scene->counter;
+        L.wavefront = 1;
+        L.regen_threshold = launch && launch->regen_threshold ? launch->regen_threshold : kDefaultRegen;
+        grid = wobl::persistent_grid(D, boundary, volume, est->bie_model, est->estimator, flavour, block, true, false);
+    }
     const bool timed = launch && (launch->flags & WOB_LAUNCH_TIME_WALK);
     hipEvent_t ev[2] = {nullptr, nullptr};
     if (timed) {
@@ -234,14 +272,14 @@ wob_status run_walk(const wob_scene *cscene, const wob_estimator_desc *est, cons
         WOB_HIP(hipEventCreate(&ev[1]), "hipEventCreate");
         WOB_HIP(hipEventRecord(ev[0], hs), "hipEventRecord");
     }
-    WOB_HIP(wobl::launch_walk(D, boundary, est->num_volume_samples > 0, L, est->bie_model, est->estimator, flavour, block,
-                              0, hs),
+    WOB_HIP(wobl::launch_walk(D, boundary, volume, L, est->bie_model, est->estimator, flavour, block, grid, hs),
             "walk launch");
     if (timed) WOB_HIP(hipEventRecord(ev[1], hs), "hipEventRecord");
     WOB_HIP(wobl::launch_reduce(L, V, hs), "reduce launch");
     if (launch) {
         launch->used_paths_per_lane = L.e0.paths_per_chunk;
         launch->used_work_items = (uint32_t)n_items;
+        launch->used_schedule = wavefront ? WOB_SCHEDULE_WAVEFRONT : WOB_SCHEDULE_LANE;
     }
     if (timed) {
         WOB_HIP(hipEventSynchronize(ev[1]), "hipEventSynchronize");
@@ -344,9 +382,19 @@ wob_status batch_configure(wob_batch *b, bool redirect) {
     L.work_counter = b->counters;
     L.pass_counter = b->counters + 1;
     L.partials = b->partials;
-    b->grid = L.persistent ? wobl::persistent_grid(b->dim, b->cfg.boundary != 0, b->volume, b->bie, b->est, b->flavour,
-                                                   b->cfg.block_size ? b->cfg.block_size : 128u)
-                           : 0u;
+    bool wavefront = false;
+    wob_status st = resolve_schedule(
+        b->cfg.schedule, wobl::supports_wavefront(b->dim, b->cfg.boundary != 0, b->volume, b->bie, b->est, b->flavour),
+        b->flavour, wavefront);
+    if (st != WOB_OK) return st;
+    L.wavefront = wavefront ? 1u : 0u;
+    L.regen_threshold = b->cfg.regen_threshold ? b->cfg.regen_threshold : kDefaultRegen;
+    const uint32_t block = b->cfg.block_size ? b->cfg.block_size : 128u;
+    b->grid = wavefront ? wobl::persistent_grid(b->dim, b->cfg.boundary != 0, b->volume, b->bie, b->est, b->flavour,
+                                                block, true, true)
+              : L.persistent ? wobl::persistent_grid(b->dim, b->cfg.boundary != 0, b->volume, b->bie, b->est, b->flavour,
+                                                     block)
+                             : 0u;
     b->versions.resize(n);
     for (uint32_t i = 0; i < n; ++i) b->versions[i] = b->entries[i].scene->version;
     batch_release_graph(b);
@@ -744,6 +792,17 @@ wob_status wob_batch_tune(wob_batch *b, uint64_t seed, wob_batch_config *best, f
     ppls.push_back(base);
     std::sort(ppls.begin(), ppls.end());
     ppls.erase(std::unique(ppls.begin(), ppls.end()), ppls.end());
+    // launch modes: per-lane walk (one lane per item, or resident waves pulling items) and,
+    // where it exists, the wavefront walk with a few regeneration thresholds
+    struct Mode { uint32_t schedule, persistent, regen; };
+    std::vector<Mode> modes;
+    const bool wf_ok = wobl::supports_wavefront(b->dim, orig.boundary != 0, b->volume, b->bie, b->est, b->flavour);
+    if (orig.schedule != WOB_SCHEDULE_WAVEFRONT) {
+        modes.push_back({WOB_SCHEDULE_LANE, 0u, 0u});
+        modes.push_back({WOB_SCHEDULE_LANE, 1u, 0u});
+    }
+    if (wf_ok && orig.schedule != WOB_SCHEDULE_LANE)
+        for (uint32_t r : {16u, 32u, 48u}) modes.push_back({WOB_SCHEDULE_WAVEFRONT, 1u, r});
     hipEvent_t e0, e1;
     WOB_HIP(hipEventCreate(&e0), "hipEventCreate");
     WOB_HIP(hipEventCreate(&e1), "hipEventCreate");
@@ -753,11 +812,13 @@ wob_status wob_batch_tune(wob_batch *b, uint64_t seed, wob_batch_config *best, f
         if (ppl > std::max(1u, min_paths) && ppl != base) continue;
         if ((double)total / ppl < 65536.0 && ppl != base) continue;   // too few items to fill the GPU
         for (uint32_t block : {64u, 128u, 256u}) {
-            for (uint32_t pers : {0u, 1u}) {
+            for (const Mode &md : modes) {
                 b->cfg = orig;
                 b->cfg.paths_per_lane = ppl;
                 b->cfg.block_size = block;
-                b->cfg.persistent = pers;
+                b->cfg.persistent = md.persistent;
+                b->cfg.schedule = md.schedule;
+                b->cfg.regen_threshold = md.regen;
                 b->cfg.use_graph = 0;
                 wob_status st = batch_configure(b, true);
                 if (st != WOB_OK) { b->cfg = orig; (void)batch_configure(b, false); return st; }

@@ -356,30 +356,34 @@ template <int D> struct LineHit {
 };
 
 // line o + t d against primitive `prim` (leaf order).  Reference predicate (scene.cuh:431-432,
-// 725-727): parameters in the closed element, |t| > 10 FLT_EPSILON, no NaN.
+// 725-727): parameters in the closed element, |t| > 10 FLT_EPSILON, no NaN.  The fused
+// multiply-adds are written out: left to the compiler, a*b + c*d may be fused either way
+// depending on the code the test is inlined into, and the hit coordinates would round
+// differently in different kernels.
 template <int D> __device__ __forceinline__ bool prim_test(const DevScene &S, uint32_t prim, const float *o,
                                                            const float *d, float &t, float &a, float &b) {
-#pragma clang fp contract(fast)
     constexpr float eps = 10 * kFltEps;
     if constexpr (D == 2) {
         const float4 g = S.prims[prim];
         const float sx = g.x - o[0], sy = g.y - o[1];
-        const float det = d[0] * g.w - d[1] * g.z;
+        const float det = __builtin_fmaf(d[0], g.w, -(d[1] * g.z));
         const float inv = __builtin_amdgcn_rcpf(det);
-        t = (sx * g.w - sy * g.z) * inv;
-        a = (d[0] * sy - d[1] * sx) * inv;
+        t = __builtin_fmaf(sx, g.w, -(sy * g.z)) * inv;
+        a = __builtin_fmaf(d[0], sy, -(d[1] * sx)) * inv;
         b = 0.f;
         return !(a < 0.0f || a > 1.0f || fabsf(t) <= eps || __builtin_isnan(t) || __builtin_isnan(a));
     } else {
         const float4 g0 = S.prims[3 * prim], g1 = S.prims[3 * prim + 1], g2 = S.prims[3 * prim + 2];
         const float s0 = g0.x - o[0], s1 = g0.y - o[1], s2 = g0.z - o[2];
         // q = d x s
-        const float q0 = d[1] * s2 - d[2] * s1, q1 = d[2] * s0 - d[0] * s2, q2 = d[0] * s1 - d[1] * s0;
-        const float det = d[0] * g2.y + d[1] * g2.z + d[2] * g2.w;       // d . (e1 x e2)
-        const float inv = __builtin_amdgcn_rcpf(det);                     // hardware reciprocal (1 ulp)
-        t = (s0 * g2.y + s1 * g2.z + s2 * g2.w) * inv;                    // s . N / det
-        a = (q0 * g1.z + q1 * g1.w + q2 * g2.x) * inv;                    // (d x s) . e2 / det
-        b = -(q0 * g0.w + q1 * g1.x + q2 * g1.y) * inv;                   // -(d x s) . e1 / det
+        const float q0 = __builtin_fmaf(d[1], s2, -(d[2] * s1));
+        const float q1 = __builtin_fmaf(d[2], s0, -(d[0] * s2));
+        const float q2 = __builtin_fmaf(d[0], s1, -(d[1] * s0));
+        const float det = __builtin_fmaf(d[0], g2.y, __builtin_fmaf(d[1], g2.z, d[2] * g2.w));   // d . (e1 x e2)
+        const float inv = __builtin_amdgcn_rcpf(det);                                            // hardware reciprocal (1 ulp)
+        t = __builtin_fmaf(s0, g2.y, __builtin_fmaf(s1, g2.z, s2 * g2.w)) * inv;                // s . N / det
+        a = __builtin_fmaf(q0, g1.z, __builtin_fmaf(q1, g1.w, q2 * g2.x)) * inv;                // (d x s) . e2 / det
+        b = -(__builtin_fmaf(q0, g0.w, __builtin_fmaf(q1, g1.x, q2 * g1.y))) * inv;            // -(d x s) . e1 / det
         return !(a < 0.0f || b < 0.0f || a + b > 1.0f || fabsf(t) <= eps || __builtin_isnan(t) || __builtin_isnan(a) ||
                  __builtin_isnan(b));
     }
@@ -403,7 +407,6 @@ template <int D> __device__ __forceinline__ bool prim_test(const DevScene &S, ui
 template <int D, int STACK, typename OnHit>
 __device__ __forceinline__ void traverse(const DevScene &S, const float *o, const float *d, uint32_t *stack,
                                          uint32_t stride, OnHit &&on_hit) {
-#pragma clang fp contract(fast)
     constexpr int kEmpty = (int)0x80000000;
     float inv[D], oi[D];
 #pragma unroll
@@ -485,12 +488,8 @@ __device__ __forceinline__ LineHit<D> line_query(const DevScene &S, const float 
     return h;
 }
 
-// scene.cuh:464-535 (2D), 760-840 (3D): a uniformly oriented line through x, all hits, one
-// hit taken uniformly, inv_pdf = N pi |r| / |rhat.n| (2D) or N 2pi |r|^2 / |rhat.n| (3D).
-template <int D, int STACK>
-__device__ __forceinline__ Sample<D> line_sample(const DevScene &S, const float *x, uint32_t skip, Rng &rng,
-                                                 uint32_t *stack, uint32_t stride) {
-    float d[D];
+// scene.cuh:471-478 (2D), 769-782 (3D): a uniformly oriented line direction
+template <int D> __device__ __forceinline__ void line_dir(Rng &rng, float *d) {
     if constexpr (D == 2) {
         const float theta = rng.uniform();
         float s, c;
@@ -504,8 +503,13 @@ __device__ __forceinline__ Sample<D> line_sample(const DevScene &S, const float 
         const float r = sqrtf(1.f - d[2] * d[2]);
         d[0] = c * r; d[1] = s * r;
     }
-    const uint32_t pick = rng.next_u32();
-    const LineHit<D> h = line_query<D, STACK>(S, x, d, pick, skip, stack, stride);
+}
+
+// scene.cuh:488-535 (2D), 791-840 (3D): the sample at the picked hit of the line x + t d,
+// inv_pdf = N pi |r| / |rhat.n| (2D) or N 2pi |r|^2 / |rhat.n| (3D); no hit gives a zero
+// point with inv_pdf 0.
+template <int D>
+__device__ __forceinline__ Sample<D> line_finish(const DevScene &S, const float *x, const float *d, const LineHit<D> &h) {
     Sample<D> r;
     if (h.count == 0) {                                                                 // scene.cuh:488-495
         r.y = bp_zero<D>();
@@ -540,6 +544,18 @@ __device__ __forceinline__ Sample<D> line_sample(const DevScene &S, const float 
     if constexpr (D == 2) r.inv_pdf = (float)h.count * kPi * nr / cosv;
     else r.inv_pdf = (float)h.count * 2.f * kPi * z / cosv;
     return r;
+}
+
+// scene.cuh:464-535 (2D), 760-840 (3D): a uniformly oriented line through x, all hits, one
+// hit taken uniformly.
+template <int D, int STACK>
+__device__ __forceinline__ Sample<D> line_sample(const DevScene &S, const float *x, uint32_t skip, Rng &rng,
+                                                 uint32_t *stack, uint32_t stride) {
+    float d[D];
+    line_dir<D>(rng, d);
+    const uint32_t pick = rng.next_u32();
+    const LineHit<D> h = line_query<D, STACK>(S, x, d, pick, skip, stack, stride);
+    return line_finish<D>(S, x, d, h);
 }
 
 // ----------------------------------------------------------------------------------------

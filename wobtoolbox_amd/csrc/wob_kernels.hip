@@ -203,10 +203,19 @@ hipError_t launch_walk(int dim, bool boundary, bool volume, const WalkLaunch &a,
                   : launch_walk_30(boundary, a, bie, est, fl, block, grid, st);
 }
 
-uint32_t persistent_grid(int dim, bool boundary, bool volume, int bie, int est, int fl, uint32_t block) {
+uint32_t persistent_grid(int dim, bool boundary, bool volume, int bie, int est, int fl, uint32_t block, bool wavefront,
+                         bool multi) {
     if (dim == 2)
-        return volume ? persistent_grid_21(boundary, bie, est, fl, block) : persistent_grid_20(boundary, bie, est, fl, block);
-    return volume ? persistent_grid_31(boundary, bie, est, fl, block) : persistent_grid_30(boundary, bie, est, fl, block);
+        return volume ? persistent_grid_21(boundary, bie, est, fl, block, wavefront, multi)
+                      : persistent_grid_20(boundary, bie, est, fl, block, wavefront, multi);
+    return volume ? persistent_grid_31(boundary, bie, est, fl, block, wavefront, multi)
+                  : persistent_grid_30(boundary, bie, est, fl, block, wavefront, multi);
+}
+
+bool supports_wavefront(int dim, bool boundary, bool volume, int bie, int est, int fl) {
+    if (dim == 2)
+        return volume ? supports_wavefront_21(boundary, bie, est, fl) : supports_wavefront_20(boundary, bie, est, fl);
+    return volume ? supports_wavefront_31(boundary, bie, est, fl) : supports_wavefront_30(boundary, bie, est, fl);
 }
 
 hipError_t launch_reduce(const WalkLaunch &a, uint32_t V, hipStream_t st) {

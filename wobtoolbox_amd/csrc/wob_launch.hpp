@@ -49,6 +49,8 @@ struct WalkLaunch {
     const uint32_t *pass_counter;  // device pass index (graph replays), may be null
     uint64_t seed;
     float *partials;
+    uint32_t wavefront;            // 1: wavefront walk (walk_kernel_wf); needs work_counter at 0
+    uint32_t regen_threshold;      // wavefront walk: lanes that must wait before a wave advances them
 };
 
 hipError_t launch_walk(int dim, bool boundary, bool volume, const WalkLaunch &a, int bie, int est, int flavour,
@@ -71,7 +73,9 @@ hipError_t launch_philox(uint32_t n, const uint32_t *ctr, const uint32_t *key, u
 #define WOB_DECLARE_WALK(d, v)                                                                                  \
     hipError_t launch_walk_##d##v(bool boundary, const WalkLaunch &a, int bie, int est, int flavour, uint32_t block, \
                                   uint32_t grid, hipStream_t st);                                               \
-    uint32_t persistent_grid_##d##v(bool boundary, int bie, int est, int flavour, uint32_t block);
+    uint32_t persistent_grid_##d##v(bool boundary, int bie, int est, int flavour, uint32_t block, bool wavefront,  \
+                                    bool multi);                                                               \
+    bool supports_wavefront_##d##v(bool boundary, int bie, int est, int flavour);
 WOB_DECLARE_WALK(2, 0)
 WOB_DECLARE_WALK(2, 1)
 WOB_DECLARE_WALK(3, 0)
@@ -79,7 +83,9 @@ WOB_DECLARE_WALK(3, 1)
 #undef WOB_DECLARE_WALK
 
 // Occupancy-derived grid for persistent launches (waves that fit at once on the device).
-uint32_t persistent_grid(int dim, bool boundary, bool volume, int bie, int est, int flavour, uint32_t block);
+uint32_t persistent_grid(int dim, bool boundary, bool volume, int bie, int est, int flavour, uint32_t block,
+                         bool wavefront = false, bool multi = false);
+bool supports_wavefront(int dim, bool boundary, bool volume, int bie, int est, int flavour);
 // LDS bytes of a walk launch (per-lane traversal stacks)
 size_t walk_lds_bytes(int flavour, bool volume, uint32_t block);
 

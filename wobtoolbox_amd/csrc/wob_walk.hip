@@ -154,6 +154,242 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkLaunch L) {
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Wavefront walk for the line-sampled backward domain walks (double layer / Dirichlet and
+// direct / Neumann).  The walk above runs each lane's query to its end before the lane's walk
+// continues, so every wave waits, query after query, for its slowest traversal.  Here a lane
+// is a small state machine: each iteration the wave either runs one traversal step (a node
+// or a primitive step, chosen by ballot as in traverse) for the lanes with a query in flight,
+// or -- once at least `regen` lanes have finished their query, or none is traversing --
+// advances those lanes: consume the hit, run the walk up to its next query (or finish the
+// path, start the next path of the item, or take a new item from the work counter).  Every
+// path keeps its own Philox stream and the operation order of walk_item, so the per-item
+// sums are bit-identical to it.
+// ---------------------------------------------------------------------------------------
+enum { kLaneIdle = 0, kLaneTrav = 1, kLaneAdv = 2 };
+
+template <int D, int BIE, int FL, bool MULTI>
+__global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
+    constexpr int STACK = wobl::kStackEntries;
+    constexpr int V = D + 1;
+    constexpr int kEmpty = (int)0x80000000;
+    constexpr int NQ = D == 2 ? 6 : 8;   // float4 per node
+    extern __shared__ uint32_t lds_stack[];
+    uint32_t *stack = lds_stack + threadIdx.x;
+    const uint32_t stride = blockDim.x;
+    const uint32_t pass = L.pass_counter ? *L.pass_counter : 0u;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t regen = L.regen_threshold;
+
+    // item
+    uint32_t ent = 0, local = 0, pt = 0, j = 0, j1 = 0, pbase = 0;
+    float x[D];
+    Kahan ks[V], kq[V];
+    // path
+    uint32_t phase = 0, it = 0;
+    float W = 0.f, spc = 0.f, known = 0.f, unk = 0.f, gk[D], gu[D];
+    BP<D> yp = bp_zero<D>();
+    Rng rng;
+    rng.init(0, 0, 0, 0);
+    // query
+    float o[D], d[D], inv[D], oi[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) { x[k] = o[k] = d[k] = inv[k] = oi[k] = gk[k] = gu[k] = 0.f; }
+    uint32_t pick = 0, skipv = 0xFFFFFFFFu;
+    int cur = kEmpty, sp = 0;
+    uint32_t lp = 0, le = 0;
+    uint32_t hcount = 0, hbest = 0xFFFFFFFFu, hid = 0xFFFFFFFFu;
+    float ht = 0.f, ha = 0.f, hb = 0.f;
+    const float4 *nodes = L.e0.scene.nodes, *prims = L.e0.scene.prims;
+    const uint32_t *pid = L.e0.scene.prim_id;
+    uint32_t state = kLaneAdv;
+    bool need_item = true;
+
+    auto entry = [&]() -> const WalkEntry & { return MULTI ? L.table[ent] : L.e0; };
+    auto model = [&](const WalkEntry &e) {
+        Model<D, BIE> m;
+        m.k.eps = e.eps;
+        m.dsign = e.dsign;
+        m.c = e.fkesc;
+        return m;
+    };
+    auto issue_query = [&](const float *origin, uint32_t skip) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) o[k] = origin[k];
+        line_dir<D>(rng, d);
+        pick = rng.next_u32();
+        skipv = skip;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const float dk = fabsf(d[k]) < 1e-30f ? copysignf(1e-30f, d[k]) : d[k];   // as in traverse
+            inv[k] = 1.0f / dk;
+            oi[k] = o[k] * inv[k];
+        }
+        cur = 0; sp = 0; lp = 0; le = 0;
+        hcount = 0; hbest = 0xFFFFFFFFu; hid = 0xFFFFFFFFu; ht = 0.f; ha = 0.f; hb = 0.f;
+        state = kLaneTrav;
+    };
+    auto start_bounce = [&](const WalkEntry &e) {
+        const Model<D, BIE> m = model(e);
+        if (it == e.path_length - 1) W = 0.5f * W;
+        if constexpr (FL == FL_LINE_BV) {
+            const Sample<D> qb = sample_bv<D>(e.scene, rng);
+            spc += W * m.fredholm_known(yp, qb.y, qb.inv_pdf, NoVol{}, rng);
+        }
+        issue_query(yp.p, (e.flags & 1u) ? yp.elem : 0xFFFFFFFFu);
+    };
+    auto start_path = [&](const WalkEntry &e) {
+        rng.init(L.seed, pbase + j, e.point_base + pt, e.stream);
+        phase = 0;
+        issue_query(x, 0xFFFFFFFFu);
+    };
+    auto advance = [&]() {
+        if (!need_item) {
+            const WalkEntry &e = entry();
+            const Model<D, BIE> m = model(e);
+            LineHit<D> h;
+            h.count = hcount; h.id = hid; h.t = ht; h.a = ha; h.b = hb;
+            const Sample<D> q = line_finish<D>(e.scene, o, d, h);
+            bool done = false;
+            if (phase == 0) {                                              // domain_path, first samples
+                Sample<D> y1 = q;
+                if constexpr (FL == FL_LINE_BV) y1 = sample_bv<D>(e.scene, rng);
+                known = m.solution_domain_known(x, y1.y, y1.inv_pdf, NoVol{}, rng);
+                m.gradient_domain_known(x, y1.y, y1.inv_pdf, NoVol{}, rng, gk);
+                unk = m.solution_domain_unknown(x, q.y, q.inv_pdf);
+                m.gradient_domain_unknown(x, q.y, q.inv_pdf, gu);
+                W = 1.f; spc = 0.f; yp = q.y; it = 0; phase = 1;
+            } else {                                                       // walk_backward, one bounce
+                Sample<D> qq = q;
+                if constexpr (FL == FL_LINE_LINE) {
+                    spc += W * m.fredholm_known(yp, qq.y, qq.inv_pdf, NoVol{}, rng);
+                    W *= m.fredholm_unknown_backward(yp, qq.y, qq.inv_pdf, rng);
+                } else {
+                    W = W * m.fredholm_unknown_backward(yp, qq.y, qq.inv_pdf, rng);
+                }
+                yp = qq.y;
+                it += 1;
+                done = W == 0.0f || it == e.path_length;
+            }
+            if (!done) { start_bounce(e); return; }
+            float out[V];
+            out[0] = known + spc * unk;
+#pragma unroll
+            for (int k = 0; k < D; ++k) out[1 + k] = gk[k] + spc * gu[k];
+#pragma unroll
+            for (int k = 0; k < V; ++k) { ks[k].add(out[k]); kq[k].add(out[k] * out[k]); }
+            j += 1;
+            if (j < j1) { start_path(e); return; }
+            const size_t n_items = (size_t)e.n_points * e.chunks;
+            float *part = L.partials + e.partial_base;
+#pragma unroll
+            for (int k = 0; k < V; ++k) {
+                part[(size_t)k * n_items + local] = ks[k].s;
+                part[(size_t)(V + k) * n_items + local] = kq[k].s;
+            }
+            need_item = true;
+        }
+        // take the next item: one counter update for all lanes of the wave that need one
+        const uint64_t want = __ballot(1);
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)want) - 1u;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(L.work_counter, (uint32_t)__popcll(want));
+        base = __shfl(base, (int)leader, 64);
+        const uint32_t item = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+        if (item >= L.total_items) { state = kLaneIdle; return; }
+        if constexpr (MULTI) {
+            uint32_t lo = 0, hi = L.n_entries - 1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (L.table[mid].item_begin <= item) lo = mid; else hi = mid - 1;
+            }
+            ent = lo;
+        }
+        const WalkEntry &e = entry();
+        local = item - e.item_begin;
+        const uint32_t chunk = local / e.n_points;
+        pt = local - chunk * e.n_points;
+        j = chunk * e.paths_per_chunk;
+        j1 = min(e.paths_per_point, j + e.paths_per_chunk);
+        pbase = (uint32_t)(e.path_base + (uint64_t)pass * e.paths_per_point);
+#pragma unroll
+        for (int k = 0; k < D; ++k) x[k] = e.points[(size_t)pt * D + k];
+#pragma unroll
+        for (int k = 0; k < V; ++k) { ks[k].init(); kq[k].init(); }
+        nodes = e.scene.nodes;
+        prims = e.scene.prims;
+        pid = e.scene.prim_id;
+        need_item = false;
+        start_path(e);
+    };
+
+    while (true) {
+        if (state == kLaneTrav) {
+            if (lp >= le && cur < 0 && cur != kEmpty) {          // leaf entry -> pending primitives
+                const uint32_t v = ~(uint32_t)cur;
+                lp = v >> 4;
+                le = lp + (v & 15u) + 1u;
+                cur = sp > 0 ? (int)stack[(uint32_t)(--sp) * stride] : kEmpty;
+            }
+            if (lp >= le && cur == kEmpty) state = kLaneAdv;      // query finished
+        }
+        const uint64_t badv = __ballot(state == kLaneAdv);
+        const uint64_t btrav = __ballot(state == kLaneTrav);
+        if ((badv | btrav) == 0) break;
+        if (btrav == 0 || (uint32_t)__popcll(badv) >= regen) {
+            if (state == kLaneAdv) advance();
+            continue;
+        }
+        const bool want_leaf = state == kLaneTrav && lp < le;
+        const bool want_node = state == kLaneTrav && !want_leaf;
+        const uint64_t bl = __ballot(want_leaf);
+        const uint64_t bn = __ballot(want_node);
+        if (__popcll(bl) >= __popcll(bn)) {
+            if (want_leaf) {
+                DevScene Sq{};
+                Sq.prims = prims;
+                float t, a, b;
+                if (prim_test<D>(Sq, lp, o, d, t, a, b)) {
+                    const uint32_t id = pid[lp];
+                    if (id != skipv) {
+                        const uint32_t key = triple32(pick ^ (id * 0x9E3779B9u));
+                        hcount += 1;
+                        if (key < hbest || (key == hbest && id < hid)) { hbest = key; hid = id; ht = t; ha = a; hb = b; }
+                    }
+                }
+                ++lp;
+            }
+        } else if (want_node) {
+            const float4 *np = nodes + (size_t)cur * NQ;
+            float4 lo[D], hi[D];
+#pragma unroll
+            for (int k = 0; k < D; ++k) { lo[k] = np[k]; hi[k] = np[D + k]; }
+            const float4 chf = np[2 * D];
+            const uint32_t valid = __float_as_uint(np[2 * D + 1].x);
+            const int ch[4] = {(int)__float_as_uint(chf.x), (int)__float_as_uint(chf.y), (int)__float_as_uint(chf.z),
+                               (int)__float_as_uint(chf.w)};
+            int next = kEmpty;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                float tmin = -INFINITY, tmax = INFINITY;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const float l = c == 0 ? lo[k].x : c == 1 ? lo[k].y : c == 2 ? lo[k].z : lo[k].w;
+                    const float u = c == 0 ? hi[k].x : c == 1 ? hi[k].y : c == 2 ? hi[k].z : hi[k].w;
+                    const float t1 = __builtin_fmaf(l, inv[k], -oi[k]), t2 = __builtin_fmaf(u, inv[k], -oi[k]);
+                    tmin = fmaxf(tmin, fminf(t1, t2));
+                    tmax = fminf(tmax, fmaxf(t1, t2));
+                }
+                if (((valid >> c) & 1u) && tmin <= tmax) {
+                    if (next == kEmpty) next = ch[c];
+                    else if (sp < STACK) { stack[(uint32_t)sp * stride] = (uint32_t)ch[c]; ++sp; }
+                }
+            }
+            cur = next != kEmpty ? next : (sp > 0 ? (int)stack[(uint32_t)(--sp) * stride] : kEmpty);
+        }
+    }
+}
+
 }  // namespace wobk
 
 namespace wobl {
@@ -166,17 +402,37 @@ template <int D, int BIE, int EST, int FL, bool BOUNDARY, bool VOL> struct WalkK
     static constexpr int STACK = lines ? kStackEntries : 0;
     static size_t lds(uint32_t block) { return lines ? (size_t)STACK * block * sizeof(uint32_t) : 0; }
     static const void *fn() { return (const void *)&walk_kernel<D, BIE, EST, FL, STACK, BOUNDARY, VOL>; }
+    // the wavefront walk covers the line-sampled backward domain walks without a volume term
+    static constexpr bool wf = EST == kBackward && !BOUNDARY && !VOL && (FL == FL_LINE_LINE || FL == FL_LINE_BV);
+    static const void *wf_fn(bool multi) {
+        if constexpr (wf) {
+            return multi ? (const void *)&walk_kernel_wf<D, BIE, FL, true> : (const void *)&walk_kernel_wf<D, BIE, FL, false>;
+        } else {
+            return nullptr;
+        }
+    }
     static hipError_t launch(const WalkLaunch &a, uint32_t block, uint32_t grid, hipStream_t st) {
+        if constexpr (wf) {
+            if (a.wavefront) {
+                if (a.table)
+                    hipLaunchKernelGGL((walk_kernel_wf<D, BIE, FL, true>), dim3(grid), dim3(block), lds(block), st, a);
+                else
+                    hipLaunchKernelGGL((walk_kernel_wf<D, BIE, FL, false>), dim3(grid), dim3(block), lds(block), st, a);
+                return hipGetLastError();
+            }
+        }
         if (grid == 0) grid = (a.total_items + block - 1) / block;
         hipLaunchKernelGGL((walk_kernel<D, BIE, EST, FL, STACK, BOUNDARY, VOL>), dim3(grid), dim3(block), lds(block), st,
                            a);
         return hipGetLastError();
     }
-    static uint32_t resident_grid(uint32_t block) {
+    static bool supports_wavefront() { return wf; }
+    static uint32_t resident_grid(uint32_t block, bool wavefront = false, bool multi = false) {
         int per_cu = 0, dev = 0, cus = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn(), (int)block, lds(block)) != hipSuccess)
+        const void *f = wavefront && wf ? wf_fn(multi) : fn();
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, (int)block, lds(block)) != hipSuccess)
             per_cu = 1;
         return (uint32_t)std::max(1, per_cu) * (uint32_t)std::max(1, cus);
     }
@@ -209,8 +465,14 @@ hipError_t WOB_NAME(launch_walk_, WOB_WALK_DIM, WOB_WALK_VOL)(bool boundary, con
 }
 
 uint32_t WOB_NAME(persistent_grid_, WOB_WALK_DIM, WOB_WALK_VOL)(bool boundary, int bie, int est, int fl,
-                                                                  uint32_t block) {
-    auto go = [&](auto k) { return decltype(k)::resident_grid(block); };
+                                                                  uint32_t block, bool wavefront, bool multi) {
+    auto go = [&](auto k) { return decltype(k)::resident_grid(block, wavefront, multi); };
+    return boundary ? dispatch<WOB_WALK_DIM, true, WOB_WALK_VOL != 0>(bie, est, fl, go)
+                    : dispatch<WOB_WALK_DIM, false, WOB_WALK_VOL != 0>(bie, est, fl, go);
+}
+
+bool WOB_NAME(supports_wavefront_, WOB_WALK_DIM, WOB_WALK_VOL)(bool boundary, int bie, int est, int fl) {
+    auto go = [&](auto k) { return decltype(k)::supports_wavefront(); };
     return boundary ? dispatch<WOB_WALK_DIM, true, WOB_WALK_VOL != 0>(bie, est, fl, go)
                     : dispatch<WOB_WALK_DIM, false, WOB_WALK_VOL != 0>(bie, est, fl, go);
 }
