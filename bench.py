@@ -57,6 +57,19 @@ def cpu_baseline(workload, cores, target_s=10.0):
                       f"runtime_utils.cuh:89-103, OpenMP static schedule over points"}
 
 
+def mlp_train_roofline(steps, batch, n_outputs, seconds):
+    """BF16 MFMA training throughput: forward 2 x (4 x 128 x 128 + 128 x n_out) FLOP per sample
+    (64 Fourier frequencies -> 128 features -> 4 ReLU layers of 128 -> outputs), a training step
+    about 3 x forward (forward, activation and weight gradients)."""
+    from wobtoolbox_amd.roofline import MI355X_PEAKS
+    fwd = 2 * (4 * 128 * 128 + 128 * 16)
+    flop = 3 * fwd * batch * steps
+    tf = flop / seconds / 1e12
+    return {"bound": "mfma", "achieved": tf, "peak": MI355X_PEAKS["bf16_TFLOPs"], "unit": "TFLOP/s",
+            "frac": tf / MI355X_PEAKS["bf16_TFLOPs"], "seconds": seconds, "steps": steps, "batch": batch,
+            "flop_per_sample_step": 3 * fwd}
+
+
 def time_to_rmse_leg(args, torch, wob, G):
     """Plain WoB vs learned control variate on the 200k-triangle geodesic mixed workload.
 
@@ -78,12 +91,16 @@ def time_to_rmse_leg(args, torch, wob, G):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     pos, idx, tgt = CV.training_data(plain, args.cv_train_points, seed=2, paths_per_sample=args.cv_train_paths)
-    net = CV.ControlVariate(3, L, sigma=1.0, seed=3)
+    net = CV.ControlVariate(3, L, sigma=args.cv_sigma, seed=3)
+    torch.cuda.synchronize()
+    t_train0 = time.perf_counter()
     net.train(pos, idx, tgt, batch=16384, steps=args.cv_train_steps, lr=2e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t_train0
     net.bind(scene, L)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
-    cvest = wob.Estimator.from_config(scene, w.solver, flags=2)
+    cvest = wob.Estimator.from_config(scene, w.solver, flags=2, cv_samples=args.cv_samples)
     curve_cv = progressive.run(cvest, pts, exact_u, args.cv_paths_per_pass, t_setup + args.cv_budget, t0=t0, seed=1)
     target = 1e-3
     tp, reached_p = progressive.time_to_rmse(curve_plain, target)
@@ -96,6 +113,10 @@ def time_to_rmse_leg(args, torch, wob, G):
                                                  "first_kind_equation_scaling_constant")},
             "plain_s": tp, "plain_reached": reached_p, "cv_s": tc, "cv_reached": reached_c, "speedup": tp / tc,
             "cv_setup_s": t_setup, "cv_training_walks": args.cv_train_points * L * args.cv_train_paths,
+            "cv_config": {"train_points_per_value": args.cv_train_points, "paths_per_target": args.cv_train_paths,
+                          "train_steps": args.cv_train_steps, "batch": 16384, "sigma": args.cv_sigma,
+                          "cv_samples": args.cv_samples},
+            "mlp_train": mlp_train_roofline(args.cv_train_steps, 16384, L, t_train),
             "efficiency_ratio": eff_p / eff_c,
             "note": "not reached: extrapolated as setup + rmse^2 x walk time / target^2 (both arms estimate the "
                     "same expectation; truncation bias of the path_length-4 series is not resolved at these "
@@ -206,9 +227,11 @@ def main():
     ap.add_argument("--cv-points", type=int, default=1_000_000)
     ap.add_argument("--cv-budget", type=float, default=15.0, help="walk seconds per arm")
     ap.add_argument("--cv-paths-per-pass", type=int, default=4)
-    ap.add_argument("--cv-train-points", type=int, default=262144, help="boundary points per value function")
-    ap.add_argument("--cv-train-paths", type=int, default=4, help="walks averaged per training point")
+    ap.add_argument("--cv-train-points", type=int, default=1048576, help="boundary points per value function")
+    ap.add_argument("--cv-train-paths", type=int, default=16, help="walks averaged per training point")
     ap.add_argument("--cv-train-steps", type=int, default=1500)
+    ap.add_argument("--cv-sigma", type=float, default=0.5, help="scale of the Gaussian Fourier frequencies")
+    ap.add_argument("--cv-samples", type=int, default=16, help="samples of the domain-level control-variate integral")
     ap.add_argument("--profile-copy-mib", type=int, default=0,
                     help="after the timed steps, copy this many MiB once (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -220,7 +243,7 @@ def main():
     import torch
     import wobtoolbox_amd as wob
     from wobtoolbox_amd import geometry as G
-    from wobtoolbox_amd.roofline import MI355X_PEAKS, workload_model
+    from wobtoolbox_amd.roofline import MEASURED_TRAFFIC, MI355X_PEAKS, workload_model
 
     print(f"[bench] building workload {args.workload}", file=sys.stderr, flush=True)
     w = G.icosphere_neumann_3d() if args.workload == "icosphere_neumann_3d" else G.circle_dirichlet_2d()
@@ -307,7 +330,7 @@ def main():
                    "estimator": w.solver["estimator"], "paths_per_lane": est.last_launch.used_paths_per_lane,
                    "work_items": est.last_launch.used_work_items},
         "roofline": {"bound": "hbm", "achieved": gbps, "peak": MI355X_PEAKS["hbm_GBps"], "unit": "GB/s",
-                     "frac": gbps / MI355X_PEAKS["hbm_GBps"], "traffic": None,
+                     "frac": gbps / MI355X_PEAKS["hbm_GBps"], "traffic": MEASURED_TRAFFIC.get(w.name),
                      "kernel": "walk_kernel", "avg_kernel_ms": avg_walk_s * 1e3,
                      "algorithmic_bytes_per_path": byts, "algorithmic_flop_per_path": flop,
                      "fp32_valu": {"achieved": tflops, "peak": MI355X_PEAKS["fp32_TFLOPs"], "unit": "TFLOP/s",
