@@ -141,8 +141,8 @@ template <typename S, unsigned Dim, bool IsVectorProblem = false> class SceneHos
         if (scene_) wob_scene_destroy(scene_);
     }
 
-    // scene.cuh:301-306.  The grid volume source is not supported by this engine
-    // (wob_scene_set_volume_grid reports WOB_UNSUPPORTED_COMBO), so this throws.
+    // scene.cuh:301-306: grid source over (-domain_size/2, domain_size/2)^Dim, grid_res^Dim
+    // host values (x fastest); estimators with num_volume_samples > 0 sample it.
     void set_volume_cache(unsigned grid_res, float domain_size, const std::vector<float> &cache) {
         check(wob_scene_set_volume_grid(scene_, grid_res, domain_size, cache.data()));
     }
