@@ -355,6 +355,23 @@ template <int D> struct LineHit {
     float t, a, b;    // line parameter and element parameters (2D: a; 3D: barycentrics of b, c)
 };
 
+// Running uniform pick over a query's hits: count them and keep the hit with the smallest
+// triple32(pick ^ id * 0x9E3779B9) (ties: smaller id), which is uniform over the hits for a
+// uniform pick and independent of the traversal order.  Selects only, no lane branches.
+template <int D>
+__device__ __forceinline__ void hit_update(bool hit, uint32_t id, uint32_t skip, uint32_t pick, float t, float a,
+                                           float b, LineHit<D> &h, uint32_t &best) {
+    const bool ok = hit && id != skip;
+    const uint32_t key = triple32(pick ^ (id * 0x9E3779B9u));
+    const bool take = ok && (key < best || (key == best && id < h.id));
+    h.count += ok ? 1u : 0u;
+    best = take ? key : best;
+    h.id = take ? id : h.id;
+    h.t = take ? t : h.t;
+    h.a = take ? a : h.a;
+    h.b = take ? b : h.b;
+}
+
 // line o + t d against primitive `prim` (leaf order).  Reference predicate (scene.cuh:431-432,
 // 725-727): parameters in the closed element, |t| > 10 FLT_EPSILON, no NaN.  The fused
 // multiply-adds are written out: left to the compiler, a*b + c*d may be fused either way
@@ -394,76 +411,112 @@ template <int D> __device__ __forceinline__ bool prim_test(const DevScene &S, ui
 // therefore always meets every box on the element's path to the root, and the plain
 // comparison tmin <= tmax decides.
 
+// Traversal state of one lane's query: the line's slab parameters, the current entry
+// (internal node >= 0, leaf < 0, or kTravEmpty), the stack depth as an element offset
+// (depth * stride) and the range of pending primitives [lp, le).
+constexpr int kTravEmpty = (int)0x80000000;
+template <int D> struct Trav {
+    float inv[D], oi[D];
+    int cur;
+    uint32_t spo;
+    uint32_t lp, le;
+
+    __device__ __forceinline__ void begin(const float *o, const float *d) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const float dk = fabsf(d[k]) < 1e-30f ? copysignf(1e-30f, d[k]) : d[k];   // finite slab parameters
+            inv[k] = 1.0f / dk;
+            oi[k] = o[k] * inv[k];
+        }
+        cur = 0; spo = 0; lp = 0; le = 0;
+    }
+    // a leaf entry becomes the pending primitive range and the next entry is popped
+    __device__ __forceinline__ void settle(const uint32_t *stack, uint32_t stride) {
+        const bool leaf = lp >= le && cur < 0 && cur != kTravEmpty;
+        const uint32_t v = ~(uint32_t)cur;
+        const bool pop = leaf && spo != 0u;
+        const int top = (int)stack[spo != 0u ? spo - stride : 0u];
+        lp = leaf ? v >> 4 : lp;
+        le = leaf ? (v >> 4) + (v & 15u) + 1u : le;
+        cur = leaf ? (pop ? top : kTravEmpty) : cur;
+        spo = pop ? spo - stride : spo;
+    }
+    __device__ __forceinline__ bool want_prim() const { return lp < le; }
+    __device__ __forceinline__ bool want_node() const { return lp >= le && cur >= 0; }
+    __device__ __forceinline__ bool done() const { return lp >= le && cur == kTravEmpty; }
+
+    // four slab tests on the current node; the first hit child becomes the current entry, the
+    // others are pushed.  Every child is written to the slot above the top (a spare slot
+    // backs a full stack), only pushes move the top, so no lane branches.
+    template <int STACK>
+    __device__ __forceinline__ void node_step(const float4 *nodes, uint32_t *stack, uint32_t stride) {
+        constexpr int NQ = D == 2 ? 6 : 8;   // float4 per node
+        const float4 *np = nodes + (size_t)cur * NQ;
+        float4 lo[D], hi[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) { lo[k] = np[k]; hi[k] = np[D + k]; }
+        const float4 chf = np[2 * D];
+        const uint32_t valid = __float_as_uint(np[2 * D + 1].x);
+        const int ch[4] = {(int)__float_as_uint(chf.x), (int)__float_as_uint(chf.y), (int)__float_as_uint(chf.z),
+                           (int)__float_as_uint(chf.w)};
+        int next = kTravEmpty;
+        const uint32_t full = (uint32_t)STACK * stride;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float tmin = 0.f, tmax = 0.f;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const float l = c == 0 ? lo[k].x : c == 1 ? lo[k].y : c == 2 ? lo[k].z : lo[k].w;
+                const float u = c == 0 ? hi[k].x : c == 1 ? hi[k].y : c == 2 ? hi[k].z : hi[k].w;
+                const float t1 = __builtin_fmaf(l, inv[k], -oi[k]), t2 = __builtin_fmaf(u, inv[k], -oi[k]);
+                tmin = k == 0 ? fminf(t1, t2) : fmaxf(tmin, fminf(t1, t2));
+                tmax = k == 0 ? fmaxf(t1, t2) : fminf(tmax, fmaxf(t1, t2));
+            }
+            const bool h = ((valid >> c) & 1u) && tmin <= tmax;
+            const bool take = h && next == kTravEmpty;
+            const bool push = h && !take && spo < full;
+            stack[spo] = (uint32_t)ch[c];
+            spo = push ? spo + stride : spo;
+            next = take ? ch[c] : next;
+        }
+        const bool pop = next == kTravEmpty && spo != 0u;
+        const int top = (int)stack[spo != 0u ? spo - stride : 0u];
+        spo = pop ? spo - stride : spo;
+        cur = next != kTravEmpty ? next : (pop ? top : kTravEmpty);
+    }
+};
+
 // Depth-first traversal of the 4-wide BVH for the infinite line o + t d, calling
-// on_hit(prim, t, a, b) for every primitive the reference predicate accepts.
+// on_hit(hit, prim, t, a, b) for every primitive tested, hit = the reference predicate.
 //
 // Wave-scheduled: every lane holds one current entry (internal node or leaf) and a range of
 // pending primitives; hit children beyond the first are pushed on the lane's LDS stack
-// (`stack`, entries `stride` words apart; the builder bounds the depth), leaves included.
-// Each iteration the wave runs ONE kind of step, chosen by ballot: a primitive test for the
-// lanes with pending primitives, or a node step (four slab tests) for the lanes that are
-// ready to descend -- whichever more lanes want.  Lanes therefore test primitives together
-// instead of one at a time inside divergent per-child branches.
+// (`stack`, entries `stride` words apart, STACK entries plus a spare; the builder bounds the
+// depth), leaves included.  Each iteration the wave runs ONE kind of step, chosen by ballot:
+// a primitive test for the lanes with pending primitives, or a node step (four slab tests)
+// for the lanes that are ready to descend -- whichever more lanes want.  Lanes therefore test
+// primitives together instead of one at a time inside divergent per-child branches.
 template <int D, int STACK, typename OnHit>
 __device__ __forceinline__ void traverse(const DevScene &S, const float *o, const float *d, uint32_t *stack,
                                          uint32_t stride, OnHit &&on_hit) {
-    constexpr int kEmpty = (int)0x80000000;
-    float inv[D], oi[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-        const float dk = fabsf(d[k]) < 1e-30f ? copysignf(1e-30f, d[k]) : d[k];   // finite slab parameters
-        inv[k] = 1.0f / dk;
-        oi[k] = o[k] * inv[k];
-    }
-    int cur = 0;
-    int sp = 0;
-    uint32_t lp = 0, le = 0;
-    constexpr int NQ = D == 2 ? 6 : 8;   // float4 per node
+    Trav<D> tr;
+    tr.begin(o, d);
     while (true) {
-        if (lp >= le && cur < 0 && cur != kEmpty) {          // leaf entry -> pending primitives
-            const uint32_t v = ~(uint32_t)cur;
-            lp = v >> 4;
-            le = lp + (v & 15u) + 1u;
-            cur = sp > 0 ? (int)stack[(uint32_t)(--sp) * stride] : kEmpty;
-        }
-        const bool want_leaf = lp < le;
-        const bool want_node = !want_leaf && cur >= 0;
+        tr.settle(stack, stride);
+        const bool want_leaf = tr.want_prim();
+        const bool want_node = tr.want_node();
         const uint64_t bl = __ballot(want_leaf);
         const uint64_t bn = __ballot(want_node);
         if ((bl | bn) == 0) break;
         if (__popcll(bl) >= __popcll(bn)) {
             if (want_leaf) {
                 float t, a, b;
-                if (prim_test<D>(S, lp, o, d, t, a, b)) on_hit(lp, t, a, b);
-                ++lp;
+                const bool hit = prim_test<D>(S, tr.lp, o, d, t, a, b);
+                on_hit(hit, tr.lp, t, a, b);
+                ++tr.lp;
             }
         } else if (want_node) {
-            const float4 *np = S.nodes + (size_t)cur * NQ;
-            float4 lo[D], hi[D];
-#pragma unroll
-            for (int k = 0; k < D; ++k) { lo[k] = np[k]; hi[k] = np[D + k]; }
-            const float4 chf = np[2 * D];
-            const uint32_t valid = __float_as_uint(np[2 * D + 1].x);
-            const int ch[4] = {(int)__float_as_uint(chf.x), (int)__float_as_uint(chf.y), (int)__float_as_uint(chf.z),
-                               (int)__float_as_uint(chf.w)};
-            int next = kEmpty;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                float tmin = -INFINITY, tmax = INFINITY;
-#pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    const float l = c == 0 ? lo[k].x : c == 1 ? lo[k].y : c == 2 ? lo[k].z : lo[k].w;
-                    const float u = c == 0 ? hi[k].x : c == 1 ? hi[k].y : c == 2 ? hi[k].z : hi[k].w;
-                    const float t1 = __builtin_fmaf(l, inv[k], -oi[k]), t2 = __builtin_fmaf(u, inv[k], -oi[k]);
-                    tmin = fmaxf(tmin, fminf(t1, t2));
-                    tmax = fminf(tmax, fmaxf(t1, t2));
-                }
-                if (((valid >> c) & 1u) && tmin <= tmax) {
-                    if (next == kEmpty) next = ch[c];
-                    else if (sp < STACK) { stack[(uint32_t)sp * stride] = (uint32_t)ch[c]; ++sp; }
-                }
-            }
-            cur = next != kEmpty ? next : (sp > 0 ? (int)stack[(uint32_t)(--sp) * stride] : kEmpty);
+            tr.template node_step<STACK>(S.nodes, stack, stride);
         }
     }
 }
@@ -476,14 +529,8 @@ __device__ __forceinline__ LineHit<D> line_query(const DevScene &S, const float 
     LineHit<D> h;
     h.count = 0; h.id = 0xFFFFFFFFu; h.t = 0.f; h.a = 0.f; h.b = 0.f;
     uint32_t best = 0xFFFFFFFFu;
-    traverse<D, STACK>(S, o, d, stack, stride, [&](uint32_t prim, float t, float a, float b) {
-        const uint32_t id = S.prim_id[prim];
-        if (id == skip) return;
-        const uint32_t key = triple32(pick ^ (id * 0x9E3779B9u));
-        h.count += 1;
-        if (key < best || (key == best && id < h.id)) {
-            best = key; h.id = id; h.t = t; h.a = a; h.b = b;
-        }
+    traverse<D, STACK>(S, o, d, stack, stride, [&](bool hit, uint32_t prim, float t, float a, float b) {
+        hit_update<D>(hit, S.prim_id[prim], skip, pick, t, a, b, h, best);
     });
     return h;
 }
@@ -581,8 +628,8 @@ __device__ __noinline__ uint32_t ray_crossings(const float4 *nodes, const float4
     if constexpr (D == 2) { d[0] = 0.58778525f; d[1] = 0.80901699f; }
     else { d[0] = 0.48218104f; d[1] = 0.66748405f; d[2] = 0.56749094f; }
     uint32_t c = 0;
-    traverse<D, STACK>(S, o, d, stack, stride, [&](uint32_t, float t, float, float) {
-        if (t > 0.f) ++c;
+    traverse<D, STACK>(S, o, d, stack, stride, [&](bool hit, uint32_t, float t, float, float) {
+        c += (hit && t > 0.f) ? 1u : 0u;
     });
     return c;
 }

@@ -146,7 +146,8 @@ __global__ __launch_bounds__(256) void intersect_kernel(DevScene S, uint32_t n, 
 #pragma unroll
     for (int k = 0; k < D; ++k) { o[k] = origins[(size_t)i * D + k]; d[k] = dirs[(size_t)i * D + k]; }
     uint32_t c = 0;
-    traverse<D, STACK>(S, o, d, lds_stack + threadIdx.x, blockDim.x, [&](uint32_t prim, float t, float, float) {
+    traverse<D, STACK>(S, o, d, lds_stack + threadIdx.x, blockDim.x, [&](bool hit, uint32_t prim, float t, float, float) {
+        if (!hit) return;
         if (c < max_hits) {
             ids[(size_t)i * max_hits + c] = S.prim_id[prim];
             ts[(size_t)i * max_hits + c] = t;
@@ -191,7 +192,7 @@ using namespace wobk;
 
 size_t walk_lds_bytes(int fl, bool volume, uint32_t block) {
     const bool lines = fl == FL_LINE_LINE || fl == FL_LINE_BV || fl == FL_FWD_LINE || volume;
-    return lines ? (size_t)kStackEntries * block * sizeof(uint32_t) : 0;
+    return lines ? (size_t)(kStackEntries + 1) * block * sizeof(uint32_t) : 0;
 }
 
 hipError_t launch_walk(int dim, bool boundary, bool volume, const WalkLaunch &a, int bie, int est, int fl,
@@ -260,7 +261,7 @@ hipError_t launch_eval_weights(int dim, int bie, float dsign, float eps, float f
 hipError_t launch_intersect(int dim, const DevScene &S, uint32_t n, const float *o, const float *d, uint32_t max_hits,
                             uint32_t *counts, uint32_t *ids, float *ts, hipStream_t st) {
     const uint32_t block = 128, grid = (n + block - 1) / block;
-    const size_t lds = (size_t)kStackEntries * block * sizeof(uint32_t);
+    const size_t lds = (size_t)(kStackEntries + 1) * block * sizeof(uint32_t);
     if (dim == 2)
         hipLaunchKernelGGL((intersect_kernel<2, kStackEntries>), dim3(grid), dim3(block), lds, st, S, n, o, d, max_hits, counts, ids, ts);
     else

@@ -172,8 +172,6 @@ template <int D, int BIE, int FL, bool MULTI>
 __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
     constexpr int STACK = wobl::kStackEntries;
     constexpr int V = D + 1;
-    constexpr int kEmpty = (int)0x80000000;
-    constexpr int NQ = D == 2 ? 6 : 8;   // float4 per node
     extern __shared__ uint32_t lds_stack[];
     uint32_t *stack = lds_stack + threadIdx.x;
     const uint32_t stride = blockDim.x;
@@ -192,14 +190,16 @@ __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
     Rng rng;
     rng.init(0, 0, 0, 0);
     // query
-    float o[D], d[D], inv[D], oi[D];
+    float o[D], d[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) { x[k] = o[k] = d[k] = inv[k] = oi[k] = gk[k] = gu[k] = 0.f; }
+    for (int k = 0; k < D; ++k) { x[k] = gk[k] = gu[k] = 0.f; o[k] = 0.f; d[k] = 1.f; }
     uint32_t pick = 0, skipv = 0xFFFFFFFFu;
-    int cur = kEmpty, sp = 0;
-    uint32_t lp = 0, le = 0;
-    uint32_t hcount = 0, hbest = 0xFFFFFFFFu, hid = 0xFFFFFFFFu;
-    float ht = 0.f, ha = 0.f, hb = 0.f;
+    Trav<D> tr;
+    tr.begin(o, d);
+    tr.cur = kTravEmpty;
+    LineHit<D> h;
+    h.count = 0; h.id = 0xFFFFFFFFu; h.t = 0.f; h.a = 0.f; h.b = 0.f;
+    uint32_t hbest = 0xFFFFFFFFu;
     const float4 *nodes = L.e0.scene.nodes, *prims = L.e0.scene.prims;
     const uint32_t *pid = L.e0.scene.prim_id;
     uint32_t state = kLaneAdv;
@@ -219,14 +219,9 @@ __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
         line_dir<D>(rng, d);
         pick = rng.next_u32();
         skipv = skip;
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            const float dk = fabsf(d[k]) < 1e-30f ? copysignf(1e-30f, d[k]) : d[k];   // as in traverse
-            inv[k] = 1.0f / dk;
-            oi[k] = o[k] * inv[k];
-        }
-        cur = 0; sp = 0; lp = 0; le = 0;
-        hcount = 0; hbest = 0xFFFFFFFFu; hid = 0xFFFFFFFFu; ht = 0.f; ha = 0.f; hb = 0.f;
+        tr.begin(o, d);
+        h.count = 0; h.id = 0xFFFFFFFFu; h.t = 0.f; h.a = 0.f; h.b = 0.f;
+        hbest = 0xFFFFFFFFu;
         state = kLaneTrav;
     };
     auto start_bounce = [&](const WalkEntry &e) {
@@ -247,8 +242,6 @@ __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
         if (!need_item) {
             const WalkEntry &e = entry();
             const Model<D, BIE> m = model(e);
-            LineHit<D> h;
-            h.count = hcount; h.id = hid; h.t = ht; h.a = ha; h.b = hb;
             const Sample<D> q = line_finish<D>(e.scene, o, d, h);
             bool done = false;
             if (phase == 0) {                                              // domain_path, first samples
@@ -325,13 +318,8 @@ __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
 
     while (true) {
         if (state == kLaneTrav) {
-            if (lp >= le && cur < 0 && cur != kEmpty) {          // leaf entry -> pending primitives
-                const uint32_t v = ~(uint32_t)cur;
-                lp = v >> 4;
-                le = lp + (v & 15u) + 1u;
-                cur = sp > 0 ? (int)stack[(uint32_t)(--sp) * stride] : kEmpty;
-            }
-            if (lp >= le && cur == kEmpty) state = kLaneAdv;      // query finished
+            tr.settle(stack, stride);
+            if (tr.done()) state = kLaneAdv;                    // query finished
         }
         const uint64_t badv = __ballot(state == kLaneAdv);
         const uint64_t btrav = __ballot(state == kLaneTrav);
@@ -340,8 +328,8 @@ __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
             if (state == kLaneAdv) advance();
             continue;
         }
-        const bool want_leaf = state == kLaneTrav && lp < le;
-        const bool want_node = state == kLaneTrav && !want_leaf;
+        const bool want_leaf = state == kLaneTrav && tr.want_prim();
+        const bool want_node = state == kLaneTrav && tr.want_node();
         const uint64_t bl = __ballot(want_leaf);
         const uint64_t bn = __ballot(want_node);
         if (__popcll(bl) >= __popcll(bn)) {
@@ -349,43 +337,12 @@ __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
                 DevScene Sq{};
                 Sq.prims = prims;
                 float t, a, b;
-                if (prim_test<D>(Sq, lp, o, d, t, a, b)) {
-                    const uint32_t id = pid[lp];
-                    if (id != skipv) {
-                        const uint32_t key = triple32(pick ^ (id * 0x9E3779B9u));
-                        hcount += 1;
-                        if (key < hbest || (key == hbest && id < hid)) { hbest = key; hid = id; ht = t; ha = a; hb = b; }
-                    }
-                }
-                ++lp;
+                const bool hit = prim_test<D>(Sq, tr.lp, o, d, t, a, b);
+                hit_update<D>(hit, pid[tr.lp], skipv, pick, t, a, b, h, hbest);
+                ++tr.lp;
             }
         } else if (want_node) {
-            const float4 *np = nodes + (size_t)cur * NQ;
-            float4 lo[D], hi[D];
-#pragma unroll
-            for (int k = 0; k < D; ++k) { lo[k] = np[k]; hi[k] = np[D + k]; }
-            const float4 chf = np[2 * D];
-            const uint32_t valid = __float_as_uint(np[2 * D + 1].x);
-            const int ch[4] = {(int)__float_as_uint(chf.x), (int)__float_as_uint(chf.y), (int)__float_as_uint(chf.z),
-                               (int)__float_as_uint(chf.w)};
-            int next = kEmpty;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                float tmin = -INFINITY, tmax = INFINITY;
-#pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    const float l = c == 0 ? lo[k].x : c == 1 ? lo[k].y : c == 2 ? lo[k].z : lo[k].w;
-                    const float u = c == 0 ? hi[k].x : c == 1 ? hi[k].y : c == 2 ? hi[k].z : hi[k].w;
-                    const float t1 = __builtin_fmaf(l, inv[k], -oi[k]), t2 = __builtin_fmaf(u, inv[k], -oi[k]);
-                    tmin = fmaxf(tmin, fminf(t1, t2));
-                    tmax = fminf(tmax, fmaxf(t1, t2));
-                }
-                if (((valid >> c) & 1u) && tmin <= tmax) {
-                    if (next == kEmpty) next = ch[c];
-                    else if (sp < STACK) { stack[(uint32_t)sp * stride] = (uint32_t)ch[c]; ++sp; }
-                }
-            }
-            cur = next != kEmpty ? next : (sp > 0 ? (int)stack[(uint32_t)(--sp) * stride] : kEmpty);
+            tr.template node_step<STACK>(nodes, stack, stride);
         }
     }
 }
@@ -400,7 +357,8 @@ template <int D, int BIE, int EST, int FL, bool BOUNDARY, bool VOL> struct WalkK
     // line flavours traverse the BVH; volume kernels also run the inside test of mixture samples
     static constexpr bool lines = FL == FL_LINE_LINE || FL == FL_LINE_BV || FL == FL_FWD_LINE || VOL;
     static constexpr int STACK = lines ? kStackEntries : 0;
-    static size_t lds(uint32_t block) { return lines ? (size_t)STACK * block * sizeof(uint32_t) : 0; }
+    // per-lane stack of STACK entries plus the spare slot the node step writes above a full stack
+    static size_t lds(uint32_t block) { return lines ? (size_t)(STACK + 1) * block * sizeof(uint32_t) : 0; }
     static const void *fn() { return (const void *)&walk_kernel<D, BIE, EST, FL, STACK, BOUNDARY, VOL>; }
     // the wavefront walk covers the line-sampled backward domain walks without a volume term
     static constexpr bool wf = EST == kBackward && !BOUNDARY && !VOL && (FL == FL_LINE_LINE || FL == FL_LINE_BV);
