@@ -1,0 +1,125 @@
+"""The reference apps' runner (wobtoolbox_amd/apps/wob_general.cpp): configuration keys, mesh
+input, grid masks, analytic fields and output formats of apps/general_2d.cu and
+apps/general_3d.cu.  The dry runs need no GPU; the short progressive runs do."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from wobtoolbox_amd import build as B
+from wobtoolbox_amd import geometry as G
+
+
+def _write_circle_obj(path, r=0.45, n=128):
+    th = -2 * np.pi * np.arange(n) / n          # clockwise, as the apps' 2D meshes
+    with open(path, "w") as f:
+        for t in th:
+            f.write(f"v {r * np.cos(t):.9f} {r * np.sin(t):.9f} 0.0\n")
+        for i in range(n):
+            f.write(f"f {i + 1} {(i + 1) % n + 1}\n")
+
+
+def _write_sphere_obj(path, level=2):
+    V, F = G.icosphere(level)
+    V = G.displace_radial(V)
+    with open(path, "w") as f:
+        for v in V:
+            f.write(f"v {v[0]:.9f} {v[1]:.9f} {v[2]:.9f}\n")
+        for t in F:
+            f.write(f"f {t[0] + 1} {t[1] + 1} {t[2] + 1}\n")
+
+
+def _config(tmp_path, obj, **kw):
+    cfg = {"input_obj_file": str(obj), "output_dir": str(tmp_path / "out"), "path_length": 8,
+           "num_sample_paths": 256, "grid_res": 32, "eps": 1e-5, "max_time": 1, "equation_kernel": "PoissonKernel",
+           "bie_model": "IndirectDoubleLayer", "domain": "InteriorDomain", "estimator": "BackwardEstimator",
+           "problem": "DirichletProblem"}
+    cfg.update(kw)
+    p = tmp_path / "config.json"
+    p.write_text(json.dumps(cfg, indent=4))
+    return p
+
+
+def _read_field(path, comps=1):
+    raw = open(path, "rb").read()
+    res = int(np.frombuffer(raw[:4], np.uint32)[0])
+    return res, np.frombuffer(raw[4:], np.float32).reshape(res, res, comps) if comps > 1 else \
+        np.frombuffer(raw[4:], np.float32).reshape(res, res)
+
+
+def _run(app, cfg, *extra, timeout=300):
+    if not os.path.exists(app):
+        pytest.fail(f"{app} is not built (run the package build)")
+    r = subprocess.run([app, str(cfg), *extra], capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def test_app_2d_dry_run_fields(tmp_path):
+    obj = tmp_path / "circle.obj"
+    _write_circle_obj(obj)
+    cfg = _config(tmp_path, obj)
+    out = _run(B.APPS[2], cfg, "--dry-run")
+    assert "IndirectDoubleLayer" in out and "DirichletProblem" in out
+    od = tmp_path / "out"
+    assert json.loads((od / "config.json").read_text())["grid_res"] == 32
+    res, u = _read_field(od / "potential_analytical.scalar")
+    _, g = _read_field(od / "gradient_analytical.vector2", 2)
+    assert res == 32
+    x = (np.arange(32) + 0.5) / 32 - 0.5          # utils.cuh:92-99, domain size 1
+    X, Y = np.meshgrid(x, x)                      # row = y, column = x
+    inside = X ** 2 + Y ** 2 < 0.45 ** 2 * np.cos(np.pi / 128) ** 2 - 1e-6
+    outside = X ** 2 + Y ** 2 > 0.45 ** 2 + 1e-6
+    assert np.all(np.isfinite(u[inside])) and np.all(np.isnan(u[outside]))
+    exact = np.cos(np.pi * X) * np.sinh(np.pi * Y)            # general_2d.cu:86-91
+    np.testing.assert_allclose(u[inside], exact[inside], rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(g[inside][:, 1], (np.pi * np.cos(np.pi * X) * np.cosh(np.pi * Y))[inside], rtol=1e-5)
+
+
+def test_app_3d_dry_run_slice_mask(tmp_path):
+    obj = tmp_path / "sphere.obj"
+    _write_sphere_obj(obj)
+    cfg = _config(tmp_path, obj, bie_model="Direct", problem="NeumannProblem", path_length=6)
+    _run(B.APPS[3], cfg, "--dry-run")
+    res, u = _read_field(tmp_path / "out" / "potential_analytical.scalar")
+    # the mesh is refit to [-1/2, 1/2]^3; the slice x = 0 cuts a disc of radius about 0.4
+    frac = np.isfinite(u).mean()
+    assert 0.3 < frac < 0.8
+    x = (np.arange(res) + 0.5) / res - 0.5
+    Z, Yg = np.meshgrid(x, x)                     # column = z, row = y (general_3d.cu:296-297)
+    m = np.isfinite(u)
+    np.testing.assert_allclose(u[m], (-np.cos(np.pi * Yg) * np.sinh(np.pi * Z))[m], rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_app_2d_progressive_run(tmp_path, gpu):
+    obj = tmp_path / "circle.obj"
+    _write_circle_obj(obj)
+    cfg = _config(tmp_path, obj)
+    _run(B.APPS[2], cfg, "--max-iterations", "3")
+    od = tmp_path / "out"
+    rows = np.loadtxt(od / "rmse_list.csv", delimiter=",", ndmin=2)
+    assert rows.shape == (3, 5)
+    assert list(rows[:, 0]) == [256, 512, 768]
+    print(f"2D app RMSE per pass: {rows[:, 2]}", flush=True)
+    # unbiased, noise-limited: RMSE falls like 1 / sqrt(paths) (sqrt(1/3) = 0.58 after three passes)
+    assert 0.4 < rows[-1, 2] / rows[0, 2] < 0.8 and rows[-1, 2] < 0.4
+    _, u3 = _read_field(od / "potential3.scalar")
+    _, ua = _read_field(od / "potential_analytical.scalar")
+    assert np.array_equal(np.isnan(u3), np.isnan(ua))
+
+
+@pytest.mark.gpu
+def test_app_3d_progressive_run(tmp_path, gpu):
+    obj = tmp_path / "sphere.obj"
+    _write_sphere_obj(obj)
+    cfg = _config(tmp_path, obj, bie_model="Direct", problem="NeumannProblem", path_length=6, num_sample_paths=128)
+    _run(B.APPS[3], cfg, "--max-iterations", "2")
+    rows = np.loadtxt(tmp_path / "out" / "rmse_list.csv", delimiter=",", ndmin=2)
+    assert rows.shape == (2, 6)
+    print(f"3D app RMSE per pass: {rows[:, 2]}", flush=True)
+    assert np.all(np.isfinite(rows))
+    _, g = _read_field(tmp_path / "out" / "gradient2.vector3", 3)
+    assert g.shape == (32, 32, 3)

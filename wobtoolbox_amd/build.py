@@ -76,6 +76,7 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError("libwob_hip link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
     _build_shim_check()
+    _build_apps()
     return LIB
 
 
@@ -93,6 +94,25 @@ def _build_shim_check():
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("shim_check build failed:\n" + r.stdout + r.stderr)
+
+
+APPS = {2: os.path.join(LIBDIR, "wob_general_2d"), 3: os.path.join(LIBDIR, "wob_general_3d")}
+
+
+def _build_apps():
+    """The reference apps' runner (apps/wob_general.cpp) for 2D and 3D."""
+    src = os.path.join(PKG, "apps", "wob_general.cpp")
+    procs = []
+    for dim, out in APPS.items():
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", f"-DWOB_APP_DIM={dim}",
+               "-I" + os.path.join(ROCM, "include"), src, "-o", out, "-L" + LIBDIR, "-lwob_hip",
+               "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+               "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+        procs.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT))
+    for p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError("app build failed:\n" + out.decode(errors="replace"))
 
 
 if __name__ == "__main__":
