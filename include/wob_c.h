@@ -156,6 +156,9 @@ enum { WOB_EST_INDEPENDENT_VOLUME_SAMPLES = 8 };
  * results are bit-identical either way.  regen_threshold: lanes that must have finished their
  * query before a wave advances them (wavefront walk; 0 = default 32).                       */
 enum { WOB_LAUNCH_TIME_WALK = 1 };
+/* WOB_LAUNCH_ONE_LANE_PER_ITEM: launch one lane per work item instead of the default resident
+ * grid whose waves take work items from a counter (same results).                          */
+enum { WOB_LAUNCH_ONE_LANE_PER_ITEM = 2 };
 enum { WOB_SCHEDULE_AUTO = 0, WOB_SCHEDULE_LANE = 1, WOB_SCHEDULE_WAVEFRONT = 2 };
 typedef struct wob_launch_config {
     uint32_t paths_per_lane;

@@ -162,6 +162,7 @@ class Estimator:
     block_size: int = 0
     schedule: int = 0            # _capi.SCHEDULE_AUTO / SCHEDULE_LANE / SCHEDULE_WAVEFRONT
     regen_threshold: int = 0
+    launch_flags: int = 0        # _capi.LAUNCH_ONE_LANE_PER_ITEM
 
     @classmethod
     def from_config(cls, scene, cfg: dict, **overrides):
@@ -183,8 +184,8 @@ class Estimator:
 
     def _launch(self):
         lc = _capi.LaunchConfig(int(self.paths_per_lane), int(self.block_size),
-                                _capi.LAUNCH_TIME_WALK if self.time_walk else 0, 0.0, 0, 0, int(self.schedule),
-                                int(self.regen_threshold), 0)
+                                (_capi.LAUNCH_TIME_WALK if self.time_walk else 0) | int(self.launch_flags), 0.0, 0, 0,
+                                int(self.schedule), int(self.regen_threshold), 0)
         self.last_launch = lc
         return lc
 

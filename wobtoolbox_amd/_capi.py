@@ -75,6 +75,7 @@ class LaunchConfig(C.Structure):
 
 
 LAUNCH_TIME_WALK = 1
+LAUNCH_ONE_LANE_PER_ITEM = 2
 SCHEDULE_AUTO, SCHEDULE_LANE, SCHEDULE_WAVEFRONT = 0, 1, 2
 
 

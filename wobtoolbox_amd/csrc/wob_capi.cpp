@@ -253,7 +253,10 @@ wob_status run_walk(const wob_scene *cscene, const wob_estimator_desc *est, cons
     if (st != WOB_OK) return st;
     uint32_t grid = 0;
     hipStream_t hs = (hipStream_t)stream;
-    if (wavefront) {
+    // resident waves pulling 64-item blocks from a counter (the per-lane walk) or single items
+    // (the wavefront walk): no partly filled last round of waves
+    const bool resident = wavefront || !(launch && (launch->flags & WOB_LAUNCH_ONE_LANE_PER_ITEM));
+    if (resident) {
         if (!scene->counter) {
             void *p = nullptr;
             if (hipMalloc(&p, 256) != hipSuccess) return fail(WOB_OOM, "hipMalloc work counter");
@@ -261,9 +264,11 @@ wob_status run_walk(const wob_scene *cscene, const wob_estimator_desc *est, cons
         }
         WOB_HIP(wobl::launch_fill_u32(scene->counter, 1, 0u, hs), "counter reset");
         L.work_counter = scene->counter;
-        L.wavefront = 1;
+        L.persistent = 1;
+        L.wavefront = wavefront ? 1u : 0u;
         L.regen_threshold = launch && launch->regen_threshold ? launch->regen_threshold : kDefaultRegen;
-        grid = wobl::persistent_grid(D, boundary, volume, est->bie_model, est->estimator, flavour, block, true, false);
+        grid = wobl::persistent_grid(D, boundary, volume, est->bie_model, est->estimator, flavour, block, wavefront,
+                                     false);
     }
     const bool timed = launch && (launch->flags & WOB_LAUNCH_TIME_WALK);
     hipEvent_t ev[2] = {nullptr, nullptr};
