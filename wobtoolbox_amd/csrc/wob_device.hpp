@@ -319,6 +319,65 @@ template <int D> __device__ __forceinline__ Sample<D> sample_uniform(const DevSc
     return r;
 }
 
+// The geometric part of a uniform boundary sample (element, vertex weights, point, normal):
+// the same draws and arithmetic as sample_uniform, reading only the element's vertices and
+// normal (2 of 3 attribute rows in 2D, 3 of 5 in 3D).  complete_uniform adds the boundary
+// value, Robin coefficient and type, so complete_uniform(S, uniform_geometry(S, rng)) ==
+// sample_uniform(S, rng).y bit for bit.
+template <int D> struct GeomSample {
+    uint32_t elem;
+    float bw[D];
+    float p[D], n[D];
+};
+
+template <int D> __device__ __forceinline__ GeomSample<D> uniform_geometry(const DevScene &S, Rng &rng) {
+    GeomSample<D> g;
+    const float u1 = rng.uniform();
+    const uint32_t e = cdf_lookup(S.area_cdf, S.area_guide, S.n, u1);
+    g.elem = e;
+    if constexpr (D == 2) {
+        const float4 f0 = S.attr[3 * e], f1 = S.attr[3 * e + 1];
+        const float v[4] = {f0.x, f0.y, f0.z, f0.w};
+        g.n[0] = f1.x; g.n[1] = f1.y;
+        const float u2 = rng.uniform();
+#pragma unroll
+        for (int k = 0; k < 2; ++k) g.p[k] = v[k] + u2 * (v[2 + k] - v[k]);            // as uniform_in_element
+        g.bw[0] = 1.0f - u2; g.bw[1] = u2;
+    } else {
+        const float4 f0 = S.attr[5 * e], f1 = S.attr[5 * e + 1], f2 = S.attr[5 * e + 2];
+        const float v[9] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w, f2.x};
+        g.n[0] = f2.y; g.n[1] = f2.z; g.n[2] = f2.w;
+        const float u2 = rng.uniform();
+        const float u3 = rng.uniform();
+        const float sq = sqrtf(u2);
+        const float w0 = 1.0f - sq, w1 = sq * (1.0f - u3), w2 = u3 * sq;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) g.p[k] = w0 * v[k] + w1 * v[3 + k] + w2 * v[6 + k];
+        g.bw[0] = w0; g.bw[1] = w1; g.bw[2] = w2;
+    }
+    return g;
+}
+
+template <int D> __device__ __forceinline__ BP<D> complete_uniform(const DevScene &S, const GeomSample<D> &g) {
+    BP<D> b;
+    b.elem = g.elem;
+#pragma unroll
+    for (int k = 0; k < D; ++k) { b.p[k] = g.p[k]; b.n[k] = g.n[k]; b.bw[k] = g.bw[k]; }
+    if constexpr (D == 2) {
+        const float4 f1 = S.attr[3 * g.elem + 1], f2 = S.attr[3 * g.elem + 2];
+        const float u2 = g.bw[1];
+        b.bv = f1.z + u2 * (f1.w - f1.z);
+        b.alpha = f2.x + u2 * (f2.y - f2.x);
+        b.type = (int)f2.z;
+    } else {
+        const float4 f3 = S.attr[5 * g.elem + 3], f4 = S.attr[5 * g.elem + 4];
+        b.bv = g.bw[0] * f3.x + g.bw[1] * f3.y + g.bw[2] * f3.z;
+        b.alpha = g.bw[0] * f4.x + g.bw[1] * f4.y + g.bw[2] * f4.z;
+        b.type = (int)f3.w;
+    }
+    return b;
+}
+
 // scene.cuh:586-622 / 891-918
 template <int D> __device__ __forceinline__ Sample<D> sample_bv(const DevScene &S, Rng &rng) {
     const float u1 = rng.uniform();
@@ -1003,25 +1062,44 @@ template <int D> __device__ __forceinline__ float cv_value(const DevScene &S, ui
 template <int D, int BIE>
 __device__ __forceinline__ Sample<D> ris(const DevScene &S, const Model<D, BIE> &m, const BP<D> &yp, uint32_t nrc,
                                          Rng &rng, uint32_t cv_r, float &ksum) {
-    Sample<D> r;
-    r.y = bp_zero<D>();
+    // the target needs only the candidate's position; the boundary data of the chosen one are
+    // read once, after the loop
     float w_sum = 0.f;
     ksum = 0.f;
     const int tp = yp.type;
-    auto signed_kernel = [&](const BP<D> &y) -> float {
-        if (tp == kDirichlet) return m.k.G(y.p, yp.p);
-        if (tp == kNeumann) return m.k.dG_dny(yp.p, y.p, yp.n);
-        return m.k.dG_dny(yp.p, y.p, yp.n) + yp.alpha * m.k.G(y.p, yp.p);
+    auto signed_kernel = [&](const float *y) -> float {
+        if (tp == kDirichlet) return m.k.G(y, yp.p);
+        if (tp == kNeumann) return m.k.dG_dny(yp.p, y, yp.n);
+        return m.k.dG_dny(yp.p, y, yp.n) + yp.alpha * m.k.G(y, yp.p);
     };
+    GeomSample<D> pick;
+    pick.elem = 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < D; ++k) { pick.p[k] = 0.f; pick.n[k] = 0.f; pick.bw[k] = 0.f; }
     for (uint32_t i = 0; i < nrc; ++i) {
-        const Sample<D> xi = sample_uniform<D>(S, rng);
-        const float sk = signed_kernel(xi.y);
-        const float w = fabsf(sk) * xi.inv_pdf;
-        if (cv_r) ksum += sk * cv_value<D>(S, cv_r, xi.y);
+        const GeomSample<D> xi = uniform_geometry<D>(S, rng);
+        const float sk = signed_kernel(xi.p);
+        const float w = fabsf(sk) * S.total_area;
+        if (cv_r) {
+            BP<D> c;
+            c.elem = xi.elem;
+#pragma unroll
+            for (int k = 0; k < D; ++k) c.bw[k] = xi.bw[k];
+            ksum += sk * cv_value<D>(S, cv_r, c);
+        }
         w_sum += w;
-        if (rng.uniform() <= w / w_sum) r.y = xi.y;
+        const bool take = rng.uniform() <= w / w_sum;
+        pick.elem = take ? xi.elem : pick.elem;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            pick.p[k] = take ? xi.p[k] : pick.p[k];
+            pick.n[k] = take ? xi.n[k] : pick.n[k];
+            pick.bw[k] = take ? xi.bw[k] : pick.bw[k];
+        }
     }
-    r.inv_pdf = w_sum / fmaxf(fabsf(signed_kernel(r.y)) * (float)nrc, 10 * kFltEps);
+    Sample<D> r;
+    r.y = pick.elem == 0xFFFFFFFFu ? bp_zero<D>() : complete_uniform<D>(S, pick);
+    r.inv_pdf = w_sum / fmaxf(fabsf(signed_kernel(r.y.p)) * (float)nrc, 10 * kFltEps);
     return r;
 }
 
