@@ -20,6 +20,9 @@
  *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
  *   - No exceptions cross the ABI; every entry point returns a wob_status and records a
  *     message retrievable with wob_last_error_message() (thread-local).
+ *   - A scene owns the scratch memory of its walks (per-item partial sums, a work counter):
+ *     estimate calls on one scene run in stream order, so issue them on one stream (or
+ *     synchronise between streams).  Handles are not thread-safe.
  *   - Element records follow ElementXs (scene.cuh:378-382, 646-650): per element its Dim
  *     vertices, one normal, Dim boundary values, a boundary type, Dim Robin coefficients.
  *   - Boundary point records follow BoundaryPoint (points.hpp:32-39) as 2*Dim+4 floats:

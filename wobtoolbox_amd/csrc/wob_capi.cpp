@@ -513,7 +513,10 @@ wob_status wob_scene_set_volume_grid(wob_scene *scene, uint32_t grid_res, float 
     const size_t bytes[3] = {cells * 4, cells * 4, guide.size() * 4};
     for (int i = 0; i < 3; ++i) {
         if (hipMalloc(&p[i], bytes[i]) != hipSuccess) {
-            for (int j = 0; j < i; ++j) (void)hipFree(p[j]);
+            for (int j = 0; j < i; ++j) {
+                (void)hipFree(p[j]);
+                scene->vol_bufs[j] = nullptr;
+            }
             return fail(WOB_OOM, "hipMalloc volume grid");
         }
         scene->vol_bufs[i] = p[i];
