@@ -48,11 +48,26 @@ __device__ __forceinline__ bf16x4 ld4(const uint16_t *p) {
     return r;
 }
 
+// Row strides are padded (BF16 rows by 4 elements, FP32 rows by 1) so the 16 rows a
+// 16x16x16 MFMA operand spans fall in distinct LDS banks.
+constexpr int HS = W + 4;         // BF16 activation / weight row stride
+constexpr int ZS = W + 1;         // FP32 dz row stride
 struct Smem {
-    uint16_t h[LAYERS][T][W];     // layer inputs (h[0] = Fourier features), BF16
-    float dz[T][W];               // backward: dL/dZ of the current layer, FP32
+    uint16_t h[LAYERS][T][HS];    // layer inputs (h[0] = Fourier features), BF16
+    float dz[T][ZS];              // backward: dL/dZ of the current layer, FP32
     float out[T][NO];             // network outputs, FP32
+    uint16_t wbuf[W * HS];        // the current layer's weights (W_l or W_l^T), BF16, row stride cols + 4
 };
+
+// Copy a rows x cols BF16 matrix (row-major, cols a multiple of 4) into LDS with row stride
+// cols + 4, 8 bytes per access.
+__device__ __forceinline__ void stage(const uint16_t *__restrict__ src, int rows, int cols, uint16_t *dst) {
+    const int chunks = rows * cols / 4;
+    for (int c = threadIdx.x; c < chunks; c += blockDim.x) {
+        const int e = 4 * c, r = e / cols, k = e - r * cols;
+        *reinterpret_cast<uint2 *>(&dst[r * (cols + 4) + k]) = *reinterpret_cast<const uint2 *>(&src[e]);
+    }
+}
 
 __device__ __forceinline__ void features(const MlpParams &P, const float *pos, uint32_t n, uint32_t row0, Smem &s) {
     // thread t: row t/4, frequencies (t%4)*16 .. +16
@@ -72,11 +87,13 @@ __device__ __forceinline__ void features(const MlpParams &P, const float *pos, u
 }
 
 // Z = H_l W_l^T + b_l for this wave's 16 rows; hidden layers store ReLU(Z) as BF16 into
-// h[l+1]; the output layer stores Z into out.
+// h[l+1]; the output layer stores Z into out.  W_l is staged in LDS first (all threads take
+// part; the caller synchronises after the layer).
 __device__ __forceinline__ void forward_layer(const MlpParams &P, int l, Smem &s) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nout = l == LAYERS - 1 ? NO : W;
-    const uint16_t *Wl = P.w_bf16 + P.w_off[l];
+    stage(P.w_bf16 + P.w_off[l], nout, W, s.wbuf);
+    __syncthreads();
     const float *bl = P.b_f32 + P.b_off[l];
     const int r0 = wave * 16;
     for (int n0 = 0; n0 < nout; n0 += 16) {
@@ -84,7 +101,7 @@ __device__ __forceinline__ void forward_layer(const MlpParams &P, int l, Smem &s
 #pragma unroll
         for (int k0 = 0; k0 < W; k0 += 16) {
             const bf16x4 a = ld4(&s.h[l][r0 + (lane & 15)][k0 + 4 * (lane >> 4)]);
-            const bf16x4 b = ld4(&Wl[(size_t)(n0 + (lane & 15)) * W + k0 + 4 * (lane >> 4)]);
+            const bf16x4 b = ld4(&s.wbuf[(n0 + (lane & 15)) * HS + k0 + 4 * (lane >> 4)]);
             acc = mfma(a, b, acc);
         }
         const int col = n0 + (lane & 15);
@@ -184,9 +201,11 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(MlpParams P, const float
             gb[c] = t;
         }
         if (l == 0) break;
+        // dH_l = dZ W_l (T x W), masked by ReLU'(H_l > 0) -> new dz (overwrites after use);
+        // W_l^T ([in][out]) is staged in LDS with row stride nout + 4
+        stage(P.wt_bf16 + P.w_off[l], W, nout, s.wbuf);
         __syncthreads();
-        // dH_l = dZ W_l (T x W), masked by ReLU'(H_l > 0) -> new dz (overwrites after use)
-        const uint16_t *WT = P.wt_bf16 + P.w_off[l];   // [in][out]
+        const uint16_t *WT = s.wbuf;
         f32x4 accs[W / 16];
         const int r0 = wave * 16;
 #pragma unroll
@@ -197,7 +216,7 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(MlpParams P, const float
                 bf16x4 a;
 #pragma unroll
                 for (int j = 0; j < 4; ++j) a[j] = (short)f2bf(s.dz[r0 + (lane & 15)][k0 + 4 * (lane >> 4) + j]);
-                const bf16x4 b = ld4(&WT[(size_t)(n0 + (lane & 15)) * nout + k0 + 4 * (lane >> 4)]);
+                const bf16x4 b = ld4(&WT[(n0 + (lane & 15)) * (nout + 4) + k0 + 4 * (lane >> 4)]);
                 acc = mfma(a, b, acc);
             }
             accs[nt] = acc;
@@ -216,14 +235,25 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(MlpParams P, const float
     }
 }
 
-// Sum the per-workgroup partials in workgroup order, Adam-update the FP32 masters, refresh
-// the BF16 copies.  One thread per parameter.
-__global__ void mlp_adam_kernel(MlpParams P, const float *partial, uint32_t n_parts, float lr, float beta1,
-                                float beta2, float eps, float bc1, float bc2, float *master, float *m, float *v) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P.n_params) return;
+// Sum the per-workgroup partials, Adam-update the FP32 masters, refresh the BF16 copies.
+// A workgroup of 256 threads covers 64 parameters: wave s sums partials [s*n/4, (s+1)*n/4)
+// of each (coalesced over the 64 parameters), and the four slice sums are combined in slice
+// order, so the result does not depend on scheduling.
+constexpr int kAdamSlices = 4;
+__global__ __launch_bounds__(256) void mlp_adam_kernel(MlpParams P, const float *partial, uint32_t n_parts,
+                                                       float lr, float beta1, float beta2, float eps, float bc1,
+                                                       float bc2, float *master, float *m, float *v) {
+    __shared__ float red[kAdamSlices][64];
+    const uint32_t lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
+    const uint32_t p = blockIdx.x * 64 + lane;
+    const uint32_t k0 = slice * n_parts / kAdamSlices, k1 = (slice + 1) * n_parts / kAdamSlices;
     float g = 0.f;
-    for (uint32_t k = 0; k < n_parts; ++k) g += partial[(size_t)k * P.n_params_padded + p];
+    if (p < P.n_params)
+        for (uint32_t k = k0; k < k1; ++k) g += partial[(size_t)k * P.n_params_padded + p];
+    red[slice][lane] = g;
+    __syncthreads();
+    if (slice != 0 || p >= P.n_params) return;
+    g = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
     const float mm = beta1 * m[p] + (1.f - beta1) * g;
     const float vv = beta2 * v[p] + (1.f - beta2) * g * g;
     m[p] = mm;
@@ -287,7 +317,7 @@ hipError_t launch_train_step(const MlpParams &P, const float *pos, const int32_t
 
 hipError_t launch_adam(const MlpParams &P, const float *partial, uint32_t n_parts, float lr, float beta1, float beta2,
                        float eps, float bc1, float bc2, float *master, float *m, float *v, hipStream_t st) {
-    const uint32_t block = 256, grid = (P.n_params + block - 1) / block;
+    const uint32_t block = 256, grid = (P.n_params + 63) / 64;
     hipLaunchKernelGGL(wobm::mlp_adam_kernel, dim3(grid), dim3(block), 0, st, P, partial, n_parts, lr, beta1, beta2,
                        eps, bc1, bc2, master, m, v);
     return hipGetLastError();
