@@ -208,6 +208,57 @@ def poisson_leg(args, torch, wob, G):
                       "eps 1e-4, num_volume_samples 1 (Gaussian-mixture source, inside test per sample)"}
 
 
+# FP32 operations per (point, VPL) pair of the 2D single-layer aggregation (add, mul, cmp,
+# max, log2, rcp = 1; FMA = 2): r (2), |r|^2 (3), clamp (1), log2 (1), value FMA (2), rcp (1),
+# r = 0 test (1), gradient scale (1), two gradient FMAs (4).
+VPL_FLOP_PER_PAIR_2D_SL = 16
+
+
+def vpl_leg(args, torch, wob, G):
+    """The virtual-point-light app (apps/potential_flow_with_vpl.cu, data/config_potential_flow_
+    with_vpl.json): 100,000 single-layer Neumann walks (RIS 1024, path length 4) per iteration,
+    aggregated at the 39,280 grid points (256^2) outside the four obstacles.  Reports the
+    iteration time split into the path phase and the aggregation, and the aggregation's FP32
+    VALU rate."""
+    print("[bench] vpl leg", file=sys.stderr, flush=True)
+    w = G.potential_flow_vpl_2d(grid_res=256)
+    dev = torch.device("cuda:0")
+    scene = wob.SceneHost(w.elements)
+    est = wob.Estimator.from_config(scene, w.solver)
+    pts = torch.as_tensor(w.points[w.extra["mask"]]).to(dev)
+    n_vpl = int(w.solver["num_sample_paths"])
+    acc = torch.zeros((pts.shape[0], 3), dtype=torch.float64, device=dev)
+    rec, c = est.compute_vpl_paths(n_vpl, seed=0, path_base=0)
+    est.aggregate_vpl(rec, c, pts, out=acc)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3 * args.vpl_iterations)]
+    t0 = time.perf_counter()
+    for it in range(args.vpl_iterations):
+        ev[3 * it].record()
+        rec, c = est.compute_vpl_paths(n_vpl, seed=0, path_base=1 + it)
+        ev[3 * it + 1].record()
+        est.aggregate_vpl(rec, c, pts, out=acc)
+        ev[3 * it + 2].record()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    path_ms = float(np.mean([ev[3 * i].elapsed_time(ev[3 * i + 1]) for i in range(args.vpl_iterations)]))
+    agg_ms = float(np.mean([ev[3 * i + 1].elapsed_time(ev[3 * i + 2]) for i in range(args.vpl_iterations)]))
+    pairs = pts.shape[0] * n_vpl
+    g = (acc[:, 1:] / ((1 + args.vpl_iterations) * n_vpl)).cpu().numpy()
+    p = w.points[w.extra["mask"]]
+    free = (p[:, 0] < -0.4) & (np.abs(p[:, 1]) < 0.45)
+    tflops = VPL_FLOP_PER_PAIR_2D_SL * pairs / (agg_ms * 1e-3) / 1e12
+    return {"workload": w.name, "elements": w.elements.n, "points": int(pts.shape[0]), "vpls_per_iteration": n_vpl,
+            "iterations": args.vpl_iterations, "ms_per_iteration": el / args.vpl_iterations * 1e3,
+            "path_phase_ms": path_ms, "aggregate_ms": agg_ms, "pairs_per_s": pairs / (agg_ms * 1e-3),
+            "vpl_paths_per_s": n_vpl / (path_ms * 1e-3),
+            "aggregate_fp32_valu": {"achieved": tflops, "peak": 157.3, "unit": "TFLOP/s", "frac": tflops / 157.3,
+                                    "flop_per_pair": VPL_FLOP_PER_PAIR_2D_SL},
+            "free_stream_velocity_x": float(g[free, 0].mean()),
+            "solver": "IndirectSingleLayer, InteriorDomain, BackwardEstimator, NeumannProblem, path_length 4, "
+                      "RIS 1024, eps 1e-4"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,6 +275,8 @@ def main():
     ap.add_argument("--no-poisson", action="store_true")
     ap.add_argument("--poisson-paths", type=int, default=16, help="paths per point per Poisson pass")
     ap.add_argument("--poisson-passes", type=int, default=4)
+    ap.add_argument("--no-vpl", action="store_true")
+    ap.add_argument("--vpl-iterations", type=int, default=5)
     ap.add_argument("--cv-points", type=int, default=1_000_000)
     ap.add_argument("--cv-budget", type=float, default=15.0, help="walk seconds per arm")
     ap.add_argument("--cv-paths-per-pass", type=int, default=4)
@@ -310,6 +363,10 @@ def main():
     if not args.no_poisson:
         poisson = poisson_leg(args, torch, wob, G)
 
+    vpl = None
+    if not args.no_vpl:
+        vpl = vpl_leg(args, torch, wob, G)
+
     ttr = None
     if not args.no_time_to_rmse:
         ttr = time_to_rmse_leg(args, torch, wob, G)
@@ -339,6 +396,7 @@ def main():
         "time_to_rmse": ttr,
         "batch": batch,
         "poisson": poisson,
+        "vpl": vpl,
         "accuracy": {"rmse_u": rmse, "paths_per_point_total": step * paths,
                      "note": "Neumann problem: u compared after removing the mean offset"
                      if w.solver["problem"] == "NeumannProblem" else "u compared directly"},

@@ -195,6 +195,24 @@ class Estimator {
                                     num_sample_paths, seed, path_base, 0, d_sum, d_sumsq, launch, stream));
     }
 
+    // Virtual point lights (apps/potential_flow_with_vpl.cu:73-88): n uniform boundary start
+    // points y_i and c_i = compute_sample_unknown_boundary(y_i) (estimator.cuh:232-357);
+    // iteration k of a progressive loop passes path_base = k.
+    void compute_vpl_paths(uint32_t n, uint64_t seed, uint64_t path_base, BoundaryPoint<S, Dim> *d_records,
+                           S *d_contrib, void *stream = nullptr, wob_launch_config *launch = nullptr) const {
+        check(wob_vpl_paths(scene_, &desc_, n, seed, path_base, 0, reinterpret_cast<float *>(d_records), d_contrib,
+                            launch, stream));
+    }
+
+    // AggregateFunc (apps/potential_flow_with_vpl.cu:44-71) over all points: d_sum (n_points *
+    // (Dim + 1) doubles, device) += sum_i c_i [solution_domain_unknown, gradient_domain_unknown].
+    void aggregate_vpl(const BoundaryPoint<S, Dim> *d_records, const S *d_contrib, uint32_t n_vpl,
+                       const DomainPoint<S, Dim> *d_points, uint32_t n_points, double *d_sum,
+                       void *stream = nullptr) const {
+        check(wob_vpl_aggregate(scene_, &desc_, reinterpret_cast<const float *>(d_records), d_contrib, n_vpl,
+                                reinterpret_cast<const float *>(d_points), n_points, d_sum, stream));
+    }
+
   private:
     const wob_scene *scene_;
     wob_estimator_desc desc_;

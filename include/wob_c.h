@@ -200,6 +200,35 @@ wob_status wob_estimate_boundary(const wob_scene *scene, const wob_estimator_des
                                  uint64_t path_base, uint32_t stream_id, double *d_sum, double *d_sumsq,
                                  wob_launch_config *launch, void *stream);
 
+/* Virtual point lights (apps/potential_flow_with_vpl.cu): the path phase and the dense
+ * aggregation of one iteration.
+ *
+ * wob_vpl_paths replaces PathComputeFunc (potential_flow_with_vpl.cu:73-88): for i < n_paths,
+ * y_i = sample_boundary_uniform (scene.cuh:441-462 / 734-758) and c_i =
+ * compute_sample_unknown_boundary(y_i) (estimator.cuh:232-357), one backward walk per start
+ * point.  y_i uses the Philox stream (seed, path_base, i, stream_id | 2^30) and the walk from
+ * it (seed, path_base, i, stream_id); an iteration loop advances path_base by 1.
+ *   d_records  n_paths * (2*dim + 4) floats (BoundaryPoint records, written)
+ *   d_contrib  n_paths floats (c_i, written)
+ *   stream_id  below 2^30; path_base below 2^32
+ * The estimator must be backward without the control variate; launch (may be NULL) tunes and
+ * times the walk as for wob_estimate_boundary.                                               */
+wob_status wob_vpl_paths(const wob_scene *scene, const wob_estimator_desc *est, uint32_t n_paths, uint64_t seed,
+                         uint64_t path_base, uint32_t stream_id, float *d_records, float *d_contrib,
+                         wob_launch_config *launch, void *stream);
+/* wob_vpl_aggregate replaces AggregateFunc (potential_flow_with_vpl.cu:44-71) over all points:
+ *   d_sum[i*(dim+1) + 0]     += sum_j c_j * solution_domain_unknown(x_i, y_j, |boundary|)
+ *   d_sum[i*(dim+1) + 1..dim] += sum_j c_j * gradient_domain_unknown(x_i, y_j, |boundary|)
+ * for the estimator's bie_model, domain and eps (bie_models.cuh:228-243, 341-356, 428-462;
+ * the other fields are ignored).  d_points: n_points * dim floats; d_records / d_contrib as
+ * written by wob_vpl_paths (or any BoundaryPoint records and weights).  FP32 terms,
+ * compensated FP32 sums over VPL ranges, FP64 accumulation into d_sum (deterministic).  The
+ * caller divides by the number of paths, as the reference app does (:250-257), and skips
+ * masked points by leaving them out of d_points.                                             */
+wob_status wob_vpl_aggregate(const wob_scene *scene, const wob_estimator_desc *est, const float *d_records,
+                             const float *d_contrib, uint32_t n_vpl, const float *d_points, uint32_t n_points,
+                             double *d_sum, void *stream);
+
 /* Batched launches (no counterpart in the reference, whose apps run one scene per
  * thrust::for_each): many (scene, estimator, point set) entries of uneven sizes run in ONE
  * walk launch and ONE reduction launch per pass.  All entries must share the dimension and

@@ -1111,6 +1111,7 @@ struct orc_est_desc {
     float eps, first_kind_equation_scaling_constant;
     int32_t pick_mode, rng_mode, use_bvh, exclude_origin;
     int32_t independent_volume;
+    int32_t boundary_density;   // boundary entry: return compute_sample_unknown_boundary (the walk)
 };
 
 extern "C" {
@@ -1374,7 +1375,7 @@ static void run_boundary(const Scene<D> &s, const orc_est_desc *d, const float *
         Rng rng;
         for (unsigned j = 0; j < paths; ++j) {
             rng.init_philox(seed, (uint32_t)(path_base + j), (uint32_t)i, stream);
-            const float v = e.boundary_path(x, rng);
+            const float v = d->boundary_density ? e.walk(x, rng) : e.boundary_path(x, rng);
             ks.add(v); kq.add(v * v);
             if (path_out) path_out[(size_t)i * paths + j] = v;
         }
@@ -1443,6 +1444,70 @@ int orc_eval_weights(int dim, int bie, int domain, float eps, float fkesc, int n
             Vec<D> g = m.gradient_domain_unknown(X, y, inv_pdf[i]);
             for (int k = 0; k < D; ++k) o[4 + k] = g[k];
             o[4 + D] = m.solution_boundary_unknown(x, y, inv_pdf[i]);
+        }
+    };
+    if (dim == 2) run(std::integral_constant<int, 2>{});
+    else run(std::integral_constant<int, 3>{});
+    return 0;
+}
+
+// One Scene::sample_boundary_uniform draw (scene.cuh:441-462 / 734-758) from the Philox
+// stream (seed, path, point, stream).  out_rec: 2D+4 floats, returns inv_pdf.
+float orc_sample_uniform(void *p, uint64_t seed, uint32_t path, uint32_t point, uint32_t stream, float *out_rec) {
+    auto *S = (OrcScene *)p;
+    Rng rng; rng.init_philox(seed, path, point, stream);
+    auto emit = [&](auto &s, auto dimtag) {
+        constexpr int D = decltype(dimtag)::value;
+        Sampled<D> r = sample_uniform(s, rng);
+        for (int k = 0; k < D; ++k) { out_rec[k] = r.y.p[k]; out_rec[D + k] = r.y.n[k]; }
+        out_rec[2 * D] = r.y.interior_angle; out_rec[2 * D + 1] = r.y.boundary_value;
+        out_rec[2 * D + 2] = (float)r.y.boundary_type; out_rec[2 * D + 3] = r.y.robin_alpha;
+        return r.inv_pdf;
+    };
+    if (S->dim == 2) return emit(S->s2, std::integral_constant<int, 2>{});
+    return emit(S->s3, std::integral_constant<int, 3>{});
+}
+
+// VPL aggregation (apps/potential_flow_with_vpl.cu:44-71): for each point x_i,
+//   sum[i][0] = sum_j c_j * solution_domain_unknown(x_i, y_j, inv_pdf),
+//   sum[i][1..D] = sum_j c_j * gradient_domain_unknown(x_i, y_j, inv_pdf),
+// the products in FP32 as the reference forms them, accumulated in FP64 (the reference
+// Kahan-sums in FP32); abssum receives the sums of |term| (the scale of rounding errors).
+int orc_vpl_aggregate(int dim, int bie, int domain, float eps, float inv_pdf, int n_vpl, const float *rec,
+                      const float *contrib, int n_pts, const float *pts, int threads, double *sum, double *abssum) {
+    auto run = [&](auto dimtag) {
+        constexpr int D = decltype(dimtag)::value;
+        const int W = 2 * D + 4;
+        Model<D> m;
+        m.bie = bie; m.interior = domain == InteriorDomain; m.dsign = m.interior ? 1.f : -1.f;
+        m.k.eps = eps; m.c = 1.f; m.nvs = 0; m.scene = nullptr;
+        std::vector<BPoint<D>> ys(n_vpl);
+        for (int j = 0; j < n_vpl; ++j) {
+            const float *r = rec + (size_t)j * W;
+            BPoint<D> &b = ys[j];
+            for (int k = 0; k < D; ++k) { b.p[k] = r[k]; b.n[k] = r[D + k]; }
+            b.interior_angle = r[2 * D]; b.boundary_value = r[2 * D + 1]; b.boundary_type = (int)r[2 * D + 2];
+            b.robin_alpha = r[2 * D + 3];
+        }
+#ifdef _OPENMP
+        if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(static)
+#endif
+        for (int i = 0; i < n_pts; ++i) {
+            Vec<D> x;
+            for (int k = 0; k < D; ++k) x[k] = pts[(size_t)i * D + k];
+            double acc[D + 1] = {}, aacc[D + 1] = {};
+            for (int j = 0; j < n_vpl; ++j) {
+                const float c = contrib[j];
+                const float u = c * m.solution_domain_unknown(x, ys[j], inv_pdf);
+                const Vec<D> g = m.gradient_domain_unknown(x, ys[j], inv_pdf);
+                acc[0] += u; aacc[0] += std::fabs(u);
+                for (int k = 0; k < D; ++k) {
+                    const float gk = c * g[k];
+                    acc[1 + k] += gk; aacc[1 + k] += std::fabs(gk);
+                }
+            }
+            for (int k = 0; k <= D; ++k) { sum[(size_t)i * (D + 1) + k] = acc[k]; abssum[(size_t)i * (D + 1) + k] = aacc[k]; }
         }
     };
     if (dim == 2) run(std::integral_constant<int, 2>{});

@@ -28,7 +28,7 @@ class EstDesc(C.Structure):
                 ("num_volume_samples", C.c_uint32), ("eps", C.c_float),
                 ("first_kind_equation_scaling_constant", C.c_float), ("pick_mode", C.c_int32),
                 ("rng_mode", C.c_int32), ("use_bvh", C.c_int32), ("exclude_origin", C.c_int32),
-                ("independent_volume", C.c_int32)]
+                ("independent_volume", C.c_int32), ("boundary_density", C.c_int32)]
 
 
 def build():
@@ -68,6 +68,10 @@ def lib():
         L.orc_triple32.restype = C.c_uint32
         L.orc_sample_line.argtypes = [P, P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, P]
         L.orc_sample_line.restype = C.c_float
+        L.orc_sample_uniform.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, P]
+        L.orc_sample_uniform.restype = C.c_float
+        L.orc_vpl_aggregate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, P, P, C.c_int, P,
+                                        C.c_int, P, P]
         _lib = L
     return _lib
 
@@ -78,15 +82,15 @@ def _p(a):
 
 def est_desc(solver: dict, pick_mode=1, rng_mode=0, use_bvh=1, num_volume_samples=0) -> EstDesc:
     """Solver dict (reference JSON keys + the device estimator flags) -> oracle descriptor.
-    flags bit 0: exclude the origin element of boundary line samples; bit 3: independent
-    volume samples."""
+    flags bit 0: exclude the origin element of boundary line samples; bit 2: boundary density
+    (the walk's value at boundary records); bit 3: independent volume samples."""
     fl = int(solver.get("flags", 0))
     nvs = num_volume_samples or int(solver.get("num_volume_samples", 0))
     return EstDesc(ENUMS[solver["bie_model"]], ENUMS[solver["domain"]], ENUMS[solver["estimator"]],
                    ENUMS[solver["problem"]], solver["path_length"], solver.get("num_resampling_candidates", 1),
                    nvs, solver.get("eps", float(np.finfo(np.float32).eps)),
                    solver.get("first_kind_equation_scaling_constant", 1.0), pick_mode, rng_mode, use_bvh, fl & 1,
-                   (fl >> 3) & 1)
+                   (fl >> 3) & 1, (fl >> 2) & 1)
 
 
 class OracleScene:
@@ -161,6 +165,33 @@ class OracleScene:
         lib().orc_estimate_boundary(self.h, C.byref(d), _p(r), n, paths, seed, path_base, stream, threads, _p(s),
                                     _p(q), _p(po))
         return (s, q, po) if return_paths else (s, q)
+
+    def sample_uniform(self, seed, path, point, stream):
+        """One uniform boundary sample from the Philox stream (seed, path, point, stream)."""
+        rec = np.zeros(2 * self.dim + 4, np.float32)
+        inv = lib().orc_sample_uniform(self.h, seed, path, point, stream, _p(rec))
+        return rec, inv
+
+    def vpl_paths(self, solver, n, seed=0, path_base=0, stream=0, threads=0):
+        """The VPL path phase with the device's streams: start point i from (seed, path_base, i,
+        stream | 2^30), then one boundary-density walk from (seed, path_base, i, stream)."""
+        rec = np.stack([self.sample_uniform(seed, path_base, i, stream | (1 << 30))[0] for i in range(n)])
+        sol = dict(solver, flags=int(solver.get("flags", 0)) | 4)
+        s, _ = self.estimate_boundary(sol, rec, 1, seed=seed, path_base=path_base, stream=stream, threads=threads)
+        return rec, s.astype(np.float32)
+
+
+def vpl_aggregate(dim, bie, domain, eps, inv_pdf, records, contrib, points, threads=0):
+    """FP64 sums of c_j * [solution_domain_unknown, gradient_domain_unknown](x_i, y_j, inv_pdf)
+    and of their absolute values, each (n_points, dim + 1)."""
+    r = np.ascontiguousarray(records, np.float32)
+    c = np.ascontiguousarray(contrib, np.float32)
+    x = np.ascontiguousarray(points, np.float32)
+    s = np.zeros((x.shape[0], dim + 1), np.float64)
+    a = np.zeros_like(s)
+    lib().orc_vpl_aggregate(dim, bie, domain, float(eps), float(inv_pdf), r.shape[0], _p(r), _p(c), x.shape[0],
+                            _p(x), threads, _p(s), _p(a))
+    return s, a
 
 
 def eval_kernels(dim, y, x, ny, nx, eps, f64=False):

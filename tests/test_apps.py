@@ -123,3 +123,39 @@ def test_app_3d_progressive_run(tmp_path, gpu):
     assert np.all(np.isfinite(rows))
     _, g = _read_field(tmp_path / "out" / "gradient2.vector3", 3)
     assert g.shape == (32, 32, 3)
+
+
+def _vpl_config(tmp_path, **kw):
+    cfg = {"output_dir": str(tmp_path / "vpl"), "path_length": 4, "num_sample_paths": 20000,
+           "num_resampling_candidates": 64, "grid_res": 64, "eps": 1e-4, "max_time": 1}
+    cfg.update(kw)
+    p = tmp_path / "vpl.json"
+    p.write_text(json.dumps(cfg, indent=4))
+    return p
+
+
+def test_vpl_app_dry_run(tmp_path):
+    """apps/potential_flow_with_vpl.cu's scene and mask: 4 + 360 + 180 + 180 + 90 segments and
+    the grid points outside the obstacles (the same count as the Python workload's mask)."""
+    out = _run(B.VPL_APP, _vpl_config(tmp_path, grid_res=256), "--dry-run")
+    assert "elements\t: 814" in out
+    n_eval = int(G.potential_flow_vpl_2d(grid_res=256).extra["mask"].sum())
+    assert f"evaluation points\t: {n_eval} of 65536" in out
+    assert json.load(open(tmp_path / "vpl" / "config.json"))["num_sample_paths"] == 20000
+
+
+@pytest.mark.gpu
+def test_vpl_app_iterations(tmp_path, gpu):
+    _run(B.VPL_APP, _vpl_config(tmp_path), "--max-iterations", "3")
+    d = tmp_path / "vpl"
+    mask = G.potential_flow_vpl_2d(grid_res=64).extra["mask"].reshape(64, 64)
+    for it in (1, 2, 3):
+        res, u = _read_field(d / f"potential{it}.scalar")
+        _, g = _read_field(d / f"gradient{it}.vector2", 2)
+        assert res == 64
+        assert np.all(np.isnan(u[~mask])) and np.all(np.isfinite(u[mask]))
+        assert np.all(g[~mask] == 0) and np.all(np.isfinite(g[mask]))
+    # flow from left to right: the mean x-velocity in the free stream is positive, near 1
+    c = G.grid_points(64).reshape(64, 64, 2)
+    free = mask & (np.abs(c[..., 1]) < 0.45) & (c[..., 0] < -0.4)
+    assert 0.5 < g[free][:, 0].mean() < 2.0

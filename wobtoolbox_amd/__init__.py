@@ -227,6 +227,33 @@ class Estimator:
                                           _stream_ptr(stream)))
         return s, q
 
+    def compute_vpl_paths(self, n_paths, seed=0, path_base=0, stream_id=0, device="cuda", stream=None):
+        """Path phase of the VPL method (apps/potential_flow_with_vpl.cu:73-88): n_paths uniform
+        boundary start points and the boundary-density walk value at each.
+
+        Returns (records (n, 2*dim + 4) float32, contributions (n,) float32) device tensors."""
+        torch = _torch()
+        W = 2 * self.scene.dim + 4
+        rec = torch.empty((n_paths, W), dtype=torch.float32, device=device)
+        c = torch.empty(n_paths, dtype=torch.float32, device=device)
+        d, lc = self.desc(), self._launch()
+        check(lib().wob_vpl_paths(self.scene.handle, C.byref(d), int(n_paths), seed, path_base, stream_id, _ptr(rec),
+                                  _ptr(c), C.byref(lc), _stream_ptr(stream)))
+        return rec, c
+
+    def aggregate_vpl(self, records, contrib, points, out=None, stream=None):
+        """Aggregation of the VPL method (apps/potential_flow_with_vpl.cu:44-71): adds
+        sum_j c_j [solution_domain_unknown, gradient_domain_unknown](x_i, y_j, |boundary|) into an
+        FP64 (n, dim + 1) tensor (created when `out` is None) and returns it."""
+        torch = _torch()
+        rec, c, pts = records.contiguous(), contrib.contiguous(), points.contiguous()
+        n = pts.shape[0]
+        s = torch.zeros((n, self.scene.dim + 1), dtype=torch.float64, device=pts.device) if out is None else out
+        d = self.desc()
+        check(lib().wob_vpl_aggregate(self.scene.handle, C.byref(d), _ptr(rec), _ptr(c), rec.shape[0], _ptr(pts), n,
+                                      _ptr(s), _stream_ptr(stream)))
+        return s
+
 
 class Batch:
     """Many (scene, estimator, point set) entries in one launch per pass (wob_batch_*)."""

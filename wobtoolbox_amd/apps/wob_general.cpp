@@ -41,6 +41,7 @@
 #include <vector>
 
 #include "../../include/wob/wob.hpp"
+#include "app_common.hpp"
 
 #ifndef WOB_APP_DIM
 #error "WOB_APP_DIM must be 2 or 3"
@@ -49,70 +50,7 @@ constexpr unsigned D = WOB_APP_DIM;
 
 namespace {
 
-// ---------------------------------------------------------------------------------------
-// configuration: a flat JSON object of strings, numbers and booleans
-// ---------------------------------------------------------------------------------------
-struct Config {
-    std::map<std::string, std::string> s;
-    std::map<std::string, double> n;
-    bool has(const std::string &k) const { return s.count(k) || n.count(k); }
-    std::string str(const std::string &k) const {
-        auto it = s.find(k);
-        if (it == s.end()) throw std::runtime_error("missing string key: " + k);
-        return it->second;
-    }
-    double num(const std::string &k) const {
-        auto it = n.find(k);
-        if (it == n.end()) throw std::runtime_error("missing number key: " + k);
-        return it->second;
-    }
-    double num_or(const std::string &k, double d) const { return n.count(k) ? n.at(k) : d; }
-};
-
-Config parse_config(const std::string &text) {
-    Config c;
-    size_t i = 0;
-    auto ws = [&]() { while (i < text.size() && std::isspace((unsigned char)text[i])) ++i; };
-    auto expect = [&](char ch) {
-        ws();
-        if (i >= text.size() || text[i] != ch) throw std::runtime_error(std::string("config: expected '") + ch + "'");
-        ++i;
-    };
-    auto string_lit = [&]() {
-        expect('"');
-        std::string out;
-        while (i < text.size() && text[i] != '"') {
-            if (text[i] == '\\' && i + 1 < text.size()) ++i;
-            out += text[i++];
-        }
-        expect('"');
-        return out;
-    };
-    expect('{');
-    ws();
-    if (i < text.size() && text[i] == '}') return c;
-    while (true) {
-        const std::string key = string_lit();
-        expect(':');
-        ws();
-        if (i < text.size() && text[i] == '"') {
-            c.s[key] = string_lit();
-        } else {
-            size_t j = i;
-            while (j < text.size() && text[j] != ',' && text[j] != '}' && !std::isspace((unsigned char)text[j])) ++j;
-            const std::string tok = text.substr(i, j - i);
-            i = j;
-            if (tok == "true" || tok == "false") c.n[key] = tok == "true" ? 1.0 : 0.0;
-            else if (tok == "null") {}
-            else c.n[key] = std::stod(tok);
-        }
-        ws();
-        if (i < text.size() && text[i] == ',') { ++i; continue; }
-        expect('}');
-        break;
-    }
-    return c;
-}
+using namespace wobapp;
 
 // ---------------------------------------------------------------------------------------
 // OBJ: "v x y z" and "f i j [k]" records (1-based, "i/t/n" accepted, negative = relative)
@@ -208,19 +146,6 @@ double winding(const std::vector<double> &V, const std::vector<int> &F, const do
     }
 }
 
-// utils.cuh:92-99 with domain size 1: cell centres of (-1/2, 1/2)^2, x fastest
-void grid_point(unsigned idx, unsigned res, double *p2) {
-    const double dx = 1.0 / res;
-    p2[0] = ((double)(idx % res) + 0.5) * dx - 0.5;
-    p2[1] = ((double)(idx / res) + 0.5) * dx - 0.5;
-}
-
-template <typename T> void save_field(const std::string &path, unsigned res, const std::vector<T> &v) {
-    std::ofstream out(path, std::ios::binary);
-    out.write((const char *)&res, sizeof(unsigned));
-    out.write((const char *)v.data(), (std::streamsize)(v.size() * sizeof(T)));
-}
-
 // third vertex of a triangle; the Robin alpha of that vertex stays 0 as in the apps
 inline void set_third_vertex(wob::ElementXs<float, 2> &, const double *, double) {}
 inline void set_third_vertex(wob::ElementXs<float, 3> &e, const double *v, double bv) {
@@ -228,12 +153,6 @@ inline void set_third_vertex(wob::ElementXs<float, 3> &e, const double *v, doubl
     e.boundary_value_c = (float)bv;
     e.robin_alpha_c = 0.0f;
 }
-
-#define HIP_OK(x)                                                                                \
-    do {                                                                                         \
-        hipError_t e_ = (x);                                                                     \
-        if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
-    } while (0)
 
 struct Run {
     const wob::SceneHost<float, D> *scene;

@@ -29,6 +29,7 @@ SOURCES = [
     ("wob_walk.hip", "hip", ["-DWOB_WALK_DIM=3", "-DWOB_WALK_VOL=0"], "wob_walk_30.o"),
     ("wob_walk.hip", "hip", ["-DWOB_WALK_DIM=3", "-DWOB_WALK_VOL=1"], "wob_walk_31.o"),
     ("wob_kernels.hip", "hip"),
+    ("wob_vpl.hip", "hip"),
     ("mlp_kernels.hip", "hip"),
     ("wob_capi.cpp", "hip"),
     ("wob_cv.cpp", "hip"),
@@ -97,14 +98,16 @@ def _build_shim_check():
 
 
 APPS = {2: os.path.join(LIBDIR, "wob_general_2d"), 3: os.path.join(LIBDIR, "wob_general_3d")}
+VPL_APP = os.path.join(LIBDIR, "wob_vpl")
 
 
 def _build_apps():
-    """The reference apps' runner (apps/wob_general.cpp) for 2D and 3D."""
-    src = os.path.join(PKG, "apps", "wob_general.cpp")
+    """The reference apps' runners: apps/wob_general.cpp for 2D and 3D, apps/wob_vpl.cpp."""
+    jobs = [(os.path.join(PKG, "apps", "wob_general.cpp"), out, [f"-DWOB_APP_DIM={dim}"]) for dim, out in APPS.items()]
+    jobs.append((os.path.join(PKG, "apps", "wob_vpl.cpp"), VPL_APP, []))
     procs = []
-    for dim, out in APPS.items():
-        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", f"-DWOB_APP_DIM={dim}",
+    for src, out, defs in jobs:
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", *defs,
                "-I" + os.path.join(ROCM, "include"), src, "-o", out, "-L" + LIBDIR, "-lwob_hip",
                "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN",
                "-Wl,-rpath," + os.path.join(ROCM, "lib")]

@@ -49,6 +49,8 @@ struct wob_scene {
     float *cv_table = nullptr;
     void *vol_bufs[3] = {nullptr, nullptr, nullptr};
     uint64_t version = 0;   // bumped whenever device data referenced by launches changes
+    double *aux = nullptr;  // FP64 accumulators of VPL path phases
+    size_t aux_bytes = 0;
 };
 
 const wobhost::HostScene &wob_scene_host(const wob_scene *scene) { return scene->host; }
@@ -93,6 +95,8 @@ void free_scene(wob_scene *s) {
     s->counter = nullptr;
     if (s->cv_table) (void)hipFree(s->cv_table);
     s->cv_table = nullptr;
+    if (s->aux) (void)hipFree(s->aux);
+    s->aux = nullptr;
 }
 
 wob_status ensure_scratch(wob_scene *s, size_t bytes) {
@@ -105,6 +109,19 @@ wob_status ensure_scratch(wob_scene *s, size_t bytes) {
     if (e != hipSuccess) return fail(WOB_OOM, std::string("hipMalloc scratch: ") + hipGetErrorString(e));
     s->scratch = (float *)p;
     s->scratch_bytes = bytes;
+    return WOB_OK;
+}
+
+wob_status ensure_aux(wob_scene *s, size_t bytes) {
+    if (bytes <= s->aux_bytes) return WOB_OK;
+    if (s->aux) WOB_HIP(hipFree(s->aux), "hipFree aux");
+    s->aux = nullptr;
+    s->aux_bytes = 0;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(WOB_OOM, std::string("hipMalloc aux: ") + hipGetErrorString(e));
+    s->aux = (double *)p;
+    s->aux_bytes = bytes;
     return WOB_OK;
 }
 
@@ -645,6 +662,75 @@ wob_status wob_sample_boundary(const wob_scene *scene, uint32_t n, uint64_t seed
     WOB_HIP(wobl::launch_sample_boundary((int)scene->host.dim, scene->dev, n, seed, stream_id, d_records,
                                          (hipStream_t)stream),
             "sample_boundary launch");
+    return WOB_OK;
+}
+
+// Stream word of the VPL start-point samples: distinct from every walk stream word (walks
+// use the caller's stream_id; the control variate's independent draws flip bit 31).
+constexpr uint32_t kVplSampleStream = 0x40000000u;
+
+wob_status wob_vpl_paths(const wob_scene *cscene, const wob_estimator_desc *est, uint32_t n_paths, uint64_t seed,
+                         uint64_t path_base, uint32_t stream_id, float *d_records, float *d_contrib,
+                         wob_launch_config *launch, void *stream) {
+    if (!cscene || !est) return fail(WOB_INVALID_ARG, "null scene or estimator descriptor");
+    if (est->estimator != WOB_BACKWARD) return fail(WOB_UNSUPPORTED_COMBO, "VPL paths need the backward estimator");
+    if (est->flags & WOB_EST_CONTROL_VARIATE)
+        return fail(WOB_UNSUPPORTED_COMBO, "VPL paths do not take the control variate");
+    if (n_paths == 0) return WOB_OK;
+    if (!d_records || !d_contrib) return fail(WOB_INVALID_ARG, "null device buffer");
+    if (stream_id & (kVplSampleStream | 0x80000000u)) return fail(WOB_INVALID_ARG, "stream_id must be below 2^30");
+    if (path_base > 0xFFFFFFFFull) return fail(WOB_INVALID_ARG, "path_base must fit in 32 bits");
+    wob_scene *scene = const_cast<wob_scene *>(cscene);
+    wob_estimator_desc e = *est;
+    e.flags |= WOB_EST_BOUNDARY_DENSITY;
+    int flavour = 0;
+    wob_status st = check_est(scene, &e, flavour);
+    if (st != WOB_OK) return st;
+    hipStream_t hs = (hipStream_t)stream;
+    if ((st = ensure_aux(scene, (size_t)2 * n_paths * sizeof(double))) != WOB_OK) return st;
+    // start points: VPL i of the pass uses the stream (seed, path_base, i, stream_id | 2^30)
+    WOB_HIP(wobl::launch_sample_boundary((int)scene->host.dim, scene->dev, n_paths, seed, stream_id | kVplSampleStream,
+                                         d_records, hs, (int64_t)path_base),
+            "VPL start-point launch");
+    WOB_HIP(wobl::launch_fill_u32((uint32_t *)scene->aux, 4u * n_paths, 0u, hs), "aux clear");
+    // one boundary-density walk per start point, stream (seed, path_base, i, stream_id)
+    st = run_walk(scene, &e, d_records, n_paths, 0u, 1u, seed, path_base, stream_id, scene->aux, scene->aux + n_paths,
+                  launch, stream, true);
+    if (st != WOB_OK) return st;
+    WOB_HIP(wobl::launch_f64_to_f32(scene->aux, n_paths, d_contrib, hs), "VPL contribution launch");
+    return WOB_OK;
+}
+
+wob_status wob_vpl_aggregate(const wob_scene *cscene, const wob_estimator_desc *est, const float *d_records,
+                             const float *d_contrib, uint32_t n_vpl, const float *d_points, uint32_t n_points,
+                             double *d_sum, void *stream) {
+    if (!cscene || !est) return fail(WOB_INVALID_ARG, "null scene or estimator descriptor");
+    if (est->equation_kernel != WOB_POISSON_KERNEL) return fail(WOB_INVALID_ENUM, "equation_kernel must be PoissonKernel");
+    if (est->bie_model < 0 || est->bie_model > 2) return fail(WOB_INVALID_ENUM, "bie_model out of range");
+    if (est->domain < 0 || est->domain > 1) return fail(WOB_INVALID_ENUM, "domain out of range");
+    if (!(est->eps > 0.f)) return fail(WOB_INVALID_ARG, "eps must be > 0");
+    if (n_vpl == 0 || n_points == 0) return WOB_OK;
+    if (!d_records || !d_contrib || !d_points || !d_sum) return fail(WOB_INVALID_ARG, "null device buffer");
+    wob_scene *scene = const_cast<wob_scene *>(cscene);
+    const int D = (int)scene->host.dim;
+    const uint32_t V = (uint32_t)D + 1u;
+    uint32_t splits = 0, chunk = 0;
+    wobl::vpl_plan(n_points, n_vpl, splits, chunk);
+    const size_t rec_bytes = ((size_t)n_vpl * wobl::vpl_record_floats(D, est->bie_model) * sizeof(float) + 255) & ~(size_t)255;
+    const size_t part_bytes = (size_t)splits * n_points * V * sizeof(double);
+    wob_status st = ensure_scratch(scene, rec_bytes + part_bytes);
+    if (st != WOB_OK) return st;
+    float *rec = scene->scratch;
+    double *partial = (double *)((char *)scene->scratch + rec_bytes);
+    hipStream_t hs = (hipStream_t)stream;
+    // inv_pdf = total boundary area, as apps/potential_flow_with_vpl.cu:61-63 passes it
+    WOB_HIP(wobl::launch_vpl_stage(D, est->bie_model, d_records, d_contrib, n_vpl, scene->host.total_area,
+                                   est->domain == WOB_INTERIOR ? 1.f : -1.f, rec, hs),
+            "VPL stage launch");
+    WOB_HIP(wobl::launch_vpl_aggregate(D, est->bie_model, d_points, n_points, rec, n_vpl, splits, chunk, est->eps,
+                                       partial, hs),
+            "VPL aggregate launch");
+    WOB_HIP(wobl::launch_vpl_reduce(partial, n_points, V, splits, d_sum, hs), "VPL reduce launch");
     return WOB_OK;
 }
 

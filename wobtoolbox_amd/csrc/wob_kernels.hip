@@ -157,12 +157,16 @@ __global__ __launch_bounds__(256) void intersect_kernel(DevScene S, uint32_t n, 
     counts[i] = c;
 }
 
+// per_point == 0: record i uses the Philox stream (seed, path i, point 0, stream);
+// per_point == 1: (seed, path `path`, point i, stream) -- one stream per VPL of a pass.
 template <int D>
-__global__ void sample_boundary_kernel(DevScene S, uint32_t n, uint64_t seed, uint32_t stream, float *out) {
+__global__ void sample_boundary_kernel(DevScene S, uint32_t n, uint64_t seed, uint32_t stream, uint32_t path,
+                                       uint32_t per_point, float *out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Rng rng;
-    rng.init(seed, i, 0u, stream);
+    if (per_point) rng.init(seed, path, i, stream);
+    else rng.init(seed, i, 0u, stream);
     const Sample<D> z = sample_uniform<D>(S, rng);
     float *r = out + (size_t)i * (2 * D + 4);
 #pragma unroll
@@ -270,10 +274,11 @@ hipError_t launch_intersect(int dim, const DevScene &S, uint32_t n, const float 
 }
 
 hipError_t launch_sample_boundary(int dim, const DevScene &S, uint32_t n, uint64_t seed, uint32_t stream, float *out,
-                                  hipStream_t st) {
+                                  hipStream_t st, int64_t path) {
     const uint32_t block = 256, grid = (n + block - 1) / block;
-    if (dim == 2) hipLaunchKernelGGL(sample_boundary_kernel<2>, dim3(grid), dim3(block), 0, st, S, n, seed, stream, out);
-    else hipLaunchKernelGGL(sample_boundary_kernel<3>, dim3(grid), dim3(block), 0, st, S, n, seed, stream, out);
+    const uint32_t pp = path >= 0 ? 1u : 0u, pw = path >= 0 ? (uint32_t)path : 0u;
+    if (dim == 2) hipLaunchKernelGGL(sample_boundary_kernel<2>, dim3(grid), dim3(block), 0, st, S, n, seed, stream, pw, pp, out);
+    else hipLaunchKernelGGL(sample_boundary_kernel<3>, dim3(grid), dim3(block), 0, st, S, n, seed, stream, pw, pp, out);
     return hipGetLastError();
 }
 

@@ -65,9 +65,24 @@ hipError_t launch_eval_weights(int dim, int bie, float dsign, float eps, float f
                                hipStream_t st);
 hipError_t launch_intersect(int dim, const wobdev::DevScene &S, uint32_t n, const float *o, const float *d,
                             uint32_t max_hits, uint32_t *counts, uint32_t *ids, float *ts, hipStream_t st);
+// path < 0: record i uses the stream (seed, path i, point 0, stream); path >= 0: (seed, path,
+// point i, stream)
 hipError_t launch_sample_boundary(int dim, const wobdev::DevScene &S, uint32_t n, uint64_t seed, uint32_t stream,
-                                  float *out, hipStream_t st);
+                                  float *out, hipStream_t st, int64_t path = -1);
 hipError_t launch_philox(uint32_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out, hipStream_t st);
+
+// Virtual-point-light aggregation (wob_vpl.hip).  vpl_plan picks the VPL split count and
+// chunk; the stage kernel writes n * vpl_record_floats records; the aggregate kernel writes
+// splits * n_pts * (dim + 1) doubles of partials that the reduce kernel adds into out.
+uint32_t vpl_record_floats(int dim, int bie);
+void vpl_plan(uint32_t n_pts, uint32_t n_vpl, uint32_t &splits, uint32_t &chunk);
+hipError_t launch_vpl_stage(int dim, int bie, const float *rec_in, const float *contrib, uint32_t n, float inv_pdf,
+                            float dsign, float *out, hipStream_t st);
+hipError_t launch_vpl_aggregate(int dim, int bie, const float *pts, uint32_t n_pts, const float *rec, uint32_t n_vpl,
+                                uint32_t splits, uint32_t chunk, float eps, double *partial, hipStream_t st);
+hipError_t launch_vpl_reduce(const double *partial, uint32_t n_pts, uint32_t V, uint32_t splits, double *out,
+                             hipStream_t st);
+hipError_t launch_f64_to_f32(const double *in, uint32_t n, float *out, hipStream_t st);
 
 // Per (dimension, volume term) entry points of the walk translation units (wob_walk.hip).
 #define WOB_DECLARE_WALK(d, v)                                                                                  \

@@ -18,6 +18,8 @@ Every workload is named by what it is:
   * ``star_exterior_batch_2d`` 64 star polygons, exterior Neumann / Robin / mixed data
   * ``geodesic_mixed_3d``     frequency-100 geodesic sphere (200,000 triangles), displaced,
                               Dirichlet for z > 0, Neumann elsewhere
+  * ``potential_flow_vpl_2d`` Neumann square with four circular obstacles, 256^2 grid
+                              (the virtual-point-light app's scene)
 """
 from __future__ import annotations
 
@@ -334,6 +336,62 @@ def geodesic_mixed_3d(n_points: int = 1_000_000, freq: int = 100, seed: int = 30
                   first_kind_equation_scaling_constant=2.0, num_sample_paths=256)
     return Workload("geodesic_mixed_3d", el, pts.astype(np.float32), exact_expcos_xz_3d, solver,
                     extra=dict(vertices=V, faces=F))
+
+
+# ----------------------------------------------------------------------------------------
+# reference app grids; the virtual-point-light app's scene
+# ----------------------------------------------------------------------------------------
+def grid_points(grid_res: int, dim: int = 2, domain_size: float = 1.0) -> np.ndarray:
+    """Cell-centred grid of the reference apps (utils.cuh:92-99), FP32 arithmetic as there:
+    point idx has coordinates ((idx / grid_res^k) % grid_res + 0.5) * (size / grid_res) - size / 2."""
+    n = grid_res ** dim
+    idx = np.arange(n, dtype=np.int64)
+    dx = np.float32(domain_size) / np.float32(grid_res)
+    out = np.empty((n, dim), np.float32)
+    for k in range(dim):
+        u = (idx // grid_res ** k) % grid_res if k < dim - 1 else idx // grid_res ** k
+        out[:, k] = (u.astype(np.float32) + np.float32(0.5)) * dx - np.float32(domain_size / 2)
+    return out
+
+
+def potential_flow_vpl_2d(grid_res: int = 256, circles: bool = True) -> Workload:
+    """The scene of apps/potential_flow_with_vpl.cu:141-184: the square [-0.5, 0.5]^2 as four
+    Neumann segments with flux -1 (left), +1 (right) and 0 (top, bottom), and four circular
+    obstacles (360, 180, 180 and 90 segments) with zero flux, normals pointing into the
+    obstacles.  Evaluation: the grid_res^2 grid over the square; `extra["mask"]` marks the
+    points outside every obstacle (the reference writes NaN at the others).  Without the
+    obstacles u = x + const, so grad u = (1, 0) is exact."""
+    segs = [((-0.5, -0.5), (-0.5, 0.5), (-1.0, 0.0), -1.0), ((-0.5, 0.5), (0.5, 0.5), (0.0, 1.0), 0.0),
+            ((0.5, 0.5), (0.5, -0.5), (1.0, 0.0), 1.0), ((0.5, -0.5), (-0.5, -0.5), (0.0, -1.0), 0.0)]
+    V, N, B = [], [], []
+    for a, b, nr, g in segs:
+        V.append([a, b]); N.append(nr); B.append([g, g])
+    obstacles = [((0.2, 0.1), 0.25, 360), ((-0.25, 0.3), 0.15, 180), ((-0.20, -0.25), 0.2, 180),
+                 ((0.25, -0.3), 0.05, 90)] if circles else []
+    for (cx, cy), r, m in obstacles:
+        for i in range(m):
+            t0, t1 = 2 * np.pi * i / m, 2 * np.pi * (i + 1) / m
+            v1 = np.array([cx + r * np.cos(t0), cy + r * np.sin(t0)])
+            v2 = np.array([cx + r * np.cos(t1), cy + r * np.sin(t1)])
+            t = (v1 - v2) / np.linalg.norm(v1 - v2)
+            V.append([v1, v2]); N.append([t[1], -t[0]]); B.append([0.0, 0.0])
+    n = len(V)
+    el = Elements(2, np.asarray(V, np.float64).astype(np.float32), np.asarray(N, np.float64).astype(np.float32),
+                  np.asarray(B, np.float64).astype(np.float32), np.full(n, NEUMANN, np.int32),
+                  np.zeros((n, 2), np.float32))
+    pts = grid_points(grid_res, 2, 1.0)
+    mask = np.ones(len(pts), bool)
+    for (cx, cy), r, _ in obstacles:
+        mask &= ((pts[:, 0] - np.float32(cx)) ** 2 + (pts[:, 1] - np.float32(cy)) ** 2) >= np.float32(r) ** 2
+
+    def exact(p):
+        p = np.asarray(p, np.float64)
+        return np.stack([p[:, 0], np.ones(len(p)), np.zeros(len(p))], 1)
+    solver = dict(bie_model="IndirectSingleLayer", domain="InteriorDomain", estimator="BackwardEstimator",
+                  problem="NeumannProblem", path_length=4, num_resampling_candidates=1024, eps=1e-4,
+                  first_kind_equation_scaling_constant=1.0, num_sample_paths=100000)
+    return Workload("potential_flow_vpl_2d", el, pts, None if circles else exact, solver,
+                    dict(mask=mask, grid_res=grid_res, obstacles=obstacles))
 
 
 # ----------------------------------------------------------------------------------------
