@@ -47,7 +47,7 @@ using wobdev::kSL;
 
 constexpr int kBlock = 256;
 constexpr int kPts = 2;   // evaluation points per thread
-constexpr int kGroup = 4; // VPLs whose terms are summed before one compensated addition
+constexpr int kGroup = 8; // VPLs whose terms are summed before one compensated addition
 
 template <int D, int M> constexpr int rec_floats() { return (M == kSL) ? 4 : 8; }
 
@@ -112,7 +112,7 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float t) {
 // One (point, VPL) pair: adds the value and gradient terms into t[0..D].
 template <int D, int M>
 __device__ __forceinline__ void pair_terms(const float *x, const float *q, float eps, float eps2,
-                                           float inv_eps, float *t) {
+                                           float inv_eps, float log2_eps2, float *t) {
     constexpr bool HA = M != kDL, HB = M != kSL;
     float r[D];
 #pragma unroll
@@ -127,18 +127,25 @@ __device__ __forceinline__ void pair_terms(const float *x, const float *q, float
         for (int k = 1; k < D; ++k) rn = __builtin_fmaf(r[k], q[D + k], rn);
     }
     if constexpr (D == 2) {
-        const float rz = z > 0.f ? __builtin_amdgcn_rcpf(z) : 0.f;          // 1/z, 0 at z == 0
-        // 1 / max(z, eps sqrt z) is 1/z unless sqrt z < eps; a wave skips that case unless
-        // one of its lanes is that close to the VPL
+        // common case sqrt z >= eps: 1/z, log2 z and 1 / max(z, eps sqrt z) = 1/z.  A wave takes
+        // the clamped forms only when one of its lanes is that close to the VPL (z == 0
+        // included: there r/|r| = r = 0 and the reference's terms stay finite).
+        float rz = __builtin_amdgcn_rcpf(z);
         float inv1 = rz;
+        float lz = 0.f;
+        if constexpr (HA) lz = __builtin_amdgcn_logf(z);
         const bool near = z < eps2;
         if (__builtin_amdgcn_ballot_w64(near) != 0) {
             asm volatile("");   // keeps the branch (no speculated rsqrt on the common path)
-            if (near) inv1 = z > 0.f ? __builtin_amdgcn_rsqf(z) * inv_eps : 0.f;
+            if (near) {
+                inv1 = z > 0.f ? __builtin_amdgcn_rsqf(z) * inv_eps : 0.f;
+                rz = z > 0.f ? rz : 0.f;
+                lz = log2_eps2;
+            }
         }
         if constexpr (HA) {
             const float aU = M == kSL ? q[2] : q[4], aG = M == kSL ? q[3] : q[5];
-            t[0] = __builtin_fmaf(aU, __builtin_amdgcn_logf(fmaxf(z, eps2)), t[0]);
+            t[0] = __builtin_fmaf(aU, lz, t[0]);
             const float w = aG * inv1;
             t[1] = __builtin_fmaf(w, r[0], t[1]);
             t[2] = __builtin_fmaf(w, r[1], t[2]);
@@ -185,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void vpl_aggregate_kernel(const float *__re
     const uint32_t split = blockIdx.y;
     const uint32_t j0 = split * chunk;
     const uint32_t j1 = min(j0 + chunk, n_vpl);
-    const float eps2 = eps * eps, inv_eps = 1.0f / eps;
+    const float eps2 = eps * eps, inv_eps = 1.0f / eps, log2_eps2 = __builtin_amdgcn_logf(eps2);
     float x[kPts][D], s[kPts][V], c[kPts][V];
     uint32_t idx[kPts];
 #pragma unroll
@@ -214,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void vpl_aggregate_kernel(const float *__re
 #pragma unroll
         for (int g = 0; g < kGroup; ++g)
 #pragma unroll
-            for (int p = 0; p < kPts; ++p) pair_terms<D, M>(x[p], q[g], eps, eps2, inv_eps, t[p]);
+            for (int p = 0; p < kPts; ++p) pair_terms<D, M>(x[p], q[g], eps, eps2, inv_eps, log2_eps2, t[p]);
 #pragma unroll
         for (int p = 0; p < kPts; ++p)
 #pragma unroll
@@ -227,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void vpl_aggregate_kernel(const float *__re
             float t[V];
 #pragma unroll
             for (int v = 0; v < V; ++v) t[v] = 0.f;
-            pair_terms<D, M>(x[p], q, eps, eps2, inv_eps, t);
+            pair_terms<D, M>(x[p], q, eps, eps2, inv_eps, log2_eps2, t);
 #pragma unroll
             for (int v = 0; v < V; ++v) kahan_add(s[p][v], c[p][v], t[v]);
         }
