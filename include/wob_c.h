@@ -229,6 +229,24 @@ wob_status wob_vpl_aggregate(const wob_scene *scene, const wob_estimator_desc *e
                              const float *d_contrib, uint32_t n_vpl, const float *d_points, uint32_t n_points,
                              double *d_sum, void *stream);
 
+/* Closest points (the lbvh nearest query with the point-segment distance of
+ * apps/simple_wob_and_wos.cu:48-79; triangles use the closest point on the triangle).  For
+ * each point i: d_elem[i] = the nearest element (caller order; ties go to the smaller id),
+ * d_dist[i] = its distance, d_value[i] = the boundary value interpolated at the closest point
+ * (simple_wob_and_wos.cu:70-74; barycentric on triangles).  d_points: n * dim floats.      */
+wob_status wob_closest_points(const wob_scene *scene, const float *d_points, uint32_t n, uint32_t *d_elem, float *d_dist,
+                              float *d_value, void *stream);
+/* Walk on spheres for interior Dirichlet problems (WoSFunc, simple_wob_and_wos.cu:116-141):
+ * from x, R = distance to the boundary; while R >= eps, x += R * (uniform direction); then
+ * the path scores the boundary value at the closest point.  Path j of point i uses the Philox
+ * stream (seed, path_base + j, point_base + i, stream_id), one uniform per 2D step and two
+ * per 3D step.  A path still walking after max_steps closest-point queries (0 = 65536)
+ * scores its current closest value (the reference loops without a bound).  d_sum / d_sumsq:
+ * n_points doubles each, accumulated into, as wob_estimate_domain's u column.            */
+wob_status wob_wos_estimate(const wob_scene *scene, const float *d_points, uint32_t n_points, uint32_t point_base,
+                            uint32_t paths_per_point, uint64_t seed, uint64_t path_base, uint32_t stream_id, float eps,
+                            uint32_t max_steps, double *d_sum, double *d_sumsq, void *stream);
+
 /* Batched launches (no counterpart in the reference, whose apps run one scene per
  * thrust::for_each): many (scene, estimator, point set) entries of uneven sizes run in ONE
  * walk launch and ONE reduction launch per pass.  All entries must share the dimension and

@@ -1515,6 +1515,121 @@ int orc_vpl_aggregate(int dim, int bie, int domain, float eps, float inv_pdf, in
     return 0;
 }
 
+}  // extern "C"
+
+// ---- closest points and walk on spheres (apps/simple_wob_and_wos.cu:48-79, 116-141) ----
+// Brute force over the elements in caller order: the first element with the smallest squared
+// distance wins.  Segment distance: the app's LineSqrD formula; triangle distance: the closest
+// point on the triangle by Voronoi regions (Ericson, Real-Time Collision Detection, 5.1.5).
+static float seg_dist2(const float *p, const Element<2> &e, float &t) {
+    const float l0 = e.v[1][0] - e.v[0][0], l1 = e.v[1][1] - e.v[0][1];
+    const float q0 = p[0] - e.v[0][0], q1 = p[1] - e.v[0][1];
+    t = (q0 * l0 + q1 * l1) / (l0 * l0 + l1 * l1);
+    t = std::min(std::max(t, 0.0f), 1.0f);
+    const float e0 = (e.v[0][0] + l0 * t) - p[0], e1 = (e.v[0][1] + l1 * t) - p[1];
+    return e0 * e0 + e1 * e1;
+}
+
+static float tri_dist2(const float *p, const Element<3> &e, float *w) {
+    const float *a = &e.v[0][0], *b = &e.v[1][0], *c = &e.v[2][0];
+    float ab[3], ac[3], ap[3], bp[3], cp[3];
+    for (int k = 0; k < 3; ++k) { ab[k] = b[k] - a[k]; ac[k] = c[k] - a[k]; ap[k] = p[k] - a[k]; bp[k] = p[k] - b[k]; cp[k] = p[k] - c[k]; }
+    auto dot3 = [](const float *x, const float *y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+    const float d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+    float wa, wb, wc;
+    if (d1 <= 0.f && d2 <= 0.f) { wa = 1.f; wb = 0.f; wc = 0.f; }
+    else {
+        const float d3 = dot3(ab, bp), d4 = dot3(ac, bp), d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+        const float vc = d1 * d4 - d3 * d2, vb = d5 * d2 - d1 * d6, va = d3 * d6 - d5 * d4;
+        if (d3 >= 0.f && d4 <= d3) { wa = 0.f; wb = 1.f; wc = 0.f; }
+        else if (d6 >= 0.f && d5 <= d6) { wa = 0.f; wb = 0.f; wc = 1.f; }
+        else if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f) { const float s = d1 / (d1 - d3); wa = 1.f - s; wb = s; wc = 0.f; }
+        else if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f) { const float s = d2 / (d2 - d6); wa = 1.f - s; wb = 0.f; wc = s; }
+        else if (va <= 0.f && (d4 - d3) >= 0.f && (d5 - d6) >= 0.f) {
+            const float s = (d4 - d3) / ((d4 - d3) + (d5 - d6)); wa = 0.f; wb = 1.f - s; wc = s;
+        } else { const float den = 1.f / (va + vb + vc); wb = vb * den; wc = vc * den; wa = (1.f - wb) - wc; }
+    }
+    w[0] = wa; w[1] = wb; w[2] = wc;
+    float e2 = 0.f;
+    for (int k = 0; k < 3; ++k) { const float q = ((wa * a[k] + wb * b[k]) + wc * c[k]) - p[k]; e2 = e2 + q * q; }
+    return e2;
+}
+
+template <int D> static float closest(const Scene<D> &s, const float *x, int &elem, float &value) {
+    float best = std::numeric_limits<float>::infinity();
+    elem = -1;
+    float bw[3] = {0, 0, 0};
+    for (size_t i = 0; i < s.elems.size(); ++i) {
+        float w[3] = {0, 0, 0}, d2;
+        if constexpr (D == 2) d2 = seg_dist2(x, s.elems[i], w[0]);
+        else d2 = tri_dist2(x, s.elems[i], w);
+        if (d2 < best) { best = d2; elem = (int)i; bw[0] = w[0]; bw[1] = w[1]; bw[2] = w[2]; }
+    }
+    const Element<D> &e = s.elems[elem];
+    if constexpr (D == 2) value = (e.bv[1] - e.bv[0]) * bw[0] + e.bv[0];
+    else value = (bw[0] * e.bv[0] + bw[1] * e.bv[1]) + bw[2] * e.bv[2];
+    return best;
+}
+
+extern "C" {
+
+int orc_closest_points(void *p, int n, const float *pts, int32_t *elem, float *dist, float *value) {
+    auto *S = (OrcScene *)p;
+    for (int i = 0; i < n; ++i) {
+        int e; float v;
+        const float d2 = S->dim == 2 ? closest(S->s2, pts + (size_t)i * 2, e, v) : closest(S->s3, pts + (size_t)i * 3, e, v);
+        elem[i] = e; dist[i] = std::sqrt(d2); value[i] = v;
+    }
+    return 0;
+}
+
+// Walk on spheres with the device's streams: path j of point i uses (seed, path_base + j,
+// i, stream); a 2D step draws u and moves by R (cos 2 pi u, sin 2 pi u); a 3D step draws z =
+// 1 - 2u, then phi = 2 pi u'.  sum / sumsq: n doubles, accumulated into; path_out optional.
+int orc_wos(void *p, const float *pts, int n, unsigned paths, uint64_t seed, uint64_t path_base, unsigned stream,
+            float eps, unsigned max_steps, int threads, double *sum, double *sumsq, float *path_out) {
+    auto *S = (OrcScene *)p;
+    auto run = [&](auto &s, auto dimtag) {
+        constexpr int D = decltype(dimtag)::value;
+#ifdef _OPENMP
+        if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(static)
+#endif
+        for (int i = 0; i < n; ++i) {
+            Kahan ks, kq;
+            for (unsigned j = 0; j < paths; ++j) {
+                Rng rng; rng.init_philox(seed, (uint32_t)(path_base + j), (uint32_t)i, stream);
+                float x[3] = {0, 0, 0};
+                for (int k = 0; k < D; ++k) x[k] = pts[(size_t)i * D + k];
+                float val = 0.f;
+                for (unsigned step = 0;; ++step) {
+                    int e; float v;
+                    const float R = std::sqrt(closest(s, x, e, v));
+                    if (R < eps || step + 1 >= max_steps) { val = v; break; }
+                    if constexpr (D == 2) {
+                        const float th = 2.0f * rng.uniform();
+                        x[0] = x[0] + R * (float)std::cos(th * M_PI);
+                        x[1] = x[1] + R * (float)std::sin(th * M_PI);
+                    } else {
+                        const float z = 1.0f - 2.0f * rng.uniform();
+                        const float ph = 2.0f * rng.uniform();
+                        const float r = std::sqrt(std::max(1.0f - z * z, 0.0f));
+                        x[0] = x[0] + R * (r * (float)std::cos(ph * M_PI));
+                        x[1] = x[1] + R * (r * (float)std::sin(ph * M_PI));
+                        x[2] = x[2] + R * z;
+                    }
+                }
+                ks.add(val); kq.add(val * val);
+                if (path_out) path_out[(size_t)i * paths + j] = val;
+            }
+            sum[i] += ks.sum; sumsq[i] += kq.sum;
+        }
+    };
+    if (S->dim == 2) run(S->s2, std::integral_constant<int, 2>{});
+    else run(S->s3, std::integral_constant<int, 3>{});
+    return 0;
+}
+
 // Line-sampling helper for tests: one sample_boundary_line_intersection call from x with a
 // Philox stream (seed, path, point, stream).  out_rec: 2D+4 floats, returns inv_pdf.
 float orc_sample_line(void *p, const float *x, uint64_t seed, uint32_t path, uint32_t point, uint32_t stream,

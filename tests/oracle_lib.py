@@ -70,6 +70,9 @@ def lib():
         L.orc_sample_line.restype = C.c_float
         L.orc_sample_uniform.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, P]
         L.orc_sample_uniform.restype = C.c_float
+        L.orc_closest_points.argtypes = [P, C.c_int, P, P, P, P]
+        L.orc_wos.argtypes = [P, P, C.c_int, C.c_uint, C.c_uint64, C.c_uint64, C.c_uint, C.c_float, C.c_uint, C.c_int,
+                              P, P, P]
         L.orc_vpl_aggregate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, P, P, C.c_int, P,
                                         C.c_int, P, P]
         _lib = L
@@ -171,6 +174,28 @@ class OracleScene:
         rec = np.zeros(2 * self.dim + 4, np.float32)
         inv = lib().orc_sample_uniform(self.h, seed, path, point, stream, _p(rec))
         return rec, inv
+
+    def closest_points(self, points):
+        """Brute-force nearest element, distance and interpolated boundary value per point."""
+        x = np.ascontiguousarray(points, np.float32)
+        n = x.shape[0]
+        e = np.zeros(n, np.int32)
+        d = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        lib().orc_closest_points(self.h, n, _p(x), _p(e), _p(d), _p(v))
+        return e, d, v
+
+    def wos(self, points, paths, eps, seed=0, path_base=0, stream=0, max_steps=1 << 16, threads=0,
+            return_paths=False):
+        """Walk on spheres with the device's Philox streams: (sum, sumsq[, per-path values])."""
+        x = np.ascontiguousarray(points, np.float32)
+        n = x.shape[0]
+        s = np.zeros(n, np.float64)
+        q = np.zeros(n, np.float64)
+        po = np.zeros((n, paths), np.float32) if return_paths else None
+        lib().orc_wos(self.h, _p(x), n, paths, seed, path_base, stream, float(eps), max_steps, threads, _p(s), _p(q),
+                      _p(po))
+        return (s, q, po) if return_paths else (s, q)
 
     def vpl_paths(self, solver, n, seed=0, path_base=0, stream=0, threads=0):
         """The VPL path phase with the device's streams: start point i from (seed, path_base, i,

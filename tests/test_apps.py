@@ -159,3 +159,48 @@ def test_vpl_app_iterations(tmp_path, gpu):
     c = G.grid_points(64).reshape(64, 64, 2)
     free = mask & (np.abs(c[..., 1]) < 0.45) & (c[..., 0] < -0.4)
     assert 0.5 < g[free][:, 0].mean() < 2.0
+
+
+def _wos_config(tmp_path, obj, **kw):
+    cfg = {"input_obj_file": str(obj), "output_dir": str(tmp_path / "wos"), "path_length": 4,
+           "num_sample_paths": 64, "is_wos": True, "max_time": 1, "grid_res": 32, "eps": 1e-4}
+    cfg.update(kw)
+    p = tmp_path / "wos.json"
+    p.write_text(json.dumps(cfg, indent=4))
+    return p
+
+
+def test_wos_app_dry_run(tmp_path):
+    """apps/simple_wob_and_wos.cu's mask: the grid points inside the (clockwise) polyline."""
+    obj = tmp_path / "circle.obj"
+    _write_circle_obj(obj)
+    out = _run(B.WOS_APP, _wos_config(tmp_path, obj), "--dry-run")
+    assert "#line segments: 128" in out and "is WoS\t: 1" in out
+    x = (np.arange(32) + 0.5) / 32 - 0.5
+    X, Y = np.meshgrid(x, x)
+    inside = (X ** 2 + Y ** 2 < 0.45 ** 2 * np.cos(np.pi / 128) ** 2 - 1e-6).sum()
+    edge = ((X ** 2 + Y ** 2 >= 0.45 ** 2 * np.cos(np.pi / 128) ** 2 - 1e-6) & (X ** 2 + Y ** 2 <= 0.45 ** 2 + 1e-6)).sum()
+    n = int(out.split("evaluation points\t: ")[1].split(" ")[0])
+    assert inside <= n <= inside + edge
+    out = _run(B.WOS_APP, _wos_config(tmp_path, obj, evaluate_at_all_points=True), "--dry-run")
+    assert "evaluation points\t: 1024 of 1024" in out
+
+
+@pytest.mark.gpu
+def test_wos_app_both_modes_agree(tmp_path, gpu):
+    obj = tmp_path / "circle.obj"
+    _write_circle_obj(obj)
+    _run(B.WOS_APP, _wos_config(tmp_path, obj, num_sample_paths=512), "--max-iterations", "2")
+    _, u_wos = _read_field(tmp_path / "wos" / "potential2.scalar")
+    (tmp_path / "wob").mkdir()
+    _run(B.WOS_APP, _wos_config(tmp_path, obj, is_wos=False, output_dir=str(tmp_path / "wob"),
+                                num_sample_paths=512), "--max-iterations", "2")
+    _, u_wob = _read_field(tmp_path / "wob" / "potential2.scalar")
+    m = np.isfinite(u_wos)
+    assert np.array_equal(m, np.isfinite(u_wob)) and m.sum() > 500
+    # +-1 data in 40 arc-length pieces: walk-on-spheres estimates lie in [-1, 1]; the two
+    # modes estimate the same field (their agreement within standard errors is tested in
+    # test_gpu_wos.py; here 1024 paths per point leave noise of about 0.1 in the WoB field)
+    assert np.all(np.abs(u_wos[m]) <= 1.0 + 1e-6)
+    assert np.corrcoef(u_wos[m], u_wob[m])[0, 1] > 0.8
+    assert np.mean(np.abs(u_wos[m] - u_wob[m])) < 0.15

@@ -97,6 +97,35 @@ class SceneHost:
         check(lib().wob_scene_get_info(self._h, C.byref(i)))
         return {f: getattr(i, f) for f, _ in i._fields_}
 
+    def closest_points(self, points, stream=None):
+        """Nearest element (int32, caller order), distance and boundary value at the closest
+        point for device points (n, dim) (apps/simple_wob_and_wos.cu:48-79)."""
+        torch = _torch()
+        pts = points.contiguous()
+        n = pts.shape[0]
+        e = torch.empty(n, dtype=torch.int32, device=pts.device)
+        d = torch.empty(n, dtype=torch.float32, device=pts.device)
+        v = torch.empty(n, dtype=torch.float32, device=pts.device)
+        check(lib().wob_closest_points(self._h, _ptr(pts), n, _ptr(e), _ptr(d), _ptr(v), _stream_ptr(stream)))
+        return e, d, v
+
+    def walk_on_spheres(self, points, paths_per_point, eps=1e-4, seed=0, path_base=0, point_base=0, stream_id=0,
+                        max_steps=0, out=None, stream=None):
+        """Walk-on-spheres path sums of an interior Dirichlet problem (WoSFunc,
+        apps/simple_wob_and_wos.cu:116-141): (sum, sumsq) FP64 tensors of shape (n,),
+        accumulated into `out` if given."""
+        torch = _torch()
+        pts = points.contiguous()
+        n = pts.shape[0]
+        if out is None:
+            s = torch.zeros(n, dtype=torch.float64, device=pts.device)
+            q = torch.zeros(n, dtype=torch.float64, device=pts.device)
+        else:
+            s, q = out
+        check(lib().wob_wos_estimate(self._h, _ptr(pts), n, point_base, paths_per_point, seed, path_base, stream_id,
+                                     float(eps), max_steps, _ptr(s), _ptr(q), _stream_ptr(stream)))
+        return s, q
+
     def set_volume_cache(self, grid_res, domain_size, cache):
         """Grid source over (-domain_size/2, domain_size/2)^dim (SceneHost::set_volume_cache,
         scene.cuh:301-306); cache: grid_res^dim values, x fastest.  None removes the source."""

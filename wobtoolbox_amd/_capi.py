@@ -22,7 +22,8 @@ EXPORTS = ["wob_scene_create", "wob_scene_destroy", "wob_scene_get_info", "wob_s
            "wob_last_error_message", "wob_version", "wob_batch_create", "wob_batch_run", "wob_batch_tune",
            "wob_batch_destroy", "wob_cv_create", "wob_cv_destroy", "wob_cv_get_info", "wob_cv_get_params",
            "wob_cv_set_params", "wob_cv_train", "wob_cv_gradient", "wob_cv_infer", "wob_scene_set_control_variate",
-           "wob_sample_boundary", "wob_vpl_paths", "wob_vpl_aggregate"]
+           "wob_sample_boundary", "wob_vpl_paths", "wob_vpl_aggregate",
+           "wob_closest_points", "wob_wos_estimate"]
 
 
 class WobError(RuntimeError):
@@ -127,6 +128,8 @@ def lib():
         "wob_vpl_paths": ([P, C.POINTER(EstimatorDesc), U32, U64, U64, U32, P, P, C.POINTER(LaunchConfig), P],
                           C.c_int),
         "wob_vpl_aggregate": ([P, C.POINTER(EstimatorDesc), P, P, U32, P, U32, P, P], C.c_int),
+        "wob_closest_points": ([P, P, U32, P, P, P, P], C.c_int),
+        "wob_wos_estimate": ([P, P, U32, U32, U32, U64, U64, U32, F, U32, P, P, P], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

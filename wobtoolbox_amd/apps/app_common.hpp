@@ -5,7 +5,9 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cctype>
+#include <cmath>
 #include <fstream>
+#include <sstream>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -96,5 +98,48 @@ template <typename T> void save_field(const std::string &path, unsigned res, con
         hipError_t e_ = (x);                                                                     \
         if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// ---------------------------------------------------------------------------------------
+// OBJ: "v x y z" and "f i j [k]" records (1-based, "i/t/n" accepted, negative = relative)
+// ---------------------------------------------------------------------------------------
+inline void read_obj(const std::string &path, std::vector<double> &V, std::vector<int> &F, int &verts_per_face) {
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("cannot open mesh " + path);
+    std::string line;
+    verts_per_face = 0;
+    while (std::getline(in, line)) {
+        std::istringstream ss(line);
+        std::string tag;
+        ss >> tag;
+        if (tag == "v") {
+            double x = 0, y = 0, z = 0;
+            ss >> x >> y >> z;
+            V.insert(V.end(), {x, y, z});
+        } else if (tag == "f" || tag == "l") {
+            std::vector<int> f;
+            std::string tok;
+            while (ss >> tok) {
+                int k = std::stoi(tok.substr(0, tok.find('/')));
+                k = k < 0 ? (int)(V.size() / 3) + k : k - 1;
+                f.push_back(k);
+            }
+            if (verts_per_face == 0) verts_per_face = (int)f.size();
+            if ((int)f.size() != verts_per_face) throw std::runtime_error("mixed face sizes in " + path);
+            F.insert(F.end(), f.begin(), f.end());
+        }
+    }
+}
+
+// winding number of a closed 2D polyline (segments F[2i], F[2i+1] of V, 3 doubles per vertex)
+// around x: the sum of the signed angles the segments subtend, over 2 pi
+inline double winding_2d(const std::vector<double> &V, const std::vector<int> &F, const double *x) {
+    double w = 0.0;
+    for (size_t f = 0; f + 1 < F.size(); f += 2) {
+        const double ax = V[3 * F[f]] - x[0], ay = V[3 * F[f] + 1] - x[1];
+        const double bx = V[3 * F[f + 1]] - x[0], by = V[3 * F[f + 1] + 1] - x[1];
+        w += std::atan2(ax * by - ay * bx, ax * bx + ay * by);
+    }
+    return w / (2.0 * M_PI);
+}
 
 }  // namespace wobapp

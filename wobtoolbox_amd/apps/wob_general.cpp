@@ -52,36 +52,6 @@ namespace {
 
 using namespace wobapp;
 
-// ---------------------------------------------------------------------------------------
-// OBJ: "v x y z" and "f i j [k]" records (1-based, "i/t/n" accepted, negative = relative)
-// ---------------------------------------------------------------------------------------
-void read_obj(const std::string &path, std::vector<double> &V, std::vector<int> &F, int &verts_per_face) {
-    std::ifstream in(path);
-    if (!in) throw std::runtime_error("cannot open mesh " + path);
-    std::string line;
-    verts_per_face = 0;
-    while (std::getline(in, line)) {
-        std::istringstream ss(line);
-        std::string tag;
-        ss >> tag;
-        if (tag == "v") {
-            double x = 0, y = 0, z = 0;
-            ss >> x >> y >> z;
-            V.insert(V.end(), {x, y, z});
-        } else if (tag == "f" || tag == "l") {
-            std::vector<int> f;
-            std::string tok;
-            while (ss >> tok) {
-                int k = std::stoi(tok.substr(0, tok.find('/')));
-                k = k < 0 ? (int)(V.size() / 3) + k : k - 1;
-                f.push_back(k);
-            }
-            if (verts_per_face == 0) verts_per_face = (int)f.size();
-            if ((int)f.size() != verts_per_face) throw std::runtime_error("mixed face sizes in " + path);
-            F.insert(F.end(), f.begin(), f.end());
-        }
-    }
-}
 
 // ---------------------------------------------------------------------------------------
 // the apps' analytic solutions: [u, grad u]
@@ -118,12 +88,7 @@ std::array<double, D + 1> analytic(SolutionType st, const double *y) { return an
 double winding(const std::vector<double> &V, const std::vector<int> &F, const double *x) {
     double w = 0.0;
     if constexpr (D == 2) {
-        for (size_t f = 0; f < F.size(); f += 2) {
-            const double ax = V[3 * F[f]] - x[0], ay = V[3 * F[f] + 1] - x[1];
-            const double bx = V[3 * F[f + 1]] - x[0], by = V[3 * F[f + 1] + 1] - x[1];
-            w += std::atan2(ax * by - ay * bx, ax * bx + ay * by);
-        }
-        return w / (2.0 * M_PI);
+        return winding_2d(V, F, x);
     } else {
         for (size_t f = 0; f < F.size(); f += 3) {
             double a[3], b[3], c[3];

@@ -30,6 +30,7 @@ SOURCES = [
     ("wob_walk.hip", "hip", ["-DWOB_WALK_DIM=3", "-DWOB_WALK_VOL=1"], "wob_walk_31.o"),
     ("wob_kernels.hip", "hip"),
     ("wob_vpl.hip", "hip"),
+    ("wob_wos.hip", "hip"),
     ("mlp_kernels.hip", "hip"),
     ("wob_capi.cpp", "hip"),
     ("wob_cv.cpp", "hip"),
@@ -99,12 +100,15 @@ def _build_shim_check():
 
 APPS = {2: os.path.join(LIBDIR, "wob_general_2d"), 3: os.path.join(LIBDIR, "wob_general_3d")}
 VPL_APP = os.path.join(LIBDIR, "wob_vpl")
+WOS_APP = os.path.join(LIBDIR, "wob_wos")
 
 
 def _build_apps():
-    """The reference apps' runners: apps/wob_general.cpp for 2D and 3D, apps/wob_vpl.cpp."""
+    """The reference apps' runners: apps/wob_general.cpp for 2D and 3D, apps/wob_vpl.cpp and
+    apps/wob_wos.cpp."""
     jobs = [(os.path.join(PKG, "apps", "wob_general.cpp"), out, [f"-DWOB_APP_DIM={dim}"]) for dim, out in APPS.items()]
     jobs.append((os.path.join(PKG, "apps", "wob_vpl.cpp"), VPL_APP, []))
+    jobs.append((os.path.join(PKG, "apps", "wob_wos.cpp"), WOS_APP, []))
     procs = []
     for src, out, defs in jobs:
         cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", *defs,

@@ -734,6 +734,53 @@ wob_status wob_vpl_aggregate(const wob_scene *cscene, const wob_estimator_desc *
     return WOB_OK;
 }
 
+wob_status wob_closest_points(const wob_scene *scene, const float *d_points, uint32_t n, uint32_t *d_elem, float *d_dist,
+                              float *d_value, void *stream) {
+    if (!scene) return fail(WOB_INVALID_ARG, "null scene");
+    if (n == 0) return WOB_OK;
+    if (!d_points || !d_elem || !d_dist || !d_value) return fail(WOB_INVALID_ARG, "null device buffer");
+    WOB_HIP(wobl::launch_closest((int)scene->host.dim, scene->dev, n, d_points, d_elem, d_dist, d_value,
+                                 (hipStream_t)stream),
+            "closest-point launch");
+    return WOB_OK;
+}
+
+wob_status wob_wos_estimate(const wob_scene *cscene, const float *d_points, uint32_t n_points, uint32_t point_base,
+                            uint32_t paths_per_point, uint64_t seed, uint64_t path_base, uint32_t stream_id, float eps,
+                            uint32_t max_steps, double *d_sum, double *d_sumsq, void *stream) {
+    if (!cscene) return fail(WOB_INVALID_ARG, "null scene");
+    if (!(eps > 0.f)) return fail(WOB_INVALID_ARG, "eps must be > 0");
+    if (n_points == 0 || paths_per_point == 0) return WOB_OK;
+    if (!d_points || !d_sum || !d_sumsq) return fail(WOB_INVALID_ARG, "null device buffer");
+    wob_scene *scene = const_cast<wob_scene *>(cscene);
+    const uint32_t ppl = auto_paths_per_lane((double)n_points * paths_per_point);
+    wobl::WalkLaunch L{};
+    wobl::WalkEntry &e = L.e0;
+    e.scene = scene->dev;
+    e.points = d_points;
+    e.sum = d_sum;
+    e.sumsq = d_sumsq;
+    e.n_points = n_points;
+    chunking(paths_per_point, ppl, e.paths_per_chunk, e.chunks);
+    e.paths_per_point = paths_per_point;
+    e.point_base = point_base;
+    e.stream = stream_id;
+    e.path_base = path_base;
+    const uint64_t n_items = (uint64_t)n_points * e.chunks;
+    if (n_items >= (1ull << 31)) return fail(WOB_INVALID_ARG, "too many work items; split the point set");
+    wob_status st = ensure_scratch(scene, (size_t)2 * n_items * sizeof(float));
+    if (st != WOB_OK) return st;
+    L.n_entries = 1;
+    L.total_items = (uint32_t)n_items;
+    L.total_points = n_points;
+    L.seed = seed;
+    L.partials = scene->scratch;
+    hipStream_t hs = (hipStream_t)stream;
+    WOB_HIP(wobl::launch_wos((int)scene->host.dim, L, eps, max_steps ? max_steps : 1u << 16, hs), "WoS launch");
+    WOB_HIP(wobl::launch_reduce(L, 1u, hs), "reduce launch");
+    return WOB_OK;
+}
+
 wob_status wob_philox4x32_10(uint32_t n, const uint32_t *d_ctr, const uint32_t *d_key, uint32_t *d_out, void *stream) {
     if (n == 0) return WOB_OK;
     if (!d_ctr || !d_key || !d_out) return fail(WOB_INVALID_ARG, "null device buffer");

@@ -84,6 +84,14 @@ hipError_t launch_vpl_reduce(const double *partial, uint32_t n_pts, uint32_t V, 
                              hipStream_t st);
 hipError_t launch_f64_to_f32(const double *in, uint32_t n, float *out, hipStream_t st);
 
+// Closest-point queries and walk on spheres (wob_wos.hip).  The WoS launch uses e0 of the
+// launch (points, n_points, chunks, paths_per_chunk, paths_per_point, point_base, stream,
+// path_base) and writes 2 * total_items partial floats for reduce_kernel with V = 1.
+size_t closest_lds_bytes(uint32_t block);
+hipError_t launch_closest(int dim, const wobdev::DevScene &S, uint32_t n, const float *pts, uint32_t *elem, float *dist,
+                          float *value, hipStream_t st);
+hipError_t launch_wos(int dim, const WalkLaunch &L, float eps, uint32_t max_steps, hipStream_t st);
+
 // Per (dimension, volume term) entry points of the walk translation units (wob_walk.hip).
 #define WOB_DECLARE_WALK(d, v)                                                                                  \
     hipError_t launch_walk_##d##v(bool boundary, const WalkLaunch &a, int bie, int est, int flavour, uint32_t block, \
