@@ -594,7 +594,7 @@ template <int D> static Sampled<D> sample_boundary_value(const Scene<D> &s, Rng 
 
 // scene.cuh:464-535 (2D) and 760-840 (3D)
 template <int D> static Sampled<D> sample_line_intersection(const Scene<D> &s, const Vec<D> &x, Rng &rng, const Options &opt,
-                                                            int origin_elem = -1) {
+                                                            int origin_elem = -1, unsigned *count_out = nullptr) {
     const int skip = opt.exclude_origin ? origin_elem : -1;
     Vec<D> dir;
     if constexpr (D == 2) {
@@ -634,6 +634,7 @@ template <int D> static Sampled<D> sample_line_intersection(const Scene<D> &s, c
             obj = buf[k].obj; p = buf[k].p;
         }
     }
+    if (count_out) *count_out = count;
     if (count == 0) {
         // scene.cuh:488-495: default point, zero inverse pdf
         BPoint<D> r = BPoint<D>::zero();
@@ -1623,6 +1624,47 @@ int orc_wos(void *p, const float *pts, int n, unsigned paths, uint64_t seed, uin
                 if (path_out) path_out[(size_t)i * paths + j] = val;
             }
             sum[i] += ks.sum; sumsq[i] += kq.sum;
+        }
+    };
+    if (S->dim == 2) run(S->s2, std::integral_constant<int, 2>{});
+    else run(S->s3, std::integral_constant<int, 3>{});
+    return 0;
+}
+
+// WoBFunc (apps/simple_wob_and_wos.cu:81-113), the app's hand-written double-layer walk for
+// interior Dirichlet problems: weight m * sign((y - x).n_y) from the first line sample, then
+// path_length - 1 bounces multiplying -m * sign((y - y_prev).n_y), the last one halved; each
+// visited point adds weight * g(y).  Same draws as the estimators (sample_boundary_line_
+// intersection2 per bounce) on the Philox stream (seed, path_base + j, point i, stream).
+// sum: n doubles accumulated into; path_out (optional): n * paths floats.
+int orc_simple_wob(void *p, const float *pts, int n, unsigned paths, unsigned path_length, uint64_t seed,
+                   uint64_t path_base, unsigned stream, int pick_mode, double *sum, float *path_out) {
+    auto *S = (OrcScene *)p;
+    auto run = [&](auto &s, auto dimtag) {
+        constexpr int D = decltype(dimtag)::value;
+        Options opt; opt.pick_mode = pick_mode; opt.use_bvh = true;
+        auto sgn = [](float v) { return v > 0 ? 1.0f : -1.0f; };
+        for (int i = 0; i < n; ++i) {
+            Kahan ks;
+            Vec<D> x;
+            for (int k = 0; k < D; ++k) x[k] = pts[(size_t)i * D + k];
+            for (unsigned j = 0; j < paths; ++j) {
+                Rng rng; rng.init_philox(seed, (uint32_t)(path_base + j), (uint32_t)i, stream);
+                unsigned m = 0;
+                Sampled<D> y = sample_line_intersection(s, x, rng, opt, -1, &m);
+                float weight = (float)m * sgn(dot(y.y.p - x, y.y.n));
+                float contrib = weight * y.y.boundary_value;
+                for (unsigned b = 1; b < path_length; ++b) {
+                    if (b == path_length - 1) weight *= 0.5f;
+                    const Vec<D> yp = y.y.p;
+                    y = sample_line_intersection(s, yp, rng, opt, y.y.elem, &m);
+                    weight *= -(float)m * sgn(dot(y.y.p - yp, y.y.n));
+                    contrib += weight * y.y.boundary_value;
+                }
+                ks.add(contrib);
+                if (path_out) path_out[(size_t)i * paths + j] = contrib;
+            }
+            sum[i] += ks.sum;
         }
     };
     if (S->dim == 2) run(S->s2, std::integral_constant<int, 2>{});

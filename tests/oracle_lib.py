@@ -71,6 +71,8 @@ def lib():
         L.orc_sample_uniform.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, P]
         L.orc_sample_uniform.restype = C.c_float
         L.orc_closest_points.argtypes = [P, C.c_int, P, P, P, P]
+        L.orc_simple_wob.argtypes = [P, P, C.c_int, C.c_uint, C.c_uint, C.c_uint64, C.c_uint64, C.c_uint, C.c_int, P,
+                                     P]
         L.orc_wos.argtypes = [P, P, C.c_int, C.c_uint, C.c_uint64, C.c_uint64, C.c_uint, C.c_float, C.c_uint, C.c_int,
                               P, P, P]
         L.orc_vpl_aggregate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, P, P, C.c_int, P,
@@ -196,6 +198,15 @@ class OracleScene:
         lib().orc_wos(self.h, _p(x), n, paths, seed, path_base, stream, float(eps), max_steps, threads, _p(s), _p(q),
                       _p(po))
         return (s, q, po) if return_paths else (s, q)
+
+    def simple_wob(self, points, paths, path_length, seed=0, path_base=0, stream=0, pick_mode=1):
+        """The app's double-layer walk (WoBFunc): (sum, per-path values)."""
+        x = np.ascontiguousarray(points, np.float32)
+        n = x.shape[0]
+        s = np.zeros(n, np.float64)
+        po = np.zeros((n, paths), np.float32)
+        lib().orc_simple_wob(self.h, _p(x), n, paths, path_length, seed, path_base, stream, pick_mode, _p(s), _p(po))
+        return s, po
 
     def vpl_paths(self, solver, n, seed=0, path_base=0, stream=0, threads=0):
         """The VPL path phase with the device's streams: start point i from (seed, path_base, i,

@@ -13,8 +13,9 @@
 //   * is_wos: walk on spheres (WoSFunc, :116-141) -> wob_wos_estimate;
 //   * otherwise the double-layer walk on boundary of WoBFunc (:83-113): a line sample from x,
 //     then path_length - 1 more bounces, contributions m sign(...) g(y) with the last one
-//     halved.  That is the estimator of the general apps' interior Dirichlet double layer with
-//     the same path length (estimator.cuh:154-191, 232-263), so it runs as
+//     halved.  Path by path that is the estimator of the general apps' interior Dirichlet
+//     double layer with the same path length (estimator.cuh:154-191, 232-263; for path_length
+//     1 twice it, as WoBFunc's halving sits in its bounce loop), so it runs as
 //     Estimator<..., IndirectDoubleLayer, InteriorDomain, BackwardEstimator, DirichletProblem>.
 // After iteration i it writes potential<i>.scalar in output_dir (uint32 grid_res, then
 // grid_res^2 floats, NaN at points not evaluated; runtime_utils.cuh:35-57) and a copy of the
@@ -183,8 +184,11 @@ int run_app(int argc, char **argv) {
         std::cout << "\33[2K\rtime " << elapsed << " s\t" << it << " iterations (" << total << " samples)\t"
                   << std::flush;
         HIP_OK(hipMemcpy(acc.data(), d_acc, acc.size() * sizeof(double), hipMemcpyDeviceToHost));
+        // with path_length 1 WoBFunc's single term is not halved (its halving sits in the bounce
+        // loop), while the estimator halves its last term: scale by 2
+        const double scale = (!is_wos && L == 1) ? 2.0 : 1.0;
         std::vector<float> sol(npts, nan);
-        for (uint32_t j = 0; j < m; ++j) sol[idx_of[j]] = (float)(acc[(size_t)j * V1] / (double)total);
+        for (uint32_t j = 0; j < m; ++j) sol[idx_of[j]] = (float)(scale * acc[(size_t)j * V1] / (double)total);
         save_field(out_dir + "/potential" + std::to_string(it) + ".scalar", res, sol);
     }
     std::cout << std::endl;
