@@ -70,6 +70,20 @@ struct DevScene {
     VolSrc vol;
 };
 
+// Scene pointers reach a kernel either in its argument or through an entry table in global
+// memory; where both are possible the compiler cannot tell and addresses them with FLAT
+// instructions (which also wait on LDS traffic).  They all point to device global memory, and
+// the hot loads below say so (address space 1), so they compile to GLOBAL loads.
+typedef float gf4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 gld4(const float4 *p) {
+    const gf4_t v = *(const __attribute__((address_space(1))) gf4_t *)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float gldf(const float *p) { return *(const __attribute__((address_space(1))) float *)p; }
+__device__ __forceinline__ uint32_t gldu(const uint32_t *p) {
+    return *(const __attribute__((address_space(1))) uint32_t *)p;
+}
+
 // ----------------------------------------------------------------------------------------
 // RNG
 // ----------------------------------------------------------------------------------------
@@ -250,9 +264,9 @@ __device__ __forceinline__ uint32_t cdf_lookup(const float *__restrict__ cdf, co
                                                uint32_t n, float u) {
     uint32_t k = (uint32_t)(u * (float)n);
     k = k < n ? k : n - 1;
-    uint32_t i = guide[k];
-    while (i + 1 < n && u > cdf[i]) ++i;
-    while (i > 0 && u <= cdf[i - 1]) --i;
+    uint32_t i = gldu(guide + k);
+    while (i + 1 < n && u > gldf(cdf + i)) ++i;
+    while (i > 0 && u <= gldf(cdf + i - 1)) --i;
     return i;
 }
 
@@ -261,7 +275,7 @@ template <int D> __device__ __forceinline__ void load_attr(const DevScene &S, ui
                                                           float *bvv, float *alv, float &area) {
     b.elem = e;
     if constexpr (D == 2) {
-        const float4 f0 = S.attr[3 * e], f1 = S.attr[3 * e + 1], f2 = S.attr[3 * e + 2];
+        const float4 f0 = gld4(S.attr + 3 * e), f1 = gld4(S.attr + 3 * e + 1), f2 = gld4(S.attr + 3 * e + 2);
         v[0] = f0.x; v[1] = f0.y; v[2] = f0.z; v[3] = f0.w;
         b.n[0] = f1.x; b.n[1] = f1.y;
         bvv[0] = f1.z; bvv[1] = f1.w;
@@ -269,8 +283,8 @@ template <int D> __device__ __forceinline__ void load_attr(const DevScene &S, ui
         b.type = (int)f2.z;
         area = f2.w;
     } else {
-        const float4 f0 = S.attr[5 * e], f1 = S.attr[5 * e + 1], f2 = S.attr[5 * e + 2], f3 = S.attr[5 * e + 3],
-                     f4 = S.attr[5 * e + 4];
+        const float4 f0 = gld4(S.attr + 5 * e), f1 = gld4(S.attr + 5 * e + 1), f2 = gld4(S.attr + 5 * e + 2),
+                     f3 = gld4(S.attr + 5 * e + 3), f4 = gld4(S.attr + 5 * e + 4);
         v[0] = f0.x; v[1] = f0.y; v[2] = f0.z; v[3] = f0.w; v[4] = f1.x; v[5] = f1.y; v[6] = f1.z; v[7] = f1.w; v[8] = f2.x;
         b.n[0] = f2.y; b.n[1] = f2.z; b.n[2] = f2.w;
         bvv[0] = f3.x; bvv[1] = f3.y; bvv[2] = f3.z;
@@ -336,7 +350,7 @@ template <int D> __device__ __forceinline__ GeomSample<D> uniform_geometry(const
     const uint32_t e = cdf_lookup(S.area_cdf, S.area_guide, S.n, u1);
     g.elem = e;
     if constexpr (D == 2) {
-        const float4 f0 = S.attr[3 * e], f1 = S.attr[3 * e + 1];
+        const float4 f0 = gld4(S.attr + 3 * e), f1 = gld4(S.attr + 3 * e + 1);
         const float v[4] = {f0.x, f0.y, f0.z, f0.w};
         g.n[0] = f1.x; g.n[1] = f1.y;
         const float u2 = rng.uniform();
@@ -344,7 +358,7 @@ template <int D> __device__ __forceinline__ GeomSample<D> uniform_geometry(const
         for (int k = 0; k < 2; ++k) g.p[k] = v[k] + u2 * (v[2 + k] - v[k]);            // as uniform_in_element
         g.bw[0] = 1.0f - u2; g.bw[1] = u2;
     } else {
-        const float4 f0 = S.attr[5 * e], f1 = S.attr[5 * e + 1], f2 = S.attr[5 * e + 2];
+        const float4 f0 = gld4(S.attr + 5 * e), f1 = gld4(S.attr + 5 * e + 1), f2 = gld4(S.attr + 5 * e + 2);
         const float v[9] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w, f2.x};
         g.n[0] = f2.y; g.n[1] = f2.z; g.n[2] = f2.w;
         const float u2 = rng.uniform();
@@ -364,13 +378,13 @@ template <int D> __device__ __forceinline__ BP<D> complete_uniform(const DevScen
 #pragma unroll
     for (int k = 0; k < D; ++k) { b.p[k] = g.p[k]; b.n[k] = g.n[k]; b.bw[k] = g.bw[k]; }
     if constexpr (D == 2) {
-        const float4 f1 = S.attr[3 * g.elem + 1], f2 = S.attr[3 * g.elem + 2];
+        const float4 f1 = gld4(S.attr + 3 * g.elem + 1), f2 = gld4(S.attr + 3 * g.elem + 2);
         const float u2 = g.bw[1];
         b.bv = f1.z + u2 * (f1.w - f1.z);
         b.alpha = f2.x + u2 * (f2.y - f2.x);
         b.type = (int)f2.z;
     } else {
-        const float4 f3 = S.attr[5 * g.elem + 3], f4 = S.attr[5 * g.elem + 4];
+        const float4 f3 = gld4(S.attr + 5 * g.elem + 3), f4 = gld4(S.attr + 5 * g.elem + 4);
         b.bv = g.bw[0] * f3.x + g.bw[1] * f3.y + g.bw[2] * f3.z;
         b.alpha = g.bw[0] * f4.x + g.bw[1] * f4.y + g.bw[2] * f4.z;
         b.type = (int)f3.w;
@@ -398,7 +412,7 @@ template <int D> __device__ __forceinline__ Sample<D> sample_bv(const DevScene &
         r.y.bw[0] = 1.0f - interp; r.y.bw[1] = interp;
         r.inv_pdf = S.total_bv / (ba + interp * (bb - ba));
     } else {
-        const float pmf = e == 0 ? S.bv_cdf[e] : S.bv_cdf[e] - S.bv_cdf[e - 1];
+        const float pmf = e == 0 ? gldf(S.bv_cdf + e) : gldf(S.bv_cdf + e) - gldf(S.bv_cdf + e - 1);
         uniform_in_element<D>(v, bvv, alv, r.y, rng);
         r.inv_pdf = area / pmf;
     }
@@ -440,7 +454,7 @@ template <int D> __device__ __forceinline__ bool prim_test(const DevScene &S, ui
                                                            const float *d, float &t, float &a, float &b) {
     constexpr float eps = 10 * kFltEps;
     if constexpr (D == 2) {
-        const float4 g = S.prims[prim];
+        const float4 g = gld4(S.prims + prim);
         const float sx = g.x - o[0], sy = g.y - o[1];
         const float det = __builtin_fmaf(d[0], g.w, -(d[1] * g.z));
         const float inv = __builtin_amdgcn_rcpf(det);
@@ -449,7 +463,7 @@ template <int D> __device__ __forceinline__ bool prim_test(const DevScene &S, ui
         b = 0.f;
         return !(a < 0.0f || a > 1.0f || fabsf(t) <= eps || __builtin_isnan(t) || __builtin_isnan(a));
     } else {
-        const float4 g0 = S.prims[3 * prim], g1 = S.prims[3 * prim + 1], g2 = S.prims[3 * prim + 2];
+        const float4 g0 = gld4(S.prims + 3 * prim), g1 = gld4(S.prims + 3 * prim + 1), g2 = gld4(S.prims + 3 * prim + 2);
         const float s0 = g0.x - o[0], s1 = g0.y - o[1], s2 = g0.z - o[2];
         // q = d x s
         const float q0 = __builtin_fmaf(d[1], s2, -(d[2] * s1));
@@ -513,9 +527,9 @@ template <int D> struct Trav {
         const float4 *np = nodes + (size_t)cur * NQ;
         float4 lo[D], hi[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) { lo[k] = np[k]; hi[k] = np[D + k]; }
-        const float4 chf = np[2 * D];
-        const uint32_t valid = __float_as_uint(np[2 * D + 1].x);
+        for (int k = 0; k < D; ++k) { lo[k] = gld4(np + k); hi[k] = gld4(np + D + k); }
+        const float4 chf = gld4(np + 2 * D);
+        const uint32_t valid = __float_as_uint(gld4(np + 2 * D + 1).x);
         const int ch[4] = {(int)__float_as_uint(chf.x), (int)__float_as_uint(chf.y), (int)__float_as_uint(chf.z),
                            (int)__float_as_uint(chf.w)};
         int next = kTravEmpty;
@@ -589,7 +603,7 @@ __device__ __forceinline__ LineHit<D> line_query(const DevScene &S, const float 
     h.count = 0; h.id = 0xFFFFFFFFu; h.t = 0.f; h.a = 0.f; h.b = 0.f;
     uint32_t best = 0xFFFFFFFFu;
     traverse<D, STACK>(S, o, d, stack, stride, [&](bool hit, uint32_t prim, float t, float a, float b) {
-        hit_update<D>(hit, S.prim_id[prim], skip, pick, t, a, b, h, best);
+        hit_update<D>(hit, gldu(S.prim_id + prim), skip, pick, t, a, b, h, best);
     });
     return h;
 }

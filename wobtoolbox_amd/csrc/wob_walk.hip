@@ -338,7 +338,7 @@ __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
                 Sq.prims = prims;
                 float t, a, b;
                 const bool hit = prim_test<D>(Sq, tr.lp, o, d, t, a, b);
-                hit_update<D>(hit, pid[tr.lp], skipv, pick, t, a, b, h, hbest);
+                hit_update<D>(hit, gldu(pid + tr.lp), skipv, pick, t, a, b, h, hbest);
                 ++tr.lp;
             }
         } else if (want_node) {
