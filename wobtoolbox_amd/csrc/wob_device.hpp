@@ -569,6 +569,7 @@ template <int D> struct Trav {
 // a primitive test for the lanes with pending primitives, or a node step (four slab tests)
 // for the lanes that are ready to descend -- whichever more lanes want.  Lanes therefore test
 // primitives together instead of one at a time inside divergent per-child branches.
+constexpr int kPrimsPerRound = 2;
 template <int D, int STACK, typename OnHit>
 __device__ __forceinline__ void traverse(const DevScene &S, const float *o, const float *d, uint32_t *stack,
                                          uint32_t stride, OnHit &&on_hit) {
@@ -582,11 +583,17 @@ __device__ __forceinline__ void traverse(const DevScene &S, const float *o, cons
         const uint64_t bn = __ballot(want_node);
         if ((bl | bn) == 0) break;
         if (__popcll(bl) >= __popcll(bn)) {
+            // up to kPrimsPerRound primitives of the lane's leaf per round: fewer rounds of the
+            // settle / ballot bookkeeping per leaf
             if (want_leaf) {
-                float t, a, b;
-                const bool hit = prim_test<D>(S, tr.lp, o, d, t, a, b);
-                on_hit(hit, tr.lp, t, a, b);
-                ++tr.lp;
+#pragma unroll
+                for (int u = 0; u < kPrimsPerRound; ++u) {
+                    if (u > 0 && tr.lp >= tr.le) break;
+                    float t, a, b;
+                    const bool hit = prim_test<D>(S, tr.lp, o, d, t, a, b);
+                    on_hit(hit, tr.lp, t, a, b);
+                    ++tr.lp;
+                }
             }
         } else if (want_node) {
             tr.template node_step<STACK>(S.nodes, stack, stride);

@@ -336,10 +336,14 @@ __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
             if (want_leaf) {
                 DevScene Sq{};
                 Sq.prims = prims;
-                float t, a, b;
-                const bool hit = prim_test<D>(Sq, tr.lp, o, d, t, a, b);
-                hit_update<D>(hit, gldu(pid + tr.lp), skipv, pick, t, a, b, h, hbest);
-                ++tr.lp;
+#pragma unroll
+                for (int u = 0; u < kPrimsPerRound; ++u) {
+                    if (u > 0 && tr.lp >= tr.le) break;
+                    float t, a, b;
+                    const bool hit = prim_test<D>(Sq, tr.lp, o, d, t, a, b);
+                    hit_update<D>(hit, gldu(pid + tr.lp), skipv, pick, t, a, b, h, hbest);
+                    ++tr.lp;
+                }
             }
         } else if (want_node) {
             tr.template node_step<STACK>(nodes, stack, stride);
