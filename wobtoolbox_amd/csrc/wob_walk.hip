@@ -132,8 +132,17 @@ __device__ __forceinline__ void walk_item(const WalkLaunch &L, uint32_t item, ui
 // lanes walk from neighbouring points.  Persistent mode: each wave takes 64 items at a time
 // from a device counter until the items run out, so lanes of a finished wave move on to new
 // (point, chunk) work at once.
+// Waves per SIMD the compiler must fit (the register budget) where 128 VGPRs suffice without
+// spills: the 2D single-layer RIS domain walk (the batched exterior scenes) and the 3D direct
+// Neumann boundary walk; the others take the compiler's choice.
+template <int D, int BIE, int EST, int FL, bool BOUNDARY, bool VOL> constexpr int walk_min_waves() {
+    if (D == 2 && BIE == kSL && EST == kBackward && FL == FL_UNIFORM && !BOUNDARY && !VOL) return 4;
+    if (D == 3 && BIE == kDirect && FL == FL_LINE_BV && BOUNDARY && !VOL) return 4;
+    return 1;
+}
+
 template <int D, int BIE, int EST, int FL, int STACK, bool BOUNDARY, bool VOL>
-__global__ __launch_bounds__(256) void walk_kernel(WalkLaunch L) {
+__global__ __launch_bounds__(256, (walk_min_waves<D, BIE, EST, FL, BOUNDARY, VOL>())) void walk_kernel(WalkLaunch L) {
     extern __shared__ uint32_t lds_stack[];
     uint32_t *stack = lds_stack + threadIdx.x;
     const uint32_t stride = blockDim.x;
