@@ -270,6 +270,21 @@ __device__ __forceinline__ uint32_t cdf_lookup(const float *__restrict__ cdf, co
     return i;
 }
 
+// two independent lookups with their guide loads issued together
+__device__ __forceinline__ void cdf_lookup2(const float *__restrict__ cdf, const uint32_t *__restrict__ guide, uint32_t n,
+                                            float ua, float ub, uint32_t &ia, uint32_t &ib) {
+    uint32_t ka = (uint32_t)(ua * (float)n), kb = (uint32_t)(ub * (float)n);
+    ka = ka < n ? ka : n - 1;
+    kb = kb < n ? kb : n - 1;
+    uint32_t i = gldu(guide + ka), j = gldu(guide + kb);
+    while (i + 1 < n && ua > gldf(cdf + i)) ++i;
+    while (i > 0 && ua <= gldf(cdf + i - 1)) --i;
+    while (j + 1 < n && ub > gldf(cdf + j)) ++j;
+    while (j > 0 && ub <= gldf(cdf + j - 1)) --j;
+    ia = i;
+    ib = j;
+}
+
 // point on element e at parameter(s) from the sampler; fills everything but p
 template <int D> __device__ __forceinline__ void load_attr(const DevScene &S, uint32_t e, float *v /* D*D */, BP<D> &b,
                                                           float *bvv, float *alv, float &area) {
@@ -344,16 +359,18 @@ template <int D> struct GeomSample {
     float p[D], n[D];
 };
 
-template <int D> __device__ __forceinline__ GeomSample<D> uniform_geometry(const DevScene &S, Rng &rng) {
+// The geometry of a uniform sample of element e from the sample's remaining uniforms: u[1]
+// (and u[2] in 3D) if `u` is given, otherwise drawn from rng after the element's attribute
+// loads are issued (the draws then overlap the loads).
+template <int D>
+__device__ __forceinline__ GeomSample<D> uniform_geometry_of(const DevScene &S, uint32_t e, const float *u, Rng *rng) {
     GeomSample<D> g;
-    const float u1 = rng.uniform();
-    const uint32_t e = cdf_lookup(S.area_cdf, S.area_guide, S.n, u1);
     g.elem = e;
     if constexpr (D == 2) {
         const float4 f0 = gld4(S.attr + 3 * e), f1 = gld4(S.attr + 3 * e + 1);
         const float v[4] = {f0.x, f0.y, f0.z, f0.w};
         g.n[0] = f1.x; g.n[1] = f1.y;
-        const float u2 = rng.uniform();
+        const float u2 = u ? u[1] : rng->uniform();
 #pragma unroll
         for (int k = 0; k < 2; ++k) g.p[k] = v[k] + u2 * (v[2 + k] - v[k]);            // as uniform_in_element
         g.bw[0] = 1.0f - u2; g.bw[1] = u2;
@@ -361,8 +378,8 @@ template <int D> __device__ __forceinline__ GeomSample<D> uniform_geometry(const
         const float4 f0 = gld4(S.attr + 5 * e), f1 = gld4(S.attr + 5 * e + 1), f2 = gld4(S.attr + 5 * e + 2);
         const float v[9] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w, f2.x};
         g.n[0] = f2.y; g.n[1] = f2.z; g.n[2] = f2.w;
-        const float u2 = rng.uniform();
-        const float u3 = rng.uniform();
+        const float u2 = u ? u[1] : rng->uniform();
+        const float u3 = u ? u[2] : rng->uniform();
         const float sq = sqrtf(u2);
         const float w0 = 1.0f - sq, w1 = sq * (1.0f - u3), w2 = u3 * sq;
 #pragma unroll
@@ -370,6 +387,12 @@ template <int D> __device__ __forceinline__ GeomSample<D> uniform_geometry(const
         g.bw[0] = w0; g.bw[1] = w1; g.bw[2] = w2;
     }
     return g;
+}
+
+template <int D> __device__ __forceinline__ GeomSample<D> uniform_geometry(const DevScene &S, Rng &rng) {
+    const float u1 = rng.uniform();
+    const uint32_t e = cdf_lookup(S.area_cdf, S.area_guide, S.n, u1);
+    return uniform_geometry_of<D>(S, e, nullptr, &rng);
 }
 
 template <int D> __device__ __forceinline__ BP<D> complete_uniform(const DevScene &S, const GeomSample<D> &g) {
@@ -1097,8 +1120,7 @@ __device__ __forceinline__ Sample<D> ris(const DevScene &S, const Model<D, BIE> 
     pick.elem = 0xFFFFFFFFu;
 #pragma unroll
     for (int k = 0; k < D; ++k) { pick.p[k] = 0.f; pick.n[k] = 0.f; pick.bw[k] = 0.f; }
-    for (uint32_t i = 0; i < nrc; ++i) {
-        const GeomSample<D> xi = uniform_geometry<D>(S, rng);
+    auto update = [&](const GeomSample<D> &xi, float r) {
         const float sk = signed_kernel(xi.p);
         const float w = fabsf(sk) * S.total_area;
         if (cv_r) {
@@ -1109,7 +1131,7 @@ __device__ __forceinline__ Sample<D> ris(const DevScene &S, const Model<D, BIE> 
             ksum += sk * cv_value<D>(S, cv_r, c);
         }
         w_sum += w;
-        const bool take = rng.uniform() <= w / w_sum;
+        const bool take = r <= w / w_sum;
         pick.elem = take ? xi.elem : pick.elem;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
@@ -1117,6 +1139,31 @@ __device__ __forceinline__ Sample<D> ris(const DevScene &S, const Model<D, BIE> 
             pick.n[k] = take ? xi.n[k] : pick.n[k];
             pick.bw[k] = take ? xi.bw[k] : pick.bw[k];
         }
+    };
+    // 2D: candidates two at a time.  A candidate's D uniforms and its reservoir uniform are
+    // drawn in the reference's order (the draws do not depend on the data), then both element
+    // lookups and attribute loads are in flight together before the reservoir takes them in
+    // order.  (In 3D the second candidate's registers cost more occupancy than the overlap
+    // gains.)
+    uint32_t i = 0;
+    for (; D == 2 && i + 2 <= nrc; i += 2) {
+        float ua[D], ub[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) ua[k] = rng.uniform();
+        const float ra = rng.uniform();
+#pragma unroll
+        for (int k = 0; k < D; ++k) ub[k] = rng.uniform();
+        const float rb = rng.uniform();
+        uint32_t ea, eb;
+        cdf_lookup2(S.area_cdf, S.area_guide, S.n, ua[0], ub[0], ea, eb);
+        const GeomSample<D> xa = uniform_geometry_of<D>(S, ea, ua, nullptr);
+        const GeomSample<D> xb = uniform_geometry_of<D>(S, eb, ub, nullptr);
+        update(xa, ra);
+        update(xb, rb);
+    }
+    for (; i < nrc; ++i) {
+        const GeomSample<D> xa = uniform_geometry<D>(S, rng);
+        update(xa, rng.uniform());
     }
     Sample<D> r;
     r.y = pick.elem == 0xFFFFFFFFu ? bp_zero<D>() : complete_uniform<D>(S, pick);
