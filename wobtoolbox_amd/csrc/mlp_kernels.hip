@@ -5,7 +5,7 @@
 // features) -> 4 x (Linear 128->128, ReLU) -> Linear 128->NO (NO <= 16 outputs, one per
 // remaining path length).  Layer l computes Z = H W_l^T + b_l with W_l stored [out][in].
 //
-// Tiles of 64 samples per workgroup (4 waves, 16 rows each).  Matrix products use
+// Tiles of 64 samples per workgroup (16 waves: 16 rows and a quarter of the columns each).  Matrix products use
 // v_mfma_f32_16x16x16bf16_1k: lane l holds A[l%16][4(l/16)+j], B[4(l/16)+j][l%16] and
 // D[4(l/16)+j][l%16], j = 0..3.  Activations live in LDS as BF16; the backward pass keeps
 // dZ in FP32 in LDS.  Weight gradients of a tile go to a per-workgroup partial buffer that
@@ -28,6 +28,8 @@ constexpr int W = 128;         // width
 constexpr int NF = 64;         // Fourier frequencies
 constexpr int NO = 16;         // padded outputs
 constexpr int LAYERS = 5;
+constexpr int kThreads = 1024;  // 16 waves: wave w owns rows 16 (w % 4) .. +16 and column group w / 4
+constexpr int kColGroups = kThreads / 256;
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
     uint32_t u = __float_as_uint(f);
@@ -52,11 +54,20 @@ __device__ __forceinline__ bf16x4 ld4(const uint16_t *p) {
 // 16x16x16 MFMA operand spans fall in distinct LDS banks.
 constexpr int HS = W + 4;         // BF16 activation / weight row stride
 constexpr int ZS = W + 1;         // FP32 dz row stride
+constexpr int TS = T + 4;         // row stride of the transposed (feature-major) BF16 tiles
 struct Smem {
     uint16_t h[LAYERS][T][HS];    // layer inputs (h[0] = Fourier features), BF16
     float dz[T][ZS];              // backward: dL/dZ of the current layer, FP32
     float out[T][NO];             // network outputs, FP32
-    uint16_t wbuf[W * HS];        // the current layer's weights (W_l or W_l^T), BF16, row stride cols + 4
+    // the current layer's weights (W_l or W_l^T, BF16, row stride cols + 4); during the
+    // weight-gradient product the same space holds H_l^T and dZ^T (BF16, row stride TS)
+    union {
+        uint16_t wbuf[W * HS];
+        struct {
+            uint16_t hT[W][TS];
+            uint16_t dzT[W][TS];
+        } tr;
+    };
 };
 
 // Copy a rows x cols BF16 matrix (row-major, cols a multiple of 4) into LDS with row stride
@@ -70,14 +81,16 @@ __device__ __forceinline__ void stage(const uint16_t *__restrict__ src, int rows
 }
 
 __device__ __forceinline__ void features(const MlpParams &P, const float *pos, uint32_t n, uint32_t row0, Smem &s) {
-    // thread t: row t/4, frequencies (t%4)*16 .. +16
+    // thread t: row t / (NF / PER), frequencies (t % (NF / PER)) * PER .. +PER
+    constexpr int PER = NF * T / kThreads;
     const int t = threadIdx.x;
-    const int r = t >> 2;
+    const int r = t / (NF / PER);
     const uint32_t gi = row0 + r;
     float y[3] = {0.f, 0.f, 0.f};
     if (gi < n)
         for (int k = 0; k < (int)P.dim; ++k) y[k] = pos[(size_t)gi * P.dim + k];
-    for (int j = (t & 3) * 16; j < (t & 3) * 16 + 16; ++j) {
+    const int j0 = (t % (NF / PER)) * PER;
+    for (int j = j0; j < j0 + PER; ++j) {
         const float a = 2.0f * (P.freq[3 * j] * y[0] + P.freq[3 * j + 1] * y[1] + P.freq[3 * j + 2] * y[2]);
         float sn, cs;
         sincospif(a, &sn, &cs);
@@ -95,8 +108,11 @@ __device__ __forceinline__ void forward_layer(const MlpParams &P, int l, Smem &s
     stage(P.w_bf16 + P.w_off[l], nout, W, s.wbuf);
     __syncthreads();
     const float *bl = P.b_f32 + P.b_off[l];
-    const int r0 = wave * 16;
-    for (int n0 = 0; n0 < nout; n0 += 16) {
+    const int r0 = (wave & 3) * 16;
+    // column group (wave >> 2); the 16 outputs of the last layer are one tile, on waves 0-3
+    const int nhalf = nout > 16 ? nout / kColGroups : 16;
+    const int nbeg = nout > 16 ? (wave >> 2) * nhalf : ((wave >> 2) ? nout : 0);
+    for (int n0 = nbeg; n0 < nbeg + nhalf && n0 < nout; n0 += 16) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int k0 = 0; k0 < W; k0 += 16) {
@@ -116,7 +132,7 @@ __device__ __forceinline__ void forward_layer(const MlpParams &P, int l, Smem &s
     }
 }
 
-__global__ __launch_bounds__(256) void mlp_infer_kernel(MlpParams P, const float *pos, uint32_t n, float *out,
+__global__ __launch_bounds__(kThreads) void mlp_infer_kernel(MlpParams P, const float *pos, uint32_t n, float *out,
                                                         uint32_t n_out) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Smem &s = *reinterpret_cast<Smem *>(smem_raw);
@@ -135,7 +151,7 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(MlpParams P, const float
 
 // One tile: forward, masked MSE (sample i supervises output r_idx[i]), backward.
 // partial layout per workgroup: for each layer l, dW_l (out x in) then db_l (out); then loss.
-__global__ __launch_bounds__(256) void mlp_train_kernel(MlpParams P, const float *pos, const int32_t *ridx,
+__global__ __launch_bounds__(kThreads) void mlp_train_kernel(MlpParams P, const float *pos, const int32_t *ridx,
                                                         const float *target, uint32_t n, float inv_batch,
                                                         float *partial) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -175,20 +191,30 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(MlpParams P, const float
         const int nout = l == LAYERS - 1 ? NO : W;
         float *gW = part + P.w_off[l];
         float *gb = part + P.b_goff[l];
-        // dW_l = dZ^T H_l  (nout x W), reduction over the 64 rows of the tile
+        // dW_l = dZ^T H_l  (nout x W), reduction over the 64 rows of the tile.  Its MFMA operands
+        // run along the samples, so H_l and dZ (rounded to BF16) are first transposed into
+        // feature-major tiles; each operand is then one 8-byte LDS read.
+        for (int it = threadIdx.x; it < W * (T / 4); it += blockDim.x) {
+            const int f = it % W, r4 = (it / W) * 4;   // feature f, samples r4 .. r4 + 3
+            uint2 hv, zv;
+            hv.x = (uint32_t)s.h[l][r4][f] | ((uint32_t)s.h[l][r4 + 1][f] << 16);
+            hv.y = (uint32_t)s.h[l][r4 + 2][f] | ((uint32_t)s.h[l][r4 + 3][f] << 16);
+            *reinterpret_cast<uint2 *>(&s.tr.hT[f][r4]) = hv;
+            if (f < nout) {
+                zv.x = (uint32_t)f2bf(s.dz[r4][f]) | ((uint32_t)f2bf(s.dz[r4 + 1][f]) << 16);
+                zv.y = (uint32_t)f2bf(s.dz[r4 + 2][f]) | ((uint32_t)f2bf(s.dz[r4 + 3][f]) << 16);
+                *reinterpret_cast<uint2 *>(&s.tr.dzT[f][r4]) = zv;
+            }
+        }
+        __syncthreads();
         const int tiles = (nout / 16) * (W / 16);
-        for (int tix = wave; tix < tiles; tix += 4) {
+        for (int tix = wave; tix < tiles; tix += kThreads / 64) {
             const int m0 = (tix / (W / 16)) * 16, n0 = (tix % (W / 16)) * 16;
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int k0 = 0; k0 < T; k0 += 16) {
-                bf16x4 a, b;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int k = k0 + 4 * (lane >> 4) + j;
-                    a[j] = (short)f2bf(s.dz[k][m0 + (lane & 15)]);
-                    b[j] = (short)s.h[l][k][n0 + (lane & 15)];
-                }
+                const bf16x4 a = ld4(&s.tr.dzT[m0 + (lane & 15)][k0 + 4 * (lane >> 4)]);
+                const bf16x4 b = ld4(&s.tr.hT[n0 + (lane & 15)][k0 + 4 * (lane >> 4)]);
                 acc = mfma(a, b, acc);
             }
 #pragma unroll
@@ -202,15 +228,17 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(MlpParams P, const float
         }
         if (l == 0) break;
         // dH_l = dZ W_l (T x W), masked by ReLU'(H_l > 0) -> new dz (overwrites after use);
-        // W_l^T ([in][out]) is staged in LDS with row stride nout + 4
+        // W_l^T ([in][out]) is staged in LDS with row stride nout + 4, over the transposed tiles
+        __syncthreads();
         stage(P.wt_bf16 + P.w_off[l], W, nout, s.wbuf);
         __syncthreads();
         const uint16_t *WT = s.wbuf;
-        f32x4 accs[W / 16];
-        const int r0 = wave * 16;
+        constexpr int NT = W / 16 / kColGroups;   // column tiles per wave
+        f32x4 accs[NT];
+        const int r0 = (wave & 3) * 16, nt0 = (wave >> 2) * NT;
 #pragma unroll
-        for (int nt = 0; nt < W / 16; ++nt) {
-            const int n0 = nt * 16;
+        for (int q = 0; q < NT; ++q) {
+            const int n0 = (nt0 + q) * 16;
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
             for (int k0 = 0; k0 < nout; k0 += 16) {
                 bf16x4 a;
@@ -219,16 +247,16 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(MlpParams P, const float
                 const bf16x4 b = ld4(&WT[(n0 + (lane & 15)) * (nout + 4) + k0 + 4 * (lane >> 4)]);
                 acc = mfma(a, b, acc);
             }
-            accs[nt] = acc;
+            accs[q] = acc;
         }
         __syncthreads();   // all reads of dz done
 #pragma unroll
-        for (int nt = 0; nt < W / 16; ++nt) {
-            const int col = nt * 16 + (lane & 15);
+        for (int q = 0; q < NT; ++q) {
+            const int col = (nt0 + q) * 16 + (lane & 15);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int row = r0 + 4 * (lane >> 4) + j;
-                s.dz[row][col] = bf2f(s.h[l][row][col]) > 0.f ? accs[nt][j] : 0.f;
+                s.dz[row][col] = bf2f(s.h[l][row][col]) > 0.f ? accs[q][j] : 0.f;
             }
         }
         __syncthreads();
@@ -236,24 +264,12 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(MlpParams P, const float
 }
 
 // Sum the per-workgroup partials, Adam-update the FP32 masters, refresh the BF16 copies.
-// A workgroup of 256 threads covers 64 parameters: wave s sums partials [s*n/4, (s+1)*n/4)
-// of each (coalesced over the 64 parameters), and the four slice sums are combined in slice
+// A workgroup of kAdamSlices waves covers 256 parameters, 4 consecutive ones per lane (16-byte
+// loads): wave s sums partials [s*n/S, (s+1)*n/S), and the slice sums are combined in slice
 // order, so the result does not depend on scheduling.
-constexpr int kAdamSlices = 4;
-__global__ __launch_bounds__(256) void mlp_adam_kernel(MlpParams P, const float *partial, uint32_t n_parts,
-                                                       float lr, float beta1, float beta2, float eps, float bc1,
-                                                       float bc2, float *master, float *m, float *v) {
-    __shared__ float red[kAdamSlices][64];
-    const uint32_t lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
-    const uint32_t p = blockIdx.x * 64 + lane;
-    const uint32_t k0 = slice * n_parts / kAdamSlices, k1 = (slice + 1) * n_parts / kAdamSlices;
-    float g = 0.f;
-    if (p < P.n_params)
-        for (uint32_t k = k0; k < k1; ++k) g += partial[(size_t)k * P.n_params_padded + p];
-    red[slice][lane] = g;
-    __syncthreads();
-    if (slice != 0 || p >= P.n_params) return;
-    g = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+constexpr int kAdamSlices = 8;
+__device__ __forceinline__ void adam_one(const MlpParams &P, uint32_t p, float g, float lr, float beta1, float beta2,
+                                         float eps, float bc1, float bc2, float *master, float *m, float *v) {
     const float mm = beta1 * m[p] + (1.f - beta1) * g;
     const float vv = beta2 * v[p] + (1.f - beta2) * g * g;
     m[p] = mm;
@@ -276,6 +292,34 @@ __global__ __launch_bounds__(256) void mlp_adam_kernel(MlpParams P, const float 
             return;
         }
     }
+}
+
+__global__ __launch_bounds__(64 * kAdamSlices) void mlp_adam_kernel(MlpParams P, const float *partial,
+                                                                    uint32_t n_parts, float lr, float beta1,
+                                                                    float beta2, float eps, float bc1, float bc2,
+                                                                    float *master, float *m, float *v) {
+    __shared__ float4 red[kAdamSlices][64];
+    const uint32_t lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
+    const uint32_t p4 = blockIdx.x * 256 + 4 * lane;   // n_params_padded is a multiple of 64
+    const uint32_t k0 = slice * n_parts / kAdamSlices, k1 = (slice + 1) * n_parts / kAdamSlices;
+    float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (p4 < P.n_params_padded)
+        for (uint32_t k = k0; k < k1; ++k) {
+            const float4 t = *reinterpret_cast<const float4 *>(partial + (size_t)k * P.n_params_padded + p4);
+            g.x += t.x; g.y += t.y; g.z += t.z; g.w += t.w;
+        }
+    red[slice][lane] = g;
+    __syncthreads();
+    if (slice != 0) return;
+    float4 tot = red[0][lane];
+    for (int q = 1; q < kAdamSlices; ++q) {
+        const float4 t = red[q][lane];
+        tot.x += t.x; tot.y += t.y; tot.z += t.z; tot.w += t.w;
+    }
+    const float gs[4] = {tot.x, tot.y, tot.z, tot.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (p4 + j < P.n_params) adam_one(P, p4 + j, gs[j], lr, beta1, beta2, eps, bc1, bc2, master, m, v);
 }
 
 __global__ void mlp_loss_sum_kernel(const float *partial, uint32_t n_parts, uint32_t stride, uint32_t off,
@@ -303,21 +347,21 @@ hipError_t set_smem_limits() {
 hipError_t launch_infer(const MlpParams &P, const float *pos, uint32_t n, float *out, uint32_t n_out,
                         hipStream_t st) {
     const uint32_t grid = (n + wobm::T - 1) / wobm::T;
-    hipLaunchKernelGGL(wobm::mlp_infer_kernel, dim3(grid), dim3(256), smem_bytes(), st, P, pos, n, out, n_out);
+    hipLaunchKernelGGL(wobm::mlp_infer_kernel, dim3(grid), dim3(wobm::kThreads), smem_bytes(), st, P, pos, n, out, n_out);
     return hipGetLastError();
 }
 
 hipError_t launch_train_step(const MlpParams &P, const float *pos, const int32_t *ridx, const float *target, uint32_t n,
                              float *partial, hipStream_t st) {
     const uint32_t grid = (n + wobm::T - 1) / wobm::T;
-    hipLaunchKernelGGL(wobm::mlp_train_kernel, dim3(grid), dim3(256), smem_bytes(), st, P, pos, ridx, target, n,
+    hipLaunchKernelGGL(wobm::mlp_train_kernel, dim3(grid), dim3(wobm::kThreads), smem_bytes(), st, P, pos, ridx, target, n,
                        1.0f / (float)n, partial);
     return hipGetLastError();
 }
 
 hipError_t launch_adam(const MlpParams &P, const float *partial, uint32_t n_parts, float lr, float beta1, float beta2,
                        float eps, float bc1, float bc2, float *master, float *m, float *v, hipStream_t st) {
-    const uint32_t block = 256, grid = (P.n_params + 63) / 64;
+    const uint32_t block = 64 * wobm::kAdamSlices, grid = (P.n_params + 255) / 256;
     hipLaunchKernelGGL(wobm::mlp_adam_kernel, dim3(grid), dim3(block), 0, st, P, partial, n_parts, lr, beta1, beta2,
                        eps, bc1, bc2, master, m, v);
     return hipGetLastError();
