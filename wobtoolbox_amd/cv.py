@@ -121,3 +121,45 @@ def training_data(estimator, n_per_value, seed=0, paths_per_sample=1, stream=Non
     g = torch.Generator(device="cpu").manual_seed(seed)
     perm = torch.randperm(pos.shape[0], generator=g).to(pos.device)
     return pos[perm].contiguous(), idx[perm].contiguous(), val[perm].contiguous()
+
+
+def _timing_main():
+    """`python -m wobtoolbox_amd.cv [--steps N] [--batch B] [--samples M]`: times the MLP's
+    training steps and inference on random data (the network of the time-to-RMSE workload:
+    3D inputs, 4 outputs) and prints one JSON line."""
+    import argparse
+    import json
+    import time
+    torch = _torch()
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--batch", type=int, default=16384)
+    ap.add_argument("--samples", type=int, default=1 << 20)
+    args = ap.parse_args()
+    g = torch.Generator(device="cuda").manual_seed(0)
+    pos = (torch.rand((args.samples, 3), device="cuda", generator=g) * 2 - 1).contiguous()
+    idx = torch.randint(0, 4, (args.samples,), dtype=torch.int32, device="cuda", generator=g)
+    tgt = (torch.sin(pos[:, 0] * 3) + pos[:, 1] * pos[:, 2]).contiguous()
+    net = ControlVariate(3, 4, sigma=1.0, seed=1)
+    net.train(pos, idx, tgt, batch=args.batch, steps=20)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    net.train(pos, idx, tgt, batch=args.batch, steps=args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    net.infer(pos)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(5):
+        net.infer(pos)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    flop_step = 405504 * args.batch          # forward + backward per sample, 128-wide layers
+    step_ms = (t1 - t0) / args.steps * 1e3
+    print(json.dumps({"train_ms_per_step": step_ms, "batch": args.batch,
+                      "train_tflops": flop_step / (step_ms * 1e-3) / 1e12,
+                      "infer_ms_per_million": (t3 - t2) / 5 * 1e3 * (1 << 20) / args.samples}), flush=True)
+
+
+if __name__ == "__main__":
+    _timing_main()
