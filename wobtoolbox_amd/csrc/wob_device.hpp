@@ -538,7 +538,10 @@ template <int D> struct Trav {
         spo = pop ? spo - stride : spo;
     }
     __device__ __forceinline__ bool want_prim() const { return lp < le; }
-    __device__ __forceinline__ bool want_node() const { return lp >= le && cur >= 0; }
+    // A lane with pending primitives may still descend: the node step only touches `cur` and
+    // the stack, and a leaf it reaches waits in `cur` until the range is used up, so the
+    // stack sees the same depth-first order as without the overlap.
+    __device__ __forceinline__ bool want_node() const { return cur >= 0; }
     __device__ __forceinline__ bool done() const { return lp >= le && cur == kTravEmpty; }
 
     // four slab tests on the current node; the first hit child becomes the current entry, the
@@ -590,8 +593,9 @@ template <int D> struct Trav {
 // (`stack`, entries `stride` words apart, STACK entries plus a spare; the builder bounds the
 // depth), leaves included.  Each iteration the wave runs ONE kind of step, chosen by ballot:
 // a primitive test for the lanes with pending primitives, or a node step (four slab tests)
-// for the lanes that are ready to descend -- whichever more lanes want.  Lanes therefore test
-// primitives together instead of one at a time inside divergent per-child branches.
+// for the lanes whose current entry is an internal node (pending primitives or not) --
+// whichever more lanes want.  Lanes therefore test primitives together instead of one at a
+// time inside divergent per-child branches.
 constexpr int kPrimsPerRound = 2;
 template <int D, int STACK, typename OnHit>
 __device__ __forceinline__ void traverse(const DevScene &S, const float *o, const float *d, uint32_t *stack,
