@@ -508,13 +508,14 @@ template <int D> __device__ __forceinline__ bool prim_test(const DevScene &S, ui
 // comparison tmin <= tmax decides.
 
 // Traversal state of one lane's query: the line's slab parameters, the current entry
-// (internal node >= 0, leaf < 0, or kTravEmpty), the stack depth as an element offset
-// (depth * stride) and the range of pending primitives [lp, le).
+// (internal node >= 0, leaf < 0, or kTravEmpty), the stack's top and bottom as element
+// offsets (depth * stride; entries [sb, spo), the bottom moves only when another lane takes
+// the bottom entry) and the range of pending primitives [lp, le).
 constexpr int kTravEmpty = (int)0x80000000;
 template <int D> struct Trav {
     float inv[D], oi[D];
     int cur;
-    uint32_t spo;
+    uint32_t spo, sb;
     uint32_t lp, le;
 
     __device__ __forceinline__ void begin(const float *o, const float *d) {
@@ -524,14 +525,14 @@ template <int D> struct Trav {
             inv[k] = 1.0f / dk;
             oi[k] = o[k] * inv[k];
         }
-        cur = 0; spo = 0; lp = 0; le = 0;
+        cur = 0; spo = 0; sb = 0; lp = 0; le = 0;
     }
     // a leaf entry becomes the pending primitive range and the next entry is popped
     __device__ __forceinline__ void settle(const uint32_t *stack, uint32_t stride) {
         const bool leaf = lp >= le && cur < 0 && cur != kTravEmpty;
         const uint32_t v = ~(uint32_t)cur;
-        const bool pop = leaf && spo != 0u;
-        const int top = (int)stack[spo != 0u ? spo - stride : 0u];
+        const bool pop = leaf && spo != sb;
+        const int top = (int)stack[spo != sb ? spo - stride : 0u];
         lp = leaf ? v >> 4 : lp;
         le = leaf ? (v >> 4) + (v & 15u) + 1u : le;
         cur = leaf ? (pop ? top : kTravEmpty) : cur;
@@ -549,6 +550,13 @@ template <int D> struct Trav {
     // backs a full stack), only pushes move the top, so no lane branches.
     template <int STACK>
     __device__ __forceinline__ void node_step(const float4 *nodes, uint32_t *stack, uint32_t stride) {
+        node_step<STACK>(nodes, stack, stride, inv, oi);
+    }
+    // the same with the slab parameters of another lane's line (a lane working on a subtree
+    // taken from that lane's query)
+    template <int STACK>
+    __device__ __forceinline__ void node_step(const float4 *nodes, uint32_t *stack, uint32_t stride, const float *inv,
+                                              const float *oi) {
         constexpr int NQ = D == 2 ? 6 : 8;   // float4 per node
         const float4 *np = nodes + (size_t)cur * NQ;
         float4 lo[D], hi[D];
@@ -578,8 +586,8 @@ template <int D> struct Trav {
             spo = push ? spo + stride : spo;
             next = take ? ch[c] : next;
         }
-        const bool pop = next == kTravEmpty && spo != 0u;
-        const int top = (int)stack[spo != 0u ? spo - stride : 0u];
+        const bool pop = next == kTravEmpty && spo != sb;
+        const int top = (int)stack[spo != sb ? spo - stride : 0u];
         spo = pop ? spo - stride : spo;
         cur = next != kTravEmpty ? next : (pop ? top : kTravEmpty);
     }
@@ -639,6 +647,138 @@ __device__ __forceinline__ LineHit<D> line_query(const DevScene &S, const float 
     traverse<D, STACK>(S, o, d, stack, stride, [&](bool hit, uint32_t prim, float t, float a, float b) {
         hit_update<D>(hit, gldu(S.prim_id + prim), skip, pick, t, a, b, h, best);
     });
+    return h;
+}
+
+// Lane `src4 / 4`'s value of v (ds_bpermute; every lane of the query loop is active when it
+// runs, so the source lane is too).
+__device__ __forceinline__ float lane_read(float v, int src4) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(v)));
+}
+__device__ __forceinline__ uint32_t lane_read(uint32_t v, int src4) {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(src4, (int)v);
+}
+
+// LDS words per lane of a line-query kernel: the traversal stack (STACK entries and the spare
+// slot) and the query's result slots (a hit count and a 64-bit (key, leaf) minimum).
+template <int STACK> constexpr int line_query_lds_words() { return STACK + 1 + 3; }
+
+// A steal round runs once at least this many lanes of the wave have finished their work.
+constexpr int kStealMinIdle = 16;
+
+// line_query with the wave's lanes sharing the work of all its queries.  The lines of one
+// wave need very different numbers of steps, and with one line per lane the wave runs until
+// its longest query is done.  Here a lane whose work is done takes the bottom entry (the
+// subtree nearest the root) of another lane's stack and traverses it with that lane's line,
+// read from the owner lane's registers; it adds its hit count and its smallest
+// (key, leaf index) to the owner's result slots in LDS with atomics.  Every primitive of the
+// line's hit set is still tested exactly once, the count is a sum and the pick a minimum, so
+// the result is the same whichever lane does which part.  The keys are distinct for distinct
+// elements (triple32 and x -> pick ^ x * 0x9E3779B9 are bijections of 32-bit words), so the
+// minimum over (key, leaf) picks the same hit as line_query's (key, id) order; its t and
+// barycentrics are recomputed at the end by the owner with the same primitive test.
+//
+// LDS layout (`stack` = block base + threadIdx.x, entries `stride` words apart): the stacks,
+// STACK + 1 entries per lane; then one count word per lane; then one 64-bit (key, leaf) word
+// per lane (stride is a multiple of 64, so it is 8-byte aligned).  Requires all lanes of the
+// wave that call it to call it together (it is called in uniform control flow of the walk).
+template <int D, int STACK>
+__device__ __forceinline__ LineHit<D> line_query_ws(const DevScene &S, const float *o, const float *d, uint32_t pick,
+                                                    uint32_t skip, uint32_t *stack, uint32_t stride) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    uint32_t *const block_base = stack - tid;
+    uint32_t *const wave_col = block_base + (tid - lane);                          // lane 0's stack column
+    uint32_t *const res_cnt = block_base + (STACK + 1) * stride + (tid - lane);
+    uint64_t *const res_key = reinterpret_cast<uint64_t *>(block_base + (STACK + 2) * stride) + (tid - lane);
+    res_cnt[lane] = 0u;
+    res_key[lane] = ~0ull;
+    Trav<D> tr;
+    tr.begin(o, d);
+    const int self4 = (int)lane * 4;
+    int own4 = self4;            // ds_bpermute address of the lane whose line this lane traverses
+    uint32_t cnt = 0u;           // hits of the current work
+    uint64_t best = ~0ull;       // smallest (key << 32 | leaf) of the current work
+    while (true) {
+        tr.settle(stack, stride);
+        const bool idle = tr.done();
+        const uint64_t bi = __ballot(idle);
+        const uint64_t bd = __ballot(tr.spo != tr.sb);
+        if (__popcll(bi) >= kStealMinIdle && bd != 0ull) {
+            // idle lanes hand their results to the owners, then the i-th idle lane takes the
+            // bottom stack entry of the i-th lane that has one
+            if (idle && cnt != 0u) {
+                atomicAdd(&res_cnt[own4 >> 2], cnt);
+                atomicMin((unsigned long long *)&res_key[own4 >> 2], (unsigned long long)best);
+                cnt = 0u;
+                best = ~0ull;
+            }
+            uint64_t im = bi, dm = bd;
+            const int np = min(__popcll(bi), __popcll(bd));
+            int src = -1, given = 0;
+            for (int p = 0; p < np; ++p) {
+                const int il = __builtin_ctzll(im), dl = __builtin_ctzll(dm);
+                im &= im - 1ull;
+                dm &= dm - 1ull;
+                src = (int)lane == il ? dl : src;
+                given = (int)lane == dl ? 1 : given;
+            }
+            const int s4 = src >= 0 ? src * 4 : self4;
+            const uint32_t dsb = lane_read(tr.sb, s4);
+            const int down4 = __builtin_amdgcn_ds_bpermute(s4, own4);
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            if (src >= 0) {
+                tr.cur = (int)wave_col[(uint32_t)src + dsb];
+                tr.spo = 0u;
+                tr.sb = 0u;
+                own4 = down4;
+            }
+            if (given) tr.sb += stride;
+            continue;
+        }
+        const bool want_leaf = tr.want_prim();
+        const bool want_node = tr.want_node();
+        const uint64_t bl = __ballot(want_leaf);
+        const uint64_t bn = __ballot(want_node);
+        if ((bl | bn) == 0ull) break;
+        if (__popcll(bl) >= __popcll(bn)) {
+            float oo[D], dd[D];
+#pragma unroll
+            for (int k = 0; k < D; ++k) { oo[k] = lane_read(o[k], own4); dd[k] = lane_read(d[k], own4); }
+            const uint32_t pk = lane_read(pick, own4), sk = lane_read(skip, own4);
+            if (want_leaf) {
+#pragma unroll
+                for (int u = 0; u < kPrimsPerRound; ++u) {
+                    if (u > 0 && tr.lp >= tr.le) break;
+                    float t, a, b;
+                    const bool hit = prim_test<D>(S, tr.lp, oo, dd, t, a, b);
+                    const uint32_t id = gldu(S.prim_id + tr.lp);
+                    const bool ok = hit && id != sk;
+                    const uint64_t v = ((uint64_t)triple32(pk ^ (id * 0x9E3779B9u)) << 32) | tr.lp;
+                    cnt += ok ? 1u : 0u;
+                    best = ok && v < best ? v : best;
+                    ++tr.lp;
+                }
+            }
+        } else {
+            float inv[D], oi[D];
+#pragma unroll
+            for (int k = 0; k < D; ++k) { inv[k] = lane_read(tr.inv[k], own4); oi[k] = lane_read(tr.oi[k], own4); }
+            if (want_node) tr.template node_step<STACK>(S.nodes, stack, stride, inv, oi);
+        }
+    }
+    if (cnt != 0u) {
+        atomicAdd(&res_cnt[own4 >> 2], cnt);
+        atomicMin((unsigned long long *)&res_key[own4 >> 2], (unsigned long long)best);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    LineHit<D> h;
+    h.count = res_cnt[lane];
+    h.id = 0xFFFFFFFFu; h.t = 0.f; h.a = 0.f; h.b = 0.f;
+    if (h.count != 0u) {
+        const uint32_t leaf = (uint32_t)res_key[lane];
+        (void)prim_test<D>(S, leaf, o, d, h.t, h.a, h.b);
+        h.id = gldu(S.prim_id + leaf);
+    }
     return h;
 }
 
@@ -708,7 +848,7 @@ __device__ __forceinline__ Sample<D> line_sample(const DevScene &S, const float 
     float d[D];
     line_dir<D>(rng, d);
     const uint32_t pick = rng.next_u32();
-    const LineHit<D> h = line_query<D, STACK>(S, x, d, pick, skip, stack, stride);
+    const LineHit<D> h = line_query_ws<D, STACK>(S, x, d, pick, skip, stack, stride);
     return line_finish<D>(S, x, d, h);
 }
 

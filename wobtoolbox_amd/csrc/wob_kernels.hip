@@ -196,7 +196,7 @@ using namespace wobk;
 
 size_t walk_lds_bytes(int fl, bool volume, uint32_t block) {
     const bool lines = fl == FL_LINE_LINE || fl == FL_LINE_BV || fl == FL_FWD_LINE || volume;
-    return lines ? (size_t)(kStackEntries + 1) * block * sizeof(uint32_t) : 0;
+    return lines ? (size_t)line_query_lds_words<kStackEntries>() * block * sizeof(uint32_t) : 0;
 }
 
 hipError_t launch_walk(int dim, bool boundary, bool volume, const WalkLaunch &a, int bie, int est, int fl,

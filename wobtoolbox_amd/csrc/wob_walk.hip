@@ -132,12 +132,13 @@ __device__ __forceinline__ void walk_item(const WalkLaunch &L, uint32_t item, ui
 // lanes walk from neighbouring points.  Persistent mode: each wave takes 64 items at a time
 // from a device counter until the items run out, so lanes of a finished wave move on to new
 // (point, chunk) work at once.
-// Waves per SIMD the compiler must fit (the register budget) where 128 VGPRs suffice without
-// spills: the 2D single-layer RIS domain walk (the batched exterior scenes) and the 3D direct
-// Neumann boundary walk; the others take the compiler's choice.
+// Waves per SIMD the compiler must fit (the register budget): 4 (128 VGPRs) for the 2D
+// single-layer RIS domain walk (the batched exterior scenes), which fits them without spills;
+// 3 (168 VGPRs) for the 3D direct Neumann boundary walk, as many as its LDS allows; the others
+// take the compiler's choice.
 template <int D, int BIE, int EST, int FL, bool BOUNDARY, bool VOL> constexpr int walk_min_waves() {
     if (D == 2 && BIE == kSL && EST == kBackward && FL == FL_UNIFORM && !BOUNDARY && !VOL) return 4;
-    if (D == 3 && BIE == kDirect && FL == FL_LINE_BV && BOUNDARY && !VOL) return 4;
+    if (D == 3 && BIE == kDirect && FL == FL_LINE_BV && BOUNDARY && !VOL) return 3;
     return 1;
 }
 
@@ -370,8 +371,11 @@ template <int D, int BIE, int EST, int FL, bool BOUNDARY, bool VOL> struct WalkK
     // line flavours traverse the BVH; volume kernels also run the inside test of mixture samples
     static constexpr bool lines = FL == FL_LINE_LINE || FL == FL_LINE_BV || FL == FL_FWD_LINE || VOL;
     static constexpr int STACK = lines ? kStackEntries : 0;
-    // per-lane stack of STACK entries plus the spare slot the node step writes above a full stack
-    static size_t lds(uint32_t block) { return lines ? (size_t)(STACK + 1) * block * sizeof(uint32_t) : 0; }
+    // per lane: the stack of STACK entries plus the spare slot the node step writes above a full
+    // stack, and the line query's result slots
+    static size_t lds(uint32_t block) {
+        return lines ? (size_t)line_query_lds_words<STACK>() * block * sizeof(uint32_t) : 0;
+    }
     static const void *fn() { return (const void *)&walk_kernel<D, BIE, EST, FL, STACK, BOUNDARY, VOL>; }
     // the wavefront walk covers the line-sampled backward domain walks without a volume term
     static constexpr bool wf = EST == kBackward && !BOUNDARY && !VOL && (FL == FL_LINE_LINE || FL == FL_LINE_BV);
