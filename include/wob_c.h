@@ -152,9 +152,9 @@ enum { WOB_EST_INDEPENDENT_VOLUME_SAMPLES = 8 };
  * (64..256); 0 = automatic.  With WOB_LAUNCH_TIME_WALK in flags the call brackets the walk
  * kernel with HIP events on the launch stream, waits for the end event and writes the
  * kernel's duration to walk_ms (and the work-item geometry it chose to the other outputs).
- * schedule: WOB_SCHEDULE_AUTO picks the wavefront walk for double-layer line walks (where it
- * is faster) and the per-lane walk otherwise; WOB_SCHEDULE_LANE runs
- * every lane's queries to their end; WOB_SCHEDULE_WAVEFRONT lets a lane whose query ended run
+ * schedule: WOB_SCHEDULE_AUTO picks the per-lane walk (the faster one on every line walk
+ * measured); WOB_SCHEDULE_LANE runs the wave's queries to their end, lanes whose query ended
+ * taking traversal work from the others; WOB_SCHEDULE_WAVEFRONT lets a lane whose query ended run
  * its walk to the next query while the others traverse (an error where unsupported).  The
  * results are bit-identical either way.  regen_threshold: lanes that must have finished their
  * query before a wave advances them (wavefront walk; 0 = default 32).                       */

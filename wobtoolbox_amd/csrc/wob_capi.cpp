@@ -210,18 +210,17 @@ wobl::WalkEntry make_entry(const wob_scene *scene, const wob_estimator_desc *est
 
 constexpr uint32_t kDefaultRegen = 32;
 
-// Resolve a requested schedule: true = wavefront walk.  Automatic: the wavefront walk where it
-// measured faster on MI355X, the double-layer line walk (2D circle, 1,024 segments: 501 vs 452
-// M paths/s); the direct Neumann line walk stays per lane (icosphere, 20,480 triangles: 163 vs
-// 200 M paths/s -- its longer walk step between queries costs more in partial waves than the
-// traversal it saves).
+// Resolve a requested schedule: true = wavefront walk.  Automatic: the per-lane walk, whose
+// line queries share traversal work across the wave; measured on MI355X it is the faster one
+// for every line walk (2D circle, 1,024 segments, double layer: 15.0 vs 19.2 ms for 51,468
+// points x 256 paths; level-5 icosphere, double layer: 24.8 vs 27.1 ms for 100k x 64).
 wob_status resolve_schedule(uint32_t schedule, bool supported, int flavour, bool &wavefront) {
+    (void)flavour;
     if (schedule > WOB_SCHEDULE_WAVEFRONT) return fail(WOB_INVALID_ARG, "unknown schedule");
     if (schedule == WOB_SCHEDULE_WAVEFRONT && !supported)
         return fail(WOB_UNSUPPORTED_COMBO, "the wavefront walk covers line-sampled backward domain walks without a "
                                            "volume term only");
-    if (schedule == WOB_SCHEDULE_AUTO) wavefront = supported && flavour == wobdev::FL_LINE_LINE;
-    else wavefront = supported && schedule == WOB_SCHEDULE_WAVEFRONT;
+    wavefront = supported && schedule == WOB_SCHEDULE_WAVEFRONT;
     return WOB_OK;
 }
 
