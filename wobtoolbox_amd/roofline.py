@@ -48,7 +48,7 @@ def workload_model(name, n_elements, path_length):
 # for a 512 MiB device copy in the same run: FETCH_SIZE 262,166 KiB (x 2.0), WRITE_SIZE
 # 524,288 KiB (x 1.0).  Walk kernel: FETCH_SIZE 20,364 KiB, WRITE_SIZE 35,008 KiB.
 MEASURED_TRAFFIC = {
-    "icosphere_neumann_3d": (2.0 * 20246 + 35008) * 1024,
+    "icosphere_neumann_3d": (2.0 * 20771 + 35008) * 1024,
 }
 
 MI355X_PEAKS = {"hbm_GBps": 8000.0, "fp32_TFLOPs": 157.3, "bf16_TFLOPs": 2500.0}
