@@ -363,6 +363,7 @@ std::string build_scene(const wob_scene_desc &d, HostScene &S) {
             std::memcpy(&nd[NW - 4], &mask, 4);
         }
         if (next_prim != n) return "internal error: primitive count mismatch";
+        quantize_nodes(D, S.nodes, S.qnodes);
         // stack bound for DFS "continue with one hit child, push the others" (leaves are
         // pushed too): need(w) = (#children - 1) + max need(internal child); bottom-up.
         for (size_t w = wides.size(); w-- > 0;) {
@@ -380,6 +381,75 @@ std::string build_scene(const wob_scene_desc &d, HostScene &S) {
         if (attempt == 1) return "BVH stack bound exceeds 48 entries";
     }
     return "";
+}
+
+// Child bounds in 8-bit steps of the node box: lower bounds rounded down, upper bounds up,
+// checked in extended precision so that the dequantized box lo + q s (exact reals) contains
+// the float box.  The line traversal evaluates the slab parameters as
+// q (s inv) + (lo inv - o inv) with about three roundings, well inside the 4e-6 |coordinate|
+// inflation of the primitive boxes.
+void quantize_nodes(int dim, const std::vector<float> &nodes, std::vector<uint32_t> &qnodes) {
+    const int NW = dim == 2 ? kNodeWords2 : kNodeWords3;
+    const int QW = dim == 2 ? kQNodeWords2 : kQNodeWords3;
+    const size_t nn = nodes.size() / NW;
+    qnodes.assign(nn * QW, 0u);
+    auto bits = [](float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; };
+    for (size_t w = 0; w < nn; ++w) {
+        const float *nd = &nodes[w * NW];
+        uint32_t mask;
+        std::memcpy(&mask, &nd[NW - 4], 4);
+        const int CH = dim == 2 ? 16 : 24;   // word offset of the child entries
+        float lo[3] = {0.f, 0.f, 0.f}, s[3] = {0.f, 0.f, 0.f};
+        uint32_t qlo[3] = {0u, 0u, 0u}, qhi[3] = {0u, 0u, 0u};
+        for (int k = 0; k < dim; ++k) {
+            float a = 0.f, b = 0.f;
+            bool any = false;
+            for (int c = 0; c < 4; ++c) {
+                if (!((mask >> c) & 1u)) continue;
+                const float l = nd[4 * k + c], h = nd[4 * (dim + k) + c];
+                a = any ? std::min(a, l) : l;
+                b = any ? std::max(b, h) : h;
+                any = true;
+            }
+            // step: smallest float with a + 255 s >= b
+            float st = (float)(((double)b - (double)a) / 255.0);
+            while ((long double)a + 255.0L * (long double)st < (long double)b) st = std::nextafter(st, INFINITY);
+            lo[k] = a;
+            s[k] = st;
+            for (int c = 0; c < 4; ++c) {
+                uint32_t ql = 0u, qh = 0u;
+                if ((mask >> c) & 1u) {
+                    const long double l = nd[4 * k + c], h = nd[4 * (dim + k) + c];
+                    if (st > 0.f) {
+                        long double fl = std::floor((l - (long double)a) / (long double)st);
+                        long double ch = std::ceil((h - (long double)a) / (long double)st);
+                        fl = std::min(std::max(fl, 0.0L), 255.0L);
+                        ch = std::min(std::max(ch, 0.0L), 255.0L);
+                        while (fl > 0.0L && (long double)a + fl * (long double)st > l) fl -= 1.0L;
+                        while (ch < 255.0L && (long double)a + ch * (long double)st < h) ch += 1.0L;
+                        ql = (uint32_t)fl;
+                        qh = (uint32_t)ch;
+                    }
+                }
+                qlo[k] |= ql << (8 * c);
+                qhi[k] |= qh << (8 * c);
+            }
+        }
+        uint32_t *q = &qnodes[w * QW];
+        if (dim == 2) {
+            q[0] = bits(lo[0]); q[1] = bits(lo[1]); q[2] = bits(s[0]); q[3] = bits(s[1]);
+            q[4] = qlo[0]; q[5] = qlo[1]; q[6] = qhi[0]; q[7] = qhi[1];
+        } else {
+            q[0] = bits(lo[0]); q[1] = bits(lo[1]); q[2] = bits(lo[2]); q[3] = bits(s[0]);
+            q[4] = bits(s[1]); q[5] = bits(s[2]); q[6] = qlo[0]; q[7] = qlo[1];
+            q[8] = qlo[2]; q[9] = qhi[0]; q[10] = qhi[1]; q[11] = qhi[2];
+        }
+        for (int c = 0; c < 4; ++c) {
+            uint32_t ch;
+            std::memcpy(&ch, &nd[CH + c], 4);
+            q[QW - 4 + c] = ((mask >> c) & 1u) ? ch : 0x80000000u;
+        }
+    }
 }
 
 }  // namespace wobhost

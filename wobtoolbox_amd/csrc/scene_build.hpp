@@ -3,7 +3,9 @@
 // Replaces SceneHost's constructor (wob/wob/scene.cuh:264-299), which builds an lbvh binary
 // BVH over the elements and the area / boundary-value CDFs in BVH leaf order.  Here:
 //   * a binned-SAH binary BVH is built top-down and collapsed into a 4-wide BVH whose nodes
-//     hold their four child boxes as structure-of-arrays (one 128-byte line per 3D node);
+//     hold their four child boxes as structure-of-arrays (one 128-byte line per 3D node),
+//     plus a copy with the child boxes quantized to 8 bits per bound (64 bytes per 3D node)
+//     for line traversal;
 //   * primitive geometry is re-laid out in leaf order in the form the intersection test
 //     consumes (2D: a, a-b; 3D: a, a-b, a-c, (a-b)x(a-c));
 //   * element attributes stay in the caller's order, as do the CDFs (FP32 Kahan running
@@ -23,6 +25,13 @@ namespace wobhost {
 // node index, < 0 leaf ~((first_prim << 4) | (count - 1)); `meta[0]` is the valid-child mask.
 constexpr int kNodeWords2 = 24;   // lox[4] loy[4] hix[4] hiy[4] child[4] meta[4]       (96 B)
 constexpr int kNodeWords3 = 32;   // lox[4] loy[4] loz[4] hix[4] hiy[4] hiz[4] child[4] meta[4] (128 B)
+// Quantized nodes for line traversal (32-bit words): the node box's lower corner lo and a
+// step s per axis (floats), then per axis one word of four 8-bit child lower bounds and one of
+// four upper bounds (byte c = child c), then the four child words (an invalid child is
+// 0x80000000).  Child c's box along axis k is [lo_k + qlo s_k, lo_k + qhi s_k], built to
+// contain the float box exactly.
+constexpr int kQNodeWords2 = 12;  // lo.xy s.xy | qlo.x qlo.y qhi.x qhi.y | child[4]           (48 B)
+constexpr int kQNodeWords3 = 16;  // lo.xyz s.x | s.yz qlo.x qlo.y | qlo.z qhi.xyz | child[4] (64 B)
 constexpr int kPrimWords2 = 4;    // a.x a.y e.x e.y                 (e = a - b)
 constexpr int kPrimWords3 = 12;   // a.xyz e1.xyz e2.xyz N.xyz       (e1 = a - b, e2 = a - c, N = e1 x e2)
 constexpr int kAttrWords2 = 12;   // a.xy b.xy | n.xy bv0 bv1 | al0 al1 type area
@@ -32,6 +41,7 @@ constexpr int kMaxLeaf = 8;
 struct HostScene {
     uint32_t dim = 0, n = 0;
     std::vector<float> nodes;          // n_nodes * kNodeWords
+    std::vector<uint32_t> qnodes;      // n_nodes * kQNodeWords
     std::vector<float> prim_geom;      // n * kPrimWords (leaf order)
     std::vector<uint32_t> prim_id;     // leaf order -> element id
     std::vector<float> elem_attr;      // n * kAttrWords (element order)
@@ -43,6 +53,9 @@ struct HostScene {
 
 // Returns an empty string on success, otherwise an error message.
 std::string build_scene(const wob_scene_desc &desc, HostScene &out);
+
+// Quantized copy of float nodes (kNodeWords{2,3} each) in the kQNodeWords{2,3} layout.
+void quantize_nodes(int dim, const std::vector<float> &nodes, std::vector<uint32_t> &qnodes);
 
 // FP32 Kahan CDF (scene.cuh:271-298) and its guide table.
 void build_cdf(const std::vector<float> &weights, std::vector<float> &cdf, float &total,

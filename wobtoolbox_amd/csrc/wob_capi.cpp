@@ -40,7 +40,7 @@ inline wob_status hip_fail(hipError_t e, const char *what) { return wob_hip_fail
 struct wob_scene {
     wobhost::HostScene host;
     wobdev::DevScene dev{};
-    void *bufs[8] = {nullptr};
+    void *bufs[9] = {nullptr};
     float *scratch = nullptr;
     size_t scratch_bytes = 0;
     uint32_t *counter = nullptr;   // work counter of wavefront launches
@@ -463,16 +463,18 @@ wob_status wob_scene_create(const wob_scene_desc *desc, wob_scene **out_scene) {
     const auto &h = s->host;
     wob_status st;
     const float *nodes = nullptr, *prims = nullptr, *attr = nullptr, *acdf = nullptr, *bcdf = nullptr;
-    const uint32_t *pid = nullptr, *ag = nullptr, *bg = nullptr;
+    const uint32_t *pid = nullptr, *ag = nullptr, *bg = nullptr, *qn = nullptr;
     if ((st = upload(s, 0, h.nodes, &nodes)) != WOB_OK || (st = upload(s, 1, h.prim_geom, &prims)) != WOB_OK ||
         (st = upload(s, 2, h.prim_id, &pid)) != WOB_OK || (st = upload(s, 3, h.elem_attr, &attr)) != WOB_OK ||
         (st = upload(s, 4, h.area_cdf, &acdf)) != WOB_OK || (st = upload(s, 5, h.area_guide, &ag)) != WOB_OK ||
-        (st = upload(s, 6, h.bv_cdf, &bcdf)) != WOB_OK || (st = upload(s, 7, h.bv_guide, &bg)) != WOB_OK) {
+        (st = upload(s, 6, h.bv_cdf, &bcdf)) != WOB_OK || (st = upload(s, 7, h.bv_guide, &bg)) != WOB_OK ||
+        (st = upload(s, 8, h.qnodes, &qn)) != WOB_OK) {
         free_scene(s);
         delete s;
         return st;
     }
-    s->dev.nodes = (const float4 *)nodes;
+    s->dev.nodes = (const float4 *)qn;
+    s->dev.fnodes = (const float4 *)nodes;
     s->dev.prims = (const float4 *)prims;
     s->dev.prim_id = pid;
     s->dev.attr = (const float4 *)attr;

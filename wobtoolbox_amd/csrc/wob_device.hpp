@@ -54,7 +54,8 @@ struct VolSrc {
 };
 
 struct DevScene {
-    const float4 *nodes;
+    const float4 *nodes;      // quantized nodes (line traversal)
+    const float4 *fnodes;     // float nodes (closest-point queries)
     const float4 *prims;
     const uint32_t *prim_id;
     const float4 *attr;
@@ -468,16 +469,33 @@ __device__ __forceinline__ void hit_update(bool hit, uint32_t id, uint32_t skip,
     h.b = take ? b : h.b;
 }
 
-// line o + t d against primitive `prim` (leaf order).  Reference predicate (scene.cuh:431-432,
-// 725-727): parameters in the closed element, |t| > 10 FLT_EPSILON, no NaN.  The fused
-// multiply-adds are written out: left to the compiler, a*b + c*d may be fused either way
-// depending on the code the test is inlined into, and the hit coordinates would round
-// differently in different kernels.
-template <int D> __device__ __forceinline__ bool prim_test(const DevScene &S, uint32_t prim, const float *o,
-                                                           const float *d, float &t, float &a, float &b) {
+// Geometry record of primitive `prim` (leaf order): 2D one float4, 3D three.
+template <int D> struct PrimRec {
+    float4 g[D == 2 ? 1 : 3];
+};
+template <int D> __device__ __forceinline__ PrimRec<D> prim_load(const DevScene &S, uint32_t prim) {
+    PrimRec<D> r;
+    if constexpr (D == 2) {
+        r.g[0] = gld4(S.prims + prim);
+    } else {
+        r.g[0] = gld4(S.prims + 3 * prim);
+        r.g[1] = gld4(S.prims + 3 * prim + 1);
+        r.g[2] = gld4(S.prims + 3 * prim + 2);
+    }
+    return r;
+}
+
+// line o + t d against a primitive record.  Reference predicate (scene.cuh:431-432, 725-727):
+// parameters in the closed element, |t| > 10 FLT_EPSILON, no NaN.  The fused multiply-adds
+// are written out: left to the compiler, a*b + c*d may be fused either way depending on the
+// code the test is inlined into, and the hit coordinates would round differently in
+// different kernels.
+template <int D>
+__device__ __forceinline__ bool prim_eval(const PrimRec<D> &r, const float *o, const float *d, float &t, float &a,
+                                          float &b) {
     constexpr float eps = 10 * kFltEps;
     if constexpr (D == 2) {
-        const float4 g = gld4(S.prims + prim);
+        const float4 g = r.g[0];
         const float sx = g.x - o[0], sy = g.y - o[1];
         const float det = __builtin_fmaf(d[0], g.w, -(d[1] * g.z));
         const float inv = __builtin_amdgcn_rcpf(det);
@@ -486,7 +504,7 @@ template <int D> __device__ __forceinline__ bool prim_test(const DevScene &S, ui
         b = 0.f;
         return !(a < 0.0f || a > 1.0f || fabsf(t) <= eps || __builtin_isnan(t) || __builtin_isnan(a));
     } else {
-        const float4 g0 = gld4(S.prims + 3 * prim), g1 = gld4(S.prims + 3 * prim + 1), g2 = gld4(S.prims + 3 * prim + 2);
+        const float4 g0 = r.g[0], g1 = r.g[1], g2 = r.g[2];
         const float s0 = g0.x - o[0], s1 = g0.y - o[1], s2 = g0.z - o[2];
         // q = d x s
         const float q0 = __builtin_fmaf(d[1], s2, -(d[2] * s1));
@@ -502,10 +520,16 @@ template <int D> __device__ __forceinline__ bool prim_test(const DevScene &S, ui
     }
 }
 
-// Child boxes are inflated at build time by 4e-6 x |coordinate| (scene_build.cpp), which is
-// about 30 times the rounding of the FMA slab parameters below; a line meeting an element
-// therefore always meets every box on the element's path to the root, and the plain
-// comparison tmin <= tmax decides.
+// line o + t d against primitive `prim` (leaf order)
+template <int D> __device__ __forceinline__ bool prim_test(const DevScene &S, uint32_t prim, const float *o,
+                                                           const float *d, float &t, float &a, float &b) {
+    return prim_eval<D>(prim_load<D>(S, prim), o, d, t, a, b);
+}
+
+// Primitive boxes are inflated at build time by 4e-6 x |coordinate| (scene_build.cpp) and the
+// quantized child boxes contain them; that is about 10 times the rounding of the slab
+// parameters below (three roundings); a line meeting an element therefore always meets every
+// box on the element's path to the root, and the plain comparison tmin <= tmax decides.
 
 // Traversal state of one lane's query: the line's slab parameters, the current entry
 // (internal node >= 0, leaf < 0, or kTravEmpty), the stack's top and bottom as element
@@ -557,15 +581,47 @@ template <int D> struct Trav {
     template <int STACK>
     __device__ __forceinline__ void node_step(const float4 *nodes, uint32_t *stack, uint32_t stride, const float *inv,
                                               const float *oi) {
-        constexpr int NQ = D == 2 ? 6 : 8;   // float4 per node
-        const float4 *np = nodes + (size_t)cur * NQ;
-        float4 lo[D], hi[D];
+        node_apply<STACK>(node_load(nodes), stack, stride, inv, oi);
+    }
+    // the current node's quantized record (kQNodeWords layout of scene_build.hpp): node box
+    // corner lo and per-axis step s, 8-bit child bounds, child entries (0x80000000: none)
+    struct NodeRec {
+        float lo[D], s[D];
+        uint32_t qlo[D], qhi[D];
+        float4 chf;
+    };
+    __device__ __forceinline__ NodeRec node_load(const float4 *nodes) const {
+        NodeRec r;
+        if constexpr (D == 2) {
+            const float4 *np = nodes + (size_t)cur * 3;
+            const float4 a = gld4(np), b = gld4(np + 1);
+            r.chf = gld4(np + 2);
+            r.lo[0] = a.x; r.lo[1] = a.y; r.s[0] = a.z; r.s[1] = a.w;
+            r.qlo[0] = __float_as_uint(b.x); r.qlo[1] = __float_as_uint(b.y);
+            r.qhi[0] = __float_as_uint(b.z); r.qhi[1] = __float_as_uint(b.w);
+        } else {
+            const float4 *np = nodes + (size_t)cur * 4;
+            const float4 a = gld4(np), b = gld4(np + 1), c = gld4(np + 2);
+            r.chf = gld4(np + 3);
+            r.lo[0] = a.x; r.lo[1] = a.y; r.lo[2] = a.z; r.s[0] = a.w;
+            r.s[1] = b.x; r.s[2] = b.y; r.qlo[0] = __float_as_uint(b.z); r.qlo[1] = __float_as_uint(b.w);
+            r.qlo[2] = __float_as_uint(c.x);
+            r.qhi[0] = __float_as_uint(c.y); r.qhi[1] = __float_as_uint(c.z); r.qhi[2] = __float_as_uint(c.w);
+        }
+        return r;
+    }
+    // slab parameters of child bound lo + q s: q (s inv) + (lo inv - o inv)
+    template <int STACK>
+    __device__ __forceinline__ void node_apply(const NodeRec &r, uint32_t *stack, uint32_t stride, const float *inv,
+                                               const float *oi) {
+        const int ch[4] = {(int)__float_as_uint(r.chf.x), (int)__float_as_uint(r.chf.y), (int)__float_as_uint(r.chf.z),
+                           (int)__float_as_uint(r.chf.w)};
+        float A[D], B[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) { lo[k] = gld4(np + k); hi[k] = gld4(np + D + k); }
-        const float4 chf = gld4(np + 2 * D);
-        const uint32_t valid = __float_as_uint(gld4(np + 2 * D + 1).x);
-        const int ch[4] = {(int)__float_as_uint(chf.x), (int)__float_as_uint(chf.y), (int)__float_as_uint(chf.z),
-                           (int)__float_as_uint(chf.w)};
+        for (int k = 0; k < D; ++k) {
+            A[k] = r.s[k] * inv[k];
+            B[k] = __builtin_fmaf(r.lo[k], inv[k], -oi[k]);
+        }
         int next = kTravEmpty;
         const uint32_t full = (uint32_t)STACK * stride;
 #pragma unroll
@@ -573,13 +629,12 @@ template <int D> struct Trav {
             float tmin = 0.f, tmax = 0.f;
 #pragma unroll
             for (int k = 0; k < D; ++k) {
-                const float l = c == 0 ? lo[k].x : c == 1 ? lo[k].y : c == 2 ? lo[k].z : lo[k].w;
-                const float u = c == 0 ? hi[k].x : c == 1 ? hi[k].y : c == 2 ? hi[k].z : hi[k].w;
-                const float t1 = __builtin_fmaf(l, inv[k], -oi[k]), t2 = __builtin_fmaf(u, inv[k], -oi[k]);
+                const float l = (float)((r.qlo[k] >> (8 * c)) & 0xFFu), u = (float)((r.qhi[k] >> (8 * c)) & 0xFFu);
+                const float t1 = __builtin_fmaf(l, A[k], B[k]), t2 = __builtin_fmaf(u, A[k], B[k]);
                 tmin = k == 0 ? fminf(t1, t2) : fmaxf(tmin, fminf(t1, t2));
                 tmax = k == 0 ? fmaxf(t1, t2) : fminf(tmax, fmaxf(t1, t2));
             }
-            const bool h = ((valid >> c) & 1u) && tmin <= tmax;
+            const bool h = ch[c] != kTravEmpty && tmin <= tmax;
             const bool take = h && next == kTravEmpty;
             const bool push = h && !take && spo < full;
             stack[spo] = (uint32_t)ch[c];

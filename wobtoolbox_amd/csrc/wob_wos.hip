@@ -138,7 +138,7 @@ __device__ Closest<D> closest_point(const DevScene &S, const float *x, uint32_t 
         float m = 0.f;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            const float4 l = S.nodes[k], u = S.nodes[D + k];
+            const float4 l = S.fnodes[k], u = S.fnodes[D + k];
             m = fmaxf(m, fmaxf(fmaxf(fmaxf(fabsf(l.x), fabsf(l.y)), fmaxf(fabsf(l.z), fabsf(l.w))),
                                fmaxf(fmaxf(fabsf(u.x), fabsf(u.y)), fmaxf(fabsf(u.z), fabsf(u.w)))));
             m = fmaxf(m, fabsf(x[k]));
@@ -150,7 +150,7 @@ __device__ Closest<D> closest_point(const DevScene &S, const float *x, uint32_t 
     uint32_t sp = 0;
     while (true) {
         if (cur >= 0) {
-            const float4 *np = S.nodes + (size_t)cur * NQ;
+            const float4 *np = S.fnodes + (size_t)cur * NQ;
             float4 lo[D], hi[D];
 #pragma unroll
             for (int k = 0; k < D; ++k) { lo[k] = np[k]; hi[k] = np[D + k]; }
