@@ -336,9 +336,9 @@ std::string build_scene(const wob_scene_desc &d, HostScene &S) {
                                     g[3 + k] = e1[k];
                                     g[6 + k] = e2[k];
                                 }
-                                g[9] = e1[1] * e2[2] - e1[2] * e2[1];
-                                g[10] = e1[2] * e2[0] - e1[0] * e2[2];
-                                g[11] = e1[0] * e2[1] - e1[1] * e2[0];
+                                std::memcpy(&g[9], &e, 4);   // element id; N = e1 x e2 is formed in the test
+                                g[10] = 0.f;
+                                g[11] = 0.f;
                             }
                             ++next_prim;
                         }

@@ -7,7 +7,7 @@
 //     plus a copy with the child boxes quantized to 8 bits per bound (64 bytes per 3D node)
 //     for line traversal;
 //   * primitive geometry is re-laid out in leaf order in the form the intersection test
-//     consumes (2D: a, a-b; 3D: a, a-b, a-c, (a-b)x(a-c));
+//     consumes (2D: a, a-b; 3D: a, a-b, a-c and the element id);
 //   * element attributes stay in the caller's order, as do the CDFs (FP32 Kahan running
 //     sums, scene.cuh:271-298), each with a guide table for O(1) inverse-CDF sampling that
 //     returns exactly what binary_search (scene.cuh:30-43) returns.
@@ -33,7 +33,7 @@ constexpr int kNodeWords3 = 32;   // lox[4] loy[4] loz[4] hix[4] hiy[4] hiz[4] c
 constexpr int kQNodeWords2 = 12;  // lo.xy s.xy | qlo.x qlo.y qhi.x qhi.y | child[4]           (48 B)
 constexpr int kQNodeWords3 = 16;  // lo.xyz s.x | s.yz qlo.x qlo.y | qlo.z qhi.xyz | child[4] (64 B)
 constexpr int kPrimWords2 = 4;    // a.x a.y e.x e.y                 (e = a - b)
-constexpr int kPrimWords3 = 12;   // a.xyz e1.xyz e2.xyz N.xyz       (e1 = a - b, e2 = a - c, N = e1 x e2)
+constexpr int kPrimWords3 = 12;   // a.xyz e1.xyz e2.xyz id 0 0      (e1 = a - b, e2 = a - c, id = element id)
 constexpr int kAttrWords2 = 12;   // a.xy b.xy | n.xy bv0 bv1 | al0 al1 type area
 constexpr int kAttrWords3 = 20;   // a.xyz b.x | b.yz c.xy | c.z n.xyz | bv0 bv1 bv2 type | al0 al1 al2 area
 constexpr int kMaxLeaf = 8;

@@ -505,14 +505,16 @@ __device__ __forceinline__ bool prim_eval(const PrimRec<D> &r, const float *o, c
         return !(a < 0.0f || a > 1.0f || fabsf(t) <= eps || __builtin_isnan(t) || __builtin_isnan(a));
     } else {
         const float4 g0 = r.g[0], g1 = r.g[1], g2 = r.g[2];
+        // N = e1 x e2, rounded as the scene build used to store it (separate products)
+        const float n0 = g1.x * g2.x - g1.y * g1.w, n1 = g1.y * g1.z - g0.w * g2.x, n2 = g0.w * g1.w - g1.x * g1.z;
         const float s0 = g0.x - o[0], s1 = g0.y - o[1], s2 = g0.z - o[2];
         // q = d x s
         const float q0 = __builtin_fmaf(d[1], s2, -(d[2] * s1));
         const float q1 = __builtin_fmaf(d[2], s0, -(d[0] * s2));
         const float q2 = __builtin_fmaf(d[0], s1, -(d[1] * s0));
-        const float det = __builtin_fmaf(d[0], g2.y, __builtin_fmaf(d[1], g2.z, d[2] * g2.w));   // d . (e1 x e2)
+        const float det = __builtin_fmaf(d[0], n0, __builtin_fmaf(d[1], n1, d[2] * n2));         // d . (e1 x e2)
         const float inv = __builtin_amdgcn_rcpf(det);                                            // hardware reciprocal (1 ulp)
-        t = __builtin_fmaf(s0, g2.y, __builtin_fmaf(s1, g2.z, s2 * g2.w)) * inv;                // s . N / det
+        t = __builtin_fmaf(s0, n0, __builtin_fmaf(s1, n1, s2 * n2)) * inv;                      // s . N / det
         a = __builtin_fmaf(q0, g1.z, __builtin_fmaf(q1, g1.w, q2 * g2.x)) * inv;                // (d x s) . e2 / det
         b = -(__builtin_fmaf(q0, g0.w, __builtin_fmaf(q1, g1.x, q2 * g1.y))) * inv;            // -(d x s) . e1 / det
         return !(a < 0.0f || b < 0.0f || a + b > 1.0f || fabsf(t) <= eps || __builtin_isnan(t) || __builtin_isnan(a) ||
@@ -805,8 +807,10 @@ __device__ __forceinline__ LineHit<D> line_query_ws(const DevScene &S, const flo
                 for (int u = 0; u < kPrimsPerRound; ++u) {
                     if (u > 0 && tr.lp >= tr.le) break;
                     float t, a, b;
-                    const bool hit = prim_test<D>(S, tr.lp, oo, dd, t, a, b);
-                    const uint32_t id = gldu(S.prim_id + tr.lp);
+                    const PrimRec<D> pr = prim_load<D>(S, tr.lp);
+                    const bool hit = prim_eval<D>(pr, oo, dd, t, a, b);
+                    // the 3D record holds the element id
+                    const uint32_t id = D == 3 ? __float_as_uint(pr.g[D == 3 ? 2 : 0].y) : gldu(S.prim_id + tr.lp);
                     const bool ok = hit && id != sk;
                     const uint64_t v = ((uint64_t)triple32(pk ^ (id * 0x9E3779B9u)) << 32) | tr.lp;
                     cnt += ok ? 1u : 0u;
