@@ -502,7 +502,7 @@ __device__ __forceinline__ bool prim_eval(const PrimRec<D> &r, const float *o, c
         t = __builtin_fmaf(sx, g.w, -(sy * g.z)) * inv;
         a = __builtin_fmaf(d[0], sy, -(d[1] * sx)) * inv;
         b = 0.f;
-        return !(a < 0.0f || a > 1.0f || fabsf(t) <= eps || __builtin_isnan(t) || __builtin_isnan(a));
+        return a >= 0.0f && a <= 1.0f && fabsf(t) > eps;   // ordered comparisons reject NaN
     } else {
         const float4 g0 = r.g[0], g1 = r.g[1], g2 = r.g[2];
         // N = e1 x e2, rounded as the scene build used to store it (separate products)
@@ -517,8 +517,9 @@ __device__ __forceinline__ bool prim_eval(const PrimRec<D> &r, const float *o, c
         t = __builtin_fmaf(s0, n0, __builtin_fmaf(s1, n1, s2 * n2)) * inv;                      // s . N / det
         a = __builtin_fmaf(q0, g1.z, __builtin_fmaf(q1, g1.w, q2 * g2.x)) * inv;                // (d x s) . e2 / det
         b = -(__builtin_fmaf(q0, g0.w, __builtin_fmaf(q1, g1.x, q2 * g1.y))) * inv;            // -(d x s) . e1 / det
-        return !(a < 0.0f || b < 0.0f || a + b > 1.0f || fabsf(t) <= eps || __builtin_isnan(t) || __builtin_isnan(a) ||
-                 __builtin_isnan(b));
+        // the reference's !(a < 0 || b < 0 || a + b > 1 || |t| <= eps || isnan(t, a, b)): every
+        // comparison with a NaN is false, so the ordered form below rejects NaN too
+        return a >= 0.0f && b >= 0.0f && a + b <= 1.0f && fabsf(t) > eps;
     }
 }
 
@@ -612,17 +613,24 @@ template <int D> struct Trav {
         }
         return r;
     }
-    // slab parameters of child bound lo + q s: q (s inv) + (lo inv - o inv)
+    // slab parameters of child bound lo + q s: q (s inv) + (lo inv - o inv).  They grow with q
+    // where inv >= 0 and fall where inv < 0 (the rounded FMA is monotone in q), so the near
+    // bound of every child along an axis is its lower bound for inv >= 0 and its upper bound
+    // otherwise: one selection per node and axis instead of a min and a max per child.
     template <int STACK>
     __device__ __forceinline__ void node_apply(const NodeRec &r, uint32_t *stack, uint32_t stride, const float *inv,
                                                const float *oi) {
         const int ch[4] = {(int)__float_as_uint(r.chf.x), (int)__float_as_uint(r.chf.y), (int)__float_as_uint(r.chf.z),
                            (int)__float_as_uint(r.chf.w)};
         float A[D], B[D];
+        uint32_t qn[D], qf[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             A[k] = r.s[k] * inv[k];
             B[k] = __builtin_fmaf(r.lo[k], inv[k], -oi[k]);
+            const bool up = !(inv[k] < 0.f);
+            qn[k] = up ? r.qlo[k] : r.qhi[k];
+            qf[k] = up ? r.qhi[k] : r.qlo[k];
         }
         int next = kTravEmpty;
         const uint32_t full = (uint32_t)STACK * stride;
@@ -631,10 +639,10 @@ template <int D> struct Trav {
             float tmin = 0.f, tmax = 0.f;
 #pragma unroll
             for (int k = 0; k < D; ++k) {
-                const float l = (float)((r.qlo[k] >> (8 * c)) & 0xFFu), u = (float)((r.qhi[k] >> (8 * c)) & 0xFFu);
-                const float t1 = __builtin_fmaf(l, A[k], B[k]), t2 = __builtin_fmaf(u, A[k], B[k]);
-                tmin = k == 0 ? fminf(t1, t2) : fmaxf(tmin, fminf(t1, t2));
-                tmax = k == 0 ? fmaxf(t1, t2) : fminf(tmax, fmaxf(t1, t2));
+                const float n = (float)((qn[k] >> (8 * c)) & 0xFFu), f = (float)((qf[k] >> (8 * c)) & 0xFFu);
+                const float t1 = __builtin_fmaf(n, A[k], B[k]), t2 = __builtin_fmaf(f, A[k], B[k]);
+                tmin = k == 0 ? t1 : fmaxf(tmin, t1);
+                tmax = k == 0 ? t2 : fminf(tmax, t2);
             }
             const bool h = ch[c] != kTravEmpty && tmin <= tmax;
             const bool take = h && next == kTravEmpty;
