@@ -1,0 +1,144 @@
+"""GPU: scenes whose BVH is built on the device (wob_scene_create_ex, WOB_SCENE_DEVICE_BVH).
+
+The device build (Morton codes, radix tree, level-by-level collapse to the 4-wide layouts)
+gives a different tree from the host's SAH build, but nothing the engine returns depends on
+the tree: hit counts are sums and the hit pick is a minimum over the hits.  So every result
+is compared bit for bit with the host-built scene:
+* walk sums of line-sampled and uniformly sampled estimators, 2D and 3D;
+* all-hit line queries (count, hit set and line parameters);
+* closest points;
+* tiny and degenerate scenes (1, 2 and 3 elements; all elements on one centre).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import boundary_records, small_icosphere, star_scene
+from wobtoolbox_amd import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _dev(a, gpu):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a)).to(gpu)
+
+
+def _scenes(el):
+    import wobtoolbox_amd as w
+    return w.SceneHost(el), w.SceneHost(el, device_bvh=True)
+
+
+def _walk_cases():
+    circle = G.circle_dirichlet_2d(n_segments=1024, grid_res=32)
+    ico = G.icosphere_neumann_3d(n_points=2000, level=4)
+    star = star_scene(seed=5, n_segments=900, kind=2)
+    geo = G.geodesic_mixed_3d(n_points=2000, freq=24)
+    return [("circle_dl_2d", circle.elements, circle.solver, circle.points[::8], 64),
+            ("icosphere_direct_3d", ico.elements, ico.solver, ico.points, 32),
+            ("star_sl_ris_2d", star.elements, star.solver, star.points, 32),
+            ("geodesic_sl_mixed_3d", geo.elements, geo.solver, geo.points, 8)]
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_device_bvh_walks_equal_host_build(case, gpu):
+    import wobtoolbox_amd as w
+    name, el, solver, pts, paths = _walk_cases()[case]
+    host, dev = _scenes(el)
+    ih, idv = host.info(), dev.info()
+    _say(f"{name}: {el.n} elements; host BVH {ih['n_nodes']} nodes depth {ih['depth']}; "
+         f"device BVH {idv['n_nodes']} nodes depth {idv['depth']}")
+    assert 0 < idv["n_nodes"] < el.n and idv["depth"] > 0
+    x = _dev(pts.astype(np.float32), gpu)
+    out = []
+    for sc in (host, dev):
+        est = w.Estimator.from_config(sc, solver)
+        s, q = est.compute_domain(x, paths, seed=11, path_base=3)
+        out.append((s.cpu().numpy(), q.cpu().numpy()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_device_bvh_intersections_equal_host_build(dim, gpu):
+    el = star_scene(seed=2, n_segments=1500, kind=1).elements if dim == 2 else small_icosphere(4, "mixed")[0]
+    host, dev = _scenes(el)
+    rng = np.random.default_rng(40 + dim)
+    n = 4000
+    o = rng.uniform(-0.8, 0.8, (n, dim)).astype(np.float32)
+    o[: n // 3] = boundary_records(el, n // 3, rng)[:, :dim]
+    d = rng.normal(size=(n, dim))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    res = [[t.cpu().numpy() for t in sc.intersect_lines(_dev(o, gpu), _dev(d, gpu), max_hits=64)] for sc in (host, dev)]
+    (c0, i0, t0), (c1, i1, t1) = res
+    assert np.array_equal(c0, c1)
+    assert c0.max() <= 64
+    for k in range(n):
+        a = sorted(zip(i0[k, :c0[k]].tolist(), t0[k, :c0[k]].tolist()))
+        b = sorted(zip(i1[k, :c1[k]].tolist(), t1[k, :c1[k]].tolist()))
+        assert a == b, k
+    _say(f"dim {dim}: {n} lines, {int(c0.sum())} hits, identical")
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_device_bvh_closest_points_equal_host_build(dim, gpu):
+    el = star_scene(seed=4, n_segments=800, kind=0).elements if dim == 2 else small_icosphere(4, "neumann")[0]
+    host, dev = _scenes(el)
+    rng = np.random.default_rng(60 + dim)
+    x = rng.uniform(-1.2, 1.2, (5000, dim)).astype(np.float32)
+    a = [t.cpu().numpy() for t in host.closest_points(_dev(x, gpu))]
+    b = [t.cpu().numpy() for t in dev.closest_points(_dev(x, gpu))]
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v)
+
+
+def _tiny(dim, n, same_centre=False):
+    rng = np.random.default_rng(dim * 10 + n)
+    V = rng.uniform(-1, 1, (n, dim, dim)).astype(np.float32)
+    if same_centre:   # every element a rotation of one element about the origin: equal centres
+        base = rng.uniform(-1, 1, (dim, dim)).astype(np.float32)
+        base -= base.mean(0)
+        for i in range(n):
+            V[i] = base[np.roll(np.arange(dim), i % dim)]
+    nr = rng.normal(size=(n, dim)).astype(np.float32)
+    nr /= np.linalg.norm(nr, axis=1, keepdims=True)
+    return G.Elements(dim, V, nr, np.ones((n, dim), np.float32), np.zeros(n, np.int32), np.zeros((n, dim), np.float32))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("n,same", [(1, False), (2, False), (3, False), (64, True)])
+def test_device_bvh_tiny_and_degenerate_scenes(dim, n, same, gpu):
+    el = _tiny(dim, n, same)
+    host, dev = _scenes(el)
+    rng = np.random.default_rng(7)
+    m = 500
+    o = rng.uniform(-1, 1, (m, dim)).astype(np.float32)
+    d = rng.normal(size=(m, dim))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    (c0, i0, _), (c1, i1, _) = [[t.cpu().numpy() for t in sc.intersect_lines(_dev(o, gpu), _dev(d, gpu), max_hits=64)]
+                                for sc in (host, dev)]
+    assert np.array_equal(c0, c1)
+    for k in range(m):
+        assert sorted(i0[k, :c0[k]].tolist()) == sorted(i1[k, :c1[k]].tolist())
+    e0 = host.closest_points(_dev(o, gpu))[0].cpu().numpy()
+    e1 = dev.closest_points(_dev(o, gpu))[0].cpu().numpy()
+    assert np.array_equal(e0, e1)
+
+
+def test_device_bvh_flags(gpu):
+    import ctypes as C
+    import wobtoolbox_amd as w
+    from wobtoolbox_amd import _capi
+    el = _tiny(3, 8)
+    sc = w.SceneHost(el)
+    d = _capi.SceneDesc(el.dim, el.n, sc.elements.vertices.ctypes.data, sc.elements.normals.ctypes.data,
+                        sc.elements.boundary_values.ctypes.data, sc.elements.boundary_types.ctypes.data,
+                        sc.elements.robin_alphas.ctypes.data)
+    h = C.c_void_p()
+    with pytest.raises(w.WobError):
+        _capi.check(_capi.lib().wob_scene_create_ex(C.byref(d), 2, C.byref(h)))

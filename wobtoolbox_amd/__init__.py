@@ -76,7 +76,9 @@ def _stream_ptr(stream):
 class SceneHost:
     """Device scene: wide BVH, CDFs and element data (SceneHost, scene.cuh:264-326)."""
 
-    def __init__(self, elements: Elements):
+    def __init__(self, elements: Elements, device_bvh: bool = False):
+        """device_bvh: build the BVH on the GPU (wob_scene_create_ex, WOB_SCENE_DEVICE_BVH)
+        instead of the host's SAH build; walks give the same results either way."""
         _torch()
         el = elements.astype_contiguous()
         self.elements = el
@@ -85,7 +87,8 @@ class SceneHost:
                             el.boundary_values.ctypes.data, el.boundary_types.ctypes.data,
                             el.robin_alphas.ctypes.data)
         h = C.c_void_p()
-        check(lib().wob_scene_create(C.byref(d), C.byref(h)))
+        flags = _capi.SCENE_DEVICE_BVH if device_bvh else 0
+        check(lib().wob_scene_create_ex(C.byref(d), flags, C.byref(h)))
         self._h = h
 
     @property

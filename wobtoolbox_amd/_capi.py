@@ -15,7 +15,7 @@ STATUS_NAMES = {0: "WOB_OK", 1: "WOB_INVALID_ARG", 2: "WOB_INVALID_ENUM", 3: "WO
                 5: "WOB_DEVICE", 6: "WOB_CANCELLED"}
 
 # every symbol declared in include/wob_c.h
-EXPORTS = ["wob_scene_create", "wob_scene_destroy", "wob_scene_get_info", "wob_scene_set_volume_grid",
+EXPORTS = ["wob_scene_create", "wob_scene_create_ex", "wob_scene_destroy", "wob_scene_get_info", "wob_scene_set_volume_grid",
            "wob_scene_set_volume_gaussians",
            "wob_estimate_domain", "wob_estimate_boundary", "wob_intersect_lines", "wob_eval_kernels",
            "wob_eval_weights", "wob_philox4x32_10", "wob_string_to_enum", "wob_enum_to_string",
@@ -55,6 +55,7 @@ EST_CONTROL_VARIATE = 2
 EST_BOUNDARY_DENSITY = 4
 EST_INDEPENDENT_VOLUME_SAMPLES = 8
 VOLUME_NO_MASK = 1
+SCENE_DEVICE_BVH = 1
 
 
 class BatchEntry(C.Structure):
@@ -95,6 +96,7 @@ def lib():
     P, U32, U64, I32, F = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32, C.c_float
     sig = {
         "wob_scene_create": ([C.POINTER(SceneDesc), C.POINTER(P)], C.c_int),
+        "wob_scene_create_ex": ([C.POINTER(SceneDesc), U32, C.POINTER(P)], C.c_int),
         "wob_scene_destroy": ([P], C.c_int),
         "wob_scene_get_info": ([P, C.POINTER(SceneInfo)], C.c_int),
         "wob_scene_set_volume_grid": ([P, U32, F, P], C.c_int),

@@ -31,6 +31,7 @@ SOURCES = [
     ("wob_kernels.hip", "hip"),
     ("wob_vpl.hip", "hip"),
     ("wob_wos.hip", "hip"),
+    ("wob_bvh.hip", "hip"),
     ("mlp_kernels.hip", "hip"),
     ("wob_capi.cpp", "hip"),
     ("wob_cv.cpp", "hip"),

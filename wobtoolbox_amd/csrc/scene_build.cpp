@@ -165,7 +165,7 @@ void build_guide(const std::vector<float> &cdf, std::vector<uint32_t> &guide) {
     }
 }
 
-std::string build_scene(const wob_scene_desc &d, HostScene &S) {
+std::string build_scene(const wob_scene_desc &d, HostScene &S, bool with_bvh) {
     if (d.dim != 2 && d.dim != 3) return "dim must be 2 or 3";
     if (d.n_elements == 0) return "scene has no elements";
     if (!d.vertices || !d.normals || !d.boundary_values || !d.boundary_types) return "null element array";
@@ -215,6 +215,7 @@ std::string build_scene(const wob_scene_desc &d, HostScene &S) {
     }
     build_cdf(area, S.area_cdf, S.total_area, S.area_guide);
     build_cdf(bvw, S.bv_cdf, S.total_bv, S.bv_guide);
+    if (!with_bvh) return "";
 
     // ---- binary SAH build
     Builder B;

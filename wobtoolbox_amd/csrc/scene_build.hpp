@@ -51,8 +51,10 @@ struct HostScene {
     uint32_t n_nodes = 0, depth = 0, stack_need = 0;
 };
 
-// Returns an empty string on success, otherwise an error message.
-std::string build_scene(const wob_scene_desc &desc, HostScene &out);
+// Returns an empty string on success, otherwise an error message.  Without with_bvh only the
+// element attributes and the CDFs are built (the BVH arrays stay empty: the device build
+// supplies them).
+std::string build_scene(const wob_scene_desc &desc, HostScene &out, bool with_bvh = true);
 
 // Quantized copy of float nodes (kNodeWords{2,3} each) in the kQNodeWords{2,3} layout.
 void quantize_nodes(int dim, const std::vector<float> &nodes, std::vector<uint32_t> &qnodes);

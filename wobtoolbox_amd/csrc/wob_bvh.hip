@@ -1,0 +1,502 @@
+// Device BVH build for the MI355X walk-on-boundary engine.
+//
+// The reference builds its lbvh binary BVH on the device when a SceneHost is constructed
+// (scene.cuh:264-299, deps/lbvh: Morton codes, a radix tree, bottom-up boxes).  This build
+// follows the same published pipeline (Karras 2012, "Maximizing parallelism in the
+// construction of BVHs, octrees, and k-d trees") and then writes the 4-wide layouts the
+// traversal reads (scene_build.hpp), so a scene built here walks exactly like a host-built
+// one: hit counts and picks do not depend on the tree.
+//   1. per element: the inflated box (4e-6 |coordinate| + 1e-7, as the host build) and the
+//      Morton code of its centre in the vertex bounds (3 x 10 bits in 3D, 2 x 16 in 2D);
+//   2. radix sort of (code, element) pairs (hipCUB);
+//   3. the binary radix tree, one internal node per thread; equal codes are ordered by
+//      index (the delta function falls back to the index bits);
+//   4. boxes bottom-up: each leaf walks to the root, the second child to arrive at a node
+//      writes its box (device-scope fences and coherent loads);
+//   5. top-down, one level per launch: each binary node of the level becomes a wide node
+//      whose children are its binary children, expanded largest-measure-first until there
+//      are four (the host's collapse rule); child indices of the next level come from an
+//      exclusive scan of the internal-child counts, so the layout is deterministic; the
+//      float node and its quantized copy are written in the same kernel;
+//   6. the traversal stack bound, bottom-up over the levels;
+//   7. primitive records in sorted (leaf) order.
+// Leaves hold one primitive each.
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "scene_build.hpp"
+#include "wob_launch.hpp"
+
+namespace wobb {
+
+using wobhost::kNodeWords2;
+using wobhost::kNodeWords3;
+using wobhost::kPrimWords2;
+using wobhost::kPrimWords3;
+using wobhost::kQNodeWords2;
+using wobhost::kQNodeWords3;
+
+struct Box6 {
+    float lo[3], hi[3];
+};
+
+// spread the low bits of v so that there are (gap) zero bits between them
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {   // 10 bits -> every third bit
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ __forceinline__ uint32_t spread2(uint32_t v) {   // 16 bits -> every second bit
+    v &= 0xFFFFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+// 1: element boxes (inflated as scene_build.cpp), Morton codes of their centres
+template <int D>
+__global__ __launch_bounds__(256) void leaf_kernel(uint32_t n, const float *__restrict__ V, float3 vmin, float3 vscale,
+                                                   uint32_t *__restrict__ code, uint32_t *__restrict__ idx,
+                                                   Box6 *__restrict__ pbox) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    Box6 b;
+    for (int k = 0; k < 3; ++k) { b.lo[k] = INFINITY; b.hi[k] = -INFINITY; }
+    for (int j = 0; j < D; ++j)
+        for (int k = 0; k < 3; ++k) {
+            const float p = k < D ? V[((size_t)e * D + j) * D + k] : 0.f;
+            b.lo[k] = fminf(b.lo[k], p);
+            b.hi[k] = fmaxf(b.hi[k], p);
+        }
+    for (int k = 0; k < 3; ++k) {
+        const float m = fmaxf(fabsf(b.lo[k]), fabsf(b.hi[k]));
+        const float pad = m * 4.0e-6f + 1.0e-7f;
+        b.lo[k] -= pad;
+        b.hi[k] += pad;
+    }
+    pbox[e] = b;
+    const float c[3] = {0.5f * (b.lo[0] + b.hi[0]), 0.5f * (b.lo[1] + b.hi[1]), 0.5f * (b.lo[2] + b.hi[2])};
+    const float mn[3] = {vmin.x, vmin.y, vmin.z}, sc[3] = {vscale.x, vscale.y, vscale.z};
+    uint32_t q[3];
+    const float cells = D == 3 ? 1024.f : 65536.f;
+    for (int k = 0; k < 3; ++k) {
+        const float u = fminf(fmaxf((c[k] - mn[k]) * sc[k], 0.f), 1.f);
+        q[k] = (uint32_t)fminf(u * cells, cells - 1.f);
+    }
+    code[e] = D == 3 ? (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]) : (spread2(q[0]) << 1) | spread2(q[1]);
+    idx[e] = e;
+}
+
+// length of the common prefix of sorted codes i and j (index bits break ties); -1 outside
+__device__ __forceinline__ int delta(const uint32_t *code, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    const uint32_t a = code[i], b = code[j];
+    if (a == b) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+    return __clz(a ^ b);
+}
+
+// 3: binary radix tree.  Node ids: internal i in [0, n-1), leaf j as (n - 1) + j.
+__global__ __launch_bounds__(256) void tree_kernel(int n, const uint32_t *__restrict__ code, uint32_t *__restrict__ left,
+                                                   uint32_t *__restrict__ right, uint32_t *__restrict__ parent) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n - 1) return;
+    const int d = delta(code, n, i, i + 1) - delta(code, n, i, i - 1) >= 0 ? 1 : -1;
+    const int dmin = delta(code, n, i, i - d);
+    int lmax = 2;
+    while (delta(code, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (delta(code, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = delta(code, n, i, j);
+    int s = 0;
+    for (int div = 2;; div *= 2) {
+        const int t = (l + div - 1) / div;
+        if (delta(code, n, i, i + (s + t) * d) > dnode) s += t;
+        if (t <= 1) break;
+    }
+    const int g = i + s * d + min(d, 0);
+    const uint32_t lc = min(i, j) == g ? (uint32_t)(n - 1 + g) : (uint32_t)g;
+    const uint32_t rc = max(i, j) == g + 1 ? (uint32_t)(n - 1 + g + 1) : (uint32_t)(g + 1);
+    left[i] = lc;
+    right[i] = rc;
+    parent[lc] = (uint32_t)i;
+    parent[rc] = (uint32_t)i;
+}
+
+__device__ __forceinline__ float coherent_load(const float *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// 4: boxes bottom-up; box[id] for every node id (leaves in sorted order)
+__global__ __launch_bounds__(256) void bounds_kernel(int n, const uint32_t *__restrict__ idx, const Box6 *__restrict__ pbox,
+                                                     const uint32_t *__restrict__ left, const uint32_t *__restrict__ right,
+                                                     const uint32_t *__restrict__ parent, uint32_t *__restrict__ flag,
+                                                     Box6 *box) {
+    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= n) return;
+    const uint32_t leaf = (uint32_t)(n - 1 + j);
+    box[leaf] = pbox[idx[j]];
+    if (n == 1) return;
+    uint32_t p = parent[leaf];
+    for (int up = 0; up < 128; ++up) {   // the radix tree over 64-bit (code, index) keys is at most 64 deep
+        __threadfence();
+        if (atomicAdd(&flag[p], 1u) == 0u) return;   // the sibling subtree is not done yet
+        Box6 b;
+        const float *bl = box[left[p]].lo, *br = box[right[p]].lo;
+        for (int k = 0; k < 6; ++k) {
+            const float u = coherent_load(bl + k), v = coherent_load(br + k);
+            (k < 3 ? b.lo[k] : b.hi[k - 3]) = k < 3 ? fminf(u, v) : fmaxf(u, v);
+        }
+        box[p] = b;
+        if (p == 0u) return;
+        p = parent[p];
+    }
+}
+
+template <int D> __device__ __forceinline__ float measure(const Box6 &b) {
+    float e[3];
+    for (int k = 0; k < 3; ++k) e[k] = fmaxf(0.f, b.hi[k] - b.lo[k]);
+    return D == 2 ? e[0] + e[1] : e[0] * e[1] + e[1] * e[2] + e[2] * e[0];
+}
+
+// children of binary node b as a wide node (host rule: expand the internal child of largest
+// measure, first one on ties, until there are four or only leaves)
+template <int D>
+__device__ __forceinline__ int expand(int n, uint32_t b, const uint32_t *left, const uint32_t *right, const Box6 *box,
+                                      uint32_t kids[4]) {
+    const uint32_t n_int = (uint32_t)(n - 1);
+    int nk;
+    if (b >= n_int) { kids[0] = b; nk = 1; }
+    else { kids[0] = left[b]; kids[1] = right[b]; nk = 2; }
+    while (nk < 4) {
+        int best = -1;
+        float bestA = -1.f;
+        for (int i = 0; i < nk; ++i)
+            if (kids[i] < n_int) {
+                const float A = measure<D>(box[kids[i]]);
+                if (A > bestA) { bestA = A; best = i; }
+            }
+        if (best < 0) break;
+        const uint32_t k = kids[best];
+        kids[best] = left[k];
+        kids[nk++] = right[k];
+    }
+    return nk;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void count_kernel(int n, uint32_t m, const uint32_t *__restrict__ level,
+                                                    const uint32_t *__restrict__ left, const uint32_t *__restrict__ right,
+                                                    const Box6 *__restrict__ box, uint32_t *__restrict__ counts) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    uint32_t kids[4];
+    const int nk = expand<D>(n, level[t], left, right, box, kids);
+    uint32_t c = 0;
+    for (int i = 0; i < nk; ++i) c += kids[i] < (uint32_t)(n - 1) ? 1u : 0u;
+    counts[t] = c;
+}
+
+// 8-bit child bounds of one axis (the rule of wobhost::quantize_nodes, checked in double)
+__device__ void quantize_axis(const float *clo, const float *chi, int nk, float &lo, float &s, uint32_t &qlo,
+                              uint32_t &qhi) {
+    float a = clo[0], b = chi[0];
+    for (int c = 1; c < nk; ++c) { a = fminf(a, clo[c]); b = fmaxf(b, chi[c]); }
+    float st = (float)(((double)b - (double)a) / 255.0);
+    while ((double)a + 255.0 * (double)st < (double)b) st = nextafterf(st, INFINITY);
+    lo = a;
+    s = st;
+    qlo = 0u;
+    qhi = 0u;
+    for (int c = 0; c < nk; ++c) {
+        uint32_t ql = 0u, qh = 0u;
+        if (st > 0.f) {
+            const double l = clo[c], h = chi[c];
+            double fl = floor((l - (double)a) / (double)st), ch = ceil((h - (double)a) / (double)st);
+            fl = fmin(fmax(fl, 0.0), 255.0);
+            ch = fmin(fmax(ch, 0.0), 255.0);
+            while (fl > 0.0 && (double)a + fl * (double)st > l) fl -= 1.0;
+            while (ch < 255.0 && (double)a + ch * (double)st < h) ch += 1.0;
+            ql = (uint32_t)fl;
+            qh = (uint32_t)ch;
+        }
+        qlo |= ql << (8 * c);
+        qhi |= qh << (8 * c);
+    }
+}
+
+// 5: wide node w = base + t for binary node level[t]; next-level entries at next[offs[t] + r]
+template <int D>
+__global__ __launch_bounds__(256) void write_kernel(int n, uint32_t m, uint32_t base, uint32_t next_base,
+                                                    const uint32_t *__restrict__ level, const uint32_t *__restrict__ offs,
+                                                    const uint32_t *__restrict__ left, const uint32_t *__restrict__ right,
+                                                    const Box6 *__restrict__ box, uint32_t *__restrict__ next,
+                                                    float *__restrict__ fnodes, uint32_t *__restrict__ qnodes) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    constexpr int NW = D == 2 ? kNodeWords2 : kNodeWords3;
+    constexpr int QW = D == 2 ? kQNodeWords2 : kQNodeWords3;
+    const uint32_t n_int = (uint32_t)(n - 1);
+    uint32_t kids[4];
+    const int nk = expand<D>(n, level[t], left, right, box, kids);
+    const uint32_t w = base + t;
+    float *nd = fnodes + (size_t)w * NW;
+    uint32_t *q = qnodes + (size_t)w * QW;
+    uint32_t r = 0, child[4], mask = 0;
+    float clo[3][4], chi[3][4];
+    for (int c = 0; c < 4; ++c) {
+        Box6 b;
+        for (int k = 0; k < 3; ++k) { b.lo[k] = 0.f; b.hi[k] = 0.f; }
+        child[c] = 0x80000000u;
+        if (c < nk) {
+            b = box[kids[c]];
+            mask |= 1u << c;
+            if (kids[c] < n_int) {
+                const uint32_t slot = offs[t] + r++;
+                next[slot] = kids[c];
+                child[c] = next_base + slot;
+            } else {
+                child[c] = ~((kids[c] - n_int) << 4);   // one primitive: ~((first << 4) | 0)
+            }
+        }
+        for (int k = 0; k < D; ++k) {
+            clo[k][c] = b.lo[k];
+            chi[k][c] = b.hi[k];
+            nd[4 * k + c] = b.lo[k];
+            nd[4 * (D + k) + c] = b.hi[k];
+        }
+        nd[4 * 2 * D + c] = __uint_as_float(c < nk ? child[c] : 0u);
+    }
+    nd[NW - 4] = __uint_as_float(mask);
+    nd[NW - 3] = 0.f;
+    nd[NW - 2] = 0.f;
+    nd[NW - 1] = 0.f;
+    float lo[3] = {0.f, 0.f, 0.f}, s[3] = {0.f, 0.f, 0.f};
+    uint32_t qlo[3] = {0u, 0u, 0u}, qhi[3] = {0u, 0u, 0u};
+    for (int k = 0; k < D; ++k) quantize_axis(clo[k], chi[k], nk, lo[k], s[k], qlo[k], qhi[k]);
+    if constexpr (D == 2) {
+        q[0] = __float_as_uint(lo[0]); q[1] = __float_as_uint(lo[1]); q[2] = __float_as_uint(s[0]); q[3] = __float_as_uint(s[1]);
+        q[4] = qlo[0]; q[5] = qlo[1]; q[6] = qhi[0]; q[7] = qhi[1];
+    } else {
+        q[0] = __float_as_uint(lo[0]); q[1] = __float_as_uint(lo[1]); q[2] = __float_as_uint(lo[2]); q[3] = __float_as_uint(s[0]);
+        q[4] = __float_as_uint(s[1]); q[5] = __float_as_uint(s[2]); q[6] = qlo[0]; q[7] = qlo[1];
+        q[8] = qlo[2]; q[9] = qhi[0]; q[10] = qhi[1]; q[11] = qhi[2];
+    }
+    for (int c = 0; c < 4; ++c) q[QW - 4 + c] = child[c];
+}
+
+// 6: stack bound of wide node w (children of the next level are done):
+// need = (#children - 1) + max need(internal child)
+template <int D>
+__global__ __launch_bounds__(256) void need_kernel(uint32_t base, uint32_t m, const float *__restrict__ fnodes,
+                                                   uint32_t *need) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    constexpr int NW = D == 2 ? kNodeWords2 : kNodeWords3;
+    const float *nd = fnodes + (size_t)(base + t) * NW;
+    const uint32_t mask = __float_as_uint(nd[NW - 4]);
+    uint32_t nk = 0, mx = 0;
+    for (int c = 0; c < 4; ++c) {
+        if (!((mask >> c) & 1u)) continue;
+        ++nk;
+        const int32_t ch = (int32_t)__float_as_uint(nd[4 * 2 * D + c]);
+        if (ch >= 0) mx = max(mx, need[ch]);
+    }
+    need[base + t] = (nk > 0 ? nk - 1 : 0) + mx;
+}
+
+// 7: primitive records in leaf (sorted) order, as scene_build.cpp writes them
+template <int D>
+__global__ __launch_bounds__(256) void prims_kernel(uint32_t n, const float *__restrict__ V, const uint32_t *__restrict__ idx,
+                                                    float *__restrict__ prims, uint32_t *__restrict__ prim_id) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t e = idx[j];
+    prim_id[j] = e;
+    auto v = [&](int a, int k) { return V[((size_t)e * D + a) * D + k]; };
+    if constexpr (D == 2) {
+        float *g = prims + (size_t)j * kPrimWords2;
+        g[0] = v(0, 0); g[1] = v(0, 1);
+        g[2] = v(0, 0) - v(1, 0); g[3] = v(0, 1) - v(1, 1);
+    } else {
+        float *g = prims + (size_t)j * kPrimWords3;
+        for (int k = 0; k < 3; ++k) {
+            g[k] = v(0, k);
+            g[3 + k] = v(0, k) - v(1, k);
+            g[6 + k] = v(0, k) - v(2, k);
+        }
+        g[9] = __uint_as_float(e);
+        g[10] = 0.f;
+        g[11] = 0.f;
+    }
+}
+
+}  // namespace wobb
+
+namespace wobl {
+
+using namespace wobb;
+
+namespace {
+
+struct DevBuf {
+    std::vector<void *> ptrs;
+    hipError_t err = hipSuccess;
+    template <typename T> T *get(size_t count) {
+        void *p = nullptr;
+        if (err == hipSuccess) err = hipMalloc(&p, std::max<size_t>(count * sizeof(T), 16));
+        if (err == hipSuccess) ptrs.push_back(p);
+        return (T *)p;
+    }
+    ~DevBuf() {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+};
+
+inline uint32_t blocks(uint64_t n) { return (uint32_t)((n + 255) / 256); }
+
+template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBvh &out) {
+    constexpr int NW = D == 2 ? kNodeWords2 : kNodeWords3;
+    constexpr int QW = D == 2 ? kQNodeWords2 : kQNodeWords3;
+    constexpr int PW = D == 2 ? kPrimWords2 : kPrimWords3;
+    const hipStream_t st = nullptr;
+    // vertex bounds (the Morton grid)
+    float mn[3] = {0.f, 0.f, 0.f}, mx[3] = {0.f, 0.f, 0.f};
+    for (int k = 0; k < D; ++k) { mn[k] = INFINITY; mx[k] = -INFINITY; }
+    for (size_t i = 0; i < (size_t)n * D; ++i)
+        for (int k = 0; k < D; ++k) {
+            mn[k] = std::min(mn[k], h_vertices[i * D + k]);
+            mx[k] = std::max(mx[k], h_vertices[i * D + k]);
+        }
+    float sc[3] = {0.f, 0.f, 0.f};
+    for (int k = 0; k < D; ++k) sc[k] = mx[k] > mn[k] ? 1.0f / (mx[k] - mn[k]) : 0.f;
+    if (!std::isfinite(mn[0]) || !std::isfinite(mx[0])) return "non-finite vertex coordinates";
+
+    DevBuf tmp;
+    float *V = tmp.get<float>((size_t)n * D * D);
+    uint32_t *code = tmp.get<uint32_t>(n), *code2 = tmp.get<uint32_t>(n);
+    uint32_t *idx = tmp.get<uint32_t>(n), *idx2 = tmp.get<uint32_t>(n);
+    Box6 *pbox = tmp.get<Box6>(n);
+    const uint32_t n_int = n - 1, n_all = 2 * n - 1;
+    uint32_t *left = tmp.get<uint32_t>(std::max<uint32_t>(n_int, 1)), *right = tmp.get<uint32_t>(std::max<uint32_t>(n_int, 1));
+    uint32_t *parent = tmp.get<uint32_t>(n_all), *flag = tmp.get<uint32_t>(std::max<uint32_t>(n_int, 1));
+    Box6 *box = tmp.get<Box6>(n_all);
+    // wide nodes: at most one per binary internal node (or the single leaf root)
+    const uint32_t max_wide = std::max<uint32_t>(n_int, 1);
+    uint32_t *lv_a = tmp.get<uint32_t>(max_wide), *lv_b = tmp.get<uint32_t>(max_wide);
+    uint32_t *counts = tmp.get<uint32_t>(max_wide), *offs = tmp.get<uint32_t>(max_wide);
+    uint32_t *need = tmp.get<uint32_t>(max_wide);
+    if (tmp.err != hipSuccess) return std::string("hipMalloc during the device BVH build: ") + hipGetErrorString(tmp.err);
+
+#define WOB_TRY(x)                                                                              \
+    do {                                                                                        \
+        const hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess) return std::string("device BVH build: ") + hipGetErrorString(e_); \
+    } while (0)
+    WOB_TRY(hipMemcpyAsync(V, h_vertices, (size_t)n * D * D * sizeof(float), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL((leaf_kernel<D>), dim3(blocks(n)), dim3(256), 0, st, n, V, make_float3(mn[0], mn[1], mn[2]),
+                       make_float3(sc[0], sc[1], sc[2]), code, idx, pbox);
+    WOB_TRY(hipGetLastError());
+    size_t sort_bytes = 0, scan_bytes = 0;
+    WOB_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, code, code2, idx, idx2, (int)n, 0, 32, st));
+    WOB_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, counts, offs, (int)max_wide, st));
+    void *cub_tmp = tmp.get<char>(std::max(sort_bytes, scan_bytes));
+    if (tmp.err != hipSuccess) return std::string("hipMalloc during the device BVH build: ") + hipGetErrorString(tmp.err);
+    WOB_TRY(hipcub::DeviceRadixSort::SortPairs(cub_tmp, sort_bytes, code, code2, idx, idx2, (int)n, 0, 32, st));
+    if (n > 1) {
+        hipLaunchKernelGGL(tree_kernel, dim3(blocks(n_int)), dim3(256), 0, st, (int)n, code2, left, right, parent);
+        WOB_TRY(hipGetLastError());
+        WOB_TRY(hipMemsetAsync(flag, 0, (size_t)n_int * sizeof(uint32_t), st));
+    }
+    hipLaunchKernelGGL(bounds_kernel, dim3(blocks(n)), dim3(256), 0, st, (int)n, idx2, pbox, left, right, parent, flag, box);
+    WOB_TRY(hipGetLastError());
+
+    // outputs
+    float *fnodes = nullptr, *prims = nullptr;
+    uint32_t *qnodes = nullptr, *prim_id = nullptr;
+    WOB_TRY(hipMalloc(&fnodes, (size_t)max_wide * NW * sizeof(float)));
+    out.fnodes = fnodes;
+    WOB_TRY(hipMalloc(&qnodes, (size_t)max_wide * QW * sizeof(uint32_t)));
+    out.qnodes = qnodes;
+    WOB_TRY(hipMalloc(&prims, (size_t)n * PW * sizeof(float)));
+    out.prims = prims;
+    WOB_TRY(hipMalloc(&prim_id, (size_t)n * sizeof(uint32_t)));
+    out.prim_id = prim_id;
+    out.bytes = (size_t)max_wide * (NW + QW) * 4 + (size_t)n * (PW + 1) * 4;
+
+    // top-down levels
+    const uint32_t root = 0u;   // internal node 0, or the single leaf (id n - 1 = 0) when n == 1
+    WOB_TRY(hipMemcpyAsync(lv_a, &root, sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    std::vector<uint32_t> level_base{0u}, level_size{1u};
+    uint32_t *cur = lv_a, *nxt = lv_b;
+    uint32_t base = 0, m = 1;
+    while (m > 0) {
+        if (level_base.size() > 256) return "device BVH build: tree deeper than 256 levels";
+        hipLaunchKernelGGL((count_kernel<D>), dim3(blocks(m)), dim3(256), 0, st, (int)n, m, cur, left, right, box, counts);
+        WOB_TRY(hipGetLastError());
+        WOB_TRY(hipcub::DeviceScan::ExclusiveSum(cub_tmp, scan_bytes, counts, offs, (int)m, st));
+        uint32_t last[2] = {0u, 0u};
+        WOB_TRY(hipMemcpyAsync(&last[0], counts + (m - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        WOB_TRY(hipMemcpyAsync(&last[1], offs + (m - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        const uint32_t next_base = base + m;
+        hipLaunchKernelGGL((write_kernel<D>), dim3(blocks(m)), dim3(256), 0, st, (int)n, m, base, next_base, cur, offs, left,
+                           right, box, nxt, fnodes, qnodes);
+        WOB_TRY(hipGetLastError());
+        WOB_TRY(hipStreamSynchronize(st));
+        const uint32_t m_next = last[0] + last[1];
+        if (next_base + m_next > max_wide) return "device BVH build: internal error (wide node count)";
+        std::swap(cur, nxt);
+        base = next_base;
+        m = m_next;
+        if (m > 0) {
+            level_base.push_back(base);
+            level_size.push_back(m);
+        }
+    }
+    const uint32_t n_wide = base;
+    for (size_t l = level_base.size(); l-- > 0;) {
+        hipLaunchKernelGGL((need_kernel<D>), dim3(blocks(level_size[l])), dim3(256), 0, st, level_base[l], level_size[l],
+                           fnodes, need);
+        WOB_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL((prims_kernel<D>), dim3(blocks(n)), dim3(256), 0, st, n, V, idx2, prims, prim_id);
+    WOB_TRY(hipGetLastError());
+    uint32_t need0 = 0;
+    WOB_TRY(hipMemcpyAsync(&need0, need, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    WOB_TRY(hipStreamSynchronize(st));
+#undef WOB_TRY
+    out.n_nodes = n_wide;
+    out.depth = (uint32_t)level_base.size();
+    out.stack_need = need0 + 1;
+    return "";
+}
+
+}  // namespace
+
+std::string build_bvh_device(int dim, uint32_t n, const float *h_vertices, DeviceBvh &out) {
+    out = DeviceBvh{};
+    std::string err = dim == 2 ? build<2>(n, h_vertices, out) : build<3>(n, h_vertices, out);
+    if (!err.empty()) {
+        (void)hipFree(out.fnodes);
+        (void)hipFree(out.qnodes);
+        (void)hipFree(out.prims);
+        (void)hipFree(out.prim_id);
+        out = DeviceBvh{};
+    }
+    return err;
+}
+
+}  // namespace wobl

@@ -438,13 +438,19 @@ wob_status batch_pass(wob_batch *b, uint64_t seed, hipStream_t hs) {
 extern "C" {
 
 wob_status wob_scene_create(const wob_scene_desc *desc, wob_scene **out_scene) {
+    return wob_scene_create_ex(desc, 0u, out_scene);
+}
+
+wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_scene **out_scene) {
     if (!desc || !out_scene) return fail(WOB_INVALID_ARG, "null argument");
     *out_scene = nullptr;
+    if (flags & ~(uint32_t)WOB_SCENE_DEVICE_BVH) return fail(WOB_INVALID_ARG, "unknown scene flags");
+    const bool device_bvh = (flags & WOB_SCENE_DEVICE_BVH) != 0u;
     wob_scene *s = new (std::nothrow) wob_scene();
     if (!s) return fail(WOB_OOM, "host allocation failed");
     std::string err;
     try {
-        err = wobhost::build_scene(*desc, s->host);
+        err = wobhost::build_scene(*desc, s->host, !device_bvh);
     } catch (const std::bad_alloc &) {
         delete s;
         return fail(WOB_OOM, "host allocation failed during scene build");
@@ -453,22 +459,44 @@ wob_status wob_scene_create(const wob_scene_desc *desc, wob_scene **out_scene) {
         delete s;
         return fail(WOB_INVALID_ARG, err);
     }
+    (void)hipGetDevice(&s->device);
+    wobl::DeviceBvh dv;
+    if (device_bvh) {
+        err = wobl::build_bvh_device((int)desc->dim, desc->n_elements, desc->vertices, dv);
+        if (!err.empty()) {
+            delete s;
+            return fail(WOB_DEVICE, err);
+        }
+        s->bufs[0] = dv.fnodes;
+        s->bufs[1] = dv.prims;
+        s->bufs[2] = dv.prim_id;
+        s->bufs[8] = dv.qnodes;
+        s->device_bytes += dv.bytes;
+        s->host.n_nodes = dv.n_nodes;
+        s->host.depth = dv.depth;
+        s->host.stack_need = dv.stack_need;
+    }
     if (s->host.stack_need > (uint32_t)wobl::kStackEntries) {
         const std::string m = "BVH traversal stack bound " + std::to_string(s->host.stack_need) + " exceeds " +
                               std::to_string(wobl::kStackEntries);
+        free_scene(s);
         delete s;
         return fail(WOB_INVALID_ARG, m);
     }
-    (void)hipGetDevice(&s->device);
     const auto &h = s->host;
-    wob_status st;
-    const float *nodes = nullptr, *prims = nullptr, *attr = nullptr, *acdf = nullptr, *bcdf = nullptr;
-    const uint32_t *pid = nullptr, *ag = nullptr, *bg = nullptr, *qn = nullptr;
-    if ((st = upload(s, 0, h.nodes, &nodes)) != WOB_OK || (st = upload(s, 1, h.prim_geom, &prims)) != WOB_OK ||
-        (st = upload(s, 2, h.prim_id, &pid)) != WOB_OK || (st = upload(s, 3, h.elem_attr, &attr)) != WOB_OK ||
-        (st = upload(s, 4, h.area_cdf, &acdf)) != WOB_OK || (st = upload(s, 5, h.area_guide, &ag)) != WOB_OK ||
-        (st = upload(s, 6, h.bv_cdf, &bcdf)) != WOB_OK || (st = upload(s, 7, h.bv_guide, &bg)) != WOB_OK ||
-        (st = upload(s, 8, h.qnodes, &qn)) != WOB_OK) {
+    wob_status st = WOB_OK;
+    const float *nodes = dv.fnodes, *prims = dv.prims, *attr = nullptr, *acdf = nullptr, *bcdf = nullptr;
+    const uint32_t *pid = dv.prim_id, *ag = nullptr, *bg = nullptr, *qn = dv.qnodes;
+    if (!device_bvh &&
+        ((st = upload(s, 0, h.nodes, &nodes)) != WOB_OK || (st = upload(s, 1, h.prim_geom, &prims)) != WOB_OK ||
+         (st = upload(s, 2, h.prim_id, &pid)) != WOB_OK || (st = upload(s, 8, h.qnodes, &qn)) != WOB_OK)) {
+        free_scene(s);
+        delete s;
+        return st;
+    }
+    if ((st = upload(s, 3, h.elem_attr, &attr)) != WOB_OK || (st = upload(s, 4, h.area_cdf, &acdf)) != WOB_OK ||
+        (st = upload(s, 5, h.area_guide, &ag)) != WOB_OK || (st = upload(s, 6, h.bv_cdf, &bcdf)) != WOB_OK ||
+        (st = upload(s, 7, h.bv_guide, &bg)) != WOB_OK) {
         free_scene(s);
         delete s;
         return st;

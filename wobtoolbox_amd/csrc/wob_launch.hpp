@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "wob_device.hpp"
 
 namespace wobl {
@@ -111,5 +113,19 @@ uint32_t persistent_grid(int dim, bool boundary, bool volume, int bie, int est, 
 bool supports_wavefront(int dim, bool boundary, bool volume, int bie, int est, int flavour);
 // LDS bytes of a walk launch (per-lane traversal stacks)
 size_t walk_lds_bytes(int flavour, bool volume, uint32_t block);
+
+// Device BVH build (wob_bvh.hip): the float and quantized 4-wide nodes, the primitive
+// records and the leaf-order element ids of a scene, built on the current device from host
+// vertex data (n * dim * dim floats).  The buffers are device allocations owned by the
+// caller afterwards.  Returns an empty string on success.
+struct DeviceBvh {
+    float *fnodes = nullptr;
+    uint32_t *qnodes = nullptr;
+    float *prims = nullptr;
+    uint32_t *prim_id = nullptr;
+    uint32_t n_nodes = 0, depth = 0, stack_need = 0;
+    size_t bytes = 0;
+};
+std::string build_bvh_device(int dim, uint32_t n, const float *h_vertices, DeviceBvh &out);
 
 }  // namespace wobl
