@@ -259,6 +259,37 @@ def vpl_leg(args, torch, wob, G):
                       "RIS 1024, eps 1e-4"}
 
 
+def scene_build_leg(args, torch, wob, G, headline):
+    """Scene construction with the BVH built on the host (binned SAH) and on the device (LBVH
+    collapsed to the same 4-wide layouts), on the 200k-triangle geodesic sphere, and the
+    headline walk on a device-built scene (same results; the tree differs)."""
+    print("[bench] scene build leg", file=sys.stderr, flush=True)
+    geo = G.geodesic_mixed_3d(n_points=1000)
+    out = {"workload": "geodesic_mixed_3d", "elements": int(geo.elements.n)}
+    for name, dev_bvh in (("host_bvh", False), ("device_bvh", True)):
+        ts = []
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sc = wob.SceneHost(geo.elements, device_bvh=dev_bvh)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+            info = sc.info()
+            sc.close()
+        out[name] = {"create_s": min(ts), "nodes": int(info["n_nodes"]), "depth": int(info["depth"])}
+    # the headline walk on a device-built tree
+    w, pts, paths = headline
+    sc = wob.SceneHost(w.elements, device_bvh=True)
+    est = wob.Estimator.from_config(sc, w.solver)
+    est.time_walk = True
+    ms = []
+    for i in range(3):
+        est.compute_domain(pts, paths, seed=0, path_base=i * paths)
+        ms.append(est.last_launch.walk_ms)
+    out["headline_walk_ms_device_bvh"] = float(np.mean(ms[1:]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,6 +308,7 @@ def main():
     ap.add_argument("--poisson-passes", type=int, default=4)
     ap.add_argument("--no-vpl", action="store_true")
     ap.add_argument("--vpl-iterations", type=int, default=5)
+    ap.add_argument("--no-scene-build", action="store_true")
     ap.add_argument("--cv-points", type=int, default=1_000_000)
     ap.add_argument("--cv-budget", type=float, default=15.0, help="walk seconds per arm")
     ap.add_argument("--cv-paths-per-pass", type=int, default=4)
@@ -367,6 +399,10 @@ def main():
     if not args.no_vpl:
         vpl = vpl_leg(args, torch, wob, G)
 
+    build = None
+    if not args.no_scene_build:
+        build = scene_build_leg(args, torch, wob, G, (w, pts, paths))
+
     ttr = None
     if not args.no_time_to_rmse:
         ttr = time_to_rmse_leg(args, torch, wob, G)
@@ -397,6 +433,7 @@ def main():
         "batch": batch,
         "poisson": poisson,
         "vpl": vpl,
+        "scene_build": build,
         "accuracy": {"rmse_u": rmse, "paths_per_point_total": step * paths,
                      "note": "Neumann problem: u compared after removing the mean offset"
                      if w.solver["problem"] == "NeumannProblem" else "u compared directly"},
