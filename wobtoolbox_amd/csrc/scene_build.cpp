@@ -165,6 +165,19 @@ void build_guide(const std::vector<float> &cdf, std::vector<uint32_t> &guide) {
     }
 }
 
+std::vector<float> guide_records(const std::vector<float> &cdf, const std::vector<uint32_t> &guide) {
+    const size_t n = cdf.size();
+    std::vector<float> r(4 * guide.size(), 0.f);
+    for (size_t k = 0; k < guide.size(); ++k) {
+        const uint32_t i = guide[k];
+        std::memcpy(&r[4 * k], &i, 4);
+        r[4 * k + 1] = i > 0 ? cdf[i - 1] : 0.f;
+        r[4 * k + 2] = cdf[i];
+        r[4 * k + 3] = i + 1 < n ? cdf[i + 1] : 0.f;
+    }
+    return r;
+}
+
 std::string build_scene(const wob_scene_desc &d, HostScene &S, bool with_bvh) {
     if (d.dim != 2 && d.dim != 3) return "dim must be 2 or 3";
     if (d.n_elements == 0) return "scene has no elements";

@@ -66,4 +66,9 @@ void build_cdf(const std::vector<float> &weights, std::vector<float> &cdf, float
 // Guide table of a normalised CDF: guide[k] = first i with cdf[i] >= k / n.
 void build_guide(const std::vector<float> &cdf, std::vector<uint32_t> &guide);
 
+// Guide records for the device search: per k four words (i = guide[k], cdf[i - 1], cdf[i],
+// cdf[i + 1]) (0 where the index is outside the CDF), so that the usual search that ends
+// within one step of guide[k] reads one 16-byte record and no CDF entry.
+std::vector<float> guide_records(const std::vector<float> &cdf, const std::vector<uint32_t> &guide);
+
 }  // namespace wobhost

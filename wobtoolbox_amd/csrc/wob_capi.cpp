@@ -486,7 +486,8 @@ wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_s
     const auto &h = s->host;
     wob_status st = WOB_OK;
     const float *nodes = dv.fnodes, *prims = dv.prims, *attr = nullptr, *acdf = nullptr, *bcdf = nullptr;
-    const uint32_t *pid = dv.prim_id, *ag = nullptr, *bg = nullptr, *qn = dv.qnodes;
+    const float *ag = nullptr, *bg = nullptr;
+    const uint32_t *pid = dv.prim_id, *qn = dv.qnodes;
     if (!device_bvh &&
         ((st = upload(s, 0, h.nodes, &nodes)) != WOB_OK || (st = upload(s, 1, h.prim_geom, &prims)) != WOB_OK ||
          (st = upload(s, 2, h.prim_id, &pid)) != WOB_OK || (st = upload(s, 8, h.qnodes, &qn)) != WOB_OK)) {
@@ -495,8 +496,9 @@ wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_s
         return st;
     }
     if ((st = upload(s, 3, h.elem_attr, &attr)) != WOB_OK || (st = upload(s, 4, h.area_cdf, &acdf)) != WOB_OK ||
-        (st = upload(s, 5, h.area_guide, &ag)) != WOB_OK || (st = upload(s, 6, h.bv_cdf, &bcdf)) != WOB_OK ||
-        (st = upload(s, 7, h.bv_guide, &bg)) != WOB_OK) {
+        (st = upload(s, 5, wobhost::guide_records(h.area_cdf, h.area_guide), &ag)) != WOB_OK ||
+        (st = upload(s, 6, h.bv_cdf, &bcdf)) != WOB_OK ||
+        (st = upload(s, 7, wobhost::guide_records(h.bv_cdf, h.bv_guide), &bg)) != WOB_OK) {
         free_scene(s);
         delete s;
         return st;
@@ -507,9 +509,9 @@ wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_s
     s->dev.prim_id = pid;
     s->dev.attr = (const float4 *)attr;
     s->dev.area_cdf = acdf;
-    s->dev.area_guide = ag;
+    s->dev.area_guide = (const float4 *)ag;
     s->dev.bv_cdf = bcdf;
-    s->dev.bv_guide = bg;
+    s->dev.bv_guide = (const float4 *)bg;
     s->dev.n = h.n;
     s->dev.total_area = h.total_area;
     s->dev.total_bv = h.total_bv;

@@ -60,9 +60,9 @@ struct DevScene {
     const uint32_t *prim_id;
     const float4 *attr;
     const float *area_cdf;
-    const uint32_t *area_guide;
+    const float4 *area_guide;   // guide records (i, cdf[i-1], cdf[i], cdf[i+1]) per entry
     const float *bv_cdf;
-    const uint32_t *bv_guide;
+    const float4 *bv_guide;
     uint32_t n;
     float total_area;
     float total_bv;
@@ -271,19 +271,40 @@ __device__ __forceinline__ uint32_t cdf_lookup(const float *__restrict__ cdf, co
     return i;
 }
 
-// two independent lookups with their guide loads issued together
-__device__ __forceinline__ void cdf_lookup2(const float *__restrict__ cdf, const uint32_t *__restrict__ guide, uint32_t n,
+// The same search from a guide record g = (i, cdf[i-1], cdf[i], cdf[i+1]) of entry
+// k = floor(u n): the search of cdf_lookup with the first forward and backward comparisons
+// taken from the record.  It ends within one step of i almost always and then reads no CDF
+// entry.  (After a forward step the backward condition u <= cdf[i-1] is false.)
+__device__ __forceinline__ uint32_t cdf_search(const float *__restrict__ cdf, uint32_t n, float u, float4 g) {
+    uint32_t i = __float_as_uint(g.x);
+    if (i + 1 < n && u > g.z) {
+        ++i;
+        if (i + 1 < n && u > g.w) {
+            ++i;
+            while (i + 1 < n && u > gldf(cdf + i)) ++i;
+        }
+    } else if (i > 0 && u <= g.y) {
+        --i;
+        while (i > 0 && u <= gldf(cdf + i - 1)) --i;
+    }
+    return i;
+}
+__device__ __forceinline__ float4 guide_record(const float4 *__restrict__ guide, uint32_t n, float u) {
+    uint32_t k = (uint32_t)(u * (float)n);
+    k = k < n ? k : n - 1;
+    return gld4(guide + k);
+}
+__device__ __forceinline__ uint32_t cdf_lookup(const float *__restrict__ cdf, const float4 *__restrict__ guide,
+                                               uint32_t n, float u) {
+    return cdf_search(cdf, n, u, guide_record(guide, n, u));
+}
+
+// two independent lookups with their guide records loaded together
+__device__ __forceinline__ void cdf_lookup2(const float *__restrict__ cdf, const float4 *__restrict__ guide, uint32_t n,
                                             float ua, float ub, uint32_t &ia, uint32_t &ib) {
-    uint32_t ka = (uint32_t)(ua * (float)n), kb = (uint32_t)(ub * (float)n);
-    ka = ka < n ? ka : n - 1;
-    kb = kb < n ? kb : n - 1;
-    uint32_t i = gldu(guide + ka), j = gldu(guide + kb);
-    while (i + 1 < n && ua > gldf(cdf + i)) ++i;
-    while (i > 0 && ua <= gldf(cdf + i - 1)) --i;
-    while (j + 1 < n && ub > gldf(cdf + j)) ++j;
-    while (j > 0 && ub <= gldf(cdf + j - 1)) --j;
-    ia = i;
-    ib = j;
+    const float4 ga = guide_record(guide, n, ua), gb = guide_record(guide, n, ub);
+    ia = cdf_search(cdf, n, ua, ga);
+    ib = cdf_search(cdf, n, ub, gb);
 }
 
 // point on element e at parameter(s) from the sampler; fills everything but p
@@ -363,14 +384,21 @@ template <int D> struct GeomSample {
 // The geometry of a uniform sample of element e from the sample's remaining uniforms: u[1]
 // (and u[2] in 3D) if `u` is given, otherwise drawn from rng after the element's attribute
 // loads are issued (the draws then overlap the loads).
-template <int D>
+// In 2D, NORMAL = false leaves n unset and skips its load (candidates whose normal is read
+// only for the one RIS keeps).
+template <int D, bool NORMAL = true>
 __device__ __forceinline__ GeomSample<D> uniform_geometry_of(const DevScene &S, uint32_t e, const float *u, Rng *rng) {
     GeomSample<D> g;
     g.elem = e;
     if constexpr (D == 2) {
-        const float4 f0 = gld4(S.attr + 3 * e), f1 = gld4(S.attr + 3 * e + 1);
+        const float4 f0 = gld4(S.attr + 3 * e);
         const float v[4] = {f0.x, f0.y, f0.z, f0.w};
-        g.n[0] = f1.x; g.n[1] = f1.y;
+        if constexpr (NORMAL) {
+            const float4 f1 = gld4(S.attr + 3 * e + 1);
+            g.n[0] = f1.x; g.n[1] = f1.y;
+        } else {
+            g.n[0] = 0.f; g.n[1] = 0.f;
+        }
         const float u2 = u ? u[1] : rng->uniform();
 #pragma unroll
         for (int k = 0; k < 2; ++k) g.p[k] = v[k] + u2 * (v[2 + k] - v[k]);            // as uniform_in_element
@@ -1367,14 +1395,21 @@ __device__ __forceinline__ Sample<D> ris(const DevScene &S, const Model<D, BIE> 
         const float rb = rng.uniform();
         uint32_t ea, eb;
         cdf_lookup2(S.area_cdf, S.area_guide, S.n, ua[0], ub[0], ea, eb);
-        const GeomSample<D> xa = uniform_geometry_of<D>(S, ea, ua, nullptr);
-        const GeomSample<D> xb = uniform_geometry_of<D>(S, eb, ub, nullptr);
+        const GeomSample<D> xa = uniform_geometry_of<D, false>(S, ea, ua, nullptr);
+        const GeomSample<D> xb = uniform_geometry_of<D, false>(S, eb, ub, nullptr);
         update(xa, ra);
         update(xb, rb);
     }
     for (; i < nrc; ++i) {
         const GeomSample<D> xa = uniform_geometry<D>(S, rng);
         update(xa, rng.uniform());
+    }
+    if constexpr (D == 2) {   // the kept candidate's normal (the pair loop skips candidates' normals)
+        if (pick.elem != 0xFFFFFFFFu) {
+            const float4 f1 = gld4(S.attr + 3 * pick.elem + 1);
+            pick.n[0] = f1.x;
+            pick.n[1] = f1.y;
+        }
     }
     Sample<D> r;
     r.y = pick.elem == 0xFFFFFFFFu ? bp_zero<D>() : complete_uniform<D>(S, pick);
