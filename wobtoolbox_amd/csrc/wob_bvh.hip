@@ -20,7 +20,7 @@
 //      float node and its quantized copy are written in the same kernel;
 //   6. the traversal stack bound, bottom-up over the levels;
 //   7. primitive records in sorted (leaf) order.
-// Leaves hold one primitive each.
+// Subtrees of at most kLeafPrims primitives become leaves.
 #include <hip/hip_runtime.h>
 
 #include <hipcub/hipcub.hpp>
@@ -109,7 +109,8 @@ __device__ __forceinline__ int delta(const uint32_t *code, int n, int i, int j) 
 
 // 3: binary radix tree.  Node ids: internal i in [0, n-1), leaf j as (n - 1) + j.
 __global__ __launch_bounds__(256) void tree_kernel(int n, const uint32_t *__restrict__ code, uint32_t *__restrict__ left,
-                                                   uint32_t *__restrict__ right, uint32_t *__restrict__ parent) {
+                                                   uint32_t *__restrict__ right, uint32_t *__restrict__ parent,
+                                                   uint32_t *__restrict__ first, uint32_t *__restrict__ count) {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n - 1) return;
     const int d = delta(code, n, i, i + 1) - delta(code, n, i, i - 1) >= 0 ? 1 : -1;
@@ -132,6 +133,8 @@ __global__ __launch_bounds__(256) void tree_kernel(int n, const uint32_t *__rest
     const uint32_t rc = max(i, j) == g + 1 ? (uint32_t)(n - 1 + g + 1) : (uint32_t)(g + 1);
     left[i] = lc;
     right[i] = rc;
+    first[i] = (uint32_t)min(i, j);
+    count[i] = (uint32_t)(max(i, j) - min(i, j) + 1);
     parent[lc] = (uint32_t)i;
     parent[rc] = (uint32_t)i;
 }
@@ -174,18 +177,25 @@ template <int D> __device__ __forceinline__ float measure(const Box6 &b) {
 
 // children of binary node b as a wide node (host rule: expand the internal child of largest
 // measure, first one on ties, until there are four or only leaves)
+// A subtree of at most kLeafPrims primitives (a contiguous range in sorted order) becomes one
+// leaf.
+constexpr uint32_t kLeafPrims = 2;
+__device__ __forceinline__ bool is_inner(uint32_t id, uint32_t n_int, const uint32_t *count) {
+    return id < n_int && count[id] > kLeafPrims;
+}
+
 template <int D>
 __device__ __forceinline__ int expand(int n, uint32_t b, const uint32_t *left, const uint32_t *right, const Box6 *box,
-                                      uint32_t kids[4]) {
+                                      const uint32_t *count, uint32_t kids[4]) {
     const uint32_t n_int = (uint32_t)(n - 1);
     int nk;
-    if (b >= n_int) { kids[0] = b; nk = 1; }
+    if (!is_inner(b, n_int, count)) { kids[0] = b; nk = 1; }
     else { kids[0] = left[b]; kids[1] = right[b]; nk = 2; }
     while (nk < 4) {
         int best = -1;
         float bestA = -1.f;
         for (int i = 0; i < nk; ++i)
-            if (kids[i] < n_int) {
+            if (is_inner(kids[i], n_int, count)) {
                 const float A = measure<D>(box[kids[i]]);
                 if (A > bestA) { bestA = A; best = i; }
             }
@@ -200,13 +210,14 @@ __device__ __forceinline__ int expand(int n, uint32_t b, const uint32_t *left, c
 template <int D>
 __global__ __launch_bounds__(256) void count_kernel(int n, uint32_t m, const uint32_t *__restrict__ level,
                                                     const uint32_t *__restrict__ left, const uint32_t *__restrict__ right,
-                                                    const Box6 *__restrict__ box, uint32_t *__restrict__ counts) {
+                                                    const Box6 *__restrict__ box, const uint32_t *__restrict__ count,
+                                                    uint32_t *__restrict__ counts) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m) return;
     uint32_t kids[4];
-    const int nk = expand<D>(n, level[t], left, right, box, kids);
+    const int nk = expand<D>(n, level[t], left, right, box, count, kids);
     uint32_t c = 0;
-    for (int i = 0; i < nk; ++i) c += kids[i] < (uint32_t)(n - 1) ? 1u : 0u;
+    for (int i = 0; i < nk; ++i) c += is_inner(kids[i], (uint32_t)(n - 1), count) ? 1u : 0u;
     counts[t] = c;
 }
 
@@ -243,7 +254,8 @@ template <int D>
 __global__ __launch_bounds__(256) void write_kernel(int n, uint32_t m, uint32_t base, uint32_t next_base,
                                                     const uint32_t *__restrict__ level, const uint32_t *__restrict__ offs,
                                                     const uint32_t *__restrict__ left, const uint32_t *__restrict__ right,
-                                                    const Box6 *__restrict__ box, uint32_t *__restrict__ next,
+                                                    const Box6 *__restrict__ box, const uint32_t *__restrict__ first,
+                                                    const uint32_t *__restrict__ count, uint32_t *__restrict__ next,
                                                     float *__restrict__ fnodes, uint32_t *__restrict__ qnodes) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m) return;
@@ -251,7 +263,7 @@ __global__ __launch_bounds__(256) void write_kernel(int n, uint32_t m, uint32_t 
     constexpr int QW = D == 2 ? kQNodeWords2 : kQNodeWords3;
     const uint32_t n_int = (uint32_t)(n - 1);
     uint32_t kids[4];
-    const int nk = expand<D>(n, level[t], left, right, box, kids);
+    const int nk = expand<D>(n, level[t], left, right, box, count, kids);
     const uint32_t w = base + t;
     float *nd = fnodes + (size_t)w * NW;
     uint32_t *q = qnodes + (size_t)w * QW;
@@ -264,10 +276,12 @@ __global__ __launch_bounds__(256) void write_kernel(int n, uint32_t m, uint32_t 
         if (c < nk) {
             b = box[kids[c]];
             mask |= 1u << c;
-            if (kids[c] < n_int) {
+            if (is_inner(kids[c], n_int, count)) {
                 const uint32_t slot = offs[t] + r++;
                 next[slot] = kids[c];
                 child[c] = next_base + slot;
+            } else if (kids[c] < n_int) {   // a small subtree: its sorted range is one leaf
+                child[c] = ~((first[kids[c]] << 4) | (count[kids[c]] - 1u));
             } else {
                 child[c] = ~((kids[c] - n_int) << 4);   // one primitive: ~((first << 4) | 0)
             }
@@ -393,6 +407,7 @@ template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBv
     const uint32_t n_int = n - 1, n_all = 2 * n - 1;
     uint32_t *left = tmp.get<uint32_t>(std::max<uint32_t>(n_int, 1)), *right = tmp.get<uint32_t>(std::max<uint32_t>(n_int, 1));
     uint32_t *parent = tmp.get<uint32_t>(n_all), *flag = tmp.get<uint32_t>(std::max<uint32_t>(n_int, 1));
+    uint32_t *first = tmp.get<uint32_t>(std::max<uint32_t>(n_int, 1)), *count = tmp.get<uint32_t>(std::max<uint32_t>(n_int, 1));
     Box6 *box = tmp.get<Box6>(n_all);
     // wide nodes: at most one per binary internal node (or the single leaf root)
     const uint32_t max_wide = std::max<uint32_t>(n_int, 1);
@@ -417,7 +432,8 @@ template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBv
     if (tmp.err != hipSuccess) return std::string("hipMalloc during the device BVH build: ") + hipGetErrorString(tmp.err);
     WOB_TRY(hipcub::DeviceRadixSort::SortPairs(cub_tmp, sort_bytes, code, code2, idx, idx2, (int)n, 0, 32, st));
     if (n > 1) {
-        hipLaunchKernelGGL(tree_kernel, dim3(blocks(n_int)), dim3(256), 0, st, (int)n, code2, left, right, parent);
+        hipLaunchKernelGGL(tree_kernel, dim3(blocks(n_int)), dim3(256), 0, st, (int)n, code2, left, right, parent, first,
+                           count);
         WOB_TRY(hipGetLastError());
         WOB_TRY(hipMemsetAsync(flag, 0, (size_t)n_int * sizeof(uint32_t), st));
     }
@@ -445,7 +461,8 @@ template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBv
     uint32_t base = 0, m = 1;
     while (m > 0) {
         if (level_base.size() > 256) return "device BVH build: tree deeper than 256 levels";
-        hipLaunchKernelGGL((count_kernel<D>), dim3(blocks(m)), dim3(256), 0, st, (int)n, m, cur, left, right, box, counts);
+        hipLaunchKernelGGL((count_kernel<D>), dim3(blocks(m)), dim3(256), 0, st, (int)n, m, cur, left, right, box, count,
+                           counts);
         WOB_TRY(hipGetLastError());
         WOB_TRY(hipcub::DeviceScan::ExclusiveSum(cub_tmp, scan_bytes, counts, offs, (int)m, st));
         uint32_t last[2] = {0u, 0u};
@@ -453,7 +470,7 @@ template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBv
         WOB_TRY(hipMemcpyAsync(&last[1], offs + (m - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         const uint32_t next_base = base + m;
         hipLaunchKernelGGL((write_kernel<D>), dim3(blocks(m)), dim3(256), 0, st, (int)n, m, base, next_base, cur, offs, left,
-                           right, box, nxt, fnodes, qnodes);
+                           right, box, first, count, nxt, fnodes, qnodes);
         WOB_TRY(hipGetLastError());
         WOB_TRY(hipStreamSynchronize(st));
         const uint32_t m_next = last[0] + last[1];
