@@ -871,8 +871,9 @@ __device__ __forceinline__ LineHit<D> line_query_ws(const DevScene &S, const flo
     h.id = 0xFFFFFFFFu; h.t = 0.f; h.a = 0.f; h.b = 0.f;
     if (h.count != 0u) {
         const uint32_t leaf = (uint32_t)res_key[lane];
-        (void)prim_test<D>(S, leaf, o, d, h.t, h.a, h.b);
-        h.id = gldu(S.prim_id + leaf);
+        const PrimRec<D> pr = prim_load<D>(S, leaf);
+        (void)prim_eval<D>(pr, o, d, h.t, h.a, h.b);
+        h.id = D == 3 ? __float_as_uint(pr.g[D == 3 ? 2 : 0].y) : gldu(S.prim_id + leaf);
     }
     return h;
 }
