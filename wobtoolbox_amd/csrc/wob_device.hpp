@@ -775,7 +775,10 @@ constexpr int kStealMinIdle = 16;
 // STACK + 1 entries per lane; then one count word per lane; then one 64-bit (key, leaf) word
 // per lane (stride is a multiple of 64, so it is 8-byte aligned).  Requires all lanes of the
 // wave that call it to call it together (it is called in uniform control flow of the walk).
-template <int D, int STACK>
+//
+// PICK = false counts only the hits with t > 0 (the crossings of the ray o + t d, t > 0) and
+// picks nothing; pick and skip are unused.
+template <int D, int STACK, bool PICK = true>
 __device__ __forceinline__ LineHit<D> line_query_ws(const DevScene &S, const float *o, const float *d, uint32_t pick,
                                                     uint32_t skip, uint32_t *stack, uint32_t stride) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
@@ -801,7 +804,7 @@ __device__ __forceinline__ LineHit<D> line_query_ws(const DevScene &S, const flo
             // bottom stack entry of the i-th lane that has one
             if (idle && cnt != 0u) {
                 atomicAdd(&res_cnt[own4 >> 2], cnt);
-                atomicMin((unsigned long long *)&res_key[own4 >> 2], (unsigned long long)best);
+                if constexpr (PICK) atomicMin((unsigned long long *)&res_key[own4 >> 2], (unsigned long long)best);
                 cnt = 0u;
                 best = ~0ull;
             }
@@ -837,7 +840,7 @@ __device__ __forceinline__ LineHit<D> line_query_ws(const DevScene &S, const flo
             float oo[D], dd[D];
 #pragma unroll
             for (int k = 0; k < D; ++k) { oo[k] = lane_read(o[k], own4); dd[k] = lane_read(d[k], own4); }
-            const uint32_t pk = lane_read(pick, own4), sk = lane_read(skip, own4);
+            const uint32_t pk = PICK ? lane_read(pick, own4) : 0u, sk = PICK ? lane_read(skip, own4) : 0u;
             if (want_leaf) {
 #pragma unroll
                 for (int u = 0; u < kPrimsPerRound; ++u) {
@@ -845,12 +848,16 @@ __device__ __forceinline__ LineHit<D> line_query_ws(const DevScene &S, const flo
                     float t, a, b;
                     const PrimRec<D> pr = prim_load<D>(S, tr.lp);
                     const bool hit = prim_eval<D>(pr, oo, dd, t, a, b);
-                    // the 3D record holds the element id
-                    const uint32_t id = D == 3 ? __float_as_uint(pr.g[D == 3 ? 2 : 0].y) : gldu(S.prim_id + tr.lp);
-                    const bool ok = hit && id != sk;
-                    const uint64_t v = ((uint64_t)triple32(pk ^ (id * 0x9E3779B9u)) << 32) | tr.lp;
-                    cnt += ok ? 1u : 0u;
-                    best = ok && v < best ? v : best;
+                    if constexpr (PICK) {
+                        // the 3D record holds the element id
+                        const uint32_t id = D == 3 ? __float_as_uint(pr.g[D == 3 ? 2 : 0].y) : gldu(S.prim_id + tr.lp);
+                        const bool ok = hit && id != sk;
+                        const uint64_t v = ((uint64_t)triple32(pk ^ (id * 0x9E3779B9u)) << 32) | tr.lp;
+                        cnt += ok ? 1u : 0u;
+                        best = ok && v < best ? v : best;
+                    } else {
+                        cnt += hit && t > 0.f ? 1u : 0u;
+                    }
                     ++tr.lp;
                 }
             }
@@ -863,13 +870,13 @@ __device__ __forceinline__ LineHit<D> line_query_ws(const DevScene &S, const flo
     }
     if (cnt != 0u) {
         atomicAdd(&res_cnt[own4 >> 2], cnt);
-        atomicMin((unsigned long long *)&res_key[own4 >> 2], (unsigned long long)best);
+        if constexpr (PICK) atomicMin((unsigned long long *)&res_key[own4 >> 2], (unsigned long long)best);
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     LineHit<D> h;
     h.count = res_cnt[lane];
     h.id = 0xFFFFFFFFu; h.t = 0.f; h.a = 0.f; h.b = 0.f;
-    if (h.count != 0u) {
+    if (PICK && h.count != 0u) {
         const uint32_t leaf = (uint32_t)res_key[lane];
         const PrimRec<D> pr = prim_load<D>(S, leaf);
         (void)prim_eval<D>(pr, o, d, h.t, h.a, h.b);
@@ -958,8 +965,9 @@ template <int D> struct VolSample {
 };
 
 // Crossings of the ray o + t r, t > 0, with the boundary along a fixed direction r; odd
-// means inside a closed boundary.  Kept out of line: it is the only traversal of a volume
-// sample, and inlining it at every volume-term call site multiplies the kernels' code.
+// means inside a closed boundary (a counting line query, the wave's lanes sharing the work).
+// Kept out of line: it is the only traversal of a volume sample, and inlining it at every
+// volume-term call site multiplies the kernels' code.
 template <int D, int STACK>
 __device__ __noinline__ uint32_t ray_crossings(const float4 *nodes, const float4 *prims, float o0, float o1, float o2,
                                                uint32_t *stack, uint32_t stride) {
@@ -970,11 +978,7 @@ __device__ __noinline__ uint32_t ray_crossings(const float4 *nodes, const float4
     float d[D];
     if constexpr (D == 2) { d[0] = 0.58778525f; d[1] = 0.80901699f; }
     else { d[0] = 0.48218104f; d[1] = 0.66748405f; d[2] = 0.56749094f; }
-    uint32_t c = 0;
-    traverse<D, STACK>(S, o, d, stack, stride, [&](bool hit, uint32_t, float t, float, float) {
-        c += (hit && t > 0.f) ? 1u : 0u;
-    });
-    return c;
+    return line_query_ws<D, STACK, false>(S, o, d, 0u, 0u, stack, stride).count;
 }
 
 // scene.cuh:100-120: a cell with probability |cache| / sum |cache|, a uniform point in it,
