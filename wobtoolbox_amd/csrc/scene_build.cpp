@@ -72,39 +72,44 @@ struct Builder {
         int split = -1;
         if (ext > 0 && !median_only && depth < 48) {
             constexpr int B = 32;
-            Box bb[B];
-            int bc[B] = {0};
-            for (int j = 0; j < B; ++j) bb[j].reset();
-            const float scale = B / ext;
-            auto bin_of = [&](uint32_t p) {
-                int j = (int)((cent[3 * p + axis] - cb.lo[axis]) * scale);
-                return std::min(std::max(j, 0), B - 1);
-            };
-            for (int i = start; i < start + count; ++i) {
-                const int j = bin_of(order[i]);
-                bb[j].grow(pbox[order[i]]);
-                bc[j]++;
-            }
-            float rightA[B];
-            int rightN[B];
-            Box acc; acc.reset();
-            int an = 0;
-            for (int j = B - 1; j > 0; --j) {
-                acc.grow(bb[j]); an += bc[j];
-                rightA[j] = acc.measure(dim); rightN[j] = an;
-            }
-            acc.reset(); an = 0;
+            // binned SAH over every axis with extent (the best split of all three)
             float best = std::numeric_limits<float>::infinity();
-            int bestj = -1;
-            for (int j = 1; j < B; ++j) {
-                acc.grow(bb[j - 1]); an += bc[j - 1];
-                if (an == 0 || rightN[j] == 0) continue;
-                const float c = acc.measure(dim) * an + rightA[j] * rightN[j];
-                if (c < best) { best = c; bestj = j; }
+            int bestj = -1, bestax = axis;
+            for (int ax = 0; ax < dim; ++ax) {
+                const float ea = cb.hi[ax] - cb.lo[ax];
+                if (!(ea > 0)) continue;
+                Box bb[B];
+                int bc[B] = {0};
+                for (int jj = 0; jj < B; ++jj) bb[jj].reset();
+                const float scale = B / ea;
+                for (int i = start; i < start + count; ++i) {
+                    int jb = (int)((cent[3 * order[i] + ax] - cb.lo[ax]) * scale);
+                    jb = std::min(std::max(jb, 0), B - 1);
+                    bb[jb].grow(pbox[order[i]]);
+                    bc[jb]++;
+                }
+                float rightA[B];
+                int rightN[B];
+                Box acc; acc.reset();
+                int an = 0;
+                for (int jj = B - 1; jj > 0; --jj) { acc.grow(bb[jj]); an += bc[jj]; rightA[jj] = acc.measure(dim); rightN[jj] = an; }
+                acc.reset(); an = 0;
+                for (int jj = 1; jj < B; ++jj) {
+                    acc.grow(bb[jj - 1]); an += bc[jj - 1];
+                    if (an == 0 || rightN[jj] == 0) continue;
+                    const float c = acc.measure(dim) * an + rightA[jj] * rightN[jj];
+                    if (c < best) { best = c; bestj = jj; bestax = ax; }
+                }
             }
             const float leaf_cost = b.measure(dim) * count;
             // traversal cost ~ one node visit per box test; stop at small leaves when splitting does not pay
             if (bestj > 0 && !(count <= kMaxLeaf && best + 0.5f * b.measure(dim) >= leaf_cost)) {
+                const float ea = cb.hi[bestax] - cb.lo[bestax];
+                const float scale = B / ea;
+                auto bin_of = [&](uint32_t p) {
+                    int jb = (int)((cent[3 * p + bestax] - cb.lo[bestax]) * scale);
+                    return std::min(std::max(jb, 0), B - 1);
+                };
                 auto mid = std::partition(order.begin() + start, order.begin() + start + count,
                                           [&](uint32_t p) { return bin_of(p) < bestj; });
                 split = (int)(mid - order.begin()) - start;
