@@ -729,20 +729,6 @@ __device__ __forceinline__ void traverse(const DevScene &S, const float *o, cons
     }
 }
 
-// All hits of the line; returns the count and the hit with the smallest key
-// triple32(pick ^ id * 0x9E3779B9): a uniform choice among the hits for a uniform pick.
-template <int D, int STACK>
-__device__ __forceinline__ LineHit<D> line_query(const DevScene &S, const float *o, const float *d, uint32_t pick,
-                                                 uint32_t skip, uint32_t *stack, uint32_t stride) {
-    LineHit<D> h;
-    h.count = 0; h.id = 0xFFFFFFFFu; h.t = 0.f; h.a = 0.f; h.b = 0.f;
-    uint32_t best = 0xFFFFFFFFu;
-    traverse<D, STACK>(S, o, d, stack, stride, [&](bool hit, uint32_t prim, float t, float a, float b) {
-        hit_update<D>(hit, gldu(S.prim_id + prim), skip, pick, t, a, b, h, best);
-    });
-    return h;
-}
-
 // Lane `src4 / 4`'s value of v (ds_bpermute; every lane of the query loop is active when it
 // runs, so the source lane is too).
 __device__ __forceinline__ float lane_read(float v, int src4) {
@@ -759,8 +745,10 @@ template <int STACK> constexpr int line_query_lds_words() { return STACK + 1 + 3
 // A steal round runs once at least this many lanes of the wave have finished their work.
 constexpr int kStealMinIdle = 16;
 
-// line_query with the wave's lanes sharing the work of all its queries.  The lines of one
-// wave need very different numbers of steps, and with one line per lane the wave runs until
+// All hits of the line o + t d: the count and the hit with the smallest key
+// triple32(pick ^ id * 0x9E3779B9), a uniform choice among the hits for a uniform pick, with
+// the wave's lanes sharing the work of all its queries.  The lines of one wave need very
+// different numbers of steps, and with one line per lane the wave runs until
 // its longest query is done.  Here a lane whose work is done takes the bottom entry (the
 // subtree nearest the root) of another lane's stack and traverses it with that lane's line,
 // read from the owner lane's registers; it adds its hit count and its smallest
@@ -768,7 +756,7 @@ constexpr int kStealMinIdle = 16;
 // line's hit set is still tested exactly once, the count is a sum and the pick a minimum, so
 // the result is the same whichever lane does which part.  The keys are distinct for distinct
 // elements (triple32 and x -> pick ^ x * 0x9E3779B9 are bijections of 32-bit words), so the
-// minimum over (key, leaf) picks the same hit as line_query's (key, id) order; its t and
+// minimum over (key, leaf) picks the same hit as the (key, id) order of hit_update; its t and
 // barycentrics are recomputed at the end by the owner with the same primitive test.
 //
 // LDS layout (`stack` = block base + threadIdx.x, entries `stride` words apart): the stacks,
