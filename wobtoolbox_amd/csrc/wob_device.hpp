@@ -600,9 +600,11 @@ template <int D> struct Trav {
     __device__ __forceinline__ bool want_node() const { return cur >= 0; }
     __device__ __forceinline__ bool done() const { return lp >= le && cur == kTravEmpty; }
 
-    // four slab tests on the current node; the first hit child becomes the current entry, the
-    // others are pushed.  Every child is written to the slot above the top (a spare slot
-    // backs a full stack), only pushes move the top, so no lane branches.
+    // four slab tests on the current node; the hit children are pushed and the top entry is
+    // popped as the current one.  Every child is written to the slot above the top (a spare
+    // slot backs a full stack), only pushes move the top, so no lane branches.  A node's
+    // pushes peak one entry above the depth-first bound of "continue with one hit child, push
+    // the others"; the scene builds count that entry (stack_need = bound + 1).
     template <int STACK>
     __device__ __forceinline__ void node_step(const float4 *nodes, uint32_t *stack, uint32_t stride) {
         node_step<STACK>(nodes, stack, stride, inv, oi);
@@ -660,7 +662,6 @@ template <int D> struct Trav {
             qn[k] = up ? r.qlo[k] : r.qhi[k];
             qf[k] = up ? r.qhi[k] : r.qlo[k];
         }
-        int next = kTravEmpty;
         const uint32_t full = (uint32_t)STACK * stride;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -672,17 +673,15 @@ template <int D> struct Trav {
                 tmin = k == 0 ? t1 : fmaxf(tmin, t1);
                 tmax = k == 0 ? t2 : fminf(tmax, t2);
             }
-            const bool h = ch[c] != kTravEmpty && tmin <= tmax;
-            const bool take = h && next == kTravEmpty;
-            const bool push = h && !take && spo < full;
+            const bool push = ch[c] != kTravEmpty && tmin <= tmax && spo < full;
             stack[spo] = (uint32_t)ch[c];
             spo = push ? spo + stride : spo;
-            next = take ? ch[c] : next;
         }
-        const bool pop = next == kTravEmpty && spo != sb;
-        const int top = (int)stack[spo != sb ? spo - stride : 0u];
+        // the last hit child (or the entry below, if none was hit) becomes the current entry
+        const bool pop = spo != sb;
+        const int top = (int)stack[pop ? spo - stride : 0u];
         spo = pop ? spo - stride : spo;
-        cur = next != kTravEmpty ? next : (pop ? top : kTravEmpty);
+        cur = pop ? top : kTravEmpty;
     }
 };
 
