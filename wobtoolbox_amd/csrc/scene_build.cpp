@@ -384,7 +384,9 @@ std::string build_scene(const wob_scene_desc &d, HostScene &S, bool with_bvh) {
         if (next_prim != n) return "internal error: primitive count mismatch";
         quantize_nodes(D, S.nodes, S.qnodes);
         // stack bound for DFS "continue with one hit child, push the others" (leaves are
-        // pushed too): need(w) = (#children - 1) + max need(internal child); bottom-up.
+        // pushed too): need(w) = (#children - 1) + max need(internal child); bottom-up.  The
+        // traversal pushes all hit children before popping one, one entry more: stack_need
+        // below is need + 1.
         for (size_t w = wides.size(); w-- > 0;) {
             int m = 0;
             for (int c = 0; c < wides[w].nchild; ++c) {

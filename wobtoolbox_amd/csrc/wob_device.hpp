@@ -689,7 +689,7 @@ template <int D> struct Trav {
 // on_hit(hit, prim, t, a, b) for every primitive tested, hit = the reference predicate.
 //
 // Wave-scheduled: every lane holds one current entry (internal node or leaf) and a range of
-// pending primitives; hit children beyond the first are pushed on the lane's LDS stack
+// pending primitives; hit children are pushed on the lane's LDS stack and the top is popped
 // (`stack`, entries `stride` words apart, STACK entries plus a spare; the builder bounds the
 // depth), leaves included.  Each iteration the wave runs ONE kind of step, chosen by ballot:
 // a primitive test for the lanes with pending primitives, or a node step (four slab tests)
