@@ -601,8 +601,9 @@ template <int D> struct Trav {
     __device__ __forceinline__ bool done() const { return lp >= le && cur == kTravEmpty; }
 
     // four slab tests on the current node; the hit children are pushed and the top entry is
-    // popped as the current one.  Every child is written to the slot above the top (a spare
-    // slot backs a full stack), only pushes move the top, so no lane branches.  A node's
+    // popped as the current one.  Every child is written to the slot above the top (the spare
+    // slot above STACK entries takes the write past the last push), only pushes move the top,
+    // so no lane branches.  A node's
     // pushes peak one entry above the depth-first bound of "continue with one hit child, push
     // the others"; the scene builds count that entry (stack_need = bound + 1).
     template <int STACK>
@@ -662,7 +663,6 @@ template <int D> struct Trav {
             qn[k] = up ? r.qlo[k] : r.qhi[k];
             qf[k] = up ? r.qhi[k] : r.qlo[k];
         }
-        const uint32_t full = (uint32_t)STACK * stride;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             float tmin = 0.f, tmax = 0.f;
@@ -673,7 +673,9 @@ template <int D> struct Trav {
                 tmin = k == 0 ? t1 : fmaxf(tmin, t1);
                 tmax = k == 0 ? t2 : fminf(tmax, t2);
             }
-            const bool push = ch[c] != kTravEmpty && tmin <= tmax && spo < full;
+            // no overflow test: scene creation rejects BVHs whose bound (the pushes included)
+            // exceeds STACK entries, so spo stays within them
+            const bool push = ch[c] != kTravEmpty && tmin <= tmax;
             stack[spo] = (uint32_t)ch[c];
             spo = push ? spo + stride : spo;
         }
