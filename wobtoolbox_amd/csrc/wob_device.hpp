@@ -783,12 +783,18 @@ __device__ __forceinline__ LineHit<D> line_query_ws(const DevScene &S, const flo
     int own4 = self4;            // ds_bpermute address of the lane whose line this lane traverses
     uint32_t cnt = 0u;           // hits of the current work
     uint64_t best = ~0ull;       // smallest (key << 32 | leaf) of the current work
+    const uint64_t active = __ballot(true);
     while (true) {
         tr.settle(stack, stride);
-        const bool idle = tr.done();
-        const uint64_t bi = __ballot(idle);
-        const uint64_t bd = __ballot(tr.spo != tr.sb);
-        if (__popcll(bi) >= kStealMinIdle && bd != 0ull) {
+        const bool want_leaf = tr.want_prim();
+        const bool want_node = tr.want_node();
+        const uint64_t bl = __ballot(want_leaf);
+        const uint64_t bn = __ballot(want_node);
+        // after settle a lane that wants neither step has no work left
+        const uint64_t bi = active & ~(bl | bn);
+        const uint64_t bd = __popcll(bi) >= kStealMinIdle ? __ballot(tr.spo != tr.sb) : 0ull;
+        if (bd != 0ull) {
+            const bool idle = !(want_leaf || want_node);
             // idle lanes hand their results to the owners, then the i-th idle lane takes the
             // bottom stack entry of the i-th lane that has one
             if (idle && cnt != 0u) {
@@ -820,10 +826,6 @@ __device__ __forceinline__ LineHit<D> line_query_ws(const DevScene &S, const flo
             if (given) tr.sb += stride;
             continue;
         }
-        const bool want_leaf = tr.want_prim();
-        const bool want_node = tr.want_node();
-        const uint64_t bl = __ballot(want_leaf);
-        const uint64_t bn = __ballot(want_node);
         if ((bl | bn) == 0ull) break;
         if (__popcll(bl) >= __popcll(bn)) {
             float oo[D], dd[D];
