@@ -355,6 +355,31 @@ def test_batch_equals_single_calls(gpu):
         b.close()
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_batch_of_line_sampled_scenes_equals_single_calls(dim, gpu):
+    """Different scenes with line-sampled walks in one launch: the wave at an entry boundary
+    holds lanes of two scenes, whose line queries must not share traversal work."""
+    import torch
+    import wobtoolbox_amd as w
+    if dim == 2:
+        ws = [G.circle_dirichlet_2d(n_segments=n, grid_res=16) for n in (96, 512, 2048)]
+    else:
+        ws = [G.icosphere_neumann_3d(n_points=300, level=lv, seed=lv) for lv in (2, 3, 4)]
+    scenes = [w.SceneHost(x.elements) for x in ws]
+    ests = [w.Estimator.from_config(sc, x.solver, paths_per_lane=3) for sc, x in zip(scenes, ws)]
+    pts = [_dev(x.points[: 37 + 29 * i], gpu) for i, x in enumerate(ws)]
+    single = [e.compute_domain(p, 16, seed=9, stream_id=i) for i, (e, p) in enumerate(zip(ests, pts))]
+    for persistent in (0, 1):
+        b = w.Batch([(e, p, 16, i) for i, (e, p) in enumerate(zip(ests, pts))], paths_per_lane=3,
+                    persistent=bool(persistent))
+        b.run(1, seed=9)
+        torch.cuda.synchronize()
+        for (s1, q1), (s2, q2) in zip(single, b.outputs):
+            assert torch.equal(s1, s2) and torch.equal(q1, q2)
+        _say(f"dim {dim} batch persistent={persistent}: bit-identical to single calls")
+        b.close()
+
+
 def test_batch_graph_replays_equal_progressive_calls(gpu):
     import torch
     import wobtoolbox_amd as w

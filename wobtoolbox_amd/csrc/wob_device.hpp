@@ -762,8 +762,8 @@ constexpr int kStealMinIdle = 16;
 //
 // LDS layout (`stack` = block base + threadIdx.x, entries `stride` words apart): the stacks,
 // STACK + 1 entries per lane; then one count word per lane; then one 64-bit (key, leaf) word
-// per lane (stride is a multiple of 64, so it is 8-byte aligned).  Requires all lanes of the
-// wave that call it to call it together (it is called in uniform control flow of the walk).
+// per lane (stride is a multiple of 64, so it is 8-byte aligned).  The active lanes of the
+// wave at the call share the work among themselves.
 //
 // PICK = false counts only the hits with t > 0 (the crossings of the ray o + t d, t > 0) and
 // picks nothing; pick and skip are unused.
@@ -784,6 +784,13 @@ __device__ __forceinline__ LineHit<D> line_query_ws(const DevScene &S, const flo
     uint32_t cnt = 0u;           // hits of the current work
     uint64_t best = ~0ull;       // smallest (key << 32 | leaf) of the current work
     const uint64_t active = __ballot(true);
+    // Lanes share work only when they all walk the same scene: a batch launch numbers its
+    // work items entry by entry, so the wave at an entry boundary can hold two scenes, and a
+    // stack entry is a node index of its own scene.
+    const uint64_t np_ = (uint64_t)(uintptr_t)S.nodes;
+    const uint32_t np_lo = (uint32_t)np_, np_hi = (uint32_t)(np_ >> 32);
+    const bool one_scene = __ballot(np_lo != (uint32_t)__builtin_amdgcn_readfirstlane((int)np_lo) ||
+                                    np_hi != (uint32_t)__builtin_amdgcn_readfirstlane((int)np_hi)) == 0ull;
     while (true) {
         tr.settle(stack, stride);
         const bool want_leaf = tr.want_prim();
@@ -792,7 +799,7 @@ __device__ __forceinline__ LineHit<D> line_query_ws(const DevScene &S, const flo
         const uint64_t bn = __ballot(want_node);
         // after settle a lane that wants neither step has no work left
         const uint64_t bi = active & ~(bl | bn);
-        const uint64_t bd = __popcll(bi) >= kStealMinIdle ? __ballot(tr.spo != tr.sb) : 0ull;
+        const uint64_t bd = one_scene && __popcll(bi) >= kStealMinIdle ? __ballot(tr.spo != tr.sb) : 0ull;
         if (bd != 0ull) {
             const bool idle = !(want_leaf || want_node);
             // idle lanes hand their results to the owners, then the i-th idle lane takes the
