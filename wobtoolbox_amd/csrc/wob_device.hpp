@@ -810,16 +810,18 @@ __device__ __forceinline__ LineHit<D> line_query_ws(const DevScene &S, const flo
                 cnt = 0u;
                 best = ~0ull;
             }
-            uint64_t im = bi, dm = bd;
-            const int np = min(__popcll(bi), __popcll(bd));
-            int src = -1, given = 0;
-            for (int p = 0; p < np; ++p) {
-                const int il = __builtin_ctzll(im), dl = __builtin_ctzll(dm);
-                im &= im - 1ull;
-                dm &= dm - 1ull;
-                src = (int)lane == il ? dl : src;
-                given = (int)lane == dl ? 1 : given;
-            }
+            // the i-th donor posts its lane in lane i's spare stack slot (never a live entry:
+            // it only takes the write past the last push), the i-th idle lane reads it
+            const uint32_t np = (uint32_t)min(__popcll(bi), __popcll(bd));
+            const uint64_t mine = idle ? bi : bd;
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mine >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mine, 0u));
+            const bool given = !idle && ((bd >> lane) & 1ull) && rank < np;
+            uint32_t *const mail = wave_col + (size_t)STACK * stride;
+            if (given) mail[rank] = lane;
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            const int src = idle && rank < np ? (int)mail[rank] : -1;
             const int s4 = src >= 0 ? src * 4 : self4;
             const uint32_t dsb = lane_read(tr.sb, s4);
             const int down4 = __builtin_amdgcn_ds_bpermute(s4, own4);
