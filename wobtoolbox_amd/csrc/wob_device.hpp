@@ -793,10 +793,10 @@ __device__ __forceinline__ LineHit<D> line_query_ws(const DevScene &S, const flo
                                     np_hi != (uint32_t)__builtin_amdgcn_readfirstlane((int)np_hi)) == 0ull;
     while (true) {
         tr.settle(stack, stride);
-        const bool want_leaf = tr.want_prim();
-        const bool want_node = tr.want_node();
-        const uint64_t bl = __ballot(want_leaf);
-        const uint64_t bn = __ballot(want_node);
+        bool want_leaf = tr.want_prim();
+        bool want_node = tr.want_node();
+        uint64_t bl = __ballot(want_leaf);
+        uint64_t bn = __ballot(want_node);
         // after settle a lane that wants neither step has no work left
         const uint64_t bi = active & ~(bl | bn);
         const uint64_t bd = one_scene && __popcll(bi) >= kStealMinIdle ? __ballot(tr.spo != tr.sb) : 0ull;
@@ -833,7 +833,12 @@ __device__ __forceinline__ LineHit<D> line_query_ws(const DevScene &S, const flo
                 own4 = down4;
             }
             if (given) tr.sb += stride;
-            continue;
+            // the thieves start on their entries in this round
+            tr.settle(stack, stride);
+            want_leaf = tr.want_prim();
+            want_node = tr.want_node();
+            bl = __ballot(want_leaf);
+            bn = __ballot(want_node);
         }
         if ((bl | bn) == 0ull) break;
         if (__popcll(bl) >= __popcll(bn)) {
