@@ -46,9 +46,9 @@ def workload_model(name, n_elements, path_length):
 # HBM bytes per walk-kernel launch measured with rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE
 # (100,000 points x 256 paths, Direct / Neumann, 6 bounces), each counter scaled by what it reads
 # for a 512 MiB device copy in the same run: FETCH_SIZE 262,166 KiB (x 2.0), WRITE_SIZE
-# 524,288 KiB (x 1.0).  Walk kernel: FETCH_SIZE 19,458 KiB, WRITE_SIZE 35,008 KiB.
+# 524,288 KiB (x 1.0).  Walk kernel: FETCH_SIZE 19,500 KiB, WRITE_SIZE 35,008 KiB.
 MEASURED_TRAFFIC = {
-    "icosphere_neumann_3d": (2.0 * 19458 + 35008) * 1024,
+    "icosphere_neumann_3d": (2.0 * 19500 + 35008) * 1024,
 }
 
 MI355X_PEAKS = {"hbm_GBps": 8000.0, "fp32_TFLOPs": 157.3, "bf16_TFLOPs": 2500.0}
