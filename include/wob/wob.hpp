@@ -109,7 +109,10 @@ template <typename S, unsigned Dim, bool IsVectorProblem = false> class SceneHos
   public:
     using Element = ElementXs<float, Dim, false>;
 
-    explicit SceneHost(const std::vector<Element> &elems) {
+    // scene.cuh:265: the reference builds its BVH on the device (lbvh); device_bvh = true does
+    // the same (wob_scene_create_ex with WOB_SCENE_DEVICE_BVH), false uses the host SAH build.
+    // Results do not depend on which tree is used.
+    explicit SceneHost(const std::vector<Element> &elems, bool device_bvh = false) {
         const size_t n = elems.size();
         std::vector<float> v(n * Dim * Dim), nr(n * Dim), bv(n * Dim), al(n * Dim);
         std::vector<int32_t> ty(n);
@@ -132,7 +135,7 @@ template <typename S, unsigned Dim, bool IsVectorProblem = false> class SceneHos
             ty[e] = (int32_t)x.boundary_type;
         }
         const wob_scene_desc d{Dim, (uint32_t)n, v.data(), nr.data(), bv.data(), ty.data(), al.data()};
-        check(wob_scene_create(&d, &scene_));
+        check(wob_scene_create_ex(&d, device_bvh ? WOB_SCENE_DEVICE_BVH : 0u, &scene_));
     }
     SceneHost(const SceneHost &) = delete;
     SceneHost &operator=(const SceneHost &) = delete;

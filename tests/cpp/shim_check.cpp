@@ -3,7 +3,7 @@
 // run time with wob::run (utils::run, runtime_utils.cuh:106-153) and accumulates
 // compute_sample_path_contribution_domain over device points.
 //
-// usage: shim_check <in.bin> <out.bin>
+// usage: shim_check <in.bin> <out.bin> [device]   (device: BVH built on the device)
 //   in:  uint32 dim, n_elements, n_points, paths, path_length, bie, domain, estimator, problem,
 //        nrc; float eps, fkesc; uint64 seed; then float vertices[n*dim*dim], normals[n*dim],
 //        boundary_values[n*dim]; int32 types[n]; float alphas[n*dim], points[n_points*dim]
@@ -44,7 +44,7 @@ template <unsigned Dim> struct RunEstimator {
     }
 };
 
-template <unsigned Dim> int run(const Header &h, FILE *f, FILE *out) {
+template <unsigned Dim> int run(const Header &h, FILE *f, FILE *out, bool device_bvh) {
     const size_t n = h.n;
     std::vector<float> v(n * Dim * Dim), nr(n * Dim), bv(n * Dim), al(n * Dim), pts((size_t)h.n_pts * Dim);
     std::vector<int32_t> ty(n);
@@ -77,7 +77,7 @@ template <unsigned Dim> int run(const Header &h, FILE *f, FILE *out) {
         x.boundary_type = (wob::BoundaryType)ty[e];
     }
     try {
-        wob::SceneHost<float, Dim> scene(els);
+        wob::SceneHost<float, Dim> scene(els, device_bvh);
         const size_t m = (size_t)h.n_pts * (Dim + 1);
         float *d_pts = nullptr;
         double *d_acc = nullptr;
@@ -115,8 +115,9 @@ template <unsigned Dim> int run(const Header &h, FILE *f, FILE *out) {
 }
 
 int main(int argc, char **argv) {
-    if (argc != 3) {
-        std::fprintf(stderr, "usage: %s in.bin out.bin\n", argv[0]);
+    const bool device_bvh = argc == 4 && std::strcmp(argv[3], "device") == 0;
+    if (argc != 3 && !device_bvh) {
+        std::fprintf(stderr, "usage: %s in.bin out.bin [device]\n", argv[0]);
         return 1;
     }
     FILE *f = std::fopen(argv[1], "rb");
@@ -125,7 +126,7 @@ int main(int argc, char **argv) {
     if (std::fread(&h, sizeof h, 1, f) != 1) return 1;
     FILE *out = std::fopen(argv[2], "wb");
     if (!out) return 1;
-    const int rc = h.dim == 2 ? run<2>(h, f, out) : h.dim == 3 ? run<3>(h, f, out) : 1;
+    const int rc = h.dim == 2 ? run<2>(h, f, out, device_bvh) : h.dim == 3 ? run<3>(h, f, out, device_bvh) : 1;
     std::fclose(f);
     std::fclose(out);
     return rc;

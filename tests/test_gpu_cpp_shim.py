@@ -50,8 +50,10 @@ def _write_input(path, w, points, paths, seed):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["circle_dl", "star_sl_ris", "icosphere_direct"])
-def test_cpp_shim_matches_python_path(case, gpu, tmp_path):
+@pytest.mark.parametrize("case,device_bvh", [("circle_dl", False), ("star_sl_ris", False),
+                                             ("icosphere_direct", False), ("icosphere_direct", True),
+                                             ("star_sl_ris", True)])
+def test_cpp_shim_matches_python_path(case, device_bvh, gpu, tmp_path):
     import torch
     import wobtoolbox_amd as wob
     if not os.path.exists(B.SHIM_CHECK):
@@ -65,14 +67,15 @@ def test_cpp_shim_matches_python_path(case, gpu, tmp_path):
     pts = np.ascontiguousarray(w.points[:500], np.float32)
     paths, seed = 64, 1234
     _write_input(tmp_path / "in.bin", w, pts, paths, seed)
-    print(f"shim case {case}: {len(pts)} points", flush=True)
-    r = subprocess.run([B.SHIM_CHECK, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")], capture_output=True,
-                       text=True, timeout=120)
+    print(f"shim case {case} (device BVH: {device_bvh}): {len(pts)} points", flush=True)
+    args = [B.SHIM_CHECK, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")] + (["device"] if device_bvh else [])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     V = w.elements.dim + 1
     m = len(pts) * V
     raw = (tmp_path / "out.bin").read_bytes()
     acc = np.frombuffer(raw[:32 * m], np.float64).reshape(4, len(pts), V)
+    # the Python side always uses the host-built tree: results do not depend on the tree
     scene = wob.SceneHost(w.elements)
     est = wob.Estimator.from_config(scene, w.solver)
     s, q = est.compute_domain(torch.as_tensor(pts).to(gpu), paths, seed=seed)
@@ -80,4 +83,4 @@ def test_cpp_shim_matches_python_path(case, gpu, tmp_path):
     assert np.array_equal(acc[0], s) and np.array_equal(acc[1], q), "templated Estimator differs from the Python path"
     assert np.array_equal(acc[2], s) and np.array_equal(acc[3], q), "run_domain differs from the Python path"
     nodes = struct.unpack("<I", raw[32 * m:32 * m + 4])[0]
-    assert nodes == scene.info()["n_nodes"]
+    assert nodes == wob.SceneHost(w.elements, device_bvh=device_bvh).info()["n_nodes"]
