@@ -261,8 +261,9 @@ def vpl_leg(args, torch, wob, G):
 
 def scene_build_leg(args, torch, wob, G, headline):
     """Scene construction with the BVH built on the host (binned SAH) and on the device (LBVH
-    collapsed to the same 4-wide layouts), on the 200k-triangle geodesic sphere, and the
-    headline walk on a device-built scene (same results; the tree differs)."""
+    with SAH treelet restructuring, collapsed to the same 4-wide layouts), on the 200k-triangle
+    geodesic sphere, and the headline walk on a device-built scene (same results; the tree
+    differs)."""
     print("[bench] scene build leg", file=sys.stderr, flush=True)
     geo = G.geodesic_mixed_3d(n_points=1000)
     out = {"workload": "geodesic_mixed_3d", "elements": int(geo.elements.n)}

@@ -81,10 +81,10 @@ wob_status wob_scene_create(const wob_scene_desc *desc, wob_scene **out_scene);
 
 /* wob_scene_create with build options.  WOB_SCENE_DEVICE_BVH builds the BVH on the current
  * device, as the reference's SceneHost does with lbvh (scene.cuh:264-299; deps/lbvh): Morton
- * codes of the element boxes, a radix sort, the binary radix tree and its boxes, collapsed
- * level by level into the same 4-wide node layouts (subtrees of at most two primitives become
- * leaves, as in the host build's leaf rule).  Attributes and
- * CDFs are built on the host as in wob_scene_create.  Walk results, intersections and
+ * codes of the element boxes, a radix sort, the binary radix tree and its boxes, two passes
+ * of SAH treelet restructuring, then a collapse level by level into the same 4-wide node
+ * layouts (subtrees of at most two primitives become leaves).  Attributes and CDFs are built
+ * on the host as in wob_scene_create.  Walk results, intersections and
  * closest points are the same as with the host-built scene: they do not depend on the tree.
  * WOB_DEVICE if the device build fails; WOB_INVALID_ARG for unknown flags or a stack bound
  * over 48 entries.                                                                          */

@@ -12,14 +12,16 @@
 //   3. the binary radix tree, one internal node per thread; equal codes are ordered by
 //      index (the delta function falls back to the index bits);
 //   4. boxes bottom-up: each leaf walks to the root, the second child to arrive at a node
-//      writes its box (device-scope fences and coherent loads);
+//      writes its box (device-scope fences and coherent loads); then treelet restructuring
+//      by the SAH over the same arrival order, and every node's first primitive in
+//      depth-first order (primitive records follow that order);
 //   5. top-down, one level per launch: each binary node of the level becomes a wide node
 //      whose children are its binary children, expanded largest-measure-first until there
 //      are four (the host's collapse rule); child indices of the next level come from an
 //      exclusive scan of the internal-child counts, so the layout is deterministic; the
 //      float node and its quantized copy are written in the same kernel;
 //   6. the traversal stack bound, bottom-up over the levels;
-//   7. primitive records in sorted (leaf) order.
+//   7. primitive records in depth-first leaf order.
 // Subtrees of at most kLeafPrims primitives become leaves.
 #include <hip/hip_runtime.h>
 
@@ -175,10 +177,186 @@ template <int D> __device__ __forceinline__ float measure(const Box6 &b) {
     return D == 2 ? e[0] + e[1] : e[0] * e[1] + e[1] * e[2] + e[2] * e[0];
 }
 
+__device__ __forceinline__ uint32_t coherent_load(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ Box6 coherent_box(const Box6 *b) {
+    Box6 r;
+    for (int k = 0; k < 3; ++k) {
+        r.lo[k] = coherent_load(b->lo + k);
+        r.hi[k] = coherent_load(b->hi + k);
+    }
+    return r;
+}
+__device__ __forceinline__ void grow(Box6 &a, const Box6 &b) {
+    for (int k = 0; k < 3; ++k) {
+        a.lo[k] = fminf(a.lo[k], b.lo[k]);
+        a.hi[k] = fmaxf(a.hi[k], b.hi[k]);
+    }
+}
+
+// 4b: treelet restructuring (Karras and Aila 2013, "Fast parallel construction of
+// high-quality bounding volume hierarchies"), bottom-up with the arrival counters of step 4.
+// At each binary node of at least three primitives the treelet of up to kTreeletLeaves
+// leaves is grown by opening the leaf of largest measure, then rebuilt with the topology of
+// least SAH cost over all binary trees on those leaves (dynamic programming over the leaf
+// subsets), reusing the treelet's own internal node ids.  Cost model: kTreeCi * measure per
+// internal node, kTreeCt * measure per primitive.  Boxes of rebuilt nodes are unions of
+// their leaves' boxes, so every box stays the exact union of its primitives' boxes.  A node
+// only changes when the optimum is strictly cheaper, and each treelet depends only on its
+// own subtree, so the result does not depend on the order in which threads arrive.
+constexpr int kTreeletLeaves = 7;
+constexpr int kTreeletPasses = 2;
+constexpr float kTreeCi = 1.2f, kTreeCt = 1.0f;
+
+template <int D>
+__device__ void restructure(uint32_t n_int, uint32_t p, uint32_t *left, uint32_t *right, uint32_t *parent,
+                            uint32_t *count, Box6 *box, float *cost) {
+    const uint32_t l = coherent_load(left + p), r = coherent_load(right + p);
+    uint32_t L[kTreeletLeaves], I[kTreeletLeaves - 1], LC[kTreeletLeaves];
+    Box6 LB[kTreeletLeaves];
+    float LA[kTreeletLeaves], c[1 << kTreeletLeaves];
+    uint8_t split[1 << kTreeletLeaves];
+    auto load_leaf = [&](int i, uint32_t id) {
+        L[i] = id;
+        LB[i] = coherent_box(box + id);
+        LA[i] = measure<D>(LB[i]);
+        LC[i] = id < n_int ? coherent_load(count + id) : 1u;
+        c[1 << i] = id < n_int ? coherent_load(cost + id) : kTreeCt * LA[i];
+    };
+    load_leaf(0, l);
+    load_leaf(1, r);
+    Box6 bp = LB[0];
+    grow(bp, LB[1]);
+    const float cur = kTreeCi * measure<D>(bp) + c[1] + c[2];
+    int k = 2, ni = 1;
+    I[0] = p;
+    if (LC[0] + LC[1] >= 3u) {
+        while (k < kTreeletLeaves) {
+            int best = -1;
+            float bestA = -1.f;
+            for (int i = 0; i < k; ++i)
+                if (L[i] < n_int && LA[i] > bestA) { bestA = LA[i]; best = i; }
+            if (best < 0) break;
+            const uint32_t x = L[best];
+            I[ni++] = x;
+            load_leaf(best, coherent_load(left + x));
+            load_leaf(k, coherent_load(right + x));
+            ++k;
+        }
+    }
+    if (k < 3) {
+        cost[p] = cur;
+        return;
+    }
+    const uint32_t full = (1u << k) - 1u;
+    for (uint32_t s = 1; s <= full; ++s) {
+        if ((s & (s - 1u)) == 0u) continue;   // one leaf: its subtree cost, set above
+        Box6 u;
+        for (int q = 0; q < 3; ++q) { u.lo[q] = INFINITY; u.hi[q] = -INFINITY; }
+        for (int i = 0; i < k; ++i)
+            if ((s >> i) & 1u) grow(u, LB[i]);
+        const uint32_t low = s & (0u - s);
+        float best = INFINITY;
+        uint32_t bs = 0;
+        for (uint32_t q = (s - 1u) & s; q != 0u; q = (q - 1u) & s) {
+            if (!(q & low)) continue;   // each partition once: the side holding the lowest leaf
+            const float v = c[q] + c[s ^ q];
+            if (v < best) { best = v; bs = q; }
+        }
+        c[s] = kTreeCi * measure<D>(u) + best;
+        split[s] = (uint8_t)bs;
+    }
+    if (!(c[full] < cur)) {
+        cost[p] = cur;
+        return;
+    }
+    uint32_t stk_node[kTreeletLeaves], stk_set[kTreeletLeaves];
+    int sp = 0, used = 1;
+    stk_node[sp] = p;
+    stk_set[sp++] = full;
+    while (sp > 0) {
+        --sp;
+        const uint32_t node = stk_node[sp], s = stk_set[sp];
+        const uint32_t sub[2] = {split[s], s ^ split[s]};
+        uint32_t ch[2];
+        for (int t = 0; t < 2; ++t) {
+            if ((sub[t] & (sub[t] - 1u)) == 0u) {
+                ch[t] = L[__ffs(sub[t]) - 1];
+            } else {
+                ch[t] = I[used++];
+                stk_node[sp] = ch[t];
+                stk_set[sp++] = sub[t];
+            }
+        }
+        Box6 u;
+        for (int q = 0; q < 3; ++q) { u.lo[q] = INFINITY; u.hi[q] = -INFINITY; }
+        uint32_t cn = 0;
+        for (int i = 0; i < k; ++i)
+            if ((s >> i) & 1u) { grow(u, LB[i]); cn += LC[i]; }
+        left[node] = ch[0];
+        right[node] = ch[1];
+        parent[ch[0]] = node;
+        parent[ch[1]] = node;
+        box[node] = u;
+        count[node] = cn;
+        cost[node] = c[s];
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(64) void treelet_kernel(int n, uint32_t *left, uint32_t *right, uint32_t *parent,
+                                                     uint32_t *count, Box6 *box, float *cost, uint32_t *flag) {
+    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= n || n < 2) return;
+    const uint32_t n_int = (uint32_t)(n - 1);
+    uint32_t p = parent[n_int + (uint32_t)j];
+    // every step goes to an ancestor in the tree as it was at launch (a node's parent only
+    // changes when a treelet above it is rebuilt, after this thread has passed), so the
+    // walk is at most the launch-time depth
+    for (uint32_t up = 0; up < n_int; ++up) {
+        __threadfence();
+        if (atomicAdd(&flag[p], 1u) == 0u) return;   // the sibling subtree is not done yet
+        restructure<D>(n_int, p, left, right, parent, count, box, cost);
+        if (p == 0u) return;
+        p = coherent_load(parent + p);
+    }
+}
+
+// 4c: position of every node's first primitive in depth-first (left before right) order, so
+// each subtree is a contiguous range of primitive records after the restructuring
+__global__ __launch_bounds__(256) void order_kernel(int n, const uint32_t *__restrict__ left,
+                                                    const uint32_t *__restrict__ right, const uint32_t *__restrict__ parent,
+                                                    const uint32_t *__restrict__ count, uint32_t *__restrict__ first,
+                                                    uint32_t *__restrict__ bad) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n_int = (uint32_t)(n - 1);
+    if (id >= n_int + (uint32_t)n) return;
+    uint32_t x = id, pos = 0;
+    for (uint32_t up = 0; up < n_int && x != 0u; ++up) {
+        const uint32_t pp = parent[x];
+        if (pp >= n_int) break;   // parents are internal nodes
+        if (right[pp] == x) {
+            const uint32_t lc = left[pp];
+            pos += lc < n_int ? count[lc] : 1u;
+        }
+        x = pp;
+    }
+    // a tree over the n primitives: parent links lead to the root, child and parent links
+    // agree, and the ranges fit
+    const uint32_t own = id < n_int ? count[id] : 1u;
+    bool ok = x == 0u && pos + own <= (uint32_t)n;
+    if (id < n_int) ok = ok && left[id] < n_int + (uint32_t)n && right[id] < n_int + (uint32_t)n &&
+                         parent[left[id]] == id && parent[right[id]] == id;
+    if (id != 0u) ok = ok && parent[id] < n_int && (left[parent[id]] == id || right[parent[id]] == id);
+    if (!ok) atomicOr(bad, 1u);
+    first[id] = pos;
+}
+
 // children of binary node b as a wide node (host rule: expand the internal child of largest
 // measure, first one on ties, until there are four or only leaves)
-// A subtree of at most kLeafPrims primitives (a contiguous range in sorted order) becomes one
-// leaf.
+// A subtree of at most kLeafPrims primitives (a contiguous range in depth-first order)
+// becomes one leaf.
 constexpr uint32_t kLeafPrims = 2;
 __device__ __forceinline__ bool is_inner(uint32_t id, uint32_t n_int, const uint32_t *count) {
     return id < n_int && count[id] > kLeafPrims;
@@ -283,7 +461,7 @@ __global__ __launch_bounds__(256) void write_kernel(int n, uint32_t m, uint32_t 
             } else if (kids[c] < n_int) {   // a small subtree: its sorted range is one leaf
                 child[c] = ~((first[kids[c]] << 4) | (count[kids[c]] - 1u));
             } else {
-                child[c] = ~((kids[c] - n_int) << 4);   // one primitive: ~((first << 4) | 0)
+                child[c] = ~(first[kids[c]] << 4);   // one primitive: ~((first << 4) | 0)
             }
         }
         for (int k = 0; k < D; ++k) {
@@ -332,13 +510,20 @@ __global__ __launch_bounds__(256) void need_kernel(uint32_t base, uint32_t m, co
     need[base + t] = (nk > 0 ? nk - 1 : 0) + mx;
 }
 
-// 7: primitive records in leaf (sorted) order, as scene_build.cpp writes them
+// 7: primitive records in depth-first leaf order, in the layout scene_build.cpp writes
 template <int D>
 __global__ __launch_bounds__(256) void prims_kernel(uint32_t n, const float *__restrict__ V, const uint32_t *__restrict__ idx,
-                                                    float *__restrict__ prims, uint32_t *__restrict__ prim_id) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const uint32_t e = idx[j];
+                                                    const uint32_t *__restrict__ first, float *__restrict__ prims,
+                                                    uint32_t *__restrict__ prim_id, uint32_t *__restrict__ seen,
+                                                    uint32_t *__restrict__ bad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t e = idx[i], j = first[n - 1 + i];   // sorted leaf i sits at depth-first position j
+    // the positions must be a permutation of [0, n): in range and each taken once
+    if (j >= n || atomicAdd(seen + j, 1u) != 0u) {
+        atomicOr(bad, 2u);
+        return;
+    }
     prim_id[j] = e;
     auto v = [&](int a, int k) { return V[((size_t)e * D + a) * D + k]; };
     if constexpr (D == 2) {
@@ -407,7 +592,9 @@ template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBv
     const uint32_t n_int = n - 1, n_all = 2 * n - 1;
     uint32_t *left = tmp.get<uint32_t>(std::max<uint32_t>(n_int, 1)), *right = tmp.get<uint32_t>(std::max<uint32_t>(n_int, 1));
     uint32_t *parent = tmp.get<uint32_t>(n_all), *flag = tmp.get<uint32_t>(std::max<uint32_t>(n_int, 1));
-    uint32_t *first = tmp.get<uint32_t>(std::max<uint32_t>(n_int, 1)), *count = tmp.get<uint32_t>(std::max<uint32_t>(n_int, 1));
+    uint32_t *first = tmp.get<uint32_t>(n_all), *count = tmp.get<uint32_t>(std::max<uint32_t>(n_int, 1));
+    float *cost = tmp.get<float>(std::max<uint32_t>(n_int, 1));
+    uint32_t *bad = tmp.get<uint32_t>(1);
     Box6 *box = tmp.get<Box6>(n_all);
     // wide nodes: at most one per binary internal node (or the single leaf root)
     const uint32_t max_wide = std::max<uint32_t>(n_int, 1);
@@ -421,6 +608,7 @@ template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBv
         const hipError_t e_ = (x);                                                              \
         if (e_ != hipSuccess) return std::string("device BVH build: ") + hipGetErrorString(e_); \
     } while (0)
+    WOB_TRY(hipMemsetAsync(bad, 0, sizeof(uint32_t), st));
     WOB_TRY(hipMemcpyAsync(V, h_vertices, (size_t)n * D * D * sizeof(float), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL((leaf_kernel<D>), dim3(blocks(n)), dim3(256), 0, st, n, V, make_float3(mn[0], mn[1], mn[2]),
                        make_float3(sc[0], sc[1], sc[2]), code, idx, pbox);
@@ -439,6 +627,25 @@ template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBv
     }
     hipLaunchKernelGGL(bounds_kernel, dim3(blocks(n)), dim3(256), 0, st, (int)n, idx2, pbox, left, right, parent, flag, box);
     WOB_TRY(hipGetLastError());
+    if (n > 2) {
+        for (int pass = 0; pass < kTreeletPasses; ++pass) {
+            WOB_TRY(hipMemsetAsync(flag, 0, (size_t)n_int * sizeof(uint32_t), st));
+            hipLaunchKernelGGL((treelet_kernel<D>), dim3((n + 63) / 64), dim3(64), 0, st, (int)n, left, right, parent, count,
+                               box, cost, flag);
+            WOB_TRY(hipGetLastError());
+        }
+    }
+    if (n > 1) {
+        hipLaunchKernelGGL(order_kernel, dim3(blocks(n_all)), dim3(256), 0, st, (int)n, left, right, parent, count, first,
+                           bad);
+        WOB_TRY(hipGetLastError());
+    } else {
+        WOB_TRY(hipMemsetAsync(first, 0, sizeof(uint32_t), st));
+    }
+    uint32_t bad0 = 0;
+    WOB_TRY(hipMemcpyAsync(&bad0, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    WOB_TRY(hipStreamSynchronize(st));
+    if (bad0 != 0u) return "device BVH build: internal error (tree check " + std::to_string(bad0) + ")";
 
     // outputs
     float *fnodes = nullptr, *prims = nullptr;
@@ -489,12 +696,15 @@ template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBv
                            fnodes, need);
         WOB_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL((prims_kernel<D>), dim3(blocks(n)), dim3(256), 0, st, n, V, idx2, prims, prim_id);
+    WOB_TRY(hipMemsetAsync(code, 0, (size_t)n * sizeof(uint32_t), st));   // the sort input, reused as position marks
+    hipLaunchKernelGGL((prims_kernel<D>), dim3(blocks(n)), dim3(256), 0, st, n, V, idx2, first, prims, prim_id, code, bad);
     WOB_TRY(hipGetLastError());
     uint32_t need0 = 0;
     WOB_TRY(hipMemcpyAsync(&need0, need, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    WOB_TRY(hipMemcpyAsync(&bad0, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     WOB_TRY(hipStreamSynchronize(st));
 #undef WOB_TRY
+    if (bad0 != 0u) return "device BVH build: internal error (tree check " + std::to_string(bad0) + ")";
     out.n_nodes = n_wide;
     out.depth = (uint32_t)level_base.size();
     out.stack_need = need0 + 1;
