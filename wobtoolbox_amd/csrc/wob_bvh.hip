@@ -209,37 +209,63 @@ constexpr int kTreeletLeaves = 7;
 constexpr int kTreeletPasses = 2;
 constexpr float kTreeCi = 1.2f, kTreeCt = 1.0f;
 
+// The subset tables live in LDS, one column per lane: cost of subset s at c[s * kTreeletLanes],
+// its best split at split[s * kTreeletLanes].
+constexpr int kTreeletLanes = 64;
+
 template <int D>
 __device__ void restructure(uint32_t n_int, uint32_t p, uint32_t *left, uint32_t *right, uint32_t *parent,
-                            uint32_t *count, Box6 *box, float *cost) {
-    const uint32_t l = coherent_load(left + p), r = coherent_load(right + p);
-    uint32_t L[kTreeletLeaves], I[kTreeletLeaves - 1], LC[kTreeletLeaves];
-    Box6 LB[kTreeletLeaves];
-    float LA[kTreeletLeaves], c[1 << kTreeletLeaves];
-    uint8_t split[1 << kTreeletLeaves];
+                            uint32_t *count, Box6 *box, float *cost, float *c_col, uint8_t *split_col) {
+    // Leaf arrays are only indexed by compile-time constants (unrolled loops with a
+    // predicate), so they stay in registers.
+    constexpr int K = kTreeletLeaves;
+    uint32_t L[K], I[K - 1], LC[K];
+    Box6 LB[K];
+    float LA[K];
+    auto c = [&](uint32_t set) -> float & { return c_col[set * kTreeletLanes]; };
+    auto split = [&](uint32_t set) -> uint8_t & { return split_col[set * kTreeletLanes]; };
     auto load_leaf = [&](int i, uint32_t id) {
-        L[i] = id;
-        LB[i] = coherent_box(box + id);
-        LA[i] = measure<D>(LB[i]);
-        LC[i] = id < n_int ? coherent_load(count + id) : 1u;
-        c[1 << i] = id < n_int ? coherent_load(cost + id) : kTreeCt * LA[i];
+        const Box6 b = coherent_box(box + id);
+        const float a = measure<D>(b);
+        const uint32_t cn = id < n_int ? coherent_load(count + id) : 1u;
+        c(1u << i) = id < n_int ? coherent_load(cost + id) : kTreeCt * a;
+#pragma unroll
+        for (int t = 0; t < K; ++t)
+            if (t == i) { L[t] = id; LB[t] = b; LA[t] = a; LC[t] = cn; }
     };
-    load_leaf(0, l);
-    load_leaf(1, r);
+    auto leaf_of = [&](uint32_t bit_set) {   // the leaf of a one-leaf subset
+        uint32_t id = 0;
+#pragma unroll
+        for (int t = 0; t < K; ++t)
+            if (bit_set == (1u << t)) id = L[t];
+        return id;
+    };
+    auto union_of = [&](uint32_t set, Box6 &u, uint32_t &cn) {
+        for (int q = 0; q < 3; ++q) { u.lo[q] = INFINITY; u.hi[q] = -INFINITY; }
+        cn = 0;
+#pragma unroll
+        for (int t = 0; t < K; ++t)
+            if ((set >> t) & 1u) { grow(u, LB[t]); cn += LC[t]; }
+    };
+    load_leaf(0, coherent_load(left + p));
+    load_leaf(1, coherent_load(right + p));
     Box6 bp = LB[0];
     grow(bp, LB[1]);
-    const float cur = kTreeCi * measure<D>(bp) + c[1] + c[2];
-    int k = 2, ni = 1;
+    const float cur = kTreeCi * measure<D>(bp) + c(1u) + c(2u);
+    int k = 2;
     I[0] = p;
     if (LC[0] + LC[1] >= 3u) {
-        while (k < kTreeletLeaves) {
+        while (k < K) {
             int best = -1;
             float bestA = -1.f;
-            for (int i = 0; i < k; ++i)
-                if (L[i] < n_int && LA[i] > bestA) { bestA = LA[i]; best = i; }
+            uint32_t x = 0;
+#pragma unroll
+            for (int t = 0; t < K; ++t)
+                if (t < k && L[t] < n_int && LA[t] > bestA) { bestA = LA[t]; best = t; x = L[t]; }
             if (best < 0) break;
-            const uint32_t x = L[best];
-            I[ni++] = x;
+#pragma unroll
+            for (int t = 1; t < K - 1; ++t)
+                if (t == k - 1) I[t] = x;
             load_leaf(best, coherent_load(left + x));
             load_leaf(k, coherent_load(right + x));
             ++k;
@@ -253,60 +279,65 @@ __device__ void restructure(uint32_t n_int, uint32_t p, uint32_t *left, uint32_t
     for (uint32_t s = 1; s <= full; ++s) {
         if ((s & (s - 1u)) == 0u) continue;   // one leaf: its subtree cost, set above
         Box6 u;
-        for (int q = 0; q < 3; ++q) { u.lo[q] = INFINITY; u.hi[q] = -INFINITY; }
-        for (int i = 0; i < k; ++i)
-            if ((s >> i) & 1u) grow(u, LB[i]);
+        uint32_t cn;
+        union_of(s, u, cn);
         const uint32_t low = s & (0u - s);
         float best = INFINITY;
         uint32_t bs = 0;
         for (uint32_t q = (s - 1u) & s; q != 0u; q = (q - 1u) & s) {
             if (!(q & low)) continue;   // each partition once: the side holding the lowest leaf
-            const float v = c[q] + c[s ^ q];
+            const float v = c(q) + c(s ^ q);
             if (v < best) { best = v; bs = q; }
         }
-        c[s] = kTreeCi * measure<D>(u) + best;
-        split[s] = (uint8_t)bs;
+        c(s) = kTreeCi * measure<D>(u) + best;
+        split(s) = (uint8_t)bs;
     }
-    if (!(c[full] < cur)) {
+    if (!(c(full) < cur)) {
         cost[p] = cur;
         return;
     }
-    uint32_t stk_node[kTreeletLeaves], stk_set[kTreeletLeaves];
+    // rebuild top-down; internal node ids come from I in order (k - 1 of them, p first)
+    uint32_t stk_node[K - 1], stk_set[K - 1];
     int sp = 0, used = 1;
     stk_node[sp] = p;
     stk_set[sp++] = full;
     while (sp > 0) {
         --sp;
         const uint32_t node = stk_node[sp], s = stk_set[sp];
-        const uint32_t sub[2] = {split[s], s ^ split[s]};
+        const uint32_t sub[2] = {split(s), s ^ split(s)};
         uint32_t ch[2];
         for (int t = 0; t < 2; ++t) {
             if ((sub[t] & (sub[t] - 1u)) == 0u) {
-                ch[t] = L[__ffs(sub[t]) - 1];
+                ch[t] = leaf_of(sub[t]);
             } else {
-                ch[t] = I[used++];
-                stk_node[sp] = ch[t];
+                uint32_t id = 0;
+#pragma unroll
+                for (int q = 1; q < K - 1; ++q)
+                    if (q == used) id = I[q];
+                ++used;
+                ch[t] = id;
+                stk_node[sp] = id;
                 stk_set[sp++] = sub[t];
             }
         }
         Box6 u;
-        for (int q = 0; q < 3; ++q) { u.lo[q] = INFINITY; u.hi[q] = -INFINITY; }
-        uint32_t cn = 0;
-        for (int i = 0; i < k; ++i)
-            if ((s >> i) & 1u) { grow(u, LB[i]); cn += LC[i]; }
+        uint32_t cn;
+        union_of(s, u, cn);
         left[node] = ch[0];
         right[node] = ch[1];
         parent[ch[0]] = node;
         parent[ch[1]] = node;
         box[node] = u;
         count[node] = cn;
-        cost[node] = c[s];
+        cost[node] = c(s);
     }
 }
 
 template <int D>
-__global__ __launch_bounds__(64) void treelet_kernel(int n, uint32_t *left, uint32_t *right, uint32_t *parent,
-                                                     uint32_t *count, Box6 *box, float *cost, uint32_t *flag) {
+__global__ __launch_bounds__(kTreeletLanes) void treelet_kernel(int n, uint32_t *left, uint32_t *right, uint32_t *parent,
+                                                                uint32_t *count, Box6 *box, float *cost, uint32_t *flag) {
+    __shared__ float c_tab[(1 << kTreeletLeaves) * kTreeletLanes];
+    __shared__ uint8_t split_tab[(1 << kTreeletLeaves) * kTreeletLanes];
     const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (j >= n || n < 2) return;
     const uint32_t n_int = (uint32_t)(n - 1);
@@ -317,7 +348,7 @@ __global__ __launch_bounds__(64) void treelet_kernel(int n, uint32_t *left, uint
     for (uint32_t up = 0; up < n_int; ++up) {
         __threadfence();
         if (atomicAdd(&flag[p], 1u) == 0u) return;   // the sibling subtree is not done yet
-        restructure<D>(n_int, p, left, right, parent, count, box, cost);
+        restructure<D>(n_int, p, left, right, parent, count, box, cost, c_tab + threadIdx.x, split_tab + threadIdx.x);
         if (p == 0u) return;
         p = coherent_load(parent + p);
     }
@@ -630,8 +661,8 @@ template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBv
     if (n > 2) {
         for (int pass = 0; pass < kTreeletPasses; ++pass) {
             WOB_TRY(hipMemsetAsync(flag, 0, (size_t)n_int * sizeof(uint32_t), st));
-            hipLaunchKernelGGL((treelet_kernel<D>), dim3((n + 63) / 64), dim3(64), 0, st, (int)n, left, right, parent, count,
-                               box, cost, flag);
+            hipLaunchKernelGGL((treelet_kernel<D>), dim3((n + kTreeletLanes - 1) / kTreeletLanes), dim3(kTreeletLanes), 0, st,
+                               (int)n, left, right, parent, count, box, cost, flag);
             WOB_TRY(hipGetLastError());
         }
     }
