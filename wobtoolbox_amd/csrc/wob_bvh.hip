@@ -281,13 +281,31 @@ __device__ void restructure(uint32_t n_int, uint32_t p, uint32_t *left, uint32_t
         Box6 u;
         uint32_t cn;
         union_of(s, u, cn);
-        const uint32_t low = s & (0u - s);
+        // each partition once: the side q holding the lowest leaf, q = low | r for the
+        // submasks r of the other leaves except all of them, largest first; four at a time so
+        // that their table loads are in flight together (a q past the last one is masked off;
+        // every index stays below 128)
+        const uint32_t low = s & (0u - s), rest = s ^ low;
         float best = INFINITY;
         uint32_t bs = 0;
-        for (uint32_t q = (s - 1u) & s; q != 0u; q = (q - 1u) & s) {
-            if (!(q & low)) continue;   // each partition once: the side holding the lowest leaf
-            const float v = c(q) + c(s ^ q);
-            if (v < best) { best = v; bs = q; }
+        uint32_t r = (rest - 1u) & rest;
+        bool more = true;
+        while (more) {
+            uint32_t qq[4];
+            bool ok[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                qq[u] = low | r;
+                ok[u] = more;
+                more = more && r != 0u;
+                r = (r - 1u) & rest;
+            }
+            float v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = c(qq[u]) + c(s ^ qq[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (ok[u] && v[u] < best) { best = v[u]; bs = qq[u]; }
         }
         c(s) = kTreeCi * measure<D>(u) + best;
         split(s) = (uint8_t)bs;
