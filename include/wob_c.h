@@ -92,11 +92,13 @@ enum { WOB_SCENE_DEVICE_BVH = 1 };
 wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_scene **out_scene);
 wob_status wob_scene_destroy(wob_scene *scene);
 
-/* Scene facts: dim, element count, BVH node count, BVH depth, totals (area, boundary value). */
+/* Scene facts: dim, element count, BVH node count, BVH depth, totals (area, boundary value),
+ * device memory held, and the traversal stack entries a line query of this BVH needs.       */
 typedef struct wob_scene_info {
     uint32_t dim, n_elements, n_nodes, depth;
     float total_area, total_boundary_value;
     uint64_t device_bytes;
+    uint32_t stack_need, reserved;
 } wob_scene_info;
 wob_status wob_scene_get_info(const wob_scene *scene, wob_scene_info *info);
 
@@ -193,7 +195,11 @@ typedef struct wob_launch_config {
  *   d_points   n_points * dim floats (device)
  *   point_base global index of d_points[0] (RNG counter word; lets callers split point sets)
  *   path_base  global index of the first path of every point in this call (progressive passes
- *              use path_base = pass * paths_per_point so every path has its own stream)
+ *              use path_base = pass * paths_per_point so every path has its own stream).
+ *              The stream's path word is 32 bits: path_base + paths_per_point must not exceed
+ *              2^32 (WOB_INVALID_ARG otherwise, before any work); later paths of a point take
+ *              another seed or stream_id, whose streams are independent of these.  Batches
+ *              check path_base + (passes so far + n_passes) * paths_per_point the same way.
  *   stream_id  RNG stream word (0 for a single scene; batch entries use their scene index)
  *   d_sum, d_sumsq  n_points * (dim + 1) doubles [u, grad u] each, ACCUMULATED into (+=)
  * Per path the result is [u, grad u]; per lane paths are Kahan-summed in FP32, chunks are
@@ -334,6 +340,15 @@ wob_status wob_scene_set_control_variate(wob_scene *scene, wob_cv *cv, const flo
  * stream_id).  d_records: n * (2*dim + 4) floats (device).                                  */
 wob_status wob_sample_boundary(const wob_scene *scene, uint32_t n, uint64_t seed, uint32_t stream_id, float *d_records,
                                void *stream);
+
+/* One line-intersection sample per domain point with the walk's sampler
+ * (Scene::sample_boundary_line_intersection, scene.cuh:464-535 / 760-840): a uniformly
+ * oriented line through x_i, all its hits, one hit taken uniformly (the order-independent
+ * pick of the walk) and the BoundaryPoint there with its inverse pdf.  Point i uses the
+ * Philox stream (seed, path, point i, stream_id).  d_points: n*dim floats; d_records:
+ * n * (2*dim + 4) floats; d_inv_pdf: n floats (device).  No hit: a zero record, inv_pdf 0. */
+wob_status wob_sample_lines(const wob_scene *scene, const float *d_points, uint32_t n, uint64_t seed, uint32_t path,
+                            uint32_t stream_id, float *d_records, float *d_inv_pdf, void *stream);
 
 /* All-hit line query (the lbvh query_line_intersect call of scene.cuh:481-486 / 787-790,
  * without its 64-hit cap).  For each line i (origin, direction; device arrays of n*dim

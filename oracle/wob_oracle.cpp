@@ -1289,6 +1289,19 @@ int orc_intersect_line(void *p, const float *origin, const float *dir, int use_b
     return (int)c;
 }
 
+// orc_intersect_line over n lines (origins, dirs: n * dim floats), OpenMP over the lines:
+// counts[i], and ids / t of the first `cap` hits of line i at [i * cap, (i + 1) * cap).
+void orc_intersect_lines(void *p, int n, const float *origins, const float *dirs, int use_bvh, int cap, int threads,
+                         int32_t *counts, int32_t *ids, float *ts) {
+    auto *S = (OrcScene *)p;
+    const int D = S->dim;
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i)
+        counts[i] = orc_intersect_line(p, origins + (size_t)i * D, dirs + (size_t)i * D, use_bvh, cap,
+                                       ids + (size_t)i * cap, ts + (size_t)i * cap);
+}
+
 }  // extern "C"
 
 template <int D> static Estimator<D> make_est(const Scene<D> &s, const orc_est_desc *d) {
@@ -1690,6 +1703,18 @@ float orc_sample_line(void *p, const float *x, uint64_t seed, uint32_t path, uin
     };
     if (S->dim == 2) return emit(S->s2, std::integral_constant<int, 2>{});
     return emit(S->s3, std::integral_constant<int, 3>{});
+}
+
+// orc_sample_line for points x_i (n * dim floats) on the streams (seed, path, i, stream):
+// records (n * (2 dim + 4) floats) and inverse pdfs (n floats).
+void orc_sample_lines(void *p, int n, const float *x, uint64_t seed, uint32_t path, uint32_t stream, int pick_mode,
+                      float *out_rec, float *out_inv) {
+    auto *S = (OrcScene *)p;
+    const int D = S->dim;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i)
+        out_inv[i] = orc_sample_line(p, x + (size_t)i * D, seed, path, (uint32_t)i, stream, pick_mode,
+                                     out_rec + (size_t)i * (2 * D + 4));
 }
 
 }  // extern "C"

@@ -68,6 +68,8 @@ def lib():
         L.orc_triple32.restype = C.c_uint32
         L.orc_sample_line.argtypes = [P, P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, P]
         L.orc_sample_line.restype = C.c_float
+        L.orc_intersect_lines.argtypes = [P, C.c_int, P, P, C.c_int, C.c_int, C.c_int, P, P, P]
+        L.orc_sample_lines.argtypes = [P, C.c_int, P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, P, P]
         L.orc_sample_uniform.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, P]
         L.orc_sample_uniform.restype = C.c_float
         L.orc_closest_points.argtypes = [P, C.c_int, P, P, P, P]
@@ -144,6 +146,27 @@ class OracleScene:
         c = lib().orc_intersect_line(self.h, _p(o), _p(d), int(use_bvh), cap, _p(ids), _p(ts))
         m = min(c, cap)
         return c, ids[:m], ts[:m]
+
+    def intersect_many(self, origins, directions, use_bvh=False, cap=64, threads=0):
+        """All hits of n lines (brute force by default): counts (n,), ids and t (n, cap)."""
+        o = np.ascontiguousarray(origins, np.float32)
+        d = np.ascontiguousarray(directions, np.float32)
+        n = o.shape[0]
+        counts = np.zeros(n, np.int32)
+        ids = np.full((n, cap), -1, np.int32)
+        ts = np.zeros((n, cap), np.float32)
+        lib().orc_intersect_lines(self.h, n, _p(o), _p(d), int(use_bvh), cap, threads, _p(counts), _p(ids), _p(ts))
+        return counts, ids, ts
+
+    def sample_lines(self, points, seed=0, path=0, stream=0, pick_mode=1):
+        """One line sample from each point x_i on the Philox stream (seed, path, i, stream):
+        boundary records (n, 2 dim + 4) and inverse pdfs (n,)."""
+        x = np.ascontiguousarray(points, np.float32)
+        n = x.shape[0]
+        rec = np.zeros((n, 2 * self.dim + 4), np.float32)
+        inv = np.zeros(n, np.float32)
+        lib().orc_sample_lines(self.h, n, _p(x), seed, path, stream, pick_mode, _p(rec), _p(inv))
+        return rec, inv
 
     def estimate_domain(self, solver, points, paths, seed=0, path_base=0, stream=0, threads=0, pick_mode=1,
                         rng_mode=0, point_index=None, return_paths=False, num_volume_samples=0):

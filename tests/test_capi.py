@@ -67,3 +67,26 @@ def test_estimate_rejects_null_scene_and_bad_args():
     assert L.wob_eval_kernels(4, 1, None, None, None, None, 1e-7, None, None) == 1
     assert L.wob_eval_weights(2, 7, 0, 1e-7, 1.0, 1, None, None, None, None, None, None, None) == 2
     assert L.wob_philox4x32_10(0, None, None, None, None) == 0   # empty request is a no-op
+
+
+def test_path_counter_limit_rejected_before_any_work():
+    """Every path of a point owns a Philox stream whose path word is 32 bits: a call whose
+    path range would pass 2^32 is rejected before the scene is even looked at, and the
+    largest allowed range goes on to the ordinary checks."""
+    from wobtoolbox_amd import _capi
+    L = _capi.lib()
+    e = _capi.EstimatorDesc(0, 1, 0, 0, 0, 4, 1, 0, 1e-7, 1.0)
+    lc = _capi.LaunchConfig(0, 0)
+    top = 1 << 32
+    for fn in (L.wob_estimate_domain, L.wob_estimate_boundary):
+        assert fn(None, C.byref(e), None, 1, 0, 16, 0, top - 8, 0, None, None, C.byref(lc), None) == 1
+        assert b"2^32" in L.wob_last_error_message()
+        assert fn(None, C.byref(e), None, 1, 0, 16, 0, top, 0, None, None, C.byref(lc), None) == 1
+        assert b"2^32" in L.wob_last_error_message()
+        # path_base + paths == 2^32 is the last allowed range: the call fails on the null scene
+        assert fn(None, C.byref(e), None, 1, 0, 16, 0, top - 16, 0, None, None, C.byref(lc), None) == 1
+        assert b"null scene" in L.wob_last_error_message()
+    assert L.wob_wos_estimate(None, None, 1, 0, 4, 0, top - 3, 0, 1e-4, 16, None, None, None) == 1
+    assert b"2^32" in L.wob_last_error_message()
+    assert L.wob_wos_estimate(None, None, 1, 0, 4, 0, top - 4, 0, 1e-4, 16, None, None, None) == 1
+    assert b"null scene" in L.wob_last_error_message()

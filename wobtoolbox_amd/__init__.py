@@ -162,6 +162,17 @@ class SceneHost:
                                         _ptr(counts), _ptr(ids), _ptr(ts), _stream_ptr(stream)))
         return counts, ids, ts
 
+    def sample_lines(self, points, seed=0, path=0, stream_id=0, stream=None):
+        """One line-intersection sample per point with the walk's sampler: (records, inv_pdf);
+        point i uses the Philox stream (seed, path, i, stream_id)."""
+        torch = _torch()
+        n = points.shape[0]
+        rec = torch.zeros((n, 2 * self.dim + 4), dtype=torch.float32, device=points.device)
+        inv = torch.zeros(n, dtype=torch.float32, device=points.device)
+        check(lib().wob_sample_lines(self._h, _ptr(points.contiguous()), n, seed, path, stream_id, _ptr(rec),
+                                     _ptr(inv), _stream_ptr(stream)))
+        return rec, inv
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             lib().wob_scene_destroy(self._h)
@@ -292,7 +303,7 @@ class Batch:
 
     def __init__(self, entries, boundary=False, paths_per_lane=0, block_size=0, persistent=False, use_graph=False,
                  schedule=0, regen_threshold=0):
-        """entries: list of (Estimator, device points, paths_per_point[, stream_id]).
+        """entries: list of (Estimator, device points, paths_per_point[, stream_id[, path_base]]).
 
         Each entry gets FP64 (sum, sumsq) device buffers, accumulated over passes."""
         torch = _torch()
@@ -302,13 +313,14 @@ class Batch:
         for i, ent in enumerate(entries):
             est, pts, paths = ent[0], ent[1].contiguous(), int(ent[2])
             sid = int(ent[3]) if len(ent) > 3 else i
+            pbase = int(ent[4]) if len(ent) > 4 else 0
             V = 1 if boundary else est.scene.dim + 1
             shape = (pts.shape[0],) if boundary else (pts.shape[0], V)
             s = torch.zeros(shape, dtype=torch.float64, device=pts.device)
             q = torch.zeros_like(s)
             self.outputs.append((s, q))
             self._keep += [pts, est]
-            arr[i] = _capi.BatchEntry(est.scene.handle, est.desc(), pts.data_ptr(), pts.shape[0], paths, 0, sid, 0,
+            arr[i] = _capi.BatchEntry(est.scene.handle, est.desc(), pts.data_ptr(), pts.shape[0], paths, 0, sid, pbase,
                                       s.data_ptr(), q.data_ptr())
         cfg = _capi.BatchConfig(int(boundary), paths_per_lane, block_size, int(persistent), int(use_graph),
                                 int(schedule), int(regen_threshold))

@@ -22,7 +22,7 @@ EXPORTS = ["wob_scene_create", "wob_scene_create_ex", "wob_scene_destroy", "wob_
            "wob_last_error_message", "wob_version", "wob_batch_create", "wob_batch_run", "wob_batch_tune",
            "wob_batch_destroy", "wob_cv_create", "wob_cv_destroy", "wob_cv_get_info", "wob_cv_get_params",
            "wob_cv_set_params", "wob_cv_train", "wob_cv_gradient", "wob_cv_infer", "wob_scene_set_control_variate",
-           "wob_sample_boundary", "wob_vpl_paths", "wob_vpl_aggregate",
+           "wob_sample_boundary", "wob_sample_lines", "wob_vpl_paths", "wob_vpl_aggregate",
            "wob_closest_points", "wob_wos_estimate"]
 
 
@@ -39,7 +39,8 @@ class SceneDesc(C.Structure):
 
 class SceneInfo(C.Structure):
     _fields_ = [("dim", C.c_uint32), ("n_elements", C.c_uint32), ("n_nodes", C.c_uint32), ("depth", C.c_uint32),
-                ("total_area", C.c_float), ("total_boundary_value", C.c_float), ("device_bytes", C.c_uint64)]
+                ("total_area", C.c_float), ("total_boundary_value", C.c_float), ("device_bytes", C.c_uint64),
+                ("stack_need", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class EstimatorDesc(C.Structure):
@@ -127,6 +128,7 @@ def lib():
         "wob_cv_infer": ([P, P, U32, P, P], C.c_int),
         "wob_scene_set_control_variate": ([P, P, P, U32, P], C.c_int),
         "wob_sample_boundary": ([P, U32, U64, U32, P, P], C.c_int),
+        "wob_sample_lines": ([P, P, U32, U64, U32, U32, P, P, P], C.c_int),
         "wob_vpl_paths": ([P, C.POINTER(EstimatorDesc), U32, U64, U64, U32, P, P, C.POINTER(LaunchConfig), P],
                           C.c_int),
         "wob_vpl_aggregate": ([P, C.POINTER(EstimatorDesc), P, P, U32, P, U32, P, P], C.c_int),

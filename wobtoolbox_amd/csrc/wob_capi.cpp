@@ -229,12 +229,25 @@ wob_status check_block(uint32_t block) {
     return WOB_OK;
 }
 
+// Paths are numbered per point by a 32-bit counter word of the Philox stream: a call may use
+// path indices up to 2^32 - 1 (path_base + paths <= 2^32).  Later paths of the same point
+// need another seed or stream id, whose streams are independent of these.
+constexpr uint64_t kPathLimit = 1ull << 32;
+wob_status check_paths(uint64_t path_base, uint64_t paths) {
+    if (path_base > kPathLimit || paths > kPathLimit - path_base)
+        return fail(WOB_INVALID_ARG, "path_base + paths exceeds 2^32 paths per point: the Philox stream's path word is "
+                                     "32 bits; continue with another seed or stream_id");
+    return WOB_OK;
+}
+
 wob_status run_walk(const wob_scene *cscene, const wob_estimator_desc *est, const float *d_points, uint32_t n_points,
                     uint32_t point_base, uint32_t paths, uint64_t seed, uint64_t path_base, uint32_t stream_id,
                     double *d_sum, double *d_sumsq, wob_launch_config *launch, void *stream, bool boundary) {
-    int flavour = 0;
-    wob_status st = check_est(cscene, est, flavour);
+    // the path range first: it needs no scene, so the rule holds (and is testable) on its own
+    wob_status st = check_paths(path_base, paths);
     if (st != WOB_OK) return st;
+    int flavour = 0;
+    if ((st = check_est(cscene, est, flavour)) != WOB_OK) return st;
     if (n_points == 0 || paths == 0) return WOB_OK;
     if (!d_points || !d_sum || !d_sumsq) return fail(WOB_INVALID_ARG, "null device buffer");
     wob_scene *scene = const_cast<wob_scene *>(cscene);
@@ -327,6 +340,7 @@ struct wob_batch {
     float *partials = nullptr;
     size_t partial_bytes = 0;
     uint32_t *counters = nullptr;           // [0] work counter, [1] pass counter
+    uint64_t passes_run = 0;                // passes run so far (the device pass counter's value)
     uint32_t grid = 0;                      // 0: one lane per item
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -535,6 +549,8 @@ wob_status wob_scene_get_info(const wob_scene *scene, wob_scene_info *info) {
     info->total_area = scene->host.total_area;
     info->total_boundary_value = scene->host.total_bv;
     info->device_bytes = scene->device_bytes;
+    info->stack_need = scene->host.stack_need;
+    info->reserved = 0;
     return WOB_OK;
 }
 
@@ -685,6 +701,17 @@ wob_status wob_eval_weights(uint32_t dim, int32_t bie_model, int32_t domain, flo
     return WOB_OK;
 }
 
+wob_status wob_sample_lines(const wob_scene *scene, const float *d_points, uint32_t n, uint64_t seed, uint32_t path,
+                            uint32_t stream_id, float *d_records, float *d_inv_pdf, void *stream) {
+    if (!scene) return fail(WOB_INVALID_ARG, "null scene");
+    if (n == 0) return WOB_OK;
+    if (!d_points || !d_records || !d_inv_pdf) return fail(WOB_INVALID_ARG, "null device buffer");
+    WOB_HIP(wobl::launch_sample_lines((int)scene->host.dim, scene->dev, n, d_points, seed, path, stream_id, d_records,
+                                      d_inv_pdf, (hipStream_t)stream),
+            "sample_lines launch");
+    return WOB_OK;
+}
+
 wob_status wob_sample_boundary(const wob_scene *scene, uint32_t n, uint64_t seed, uint32_t stream_id, float *d_records,
                                void *stream) {
     if (!scene) return fail(WOB_INVALID_ARG, "null scene");
@@ -779,6 +806,7 @@ wob_status wob_closest_points(const wob_scene *scene, const float *d_points, uin
 wob_status wob_wos_estimate(const wob_scene *cscene, const float *d_points, uint32_t n_points, uint32_t point_base,
                             uint32_t paths_per_point, uint64_t seed, uint64_t path_base, uint32_t stream_id, float eps,
                             uint32_t max_steps, double *d_sum, double *d_sumsq, void *stream) {
+    if (const wob_status ps = check_paths(path_base, paths_per_point); ps != WOB_OK) return ps;
     if (!cscene) return fail(WOB_INVALID_ARG, "null scene");
     if (!(eps > 0.f)) return fail(WOB_INVALID_ARG, "eps must be > 0");
     if (n_points == 0 || paths_per_point == 0) return WOB_OK;
@@ -910,6 +938,12 @@ wob_status wob_batch_create(const wob_batch_entry *entries, uint32_t n, const wo
 
 wob_status wob_batch_run(wob_batch *b, uint64_t seed, uint32_t n_passes, void *stream) {
     if (!b) return fail(WOB_INVALID_ARG, "null batch");
+    for (size_t i = 0; i < b->entries.size(); ++i) {
+        const auto &e = b->entries[i];
+        const wob_status st = check_paths(e.path_base, (b->passes_run + n_passes) * (uint64_t)e.paths_per_point);
+        if (st != WOB_OK) return fail(st, "entry " + std::to_string(i) + ": " + g_last_error);
+    }
+    b->passes_run += n_passes;
     hipStream_t hs = (hipStream_t)stream;
     // a scene whose device data changed since the entry table was built (volume source, control
     // variate) invalidates the table and any captured graph

@@ -16,6 +16,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#define WOB_HD __host__ __device__
+#include "wob_trig.hpp"
+
 // FP32 expressions are evaluated as written (no fused multiply-add contraction), the
 // operation order of the reference's formulas; transcendental calls aside, this keeps the
 // kernel and weight values bit-identical to a non-contracted CPU evaluation.
@@ -513,38 +516,52 @@ template <int D> __device__ __forceinline__ PrimRec<D> prim_load(const DevScene 
     return r;
 }
 
-// line o + t d against a primitive record.  Reference predicate (scene.cuh:431-432, 725-727):
-// parameters in the closed element, |t| > 10 FLT_EPSILON, no NaN.  The fused multiply-adds
-// are written out: left to the compiler, a*b + c*d may be fused either way depending on the
-// code the test is inlined into, and the hit coordinates would round differently in
-// different kernels.
+// line o + t d against a primitive record, solved as the reference solves it
+// (scene.cuh:414-436, 707-730): the matrix [d, a - b, a - c] (2D: [d, a - b]) is inverted as
+// adjugate / det, with 1 / det an IEEE (correctly rounded) division, and the inverse is
+// applied to a - o.  Every product is rounded on its own (contraction is off in this file)
+// and sums of three terms run left to right, so t, a and b are the same FP32 values a
+// non-contracted CPU evaluation of that algebra gives, and the hit predicate -- parameters
+// in the closed element, |t| > 10 FLT_EPSILON, no NaN (scene.cuh:431-432, 725-727) -- decides
+// exactly as it does there, grazing and vertex cases included.
 template <int D>
 __device__ __forceinline__ bool prim_eval(const PrimRec<D> &r, const float *o, const float *d, float &t, float &a,
                                           float &b) {
     constexpr float eps = 10 * kFltEps;
     if constexpr (D == 2) {
-        const float4 g = r.g[0];
-        const float sx = g.x - o[0], sy = g.y - o[1];
-        const float det = __builtin_fmaf(d[0], g.w, -(d[1] * g.z));
-        const float inv = __builtin_amdgcn_rcpf(det);
-        t = __builtin_fmaf(sx, g.w, -(sy * g.z)) * inv;
-        a = __builtin_fmaf(d[0], sy, -(d[1] * sx)) * inv;
+        const float4 g = r.g[0];                            // a, a - b
+        const float m00 = d[0], m10 = d[1], m01 = g.z, m11 = g.w;
+        const float det = m00 * m11 - m10 * m01;
+        const float invdet = 1.0f / det;
+        const float i00 = m11 * invdet, i10 = -m10 * invdet, i01 = -m01 * invdet, i11 = m00 * invdet;
+        const float r0 = g.x - o[0], r1 = g.y - o[1];
+        t = i00 * r0 + i01 * r1;
+        a = i10 * r0 + i11 * r1;
         b = 0.f;
         return a >= 0.0f && a <= 1.0f && fabsf(t) > eps;   // ordered comparisons reject NaN
     } else {
-        const float4 g0 = r.g[0], g1 = r.g[1], g2 = r.g[2];
-        // N = e1 x e2, rounded as the scene build used to store it (separate products)
-        const float n0 = g1.x * g2.x - g1.y * g1.w, n1 = g1.y * g1.z - g0.w * g2.x, n2 = g0.w * g1.w - g1.x * g1.z;
-        const float s0 = g0.x - o[0], s1 = g0.y - o[1], s2 = g0.z - o[2];
-        // q = d x s
-        const float q0 = __builtin_fmaf(d[1], s2, -(d[2] * s1));
-        const float q1 = __builtin_fmaf(d[2], s0, -(d[0] * s2));
-        const float q2 = __builtin_fmaf(d[0], s1, -(d[1] * s0));
-        const float det = __builtin_fmaf(d[0], n0, __builtin_fmaf(d[1], n1, d[2] * n2));         // d . (e1 x e2)
-        const float inv = __builtin_amdgcn_rcpf(det);                                            // hardware reciprocal (1 ulp)
-        t = __builtin_fmaf(s0, n0, __builtin_fmaf(s1, n1, s2 * n2)) * inv;                      // s . N / det
-        a = __builtin_fmaf(q0, g1.z, __builtin_fmaf(q1, g1.w, q2 * g2.x)) * inv;                // (d x s) . e2 / det
-        b = -(__builtin_fmaf(q0, g0.w, __builtin_fmaf(q1, g1.x, q2 * g1.y))) * inv;            // -(d x s) . e1 / det
+        const float4 g0 = r.g[0], g1 = r.g[1], g2 = r.g[2];   // a, a - b, a - c
+        // m = [d, a - b, a - c] by rows
+        const float m[3][3] = {{d[0], g0.w, g1.z}, {d[1], g1.x, g1.w}, {d[2], g1.y, g2.x}};
+        // cofactor (i, j) with cyclic indices, as a fixed-size 3x3 inverse forms it
+        auto cof = [&](int i, int j) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            return m[i1][j1] * m[i2][j2] - m[i1][j2] * m[i2][j1];
+        };
+        const float c00 = cof(0, 0), c10 = cof(1, 0), c20 = cof(2, 0);
+        const float det = c00 * m[0][0] + c10 * m[1][0] + c20 * m[2][0];
+        const float invdet = 1.0f / det;
+        const float r0 = g0.x - o[0], r1 = g0.y - o[1], r2 = g0.z - o[2];
+        float tab[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            // row i of the inverse is column i of the adjugate: cof(j, i) / det
+            const float i0 = (i == 0 ? c00 : cof(0, i)) * invdet;
+            const float i1 = (i == 0 ? c10 : cof(1, i)) * invdet;
+            const float i2 = (i == 0 ? c20 : cof(2, i)) * invdet;
+            tab[i] = i0 * r0 + i1 * r1 + i2 * r2;
+        }
+        t = tab[0]; a = tab[1]; b = tab[2];
         // the reference's !(a < 0 || b < 0 || a + b > 1 || |t| <= eps || isnan(t, a, b)): every
         // comparison with a NaN is false, so the ordered form below rejects NaN too
         return a >= 0.0f && b >= 0.0f && a + b <= 1.0f && fabsf(t) > eps;
@@ -890,20 +907,25 @@ __device__ __forceinline__ LineHit<D> line_query_ws(const DevScene &S, const flo
     return h;
 }
 
-// scene.cuh:471-478 (2D), 769-782 (3D): a uniformly oriented line direction
+// scene.cuh:471-478 (2D), 769-782 (3D): a uniformly oriented line direction, evaluated as the
+// reference's CPU build evaluates it: the FP32 angle times M_PI in double, cos and sin in
+// double (wobtrig::sincos_small), each rounded once to FP32.  Over all 2^24 angles a uniform
+// can give, those floats are the C library's (checked exhaustively on the host), so the
+// direction is the CPU path's to the bit.
+constexpr double kPiD = 3.14159265358979323846;
 template <int D> __device__ __forceinline__ void line_dir(Rng &rng, float *d) {
     if constexpr (D == 2) {
         const float theta = rng.uniform();
-        float s, c;
-        sincospif(theta, &s, &c);
-        d[0] = c; d[1] = s;
+        double s, c;
+        wobtrig::sincos_small((double)theta * kPiD, s, c);
+        d[0] = (float)c; d[1] = (float)s;
     } else {
         d[2] = rng.uniform();
         const float phi = 2.f * rng.uniform();
-        float s, c;
-        sincospif(phi, &s, &c);
+        double s, c;
+        wobtrig::sincos_small((double)phi * kPiD, s, c);
         const float r = sqrtf(1.f - d[2] * d[2]);
-        d[0] = c * r; d[1] = s * r;
+        d[0] = (float)c * r; d[1] = (float)s * r;
     }
 }
 
@@ -929,10 +951,19 @@ __device__ __forceinline__ Sample<D> line_finish(const DevScene &S, const float 
         r.y.alpha = alv[0] + interp * (alv[1] - alv[0]);
         r.y.bw[0] = 1.0f - interp; r.y.bw[1] = interp;
     } else {
-        const float w0 = 1.0f - (h.a + h.b);
-        r.y.bv = w0 * bvv[0] + h.a * bvv[1] + h.b * bvv[2];
-        r.y.alpha = w0 * alv[0] + h.a * alv[1] + h.b * alv[2];
-        r.y.bw[0] = w0; r.y.bw[1] = h.a; r.y.bw[2] = h.b;
+        // barycentric coordinates of the hit point (scene.cuh:45-69, 808-810)
+        float v0[3], v1[3], v2[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { v0[k] = v[3 + k] - v[k]; v1[k] = v[6 + k] - v[k]; v2[k] = r.y.p[k] - v[k]; }
+        const float d00 = dotv<3>(v0, v0), d01 = dotv<3>(v0, v1), d11 = dotv<3>(v1, v1);
+        const float d20 = dotv<3>(v2, v0), d21 = dotv<3>(v2, v1);
+        const float denom = d00 * d11 - d01 * d01;
+        const float l1 = (d11 * d20 - d01 * d21) / denom;
+        const float l2 = (d00 * d21 - d01 * d20) / denom;
+        const float l0 = 1.0f - (l1 + l2);
+        r.y.bv = l0 * bvv[0] + l1 * bvv[1] + l2 * bvv[2];
+        r.y.alpha = l0 * alv[0] + l1 * alv[1] + l2 * alv[2];
+        r.y.bw[0] = l0; r.y.bw[1] = l1; r.y.bw[2] = l2;
     }
     constexpr float eps = 10 * kFltEps;
     float rv[D];
@@ -1024,16 +1055,17 @@ __device__ __forceinline__ VolSample<D> sample_gauss(const DevScene &S, Rng &rng
     while (j + 1 < v.n && u > v.cdf[j]) ++j;
     const float u1 = rng.uniform(), u2 = rng.uniform();
     const float r1 = sqrtf(-2.f * logf(u1));
-    float sn, cs;
-    sincospif(2.f * u2, &sn, &cs);
+    // the angles' cos and sin in double, rounded once, as the oracle evaluates them
+    double sn, cs;
+    wobtrig::sincos_small(2.0 * kPiD * (double)u2, sn, cs);
     float z[3];
-    z[0] = r1 * cs;
-    z[1] = r1 * sn;
+    z[0] = r1 * (float)cs;
+    z[1] = r1 * (float)sn;
     if constexpr (D == 3) {
         const float u3 = rng.uniform(), u4 = rng.uniform();
-        float s2, c2;
-        sincospif(2.f * u4, &s2, &c2);
-        z[2] = sqrtf(-2.f * logf(u3)) * c2;
+        double s2, c2;
+        wobtrig::sincos_small(2.0 * kPiD * (double)u4, s2, c2);
+        z[2] = sqrtf(-2.f * logf(u3)) * (float)c2;
     }
     const float4 cj = v.comp[2 * j];
     const float cjv[3] = {cj.x, cj.y, cj.z};

@@ -157,6 +157,32 @@ __global__ __launch_bounds__(256) void intersect_kernel(DevScene S, uint32_t n, 
     counts[i] = c;
 }
 
+// One line-intersection sample per domain point, with the walk's own sampler (line_sample:
+// the work-sharing all-hit query, the uniform pick and the hit's attributes) on the Philox
+// stream (seed, path, point i, stream).  Lanes past n leave before the query; line_query_ws
+// shares work only among the lanes active at its call.
+template <int D, int STACK>
+__global__ __launch_bounds__(256) void line_sample_kernel(DevScene S, uint32_t n, const float *points, uint64_t seed,
+                                                          uint32_t path, uint32_t stream, float *rec, float *inv_pdf) {
+    extern __shared__ uint32_t lds_stack[];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) x[k] = points[(size_t)i * D + k];
+    Rng rng;
+    rng.init(seed, path, i, stream);
+    const Sample<D> z = line_sample<D, STACK>(S, x, 0xFFFFFFFFu, rng, lds_stack + threadIdx.x, blockDim.x);
+    float *r = rec + (size_t)i * (2 * D + 4);
+#pragma unroll
+    for (int k = 0; k < D; ++k) { r[k] = z.y.p[k]; r[D + k] = z.y.n[k]; }
+    r[2 * D] = 0.5f;
+    r[2 * D + 1] = z.y.bv;
+    r[2 * D + 2] = (float)z.y.type;
+    r[2 * D + 3] = z.y.alpha;
+    inv_pdf[i] = z.inv_pdf;
+}
+
 // per_point == 0: record i uses the Philox stream (seed, path i, point 0, stream);
 // per_point == 1: (seed, path `path`, point i, stream) -- one stream per VPL of a pass.
 template <int D>
@@ -270,6 +296,19 @@ hipError_t launch_intersect(int dim, const DevScene &S, uint32_t n, const float 
         hipLaunchKernelGGL((intersect_kernel<2, kStackEntries>), dim3(grid), dim3(block), lds, st, S, n, o, d, max_hits, counts, ids, ts);
     else
         hipLaunchKernelGGL((intersect_kernel<3, kStackEntries>), dim3(grid), dim3(block), lds, st, S, n, o, d, max_hits, counts, ids, ts);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample_lines(int dim, const DevScene &S, uint32_t n, const float *points, uint64_t seed,
+                               uint32_t path, uint32_t stream, float *rec, float *inv_pdf, hipStream_t st) {
+    const uint32_t block = 128, grid = (n + block - 1) / block;
+    const size_t lds = (size_t)line_query_lds_words<kStackEntries>() * block * sizeof(uint32_t);
+    if (dim == 2)
+        hipLaunchKernelGGL((line_sample_kernel<2, kStackEntries>), dim3(grid), dim3(block), lds, st, S, n, points, seed,
+                           path, stream, rec, inv_pdf);
+    else
+        hipLaunchKernelGGL((line_sample_kernel<3, kStackEntries>), dim3(grid), dim3(block), lds, st, S, n, points, seed,
+                           path, stream, rec, inv_pdf);
     return hipGetLastError();
 }
 

@@ -67,6 +67,8 @@ hipError_t launch_eval_weights(int dim, int bie, float dsign, float eps, float f
                                hipStream_t st);
 hipError_t launch_intersect(int dim, const wobdev::DevScene &S, uint32_t n, const float *o, const float *d,
                             uint32_t max_hits, uint32_t *counts, uint32_t *ids, float *ts, hipStream_t st);
+hipError_t launch_sample_lines(int dim, const wobdev::DevScene &S, uint32_t n, const float *points, uint64_t seed,
+                               uint32_t path, uint32_t stream, float *rec, float *inv_pdf, hipStream_t st);
 // path < 0: record i uses the stream (seed, path i, point 0, stream); path >= 0: (seed, path,
 // point i, stream)
 hipError_t launch_sample_boundary(int dim, const wobdev::DevScene &S, uint32_t n, uint64_t seed, uint32_t stream,

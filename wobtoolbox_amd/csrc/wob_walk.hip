@@ -138,7 +138,7 @@ __device__ __forceinline__ void walk_item(const WalkLaunch &L, uint32_t item, ui
 // take the compiler's choice.
 template <int D, int BIE, int EST, int FL, bool BOUNDARY, bool VOL> constexpr int walk_min_waves() {
     if (D == 2 && BIE == kSL && EST == kBackward && FL == FL_UNIFORM && !BOUNDARY && !VOL) return 4;
-    if (D == 3 && BIE == kDirect && FL == FL_LINE_BV && BOUNDARY && !VOL) return 3;
+    if (D == 3 && BIE == kDirect && FL == FL_LINE_BV && !VOL) return 3;
     return 1;
 }
 
