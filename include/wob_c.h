@@ -209,6 +209,24 @@ wob_status wob_estimate_domain(const wob_scene *scene, const wob_estimator_desc 
                                uint64_t path_base, uint32_t stream_id, double *d_sum, double *d_sumsq,
                                wob_launch_config *launch, void *stream);
 
+/* No counterpart in the reference.  wob_estimate_domain for callers that cannot allocate
+ * device memory: code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  Same arguments, kernels and results (bit for bit) as wob_estimate_domain; the
+ * caller still passes the points and the output sums.  The launch's work counter and
+ * per-work-item partial sums live in a static device array of WOB_NOALLOC_SCRATCH_BYTES
+ * per GPU, taken when the library first runs this entry point on that GPU and held until the
+ * process ends; the call itself allocates nothing.  A launch needing more than the array
+ * holds (256 bytes + 8 * (dim + 1) bytes per work item; about 1M items at the automatic
+ * paths_per_lane) returns WOB_INVALID_ARG.  launch->flags may not ask for
+ * WOB_LAUNCH_TIME_WALK (timing creates events).  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (several streams or threads) race and must be
+ * serialised by the caller.                                                                 */
+#define WOB_NOALLOC_SCRATCH_BYTES (64u << 20)
+wob_status wob_estimate_domain_noalloc(const wob_scene *scene, const wob_estimator_desc *est, const float *d_points,
+                                       uint32_t n_points, uint32_t point_base, uint32_t paths_per_point, uint64_t seed,
+                                       uint64_t path_base, uint32_t stream_id, double *d_sum, double *d_sumsq,
+                                       wob_launch_config *launch, void *stream);
+
 /* On-boundary estimates: compute_sample_path_contribution_boundary (estimator.cuh:193-222,
  * forward :456-496), as used by apps/boundary_estimate.cu:65-98.
  *   d_records  n_points * (2*dim + 4) floats (BoundaryPoint records, device)

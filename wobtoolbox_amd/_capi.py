@@ -17,7 +17,7 @@ STATUS_NAMES = {0: "WOB_OK", 1: "WOB_INVALID_ARG", 2: "WOB_INVALID_ENUM", 3: "WO
 # every symbol declared in include/wob_c.h
 EXPORTS = ["wob_scene_create", "wob_scene_create_ex", "wob_scene_destroy", "wob_scene_get_info", "wob_scene_set_volume_grid",
            "wob_scene_set_volume_gaussians",
-           "wob_estimate_domain", "wob_estimate_boundary", "wob_intersect_lines", "wob_eval_kernels",
+           "wob_estimate_domain", "wob_estimate_domain_noalloc", "wob_estimate_boundary", "wob_intersect_lines", "wob_eval_kernels",
            "wob_eval_weights", "wob_philox4x32_10", "wob_string_to_enum", "wob_enum_to_string",
            "wob_last_error_message", "wob_version", "wob_batch_create", "wob_batch_run", "wob_batch_tune",
            "wob_batch_destroy", "wob_cv_create", "wob_cv_destroy", "wob_cv_get_info", "wob_cv_get_params",
@@ -106,6 +106,8 @@ def lib():
                                  C.POINTER(LaunchConfig), P], C.c_int),
         "wob_estimate_boundary": ([P, C.POINTER(EstimatorDesc), P, U32, U32, U32, U64, U64, U32, P, P,
                                    C.POINTER(LaunchConfig), P], C.c_int),
+        "wob_estimate_domain_noalloc": ([P, C.POINTER(EstimatorDesc), P, U32, U32, U32, U64, U64, U32, P, P,
+                                         C.POINTER(LaunchConfig), P], C.c_int),
         "wob_intersect_lines": ([P, P, P, U32, U32, P, P, P, P], C.c_int),
         "wob_eval_kernels": ([U32, U32, P, P, P, P, F, P, P], C.c_int),
         "wob_eval_weights": ([U32, I32, I32, F, F, U32, P, P, P, P, P, P, P], C.c_int),

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -240,9 +241,19 @@ wob_status check_paths(uint64_t path_base, uint64_t paths) {
     return WOB_OK;
 }
 
+// Device memory a walk launch works in: the work counter of resident launches and the
+// per-work-item partial sums (2 V floats per item).  run_walk takes them from the scene
+// (allocated on first need) unless the caller passes its own.
+struct WalkScratch {
+    uint32_t *counter;
+    float *partials;
+    size_t partial_floats;
+};
+
 wob_status run_walk(const wob_scene *cscene, const wob_estimator_desc *est, const float *d_points, uint32_t n_points,
                     uint32_t point_base, uint32_t paths, uint64_t seed, uint64_t path_base, uint32_t stream_id,
-                    double *d_sum, double *d_sumsq, wob_launch_config *launch, void *stream, bool boundary) {
+                    double *d_sum, double *d_sumsq, wob_launch_config *launch, void *stream, bool boundary,
+                    const WalkScratch *ws = nullptr) {
     // the path range first: it needs no scene, so the rule holds (and is testable) on its own
     wob_status st = check_paths(path_base, paths);
     if (st != WOB_OK) return st;
@@ -263,8 +274,18 @@ wob_status run_walk(const wob_scene *cscene, const wob_estimator_desc *est, cons
     L.e0 = make_entry(scene, est, d_points, n_points, paths, ppl, point_base, stream_id, path_base, d_sum, d_sumsq);
     const uint64_t n_items = (uint64_t)n_points * L.e0.chunks;
     if (n_items >= (1ull << 31)) return fail(WOB_INVALID_ARG, "too many work items; split the point set");
-    st = ensure_scratch(scene, (size_t)2 * V * n_items * sizeof(float));
-    if (st != WOB_OK) return st;
+    const size_t partial_floats = (size_t)2 * V * n_items;
+    if (ws) {
+        if (partial_floats > ws->partial_floats)
+            return fail(WOB_INVALID_ARG, "the launch needs " + std::to_string(partial_floats * sizeof(float)) +
+                                             " bytes of partial sums; the allocation-free scratch holds " +
+                                             std::to_string(ws->partial_floats * sizeof(float)) +
+                                             " (split the point set or raise paths_per_lane)");
+        if (launch && (launch->flags & WOB_LAUNCH_TIME_WALK))
+            return fail(WOB_INVALID_ARG, "WOB_LAUNCH_TIME_WALK creates events; the allocation-free entry point does not time");
+    } else if ((st = ensure_scratch(scene, partial_floats * sizeof(float))) != WOB_OK) {
+        return st;
+    }
     L.table = nullptr;
     L.n_entries = 1;
     L.total_items = (uint32_t)n_items;
@@ -273,7 +294,7 @@ wob_status run_walk(const wob_scene *cscene, const wob_estimator_desc *est, cons
     L.work_counter = nullptr;
     L.pass_counter = nullptr;
     L.seed = seed;
-    L.partials = scene->scratch;
+    L.partials = ws ? ws->partials : scene->scratch;
     const bool volume = est->num_volume_samples > 0;
     bool wavefront = false;
     st = resolve_schedule(launch ? launch->schedule : 0u,
@@ -286,13 +307,15 @@ wob_status run_walk(const wob_scene *cscene, const wob_estimator_desc *est, cons
     // (the wavefront walk): no partly filled last round of waves
     const bool resident = wavefront || !(launch && (launch->flags & WOB_LAUNCH_ONE_LANE_PER_ITEM));
     if (resident) {
-        if (!scene->counter) {
+        uint32_t *counter = ws ? ws->counter : scene->counter;
+        if (!counter) {
             void *p = nullptr;
             if (hipMalloc(&p, 256) != hipSuccess) return fail(WOB_OOM, "hipMalloc work counter");
             scene->counter = (uint32_t *)p;
+            counter = scene->counter;
         }
-        WOB_HIP(wobl::launch_fill_u32(scene->counter, 1, 0u, hs), "counter reset");
-        L.work_counter = scene->counter;
+        WOB_HIP(wobl::launch_fill_u32(counter, 1, 0u, hs), "counter reset");
+        L.work_counter = counter;
         L.persistent = 1;
         L.wavefront = wavefront ? 1u : 0u;
         L.regen_threshold = launch && launch->regen_threshold ? launch->regen_threshold : kDefaultRegen;
@@ -655,6 +678,52 @@ wob_status wob_estimate_domain(const wob_scene *scene, const wob_estimator_desc 
                                wob_launch_config *launch, void *stream) {
     return run_walk(scene, est, d_points, n_points, point_base, paths_per_point, seed, path_base, stream_id, d_sum,
                     d_sumsq, launch, stream, false);
+}
+
+// The allocation-free entry point's device memory: one file-scope array per GPU, held until
+// the process ends.  The first 256 bytes are the work counter, the rest the partial sums.
+// In host code `scratch` names a host-side stand-in, not device memory: its device address
+// comes from hipGetSymbolAddress, once per device (with that device current), and only that
+// address is ever passed on.
+constexpr size_t kNoAllocScratchBytes = (size_t)WOB_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+wob_status noalloc_scratch(WalkScratch &ws) {
+    int dev = 0;
+    WOB_HIP(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(WOB_DEVICE, "device index out of range for the static scratch");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return fail(WOB_DEVICE, "hipGetSymbolAddress(scratch) returned no address");
+        g_scratch_ptr[dev] = p;
+    }
+    unsigned char *base = (unsigned char *)g_scratch_ptr[dev];
+    ws.counter = (uint32_t *)base;
+    ws.partials = (float *)(base + 256);
+    ws.partial_floats = (kNoAllocScratchBytes - 256) / sizeof(float);
+    return WOB_OK;
+}
+}  // namespace
+
+wob_status wob_estimate_domain_noalloc(const wob_scene *scene, const wob_estimator_desc *est, const float *d_points,
+                                       uint32_t n_points, uint32_t point_base, uint32_t paths_per_point, uint64_t seed,
+                                       uint64_t path_base, uint32_t stream_id, double *d_sum, double *d_sumsq,
+                                       wob_launch_config *launch, void *stream) {
+    wob_status st = check_paths(path_base, paths_per_point);
+    if (st != WOB_OK) return st;
+    if (!scene) return fail(WOB_INVALID_ARG, "null scene");
+    WalkScratch ws{};
+    if ((st = noalloc_scratch(ws)) != WOB_OK) return st;
+    return run_walk(scene, est, d_points, n_points, point_base, paths_per_point, seed, path_base, stream_id, d_sum,
+                    d_sumsq, launch, stream, false, &ws);
 }
 
 wob_status wob_estimate_boundary(const wob_scene *scene, const wob_estimator_desc *est, const float *d_records,
