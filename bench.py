@@ -1,18 +1,24 @@
 #!/usr/bin/env python3
-"""Benchmark: walk-on-boundary paths/s on one MI355X.
+"""Benchmark: walk-on-boundary paths/s and time-to-RMSE on one MI355X.
 
-Default workload (icosphere_neumann_3d): level-5 icosphere (20,480 triangles) radially
-displaced to r = 1 + 0.2 sin(3 theta) cos(2 phi), Neumann data of u = x^2 - y^2 + 0.5 z,
-direct BIE, interior, backward estimator, path length 6, 100,000 interior points x 256
-paths per step (25.6 M paths).  A step is one wob_estimate_domain call over all points
-(walk kernel + FP64 per-point reduction), inputs resident in HBM, fresh Philox streams per
-step (path_base advances).  Prints ONE JSON line.
+Headline workload (geodesic_mixed_3d): the learned-control-variate configuration.  A
+frequency-100 geodesic sphere (200,000 triangles) radially displaced to
+r = 1 + 0.2 sin(3 theta) cos(2 phi), Dirichlet where the face centroid has z > 0 and Neumann
+elsewhere, data from u = exp(0.6 x) cos(0.6 y) + x z; single-layer BIE, interior, backward
+estimator, RIS with 64 candidates, first-kind constant 2, path length 4
+(data/config_general_3d_interior_Mixed.json); 1,000,000 interior points.  A step is one
+wob_estimate_domain call of --paths paths at every point (walk kernel + FP64 per-point
+reduction), inputs resident in HBM, fresh Philox streams per step (path_base advances).
+`value` is plain WoB paths/s.  Prints ONE JSON line.
 
 Also reported: the walk kernel's roofline position from its own HIP-event duration (the
-library brackets the kernel on its launch stream) and the algorithmic work model in
-wobtoolbox_amd/roofline.py, and a CPU baseline: the OpenMP restatement of the reference
-(oracle/, reference hit-pick and per-point mt19937 streams) timed on a bounded sample of
-the same workload on this host's cores.
+library brackets the kernel on its launch stream) and the work model of
+wobtoolbox_amd/roofline.py; a CPU baseline (the OpenMP restatement of the reference in
+oracle/, reference hit pick and per-point mt19937 streams) timed on a bounded sample of the
+same workload on this host's cores; the time to a common RMSE of plain WoB and of WoB with
+the learned control variate (training data and training on the control variate's clock);
+and legs for the other configurations (level-5 icosphere Neumann, the 64-scene exterior
+batch, the Poisson problem, the virtual-point-light app, scene construction).
 """
 import argparse
 import json
@@ -28,14 +34,14 @@ sys.path.insert(0, ROOT)
 METRIC = "WoB paths/s and time-to-RMSE 1e-3 (learned CV vs plain WoB), 1 MI355X"
 
 
-def cpu_baseline(workload, cores, target_s=10.0):
-    """Reference-mode OpenMP oracle on a deterministic subset (every k-th point, full paths).
+def cpu_baseline(workload, cores, paths, target_s=10.0):
+    """Reference-mode OpenMP oracle on a deterministic subset (every k-th point, `paths` paths
+    per point as in a GPU step).
 
     k is chosen from a short calibration run so that the timed sample takes about target_s."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     sc = oracle_lib.OracleScene(workload.elements)
-    paths = workload.solver["num_sample_paths"]
     n_all = len(workload.points)
     cal = np.arange(0, n_all, max(1, n_all // max(cores * 4, 64)), dtype=np.uint32)[: max(cores * 4, 64)]
     t0 = time.perf_counter()
@@ -70,21 +76,76 @@ def mlp_train_roofline(steps, batch, n_outputs, seconds):
             "flop_per_sample_step": 3 * fwd}
 
 
-def time_to_rmse_leg(args, torch, wob, G):
-    """Plain WoB vs learned control variate on the 200k-triangle geodesic mixed workload.
+def walk_timing(est, pts, paths, steps, warmup, step0=0):
+    """Warm-up and timed steps of one domain estimate of `paths` paths at every point, then the
+    walk kernel's own duration (HIP events on its launch stream) over a few more steps.
+    Returns (paths/s, ms per step, mean walk-kernel ms, sums, sumsq, paths per point done)."""
+    import torch
+    n = pts.shape[0]
+    V = est.scene.dim + 1
+    s = torch.zeros((n, V), dtype=torch.float64, device=pts.device)
+    q = torch.zeros_like(s)
+    step = step0
+    for _ in range(warmup):
+        est.compute_domain(pts, paths, seed=0, path_base=step * paths, out=(s, q))
+        step += 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        est.compute_domain(pts, paths, seed=0, path_base=step * paths, out=(s, q))
+        step += 1
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    est.time_walk = True
+    walk_ms = []
+    for _ in range(max(3, min(steps, 10))):
+        est.compute_domain(pts, paths, seed=0, path_base=step * paths, out=(s, q))
+        walk_ms.append(est.last_launch.walk_ms)
+        step += 1
+    est.time_walk = False
+    return steps * n * paths / elapsed, elapsed / steps * 1e3, float(np.mean(walk_ms)), s, q, (step - step0) * paths
 
-    Both arms run progressive passes for the same walk-time budget; the control-variate arm's
-    clock starts before its training data are generated and includes training."""
+
+def roofline_block(name, n_elements, path_length, launch_paths, walk_ms, candidates=64):
+    """The walk kernel against MI355X's peaks, priced with the work model of roofline.py.
+    `bound` is what rocprof's counters show limits it: "hbm" (record bytes against the 8 TB/s
+    HBM peak) or the compute roof, "mfma" (model FP32 operations against the 157.3 TFLOP/s
+    FP32 peak; the walk has no matrix instructions, it runs on the vector ALUs).  The other roof
+    and the counter-measured HBM bytes are given beside it."""
+    from wobtoolbox_amd.roofline import BOUND, MEASURED_TRAFFIC, MI355X_PEAKS, workload_model
+    flop, byts = workload_model(name, n_elements, path_length, candidates)
+    s = walk_ms * 1e-3
+    tflops = flop * launch_paths / s / 1e12
+    gbps = byts * launch_paths / s / 1e9
+    traffic = MEASURED_TRAFFIC.get(name)
+    mem = {"achieved": gbps, "peak": MI355X_PEAKS["hbm_GBps"], "unit": "GB/s", "frac": gbps / MI355X_PEAKS["hbm_GBps"]}
+    comp = {"achieved": tflops, "peak": MI355X_PEAKS["fp32_TFLOPs"], "unit": "TFLOP/s",
+            "frac": tflops / MI355X_PEAKS["fp32_TFLOPs"], "unit_note": "FP32 vector ALU"}
+    bound = BOUND[name]
+    main, other = (mem, comp) if bound == "hbm" else (comp, mem)
+    return {"bound": bound, **{k: main[k] for k in ("achieved", "peak", "unit", "frac")}, "traffic": traffic,
+            "kernel": "walk_kernel", "avg_kernel_ms": walk_ms,
+            "algorithmic_flop_per_path": flop, "algorithmic_bytes_per_path": byts,
+            "fp32_valu" if bound == "hbm" else "model_bytes": other,
+            "measured_hbm": None if traffic is None else
+            {"bytes_per_launch": traffic, "achieved": traffic / s / 1e9, "unit": "GB/s",
+             "frac": traffic / s / 1e9 / MI355X_PEAKS["hbm_GBps"]}}
+
+
+def time_to_rmse_leg(args, torch, wob, G, w, scene, pts):
+    """Plain WoB vs the learned control variate on the headline workload, both progressive.
+
+    Each arm runs passes for the same walk-time budget; the control-variate arm's clock starts
+    before its training data are generated and includes training.  The target is the larger
+    of the two arms' final RMSE, which both reach inside the budget; the times are the first
+    checkpoints (pass counts growing by 1.25) at or below it.  RMSE is against the exact
+    solution, with the rms standard error beside it so that a bias would show."""
     from wobtoolbox_amd import cv as CV
     from wobtoolbox_amd import progressive
-    print("[bench] time-to-RMSE: building geodesic_mixed_3d", file=sys.stderr, flush=True)
-    w = G.geodesic_mixed_3d(n_points=args.cv_points)
-    dev = torch.device("cuda:0")
-    scene = wob.SceneHost(w.elements)
-    pts = torch.as_tensor(w.points).to(dev)
-    exact_u = torch.as_tensor(w.exact(w.points)[:, 0]).to(dev)
+    exact_u = torch.as_tensor(w.exact(w.points)[:, 0]).to(pts.device)
     plain = wob.Estimator.from_config(scene, w.solver)
     L = w.solver["path_length"]
+    n = pts.shape[0]
     print("[bench] time-to-RMSE: plain arm", file=sys.stderr, flush=True)
     curve_plain = progressive.run(plain, pts, exact_u, args.cv_paths_per_pass, args.cv_budget, seed=1)
     print("[bench] time-to-RMSE: control-variate arm", file=sys.stderr, flush=True)
@@ -102,27 +163,64 @@ def time_to_rmse_leg(args, torch, wob, G):
     t_setup = time.perf_counter() - t0
     cvest = wob.Estimator.from_config(scene, w.solver, flags=2, cv_samples=args.cv_samples)
     curve_cv = progressive.run(cvest, pts, exact_u, args.cv_paths_per_pass, t_setup + args.cv_budget, t0=t0, seed=1)
-    target = 1e-3
-    tp, reached_p = progressive.time_to_rmse(curve_plain, target)
-    tc, reached_c = progressive.time_to_rmse(curve_cv, target, t_setup)
-    # variance efficiency: rmse^2 x walk seconds at the last checkpoint
+    target = max(curve_plain[-1][2], curve_cv[-1][2])
+    tp = progressive.time_to_rmse(curve_plain, target)
+    tc = progressive.time_to_rmse(curve_cv, target)
+    # walk throughput of each arm over its last checkpoint interval
+    def rate(curve, t_off=0.0):
+        t, k = curve[-1][0] - t_off, curve[-1][1]
+        return n * k / t
+    # variance x time per arm: RMSE^2 x walk seconds; at the path counts reached the RMSE is the
+    # noise (RMSE ~ mean SE), so the time to 1e-3 is an extrapolation of this product
     eff_p = curve_plain[-1][2] ** 2 * curve_plain[-1][0]
     eff_c = curve_cv[-1][2] ** 2 * (curve_cv[-1][0] - t_setup)
-    return {"target_rmse": target, "workload": w.name, "elements": w.elements.n, "points": len(w.points),
+    tp_last, kp, rp, sep = curve_plain[-1]
+    # truncation bias of the path-length series: RMSE^2 = bias^2 + noise^2 over the points
+    bias2 = rp * rp - sep * sep
+    bias_bound = float(np.sqrt(max(bias2, 0.0) + 2.0 * 1.4 * sep * sep / np.sqrt(n)))
+    return {"workload": w.name, "elements": w.elements.n, "points": n,
             "solver": {k: w.solver[k] for k in ("bie_model", "problem", "path_length", "num_resampling_candidates",
                                                  "first_kind_equation_scaling_constant")},
-            "plain_s": tp, "plain_reached": reached_p, "cv_s": tc, "cv_reached": reached_c, "speedup": tp / tc,
-            "cv_setup_s": t_setup, "cv_training_walks": args.cv_train_points * L * args.cv_train_paths,
+            "target_rmse": target, "target_rule": "the larger final RMSE of the two arms (both reach it)",
+            "plain_s": tp, "cv_s": tc, "speedup": None if (tp is None or tc is None) else tp / tc,
+            "cv_setup_s": t_setup, "plain_paths_per_s": rate(curve_plain), "cv_paths_per_s": rate(curve_cv, t_setup),
+            "cv_training_walks": args.cv_train_points * L * args.cv_train_paths,
             "cv_config": {"train_points_per_value": args.cv_train_points, "paths_per_target": args.cv_train_paths,
                           "train_steps": args.cv_train_steps, "batch": 16384, "sigma": args.cv_sigma,
                           "cv_samples": args.cv_samples},
             "mlp_train": mlp_train_roofline(args.cv_train_steps, 16384, L, t_train),
-            "efficiency_ratio": eff_p / eff_c,
-            "note": "not reached: extrapolated as setup + rmse^2 x walk time / target^2 (both arms estimate the "
-                    "same expectation; truncation bias of the path_length-4 series is not resolved at these "
-                    "RMSE levels)",
-            "plain_curve": [[round(t, 3), k, r] for t, k, r in curve_plain],
-            "cv_curve": [[round(t, 3), k, r] for t, k, r in curve_cv]}
+            "variance_time_ratio": eff_p / eff_c,
+            "rmse_1e-3": {"reached": bool(min(rp, curve_cv[-1][2]) <= 1e-3),
+                          "extrapolated_plain_s": eff_p / 1e-6, "extrapolated_cv_s": t_setup + eff_c / 1e-6,
+                          "truncation_bias_rms_bound": bias_bound,
+                          "note": "not reached by either arm at these budgets: the estimator's per-path standard "
+                                  "deviation is about 200 x |u|; times are RMSE^2 x walk time / 1e-6 (noise only), and "
+                                  "the path-length-4 truncation bias is bounded only to the figure given"},
+            "plain_curve": [[round(t, 3), k, r, se] for t, k, r, se in curve_plain],
+            "cv_curve": [[round(t, 3), k, r, se] for t, k, r, se in curve_cv]}
+
+
+def icosphere_leg(args, torch, wob, G):
+    """Level-5 icosphere (20,480 triangles), displaced; Neumann data of x^2 - y^2 + 0.5 z; direct
+    BIE, path length 6 (line-intersection and boundary-value sampling); 100,000 points x 256
+    paths per step."""
+    print("[bench] icosphere leg", file=sys.stderr, flush=True)
+    w = G.icosphere_neumann_3d()
+    paths = w.solver["num_sample_paths"]
+    scene = wob.SceneHost(w.elements)
+    est = wob.Estimator.from_config(scene, w.solver)
+    pts = torch.as_tensor(w.points).to("cuda:0")
+    value, ms, walk_ms, s, q, k = walk_timing(est, pts, paths, args.leg_steps, 2)
+    m = (s[:, 0] / k).cpu().numpy()
+    var = np.maximum((q[:, 0] / k).cpu().numpy() - m * m, 0)
+    d = m - w.exact(w.points)[:, 0]
+    d -= d.mean()
+    return {"workload": w.name, "elements": w.elements.n, "points": len(w.points), "paths_per_step": len(w.points) * paths,
+            "paths_per_s": value, "ms_per_step": ms,
+            "roofline": roofline_block(w.name, w.elements.n, w.solver["path_length"], len(w.points) * paths, walk_ms),
+            "accuracy": {"rmse_u": float(np.sqrt((d ** 2).mean())), "mean_se_u": float(np.sqrt(var / k).mean()),
+                         "paths_per_point": k, "note": "Neumann problem: u compared after removing the mean offset"},
+            "solver": "Direct, InteriorDomain, BackwardEstimator, NeumannProblem, path_length 6"}
 
 
 def batch_leg(args, torch, wob, G):
@@ -155,13 +253,24 @@ def batch_leg(args, torch, wob, G):
     for w, (s, q) in zip(ws, b.outputs):
         k = b.passes * paths
         m = (s[:, 0] / k).cpu().numpy()
+        var = np.maximum((q[:, 0] / k).cpu().numpy() - m * m, 0) / k
         err = m - w.exact(w.points.astype(np.float64))[:, 0]
-        per_kind.setdefault(w.solver["problem"], []).append(err)
-    rmse = {k: float(np.sqrt(np.mean(np.concatenate(v) ** 2))) for k, v in per_kind.items()}
+        e, v = per_kind.setdefault(w.solver["problem"], ([], []))
+        e.append(err)
+        v.append(var)
+    # RMSE against the exact solution beside the mean SE; RMSE^2 - mean SE^2 estimates the
+    # squared bias (the truncation of the path-length series), which the noise would hide
+    rmse, mean_se, bias = {}, {}, {}
+    for kind, (e, v) in per_kind.items():
+        e, v = np.concatenate(e), np.concatenate(v)
+        rmse[kind] = float(np.sqrt(np.mean(e ** 2)))
+        mean_se[kind] = float(np.mean(np.sqrt(v)))
+        bias[kind] = float(np.sqrt(max(np.mean(e ** 2) - np.mean(v), 0.0)))
     return {"workload": "star_exterior_batch_2d", "scenes": len(ws), "segments": int(sum(w.elements.n for w in ws)),
             "points": n_points, "paths_per_point_per_pass": paths, "passes": args.batch_passes,
             "paths_per_s": total / el, "ms_per_pass": el / args.batch_passes * 1e3, "tuned": best,
-            "tune_s": tune_s, "rmse_u_by_problem": rmse, "paths_per_point_total": b.passes * paths,
+            "tune_s": tune_s, "rmse_u_by_problem": rmse, "mean_se_u_by_problem": mean_se,
+            "bias_rms_estimate_by_problem": bias, "paths_per_point_total": b.passes * paths,
             "solver": "IndirectSingleLayer, ExteriorDomain, BackwardEstimator, RIS 64, eps 1e-5; path_length 7/4/6 "
                       "and fkesc 1/1/2 for Neumann/Robin/mixed scenes"}
 
@@ -296,10 +405,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="icosphere_neumann_3d", choices=["icosphere_neumann_3d", "circle_dirichlet_2d"])
+    ap.add_argument("--points", type=int, default=1_000_000, help="headline evaluation points")
+    ap.add_argument("--paths", type=int, default=16, help="headline paths per point per step")
+    ap.add_argument("--headline-only", action="store_true", help="skip every leg (profiling runs)")
+    ap.add_argument("--only", choices=["icosphere", "batch", "poisson", "vpl"],
+                    help="run only this leg and print its JSON object (profiling runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--paths-per-lane", type=int, default=0)
-    ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--leg-steps", type=int, default=10)
+    ap.add_argument("--no-icosphere", action="store_true")
     ap.add_argument("--no-time-to-rmse", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-paths", type=int, default=16, help="paths per point per batch pass")
@@ -310,7 +423,6 @@ def main():
     ap.add_argument("--no-vpl", action="store_true")
     ap.add_argument("--vpl-iterations", type=int, default=5)
     ap.add_argument("--no-scene-build", action="store_true")
-    ap.add_argument("--cv-points", type=int, default=1_000_000)
     ap.add_argument("--cv-budget", type=float, default=15.0, help="walk seconds per arm")
     ap.add_argument("--cv-paths-per-pass", type=int, default=4)
     ap.add_argument("--cv-train-points", type=int, default=1048576, help="boundary points per value function")
@@ -325,119 +437,83 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested). "
               "Points would shard across GPUs with no exchange step; only one GPU is measured.", file=sys.stderr)
         sys.exit(2)
+    if args.headline_only:
+        args.no_icosphere = args.no_time_to_rmse = args.no_batch = args.no_poisson = True
+        args.no_vpl = args.no_scene_build = args.no_cpu_baseline = True
 
     import torch
     import wobtoolbox_amd as wob
     from wobtoolbox_amd import geometry as G
-    from wobtoolbox_amd.roofline import MEASURED_TRAFFIC, MI355X_PEAKS, workload_model
 
-    print(f"[bench] building workload {args.workload}", file=sys.stderr, flush=True)
-    w = G.icosphere_neumann_3d() if args.workload == "icosphere_neumann_3d" else G.circle_dirichlet_2d()
-    paths = w.solver["num_sample_paths"]
+    if args.only:
+        leg = {"icosphere": icosphere_leg, "batch": batch_leg, "poisson": poisson_leg, "vpl": vpl_leg}[args.only]
+        out = leg(args, torch, wob, G)
+        if args.profile_copy_mib:
+            src = torch.ones(args.profile_copy_mib * (1 << 18), dtype=torch.float32, device="cuda:0")
+            torch.empty_like(src).copy_(src)
+            torch.cuda.synchronize()
+        print(json.dumps(out), flush=True)
+        return
+
+    print("[bench] building the headline workload geodesic_mixed_3d", file=sys.stderr, flush=True)
+    w = G.geodesic_mixed_3d(n_points=args.points)
     dev = torch.device("cuda:0")
     scene = wob.SceneHost(w.elements)
-    est = wob.Estimator.from_config(scene, w.solver, paths_per_lane=args.paths_per_lane, block_size=args.block_size)
+    est = wob.Estimator.from_config(scene, w.solver)
     pts = torch.as_tensor(w.points).to(dev)
     n = pts.shape[0]
-    s = torch.zeros((n, w.elements.dim + 1), dtype=torch.float64, device=dev)
-    q = torch.zeros_like(s)
-    stream = torch.cuda.current_stream()
-
-    step = 0
-    print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        est.compute_domain(pts, paths, seed=0, path_base=step * paths, out=(s, q))
-        step += 1
-    torch.cuda.synchronize()
-    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        est.compute_domain(pts, paths, seed=0, path_base=step * paths, out=(s, q))
-        step += 1
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    total_paths = args.steps * n * paths
-    value = total_paths / elapsed
-
+    paths = args.paths
+    print(f"[bench] headline: {args.warmup} warm-up and {args.steps} timed steps", file=sys.stderr, flush=True)
+    value, ms_per_step, walk_ms, s, q, k = walk_timing(est, pts, paths, args.steps, args.warmup)
     if args.profile_copy_mib:
         src = torch.ones(args.profile_copy_mib * (1 << 18), dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
         print(f"[bench] copied {args.profile_copy_mib} MiB", file=sys.stderr, flush=True)
+    m = (s[:, 0] / k).cpu().numpy()
+    var = np.maximum((q[:, 0] / k).cpu().numpy() - m * m, 0)
+    err = m - w.exact(w.points)[:, 0]
 
-    # walk-kernel duration per launch, HIP events on the launch stream (inside the library)
-    est.time_walk = True
-    walk_ms = []
-    for _ in range(max(3, min(args.steps, 10))):
-        est.compute_domain(pts, paths, seed=0, path_base=step * paths, out=(s, q))
-        walk_ms.append(est.last_launch.walk_ms)
-        step += 1
-    est.time_walk = False
-    avg_walk_s = float(np.mean(walk_ms)) / 1e3
-    flop, byts = workload_model(w.name, w.elements.n, w.solver["path_length"])
-    launch_paths = n * paths
-    gbps = byts * launch_paths / avg_walk_s / 1e9
-    tflops = flop * launch_paths / avg_walk_s / 1e12
-
-    # accuracy of what was computed (all steps accumulate into s, q)
-    m = (s / (step * paths)).cpu().numpy()
-    ex = w.exact(w.points)
-    d = m[:, 0] - ex[:, 0]
-    if w.solver["problem"] == "NeumannProblem":
-        d = d - d.mean()
-    rmse = float(np.sqrt((d ** 2).mean()))
-
-    batch = None
+    legs = {}
+    if not args.no_icosphere:
+        legs["icosphere"] = icosphere_leg(args, torch, wob, G)
     if not args.no_batch:
-        batch = batch_leg(args, torch, wob, G)
-
-    poisson = None
+        legs["batch"] = batch_leg(args, torch, wob, G)
     if not args.no_poisson:
-        poisson = poisson_leg(args, torch, wob, G)
-
-    vpl = None
+        legs["poisson"] = poisson_leg(args, torch, wob, G)
     if not args.no_vpl:
-        vpl = vpl_leg(args, torch, wob, G)
-
-    build = None
+        legs["vpl"] = vpl_leg(args, torch, wob, G)
     if not args.no_scene_build:
-        build = scene_build_leg(args, torch, wob, G, (w, pts, paths))
-
+        legs["scene_build"] = scene_build_leg(args, torch, wob, G, (w, pts, paths))
     ttr = None
     if not args.no_time_to_rmse:
-        ttr = time_to_rmse_leg(args, torch, wob, G)
+        ttr = time_to_rmse_leg(args, torch, wob, G, w, scene, pts)
 
     cores = min(16, os.cpu_count() or 1)
     cpu = None
     if not args.no_cpu_baseline:
         print(f"[bench] cpu baseline on {cores} threads", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(w, cores)
+        cpu = cpu_baseline(w, cores, paths)
 
     line = {
         "metric": METRIC, "value": value, "unit": "paths/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": w.name, "elements": w.elements.n, "points": n, "paths_per_point": paths,
-                   "paths_per_step": launch_paths, "path_length": w.solver["path_length"],
+                   "paths_per_step": n * paths, "path_length": w.solver["path_length"],
                    "bie_model": w.solver["bie_model"], "problem": w.solver["problem"],
-                   "estimator": w.solver["estimator"], "paths_per_lane": est.last_launch.used_paths_per_lane,
+                   "estimator": w.solver["estimator"],
+                   "num_resampling_candidates": w.solver["num_resampling_candidates"],
+                   "paths_per_lane": est.last_launch.used_paths_per_lane,
                    "work_items": est.last_launch.used_work_items},
-        "roofline": {"bound": "hbm", "achieved": gbps, "peak": MI355X_PEAKS["hbm_GBps"], "unit": "GB/s",
-                     "frac": gbps / MI355X_PEAKS["hbm_GBps"], "traffic": MEASURED_TRAFFIC.get(w.name),
-                     "kernel": "walk_kernel", "avg_kernel_ms": avg_walk_s * 1e3,
-                     "algorithmic_bytes_per_path": byts, "algorithmic_flop_per_path": flop,
-                     "fp32_valu": {"achieved": tflops, "peak": MI355X_PEAKS["fp32_TFLOPs"], "unit": "TFLOP/s",
-                                   "frac": tflops / MI355X_PEAKS["fp32_TFLOPs"]}},
+        "roofline": roofline_block(w.name, w.elements.n, w.solver["path_length"], n * paths, walk_ms,
+                                   w.solver["num_resampling_candidates"]),
         "cpu_baseline": cpu,
         "time_to_rmse": ttr,
-        "batch": batch,
-        "poisson": poisson,
-        "vpl": vpl,
-        "scene_build": build,
-        "accuracy": {"rmse_u": rmse, "paths_per_point_total": step * paths,
-                     "note": "Neumann problem: u compared after removing the mean offset"
-                     if w.solver["problem"] == "NeumannProblem" else "u compared directly"},
+        "accuracy": {"rmse_u": float(np.sqrt((err ** 2).mean())), "mean_se_u": float(np.sqrt(var / k).mean()),
+                     "paths_per_point": k, "note": "u against the exact solution; RMSE ~ mean SE: noise-limited"},
+        **legs,
     }
     print(json.dumps(line), flush=True)
 

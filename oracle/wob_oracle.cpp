@@ -1465,6 +1465,39 @@ int orc_eval_weights(int dim, int bie, int domain, float eps, float fkesc, int n
     return 0;
 }
 
+// Domain-representation terms of a model without volume source for domain points xd and
+// boundary records yb: per pair [solution_domain_known, solution_domain_unknown,
+// gradient_domain_known (dim), gradient_domain_unknown (dim)] (bie_models.cuh:228-250,
+// 341-363, 427-478).
+int orc_eval_domain_terms(int dim, int bie, int domain, float eps, int n, const float *xd, const float *yb,
+                          const float *inv_pdf, float *out) {
+    auto run = [&](auto dimtag) {
+        constexpr int D = decltype(dimtag)::value;
+        const int W = 2 * D + 4, O = 2 + 2 * D;
+        Model<D> m;
+        m.bie = bie; m.interior = domain == InteriorDomain; m.dsign = m.interior ? 1.f : -1.f;
+        m.k.eps = eps; m.c = 1.f; m.nvs = 0; m.scene = nullptr;
+        for (int i = 0; i < n; ++i) {
+            const float *r = yb + (size_t)i * W;
+            BPoint<D> y;
+            for (int k = 0; k < D; ++k) { y.p[k] = r[k]; y.n[k] = r[D + k]; }
+            y.interior_angle = r[2 * D]; y.boundary_value = r[2 * D + 1]; y.boundary_type = (int)r[2 * D + 2];
+            y.robin_alpha = r[2 * D + 3];
+            Vec<D> X; for (int k = 0; k < D; ++k) X[k] = xd[(size_t)i * D + k];
+            Rng dummy;
+            float *o = out + (size_t)i * O;
+            o[0] = m.solution_domain_known(X, y, inv_pdf[i], dummy);
+            o[1] = m.solution_domain_unknown(X, y, inv_pdf[i]);
+            const Vec<D> gk = m.gradient_domain_known(X, y, inv_pdf[i], dummy);
+            const Vec<D> gu = m.gradient_domain_unknown(X, y, inv_pdf[i]);
+            for (int k = 0; k < D; ++k) { o[2 + k] = gk[k]; o[2 + D + k] = gu[k]; }
+        }
+    };
+    if (dim == 2) run(std::integral_constant<int, 2>{});
+    else run(std::integral_constant<int, 3>{});
+    return 0;
+}
+
 // One Scene::sample_boundary_uniform draw (scene.cuh:441-462 / 734-758) from the Philox
 // stream (seed, path, point, stream).  out_rec: 2D+4 floats, returns inv_pdf.
 float orc_sample_uniform(void *p, uint64_t seed, uint32_t path, uint32_t point, uint32_t stream, float *out_rec) {

@@ -63,6 +63,7 @@ def lib():
         L.orc_eval_kernels_f32.argtypes = [C.c_int, C.c_int, P, P, P, P, C.c_float, P]
         L.orc_eval_kernels_f64.argtypes = [C.c_int, C.c_int, P, P, P, P, C.c_double, P]
         L.orc_eval_weights.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, P, P, P, P, P, P]
+        L.orc_eval_domain_terms.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, P, P, P, P]
         L.orc_philox4x32_10.argtypes = [P, P, P]
         L.orc_triple32.argtypes = [C.c_uint32]
         L.orc_triple32.restype = C.c_uint32
@@ -272,6 +273,16 @@ def eval_weights(dim, bie, domain, eps, fkesc, xb, yb, inv_pdf, xd, u_split):
     out = np.zeros((n, 5 + dim), np.float32)
     a = [np.ascontiguousarray(v, np.float32) for v in (xb, yb, inv_pdf, xd, u_split)]
     lib().orc_eval_weights(dim, bie, domain, float(eps), float(fkesc), n, *[_p(v) for v in a], _p(out))
+    return out
+
+
+def eval_domain_terms(dim, bie, domain, eps, xd, yb, inv_pdf):
+    """Per (domain point, boundary record) pair: [known, unknown, grad known (dim), grad unknown
+    (dim)] of the model's domain representation (no volume source)."""
+    n = xd.shape[0]
+    out = np.zeros((n, 2 + 2 * dim), np.float32)
+    a = [np.ascontiguousarray(v, np.float32) for v in (xd, yb, inv_pdf)]
+    lib().orc_eval_domain_terms(dim, bie, domain, float(eps), n, *[_p(v) for v in a], _p(out))
     return out
 
 

@@ -13,8 +13,10 @@ import time
 
 
 def run(estimator, points, exact_u, paths_per_pass, time_budget_s, t0=None, seed=0, neumann_shift=False,
-        csv_path=None, max_passes=1_000_000):
-    """Returns a list of (seconds since t0, paths per point, rmse_u) checkpoints.
+        csv_path=None, max_passes=1_000_000, growth=1.25):
+    """Returns a list of (seconds since t0, paths per point, rmse_u, rms standard error of u)
+    checkpoints, taken after pass counts growing by `growth`.  With no bias, rmse_u and the rms
+    standard error agree; rmse^2 - se^2 estimates the squared bias.
 
     points: device tensor (n, dim); exact_u: device FP64 tensor (n,) of exact values.
     t0: time origin (perf_counter); setup work done before the call (training) counts.
@@ -36,28 +38,27 @@ def run(estimator, points, exact_u, paths_per_pass, time_budget_s, t0=None, seed
         if passes >= next_check:
             torch.cuda.synchronize()
             t = time.perf_counter() - t0
-            m = s[:, 0] / (passes * paths_per_pass)
+            k = passes * paths_per_pass
+            m = s[:, 0] / k
             err = m - exact_u
             if neumann_shift:
                 err = err - err.mean()
             rmse = float(torch.sqrt((err * err).mean()))
-            curve.append((t, passes * paths_per_pass, rmse))
+            se = float(torch.sqrt((torch.clamp(q[:, 0] / k - m * m, min=0) / k).mean()))
+            curve.append((t, k, rmse, se))
             if t >= time_budget_s:
                 break
-            next_check = max(passes + 1, int(math.ceil(passes * 1.5)))
+            next_check = max(passes + 1, int(math.ceil(passes * growth)))
     if csv_path:
         with open(csv_path, "w") as f:
-            for t, k, r in curve:
+            for t, k, r, _ in curve:
                 f.write(f"{k},{t},{r}\n")
     return curve
 
 
-def time_to_rmse(curve, target, t_setup=0.0):
-    """Seconds to reach `target`: measured if reached, else extrapolated with rmse^2 (t - t_setup) = const
-    fitted on the last checkpoint (variance-limited estimator, same expectation for both arms)."""
-    for t, k, r in curve:
+def time_to_rmse(curve, target):
+    """Seconds (since the curve's origin) of the first checkpoint with rmse <= target, or None."""
+    for t, k, r, _ in curve:
         if r <= target:
-            return t, True
-    t, k, r = curve[-1]
-    eff = r * r * max(t - t_setup, 1e-9)
-    return t_setup + eff / (target * target), False
+            return t
+    return None

@@ -35,7 +35,21 @@ def per_path(dim, n_elements, path_length, k_interior, k_boundary, cdf_samples_p
     return flop, byts
 
 
-def workload_model(name, n_elements, path_length):
+def ris_per_path(dim, n_elements, path_length, candidates):
+    """(flop, bytes) per path of a single-layer walk with RIS over `candidates` uniform samples
+    per bounce: each candidate is one CDF sample (L + 15 ops, 4 L + B_prim bytes) and a kernel
+    evaluation with its reservoir update (25 ops); path_length bounces plus the domain sample."""
+    p = PARAMS[dim]
+    L = math.ceil(math.log2(n_elements))
+    bounces = path_length + 1
+    flop = bounces * candidates * (L + 15 + 25)
+    byts = bounces * candidates * (4 * L + p["B_prim"])
+    return flop, byts
+
+
+def workload_model(name, n_elements, path_length, candidates=64):
+    if name == "geodesic_mixed_3d":
+        return ris_per_path(3, n_elements, path_length, candidates)
     ki, kb = MEASURED_K[name]
     if name == "icosphere_neumann_3d":
         # direct Neumann: one boundary-value CDF sample per query (y1 and one per bounce)
@@ -43,12 +57,22 @@ def workload_model(name, n_elements, path_length):
     return per_path(2, n_elements, path_length, ki, kb)
 
 
-# HBM bytes per walk-kernel launch measured with rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE
-# (100,000 points x 256 paths, Direct / Neumann, 6 bounces), each counter scaled by what it reads
-# for a 512 MiB device copy in the same run: FETCH_SIZE 262,166 KiB (x 2.0), WRITE_SIZE
-# 524,288 KiB (x 1.0).  Walk kernel: FETCH_SIZE 19,500 KiB, WRITE_SIZE 35,008 KiB.
+# HBM bytes per walk-kernel launch measured with rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE,
+# each counter scaled by what it reads for a 512 MiB device copy in the same run (FETCH_SIZE
+# 262,161 KiB: x 2.0; WRITE_SIZE 524,288 KiB: x 1.0).  FETCH_SIZE counts L2 misses, served by
+# the MALL or HBM.
+#   geodesic_mixed_3d (1,000,000 points x 16 paths, single layer, RIS 64, 4 bounces):
+#     FETCH_SIZE 401,949,842 KiB, WRITE_SIZE 31,834 KiB per launch
+#   icosphere_neumann_3d (100,000 points x 256 paths, direct Neumann, 6 bounces):
+#     FETCH_SIZE 122,634 KiB, WRITE_SIZE 70,919 KiB per launch
 MEASURED_TRAFFIC = {
-    "icosphere_neumann_3d": (2.0 * 19500 + 35008) * 1024,
+    "geodesic_mixed_3d": (2.0 * 401949842 + 31834) * 1024,
+    "icosphere_neumann_3d": (2.0 * 122634 + 70919) * 1024,
 }
+
+# What bounds each workload's walk kernel (rocprofv3 counters): the RIS walk's scattered
+# candidate loads keep the texture address / data units 94-96 % busy and read L2 misses at the
+# rate of the HBM peak; the line-sampled walk keeps the vector ALU about 90 % busy.
+BOUND = {"geodesic_mixed_3d": "hbm", "icosphere_neumann_3d": "mfma", "circle_dirichlet_2d": "mfma"}
 
 MI355X_PEAKS = {"hbm_GBps": 8000.0, "fp32_TFLOPs": 157.3, "bf16_TFLOPs": 2500.0}
