@@ -8,7 +8,7 @@
 // one: hit counts and picks do not depend on the tree.
 //   1. per element: the inflated box (4e-6 |coordinate| + 1e-7, as the host build) and the
 //      Morton code of its centre in the vertex bounds (3 x 10 bits in 3D, 2 x 16 in 2D);
-//   2. radix sort of (code, element) pairs (hipCUB);
+//   2. a stable LSD radix sort of (code, element) pairs (8-bit digits, this file's own);
 //   3. the binary radix tree, one internal node per thread; equal codes are ordered by
 //      index (the delta function falls back to the index bits);
 //   4. boxes bottom-up: each leaf walks to the root, the second child to arrive at a node
@@ -24,8 +24,6 @@
 //   7. primitive records in depth-first leaf order.
 // Subtrees of at most kLeafPrims primitives become leaves.
 #include <hip/hip_runtime.h>
-
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -598,6 +596,111 @@ namespace wobl {
 
 using namespace wobb;
 
+// ---------------------------------------------------------------------------------------
+// Exclusive scan of uint32 counts and a stable LSD radix sort of (key, value) pairs.
+// ---------------------------------------------------------------------------------------
+// One block scans 1024 entries (4 per thread): the threads' sums are scanned in LDS, each
+// thread writes its entries' exclusive prefixes and the block's total goes to block_sums.
+// In place is allowed (every thread reads its entries before writing them).
+constexpr uint32_t kScanItems = 1024;
+__global__ __launch_bounds__(256) void scan_block_kernel(uint32_t n, const uint32_t *in, uint32_t *out,
+                                                         uint32_t *block_sums) {
+    __shared__ uint32_t s[256];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t base = blockIdx.x * kScanItems + tid * 4;
+    uint32_t a[4], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        a[k] = base + k < n ? in[base + k] : 0u;
+        sum += a[k];
+    }
+    s[tid] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        const uint32_t t = tid >= off ? s[tid - off] : 0u;
+        __syncthreads();
+        s[tid] += t;
+        __syncthreads();
+    }
+    uint32_t run = s[tid] - sum;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += a[k];
+    }
+    if (tid == 255 && block_sums) block_sums[blockIdx.x] = s[255];
+}
+
+__global__ __launch_bounds__(256) void scan_add_kernel(uint32_t n, uint32_t *out, const uint32_t *block_offsets) {
+    const uint32_t add = block_offsets[blockIdx.x];
+    const uint32_t base = blockIdx.x * kScanItems + threadIdx.x * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + k < n) out[base + k] += add;
+}
+
+// Radix sort tiles: 256 threads x 8 rounds; in round r thread t holds element
+// base + 256 r + t, so a tile is visited in element order.
+constexpr uint32_t kSortRounds = 8, kSortTile = 256 * kSortRounds;
+
+// per tile and digit: the count of the tile's keys with that digit, stored digit-major
+// (counts[d * tiles + tile]) so that one exclusive scan gives every (digit, tile) its start
+__global__ __launch_bounds__(256) void radix_count_kernel(uint32_t n, uint32_t shift, const uint32_t *keys,
+                                                          uint32_t *counts) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kSortTile;
+    for (uint32_t r = 0; r < kSortRounds; ++r) {
+        const uint32_t i = base + r * 256 + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    counts[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: an element goes to its (digit, tile) start plus the number of earlier
+// elements of the tile with the same digit.  Within a round, a wave finds the lanes that share
+// a lane's digit with eight ballots; the waves' per-digit counts in LDS order the waves.
+__global__ __launch_bounds__(256) void radix_scatter_kernel(uint32_t n, uint32_t shift, const uint32_t *kin,
+                                                            const uint32_t *vin, const uint32_t *starts,
+                                                            uint32_t *kout, uint32_t *vout) {
+    __shared__ uint32_t next[256];        // next free position of each digit for this tile
+    __shared__ uint32_t wcnt[4][256];     // per wave and digit: elements of the current round
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    next[tid] = starts[tid * gridDim.x + blockIdx.x];
+    const uint32_t base = blockIdx.x * kSortTile;
+    const uint64_t below = ((uint64_t)1 << lane) - 1u;
+    for (uint32_t r = 0; r < kSortRounds; ++r) {
+        const uint32_t i = base + r * 256 + tid;
+        const bool valid = i < n;
+        const uint32_t k = valid ? kin[i] : 0u, v = valid ? vin[i] : 0u;
+        const uint32_t d = (k >> shift) & 255u;
+        uint64_t same = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = ((d >> b) & 1u) != 0u;
+            const uint64_t bb = __ballot(bit);
+            same &= bit ? bb : ~bb;
+        }
+        const uint32_t rank = (uint32_t)__popcll(same & below);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) wcnt[w][tid] = 0u;
+        __syncthreads();
+        if (valid && rank == 0u) wcnt[wave][d] = (uint32_t)__popcll(same);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = next[d] + rank;
+            for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][d];
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+        __syncthreads();
+        next[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+        __syncthreads();
+    }
+}
+
 namespace {
 
 struct DevBuf {
@@ -615,6 +718,51 @@ struct DevBuf {
 };
 
 inline uint32_t blocks(uint64_t n) { return (uint32_t)((n + 255) / 256); }
+
+// Scratch entries exclusive_scan needs for n entries: the block sums of every level.
+size_t scan_scratch(uint64_t n) {
+    size_t total = 0;
+    while (n > kScanItems) {
+        n = (n + kScanItems - 1) / kScanItems;
+        total += n;
+    }
+    return total;
+}
+
+// out[i] = in[0] + ... + in[i - 1]; `scratch` holds scan_scratch(n) entries.
+hipError_t exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *scratch, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint32_t nb = (n + kScanItems - 1) / kScanItems;
+    hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(256), 0, st, n, in, out, nb > 1 ? scratch : nullptr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || nb == 1) return e;
+    if ((e = exclusive_scan(scratch, scratch, nb, scratch + nb, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(256), 0, st, n, out, scratch);
+    return hipGetLastError();
+}
+
+// Stable sort of n (key, value) pairs by the key bits below `bits`, 8 bits per pass, ping-
+// ponging between (k0, v0) and (k1, v1); returns in `which` the pair that holds the result
+// (0: k0/v0, 1: k1/v1).  counts: 256 * tiles entries; scan scratch: scan_scratch(256 * tiles).
+hipError_t radix_sort(uint32_t n, uint32_t bits, uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1,
+                      uint32_t *counts, uint32_t *scratch, int &which, hipStream_t st) {
+    which = 0;
+    if (n <= 1) return hipSuccess;
+    const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
+    uint32_t *ki = k0, *vi = v0, *ko = k1, *vo = v1;
+    for (uint32_t shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(radix_count_kernel, dim3(tiles), dim3(256), 0, st, n, shift, ki, counts);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        if ((e = exclusive_scan(counts, counts, 256 * tiles, scratch, st)) != hipSuccess) return e;
+        hipLaunchKernelGGL(radix_scatter_kernel, dim3(tiles), dim3(256), 0, st, n, shift, ki, vi, counts, ko, vo);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+        which ^= 1;
+    }
+    return hipSuccess;
+}
 
 template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBvh &out) {
     constexpr int NW = D == 2 ? kNodeWords2 : kNodeWords3;
@@ -662,12 +810,17 @@ template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBv
     hipLaunchKernelGGL((leaf_kernel<D>), dim3(blocks(n)), dim3(256), 0, st, n, V, make_float3(mn[0], mn[1], mn[2]),
                        make_float3(sc[0], sc[1], sc[2]), code, idx, pbox);
     WOB_TRY(hipGetLastError());
-    size_t sort_bytes = 0, scan_bytes = 0;
-    WOB_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, code, code2, idx, idx2, (int)n, 0, 32, st));
-    WOB_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, counts, offs, (int)max_wide, st));
-    void *cub_tmp = tmp.get<char>(std::max(sort_bytes, scan_bytes));
+    const uint32_t sort_tiles = (n + kSortTile - 1) / kSortTile;
+    uint32_t *sort_counts = tmp.get<uint32_t>((size_t)256 * sort_tiles);
+    uint32_t *scan_tmp = tmp.get<uint32_t>(std::max(scan_scratch((size_t)256 * sort_tiles), scan_scratch(max_wide)));
     if (tmp.err != hipSuccess) return std::string("hipMalloc during the device BVH build: ") + hipGetErrorString(tmp.err);
-    WOB_TRY(hipcub::DeviceRadixSort::SortPairs(cub_tmp, sort_bytes, code, code2, idx, idx2, (int)n, 0, 32, st));
+    // Morton codes have 3 x 10 bits (3D) or 2 x 16 bits (2D)
+    int sorted_in = 0;
+    WOB_TRY(radix_sort(n, D == 3 ? 30u : 32u, code, idx, code2, idx2, sort_counts, scan_tmp, sorted_in, st));
+    if (sorted_in == 0) {   // the result is in (code, idx): the later steps read (code2, idx2)
+        std::swap(code, code2);
+        std::swap(idx, idx2);
+    }
     if (n > 1) {
         hipLaunchKernelGGL(tree_kernel, dim3(blocks(n_int)), dim3(256), 0, st, (int)n, code2, left, right, parent, first,
                            count);
@@ -720,7 +873,7 @@ template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBv
         hipLaunchKernelGGL((count_kernel<D>), dim3(blocks(m)), dim3(256), 0, st, (int)n, m, cur, left, right, box, count,
                            counts);
         WOB_TRY(hipGetLastError());
-        WOB_TRY(hipcub::DeviceScan::ExclusiveSum(cub_tmp, scan_bytes, counts, offs, (int)m, st));
+        WOB_TRY(exclusive_scan(counts, offs, m, scan_tmp, st));
         uint32_t last[2] = {0u, 0u};
         WOB_TRY(hipMemcpyAsync(&last[0], counts + (m - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         WOB_TRY(hipMemcpyAsync(&last[1], offs + (m - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -745,7 +898,7 @@ template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBv
                            fnodes, need);
         WOB_TRY(hipGetLastError());
     }
-    WOB_TRY(hipMemsetAsync(code, 0, (size_t)n * sizeof(uint32_t), st));   // the sort input, reused as position marks
+    WOB_TRY(hipMemsetAsync(code, 0, (size_t)n * sizeof(uint32_t), st));   // the sort's other buffer, reused as position marks
     hipLaunchKernelGGL((prims_kernel<D>), dim3(blocks(n)), dim3(256), 0, st, n, V, idx2, first, prims, prim_id, code, bad);
     WOB_TRY(hipGetLastError());
     uint32_t need0 = 0;
