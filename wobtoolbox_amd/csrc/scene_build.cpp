@@ -202,6 +202,7 @@ std::string build_scene(const wob_scene_desc &d, HostScene &S, bool with_bvh) {
     // ---- element attributes, areas, boundary-value weights (element order)
     const int AW = D == 2 ? kAttrWords2 : kAttrWords3;
     S.elem_attr.assign((size_t)n * AW, 0.f);
+    S.elem_geo.assign(D == 3 ? (size_t)n * kGeoWords3 : 0u, 0.f);
     std::vector<float> area(n), bvw(n);
     for (uint32_t e = 0; e < n; ++e) {
         float *a = &S.elem_attr[(size_t)e * AW];
@@ -229,6 +230,7 @@ std::string build_scene(const wob_scene_desc &d, HostScene &S, bool with_bvh) {
             a[8] = V(e, 2, 2); a[9] = d.normals[3 * e]; a[10] = d.normals[3 * e + 1]; a[11] = d.normals[3 * e + 2];
             a[12] = bv[0]; a[13] = bv[1]; a[14] = bv[2]; a[15] = (float)d.boundary_types[e];
             a[16] = al0; a[17] = al1; a[18] = al2; a[19] = ar;
+            std::copy(a, a + 12, &S.elem_geo[(size_t)e * kGeoWords3]);
         }
     }
     build_cdf(area, S.area_cdf, S.total_area, S.area_guide);

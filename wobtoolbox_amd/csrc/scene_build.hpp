@@ -36,6 +36,10 @@ constexpr int kPrimWords2 = 4;    // a.x a.y e.x e.y                 (e = a - b)
 constexpr int kPrimWords3 = 12;   // a.xyz e1.xyz e2.xyz id 0 0      (e1 = a - b, e2 = a - c, id = element id)
 constexpr int kAttrWords2 = 12;   // a.xy b.xy | n.xy bv0 bv1 | al0 al1 type area
 constexpr int kAttrWords3 = 20;   // a.xyz b.x | b.yz c.xy | c.z n.xyz | bv0 bv1 bv2 type | al0 al1 al2 area
+// 3D element geometry again, 64 bytes per element: the uniform boundary samples (RIS
+// candidates) read only vertices and normal, which then lie in one 64-byte memory block
+// (in the 80-byte attribute records they straddle two half the time).
+constexpr int kGeoWords3 = 16;    // a.xyz b.x | b.yz c.xy | c.z n.xyz | 4 pad
 constexpr int kMaxLeaf = 8;
 
 struct HostScene {
@@ -45,6 +49,7 @@ struct HostScene {
     std::vector<float> prim_geom;      // n * kPrimWords (leaf order)
     std::vector<uint32_t> prim_id;     // leaf order -> element id
     std::vector<float> elem_attr;      // n * kAttrWords (element order)
+    std::vector<float> elem_geo;       // 3D: n * kGeoWords3 (element order)
     std::vector<float> area_cdf, bv_cdf;
     std::vector<uint32_t> area_guide, bv_guide;
     float total_area = 0, total_bv = 0;

@@ -41,7 +41,7 @@ inline wob_status hip_fail(hipError_t e, const char *what) { return wob_hip_fail
 struct wob_scene {
     wobhost::HostScene host;
     wobdev::DevScene dev{};
-    void *bufs[9] = {nullptr};
+    void *bufs[10] = {nullptr};
     float *scratch = nullptr;
     size_t scratch_bytes = 0;
     uint32_t *counter = nullptr;   // work counter of wavefront launches
@@ -522,7 +522,7 @@ wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_s
     }
     const auto &h = s->host;
     wob_status st = WOB_OK;
-    const float *nodes = dv.fnodes, *prims = dv.prims, *attr = nullptr, *acdf = nullptr, *bcdf = nullptr;
+    const float *nodes = dv.fnodes, *prims = dv.prims, *attr = nullptr, *geo = nullptr, *acdf = nullptr, *bcdf = nullptr;
     const float *ag = nullptr, *bg = nullptr;
     const uint32_t *pid = dv.prim_id, *qn = dv.qnodes;
     if (!device_bvh &&
@@ -532,7 +532,8 @@ wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_s
         delete s;
         return st;
     }
-    if ((st = upload(s, 3, h.elem_attr, &attr)) != WOB_OK || (st = upload(s, 4, h.area_cdf, &acdf)) != WOB_OK ||
+    if ((st = upload(s, 3, h.elem_attr, &attr)) != WOB_OK || (st = upload(s, 9, h.elem_geo, &geo)) != WOB_OK ||
+        (st = upload(s, 4, h.area_cdf, &acdf)) != WOB_OK ||
         (st = upload(s, 5, wobhost::guide_records(h.area_cdf, h.area_guide), &ag)) != WOB_OK ||
         (st = upload(s, 6, h.bv_cdf, &bcdf)) != WOB_OK ||
         (st = upload(s, 7, wobhost::guide_records(h.bv_cdf, h.bv_guide), &bg)) != WOB_OK) {
@@ -545,6 +546,7 @@ wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_s
     s->dev.prims = (const float4 *)prims;
     s->dev.prim_id = pid;
     s->dev.attr = (const float4 *)attr;
+    s->dev.geo = (const float4 *)geo;
     s->dev.area_cdf = acdf;
     s->dev.area_guide = (const float4 *)ag;
     s->dev.bv_cdf = bcdf;

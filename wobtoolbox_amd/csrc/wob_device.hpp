@@ -62,6 +62,7 @@ struct DevScene {
     const float4 *prims;
     const uint32_t *prim_id;
     const float4 *attr;
+    const float4 *geo;        // 3D: vertices and normal, 4 rows per element (64-byte records)
     const float *area_cdf;
     const float4 *area_guide;   // guide records (i, cdf[i-1], cdf[i], cdf[i+1]) per entry
     const float *bv_cdf;
@@ -407,7 +408,7 @@ __device__ __forceinline__ GeomSample<D> uniform_geometry_of(const DevScene &S, 
         for (int k = 0; k < 2; ++k) g.p[k] = v[k] + u2 * (v[2 + k] - v[k]);            // as uniform_in_element
         g.bw[0] = 1.0f - u2; g.bw[1] = u2;
     } else {
-        const float4 f0 = gld4(S.attr + 5 * e), f1 = gld4(S.attr + 5 * e + 1), f2 = gld4(S.attr + 5 * e + 2);
+        const float4 f0 = gld4(S.geo + 4 * e), f1 = gld4(S.geo + 4 * e + 1), f2 = gld4(S.geo + 4 * e + 2);
         const float v[9] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w, f2.x};
         g.n[0] = f2.y; g.n[1] = f2.z; g.n[2] = f2.w;
         const float u2 = u ? u[1] : rng->uniform();

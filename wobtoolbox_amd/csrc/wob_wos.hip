@@ -100,7 +100,7 @@ template <int D> __device__ __forceinline__ void elem_vertices(const DevScene &S
         const float4 f0 = S.attr[3 * e];
         v[0] = f0.x; v[1] = f0.y; v[2] = f0.z; v[3] = f0.w;
     } else {
-        const float4 f0 = S.attr[5 * e], f1 = S.attr[5 * e + 1], f2 = S.attr[5 * e + 2];
+        const float4 f0 = S.geo[4 * e], f1 = S.geo[4 * e + 1], f2 = S.geo[4 * e + 2];
         v[0] = f0.x; v[1] = f0.y; v[2] = f0.z; v[3] = f0.w; v[4] = f1.x; v[5] = f1.y; v[6] = f1.z; v[7] = f1.w;
         v[8] = f2.x;
     }
