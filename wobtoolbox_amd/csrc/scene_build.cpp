@@ -157,14 +157,16 @@ void build_cdf(const std::vector<float> &w, std::vector<float> &cdf, float &tota
     build_guide(cdf, guide);
 }
 
-void build_guide(const std::vector<float> &cdf, std::vector<uint32_t> &guide) {
-    // guide[k] = first i with cdf[i] >= k / n; the device search walks forward / backward
-    // from there, so it returns the first i with u <= cdf[i] (clamped to n-1) for any u.
+void build_guide(const std::vector<float> &cdf, std::vector<uint32_t> &guide, size_t entries) {
+    // guide[k] = first i with cdf[i] >= k / entries; the device search walks forward /
+    // backward from there, so it returns the first i with u <= cdf[i] (clamped to n-1) for
+    // any u.
     const size_t n = cdf.size();
-    guide.resize(n);
+    const size_t m = entries ? entries : n;
+    guide.resize(m);
     size_t i = 0;
-    for (size_t k = 0; k < n; ++k) {
-        const float lo = (float)k / (float)n;
+    for (size_t k = 0; k < m; ++k) {
+        const float lo = (float)k / (float)m;
         while (i + 1 < n && cdf[i] < lo) ++i;
         guide[k] = (uint32_t)i;
     }
@@ -179,6 +181,19 @@ std::vector<float> guide_records(const std::vector<float> &cdf, const std::vecto
         r[4 * k + 1] = i > 0 ? cdf[i - 1] : 0.f;
         r[4 * k + 2] = cdf[i];
         r[4 * k + 3] = i + 1 < n ? cdf[i + 1] : 0.f;
+    }
+    return r;
+}
+
+std::vector<float> fused_area_records(const std::vector<float> &cdf, const std::vector<float> &geo) {
+    const size_t n = cdf.size();
+    std::vector<uint32_t> guide;
+    build_guide(cdf, guide, kFusedPerElement * n);
+    const std::vector<float> g = guide_records(cdf, guide);
+    std::vector<float> r(kGeoWords3 * guide.size());
+    for (size_t k = 0; k < guide.size(); ++k) {
+        std::copy(&g[4 * k], &g[4 * k] + 4, &r[kGeoWords3 * k]);
+        std::copy(&geo[(size_t)kGeoWords3 * guide[k]], &geo[(size_t)kGeoWords3 * guide[k]] + 12, &r[kGeoWords3 * k + 4]);
     }
     return r;
 }
@@ -234,6 +249,7 @@ std::string build_scene(const wob_scene_desc &d, HostScene &S, bool with_bvh) {
         }
     }
     build_cdf(area, S.area_cdf, S.total_area, S.area_guide);
+    if (D == 3) S.area_fused = fused_area_records(S.area_cdf, S.elem_geo);
     build_cdf(bvw, S.bv_cdf, S.total_bv, S.bv_guide);
     if (!with_bvh) return "";
 

@@ -41,7 +41,7 @@ inline wob_status hip_fail(hipError_t e, const char *what) { return wob_hip_fail
 struct wob_scene {
     wobhost::HostScene host;
     wobdev::DevScene dev{};
-    void *bufs[10] = {nullptr};
+    void *bufs[11] = {nullptr};
     float *scratch = nullptr;
     size_t scratch_bytes = 0;
     uint32_t *counter = nullptr;   // work counter of wavefront launches
@@ -532,7 +532,9 @@ wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_s
         delete s;
         return st;
     }
+    const float *fused = nullptr;
     if ((st = upload(s, 3, h.elem_attr, &attr)) != WOB_OK || (st = upload(s, 9, h.elem_geo, &geo)) != WOB_OK ||
+        (st = upload(s, 10, h.area_fused, &fused)) != WOB_OK ||
         (st = upload(s, 4, h.area_cdf, &acdf)) != WOB_OK ||
         (st = upload(s, 5, wobhost::guide_records(h.area_cdf, h.area_guide), &ag)) != WOB_OK ||
         (st = upload(s, 6, h.bv_cdf, &bcdf)) != WOB_OK ||
@@ -547,6 +549,8 @@ wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_s
     s->dev.prim_id = pid;
     s->dev.attr = (const float4 *)attr;
     s->dev.geo = (const float4 *)geo;
+    s->dev.area_fused = (const float4 *)fused;
+    s->dev.n_fused = (uint32_t)(h.area_fused.size() / wobhost::kGeoWords3);
     s->dev.area_cdf = acdf;
     s->dev.area_guide = (const float4 *)ag;
     s->dev.bv_cdf = bcdf;

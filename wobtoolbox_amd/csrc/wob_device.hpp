@@ -63,6 +63,8 @@ struct DevScene {
     const uint32_t *prim_id;
     const float4 *attr;
     const float4 *geo;        // 3D: vertices and normal, 4 rows per element (64-byte records)
+    const float4 *area_fused; // 3D: fused area-guide records, 4 rows each (guide row + geometry)
+    uint32_t n_fused;
     const float *area_cdf;
     const float4 *area_guide;   // guide records (i, cdf[i-1], cdf[i], cdf[i+1]) per entry
     const float *bv_cdf;
@@ -422,10 +424,37 @@ __device__ __forceinline__ GeomSample<D> uniform_geometry_of(const DevScene &S, 
     return g;
 }
 
+// 3D: the element from the fused record of entry floor(u1 n_fused), whose geometry rows
+// are element i's; the search usually ends at i, otherwise the element's own geometry
+// record is read.  The same draws and arithmetic as uniform_geometry_of.
 template <int D> __device__ __forceinline__ GeomSample<D> uniform_geometry(const DevScene &S, Rng &rng) {
     const float u1 = rng.uniform();
-    const uint32_t e = cdf_lookup(S.area_cdf, S.area_guide, S.n, u1);
-    return uniform_geometry_of<D>(S, e, nullptr, &rng);
+    if constexpr (D == 2) {
+        const uint32_t e = cdf_lookup(S.area_cdf, S.area_guide, S.n, u1);
+        return uniform_geometry_of<D>(S, e, nullptr, &rng);
+    } else {
+        uint32_t k = (uint32_t)(u1 * (float)S.n_fused);
+        k = k < S.n_fused ? k : S.n_fused - 1;
+        const float4 *rec = S.area_fused + 4 * (size_t)k;
+        const float4 g = gld4(rec);
+        float4 f0 = gld4(rec + 1), f1 = gld4(rec + 2), f2 = gld4(rec + 3);
+        const uint32_t e = cdf_search(S.area_cdf, S.n, u1, g);
+        if (e != __float_as_uint(g.x)) {
+            f0 = gld4(S.geo + 4 * e); f1 = gld4(S.geo + 4 * e + 1); f2 = gld4(S.geo + 4 * e + 2);
+        }
+        GeomSample<D> gs;
+        gs.elem = e;
+        const float v[9] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w, f2.x};
+        gs.n[0] = f2.y; gs.n[1] = f2.z; gs.n[2] = f2.w;
+        const float u2 = rng.uniform();
+        const float u3 = rng.uniform();
+        const float sq = sqrtf(u2);
+        const float w0 = 1.0f - sq, w1 = sq * (1.0f - u3), w2 = u3 * sq;
+#pragma unroll
+        for (int k2 = 0; k2 < 3; ++k2) gs.p[k2] = w0 * v[k2] + w1 * v[3 + k2] + w2 * v[6 + k2];
+        gs.bw[0] = w0; gs.bw[1] = w1; gs.bw[2] = w2;
+        return gs;
+    }
 }
 
 template <int D> __device__ __forceinline__ BP<D> complete_uniform(const DevScene &S, const GeomSample<D> &g) {
