@@ -21,14 +21,33 @@ struct MlpParams {
     const float *b_f32;         // FP32 masters (biases read from here)
 };
 
+// Gradient partials (one block of n_params_padded BF16 values per 64-sample tile; the tile's
+// loss sum is an FP32 value in the two slots at n_params) use the flat layout above except
+// inside each W_l: there dW_l is stored by 16x16 tile (tile t = row block
+// t / 8, column block t % 8, 256 floats each) and, inside a tile, in the MFMA accumulator order
+// (lane L's 4 values at 4L .. 4L+3 are rows 4 (L / 16) + j, column L % 16), so that each wave
+// writes a tile as one contiguous 1 KB store.  partial_param maps a partial slot to its
+// parameter index; biases, the loss slot and padding map to themselves.
+__host__ __device__ inline uint32_t partial_param(const MlpParams &P, uint32_t q) {
+    for (int l = 0; l < 5; ++l) {
+        const uint32_t nout = l == 4 ? 16u : 128u;
+        if (q >= P.w_off[l] && q < P.w_off[l] + nout * 128u) {
+            const uint32_t r = q - P.w_off[l], t = r >> 8, L = (r & 255u) >> 2, j = r & 3u;
+            const uint32_t m = (t >> 3) * 16u + 4u * (L >> 4) + j, n = (t & 7u) * 16u + (L & 15u);
+            return P.w_off[l] + m * 128u + n;
+        }
+    }
+    return q;
+}
+
 namespace wobml {
 size_t smem_bytes();
 hipError_t launch_infer(const MlpParams &P, const float *pos, uint32_t n, float *out, uint32_t n_out, hipStream_t st);
 hipError_t launch_train_step(const MlpParams &P, const float *pos, const int32_t *ridx, const float *target, uint32_t n,
-                             float *partial, hipStream_t st);
-hipError_t launch_adam(const MlpParams &P, const float *partial, uint32_t n_parts, float lr, float beta1, float beta2,
+                             uint16_t *partial, hipStream_t st);
+hipError_t launch_adam(const MlpParams &P, const uint16_t *partial, uint32_t n_parts, float lr, float beta1, float beta2,
                        float eps, float bc1, float bc2, float *master, float *m, float *v, hipStream_t st);
-hipError_t launch_loss_sum(const float *partial, uint32_t n_parts, uint32_t stride, uint32_t off, float *loss,
+hipError_t launch_loss_sum(const uint16_t *partial, uint32_t n_parts, uint32_t stride, uint32_t off, float *loss,
                            hipStream_t st);
 hipError_t set_smem_limits();
 }  // namespace wobml

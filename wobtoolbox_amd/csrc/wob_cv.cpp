@@ -35,7 +35,7 @@ struct wob_cv {
     std::vector<float> h_master, h_freq;
     float *d_master = nullptr, *d_m = nullptr, *d_v = nullptr, *d_freq = nullptr;
     uint16_t *d_w = nullptr, *d_wt = nullptr;
-    float *d_partial = nullptr;
+    uint16_t *d_partial = nullptr;   // BF16 gradient partials, one block per 64-sample tile
     size_t partial_cap = 0;
     float *d_loss = nullptr;
     uint64_t step = 0;
@@ -70,7 +70,7 @@ wob_status cv_upload(wob_cv *c) {
 
 wob_status ensure_partial(wob_cv *c, uint32_t n) {
     const size_t parts = (n + 63) / 64;
-    const size_t bytes = parts * c->P.n_params_padded * sizeof(float);
+    const size_t bytes = parts * c->P.n_params_padded * sizeof(uint16_t);
     if (bytes <= c->partial_cap) return WOB_OK;
     if (c->d_partial) WOB_HIP(hipFree(c->d_partial), "hipFree");
     c->d_partial = nullptr;
@@ -228,17 +228,27 @@ wob_status wob_cv_gradient(wob_cv *c, const float *d_pos, const int32_t *d_idx, 
     WOB_HIP(wobml::launch_train_step(c->P, d_pos, d_idx, d_target, n, c->d_partial, hs), "mlp train launch");
     WOB_HIP(hipStreamSynchronize(hs), "hipStreamSynchronize");
     const uint32_t parts = (n + 63) / 64;
-    std::vector<float> h((size_t)parts * c->P.n_params_padded);
-    WOB_HIP(hipMemcpy(h.data(), c->d_partial, h.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    std::vector<uint16_t> h((size_t)parts * c->P.n_params_padded);
+    WOB_HIP(hipMemcpy(h.data(), c->d_partial, h.size() * 2, hipMemcpyDeviceToHost), "hipMemcpy");
+    auto bf2f = [](uint16_t b) {
+        const uint32_t u = (uint32_t)b << 16;
+        float f;
+        std::memcpy(&f, &u, 4);
+        return f;
+    };
     if (grad)
-        for (uint32_t p = 0; p < c->P.n_params; ++p) {
+        for (uint32_t q = 0; q < c->P.n_params; ++q) {
             float g = 0.f;
-            for (uint32_t k = 0; k < parts; ++k) g += h[(size_t)k * c->P.n_params_padded + p];
-            grad[p] = g;
+            for (uint32_t k = 0; k < parts; ++k) g += bf2f(h[(size_t)k * c->P.n_params_padded + q]);
+            grad[partial_param(c->P, q)] = g;
         }
     if (loss) {
         float l = 0.f;
-        for (uint32_t k = 0; k < parts; ++k) l += h[(size_t)k * c->P.n_params_padded + c->P.n_params];
+        for (uint32_t k = 0; k < parts; ++k) {
+            float t;
+            std::memcpy(&t, &h[(size_t)k * c->P.n_params_padded + c->P.n_params], 4);
+            l += t;
+        }
         *loss = l / (float)n;
     }
     return WOB_OK;
