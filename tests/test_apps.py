@@ -113,16 +113,38 @@ def test_app_2d_progressive_run(tmp_path, gpu):
 
 @pytest.mark.gpu
 def test_app_3d_progressive_run(tmp_path, gpu):
+    """general_3d.cu's progressive loop on the displaced icosphere: sample counts per pass, the
+    slice mask of every field, the RMSE row equal to the written field's error, and the error
+    falling like 1 / sqrt(paths) (unbiased, noise-limited: sqrt(1/4) = 0.5 after four passes).
+    Interior Dirichlet (double layer) is checked on the RMSE itself.  Interior Neumann (Direct)
+    shifts every pass by one offset taken, as general_3d.cu:118-133 does, against the analytic
+    value at a point off the slice, so there the spread of the error about its mean is checked."""
     obj = tmp_path / "sphere.obj"
     _write_sphere_obj(obj)
-    cfg = _config(tmp_path, obj, bie_model="Direct", problem="NeumannProblem", path_length=6, num_sample_paths=128)
-    _run(B.APPS[3], cfg, "--max-iterations", "2")
-    rows = np.loadtxt(tmp_path / "out" / "rmse_list.csv", delimiter=",", ndmin=2)
-    assert rows.shape == (2, 6)
-    print(f"3D app RMSE per pass: {rows[:, 2]}", flush=True)
-    assert np.all(np.isfinite(rows))
-    _, g = _read_field(tmp_path / "out" / "gradient2.vector3", 3)
-    assert g.shape == (32, 32, 3)
+    for problem in ("DirichletProblem", "NeumannProblem"):
+        bie = "IndirectDoubleLayer" if problem == "DirichletProblem" else "Direct"
+        cfg = _config(tmp_path, obj, bie_model=bie, problem=problem, path_length=6, num_sample_paths=128)
+        _run(B.APPS[3], cfg, "--max-iterations", "4")
+        od = tmp_path / "out"
+        rows = np.loadtxt(od / "rmse_list.csv", delimiter=",", ndmin=2)
+        assert rows.shape == (4, 6)
+        assert list(rows[:, 0]) == [128, 256, 384, 512]
+        assert np.all(np.isfinite(rows))
+        _, ua = _read_field(od / "potential_analytical.scalar")
+        m = np.isfinite(ua)
+        err = []
+        for it in (1, 4):
+            _, u = _read_field(od / f"potential{it}.scalar")
+            _, g = _read_field(od / f"gradient{it}.vector3", 3)
+            assert g.shape == (32, 32, 3)
+            assert np.array_equal(np.isnan(u), np.isnan(ua))
+            assert np.all(np.isfinite(g[m])) and np.all(g[~m] == 0)   # gradients outside stay 0
+            e = (u[m] - ua[m]).astype(np.float64)
+            assert np.sqrt(np.mean(e ** 2)) == pytest.approx(rows[it - 1, 2], rel=1e-3)
+            err.append(e)
+        spread = [np.sqrt(np.mean(e ** 2)) if problem == "DirichletProblem" else np.std(e) for e in err]
+        print(f"3D app {problem}: RMSE u per pass {rows[:, 2]}, error spread pass 1 / 4 {spread}", flush=True)
+        assert 0.3 < spread[1] / spread[0] < 0.75
 
 
 def _vpl_config(tmp_path, **kw):

@@ -5,7 +5,8 @@ gives a different tree from the host's SAH build, but nothing the engine returns
 the tree: hit counts are sums and the hit pick is a minimum over the hits.  So every result
 is compared bit for bit with the host-built scene:
 * walk sums of line-sampled and uniformly sampled estimators, 2D and 3D;
-* all-hit line queries (count, hit set and line parameters);
+* all-hit line queries (count, hit set and line parameters), also against the oracle's brute
+  force over all elements;
 * closest points;
 * tiny and degenerate scenes (1, 2 and 3 elements; all elements on one centre).
 """
@@ -65,7 +66,9 @@ def test_device_bvh_walks_equal_host_build(case, gpu):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
-def test_device_bvh_intersections_equal_host_build(dim, gpu):
+def test_device_bvh_intersections_equal_host_build(dim, gpu, oracle):
+    """Device-built and host-built trees return the same hits, and both equal the oracle's
+    brute force over all elements (same elements, line parameters bit for bit)."""
     el = star_scene(seed=2, n_segments=1500, kind=1).elements if dim == 2 else small_icosphere(4, "mixed")[0]
     host, dev = _scenes(el)
     rng = np.random.default_rng(40 + dim)
@@ -82,7 +85,13 @@ def test_device_bvh_intersections_equal_host_build(dim, gpu):
         a = sorted(zip(i0[k, :c0[k]].tolist(), t0[k, :c0[k]].tolist()))
         b = sorted(zip(i1[k, :c1[k]].tolist(), t1[k, :c1[k]].tolist()))
         assert a == b, k
-    _say(f"dim {dim}: {n} lines, {int(c0.sum())} hits, identical")
+    rc, rids, rts = oracle.OracleScene(el).intersect_many(o, d, use_bvh=False, cap=64)
+    assert np.array_equal(c1, rc)
+    for k in range(n):
+        a = sorted(zip(i1[k, :c1[k]].tolist(), t1[k, :c1[k]].view(np.uint32).tolist()))
+        b = sorted(zip(rids[k, :rc[k]].tolist(), rts[k, :rc[k]].view(np.uint32).tolist()))
+        assert a == b, k
+    _say(f"dim {dim}: {n} lines, {int(c0.sum())} hits, identical (host tree, device tree, brute force)")
 
 
 @pytest.mark.parametrize("dim", [2, 3])

@@ -185,19 +185,6 @@ std::vector<float> guide_records(const std::vector<float> &cdf, const std::vecto
     return r;
 }
 
-std::vector<float> fused_area_records(const std::vector<float> &cdf, const std::vector<float> &geo) {
-    const size_t n = cdf.size();
-    std::vector<uint32_t> guide;
-    build_guide(cdf, guide, kFusedPerElement * n);
-    const std::vector<float> g = guide_records(cdf, guide);
-    std::vector<float> r(kGeoWords3 * guide.size());
-    for (size_t k = 0; k < guide.size(); ++k) {
-        std::copy(&g[4 * k], &g[4 * k] + 4, &r[kGeoWords3 * k]);
-        std::copy(&geo[(size_t)kGeoWords3 * guide[k]], &geo[(size_t)kGeoWords3 * guide[k]] + 12, &r[kGeoWords3 * k + 4]);
-    }
-    return r;
-}
-
 std::string build_scene(const wob_scene_desc &d, HostScene &S, bool with_bvh) {
     if (d.dim != 2 && d.dim != 3) return "dim must be 2 or 3";
     if (d.n_elements == 0) return "scene has no elements";
@@ -249,7 +236,7 @@ std::string build_scene(const wob_scene_desc &d, HostScene &S, bool with_bvh) {
         }
     }
     build_cdf(area, S.area_cdf, S.total_area, S.area_guide);
-    if (D == 3) S.area_fused = fused_area_records(S.area_cdf, S.elem_geo);
+    if (D == 3) build_guide(S.area_cdf, S.area_fused_guide, (size_t)kFusedPerElement * n);
     build_cdf(bvw, S.bv_cdf, S.total_bv, S.bv_guide);
     if (!with_bvh) return "";
 

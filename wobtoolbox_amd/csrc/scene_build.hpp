@@ -50,7 +50,7 @@ struct HostScene {
     std::vector<uint32_t> prim_id;     // leaf order -> element id
     std::vector<float> elem_attr;      // n * kAttrWords (element order)
     std::vector<float> elem_geo;       // 3D: n * kGeoWords3 (element order)
-    std::vector<float> area_fused;     // 3D: kFusedPerElement * n records of kGeoWords3 words
+    std::vector<uint32_t> area_fused_guide;   // 3D: the element of each fused area-guide record
     std::vector<float> area_cdf, bv_cdf;
     std::vector<uint32_t> area_guide, bv_guide;
     float total_area = 0, total_bv = 0;
@@ -77,9 +77,9 @@ void build_guide(const std::vector<float> &cdf, std::vector<uint32_t> &guide, si
 // (i = guide[k], cdf[i - 1], cdf[i], cdf[i + 1]) and then element i's geometry (the first
 // three rows of its kGeoWords3 record), 64 bytes in all.  A uniform boundary sample whose
 // search ends at i (about seven times in eight with four entries per element) reads this one
-// record.
+// record.  The host builds the guide (area_fused_guide); the device assembles the records
+// from it, the area CDF and the geometry table (launch_fused_records).
 constexpr int kFusedPerElement = 4;
-std::vector<float> fused_area_records(const std::vector<float> &cdf, const std::vector<float> &geo);
 
 // Guide records for the device search: per k four words (i = guide[k], cdf[i - 1], cdf[i],
 // cdf[i + 1]) (0 where the index is outside the CDF), so that the usual search that ends

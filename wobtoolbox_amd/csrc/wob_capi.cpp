@@ -533,8 +533,10 @@ wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_s
         return st;
     }
     const float *fused = nullptr;
+    const uint32_t *fused_guide = nullptr;
+    const uint32_t n_fused = (uint32_t)h.area_fused_guide.size();
     if ((st = upload(s, 3, h.elem_attr, &attr)) != WOB_OK || (st = upload(s, 9, h.elem_geo, &geo)) != WOB_OK ||
-        (st = upload(s, 10, h.area_fused, &fused)) != WOB_OK ||
+        (st = upload(s, 10, h.area_fused_guide, &fused_guide)) != WOB_OK ||
         (st = upload(s, 4, h.area_cdf, &acdf)) != WOB_OK ||
         (st = upload(s, 5, wobhost::guide_records(h.area_cdf, h.area_guide), &ag)) != WOB_OK ||
         (st = upload(s, 6, h.bv_cdf, &bcdf)) != WOB_OK ||
@@ -549,8 +551,29 @@ wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_s
     s->dev.prim_id = pid;
     s->dev.attr = (const float4 *)attr;
     s->dev.geo = (const float4 *)geo;
+    if (n_fused > 0) {   // slot 10 holds the guide; the records replace it
+        float4 *rec = nullptr;
+        const size_t bytes = (size_t)n_fused * wobhost::kGeoWords3 * sizeof(float);
+        if (hipMalloc(&rec, bytes) != hipSuccess) {
+            free_scene(s);
+            delete s;
+            return fail(WOB_OOM, "hipMalloc fused area records");
+        }
+        hipError_t e = wobl::launch_fused_records(fused_guide, n_fused, acdf, h.n, (const float4 *)geo, rec, 0);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        (void)hipFree(s->bufs[10]);
+        s->device_bytes -= std::max<size_t>(n_fused * sizeof(uint32_t), 16);
+        s->bufs[10] = rec;
+        s->device_bytes += bytes;
+        fused = (const float *)rec;
+        if (e != hipSuccess) {
+            free_scene(s);
+            delete s;
+            return fail(WOB_DEVICE, std::string("fused area records: ") + hipGetErrorString(e));
+        }
+    }
     s->dev.area_fused = (const float4 *)fused;
-    s->dev.n_fused = (uint32_t)(h.area_fused.size() / wobhost::kGeoWords3);
+    s->dev.n_fused = n_fused;
     s->dev.area_cdf = acdf;
     s->dev.area_guide = (const float4 *)ag;
     s->dev.bv_cdf = bcdf;

@@ -321,6 +321,25 @@ hipError_t launch_sample_boundary(int dim, const DevScene &S, uint32_t n, uint64
     return hipGetLastError();
 }
 
+__global__ void fused_records_kernel(const uint32_t *guide, uint32_t m, const float *cdf, uint32_t n, const float4 *geo,
+                                     float4 *out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const uint32_t i = guide[k];
+    out[4 * (size_t)k] = make_float4(__uint_as_float(i), i > 0 ? cdf[i - 1] : 0.f, cdf[i], i + 1 < n ? cdf[i + 1] : 0.f);
+    out[4 * (size_t)k + 1] = geo[4 * (size_t)i];
+    out[4 * (size_t)k + 2] = geo[4 * (size_t)i + 1];
+    out[4 * (size_t)k + 3] = geo[4 * (size_t)i + 2];
+}
+
+hipError_t launch_fused_records(const uint32_t *guide, uint32_t m, const float *cdf, uint32_t n, const float4 *geo,
+                                float4 *out, hipStream_t st) {
+    if (m == 0) return hipSuccess;
+    const uint32_t block = 256, grid = (m + block - 1) / block;
+    hipLaunchKernelGGL(fused_records_kernel, dim3(grid), dim3(block), 0, st, guide, m, cdf, n, geo, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_philox(uint32_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out, hipStream_t st) {
     const uint32_t block = 256, grid = (n + block - 1) / block;
     hipLaunchKernelGGL(philox_kernel, dim3(grid), dim3(block), 0, st, n, ctr, key, out);

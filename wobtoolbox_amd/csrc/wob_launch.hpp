@@ -74,6 +74,10 @@ hipError_t launch_sample_lines(int dim, const wobdev::DevScene &S, uint32_t n, c
 hipError_t launch_sample_boundary(int dim, const wobdev::DevScene &S, uint32_t n, uint64_t seed, uint32_t stream,
                                   float *out, hipStream_t st, int64_t path = -1);
 hipError_t launch_philox(uint32_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out, hipStream_t st);
+// the m fused area-guide records of a 3D scene (4 float4 rows each): row 0 = (i = guide[k],
+// cdf[i - 1], cdf[i], cdf[i + 1]) with 0 outside the CDF, rows 1-3 = geo rows 4 i .. 4 i + 2
+hipError_t launch_fused_records(const uint32_t *guide, uint32_t m, const float *cdf, uint32_t n, const float4 *geo,
+                                float4 *out, hipStream_t st);
 
 // Virtual-point-light aggregation (wob_vpl.hip).  vpl_plan picks the VPL split count and
 // chunk; the stage kernel writes n * vpl_record_floats records; the aggregate kernel writes
