@@ -308,13 +308,84 @@ def poisson_leg(args, torch, wob, G):
         var = np.maximum((q[:, 0] / k).cpu().numpy() - m * m, 0)
         rmse.append(float(np.sqrt(np.mean((m - w.exact(p)[:, 0]) ** 2))))
         se_mean.append(float(np.mean(np.sqrt(var / k))))
-    return {"workload": w.name, "elements": w.elements.n, "point_sets": [len(p) for p in sets], "points": n_points,
-            "paths_per_point_per_pass": paths, "passes": args.poisson_passes,
-            "paths_per_s": n_points * paths * args.poisson_passes / el,
-            "ms_per_pass": el / args.poisson_passes * 1e3, "tuned": best, "tune_s": tune_s,
-            "rmse_u_by_set": rmse, "mean_se_u_by_set": se_mean, "paths_per_point_total": k,
-            "solver": "IndirectDoubleLayer, InteriorDomain, BackwardEstimator, DirichletProblem, path_length 4, "
-                      "eps 1e-4, num_volume_samples 1 (Gaussian-mixture source, inside test per sample)"}
+    out = {"workload": w.name, "elements": w.elements.n, "point_sets": [len(p) for p in sets], "points": n_points,
+           "paths_per_point_per_pass": paths, "passes": args.poisson_passes,
+           "paths_per_s": n_points * paths * args.poisson_passes / el,
+           "ms_per_pass": el / args.poisson_passes * 1e3, "tuned": best, "tune_s": tune_s,
+           "rmse_u_by_set": rmse, "mean_se_u_by_set": se_mean, "paths_per_point_total": k,
+           "solver": "IndirectDoubleLayer, InteriorDomain, BackwardEstimator, DirichletProblem, path_length 4, "
+                     "eps 1e-4, num_volume_samples 1 (Gaussian-mixture source, inside test per sample)"}
+    if not args.no_poisson_cv:
+        out["control_variate"] = poisson_cv(args, torch, wob, w, scene, pts, sets, b, rmse, se_mean)
+    return out
+
+
+def poisson_cv(args, torch, wob, w, scene, pts, sets, plain_batch, plain_rmse, plain_se):
+    """The learned harmonic control variate on the Poisson workload: a fundamental-solution
+    expansion fitted to the walk's own known-term data at boundary samples (its fit on the
+    clock), then the same batched launch with WOB_EST_CONTROL_VARIATE.  Reports its paths/s,
+    RMSE and mean SE per point set beside plain WoB's at the same path count, the per-path
+    variance ratio, and the time of each arm to a common RMSE (progressive passes over all
+    points in one call, the fit on the control variate's clock)."""
+    from wobtoolbox_amd import progressive
+    print("[bench] poisson control variate: fit", file=sys.stderr, flush=True)
+    cvkw = dict(n_sources=args.poisson_cv_sources, radius_scale=1.5, n_train=16384, volume_samples=16,
+                eps=w.solver["eps"], seed=7)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rms_fit = scene.fit_harmonic_cv(**cvkw)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    est = wob.Estimator.from_config(scene, w.solver, flags=2)
+    paths = args.poisson_paths
+    b = wob.Batch([(est, p, paths, j) for j, p in enumerate(pts)], persistent=True, use_graph=True)
+    best, _ = b.tune(seed=1)
+    b.run(1, seed=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run(args.poisson_passes, seed=0)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    n_points = sum(len(p) for p in sets)
+    k = b.passes * paths
+    rmse, se_mean, var_cv, var_plain = [], [], [], []
+    for p, (s, q), (ps, pq) in zip(sets, b.outputs, plain_batch.outputs):
+        m = (s[:, 0] / k).cpu().numpy()
+        var = np.maximum((q[:, 0] / k).cpu().numpy() - m * m, 0)
+        rmse.append(float(np.sqrt(np.mean((m - w.exact(p)[:, 0]) ** 2))))
+        se_mean.append(float(np.mean(np.sqrt(var / k))))
+        var_cv.append(var)
+        mp = (ps[:, 0] / k).cpu().numpy()
+        var_plain.append(np.maximum((pq[:, 0] / k).cpu().numpy() - mp * mp, 0))
+    ratio = float(np.mean(np.concatenate(var_plain)) / np.mean(np.concatenate(var_cv)))
+    # time to a common RMSE, both arms progressive over all points in one call per pass
+    allp = torch.cat(pts)
+    exact_u = torch.as_tensor(w.exact(np.concatenate(sets))[:, 0]).to(allp.device)
+    plain = wob.Estimator.from_config(scene, w.solver)
+    print("[bench] poisson time-to-RMSE: plain arm", file=sys.stderr, flush=True)
+    curve_p = progressive.run(plain, allp, exact_u, args.poisson_ttr_paths, args.poisson_ttr_budget, seed=3)
+    print("[bench] poisson time-to-RMSE: control-variate arm", file=sys.stderr, flush=True)
+    scene.set_harmonic_cv(None)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    scene.fit_harmonic_cv(**cvkw)
+    cvest = wob.Estimator.from_config(scene, w.solver, flags=2)
+    curve_c = progressive.run(cvest, allp, exact_u, args.poisson_ttr_paths, args.poisson_ttr_budget, t0=t0, seed=3)
+    target = max(curve_p[-1][2], curve_c[-1][2])
+    tp = progressive.time_to_rmse(curve_p, target)
+    tc = progressive.time_to_rmse(curve_c, target)
+    return {"kind": "harmonic (fundamental solutions outside the domain, FP64 least squares on the walk's "
+                    "known-term data at boundary samples)",
+            "n_sources": args.poisson_cv_sources, "fit_s": fit_s, "fit_rms_residual": rms_fit,
+            "paths_per_s": n_points * paths * args.poisson_passes / el, "ms_per_pass": el / args.poisson_passes * 1e3,
+            "tuned": best, "rmse_u_by_set": rmse, "mean_se_u_by_set": se_mean,
+            "plain_rmse_u_by_set": plain_rmse, "plain_mean_se_u_by_set": plain_se, "paths_per_point_total": k,
+            "variance_ratio": ratio,
+            "time_to_rmse": {"target_rmse": target, "target_rule": "the larger final RMSE of the two arms",
+                             "plain_s": tp, "cv_s": tc, "speedup": None if (tp is None or tc is None) else tp / tc,
+                             "paths_per_pass": args.poisson_ttr_paths, "points": int(allp.shape[0]),
+                             "plain_curve": [[round(t, 3), kk, r, se_] for t, kk, r, se_ in curve_p],
+                             "cv_curve": [[round(t, 3), kk, r, se_] for t, kk, r, se_ in curve_c]}}
 
 
 # FP32 operations per (point, VPL) pair of the 2D single-layer aggregation (add, mul, cmp,
@@ -420,6 +491,10 @@ def main():
     ap.add_argument("--no-poisson", action="store_true")
     ap.add_argument("--poisson-paths", type=int, default=16, help="paths per point per Poisson pass")
     ap.add_argument("--poisson-passes", type=int, default=4)
+    ap.add_argument("--no-poisson-cv", action="store_true")
+    ap.add_argument("--poisson-cv-sources", type=int, default=64)
+    ap.add_argument("--poisson-ttr-paths", type=int, default=2, help="paths per point per progressive pass")
+    ap.add_argument("--poisson-ttr-budget", type=float, default=4.0, help="walk seconds per time-to-RMSE arm")
     ap.add_argument("--no-vpl", action="store_true")
     ap.add_argument("--vpl-iterations", type=int, default=5)
     ap.add_argument("--no-scene-build", action="store_true")

@@ -147,10 +147,12 @@ typedef struct wob_estimator_desc {
  *   few degrees of that element re-hits it at a larger |t| and adds a spurious hit (a few
  *   percent of bounces), which inflates the variance and makes paths sensitive to rounding. */
 enum { WOB_EST_EXCLUDE_ORIGIN_ELEMENT = 1 };
-/* WOB_EST_CONTROL_VARIATE (single-layer model, backward estimator): use the value table bound
+/* WOB_EST_CONTROL_VARIATE (backward estimator): single-layer model: use the value table bound
  *   with wob_scene_set_control_variate as a control variate.  Every added term has zero mean
  *   for any table, so the estimator's expectation is exactly that of plain WoB; a table close
- *   to the walk's value functions removes most of the path-choice variance.
+ *   to the walk's value functions removes most of the path-choice variance.  Double-layer
+ *   model: use the scene's harmonic function u_h (wob_scene_fit_harmonic_cv): the walk
+ *   estimates u - u_h and the estimate adds u_h.
  * WOB_EST_BOUNDARY_DENSITY (backward estimator, boundary entry points): return the walk's
  *   boundary-density estimate at each record (compute_sample_unknown_boundary,
  *   estimator.cuh:232-357) instead of the solution; this generates control-variate training
@@ -352,6 +354,46 @@ wob_status wob_cv_infer(wob_cv *cv, const float *d_pos, uint32_t n, float *d_out
  * (device) instead; both NULL unbinds.                                                      */
 wob_status wob_scene_set_control_variate(wob_scene *scene, wob_cv *cv, const float *d_table, uint32_t n_values,
                                          void *stream);
+
+/* Learned harmonic control variate (no counterpart in the reference) for double-layer
+ * Dirichlet domain walks, the Poisson problem's estimator (poisson_equation.cu:40-61):
+ *   u_h(x) = c0 + sum_j w_j phi(|x - s_j|^2),  phi(z) = z^(-1/2) (3D) or log2 z (2D),
+ * a combination of fundamental solutions with sources s_j on a sphere (circle) outside the
+ * scene, so u_h is harmonic in the interior domain.  With WOB_EST_CONTROL_VARIATE a
+ * double-layer walk carries the data g - u_h and each domain estimate adds u_h(x) (and
+ * grad u_h(x)): u - u_h solves the same problem with data g - u_h, so the expectation is
+ * that of plain WoB for any u_h (the walk's path-length truncation then applies to the
+ * residual's series), and the variance falls with the size of g - u_h.
+ * Fit: the targets are the walk's own known-term data at n_train uniform boundary samples
+ * (sample i on the Philox stream (seed, path i, point 0, stream_id)): the boundary data plus
+ * the volume term as the estimator samples it (the mean of volume_samples independent
+ * volume samples; 0 without a volume source); (c0, w) minimise the ridge-regularised squared
+ * error in FP64 (columns scaled to unit RMS, ridge relative to the mean diagonal), and the
+ * result is bound to the scene.  Interior domains only (the sources lie outside).           */
+typedef struct wob_hcv_desc {
+    uint32_t n_sources;       /* 1 .. 1024 */
+    float radius_scale;       /* source radius / the scene's bounding radius about its box centre, > 1 */
+    uint32_t n_train;         /* boundary samples, >= 2 (n_sources + 1) */
+    uint32_t volume_samples;  /* volume-term samples per target (the scene's source; 0: data only) */
+    float eps;                /* kernel clamp of the volume term (the walk's eps) */
+    float ridge;              /* relative Tikhonov weight, >= 0 */
+    uint64_t seed;
+    uint32_t stream_id;
+} wob_hcv_desc;
+/* rms_residual (host, may be NULL): RMS of target - u_h over the Dirichlet samples.        */
+wob_status wob_scene_fit_harmonic_cv(wob_scene *scene, const wob_hcv_desc *desc, float *rms_residual, void *stream);
+/* Training data of the fit: d_pos (n_train * dim floats), d_target (n_train floats, NaN for
+ * samples off Dirichlet elements), device buffers.                                           */
+wob_status wob_harmonic_cv_targets(const wob_scene *scene, const wob_hcv_desc *desc, float *d_pos, float *d_target,
+                                   void *stream);
+/* Bind explicit sources (host, n_sources * (dim + 1) floats: s_j then w_j) and c0;
+ * n_sources == 0 unbinds.                                                                   */
+wob_status wob_scene_set_harmonic_cv(wob_scene *scene, uint32_t n_sources, const float *sources, float c0);
+/* The bound function: *n_sources, and (if not NULL) the sources in the layout above and c0. */
+wob_status wob_scene_get_harmonic_cv(const wob_scene *scene, uint32_t *n_sources, float *sources, float *c0);
+/* u_h and grad u_h at n device points (d_points: n * dim, d_out: n * (dim + 1) floats), with
+ * the walk's own arithmetic.                                                                */
+wob_status wob_eval_harmonic_cv(const wob_scene *scene, const float *d_points, uint32_t n, float *d_out, void *stream);
 
 /* Uniform boundary samples as BoundaryPoint records (Scene::sample_boundary_uniform,
  * scene.cuh:441-462 / 734-758): record i uses the Philox stream (seed, path i, point 0,

@@ -288,7 +288,50 @@ template <int D> struct Scene {
     std::vector<Node> nodes;
     VolumeGrid vol;
     VolumeGauss gauss;
+    // harmonic control variate u_h(p) = hcv_c0 + sum_j w_j phi(|p - s_j|^2), phi(z) = z^(-1/2)
+    // (3D) or log2 z (2D); hcv: m records (s_j, w_j) of D + 1 floats (no counterpart in the
+    // reference)
+    std::vector<float> hcv;
+    float hcv_c0 = 0.f;
+    unsigned hcv_m = 0;
 };
+
+// u_h at p, terms added in source order after c0 (FP32)
+template <int D> static float hcv_value(const Scene<D> &s, const float *p) {
+    float acc = s.hcv_c0;
+    for (unsigned j = 0; j < s.hcv_m; ++j) {
+        const float *r = &s.hcv[(size_t)j * (D + 1)];
+        float z = 0.f;
+        for (int k = 0; k < D; ++k) {
+            const float d = p[k] - r[k];
+            z = k == 0 ? d * d : z + d * d;
+        }
+        acc += r[D] * (D == 3 ? 1.0f / std::sqrt(z) : std::log2(z));
+    }
+    return acc;
+}
+// grad u_h: 3D sum_j -w_j (p - s_j) |p - s_j|^-3; 2D (2 / ln 2) sum_j w_j (p - s_j) / |p - s_j|^2
+template <int D> static void hcv_gradient(const Scene<D> &s, const float *p, float *g) {
+    for (int k = 0; k < D; ++k) g[k] = 0.f;
+    for (unsigned j = 0; j < s.hcv_m; ++j) {
+        const float *r = &s.hcv[(size_t)j * (D + 1)];
+        float d[3], z = 0.f;
+        for (int k = 0; k < D; ++k) {
+            d[k] = p[k] - r[k];
+            z = k == 0 ? d[k] * d[k] : z + d[k] * d[k];
+        }
+        float f;
+        if (D == 3) {
+            const float ri = 1.0f / std::sqrt(z);
+            f = -r[D] * (ri * ri * ri);
+        } else {
+            f = r[D] * (1.0f / z);
+        }
+        for (int k = 0; k < D; ++k) g[k] += f * d[k];
+    }
+    if (D == 2)
+        for (int k = 0; k < D; ++k) g[k] = g[k] * 2.8853900817779268f;
+}
 
 // utils.cuh:148-166
 struct Kahan {
@@ -531,6 +574,7 @@ struct Options {
     int pick_mode = 1;      // 0: reference buffer pick, 1: matched min-hash pick
     bool use_bvh = true;
     bool exclude_origin = false;   // ignore the element the line's origin lies on
+    bool control_variate = false;  // double-layer walks: the scene's harmonic control variate
 };
 
 template <int D> static BPoint<D> point_on_element_uniform(const Element<D> &e, Rng &rng) {
@@ -992,7 +1036,13 @@ template <int D> struct Estimator {
             if (dl_dirichlet()) {
                 Sampled<D> q = sample_line_intersection(*s, yp.p, rng, opt, yp.elem);
                 y = q.y; inv = q.inv_pdf;
-                spc += W * m.fredholm_known(yp, y, inv, rng);
+                if (opt.control_variate) {   // the data minus u_h
+                    BPoint<D> yk = yp;
+                    yk.boundary_value = yp.boundary_value - hcv_value(*s, yp.p.v);
+                    spc += W * m.fredholm_known(yk, y, inv, rng);
+                } else {
+                    spc += W * m.fredholm_known(yp, y, inv, rng);
+                }
                 W *= m.fredholm_unknown(yp, y, inv, rng, true);
                 yp = y;
             } else if (direct_neumann()) {
@@ -1040,6 +1090,12 @@ template <int D> struct Estimator {
             out[0] += spc * m.solution_domain_unknown(x, y.y, y.inv_pdf);
             const Vec<D> gu = m.gradient_domain_unknown(x, y.y, y.inv_pdf);
             for (int k = 0; k < D; ++k) out[1 + k] = g[k] + spc * gu[k];
+            if (dl_dirichlet() && opt.control_variate) {   // u = u_h + the walk's estimate of u - u_h
+                float gh[3];
+                hcv_gradient(*s, x.v, gh);
+                out[0] += hcv_value(*s, x.v);
+                for (int k = 0; k < D; ++k) out[1 + k] += gh[k];
+            }
             return;
         }
         // forward :405-454
@@ -1113,6 +1169,7 @@ struct orc_est_desc {
     int32_t pick_mode, rng_mode, use_bvh, exclude_origin;
     int32_t independent_volume;
     int32_t boundary_density;   // boundary entry: return compute_sample_unknown_boundary (the walk)
+    int32_t control_variate;    // double-layer walks: the scene's harmonic control variate
 };
 
 extern "C" {
@@ -1322,6 +1379,7 @@ template <int D> static Estimator<D> make_est(const Scene<D> &s, const orc_est_d
     e.opt.pick_mode = d->pick_mode;
     e.opt.use_bvh = d->use_bvh != 0;
     e.opt.exclude_origin = d->exclude_origin != 0;
+    e.opt.control_variate = d->control_variate != 0;
     return e;
 }
 
@@ -1500,6 +1558,56 @@ int orc_eval_domain_terms(int dim, int bie, int domain, float eps, int n, const 
 
 // One Scene::sample_boundary_uniform draw (scene.cuh:441-462 / 734-758) from the Philox
 // stream (seed, path, point, stream).  out_rec: 2D+4 floats, returns inv_pdf.
+// Harmonic control variate: bind m records (s_j, w_j) of dim + 1 floats and c0 (m = 0 unbinds).
+int orc_scene_set_harmonic_cv(void *p, unsigned m, const float *src, float c0) {
+    auto *S = (OrcScene *)p;
+    auto set = [&](auto &s) {
+        const int D = S->dim;
+        s.hcv.assign(src, src + (size_t)m * (D + 1));
+        s.hcv_c0 = m ? c0 : 0.f;
+        s.hcv_m = m;
+    };
+    if (S->dim == 2) set(S->s2); else set(S->s3);
+    return 0;
+}
+
+// u_h and grad u_h at n points: out n * (dim + 1)
+void orc_eval_harmonic_cv(void *p, int n, const float *pts, float *out) {
+    auto *S = (OrcScene *)p;
+    const int D = S->dim;
+    for (int i = 0; i < n; ++i) {
+        float g[3];
+        const float *x = pts + (size_t)i * D;
+        if (D == 2) { out[(size_t)i * 3] = hcv_value(S->s2, x); hcv_gradient(S->s2, x, g); }
+        else { out[(size_t)i * 4] = hcv_value(S->s3, x); hcv_gradient(S->s3, x, g); }
+        for (int k = 0; k < D; ++k) out[(size_t)i * (D + 1) + 1 + k] = g[k];
+    }
+}
+
+// Training targets of the harmonic control variate: sample i = the uniform boundary point of
+// the Philox stream (seed, path i, point 0, stream), target = its data plus the double-layer
+// known term's volume part with nvs independent volume samples (interior mask), NaN off
+// Dirichlet elements.  pos: n * dim floats.
+void orc_hcv_targets(void *p, int n, uint64_t seed, uint32_t stream, unsigned nvs, float eps, float *pos,
+                     float *target) {
+    auto *S = (OrcScene *)p;
+    auto run = [&](auto &s, auto dimtag) {
+        constexpr int D = decltype(dimtag)::value;
+        Model<D> m;
+        m.bie = IndirectDoubleLayer; m.interior = true; m.dsign = 1.0f; m.k.eps = eps; m.c = 1.f;
+        m.nvs = nvs; m.indep = true; m.scene = &s;
+        for (int i = 0; i < n; ++i) {
+            Rng rng; rng.init_philox(seed, (uint32_t)i, 0u, stream);
+            Sampled<D> y = sample_uniform(s, rng);
+            const float t = y.y.boundary_value + m.vol_term(y.y, rng);
+            for (int k = 0; k < D; ++k) pos[(size_t)i * D + k] = y.y.p[k];
+            target[i] = y.y.boundary_type == Dirichlet ? t : std::numeric_limits<float>::quiet_NaN();
+        }
+    };
+    if (S->dim == 2) run(S->s2, std::integral_constant<int, 2>{});
+    else run(S->s3, std::integral_constant<int, 3>{});
+}
+
 float orc_sample_uniform(void *p, uint64_t seed, uint32_t path, uint32_t point, uint32_t stream, float *out_rec) {
     auto *S = (OrcScene *)p;
     Rng rng; rng.init_philox(seed, path, point, stream);

@@ -104,3 +104,55 @@ def boundary_records(el, n, rng):
 def mean_se(s, q, n):
     m = s / n
     return m, np.sqrt(np.maximum(q / n - m * m, 0) / n)
+
+
+def hcv_sources(el, m, radius_scale):
+    """Source positions of the harmonic control variate: a Fibonacci sphere (3D) or equal
+    angles on a circle (2D) of radius radius_scale x (largest vertex distance from the centre
+    of the vertex box), about that centre (FP64)."""
+    D = el.dim
+    V = el.vertices.reshape(-1, D).astype(np.float64)
+    c = 0.5 * (V.min(0) + V.max(0))
+    R = radius_scale * np.sqrt(((V - c) ** 2).sum(1).max())
+    j = np.arange(m) + 0.5
+    if D == 2:
+        a = 2 * np.pi * j / m
+        return c + R * np.stack([np.cos(a), np.sin(a)], 1)
+    z = 1 - 2 * j / m
+    r = np.sqrt(np.maximum(0, 1 - z * z))
+    ph = np.pi * (1 + np.sqrt(5.0)) * j
+    return c + R * np.stack([r * np.cos(ph), r * np.sin(ph), z], 1)
+
+
+def hcv_basis(pos, src):
+    """[1, phi(|x - s_j|^2)] with phi = z^(-1/2) (3D) or log2 z (2D), FP64."""
+    z = ((np.asarray(pos, np.float64)[:, None, :] - src[None]) ** 2).sum(2)
+    phi = 1 / np.sqrt(z) if src.shape[1] == 3 else np.log2(z)
+    return np.concatenate([np.ones((len(z), 1)), phi], 1)
+
+
+def hcv_eval_f64(pos, src, w, c0):
+    """u_h and grad u_h in FP64: (n, dim + 1)."""
+    x = np.asarray(pos, np.float64)
+    d = x[:, None, :] - src[None]
+    z = (d ** 2).sum(2)
+    if src.shape[1] == 3:
+        u = c0 + (w[None] / np.sqrt(z)).sum(1)
+        g = -(w[None, :, None] * d / z[:, :, None] ** 1.5).sum(1)
+    else:
+        u = c0 + (w[None] * np.log2(z)).sum(1)
+        g = (2 / np.log(2)) * (w[None, :, None] * d / z[:, :, None]).sum(1)
+    return np.concatenate([u[:, None], g], 1)
+
+
+def hcv_fit_numpy(pos, tgt, src, ridge=1e-10):
+    """Ridge least squares of the targets on [1, phi_j] with unit-RMS columns (FP64):
+    returns (c0, w)."""
+    ok = np.isfinite(tgt)
+    B = hcv_basis(pos[ok], src)
+    sc = 1 / np.sqrt((B ** 2).mean(0))
+    Bs = B * sc
+    A = Bs.T @ Bs
+    A += ridge * np.trace(A) / len(A) * np.eye(len(A))
+    coef = np.linalg.solve(A, Bs.T @ tgt[ok].astype(np.float64)) * sc
+    return coef[0], coef[1:]

@@ -1,0 +1,251 @@
+"""GPU parity of the learned harmonic control variate (double-layer Dirichlet walks, the
+Poisson problem's estimator) through the C ABI.
+
+  * u_h and grad u_h on the device against the oracle and the FP64 closed form;
+  * the fit's training targets against the oracle on the same Philox streams (positions bit
+    for bit; targets to FP32 rounding of the volume term);
+  * the fit recovers a field it represents exactly, and agrees with an FP64 least-squares
+    solve of its own targets;
+  * the walk with the control variate against the oracle path by path, in single calls,
+    batches and the wavefront schedule;
+  * unbiasedness against the closed-form solution u = N_f + H and the variance it removes.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import hcv_eval_f64, hcv_fit_numpy, hcv_sources, mean_se
+from wobtoolbox_amd import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+DL = dict(bie_model="IndirectDoubleLayer", domain="InteriorDomain", estimator="BackwardEstimator",
+          problem="DirichletProblem", eps=1e-4)
+
+
+def _dev(a, gpu):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a)).to(gpu)
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _poisson(level=3):
+    V, F = G.icosphere(level)
+    V = G.displace_radial(V)
+    c, wd, a = G.gaussian_sources_3d()
+    ex = G.poisson_exact_3d(c, wd, a)
+    el = G.mesh_elements(V, F, G.DIRICHLET, lambda p, nr, ty: ex(p)[:, 0])
+    return el, (c, wd, a), ex
+
+
+def _scenes(el, src, oracle):
+    import wobtoolbox_amd as wob
+    scene = wob.SceneHost(el)
+    osc = oracle.OracleScene(el)
+    if src is not None:
+        scene.set_volume_gaussians(*src)
+        osc.set_volume_gaussians(*src)
+    return scene, osc
+
+
+def test_hcv_eval_matches_oracle_and_closed_form(gpu, oracle):
+    rng = np.random.default_rng(0)
+    for dim in (2, 3):
+        zero = lambda p, nr, ty: np.zeros(len(p))  # noqa: E731
+        th = 2 * np.pi * np.arange(64) / 64
+        el = (G.polygon_elements(np.stack([np.cos(th), np.sin(th)], 1), G.DIRICHLET, zero) if dim == 2 else
+              G.mesh_elements(*G.icosphere(1), G.DIRICHLET, zero))
+        scene, osc = _scenes(el, None, oracle)
+        src = rng.normal(size=(48, dim))
+        src = 2.5 * src / np.linalg.norm(src, axis=1, keepdims=True)
+        w = rng.uniform(-1, 1, 48)
+        rec = np.concatenate([src, w[:, None]], 1).astype(np.float32)
+        scene.set_harmonic_cv(rec, 0.25)
+        osc.set_harmonic_cv(rec, 0.25)
+        got_src, got_c0 = scene.harmonic_cv()
+        assert np.array_equal(got_src, rec) and got_c0 == np.float32(0.25)
+        pts = rng.uniform(-0.8, 0.8, (2000, dim)).astype(np.float32)
+        d = scene.eval_harmonic_cv(_dev(pts, gpu)).cpu().numpy().astype(np.float64)
+        o = osc.eval_harmonic_cv(pts).astype(np.float64)
+        ref = hcv_eval_f64(pts, rec[:, :dim].astype(np.float64), rec[:, dim].astype(np.float64), np.float32(0.25))
+        z = ((pts[:, None, :].astype(np.float64) - rec[None, :, :dim]) ** 2).sum(2)
+        mag = 0.25 + (np.abs(w)[None] * (1 / np.sqrt(z) if dim == 3 else np.abs(np.log2(z)))).sum(1)
+        gmag = (np.abs(w)[None] / (z if dim == 3 else np.sqrt(z))).sum(1) * (1 if dim == 3 else 2 / np.log(2))
+        err_d = np.abs(d[:, 0] - ref[:, 0]) / mag
+        err_o = np.abs(d[:, 0] - o[:, 0]) / mag
+        gerr = np.abs(d[:, 1:] - ref[:, 1:]) / gmag[:, None]
+        _say(f"u_h {dim}D: device vs FP64 {err_d.max():.2e}, vs oracle {err_o.max():.2e} (of the term scale); "
+             f"gradient {gerr.max():.2e}")
+        assert err_d.max() < 2e-6 and err_o.max() < 2e-6 and gerr.max() < 4e-6
+
+
+def test_hcv_targets_match_oracle(gpu, oracle):
+    el, src, _ = _poisson(3)
+    scene, osc = _scenes(el, src, oracle)
+    pos, tgt = scene.harmonic_cv_targets(4096, volume_samples=4, eps=1e-4, seed=7, stream_id=3)
+    pos, tgt = pos.cpu().numpy(), tgt.cpu().numpy()
+    opos, otgt = osc.hcv_targets(4096, seed=7, stream=3, volume_samples=4, eps=1e-4)
+    assert np.array_equal(pos, opos)
+    diff = np.abs(tgt - otgt)
+    _say(f"targets: max |device - oracle| {diff.max():.2e}, over 1e-5 relative {(diff > 1e-5 * (1 + np.abs(otgt))).mean()}")
+    assert (diff > 1e-5 * (1 + np.abs(otgt))).mean() < 0.01 and diff.max() < 1e-3
+    # without a volume source the targets are the data at the samples
+    scene2, _ = _scenes(el, None, oracle)
+    p2, t2 = scene2.harmonic_cv_targets(2048, volume_samples=0, seed=7, stream_id=3)
+    op2, ot2 = osc.hcv_targets(2048, seed=7, stream=3, volume_samples=0)
+    assert np.array_equal(p2.cpu().numpy(), op2) and np.array_equal(t2.cpu().numpy(), ot2)
+
+
+def test_hcv_fit_recovers_a_representable_field(gpu, oracle):
+    """Data equal to a combination of the fit's own basis functions: the fit reproduces it."""
+    V, F = G.icosphere(3)
+    V = G.displace_radial(V)
+    el0 = G.mesh_elements(V, F, G.DIRICHLET, lambda p, nr, ty: np.zeros(len(p)))
+    src = hcv_sources(el0, 32, 1.5)
+    w = np.random.default_rng(3).uniform(-0.3, 0.3, 32)
+
+    def h(p):
+        return hcv_eval_f64(p, src, w, 0.5)
+    # the data of each sample point is the linear interpolant of h at the element's vertices,
+    # so the fit sees h up to that interpolation; a finer mesh keeps it small
+    V4, F4 = G.icosphere(5)
+    el = G.mesh_elements(G.displace_radial(V4), F4, G.DIRICHLET, lambda p, nr, ty: h(p)[:, 0])
+    import wobtoolbox_amd as wob
+    scene = wob.SceneHost(el)
+    rms = scene.fit_harmonic_cv(n_sources=32, radius_scale=1.5, n_train=8192, volume_samples=0, seed=1)
+    pts = G.interior_points_star(512, np.random.default_rng(4), margin=0.05).astype(np.float32)
+    got = scene.eval_harmonic_cv(_dev(pts, gpu)).cpu().numpy()
+    err = np.abs(got[:, 0] - h(pts)[:, 0]).max()
+    _say(f"representable field: fit residual {rms:.2e}, interior error {err:.2e} (|h| ~ {np.abs(h(pts)[:, 0]).max():.2f})")
+    assert rms < 1e-4 and err < 2e-4
+
+
+def test_hcv_fit_matches_f64_least_squares(gpu, oracle):
+    el, srcv, _ = _poisson(3)
+    import wobtoolbox_amd as wob
+    scene = wob.SceneHost(el)
+    scene.set_volume_gaussians(*srcv)
+    pos, tgt = scene.harmonic_cv_targets(8192, volume_samples=8, eps=1e-4, seed=2)
+    rms = scene.fit_harmonic_cv(n_sources=64, radius_scale=1.5, n_train=8192, volume_samples=8, eps=1e-4, seed=2)
+    src = hcv_sources(el, 64, 1.5)
+    c0, w = hcv_fit_numpy(pos.cpu().numpy(), tgt.cpu().numpy(), src)
+    got_src, got_c0 = scene.harmonic_cv()
+    assert np.allclose(got_src[:, :3], src, rtol=0, atol=1e-6)
+    pts = G.interior_points_star(256, np.random.default_rng(4), margin=0.02).astype(np.float32)
+    dev = scene.eval_harmonic_cv(_dev(pts, gpu)).cpu().numpy()[:, 0]
+    ref = hcv_eval_f64(pts, src, w, c0)[:, 0]
+    _say(f"fit vs FP64 least squares: max |u_h - u_ref| {np.abs(dev - ref).max():.2e}; residual {rms:.4f}")
+    assert np.abs(dev - ref).max() < 1e-4 * (1 + np.abs(ref).max())
+
+
+def _fit_both(scene, osc, n_sources=64):
+    rms = scene.fit_harmonic_cv(n_sources=n_sources, radius_scale=1.5, n_train=16384, volume_samples=16, eps=1e-4,
+                                seed=5)
+    rec, c0 = scene.harmonic_cv()
+    osc.set_harmonic_cv(rec, c0)
+    return rms
+
+
+def test_hcv_walk_matches_oracle(gpu, oracle):
+    """Path by path on the same streams, origin element excluded from line queries (so the two
+    sides take the same paths to FP32 rounding), and in the reference's own behaviour."""
+    import wobtoolbox_amd as wob
+    el, srcv, _ = _poisson(3)
+    scene, osc = _scenes(el, srcv, oracle)
+    _fit_both(scene, osc)
+    pts = G.interior_points_star(64, np.random.default_rng(7), margin=0.05).astype(np.float32)
+    paths = 256
+    for flags in (1 | 2, 2):
+        solver = dict(DL, path_length=4, num_volume_samples=1, flags=flags)
+        est = wob.Estimator.from_config(scene, solver)
+        s, q = est.compute_domain(_dev(pts, gpu), paths, seed=11)
+        m, se = mean_se(s.cpu().numpy(), q.cpu().numpy(), paths)
+        so, qo = osc.estimate_domain(solver, pts, paths, seed=11, pick_mode=1)
+        mo, seo = mean_se(so, qo, paths)
+        z = np.abs(m - mo) / np.sqrt(se ** 2 + seo ** 2 + 1e-30)
+        frac = (np.abs(m - mo) > 1e-4 * (np.abs(mo) + se + 1e-3)).mean()
+        _say(f"control-variate walk flags {flags}: max |z| {z.max():.3f}, points over 1e-4 rel {frac:.3f}")
+        assert z.max() < 0.05 and frac < 0.02
+
+
+def test_hcv_batch_and_wavefront_bit_identical(gpu, oracle):
+    """Batches and the wavefront schedule give the single call's sums bit for bit."""
+    import wobtoolbox_amd as wob
+    el, srcv, _ = _poisson(3)
+    scene, osc = _scenes(el, srcv, oracle)
+    _fit_both(scene, osc, 32)
+    sets = [G.interior_points_star(n, np.random.default_rng(20 + n), margin=0.05).astype(np.float32)
+            for n in (100, 333, 1000)]
+    solver = dict(DL, path_length=4, num_volume_samples=1, flags=2)
+    est = wob.Estimator.from_config(scene, solver)
+    single = [est.compute_domain(_dev(p, gpu), 32, seed=5, stream_id=j) for j, p in enumerate(sets)]
+    b = wob.Batch([(est, _dev(p, gpu), 32, j) for j, p in enumerate(sets)], persistent=True)
+    b.run(1, seed=5)
+    for (s, q), (bs, bq) in zip(single, b.outputs):
+        assert np.array_equal(s.cpu().numpy(), bs.cpu().numpy()) and np.array_equal(q.cpu().numpy(), bq.cpu().numpy())
+    # wavefront schedule (no volume source: the wavefront walk covers walks without one)
+    V, F = G.icosphere(3)
+    el2 = G.mesh_elements(G.displace_radial(V), F, G.DIRICHLET, lambda p, nr, ty: G.exact_expcos_xz_3d(p)[:, 0])
+    sc2 = wob.SceneHost(el2)
+    sc2.fit_harmonic_cv(n_sources=32, n_train=4096, seed=1)
+    e2 = wob.Estimator.from_config(sc2, dict(DL, path_length=4, flags=2))
+    p = _dev(sets[1], gpu)
+    ref = e2.compute_domain(p, 64, seed=9)
+    for thr in (1, 32):
+        e2.schedule, e2.regen_threshold = 2, thr
+        got = e2.compute_domain(p, 64, seed=9)
+        assert np.array_equal(ref[0].cpu().numpy(), got[0].cpu().numpy())
+        assert np.array_equal(ref[1].cpu().numpy(), got[1].cpu().numpy())
+
+
+def test_hcv_unbiased_and_variance(gpu):
+    """Against u = N_f + H on a level-4 icosphere: both estimators within their standard errors,
+    the control variate's variance far below plain WoB's."""
+    import wobtoolbox_amd as wob
+    el, srcv, ex = _poisson(4)
+    scene = wob.SceneHost(el)
+    scene.set_volume_gaussians(*srcv)
+    rms = scene.fit_harmonic_cv(n_sources=64, radius_scale=1.5, n_train=16384, volume_samples=16, eps=1e-4, seed=5)
+    pts = G.interior_points_star(256, np.random.default_rng(9), margin=0.05).astype(np.float32)
+    n = 8192
+    var = {}
+    for flags in (0, 2):
+        est = wob.Estimator.from_config(scene, dict(DL, path_length=4, num_volume_samples=1, flags=flags))
+        s, q = est.compute_domain(_dev(pts, gpu), n, seed=13)
+        m, se = mean_se(s.cpu().numpy(), q.cpu().numpy(), n)
+        z = (m - ex(pts)) / se
+        var[flags] = float(np.mean(se[:, 0] ** 2))
+        _say(f"flags {flags}: max |z| u {np.abs(z[:, 0]).max():.2f}, grad {np.abs(z[:, 1:]).max():.2f}, "
+             f"mean z {z[:, 0].mean():+.3f}, per-path variance {var[flags] * n:.4f} (fit residual {rms:.4f})")
+        assert np.abs(z[:, 0]).max() < 4.5 and np.abs(z[:, 1:]).max() < 5.0
+        assert abs(z[:, 0].mean()) < 4 / np.sqrt(len(pts)) * 1.5
+    assert var[0] / var[2] > 40, var
+
+
+def test_hcv_errors(gpu):
+    import wobtoolbox_amd as wob
+    el, _, _ = _poisson(2)
+    scene = wob.SceneHost(el)
+    est = wob.Estimator.from_config(scene, dict(DL, path_length=4, flags=2))
+    pts = _dev(np.zeros((4, 3), np.float32), gpu)
+    with pytest.raises(wob.WobError, match="harmonic control variate"):
+        est.compute_domain(pts, 4)
+    with pytest.raises(wob.WobError, match="volume source"):
+        from wobtoolbox_amd import _capi
+        import ctypes as C
+        d = _capi.HcvDesc(8, 1.5, 64, 4, 1e-4, 0.0, 0, 0)
+        _capi.check(_capi.lib().wob_scene_fit_harmonic_cv(scene.handle, C.byref(d), None, None))
+    with pytest.raises(wob.WobError, match="radius_scale"):
+        scene.fit_harmonic_cv(n_sources=8, radius_scale=0.9, n_train=64)
+    scene.fit_harmonic_cv(n_sources=8, n_train=64)
+    s, _ = est.compute_domain(pts, 4)
+    assert np.isfinite(s.cpu().numpy()).all()
+    scene.set_harmonic_cv(None)
+    assert scene.harmonic_cv()[0].shape[0] == 0
+    with pytest.raises(wob.WobError):
+        est.compute_domain(pts, 4)

@@ -90,6 +90,7 @@ class SceneHost:
         flags = _capi.SCENE_DEVICE_BVH if device_bvh else 0
         check(lib().wob_scene_create_ex(C.byref(d), flags, C.byref(h)))
         self._h = h
+        self._has_volume = False
 
     @property
     def handle(self):
@@ -134,11 +135,13 @@ class SceneHost:
         scene.cuh:301-306); cache: grid_res^dim values, x fastest.  None removes the source."""
         if cache is None:
             check(lib().wob_scene_set_volume_grid(self._h, 0, 0.0, None))
+            self._has_volume = False
             return
         c = np.ascontiguousarray(cache, np.float32).reshape(-1)
         if c.size != grid_res ** self.dim:
             raise ValueError(f"cache has {c.size} values, expected grid_res^dim = {grid_res ** self.dim}")
         check(lib().wob_scene_set_volume_grid(self._h, int(grid_res), float(domain_size), c.ctypes.data))
+        self._has_volume = True
 
     def set_volume_gaussians(self, centers, widths, amplitudes, mask=True):
         """Gaussian-mixture source sum_j A_j exp(-|x - c_j|^2 / (2 w_j^2)); with mask, samples
@@ -150,6 +153,63 @@ class SceneHost:
             raise ValueError("centers, widths and amplitudes must have one entry per Gaussian")
         check(lib().wob_scene_set_volume_gaussians(self._h, len(w), c.ctypes.data, w.ctypes.data, a.ctypes.data,
                                                    0 if mask else _capi.VOLUME_NO_MASK))
+        self._has_volume = len(w) > 0
+
+    @staticmethod
+    def _hcv_desc(n_sources, radius_scale, n_train, volume_samples, eps, ridge, seed, stream_id):
+        return _capi.HcvDesc(int(n_sources), float(radius_scale), int(n_train), int(volume_samples), float(eps),
+                             float(ridge), int(seed), int(stream_id))
+
+    def fit_harmonic_cv(self, n_sources=64, radius_scale=1.5, n_train=16384, volume_samples=16, eps=1e-4,
+                        ridge=1e-10, seed=0, stream_id=0, stream=None):
+        """Fit and bind the harmonic control variate u_h = c0 + sum_j w_j phi(|x - s_j|^2) of
+        double-layer Dirichlet walks (wob_scene_fit_harmonic_cv) to the walk's known-term data
+        at n_train uniform boundary samples; volume_samples is ignored without a volume source.
+        Returns the RMS residual of the fit."""
+        _torch()
+        if not self._has_volume:
+            volume_samples = 0
+        d = self._hcv_desc(n_sources, radius_scale, n_train, volume_samples, eps, ridge, seed, stream_id)
+        rms = C.c_float()
+        check(lib().wob_scene_fit_harmonic_cv(self._h, C.byref(d), C.byref(rms), _stream_ptr(stream)))
+        return rms.value
+
+    def harmonic_cv_targets(self, n_train=16384, volume_samples=16, eps=1e-4, seed=0, stream_id=0, n_sources=1,
+                            stream=None):
+        """The fit's training data: (positions (n, dim), targets (n,), NaN off Dirichlet elements)."""
+        torch = _torch()
+        if not self._has_volume:
+            volume_samples = 0
+        d = self._hcv_desc(n_sources, 1.5, n_train, volume_samples, eps, 0.0, seed, stream_id)
+        pos = torch.zeros((n_train, self.dim), dtype=torch.float32, device="cuda")
+        tgt = torch.zeros(n_train, dtype=torch.float32, device="cuda")
+        check(lib().wob_harmonic_cv_targets(self._h, C.byref(d), _ptr(pos), _ptr(tgt), _stream_ptr(stream)))
+        return pos, tgt
+
+    def set_harmonic_cv(self, sources=None, c0=0.0):
+        """Bind explicit sources (n, dim + 1): positions then weights; None unbinds."""
+        if sources is None:
+            check(lib().wob_scene_set_harmonic_cv(self._h, 0, None, 0.0))
+            return
+        s = np.ascontiguousarray(sources, np.float32).reshape(-1, self.dim + 1)
+        check(lib().wob_scene_set_harmonic_cv(self._h, len(s), s.ctypes.data, float(c0)))
+
+    def harmonic_cv(self):
+        """(sources (n, dim + 1), c0) of the bound harmonic control variate (n = 0: none)."""
+        m = C.c_uint32()
+        check(lib().wob_scene_get_harmonic_cv(self._h, C.byref(m), None, None))
+        s = np.zeros((m.value, self.dim + 1), np.float32)
+        c0 = C.c_float()
+        check(lib().wob_scene_get_harmonic_cv(self._h, C.byref(m), s.ctypes.data if m.value else None, C.byref(c0)))
+        return s, c0.value
+
+    def eval_harmonic_cv(self, points, stream=None):
+        """u_h and grad u_h at device points (n, dim) with the walk's arithmetic: (n, dim + 1)."""
+        torch = _torch()
+        pts = points.contiguous()
+        out = torch.zeros((pts.shape[0], self.dim + 1), dtype=torch.float32, device=pts.device)
+        check(lib().wob_eval_harmonic_cv(self._h, _ptr(pts), pts.shape[0], _ptr(out), _stream_ptr(stream)))
+        return out
 
     def intersect_lines(self, origins, dirs, max_hits=64, stream=None):
         """All-hit line query: (counts, ids, t) for device tensors of lines."""

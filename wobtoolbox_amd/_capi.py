@@ -23,7 +23,8 @@ EXPORTS = ["wob_scene_create", "wob_scene_create_ex", "wob_scene_destroy", "wob_
            "wob_batch_destroy", "wob_cv_create", "wob_cv_destroy", "wob_cv_get_info", "wob_cv_get_params",
            "wob_cv_set_params", "wob_cv_train", "wob_cv_gradient", "wob_cv_infer", "wob_scene_set_control_variate",
            "wob_sample_boundary", "wob_sample_lines", "wob_vpl_paths", "wob_vpl_aggregate",
-           "wob_closest_points", "wob_wos_estimate"]
+           "wob_closest_points", "wob_wos_estimate", "wob_scene_fit_harmonic_cv", "wob_harmonic_cv_targets",
+           "wob_scene_set_harmonic_cv", "wob_scene_get_harmonic_cv", "wob_eval_harmonic_cv"]
 
 
 class WobError(RuntimeError):
@@ -49,6 +50,12 @@ class EstimatorDesc(C.Structure):
                 ("num_resampling_candidates", C.c_uint32), ("num_volume_samples", C.c_uint32), ("eps", C.c_float),
                 ("first_kind_equation_scaling_constant", C.c_float), ("flags", C.c_uint32),
                 ("cv_samples", C.c_uint32)]
+
+
+class HcvDesc(C.Structure):
+    _fields_ = [("n_sources", C.c_uint32), ("radius_scale", C.c_float), ("n_train", C.c_uint32),
+                ("volume_samples", C.c_uint32), ("eps", C.c_float), ("ridge", C.c_float), ("seed", C.c_uint64),
+                ("stream_id", C.c_uint32)]
 
 
 EST_EXCLUDE_ORIGIN_ELEMENT = 1
@@ -136,6 +143,11 @@ def lib():
         "wob_vpl_aggregate": ([P, C.POINTER(EstimatorDesc), P, P, U32, P, U32, P, P], C.c_int),
         "wob_closest_points": ([P, P, U32, P, P, P, P], C.c_int),
         "wob_wos_estimate": ([P, P, U32, U32, U32, U64, U64, U32, F, U32, P, P, P], C.c_int),
+        "wob_scene_fit_harmonic_cv": ([P, C.POINTER(HcvDesc), C.POINTER(F), P], C.c_int),
+        "wob_harmonic_cv_targets": ([P, C.POINTER(HcvDesc), P, P, P], C.c_int),
+        "wob_scene_set_harmonic_cv": ([P, U32, P, F], C.c_int),
+        "wob_scene_get_harmonic_cv": ([P, C.POINTER(U32), P, C.POINTER(F)], C.c_int),
+        "wob_eval_harmonic_cv": ([P, P, U32, P, P], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -35,6 +35,7 @@ SOURCES = [
     ("mlp_kernels.hip", "hip"),
     ("wob_capi.cpp", "hip"),
     ("wob_cv.cpp", "hip"),
+    ("wob_hcv.cpp", "hip"),
     ("scene_build.cpp", "host"),
 ]
 

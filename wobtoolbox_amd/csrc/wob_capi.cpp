@@ -48,6 +48,7 @@ struct wob_scene {
     uint64_t device_bytes = 0;
     int device = 0;
     float *cv_table = nullptr;
+    float4 *hcv_src = nullptr;
     void *vol_bufs[3] = {nullptr, nullptr, nullptr};
     uint64_t version = 0;   // bumped whenever device data referenced by launches changes
     double *aux = nullptr;  // FP64 accumulators of VPL path phases
@@ -55,6 +56,17 @@ struct wob_scene {
 };
 
 const wobhost::HostScene &wob_scene_host(const wob_scene *scene) { return scene->host; }
+const wobdev::DevScene &wob_scene_dev(const wob_scene *scene) { return scene->dev; }
+
+wob_status wob_scene_bind_hcv(wob_scene *scene, float4 *d_src, uint32_t m, float c0) {
+    if (scene->hcv_src) (void)hipFree(scene->hcv_src);
+    scene->hcv_src = d_src;
+    scene->dev.hcv_src = d_src;
+    scene->dev.hcv_m = d_src ? m : 0u;
+    scene->dev.hcv_c0 = d_src ? c0 : 0.f;
+    scene->version += 1;
+    return WOB_OK;
+}
 
 wob_status wob_scene_bind_cv_table(wob_scene *scene, float *d_table, uint32_t R) {
     if (scene->cv_table) (void)hipFree(scene->cv_table);
@@ -96,6 +108,8 @@ void free_scene(wob_scene *s) {
     s->counter = nullptr;
     if (s->cv_table) (void)hipFree(s->cv_table);
     s->cv_table = nullptr;
+    if (s->hcv_src) (void)hipFree(s->hcv_src);
+    s->hcv_src = nullptr;
     if (s->aux) (void)hipFree(s->aux);
     s->aux = nullptr;
 }
@@ -138,10 +152,14 @@ wob_status check_est(const wob_scene *scene, const wob_estimator_desc *est, int 
                                  WOB_EST_INDEPENDENT_VOLUME_SAMPLES))
         return fail(WOB_INVALID_ARG, "unknown estimator flags");
     if (est->flags & WOB_EST_CONTROL_VARIATE) {
-        if (est->bie_model != WOB_SINGLE_LAYER || est->estimator != WOB_BACKWARD)
-            return fail(WOB_UNSUPPORTED_COMBO, "the control variate needs the single-layer model and the backward estimator");
-        if (scene->dev.cv_R < est->path_length)
+        if (est->estimator != WOB_BACKWARD || est->bie_model == WOB_DIRECT)
+            return fail(WOB_UNSUPPORTED_COMBO, "the control variate needs the backward estimator and the single-layer "
+                                               "(value table) or double-layer (harmonic function) model");
+        if (est->bie_model == WOB_SINGLE_LAYER && scene->dev.cv_R < est->path_length)
             return fail(WOB_INVALID_ARG, "no control-variate table with path_length value functions is bound to the scene");
+        if (est->bie_model == WOB_DOUBLE_LAYER && scene->dev.hcv_m == 0)
+            return fail(WOB_INVALID_ARG, "no harmonic control variate is bound to the scene "
+                                         "(wob_scene_fit_harmonic_cv / wob_scene_set_harmonic_cv)");
     }
     if ((est->flags & WOB_EST_BOUNDARY_DENSITY) && est->estimator != WOB_BACKWARD)
         return fail(WOB_UNSUPPORTED_COMBO, "boundary density estimates need the backward estimator");

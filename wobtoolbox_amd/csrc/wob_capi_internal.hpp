@@ -1,4 +1,4 @@
-// Helpers shared by the C-ABI translation units (wob_capi.cpp, wob_cv.cpp).
+// Helpers shared by the C-ABI translation units (wob_capi.cpp, wob_cv.cpp, wob_hcv.cpp).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,6 +7,7 @@
 
 #include "../../include/wob_c.h"
 #include "scene_build.hpp"
+#include "wob_device.hpp"
 
 // record a message for wob_last_error_message() and return the status
 wob_status wob_fail(wob_status s, const std::string &msg);
@@ -21,3 +22,7 @@ wob_status wob_hip_fail(hipError_t e, const char *what);
 const wobhost::HostScene &wob_scene_host(const wob_scene *scene);
 // takes ownership of d_table (device, [R][n][dim] floats); frees the previous table
 wob_status wob_scene_bind_cv_table(wob_scene *scene, float *d_table, uint32_t R);
+const wobdev::DevScene &wob_scene_dev(const wob_scene *scene);
+// takes ownership of d_src (device, m float4 records); frees the previous sources; m == 0
+// (d_src == nullptr) unbinds
+wob_status wob_scene_bind_hcv(wob_scene *scene, float4 *d_src, uint32_t m, float c0);

@@ -75,6 +75,12 @@ struct DevScene {
     uint32_t cv_R;            // number of value functions in cv_table (0: none)
     const float *cv_table;    // control variate values per (r - 1, element, vertex): [cv_R][n][D]
     VolSrc vol;
+    // harmonic control variate of double-layer Dirichlet walks (0 sources: none):
+    // u_h(p) = hcv_c0 + sum_j w_j phi(|p - s_j|^2), phi(z) = z^(-1/2) (3D) or log2 z (2D);
+    // hcv_src[j] = (s_j, w_j) in 3D, (s_j, w_j, 0) in 2D
+    uint32_t hcv_m;
+    float hcv_c0;
+    const float4 *hcv_src;
 };
 
 // Scene pointers reach a kernel either in its argument or through an entry table in global
@@ -1394,7 +1400,7 @@ struct WalkParams {
     uint32_t path_length;
     uint32_t nrc;
     uint32_t exclude_origin;   // WOB_EST_EXCLUDE_ORIGIN_ELEMENT
-    uint32_t cv;               // WOB_EST_CONTROL_VARIATE (single-layer backward walks)
+    uint32_t cv;               // WOB_EST_CONTROL_VARIATE (single layer: value table; double layer: u_h)
     uint32_t cv_samples;       // samples of the domain-level control-variate integral
     uint32_t density;          // WOB_EST_BOUNDARY_DENSITY: boundary estimates return the walk value
     __device__ __forceinline__ uint32_t skip(uint32_t elem) const { return exclude_origin ? elem : 0xFFFFFFFFu; }
@@ -1409,6 +1415,57 @@ template <int D> __device__ __forceinline__ float cv_value(const DevScene &S, ui
 #pragma unroll
     for (int k = 1; k < D; ++k) v += b.bw[k] * t[k];
     return v;
+}
+
+// Harmonic control variate u_h and its gradient at p.  The sources are the same for every
+// lane of a wave, so their records are read through the constant address space (scalar
+// loads).  Terms are added in source order after c0; phi uses the hardware reciprocal square
+// root (3D) or base-2 logarithm (2D).
+__device__ __forceinline__ float4 hcv_source(const float4 *src, uint32_t j) {
+    const gf4_t v = *(const __attribute__((address_space(4))) gf4_t *)(src + j);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+template <int D> __device__ __forceinline__ float hcv_value(const DevScene &S, const float *p) {
+    float acc = S.hcv_c0;
+    for (uint32_t j = 0; j < S.hcv_m; ++j) {
+        const float4 s = hcv_source(S.hcv_src, j);
+        if constexpr (D == 3) {
+            const float d0 = p[0] - s.x, d1 = p[1] - s.y, d2 = p[2] - s.z;
+            const float z = d0 * d0 + d1 * d1 + d2 * d2;
+            acc += s.w * __builtin_amdgcn_rsqf(z);
+        } else {
+            const float d0 = p[0] - s.x, d1 = p[1] - s.y;
+            const float z = d0 * d0 + d1 * d1;
+            acc += s.z * __builtin_amdgcn_logf(z);
+        }
+    }
+    return acc;
+}
+// grad u_h: 3D sum_j -w_j (p - s_j) |p - s_j|^-3; 2D (2 / ln 2) sum_j w_j (p - s_j) / |p - s_j|^2
+template <int D> __device__ __forceinline__ void hcv_gradient(const DevScene &S, const float *p, float *g) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) g[k] = 0.f;
+    for (uint32_t j = 0; j < S.hcv_m; ++j) {
+        const float4 s = hcv_source(S.hcv_src, j);
+        if constexpr (D == 3) {
+            const float d[3] = {p[0] - s.x, p[1] - s.y, p[2] - s.z};
+            const float z = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            const float ri = __builtin_amdgcn_rsqf(z);
+            const float f = -s.w * (ri * ri * ri);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) g[k] += f * d[k];
+        } else {
+            const float d[2] = {p[0] - s.x, p[1] - s.y};
+            const float z = d[0] * d[0] + d[1] * d[1];
+            const float f = s.z * __builtin_amdgcn_rcpf(z);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) g[k] += f * d[k];
+        }
+    }
+    if constexpr (D == 2) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) g[k] = g[k] * 2.8853900817779268f;   // 2 / ln 2
+    }
 }
 
 // resampled importance sampling, scene.cuh:542-584 (target by the type of y_prev,
@@ -1497,6 +1554,11 @@ __device__ __forceinline__ Sample<D> ris(const DevScene &S, const Model<D, BIE> 
 //   Neumann / Robin y_m:              -(dsign / ift) |G| / M sum_i s_i V_r(z_i).
 // The walk's expectation is unchanged for any table V; with V equal to the value functions
 // of the walk the path-choice randomness cancels.
+// With the control variate on a double-layer Dirichlet walk (line/line flavour), the known
+// term at y_m uses the data g(y_m) - u_h(y_m) of the scene's harmonic function u_h, and the
+// domain estimate adds u_h(x): u - u_h solves the same problem with data g - u_h, so for any
+// u_h harmonic in the domain the expectation is unchanged (up to the walk's truncation, now of
+// the residual's series), and the variance falls with the size of g - u_h.
 template <int D, int BIE, int FL, int STACK, typename VT>
 __device__ __forceinline__ float walk_backward(const DevScene &S, const Model<D, BIE> &m, const WalkParams &wp,
                                                const VT &vt, const BP<D> &x0, Rng &rng, uint32_t *stack,
@@ -1507,7 +1569,13 @@ __device__ __forceinline__ float walk_backward(const DevScene &S, const Model<D,
         if (it == wp.path_length - 1) W = 0.5f * W;
         if constexpr (FL == FL_LINE_LINE) {
             Sample<D> q = line_sample<D, STACK>(S, yp.p, wp.skip(yp.elem), rng, stack, stride);
-            spc += W * m.fredholm_known(yp, q.y, q.inv_pdf, vt, rng);
+            if (wp.cv) {   // the harmonic control variate: the walk carries the data minus u_h
+                BP<D> yk = yp;
+                yk.bv = yp.bv - hcv_value<D>(S, yp.p);
+                spc += W * m.fredholm_known(yk, q.y, q.inv_pdf, vt, rng);
+            } else {
+                spc += W * m.fredholm_known(yp, q.y, q.inv_pdf, vt, rng);
+            }
             W *= m.fredholm_unknown_backward(yp, q.y, q.inv_pdf, rng);
             yp = q.y;
         } else if constexpr (FL == FL_LINE_BV) {
@@ -1586,6 +1654,15 @@ __device__ __forceinline__ void domain_path(const DevScene &S, const Model<D, BI
         out[0] = known + spc * unk;
 #pragma unroll
         for (int k = 0; k < D; ++k) out[1 + k] = gk[k] + spc * gu[k];
+        if constexpr (FL == FL_LINE_LINE) {
+            if (wp.cv) {   // u = u_h + the walk's estimate of u - u_h (data g - u_h, the same source)
+                float gh[D];
+                hcv_gradient<D>(S, x, gh);
+                out[0] += hcv_value<D>(S, x);
+#pragma unroll
+                for (int k = 0; k < D; ++k) out[1 + k] += gh[k];
+            }
+        }
         if constexpr (BIE == kSL && FL == FL_UNIFORM) {
             if (wp.cv) {
                 // I(x) - unk V_L(y0): I(x) = |G| / M0 sum_j G(z_j, x) V_L(z_j) from an independent

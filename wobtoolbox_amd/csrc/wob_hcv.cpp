@@ -1,0 +1,275 @@
+// C ABI of the learned harmonic control variate (include/wob_c.h): a combination of
+// fundamental solutions u_h(x) = c0 + sum_j w_j phi(|x - s_j|^2) (phi(z) = z^(-1/2) in 3D,
+// log2 z in 2D) with its sources on a sphere (circle) outside the scene, fitted by FP64
+// least squares to the double-layer walk's own known-term data at uniform boundary samples
+// (the data plus the volume term as the estimator samples it, bie_models.cuh:389-399), and
+// bound to the scene for WOB_EST_CONTROL_VARIATE double-layer walks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/wob_c.h"
+#include "scene_build.hpp"
+#include "wob_capi_internal.hpp"
+#include "wob_launch.hpp"
+
+namespace {
+
+constexpr uint32_t kMaxSources = 1024;
+
+inline wob_status fail(wob_status s, const std::string &msg) { return wob_fail(s, msg); }
+
+wob_status check_desc(const wob_scene *scene, const wob_hcv_desc *d) {
+    if (!scene || !d) return fail(WOB_INVALID_ARG, "null scene or harmonic control-variate descriptor");
+    if (d->n_sources == 0 || d->n_sources > kMaxSources) return fail(WOB_INVALID_ARG, "n_sources must be 1 .. 1024");
+    if (!(d->radius_scale > 1.f)) return fail(WOB_INVALID_ARG, "radius_scale must be > 1");
+    if (d->n_train < 2 * (d->n_sources + 1)) return fail(WOB_INVALID_ARG, "n_train must be >= 2 (n_sources + 1)");
+    if (!(d->ridge >= 0.f)) return fail(WOB_INVALID_ARG, "ridge must be >= 0");
+    if (d->volume_samples > 0 && wob_scene_dev(scene).vol.kind == wobdev::kVolNone)
+        return fail(WOB_INVALID_ARG, "volume_samples > 0 needs a volume source on the scene");
+    if (d->volume_samples > 0 && !(d->eps > 0.f)) return fail(WOB_INVALID_ARG, "eps must be > 0");
+    return WOB_OK;
+}
+
+// Sources on a sphere (Fibonacci lattice) or a circle (equal angles) of radius
+// radius_scale * R about the centre of the scene's vertex box, R the largest vertex distance
+// from that centre; every source lies outside the scene.
+std::vector<double> source_positions(const wobhost::HostScene &h, uint32_t m, double scale) {
+    const int D = (int)h.dim;
+    const int aw = D == 2 ? wobhost::kAttrWords2 : wobhost::kAttrWords3;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t e = 0; e < h.n; ++e)
+        for (int v = 0; v < D; ++v)
+            for (int k = 0; k < D; ++k) {
+                const double x = h.elem_attr[(size_t)e * aw + v * D + k];
+                lo[k] = std::min(lo[k], x);
+                hi[k] = std::max(hi[k], x);
+            }
+    double c[3] = {0, 0, 0}, R2 = 0;
+    for (int k = 0; k < D; ++k) c[k] = 0.5 * (lo[k] + hi[k]);
+    for (uint32_t e = 0; e < h.n; ++e)
+        for (int v = 0; v < D; ++v) {
+            double r2 = 0;
+            for (int k = 0; k < D; ++k) {
+                const double d = h.elem_attr[(size_t)e * aw + v * D + k] - c[k];
+                r2 += d * d;
+            }
+            R2 = std::max(R2, r2);
+        }
+    const double R = scale * std::sqrt(R2);
+    std::vector<double> s((size_t)m * D);
+    const double pi = 3.14159265358979323846;
+    for (uint32_t j = 0; j < m; ++j) {
+        if (D == 2) {
+            const double a = 2 * pi * (j + 0.5) / m;
+            s[2 * j] = c[0] + R * std::cos(a);
+            s[2 * j + 1] = c[1] + R * std::sin(a);
+        } else {
+            const double z = 1.0 - 2.0 * (j + 0.5) / m, r = std::sqrt(std::max(0.0, 1.0 - z * z));
+            const double ph = pi * (1.0 + std::sqrt(5.0)) * (j + 0.5);
+            s[3 * j] = c[0] + R * r * std::cos(ph);
+            s[3 * j + 1] = c[1] + R * r * std::sin(ph);
+            s[3 * j + 2] = c[2] + R * z;
+        }
+    }
+    return s;
+}
+
+inline double phi(int D, const double *x, const double *s) {
+    double z = 0;
+    for (int k = 0; k < D; ++k) {
+        const double d = x[k] - s[k];
+        z += d * d;
+    }
+    return D == 3 ? 1.0 / std::sqrt(z) : std::log2(z);
+}
+
+// In-place Cholesky solve of the SPD system A x = b (n x n, row-major); false if A is not
+// positive definite.
+bool cholesky_solve(std::vector<double> &A, std::vector<double> &b, int n) {
+    for (int j = 0; j < n; ++j) {
+        double d = A[(size_t)j * n + j];
+        for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+        if (!(d > 0)) return false;
+        const double l = std::sqrt(d);
+        A[(size_t)j * n + j] = l;
+        for (int i = j + 1; i < n; ++i) {
+            double v = A[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) v -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+            A[(size_t)i * n + j] = v / l;
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        double v = b[i];
+        for (int k = 0; k < i; ++k) v -= A[(size_t)i * n + k] * b[k];
+        b[i] = v / A[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double v = b[i];
+        for (int k = i + 1; k < n; ++k) v -= A[(size_t)k * n + i] * b[k];
+        b[i] = v / A[(size_t)i * n + i];
+    }
+    return true;
+}
+
+wob_status bind_sources(wob_scene *scene, const std::vector<float> &rec, uint32_t m, float c0) {
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, (size_t)m * sizeof(float4));
+    if (e != hipSuccess) return fail(WOB_OOM, std::string("hipMalloc harmonic control variate: ") + hipGetErrorString(e));
+    e = hipMemcpy(p, rec.data(), (size_t)m * sizeof(float4), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        return wob_hip_fail(e, "hipMemcpy harmonic control variate");
+    }
+    return wob_scene_bind_hcv(scene, (float4 *)p, m, c0);
+}
+
+}  // namespace
+
+extern "C" {
+
+wob_status wob_harmonic_cv_targets(const wob_scene *scene, const wob_hcv_desc *d, float *d_pos, float *d_target,
+                                   void *stream) {
+    wob_status st = check_desc(scene, d);
+    if (st != WOB_OK) return st;
+    if (!d_pos || !d_target) return fail(WOB_INVALID_ARG, "null device buffer");
+    const wobhost::HostScene &h = wob_scene_host(scene);
+    const size_t lds = wobl::walk_lds_bytes(wobdev::FL_UNIFORM, true, 256);
+    const float eps = d->volume_samples > 0 ? d->eps : 1.f;
+    hipError_t e = h.dim == 2 ? wobl::launch_hcv_targets_2(wob_scene_dev(scene), d->n_train, d->seed, d->stream_id,
+                                                           d->volume_samples, eps, true, d_pos, d_target, lds,
+                                                           (hipStream_t)stream)
+                              : wobl::launch_hcv_targets_3(wob_scene_dev(scene), d->n_train, d->seed, d->stream_id,
+                                                           d->volume_samples, eps, true, d_pos, d_target, lds,
+                                                           (hipStream_t)stream);
+    WOB_HIP(e, "harmonic control-variate targets launch");
+    return WOB_OK;
+}
+
+wob_status wob_scene_fit_harmonic_cv(wob_scene *scene, const wob_hcv_desc *d, float *rms_residual, void *stream) {
+    wob_status st = check_desc(scene, d);
+    if (st != WOB_OK) return st;
+    const wobhost::HostScene &h = wob_scene_host(scene);
+    const int D = (int)h.dim;
+    const uint32_t n = d->n_train, m = d->n_sources, P = m + 1;
+    float *buf = nullptr;
+    WOB_HIP(hipMalloc((void **)&buf, (size_t)n * (D + 1) * sizeof(float)), "hipMalloc harmonic control-variate samples");
+    std::vector<float> pos((size_t)n * D), tgt(n);
+    st = wob_harmonic_cv_targets(scene, d, buf, buf + (size_t)n * D, stream);
+    hipError_t e = hipSuccess;
+    if (st == WOB_OK) e = hipStreamSynchronize((hipStream_t)stream);
+    if (st == WOB_OK && e == hipSuccess) e = hipMemcpy(pos.data(), buf, pos.size() * sizeof(float), hipMemcpyDeviceToHost);
+    if (st == WOB_OK && e == hipSuccess)
+        e = hipMemcpy(tgt.data(), buf + (size_t)n * D, tgt.size() * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(buf);
+    if (st != WOB_OK) return st;
+    WOB_HIP(e, "harmonic control-variate samples");
+
+    const std::vector<double> src = source_positions(h, m, d->radius_scale);
+    // normal equations of the basis [1, phi_1 .. phi_m] over the Dirichlet samples, columns
+    // scaled to unit RMS
+    std::vector<double> row(P), G((size_t)P * P, 0.0), r(P, 0.0), scale(P, 0.0);
+    uint32_t used = 0;
+    auto basis = [&](uint32_t i) {
+        double x[3];
+        for (int k = 0; k < D; ++k) x[k] = pos[(size_t)i * D + k];
+        row[0] = 1.0;
+        for (uint32_t j = 0; j < m; ++j) row[1 + j] = phi(D, x, &src[(size_t)j * D]);
+    };
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!std::isfinite(tgt[i])) continue;
+        basis(i);
+        for (uint32_t a = 0; a < P; ++a) scale[a] += row[a] * row[a];
+        ++used;
+    }
+    if (used < P) return fail(WOB_INVALID_ARG, "fewer Dirichlet boundary samples than coefficients");
+    for (uint32_t a = 0; a < P; ++a) scale[a] = scale[a] > 0 ? 1.0 / std::sqrt(scale[a] / used) : 1.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!std::isfinite(tgt[i])) continue;
+        basis(i);
+        for (uint32_t a = 0; a < P; ++a) row[a] *= scale[a];
+        for (uint32_t a = 0; a < P; ++a) {
+            r[a] += row[a] * tgt[i];
+            const double ra = row[a];
+            double *g = &G[(size_t)a * P];
+            for (uint32_t b = 0; b <= a; ++b) g[b] += ra * row[b];
+        }
+    }
+    double tr = 0;
+    for (uint32_t a = 0; a < P; ++a) tr += G[(size_t)a * P + a];
+    const double lam = (double)d->ridge * tr / P;
+    for (uint32_t a = 0; a < P; ++a) {
+        G[(size_t)a * P + a] += lam;
+        for (uint32_t b = a + 1; b < P; ++b) G[(size_t)a * P + b] = G[(size_t)b * P + a];
+    }
+    if (!cholesky_solve(G, r, (int)P))
+        return fail(WOB_INVALID_ARG, "harmonic control-variate fit: singular normal equations (raise ridge)");
+    for (uint32_t a = 0; a < P; ++a) r[a] *= scale[a];
+
+    // device records (FP32) and the fit's residual with the coefficients as bound
+    std::vector<float> rec((size_t)m * 4, 0.f);
+    for (uint32_t j = 0; j < m; ++j) {
+        for (int k = 0; k < D; ++k) rec[(size_t)j * 4 + k] = (float)src[(size_t)j * D + k];
+        rec[(size_t)j * 4 + D] = (float)r[1 + j];
+    }
+    const float c0 = (float)r[0];
+    if (rms_residual) {
+        double ss = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!std::isfinite(tgt[i])) continue;
+            double x[3], v = c0;
+            for (int k = 0; k < D; ++k) x[k] = pos[(size_t)i * D + k];
+            for (uint32_t j = 0; j < m; ++j) {
+                double s[3];
+                for (int k = 0; k < D; ++k) s[k] = rec[(size_t)j * 4 + k];
+                v += (double)rec[(size_t)j * 4 + D] * phi(D, x, s);
+            }
+            ss += (tgt[i] - v) * (tgt[i] - v);
+        }
+        *rms_residual = (float)std::sqrt(ss / used);
+    }
+    return bind_sources(scene, rec, m, c0);
+}
+
+wob_status wob_scene_set_harmonic_cv(wob_scene *scene, uint32_t n_sources, const float *sources, float c0) {
+    if (!scene) return fail(WOB_INVALID_ARG, "null scene");
+    if (n_sources == 0) return wob_scene_bind_hcv(scene, nullptr, 0, 0.f);
+    if (n_sources > kMaxSources) return fail(WOB_INVALID_ARG, "n_sources must be <= 1024");
+    if (!sources) return fail(WOB_INVALID_ARG, "null sources");
+    const int D = (int)wob_scene_host(scene).dim;
+    std::vector<float> rec((size_t)n_sources * 4, 0.f);
+    for (uint32_t j = 0; j < n_sources; ++j)
+        for (int k = 0; k <= D; ++k) rec[(size_t)j * 4 + k] = sources[(size_t)j * (D + 1) + k];
+    return bind_sources(scene, rec, n_sources, c0);
+}
+
+wob_status wob_scene_get_harmonic_cv(const wob_scene *scene, uint32_t *n_sources, float *sources, float *c0) {
+    if (!scene || !n_sources) return fail(WOB_INVALID_ARG, "null scene or output");
+    const wobdev::DevScene &S = wob_scene_dev(scene);
+    *n_sources = S.hcv_m;
+    if (c0) *c0 = S.hcv_c0;
+    if (sources && S.hcv_m) {
+        const int D = (int)wob_scene_host(scene).dim;
+        std::vector<float> rec((size_t)S.hcv_m * 4);
+        WOB_HIP(hipMemcpy(rec.data(), S.hcv_src, rec.size() * sizeof(float), hipMemcpyDeviceToHost),
+                "hipMemcpy harmonic control variate");
+        for (uint32_t j = 0; j < S.hcv_m; ++j)
+            for (int k = 0; k <= D; ++k) sources[(size_t)j * (D + 1) + k] = rec[(size_t)j * 4 + k];
+    }
+    return WOB_OK;
+}
+
+wob_status wob_eval_harmonic_cv(const wob_scene *scene, const float *d_points, uint32_t n, float *d_out, void *stream) {
+    if (!scene) return fail(WOB_INVALID_ARG, "null scene");
+    if (n == 0) return WOB_OK;
+    if (!d_points || !d_out) return fail(WOB_INVALID_ARG, "null device buffer");
+    WOB_HIP(wobl::launch_hcv_eval((int)wob_scene_host(scene).dim, wob_scene_dev(scene), n, d_points, d_out,
+                                  (hipStream_t)stream),
+            "harmonic control-variate evaluation launch");
+    return WOB_OK;
+}
+
+}  // extern "C"

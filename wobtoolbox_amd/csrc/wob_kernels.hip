@@ -321,6 +321,27 @@ hipError_t launch_sample_boundary(int dim, const DevScene &S, uint32_t n, uint64
     return hipGetLastError();
 }
 
+// u_h and grad u_h of the scene's harmonic control variate (the walk's own functions)
+template <int D> __global__ void hcv_eval_kernel(DevScene S, uint32_t n, const float *pts, float *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x[D], g[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) x[k] = pts[(size_t)i * D + k];
+    hcv_gradient<D>(S, x, g);
+    out[(size_t)i * (D + 1)] = hcv_value<D>(S, x);
+#pragma unroll
+    for (int k = 0; k < D; ++k) out[(size_t)i * (D + 1) + 1 + k] = g[k];
+}
+
+hipError_t launch_hcv_eval(int dim, const DevScene &S, uint32_t n, const float *pts, float *out, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint32_t block = 256, grid = (n + block - 1) / block;
+    if (dim == 2) hipLaunchKernelGGL(hcv_eval_kernel<2>, dim3(grid), dim3(block), 0, st, S, n, pts, out);
+    else hipLaunchKernelGGL(hcv_eval_kernel<3>, dim3(grid), dim3(block), 0, st, S, n, pts, out);
+    return hipGetLastError();
+}
+
 __global__ void fused_records_kernel(const uint32_t *guide, uint32_t m, const float *cdf, uint32_t n, const float4 *geo,
                                      float4 *out) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;

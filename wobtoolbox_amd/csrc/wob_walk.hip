@@ -265,7 +265,13 @@ __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
             } else {                                                       // walk_backward, one bounce
                 Sample<D> qq = q;
                 if constexpr (FL == FL_LINE_LINE) {
-                    spc += W * m.fredholm_known(yp, qq.y, qq.inv_pdf, NoVol{}, rng);
+                    if (e.flags & 2u) {   // harmonic control variate, as in walk_backward
+                        BP<D> yk = yp;
+                        yk.bv = yp.bv - hcv_value<D>(e.scene, yp.p);
+                        spc += W * m.fredholm_known(yk, qq.y, qq.inv_pdf, NoVol{}, rng);
+                    } else {
+                        spc += W * m.fredholm_known(yp, qq.y, qq.inv_pdf, NoVol{}, rng);
+                    }
                     W *= m.fredholm_unknown_backward(yp, qq.y, qq.inv_pdf, rng);
                 } else {
                     W = W * m.fredholm_unknown_backward(yp, qq.y, qq.inv_pdf, rng);
@@ -279,6 +285,15 @@ __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
             out[0] = known + spc * unk;
 #pragma unroll
             for (int k = 0; k < D; ++k) out[1 + k] = gk[k] + spc * gu[k];
+            if constexpr (FL == FL_LINE_LINE) {
+                if (e.flags & 2u) {
+                    float gh[D];
+                    hcv_gradient<D>(e.scene, x, gh);
+                    out[0] += hcv_value<D>(e.scene, x);
+#pragma unroll
+                    for (int k = 0; k < D; ++k) out[1 + k] += gh[k];
+                }
+            }
 #pragma unroll
             for (int k = 0; k < V; ++k) { ks[k].add(out[k]); kq[k].add(out[k] * out[k]); }
             j += 1;
@@ -360,6 +375,32 @@ __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
         }
     }
 }
+
+#if WOB_WALK_VOL
+// Training targets of the harmonic control variate.  Sample i is a uniform boundary point y
+// (Philox stream (seed, path i, point 0, stream)); its target is the quantity the
+// double-layer walk's known term carries at y (bie_models.cuh:389-399): the data g(y) plus
+// the volume term at y, here the mean of nvs independent volume samples (nvs = 0: no volume
+// source).  Samples off Dirichlet elements get a NaN target.
+template <int D>
+__global__ __launch_bounds__(256) void hcv_targets_kernel(DevScene S, uint32_t n, uint64_t seed, uint32_t stream,
+                                                          uint32_t nvs, float eps, uint32_t interior, float *pos,
+                                                          float *target) {
+    extern __shared__ uint32_t lds_stack[];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Rng rng;
+    rng.init(seed, i, 0u, stream);
+    const Sample<D> y = sample_uniform<D>(S, rng);
+    Kern<D> k;
+    k.eps = eps;
+    const VolCtx<D, wobl::kStackEntries> vt{&S, nvs, 1u, interior != 0u, lds_stack + threadIdx.x, blockDim.x};
+    const float t = y.y.bv + vt.term(y.y, k, rng);
+#pragma unroll
+    for (int c = 0; c < D; ++c) pos[(size_t)i * D + c] = y.y.p[c];
+    target[i] = y.y.type == kDirichlet ? t : __builtin_nanf("");
+}
+#endif
 
 }  // namespace wobk
 
@@ -445,6 +486,18 @@ uint32_t WOB_NAME(persistent_grid_, WOB_WALK_DIM, WOB_WALK_VOL)(bool boundary, i
     return boundary ? dispatch<WOB_WALK_DIM, true, WOB_WALK_VOL != 0>(bie, est, fl, go)
                     : dispatch<WOB_WALK_DIM, false, WOB_WALK_VOL != 0>(bie, est, fl, go);
 }
+
+#if WOB_WALK_VOL
+hipError_t WOB_NAME(launch_hcv_targets_, WOB_WALK_DIM, )(const wobdev::DevScene &S, uint32_t n, uint64_t seed,
+                                                          uint32_t stream, uint32_t nvs, float eps, bool interior,
+                                                          float *pos, float *target, size_t lds_bytes, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint32_t block = 256;
+    hipLaunchKernelGGL((hcv_targets_kernel<WOB_WALK_DIM>), dim3((n + block - 1) / block), dim3(block), lds_bytes, st, S,
+                       n, seed, stream, nvs, eps, interior ? 1u : 0u, pos, target);
+    return hipGetLastError();
+}
+#endif
 
 bool WOB_NAME(supports_wavefront_, WOB_WALK_DIM, WOB_WALK_VOL)(bool boundary, int bie, int est, int fl) {
     auto go = [&](auto k) { return decltype(k)::supports_wavefront(); };
