@@ -147,12 +147,10 @@ typedef struct wob_estimator_desc {
  *   few degrees of that element re-hits it at a larger |t| and adds a spurious hit (a few
  *   percent of bounces), which inflates the variance and makes paths sensitive to rounding. */
 enum { WOB_EST_EXCLUDE_ORIGIN_ELEMENT = 1 };
-/* WOB_EST_CONTROL_VARIATE (backward estimator): single-layer model: use the value table bound
+/* WOB_EST_CONTROL_VARIATE (single-layer model, backward estimator): use the value table bound
  *   with wob_scene_set_control_variate as a control variate.  Every added term has zero mean
  *   for any table, so the estimator's expectation is exactly that of plain WoB; a table close
- *   to the walk's value functions removes most of the path-choice variance.  Double-layer
- *   model: use the scene's harmonic function u_h (wob_scene_fit_harmonic_cv): the walk
- *   estimates u - u_h and the estimate adds u_h.
+ *   to the walk's value functions removes most of the path-choice variance.
  * WOB_EST_BOUNDARY_DENSITY (backward estimator, boundary entry points): return the walk's
  *   boundary-density estimate at each record (compute_sample_unknown_boundary,
  *   estimator.cuh:232-357) instead of the solution; this generates control-variate training
@@ -162,6 +160,15 @@ enum { WOB_EST_CONTROL_VARIATE = 2, WOB_EST_BOUNDARY_DENSITY = 4 };
  *   uses its own sample.  The reference draws one sample per term and adds it
  *   num_volume_samples times (bie_models.cuh:85-101), which leaves the variance unchanged.   */
 enum { WOB_EST_INDEPENDENT_VOLUME_SAMPLES = 8 };
+/* WOB_EST_HARMONIC_CV (single-layer or double-layer model, backward estimator, interior
+ *   domain): use the scene's harmonic function u_h (wob_scene_fit_harmonic_cv).  Every
+ *   boundary datum the walk reads is replaced by the residual's: g - u_h at Dirichlet points,
+ *   g - du_h/dn at Neumann points, g - (du_h/dn + alpha u_h) at Robin points; each domain
+ *   estimate adds u_h(x) and grad u_h(x).  u - u_h solves the same problem with the residual
+ *   data, so the expectation is that of plain WoB for any u_h harmonic in the domain (the
+ *   path-length truncation then applies to the residual's series).  Combines with
+ *   WOB_EST_CONTROL_VARIATE (a value table of the residual walk).                          */
+enum { WOB_EST_HARMONIC_CV = 16 };
 
 /* Launch tuning and timing (in/out; may be NULL).  paths_per_lane: paths one lane
  * accumulates before it writes its partial sums; block_size: threads per workgroup
@@ -355,21 +362,20 @@ wob_status wob_cv_infer(wob_cv *cv, const float *d_pos, uint32_t n, float *d_out
 wob_status wob_scene_set_control_variate(wob_scene *scene, wob_cv *cv, const float *d_table, uint32_t n_values,
                                          void *stream);
 
-/* Learned harmonic control variate (no counterpart in the reference) for double-layer
- * Dirichlet domain walks, the Poisson problem's estimator (poisson_equation.cu:40-61):
+/* Learned harmonic control variate (no counterpart in the reference) for interior
+ * single-layer and double-layer walks (WOB_EST_HARMONIC_CV), e.g. the Poisson problem's
+ * double-layer estimator (poisson_equation.cu:40-61) and the mixed single-layer workload:
  *   u_h(x) = c0 + sum_j w_j phi(|x - s_j|^2),  phi(z) = z^(-1/2) (3D) or log2 z (2D),
  * a combination of fundamental solutions with sources s_j on a sphere (circle) outside the
- * scene, so u_h is harmonic in the interior domain.  With WOB_EST_CONTROL_VARIATE a
- * double-layer walk carries the data g - u_h and each domain estimate adds u_h(x) (and
- * grad u_h(x)): u - u_h solves the same problem with data g - u_h, so the expectation is
- * that of plain WoB for any u_h (the walk's path-length truncation then applies to the
- * residual's series), and the variance falls with the size of g - u_h.
+ * scene, so u_h is harmonic in the interior domain.  The variance of the walk falls with the
+ * size of the residual data.
  * Fit: the targets are the walk's own known-term data at n_train uniform boundary samples
- * (sample i on the Philox stream (seed, path i, point 0, stream_id)): the boundary data plus
+ * (sample i on the Philox stream (seed, path i, point 0, stream_id)): the boundary datum plus
  * the volume term as the estimator samples it (the mean of volume_samples independent
- * volume samples; 0 without a volume source); (c0, w) minimise the ridge-regularised squared
- * error in FP64 (columns scaled to unit RMS, ridge relative to the mean diagonal), and the
- * result is bound to the scene.  Interior domains only (the sources lie outside).           */
+ * volume samples; 0 without a volume source); each sample contributes the row of its type
+ * (u_h, du_h/dn or du_h/dn + alpha u_h at the sample), (c0, w) minimise the ridge-regularised
+ * squared error in FP64 (columns scaled to unit RMS, ridge relative to the mean diagonal),
+ * and the result is bound to the scene.  Interior domains only (the sources lie outside).   */
 typedef struct wob_hcv_desc {
     uint32_t n_sources;       /* 1 .. 1024 */
     float radius_scale;       /* source radius / the scene's bounding radius about its box centre, > 1 */
@@ -380,12 +386,11 @@ typedef struct wob_hcv_desc {
     uint64_t seed;
     uint32_t stream_id;
 } wob_hcv_desc;
-/* rms_residual (host, may be NULL): RMS of target - u_h over the Dirichlet samples.        */
+/* rms_residual (host, may be NULL): RMS of target - (the sample's row) over the samples.  */
 wob_status wob_scene_fit_harmonic_cv(wob_scene *scene, const wob_hcv_desc *desc, float *rms_residual, void *stream);
-/* Training data of the fit: d_pos (n_train * dim floats), d_target (n_train floats, NaN for
- * samples off Dirichlet elements), device buffers.                                           */
-wob_status wob_harmonic_cv_targets(const wob_scene *scene, const wob_hcv_desc *desc, float *d_pos, float *d_target,
-                                   void *stream);
+/* Training data of the fit as BoundaryPoint records (n_train * (2 dim + 4) floats, device):
+ * [p, n, 0.5, target, type, robin_alpha].                                                   */
+wob_status wob_harmonic_cv_targets(const wob_scene *scene, const wob_hcv_desc *desc, float *d_records, void *stream);
 /* Bind explicit sources (host, n_sources * (dim + 1) floats: s_j then w_j) and c0;
  * n_sources == 0 unbinds.                                                                   */
 wob_status wob_scene_set_harmonic_cv(wob_scene *scene, uint32_t n_sources, const float *sources, float c0);

@@ -333,6 +333,22 @@ template <int D> static void hcv_gradient(const Scene<D> &s, const float *p, flo
         for (int k = 0; k < D; ++k) g[k] = g[k] * 2.8853900817779268f;
 }
 
+// the residual datum at y: y's datum minus u_h's datum of y's type
+template <int D> static BPoint<D> hcv_residual(const Scene<D> &s, const BPoint<D> &y) {
+    BPoint<D> r = y;
+    if (y.boundary_type == Dirichlet) {
+        r.boundary_value = y.boundary_value - hcv_value(s, y.p.v);
+    } else {
+        float g[3];
+        hcv_gradient(s, y.p.v, g);
+        float dn = g[0] * y.n[0];
+        for (int k = 1; k < D; ++k) dn += g[k] * y.n[k];
+        if (y.boundary_type == Neumann) r.boundary_value = y.boundary_value - dn;
+        else r.boundary_value = y.boundary_value - (dn + y.robin_alpha * hcv_value(s, y.p.v));
+    }
+    return r;
+}
+
 // utils.cuh:148-166
 struct Kahan {
     float sum = 0, c = 0;
@@ -574,7 +590,7 @@ struct Options {
     int pick_mode = 1;      // 0: reference buffer pick, 1: matched min-hash pick
     bool use_bvh = true;
     bool exclude_origin = false;   // ignore the element the line's origin lies on
-    bool control_variate = false;  // double-layer walks: the scene's harmonic control variate
+    bool harmonic_cv = false;      // single/double-layer walks: the residual of the scene's harmonic u_h
 };
 
 template <int D> static BPoint<D> point_on_element_uniform(const Element<D> &e, Rng &rng) {
@@ -1036,13 +1052,8 @@ template <int D> struct Estimator {
             if (dl_dirichlet()) {
                 Sampled<D> q = sample_line_intersection(*s, yp.p, rng, opt, yp.elem);
                 y = q.y; inv = q.inv_pdf;
-                if (opt.control_variate) {   // the data minus u_h
-                    BPoint<D> yk = yp;
-                    yk.boundary_value = yp.boundary_value - hcv_value(*s, yp.p.v);
-                    spc += W * m.fredholm_known(yk, y, inv, rng);
-                } else {
-                    spc += W * m.fredholm_known(yp, y, inv, rng);
-                }
+                if (opt.harmonic_cv) spc += W * m.fredholm_known(hcv_residual(*s, yp), y, inv, rng);
+                else spc += W * m.fredholm_known(yp, y, inv, rng);
                 W *= m.fredholm_unknown(yp, y, inv, rng, true);
                 yp = y;
             } else if (direct_neumann()) {
@@ -1054,7 +1065,8 @@ template <int D> struct Estimator {
                 W = W * m.fredholm_unknown(yp, y, inv, rng, true);
                 yp = y;
             } else if (m.bie == IndirectSingleLayer) {
-                spc += W * m.fredholm_known(yp, y, inv, rng);
+                if (opt.harmonic_cv) spc += W * m.fredholm_known(hcv_residual(*s, yp), y, inv, rng);
+                else spc += W * m.fredholm_known(yp, y, inv, rng);
                 Sampled<D> q = nrc == 1 ? sample_uniform(*s, rng) : ris(yp, rng);
                 y = q.y; inv = q.inv_pdf;
                 W = W * m.fredholm_unknown(yp, y, inv, rng, true);
@@ -1090,7 +1102,7 @@ template <int D> struct Estimator {
             out[0] += spc * m.solution_domain_unknown(x, y.y, y.inv_pdf);
             const Vec<D> gu = m.gradient_domain_unknown(x, y.y, y.inv_pdf);
             for (int k = 0; k < D; ++k) out[1 + k] = g[k] + spc * gu[k];
-            if (dl_dirichlet() && opt.control_variate) {   // u = u_h + the walk's estimate of u - u_h
+            if (opt.harmonic_cv && m.bie != Direct) {   // u = u_h + the walk's estimate of u - u_h
                 float gh[3];
                 hcv_gradient(*s, x.v, gh);
                 out[0] += hcv_value(*s, x.v);
@@ -1169,7 +1181,7 @@ struct orc_est_desc {
     int32_t pick_mode, rng_mode, use_bvh, exclude_origin;
     int32_t independent_volume;
     int32_t boundary_density;   // boundary entry: return compute_sample_unknown_boundary (the walk)
-    int32_t control_variate;    // double-layer walks: the scene's harmonic control variate
+    int32_t harmonic_cv;        // single/double-layer walks: the residual of the scene's harmonic u_h
 };
 
 extern "C" {
@@ -1379,7 +1391,7 @@ template <int D> static Estimator<D> make_est(const Scene<D> &s, const orc_est_d
     e.opt.pick_mode = d->pick_mode;
     e.opt.use_bvh = d->use_bvh != 0;
     e.opt.exclude_origin = d->exclude_origin != 0;
-    e.opt.control_variate = d->control_variate != 0;
+    e.opt.harmonic_cv = d->harmonic_cv != 0;
     return e;
 }
 
@@ -1585,11 +1597,10 @@ void orc_eval_harmonic_cv(void *p, int n, const float *pts, float *out) {
 }
 
 // Training targets of the harmonic control variate: sample i = the uniform boundary point of
-// the Philox stream (seed, path i, point 0, stream), target = its data plus the double-layer
-// known term's volume part with nvs independent volume samples (interior mask), NaN off
-// Dirichlet elements.  pos: n * dim floats.
-void orc_hcv_targets(void *p, int n, uint64_t seed, uint32_t stream, unsigned nvs, float eps, float *pos,
-                     float *target) {
+// the Philox stream (seed, path i, point 0, stream), target = its datum plus the walk's
+// volume term there with nvs independent volume samples (interior mask), as a BoundaryPoint
+// record [p, n, ia, target, type, alpha] (2 dim + 4 floats).
+void orc_hcv_targets(void *p, int n, uint64_t seed, uint32_t stream, unsigned nvs, float eps, float *rec) {
     auto *S = (OrcScene *)p;
     auto run = [&](auto &s, auto dimtag) {
         constexpr int D = decltype(dimtag)::value;
@@ -1600,8 +1611,10 @@ void orc_hcv_targets(void *p, int n, uint64_t seed, uint32_t stream, unsigned nv
             Rng rng; rng.init_philox(seed, (uint32_t)i, 0u, stream);
             Sampled<D> y = sample_uniform(s, rng);
             const float t = y.y.boundary_value + m.vol_term(y.y, rng);
-            for (int k = 0; k < D; ++k) pos[(size_t)i * D + k] = y.y.p[k];
-            target[i] = y.y.boundary_type == Dirichlet ? t : std::numeric_limits<float>::quiet_NaN();
+            float *r = rec + (size_t)i * (2 * D + 4);
+            for (int k = 0; k < D; ++k) { r[k] = y.y.p[k]; r[D + k] = y.y.n[k]; }
+            r[2 * D] = y.y.interior_angle; r[2 * D + 1] = t; r[2 * D + 2] = (float)y.y.boundary_type;
+            r[2 * D + 3] = y.y.robin_alpha;
         }
     };
     if (S->dim == 2) run(S->s2, std::integral_constant<int, 2>{});

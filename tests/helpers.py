@@ -124,11 +124,26 @@ def hcv_sources(el, m, radius_scale):
     return c + R * np.stack([r * np.cos(ph), r * np.sin(ph), z], 1)
 
 
-def hcv_basis(pos, src):
-    """[1, phi(|x - s_j|^2)] with phi = z^(-1/2) (3D) or log2 z (2D), FP64."""
-    z = ((np.asarray(pos, np.float64)[:, None, :] - src[None]) ** 2).sum(2)
-    phi = 1 / np.sqrt(z) if src.shape[1] == 3 else np.log2(z)
-    return np.concatenate([np.ones((len(z), 1)), phi], 1)
+def hcv_rows(rec, src):
+    """The fit's rows of BoundaryPoint records [p, n, ia, target, type, alpha] (FP64): the
+    datum of each basis function [1, phi_j] of the record's type -- the value (Dirichlet),
+    the normal derivative (Neumann) or normal derivative + alpha value (Robin);
+    phi = z^(-1/2) (3D) or log2 z (2D) of z = |x - s_j|^2."""
+    D = src.shape[1]
+    rec = np.asarray(rec, np.float64)
+    x, n, ty, al = rec[:, :D], rec[:, D:2 * D], rec[:, 2 * D + 2].astype(int), rec[:, 2 * D + 3]
+    d = x[:, None, :] - src[None]
+    z = (d ** 2).sum(2)
+    dn = (d * n[:, None, :]).sum(2)
+    if D == 3:
+        val, der = 1 / np.sqrt(z), -dn / z ** 1.5
+    else:
+        val, der = np.log2(z), 2 / np.log(2) * dn / z
+    one = np.ones((len(rec), 1))
+    rows_d = np.concatenate([one, val], 1)
+    rows_n = np.concatenate([0 * one, der], 1)
+    rows_r = np.concatenate([al[:, None], der + al[:, None] * val], 1)
+    return np.where((ty == 0)[:, None], rows_d, np.where((ty == 1)[:, None], rows_n, rows_r))
 
 
 def hcv_eval_f64(pos, src, w, c0):
@@ -145,14 +160,14 @@ def hcv_eval_f64(pos, src, w, c0):
     return np.concatenate([u[:, None], g], 1)
 
 
-def hcv_fit_numpy(pos, tgt, src, ridge=1e-10):
-    """Ridge least squares of the targets on [1, phi_j] with unit-RMS columns (FP64):
+def hcv_fit_numpy(rec, src, ridge=1e-10):
+    """Ridge least squares of the records' targets on their rows, unit-RMS columns (FP64):
     returns (c0, w)."""
-    ok = np.isfinite(tgt)
-    B = hcv_basis(pos[ok], src)
+    B = hcv_rows(rec, src)
+    t = np.asarray(rec, np.float64)[:, 2 * src.shape[1] + 1]
     sc = 1 / np.sqrt((B ** 2).mean(0))
     Bs = B * sc
     A = Bs.T @ Bs
     A += ridge * np.trace(A) / len(A) * np.eye(len(A))
-    coef = np.linalg.solve(A, Bs.T @ tgt[ok].astype(np.float64)) * sc
+    coef = np.linalg.solve(A, Bs.T @ t) * sc
     return coef[0], coef[1:]

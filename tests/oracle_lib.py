@@ -28,7 +28,7 @@ class EstDesc(C.Structure):
                 ("num_volume_samples", C.c_uint32), ("eps", C.c_float),
                 ("first_kind_equation_scaling_constant", C.c_float), ("pick_mode", C.c_int32),
                 ("rng_mode", C.c_int32), ("use_bvh", C.c_int32), ("exclude_origin", C.c_int32),
-                ("independent_volume", C.c_int32), ("boundary_density", C.c_int32), ("control_variate", C.c_int32)]
+                ("independent_volume", C.c_int32), ("boundary_density", C.c_int32), ("harmonic_cv", C.c_int32)]
 
 
 def build():
@@ -82,7 +82,7 @@ def lib():
                                         C.c_int, P, P]
         L.orc_scene_set_harmonic_cv.argtypes = [P, C.c_uint, P, C.c_float]
         L.orc_eval_harmonic_cv.argtypes = [P, C.c_int, P, P]
-        L.orc_hcv_targets.argtypes = [P, C.c_int, C.c_uint64, C.c_uint32, C.c_uint, C.c_float, P, P]
+        L.orc_hcv_targets.argtypes = [P, C.c_int, C.c_uint64, C.c_uint32, C.c_uint, C.c_float, P]
         _lib = L
     return _lib
 
@@ -93,16 +93,16 @@ def _p(a):
 
 def est_desc(solver: dict, pick_mode=1, rng_mode=0, use_bvh=1, num_volume_samples=0) -> EstDesc:
     """Solver dict (reference JSON keys + the device estimator flags) -> oracle descriptor.
-    flags bit 0: exclude the origin element of boundary line samples; bit 1: the scene's
-    harmonic control variate (double-layer walks); bit 2: boundary density (the walk's value
-    at boundary records); bit 3: independent volume samples."""
+    flags bit 0: exclude the origin element of boundary line samples; bit 2: boundary density
+    (the walk's value at boundary records); bit 3: independent volume samples; bit 4: the
+    scene's harmonic control variate (single and double layer)."""
     fl = int(solver.get("flags", 0))
     nvs = num_volume_samples or int(solver.get("num_volume_samples", 0))
     return EstDesc(ENUMS[solver["bie_model"]], ENUMS[solver["domain"]], ENUMS[solver["estimator"]],
                    ENUMS[solver["problem"]], solver["path_length"], solver.get("num_resampling_candidates", 1),
                    nvs, solver.get("eps", float(np.finfo(np.float32).eps)),
                    solver.get("first_kind_equation_scaling_constant", 1.0), pick_mode, rng_mode, use_bvh, fl & 1,
-                   (fl >> 3) & 1, (fl >> 2) & 1, (fl >> 1) & 1)
+                   (fl >> 3) & 1, (fl >> 2) & 1, (fl >> 4) & 1)
 
 
 class OracleScene:
@@ -125,10 +125,11 @@ class OracleScene:
         return out
 
     def hcv_targets(self, n, seed=0, stream=0, volume_samples=0, eps=1e-4):
-        pos = np.zeros((n, self.dim), np.float32)
-        tgt = np.zeros(n, np.float32)
-        lib().orc_hcv_targets(self.h, n, seed, stream, volume_samples, eps, _p(pos), _p(tgt))
-        return pos, tgt
+        """BoundaryPoint records (n, 2 dim + 4) of the harmonic control variate's training
+        samples, the boundary value replaced by the target."""
+        rec = np.zeros((n, 2 * self.dim + 4), np.float32)
+        lib().orc_hcv_targets(self.h, n, seed, stream, volume_samples, eps, _p(rec))
+        return rec
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -1,8 +1,9 @@
-"""CPU pins of the oracle's harmonic control variate for double-layer Dirichlet walks.
+"""CPU pins of the oracle's harmonic control variate (single- and double-layer walks).
 
 u_h(x) = c0 + sum_j w_j phi(|x - s_j|^2) (phi = z^(-1/2) in 3D, log2 z in 2D) is harmonic inside
-the scene.  The walk with the control variate carries the data g - u_h and each domain
-estimate adds u_h(x); u - u_h solves the same problem with data g - u_h.  The pins:
+the scene.  The walk with the control variate carries the residual data (g - u_h at Dirichlet
+points, g - du_h/dn at Neumann points) and each domain estimate adds u_h(x); u - u_h solves
+the same problem with the residual data.  The pins:
   * u_h and its gradient against the closed form in FP64;
   * without a volume source the training targets are the boundary data at the sampled points;
   * data equal to a function the control variate represents exactly leave only the linear
@@ -11,11 +12,14 @@ estimate adds u_h(x); u - u_h solves the same problem with data g - u_h.  The pi
     cancel u_h instead);
   * on the Poisson problem (16 Gaussians, double layer, path length 4) the estimate stays
     within the standard error of the closed-form solution u = N_f + H while the variance falls
-    about 90-fold.
+    about 90-fold;
+  * on the mixed Dirichlet / Neumann single-layer problem (RIS 64, first-kind constant 2,
+    path length 4) against exp(0.6 x) cos(0.6 y) + x z, likewise, with the variance falling
+    by more than four orders of magnitude.
 """
 import numpy as np
 
-from helpers import hcv_eval_f64, hcv_fit_numpy, hcv_sources, mean_se, small_circle
+from helpers import hcv_eval_f64, hcv_fit_numpy, hcv_rows, hcv_sources, mean_se, small_circle, small_icosphere
 from wobtoolbox_amd import geometry as G
 
 DL = dict(bie_model="IndirectDoubleLayer", domain="InteriorDomain", estimator="BackwardEstimator",
@@ -49,18 +53,37 @@ def test_hcv_eval_closed_form(oracle):
 
 
 def test_hcv_targets_are_the_data_without_volume(oracle):
-    from helpers import small_icosphere
     el, _, _ = small_icosphere(2, kind="mixed")
     sc = oracle.OracleScene(el)
-    pos, tgt = sc.hcv_targets(300, seed=4, stream=7, volume_samples=0)
+    rec = sc.hcv_targets(300, seed=4, stream=7, volume_samples=0)
     for i in range(300):
-        rec = np.zeros(10, np.float32)
-        oracle.lib().orc_sample_uniform(sc.h, 4, i, 0, 7, rec.ctypes.data)
-        assert np.array_equal(pos[i], rec[:3])
-        if int(rec[8]) == G.DIRICHLET:
-            assert tgt[i] == rec[7]
-        else:
-            assert np.isnan(tgt[i])
+        r = np.zeros(10, np.float32)
+        oracle.lib().orc_sample_uniform(sc.h, 4, i, 0, 7, r.ctypes.data)
+        assert np.array_equal(rec[i], r)
+
+
+def test_hcv_rows_are_the_datum_of_each_type():
+    """The fit's rows: value, normal derivative and Robin combination of each basis function,
+    the derivative against central differences in FP64."""
+    rng = np.random.default_rng(3)
+    for D in (2, 3):
+        src = 3 * rng.normal(size=(5, D))
+        x = rng.uniform(-0.5, 0.5, (3, D))
+        n = rng.normal(size=(3, D))
+        n /= np.linalg.norm(n, axis=1, keepdims=True)
+        rec = np.zeros((3, 2 * D + 4))
+        rec[:, :D], rec[:, D:2 * D], rec[:, 2 * D + 2], rec[:, 2 * D + 3] = x, n, [0, 1, 2], [0, 0, 0.7]
+        rows = hcv_rows(rec, src)
+        h = 1e-6
+
+        def val(p):
+            z = ((p[None] - src) ** 2).sum(1)
+            return 1 / np.sqrt(z) if D == 3 else np.log2(z)
+        assert np.allclose(rows[0], np.concatenate([[1], val(x[0])]))
+        fd = (val(x[1] + h * n[1]) - val(x[1] - h * n[1])) / (2 * h)
+        assert np.allclose(rows[1], np.concatenate([[0], fd]), rtol=1e-6, atol=1e-8)
+        fd2 = (val(x[2] + h * n[2]) - val(x[2] - h * n[2])) / (2 * h) + 0.7 * val(x[2])
+        assert np.allclose(rows[2], np.concatenate([[0.7], fd2]), rtol=1e-6, atol=1e-8)
 
 
 def _representable(dim, rng):
@@ -91,7 +114,7 @@ def test_hcv_representable_data_leave_only_interpolation_error(oracle):
         n = 1024
         solver = dict(DL, path_length=6)
         s0, q0 = sc.estimate_domain(solver, pts, n, seed=3)
-        s1, q1 = sc.estimate_domain(dict(solver, flags=2), pts, n, seed=3)
+        s1, q1 = sc.estimate_domain(dict(solver, flags=16), pts, n, seed=3)
         m0, se0 = mean_se(s0, q0, n)
         m1, se1 = mean_se(s1, q1, n)
         exact = h(pts)
@@ -108,19 +131,44 @@ def test_hcv_poisson_unbiased_and_variance(oracle):
     el = G.mesh_elements(V, F, G.DIRICHLET, lambda p, nr, ty: ex(p)[:, 0])
     sc = oracle.OracleScene(el)
     sc.set_volume_gaussians(c, wd, a)
-    pos, tgt = sc.hcv_targets(16384, seed=5, volume_samples=16, eps=1e-4)
+    rec = sc.hcv_targets(16384, seed=5, volume_samples=16, eps=1e-4)
     src = hcv_sources(el, 64, 1.5)
-    c0, w = hcv_fit_numpy(pos, tgt, src)
+    c0, w = hcv_fit_numpy(rec, src)
     sc.set_harmonic_cv(np.concatenate([src, w[:, None]], 1), c0)
     pts = G.interior_points_star(32, np.random.default_rng(5), margin=0.05).astype(np.float32)
     solver = dict(DL, path_length=4, num_volume_samples=1)
     n = 4096
     var = {}
-    for fl in (0, 2):
+    for fl in (0, 16):
         s, q = sc.estimate_domain(dict(solver, flags=fl), pts, n, seed=3, threads=8)
         m, se = mean_se(s, q, n)
         z = (m - ex(pts)) / se
         assert np.abs(z[:, 0]).max() < 4.5, (fl, np.abs(z[:, 0]).max())
         assert np.abs(z[:, 1:]).max() < 5.0, (fl, np.abs(z[:, 1:]).max())
         var[fl] = np.mean(se[:, 0] ** 2)
-    assert var[0] / var[2] > 40, var
+    assert var[0] / var[16] > 40, var
+
+
+def test_hcv_single_layer_mixed_unbiased_and_variance(oracle):
+    """The learned-control-variate workload's solver (single layer, interior, mixed Dirichlet
+    z > 0 / Neumann, RIS 64, c = 2, path length 4) on a level-3 icosphere."""
+    el, _, _ = small_icosphere(3, kind="mixed")
+    sc = oracle.OracleScene(el)
+    rec = sc.hcv_targets(16384, seed=5, volume_samples=0)
+    src = hcv_sources(el, 64, 1.5)
+    c0, w = hcv_fit_numpy(rec, src)
+    sc.set_harmonic_cv(np.concatenate([src, w[:, None]], 1), c0)
+    pts = G.interior_points_star(32, np.random.default_rng(5), margin=0.05).astype(np.float32)
+    solver = dict(bie_model="IndirectSingleLayer", domain="InteriorDomain", estimator="BackwardEstimator",
+                  problem="MixedProblem", path_length=4, eps=1e-5, num_resampling_candidates=64,
+                  first_kind_equation_scaling_constant=2.0)
+    ex = G.exact_expcos_xz_3d(pts)
+    var = {}
+    for fl, n in ((0, 1024), (16, 1024)):
+        s, q = sc.estimate_domain(dict(solver, flags=fl), pts, n, seed=3, threads=8)
+        m, se = mean_se(s, q, n)
+        z = (m - ex) / se
+        assert np.abs(z[:, 0]).max() < 4.5, (fl, np.abs(z[:, 0]).max())
+        assert np.abs(z[:, 1:]).max() < 5.0, (fl, np.abs(z[:, 1:]).max())
+        var[fl] = np.mean(se[:, 0] ** 2)
+    assert var[0] / var[16] > 1e4, var

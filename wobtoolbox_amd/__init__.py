@@ -163,9 +163,9 @@ class SceneHost:
     def fit_harmonic_cv(self, n_sources=64, radius_scale=1.5, n_train=16384, volume_samples=16, eps=1e-4,
                         ridge=1e-10, seed=0, stream_id=0, stream=None):
         """Fit and bind the harmonic control variate u_h = c0 + sum_j w_j phi(|x - s_j|^2) of
-        double-layer Dirichlet walks (wob_scene_fit_harmonic_cv) to the walk's known-term data
-        at n_train uniform boundary samples; volume_samples is ignored without a volume source.
-        Returns the RMS residual of the fit."""
+        WOB_EST_HARMONIC_CV walks (wob_scene_fit_harmonic_cv) to the walk's known-term data
+        at n_train uniform boundary samples; volume_samples is ignored without a volume
+        source.  Returns the RMS residual of the fit."""
         _torch()
         if not self._has_volume:
             volume_samples = 0
@@ -176,15 +176,15 @@ class SceneHost:
 
     def harmonic_cv_targets(self, n_train=16384, volume_samples=16, eps=1e-4, seed=0, stream_id=0, n_sources=1,
                             stream=None):
-        """The fit's training data: (positions (n, dim), targets (n,), NaN off Dirichlet elements)."""
+        """The fit's training data: BoundaryPoint records (n, 2 dim + 4) whose boundary value
+        is the target."""
         torch = _torch()
         if not self._has_volume:
             volume_samples = 0
         d = self._hcv_desc(n_sources, 1.5, n_train, volume_samples, eps, 0.0, seed, stream_id)
-        pos = torch.zeros((n_train, self.dim), dtype=torch.float32, device="cuda")
-        tgt = torch.zeros(n_train, dtype=torch.float32, device="cuda")
-        check(lib().wob_harmonic_cv_targets(self._h, C.byref(d), _ptr(pos), _ptr(tgt), _stream_ptr(stream)))
-        return pos, tgt
+        rec = torch.zeros((n_train, 2 * self.dim + 4), dtype=torch.float32, device="cuda")
+        check(lib().wob_harmonic_cv_targets(self._h, C.byref(d), _ptr(rec), _stream_ptr(stream)))
+        return rec
 
     def set_harmonic_cv(self, sources=None, c0=0.0):
         """Bind explicit sources (n, dim + 1): positions then weights; None unbinds."""

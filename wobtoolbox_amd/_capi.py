@@ -62,6 +62,7 @@ EST_EXCLUDE_ORIGIN_ELEMENT = 1
 EST_CONTROL_VARIATE = 2
 EST_BOUNDARY_DENSITY = 4
 EST_INDEPENDENT_VOLUME_SAMPLES = 8
+EST_HARMONIC_CV = 16
 VOLUME_NO_MASK = 1
 SCENE_DEVICE_BVH = 1
 
@@ -144,7 +145,7 @@ def lib():
         "wob_closest_points": ([P, P, U32, P, P, P, P], C.c_int),
         "wob_wos_estimate": ([P, P, U32, U32, U32, U64, U64, U32, F, U32, P, P, P], C.c_int),
         "wob_scene_fit_harmonic_cv": ([P, C.POINTER(HcvDesc), C.POINTER(F), P], C.c_int),
-        "wob_harmonic_cv_targets": ([P, C.POINTER(HcvDesc), P, P, P], C.c_int),
+        "wob_harmonic_cv_targets": ([P, C.POINTER(HcvDesc), P, P], C.c_int),
         "wob_scene_set_harmonic_cv": ([P, U32, P, F], C.c_int),
         "wob_scene_get_harmonic_cv": ([P, C.POINTER(U32), P, C.POINTER(F)], C.c_int),
         "wob_eval_harmonic_cv": ([P, P, U32, P, P], C.c_int),

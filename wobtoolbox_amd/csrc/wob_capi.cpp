@@ -149,15 +149,19 @@ wob_status check_est(const wob_scene *scene, const wob_estimator_desc *est, int 
     if (est->problem < 0 || est->problem > 3) return fail(WOB_INVALID_ENUM, "problem out of range");
     if (est->path_length == 0) return fail(WOB_INVALID_ARG, "path_length must be >= 1");
     if (est->flags & ~(uint32_t)(WOB_EST_EXCLUDE_ORIGIN_ELEMENT | WOB_EST_CONTROL_VARIATE | WOB_EST_BOUNDARY_DENSITY |
-                                 WOB_EST_INDEPENDENT_VOLUME_SAMPLES))
+                                 WOB_EST_INDEPENDENT_VOLUME_SAMPLES | WOB_EST_HARMONIC_CV))
         return fail(WOB_INVALID_ARG, "unknown estimator flags");
     if (est->flags & WOB_EST_CONTROL_VARIATE) {
-        if (est->estimator != WOB_BACKWARD || est->bie_model == WOB_DIRECT)
-            return fail(WOB_UNSUPPORTED_COMBO, "the control variate needs the backward estimator and the single-layer "
-                                               "(value table) or double-layer (harmonic function) model");
-        if (est->bie_model == WOB_SINGLE_LAYER && scene->dev.cv_R < est->path_length)
+        if (est->bie_model != WOB_SINGLE_LAYER || est->estimator != WOB_BACKWARD)
+            return fail(WOB_UNSUPPORTED_COMBO, "the control variate needs the single-layer model and the backward estimator");
+        if (scene->dev.cv_R < est->path_length)
             return fail(WOB_INVALID_ARG, "no control-variate table with path_length value functions is bound to the scene");
-        if (est->bie_model == WOB_DOUBLE_LAYER && scene->dev.hcv_m == 0)
+    }
+    if (est->flags & WOB_EST_HARMONIC_CV) {
+        if (est->bie_model == WOB_DIRECT || est->estimator != WOB_BACKWARD || est->domain != WOB_INTERIOR)
+            return fail(WOB_UNSUPPORTED_COMBO, "the harmonic control variate needs the single-layer or double-layer "
+                                               "model, the backward estimator and an interior domain");
+        if (scene->dev.hcv_m == 0)
             return fail(WOB_INVALID_ARG, "no harmonic control variate is bound to the scene "
                                          "(wob_scene_fit_harmonic_cv / wob_scene_set_harmonic_cv)");
     }
@@ -277,6 +281,9 @@ wob_status run_walk(const wob_scene *cscene, const wob_estimator_desc *est, cons
     if (st != WOB_OK) return st;
     int flavour = 0;
     if ((st = check_est(cscene, est, flavour)) != WOB_OK) return st;
+    if (boundary && (est->flags & WOB_EST_HARMONIC_CV) && !(est->flags & WOB_EST_BOUNDARY_DENSITY))
+        return fail(WOB_UNSUPPORTED_COMBO, "the harmonic control variate applies to domain estimates (and to boundary "
+                                           "density walks, which then walk the residual)");
     if (n_points == 0 || paths == 0) return WOB_OK;
     if (!d_points || !d_sum || !d_sumsq) return fail(WOB_INVALID_ARG, "null device buffer");
     wob_scene *scene = const_cast<wob_scene *>(cscene);
@@ -848,8 +855,8 @@ wob_status wob_vpl_paths(const wob_scene *cscene, const wob_estimator_desc *est,
                          wob_launch_config *launch, void *stream) {
     if (!cscene || !est) return fail(WOB_INVALID_ARG, "null scene or estimator descriptor");
     if (est->estimator != WOB_BACKWARD) return fail(WOB_UNSUPPORTED_COMBO, "VPL paths need the backward estimator");
-    if (est->flags & WOB_EST_CONTROL_VARIATE)
-        return fail(WOB_UNSUPPORTED_COMBO, "VPL paths do not take the control variate");
+    if (est->flags & (WOB_EST_CONTROL_VARIATE | WOB_EST_HARMONIC_CV))
+        return fail(WOB_UNSUPPORTED_COMBO, "VPL paths do not take a control variate");
     if (n_paths == 0) return WOB_OK;
     if (!d_records || !d_contrib) return fail(WOB_INVALID_ARG, "null device buffer");
     if (stream_id & (kVplSampleStream | 0x80000000u)) return fail(WOB_INVALID_ARG, "stream_id must be below 2^30");
@@ -1008,6 +1015,12 @@ wob_status wob_batch_create(const wob_batch_entry *entries, uint32_t n, const wo
         int fl = 0;
         wob_status st = check_est(e.scene, &e.est, fl);
         if (st != WOB_OK) { delete b; return fail(st, "entry " + std::to_string(i) + ": " + g_last_error); }
+        if (c.boundary && (e.est.flags & WOB_EST_HARMONIC_CV) && !(e.est.flags & WOB_EST_BOUNDARY_DENSITY)) {
+            delete b;
+            return fail(WOB_UNSUPPORTED_COMBO, "entry " + std::to_string(i) +
+                                                   ": the harmonic control variate applies to domain estimates and "
+                                                   "boundary density walks");
+        }
         if (e.n_points == 0 || e.paths_per_point == 0 || !e.d_points || !e.d_sum || !e.d_sumsq) {
             delete b;
             return fail(WOB_INVALID_ARG, "entry " + std::to_string(i) + ": empty point set or null buffer");

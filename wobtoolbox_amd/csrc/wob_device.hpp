@@ -1400,9 +1400,10 @@ struct WalkParams {
     uint32_t path_length;
     uint32_t nrc;
     uint32_t exclude_origin;   // WOB_EST_EXCLUDE_ORIGIN_ELEMENT
-    uint32_t cv;               // WOB_EST_CONTROL_VARIATE (single layer: value table; double layer: u_h)
+    uint32_t cv;               // WOB_EST_CONTROL_VARIATE (single-layer backward walks)
     uint32_t cv_samples;       // samples of the domain-level control-variate integral
     uint32_t density;          // WOB_EST_BOUNDARY_DENSITY: boundary estimates return the walk value
+    uint32_t hcv;              // WOB_EST_HARMONIC_CV: the walk carries the residual data of u_h
     __device__ __forceinline__ uint32_t skip(uint32_t elem) const { return exclude_origin ? elem : 0xFFFFFFFFu; }
 };
 
@@ -1466,6 +1467,24 @@ template <int D> __device__ __forceinline__ void hcv_gradient(const DevScene &S,
 #pragma unroll
         for (int k = 0; k < 2; ++k) g[k] = g[k] * 2.8853900817779268f;   // 2 / ln 2
     }
+}
+
+// The residual datum at a boundary point: the point's datum minus u_h's datum of the same
+// type (u_h at Dirichlet points, du_h/dn at Neumann points, du_h/dn + alpha u_h at Robin points).
+template <int D> __device__ __forceinline__ BP<D> hcv_residual(const DevScene &S, const BP<D> &y) {
+    BP<D> r = y;
+    if (y.type == kDirichlet) {
+        r.bv = y.bv - hcv_value<D>(S, y.p);
+    } else {
+        float g[D];
+        hcv_gradient<D>(S, y.p, g);
+        float dn = g[0] * y.n[0];
+#pragma unroll
+        for (int k = 1; k < D; ++k) dn += g[k] * y.n[k];
+        if (y.type == kNeumann) r.bv = y.bv - dn;
+        else r.bv = y.bv - (dn + y.alpha * hcv_value<D>(S, y.p));
+    }
+    return r;
 }
 
 // resampled importance sampling, scene.cuh:542-584 (target by the type of y_prev,
@@ -1554,11 +1573,11 @@ __device__ __forceinline__ Sample<D> ris(const DevScene &S, const Model<D, BIE> 
 //   Neumann / Robin y_m:              -(dsign / ift) |G| / M sum_i s_i V_r(z_i).
 // The walk's expectation is unchanged for any table V; with V equal to the value functions
 // of the walk the path-choice randomness cancels.
-// With the control variate on a double-layer Dirichlet walk (line/line flavour), the known
-// term at y_m uses the data g(y_m) - u_h(y_m) of the scene's harmonic function u_h, and the
-// domain estimate adds u_h(x): u - u_h solves the same problem with data g - u_h, so for any
-// u_h harmonic in the domain the expectation is unchanged (up to the walk's truncation, now of
-// the residual's series), and the variance falls with the size of g - u_h.
+// With the harmonic control variate (single and double layer), the known term at y_m uses the
+// residual datum of the scene's harmonic function u_h (hcv_residual), and the domain estimate
+// adds u_h(x): u - u_h solves the same problem with the residual data, so for any u_h harmonic
+// in the domain the expectation is unchanged (up to the walk's truncation, now of the
+// residual's series), and the variance falls with the size of the residual data.
 template <int D, int BIE, int FL, int STACK, typename VT>
 __device__ __forceinline__ float walk_backward(const DevScene &S, const Model<D, BIE> &m, const WalkParams &wp,
                                                const VT &vt, const BP<D> &x0, Rng &rng, uint32_t *stack,
@@ -1569,13 +1588,8 @@ __device__ __forceinline__ float walk_backward(const DevScene &S, const Model<D,
         if (it == wp.path_length - 1) W = 0.5f * W;
         if constexpr (FL == FL_LINE_LINE) {
             Sample<D> q = line_sample<D, STACK>(S, yp.p, wp.skip(yp.elem), rng, stack, stride);
-            if (wp.cv) {   // the harmonic control variate: the walk carries the data minus u_h
-                BP<D> yk = yp;
-                yk.bv = yp.bv - hcv_value<D>(S, yp.p);
-                spc += W * m.fredholm_known(yk, q.y, q.inv_pdf, vt, rng);
-            } else {
-                spc += W * m.fredholm_known(yp, q.y, q.inv_pdf, vt, rng);
-            }
+            if (wp.hcv) spc += W * m.fredholm_known(hcv_residual<D>(S, yp), q.y, q.inv_pdf, vt, rng);
+            else spc += W * m.fredholm_known(yp, q.y, q.inv_pdf, vt, rng);
             W *= m.fredholm_unknown_backward(yp, q.y, q.inv_pdf, rng);
             yp = q.y;
         } else if constexpr (FL == FL_LINE_BV) {
@@ -1585,7 +1599,9 @@ __device__ __forceinline__ float walk_backward(const DevScene &S, const Model<D,
             W = W * m.fredholm_unknown_backward(yp, q.y, q.inv_pdf, rng);
             yp = q.y;
         } else if constexpr (BIE == kSL) {
-            spc += W * m.fredholm_known(yp, yp, 0.f, vt, rng);   // SL ignores y here (estimator.cuh:278-281)
+            // SL ignores y here (estimator.cuh:278-281)
+            if (wp.hcv) spc += W * m.fredholm_known(hcv_residual<D>(S, yp), yp, 0.f, vt, rng);
+            else spc += W * m.fredholm_known(yp, yp, 0.f, vt, rng);
             const uint32_t cv_r = (wp.cv && it + 1 < wp.path_length) ? wp.path_length - it - 1 : 0u;
             float ksum = 0.f;
             Sample<D> q;
@@ -1654,8 +1670,8 @@ __device__ __forceinline__ void domain_path(const DevScene &S, const Model<D, BI
         out[0] = known + spc * unk;
 #pragma unroll
         for (int k = 0; k < D; ++k) out[1 + k] = gk[k] + spc * gu[k];
-        if constexpr (FL == FL_LINE_LINE) {
-            if (wp.cv) {   // u = u_h + the walk's estimate of u - u_h (data g - u_h, the same source)
+        if constexpr (BIE != kDirect) {
+            if (wp.hcv) {   // u = u_h + the walk's estimate of u - u_h (residual data, the same source)
                 float gh[D];
                 hcv_gradient<D>(S, x, gh);
                 out[0] += hcv_value<D>(S, x);

@@ -1,9 +1,10 @@
 // C ABI of the learned harmonic control variate (include/wob_c.h): a combination of
 // fundamental solutions u_h(x) = c0 + sum_j w_j phi(|x - s_j|^2) (phi(z) = z^(-1/2) in 3D,
 // log2 z in 2D) with its sources on a sphere (circle) outside the scene, fitted by FP64
-// least squares to the double-layer walk's own known-term data at uniform boundary samples
-// (the data plus the volume term as the estimator samples it, bie_models.cuh:389-399), and
-// bound to the scene for WOB_EST_CONTROL_VARIATE double-layer walks.
+// least squares to the walk's own known-term data at uniform boundary samples (the datum
+// plus the volume term as the estimator samples it, bie_models.cuh:301-316, 389-399), each
+// sample through the datum of its type (u_h, du_h/dn or du_h/dn + alpha u_h), and bound to
+// the scene for WOB_EST_HARMONIC_CV walks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -87,6 +88,33 @@ inline double phi(int D, const double *x, const double *s) {
     return D == 3 ? 1.0 / std::sqrt(z) : std::log2(z);
 }
 
+// d phi / dn at x: 3D -(x - s).n |x - s|^-3, 2D (2 / ln 2) (x - s).n / |x - s|^2
+inline double dphi_dn(int D, const double *x, const double *s, const double *n) {
+    double z = 0, dn = 0;
+    for (int k = 0; k < D; ++k) {
+        const double d = x[k] - s[k];
+        z += d * d;
+        dn += d * n[k];
+    }
+    return D == 3 ? -dn / (z * std::sqrt(z)) : 2.0 / std::log(2.0) * dn / z;
+}
+
+// The fit's row of one sample record [p, n, ia, target, type, alpha]: the datum of its type
+// of each basis function (1, phi_1 .. phi_m).
+void basis_row(int D, const float *rec, const std::vector<double> &src, uint32_t m, std::vector<double> &row) {
+    double x[3], n[3];
+    for (int k = 0; k < D; ++k) { x[k] = rec[k]; n[k] = rec[D + k]; }
+    const int type = (int)rec[2 * D + 2];
+    const double alpha = rec[2 * D + 3];
+    row[0] = type == WOB_DIRICHLET ? 1.0 : type == WOB_NEUMANN ? 0.0 : alpha;
+    for (uint32_t j = 0; j < m; ++j) {
+        const double *s = &src[(size_t)j * D];
+        if (type == WOB_DIRICHLET) row[1 + j] = phi(D, x, s);
+        else if (type == WOB_NEUMANN) row[1 + j] = dphi_dn(D, x, s, n);
+        else row[1 + j] = dphi_dn(D, x, s, n) + alpha * phi(D, x, s);
+    }
+}
+
 // In-place Cholesky solve of the SPD system A x = b (n x n, row-major); false if A is not
 // positive definite.
 bool cholesky_solve(std::vector<double> &A, std::vector<double> &b, int n) {
@@ -131,19 +159,18 @@ wob_status bind_sources(wob_scene *scene, const std::vector<float> &rec, uint32_
 
 extern "C" {
 
-wob_status wob_harmonic_cv_targets(const wob_scene *scene, const wob_hcv_desc *d, float *d_pos, float *d_target,
-                                   void *stream) {
+wob_status wob_harmonic_cv_targets(const wob_scene *scene, const wob_hcv_desc *d, float *d_records, void *stream) {
     wob_status st = check_desc(scene, d);
     if (st != WOB_OK) return st;
-    if (!d_pos || !d_target) return fail(WOB_INVALID_ARG, "null device buffer");
+    if (!d_records) return fail(WOB_INVALID_ARG, "null device buffer");
     const wobhost::HostScene &h = wob_scene_host(scene);
     const size_t lds = wobl::walk_lds_bytes(wobdev::FL_UNIFORM, true, 256);
     const float eps = d->volume_samples > 0 ? d->eps : 1.f;
     hipError_t e = h.dim == 2 ? wobl::launch_hcv_targets_2(wob_scene_dev(scene), d->n_train, d->seed, d->stream_id,
-                                                           d->volume_samples, eps, true, d_pos, d_target, lds,
+                                                           d->volume_samples, eps, true, d_records, lds,
                                                            (hipStream_t)stream)
                               : wobl::launch_hcv_targets_3(wob_scene_dev(scene), d->n_train, d->seed, d->stream_id,
-                                                           d->volume_samples, eps, true, d_pos, d_target, lds,
+                                                           d->volume_samples, eps, true, d_records, lds,
                                                            (hipStream_t)stream);
     WOB_HIP(e, "harmonic control-variate targets launch");
     return WOB_OK;
@@ -153,46 +180,34 @@ wob_status wob_scene_fit_harmonic_cv(wob_scene *scene, const wob_hcv_desc *d, fl
     wob_status st = check_desc(scene, d);
     if (st != WOB_OK) return st;
     const wobhost::HostScene &h = wob_scene_host(scene);
-    const int D = (int)h.dim;
+    const int D = (int)h.dim, W = 2 * D + 4;
     const uint32_t n = d->n_train, m = d->n_sources, P = m + 1;
     float *buf = nullptr;
-    WOB_HIP(hipMalloc((void **)&buf, (size_t)n * (D + 1) * sizeof(float)), "hipMalloc harmonic control-variate samples");
-    std::vector<float> pos((size_t)n * D), tgt(n);
-    st = wob_harmonic_cv_targets(scene, d, buf, buf + (size_t)n * D, stream);
+    WOB_HIP(hipMalloc((void **)&buf, (size_t)n * W * sizeof(float)), "hipMalloc harmonic control-variate samples");
+    std::vector<float> rec((size_t)n * W);
+    st = wob_harmonic_cv_targets(scene, d, buf, stream);
     hipError_t e = hipSuccess;
     if (st == WOB_OK) e = hipStreamSynchronize((hipStream_t)stream);
-    if (st == WOB_OK && e == hipSuccess) e = hipMemcpy(pos.data(), buf, pos.size() * sizeof(float), hipMemcpyDeviceToHost);
-    if (st == WOB_OK && e == hipSuccess)
-        e = hipMemcpy(tgt.data(), buf + (size_t)n * D, tgt.size() * sizeof(float), hipMemcpyDeviceToHost);
+    if (st == WOB_OK && e == hipSuccess) e = hipMemcpy(rec.data(), buf, rec.size() * sizeof(float), hipMemcpyDeviceToHost);
     (void)hipFree(buf);
     if (st != WOB_OK) return st;
     WOB_HIP(e, "harmonic control-variate samples");
 
     const std::vector<double> src = source_positions(h, m, d->radius_scale);
-    // normal equations of the basis [1, phi_1 .. phi_m] over the Dirichlet samples, columns
-    // scaled to unit RMS
+    // normal equations of the rows, columns scaled to unit RMS
     std::vector<double> row(P), G((size_t)P * P, 0.0), r(P, 0.0), scale(P, 0.0);
-    uint32_t used = 0;
-    auto basis = [&](uint32_t i) {
-        double x[3];
-        for (int k = 0; k < D; ++k) x[k] = pos[(size_t)i * D + k];
-        row[0] = 1.0;
-        for (uint32_t j = 0; j < m; ++j) row[1 + j] = phi(D, x, &src[(size_t)j * D]);
-    };
     for (uint32_t i = 0; i < n; ++i) {
-        if (!std::isfinite(tgt[i])) continue;
-        basis(i);
+        basis_row(D, &rec[(size_t)i * W], src, m, row);
         for (uint32_t a = 0; a < P; ++a) scale[a] += row[a] * row[a];
-        ++used;
     }
-    if (used < P) return fail(WOB_INVALID_ARG, "fewer Dirichlet boundary samples than coefficients");
-    for (uint32_t a = 0; a < P; ++a) scale[a] = scale[a] > 0 ? 1.0 / std::sqrt(scale[a] / used) : 1.0;
+    for (uint32_t a = 0; a < P; ++a) scale[a] = scale[a] > 0 ? 1.0 / std::sqrt(scale[a] / n) : 1.0;
     for (uint32_t i = 0; i < n; ++i) {
-        if (!std::isfinite(tgt[i])) continue;
-        basis(i);
+        const float *ri = &rec[(size_t)i * W];
+        basis_row(D, ri, src, m, row);
+        const double t = ri[2 * D + 1];
         for (uint32_t a = 0; a < P; ++a) row[a] *= scale[a];
         for (uint32_t a = 0; a < P; ++a) {
-            r[a] += row[a] * tgt[i];
+            r[a] += row[a] * t;
             const double ra = row[a];
             double *g = &G[(size_t)a * P];
             for (uint32_t b = 0; b <= a; ++b) g[b] += ra * row[b];
@@ -206,32 +221,34 @@ wob_status wob_scene_fit_harmonic_cv(wob_scene *scene, const wob_hcv_desc *d, fl
         for (uint32_t b = a + 1; b < P; ++b) G[(size_t)a * P + b] = G[(size_t)b * P + a];
     }
     if (!cholesky_solve(G, r, (int)P))
-        return fail(WOB_INVALID_ARG, "harmonic control-variate fit: singular normal equations (raise ridge)");
+        return fail(WOB_INVALID_ARG, "harmonic control-variate fit: singular normal equations (raise ridge, or the "
+                                     "samples hold no Dirichlet or Robin datum to fix the constant)");
     for (uint32_t a = 0; a < P; ++a) r[a] *= scale[a];
 
     // device records (FP32) and the fit's residual with the coefficients as bound
-    std::vector<float> rec((size_t)m * 4, 0.f);
+    std::vector<float> out((size_t)m * 4, 0.f);
+    std::vector<double> src32((size_t)m * D), coef(P);
+    coef[0] = (float)r[0];
     for (uint32_t j = 0; j < m; ++j) {
-        for (int k = 0; k < D; ++k) rec[(size_t)j * 4 + k] = (float)src[(size_t)j * D + k];
-        rec[(size_t)j * 4 + D] = (float)r[1 + j];
+        for (int k = 0; k < D; ++k) {
+            out[(size_t)j * 4 + k] = (float)src[(size_t)j * D + k];
+            src32[(size_t)j * D + k] = out[(size_t)j * 4 + k];
+        }
+        out[(size_t)j * 4 + D] = (float)r[1 + j];
+        coef[1 + j] = out[(size_t)j * 4 + D];
     }
-    const float c0 = (float)r[0];
     if (rms_residual) {
         double ss = 0;
         for (uint32_t i = 0; i < n; ++i) {
-            if (!std::isfinite(tgt[i])) continue;
-            double x[3], v = c0;
-            for (int k = 0; k < D; ++k) x[k] = pos[(size_t)i * D + k];
-            for (uint32_t j = 0; j < m; ++j) {
-                double s[3];
-                for (int k = 0; k < D; ++k) s[k] = rec[(size_t)j * 4 + k];
-                v += (double)rec[(size_t)j * 4 + D] * phi(D, x, s);
-            }
-            ss += (tgt[i] - v) * (tgt[i] - v);
+            const float *ri = &rec[(size_t)i * W];
+            basis_row(D, ri, src32, m, row);
+            double v = 0;
+            for (uint32_t a = 0; a < P; ++a) v += row[a] * coef[a];
+            ss += (ri[2 * D + 1] - v) * (ri[2 * D + 1] - v);
         }
-        *rms_residual = (float)std::sqrt(ss / used);
+        *rms_residual = (float)std::sqrt(ss / n);
     }
-    return bind_sources(scene, rec, m, c0);
+    return bind_sources(scene, out, m, (float)r[0]);
 }
 
 wob_status wob_scene_set_harmonic_cv(wob_scene *scene, uint32_t n_sources, const float *sources, float c0) {

@@ -114,12 +114,12 @@ WOB_DECLARE_WALK(3, 1)
 #undef WOB_DECLARE_WALK
 
 // Training targets of the harmonic control variate (wob_walk.hip, volume translation units):
-// n uniform boundary samples, positions (n * dim floats) and targets (n floats, NaN off
-// Dirichlet elements); lds_bytes = walk_lds_bytes(FL_UNIFORM, true, 256).
+// n uniform boundary samples as BoundaryPoint records (n * (2 dim + 4) floats) whose boundary
+// value is the target; lds_bytes = walk_lds_bytes(FL_UNIFORM, true, 256).
 hipError_t launch_hcv_targets_2(const wobdev::DevScene &S, uint32_t n, uint64_t seed, uint32_t stream, uint32_t nvs,
-                                float eps, bool interior, float *pos, float *target, size_t lds_bytes, hipStream_t st);
+                                float eps, bool interior, float *rec, size_t lds_bytes, hipStream_t st);
 hipError_t launch_hcv_targets_3(const wobdev::DevScene &S, uint32_t n, uint64_t seed, uint32_t stream, uint32_t nvs,
-                                float eps, bool interior, float *pos, float *target, size_t lds_bytes, hipStream_t st);
+                                float eps, bool interior, float *rec, size_t lds_bytes, hipStream_t st);
 // u_h and grad u_h of the scene's harmonic control variate at n points (out: n * (dim + 1))
 hipError_t launch_hcv_eval(int dim, const wobdev::DevScene &S, uint32_t n, const float *pts, float *out, hipStream_t st);
 

@@ -79,7 +79,8 @@ __device__ __forceinline__ void walk_item(const WalkLaunch &L, uint32_t item, ui
     m.k.eps = e.eps;
     m.dsign = e.dsign;
     m.c = e.fkesc;
-    const WalkParams wp{e.path_length, e.nrc, e.flags & 1u, (e.flags >> 1) & 1u, e.cv_samples, (e.flags >> 2) & 1u};
+    const WalkParams wp{e.path_length, e.nrc,           e.flags & 1u,        (e.flags >> 1) & 1u,
+                        e.cv_samples,  (e.flags >> 2) & 1u, (e.flags >> 4) & 1u};
     using VT = typename std::conditional<VOL, VolCtx<D, STACK>, NoVol>::type;
     VT vt;
     if constexpr (VOL) vt = VolCtx<D, STACK>{&S, e.nvs, (e.flags >> 3) & 1u, e.dsign > 0.f, stack, stride};
@@ -265,13 +266,10 @@ __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
             } else {                                                       // walk_backward, one bounce
                 Sample<D> qq = q;
                 if constexpr (FL == FL_LINE_LINE) {
-                    if (e.flags & 2u) {   // harmonic control variate, as in walk_backward
-                        BP<D> yk = yp;
-                        yk.bv = yp.bv - hcv_value<D>(e.scene, yp.p);
-                        spc += W * m.fredholm_known(yk, qq.y, qq.inv_pdf, NoVol{}, rng);
-                    } else {
+                    if (e.flags & 16u)   // harmonic control variate, as in walk_backward
+                        spc += W * m.fredholm_known(hcv_residual<D>(e.scene, yp), qq.y, qq.inv_pdf, NoVol{}, rng);
+                    else
                         spc += W * m.fredholm_known(yp, qq.y, qq.inv_pdf, NoVol{}, rng);
-                    }
                     W *= m.fredholm_unknown_backward(yp, qq.y, qq.inv_pdf, rng);
                 } else {
                     W = W * m.fredholm_unknown_backward(yp, qq.y, qq.inv_pdf, rng);
@@ -286,7 +284,7 @@ __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
 #pragma unroll
             for (int k = 0; k < D; ++k) out[1 + k] = gk[k] + spc * gu[k];
             if constexpr (FL == FL_LINE_LINE) {
-                if (e.flags & 2u) {
+                if (e.flags & 16u) {
                     float gh[D];
                     hcv_gradient<D>(e.scene, x, gh);
                     out[0] += hcv_value<D>(e.scene, x);
@@ -378,14 +376,13 @@ __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
 
 #if WOB_WALK_VOL
 // Training targets of the harmonic control variate.  Sample i is a uniform boundary point y
-// (Philox stream (seed, path i, point 0, stream)); its target is the quantity the
-// double-layer walk's known term carries at y (bie_models.cuh:389-399): the data g(y) plus
-// the volume term at y, here the mean of nvs independent volume samples (nvs = 0: no volume
-// source).  Samples off Dirichlet elements get a NaN target.
+// (Philox stream (seed, path i, point 0, stream)); its target is the quantity the walk's known
+// term carries at y (bie_models.cuh:301-316, 389-399): the datum g(y) plus the volume term at
+// y, here the mean of nvs independent volume samples (nvs = 0: no volume source).  Output: a
+// BoundaryPoint record [p, n, 0.5, target, type, alpha] per sample.
 template <int D>
 __global__ __launch_bounds__(256) void hcv_targets_kernel(DevScene S, uint32_t n, uint64_t seed, uint32_t stream,
-                                                          uint32_t nvs, float eps, uint32_t interior, float *pos,
-                                                          float *target) {
+                                                          uint32_t nvs, float eps, uint32_t interior, float *rec) {
     extern __shared__ uint32_t lds_stack[];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -396,9 +393,13 @@ __global__ __launch_bounds__(256) void hcv_targets_kernel(DevScene S, uint32_t n
     k.eps = eps;
     const VolCtx<D, wobl::kStackEntries> vt{&S, nvs, 1u, interior != 0u, lds_stack + threadIdx.x, blockDim.x};
     const float t = y.y.bv + vt.term(y.y, k, rng);
+    float *r = rec + (size_t)i * (2 * D + 4);
 #pragma unroll
-    for (int c = 0; c < D; ++c) pos[(size_t)i * D + c] = y.y.p[c];
-    target[i] = y.y.type == kDirichlet ? t : __builtin_nanf("");
+    for (int c = 0; c < D; ++c) { r[c] = y.y.p[c]; r[D + c] = y.y.n[c]; }
+    r[2 * D] = 0.5f;
+    r[2 * D + 1] = t;
+    r[2 * D + 2] = (float)y.y.type;
+    r[2 * D + 3] = y.y.alpha;
 }
 #endif
 
@@ -490,11 +491,11 @@ uint32_t WOB_NAME(persistent_grid_, WOB_WALK_DIM, WOB_WALK_VOL)(bool boundary, i
 #if WOB_WALK_VOL
 hipError_t WOB_NAME(launch_hcv_targets_, WOB_WALK_DIM, )(const wobdev::DevScene &S, uint32_t n, uint64_t seed,
                                                           uint32_t stream, uint32_t nvs, float eps, bool interior,
-                                                          float *pos, float *target, size_t lds_bytes, hipStream_t st) {
+                                                          float *rec, size_t lds_bytes, hipStream_t st) {
     if (n == 0) return hipSuccess;
     const uint32_t block = 256;
     hipLaunchKernelGGL((hcv_targets_kernel<WOB_WALK_DIM>), dim3((n + block - 1) / block), dim3(block), lds_bytes, st, S,
-                       n, seed, stream, nvs, eps, interior ? 1u : 0u, pos, target);
+                       n, seed, stream, nvs, eps, interior ? 1u : 0u, rec);
     return hipGetLastError();
 }
 #endif

@@ -13,10 +13,11 @@ import time
 
 
 def run(estimator, points, exact_u, paths_per_pass, time_budget_s, t0=None, seed=0, neumann_shift=False,
-        csv_path=None, max_passes=1_000_000, growth=1.25):
+        csv_path=None, max_passes=1_000_000, growth=1.25, stop_rmse=None):
     """Returns a list of (seconds since t0, paths per point, rmse_u, rms standard error of u)
-    checkpoints, taken after pass counts growing by `growth`.  With no bias, rmse_u and the rms
-    standard error agree; rmse^2 - se^2 estimates the squared bias.
+    checkpoints, taken after pass counts growing by `growth`, until the time budget is spent
+    or (stop_rmse) the first checkpoint with rmse_u <= stop_rmse.  With no bias, rmse_u and the
+    rms standard error agree; rmse^2 - se^2 estimates the squared bias.
 
     points: device tensor (n, dim); exact_u: device FP64 tensor (n,) of exact values.
     t0: time origin (perf_counter); setup work done before the call (training) counts.
@@ -46,7 +47,7 @@ def run(estimator, points, exact_u, paths_per_pass, time_budget_s, t0=None, seed
             rmse = float(torch.sqrt((err * err).mean()))
             se = float(torch.sqrt((torch.clamp(q[:, 0] / k - m * m, min=0) / k).mean()))
             curve.append((t, k, rmse, se))
-            if t >= time_budget_s:
+            if t >= time_budget_s or (stop_rmse is not None and rmse <= stop_rmse):
                 break
             next_check = max(passes + 1, int(math.ceil(passes * growth)))
     if csv_path:
