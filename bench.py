@@ -113,7 +113,7 @@ def roofline_block(name, n_elements, path_length, launch_paths, walk_ms, candida
     HBM peak) or the compute roof, "mfma" (model FP32 operations against the 157.3 TFLOP/s
     FP32 peak; the walk has no matrix instructions, it runs on the vector ALUs).  The other roof
     and the counter-measured HBM bytes are given beside it."""
-    from wobtoolbox_amd.roofline import BOUND, MEASURED_TRAFFIC, MI355X_PEAKS, workload_model
+    from wobtoolbox_amd.roofline import BOUND, MEASURED_TRAFFIC, MI355X_PEAKS, ris_per_path, workload_model
     flop, byts = workload_model(name, n_elements, path_length, candidates)
     s = walk_ms * 1e-3
     tflops = flop * launch_paths / s / 1e12
@@ -124,9 +124,15 @@ def roofline_block(name, n_elements, path_length, launch_paths, walk_ms, candida
             "frac": tflops / MI355X_PEAKS["fp32_TFLOPs"], "unit_note": "FP32 vector ALU"}
     bound = BOUND[name]
     main, other = (mem, comp) if bound == "hbm" else (comp, mem)
+    extra = {}
+    if name == "geodesic_mixed_3d":
+        # the fixed figure with a binary CDF descent per candidate instead of the guide record
+        _, desc = ris_per_path(3, n_elements, path_length, candidates, descent=True)
+        extra = {"binary_descent_model_bytes_per_path": desc,
+                 "binary_descent_model_GBps": desc * launch_paths / s / 1e9}
     return {"bound": bound, **{k: main[k] for k in ("achieved", "peak", "unit", "frac")}, "traffic": traffic,
             "kernel": "walk_kernel", "avg_kernel_ms": walk_ms,
-            "algorithmic_flop_per_path": flop, "algorithmic_bytes_per_path": byts,
+            "algorithmic_flop_per_path": flop, "algorithmic_bytes_per_path": byts, **extra,
             "fp32_valu" if bound == "hbm" else "model_bytes": other,
             "measured_hbm": None if traffic is None else
             {"bytes_per_launch": traffic, "achieved": traffic / s / 1e9, "unit": "GB/s",

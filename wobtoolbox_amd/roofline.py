@@ -5,7 +5,8 @@ sin/cos = 1) and compulsory record bytes for an ideal binary BVH descent:
   per line query with k hits: 2 L k node tests (F_node ops, B_node bytes each) and 2 k
   primitive tests (F_prim, B_prim), plus F_q of per-query work (direction, pick, inv_pdf);
   per bounce F_w of weight / kernel evaluation; per CDF sample L + 15 compares and
-  interpolation, 4 L + B_prim bytes.
+  interpolation, 4 L + B_prim bytes (binary descent), or B_GUIDE + B_prim bytes for the RIS
+  candidates, whose sampler is an O(1) guide-table lookup.
 L = ceil(log2 N_elements).  The mean hit counts k were measured once with the CPU oracle on
 20,000 random lines per origin type (seed 0) and are frozen here:
   circle_dirichlet_2d    k = 2.000 from interior points, 1.008 from boundary points
@@ -35,15 +36,23 @@ def per_path(dim, n_elements, path_length, k_interior, k_boundary, cdf_samples_p
     return flop, byts
 
 
-def ris_per_path(dim, n_elements, path_length, candidates):
+# Bytes of one guide-table record of an O(1) inverse-CDF sample: the element index and the CDF
+# values around it (i, cdf[i-1], cdf[i], cdf[i+1]).
+B_GUIDE = 16
+
+
+def ris_per_path(dim, n_elements, path_length, candidates, descent=False):
     """(flop, bytes) per path of a single-layer walk with RIS over `candidates` uniform samples
-    per bounce: each candidate is one CDF sample (L + 15 ops, 4 L + B_prim bytes) and a kernel
-    evaluation with its reservoir update (25 ops); path_length bounces plus the domain sample."""
+    per bounce: each candidate is one CDF sample (L + 15 ops) and a kernel evaluation with its
+    reservoir update (25 ops); path_length bounces plus the domain sample.  Bytes per
+    candidate: the element's record (B_prim) and, for the CDF search, one guide-table record
+    (B_GUIDE; the sampler is O(1)) or, with descent=True, the L 4-byte CDF entries of a binary
+    descent (the fixed figure of an ideal binary search)."""
     p = PARAMS[dim]
     L = math.ceil(math.log2(n_elements))
     bounces = path_length + 1
     flop = bounces * candidates * (L + 15 + 25)
-    byts = bounces * candidates * (4 * L + p["B_prim"])
+    byts = bounces * candidates * ((4 * L if descent else B_GUIDE) + p["B_prim"])
     return flop, byts
 
 
@@ -72,9 +81,8 @@ MEASURED_TRAFFIC = {
 
 # What bounds each workload's walk kernel (rocprofv3 counters): the RIS walk's scattered
 # candidate loads keep the texture data unit 98 % busy and read L2 misses at 94 % of the HBM
-# peak (about one 128-byte line per candidate; the model's 148 B per candidate assume a binary
-# CDF descent that the guide table skips, so the model's byte rate can exceed the peak); the
-# line-sampled walk keeps the vector ALU about 90 % busy.
+# peak (about one 128-byte line per candidate against 92 algorithmic bytes); the line-sampled
+# walk keeps the vector ALU about 90 % busy.
 BOUND = {"geodesic_mixed_3d": "hbm", "icosphere_neumann_3d": "mfma", "circle_dirichlet_2d": "mfma"}
 
 MI355X_PEAKS = {"hbm_GBps": 8000.0, "fp32_TFLOPs": 157.3, "bf16_TFLOPs": 2500.0}
