@@ -243,9 +243,14 @@ def icosphere_leg(args, torch, wob, G):
     print("[bench] icosphere leg", file=sys.stderr, flush=True)
     w = G.icosphere_neumann_3d()
     paths = w.solver["num_sample_paths"]
-    scene = wob.SceneHost(w.elements)
-    est = wob.Estimator.from_config(scene, w.solver)
     pts = torch.as_tensor(w.points).to("cuda:0")
+    # the BVH width from the autotuner (a 16-path walk at every point on each width)
+    t0 = time.perf_counter()
+    width, width_ms = wob.tune_bvh_width(w.elements, w.solver, pts, 16)
+    tune_s = time.perf_counter() - t0
+    print(f"[bench] icosphere BVH width {width} (walk ms by width {width_ms})", file=sys.stderr, flush=True)
+    scene = wob.SceneHost(w.elements, bvh_width=width)
+    est = wob.Estimator.from_config(scene, w.solver)
     value, ms, walk_ms, s, q, k = walk_timing(est, pts, paths, args.leg_steps, 2)
     m = (s[:, 0] / k).cpu().numpy()
     var = np.maximum((q[:, 0] / k).cpu().numpy() - m * m, 0)
@@ -256,6 +261,7 @@ def icosphere_leg(args, torch, wob, G):
             "roofline": roofline_block(w.name, w.elements.n, w.solver["path_length"], len(w.points) * paths, walk_ms),
             "accuracy": {"rmse_u": float(np.sqrt((d ** 2).mean())), "mean_se_u": float(np.sqrt(var / k).mean()),
                          "paths_per_point": k, "note": "Neumann problem: u compared after removing the mean offset"},
+            "bvh_width": {"tuned": width, "walk_ms_by_width": width_ms, "tune_s": tune_s},
             "solver": "Direct, InteriorDomain, BackwardEstimator, NeumannProblem, path_length 6"}
 
 

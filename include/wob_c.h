@@ -89,19 +89,24 @@ wob_status wob_scene_create(const wob_scene_desc *desc, wob_scene **out_scene);
  * WOB_DEVICE if the device build fails; WOB_INVALID_ARG for unknown flags or a stack bound
  * over 48 entries.                                                                          */
 enum { WOB_SCENE_DEVICE_BVH = 1 };
+/* WOB_SCENE_BVH_WIDTH(w), w = 2, 3 or 4 (default 4): every BVH node takes at most w children
+ * (the node layout stays 4-wide; a narrower tree is deeper).  wob_tune_bvh_width picks w.   */
+#define WOB_SCENE_BVH_WIDTH_SHIFT 4
+#define WOB_SCENE_BVH_WIDTH_MASK (15u << WOB_SCENE_BVH_WIDTH_SHIFT)
+#define WOB_SCENE_BVH_WIDTH(w) ((uint32_t)(w) << WOB_SCENE_BVH_WIDTH_SHIFT)
 wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_scene **out_scene);
 wob_status wob_scene_destroy(wob_scene *scene);
 
 /* Scene facts: dim, element count, BVH node count, BVH depth, totals (area, boundary value),
- * device memory held, and the traversal stack entries a line query of this BVH needs.       */
+ * device memory held, the traversal stack entries a line query of this BVH needs and the
+ * most children a BVH node takes.                                                           */
 typedef struct wob_scene_info {
     uint32_t dim, n_elements, n_nodes, depth;
     float total_area, total_boundary_value;
     uint64_t device_bytes;
-    uint32_t stack_need, reserved;
+    uint32_t stack_need, bvh_width;
 } wob_scene_info;
 wob_status wob_scene_get_info(const wob_scene *scene, wob_scene_info *info);
-
 /* Replaces SceneHost::set_volume_cache(grid_res, domain_size, cache) (scene.cuh:301-306,
  * 206-240): a grid source over (-domain_size/2, domain_size/2)^dim; estimators with
  * num_volume_samples > 0 sample a cell with probability |cache| / sum |cache| and a uniform
@@ -327,6 +332,17 @@ typedef struct wob_batch_config {
 wob_status wob_batch_create(const wob_batch_entry *entries, uint32_t n_entries, const wob_batch_config *cfg,
                             wob_batch **out_batch);
 wob_status wob_batch_run(wob_batch *batch, uint64_t seed, uint32_t n_passes, void *stream);
+/* BVH width autotuner (no counterpart in the reference; the reference's lbvh is binary):
+ * builds the scene (desc, flags as in wob_scene_create_ex without a width) with nodes of at
+ * most 2, 3 and 4 children, runs one untimed and one timed estimate of paths_per_point paths
+ * at the device points with each (the walk kernel's time from HIP events on its stream),
+ * and returns the fastest width in *best_width.  ms_by_width (host, 3 floats, may be NULL):
+ * the times for widths 2, 3, 4, NaN where the scene cannot be built (a stack bound over 48).
+ * Walk results do not depend on the width.  Plain walks without a volume term only.        */
+wob_status wob_tune_bvh_width(const wob_scene_desc *desc, uint32_t flags, const wob_estimator_desc *est,
+                              const float *d_points, uint32_t n_points, uint32_t paths_per_point, uint64_t seed,
+                              uint32_t *best_width, float *ms_by_width, void *stream);
+
 /* Autotuner: times one pass for each candidate (paths per lane x block size x persistent)
  * into scratch outputs (the entries' outputs are not touched), keeps the fastest and
  * reports it in *best (may be NULL) with its pass time in *best_ms.                          */

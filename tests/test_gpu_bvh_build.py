@@ -151,3 +151,47 @@ def test_device_bvh_flags(gpu):
     h = C.c_void_p()
     with pytest.raises(w.WobError):
         _capi.check(_capi.lib().wob_scene_create_ex(C.byref(d), 2, C.byref(h)))
+
+
+@pytest.mark.parametrize("case", [0, 1, 3])
+@pytest.mark.parametrize("device_bvh", [False, True])
+def test_bvh_width_results_identical(case, device_bvh, gpu):
+    """Nodes of at most 2, 3 or 4 children (WOB_SCENE_BVH_WIDTH): different trees, the same
+    walk sums and hit sets bit for bit."""
+    import wobtoolbox_amd as w
+    name, el, solver, pts, paths = _walk_cases()[case]
+    x = _dev(pts.astype(np.float32), gpu)
+    rng = np.random.default_rng(3)
+    o = _dev(pts[:512].astype(np.float32), gpu)
+    d = rng.normal(size=(512, el.dim))
+    d = _dev((d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32), gpu)
+    ref = None
+    for width in (4, 3, 2):
+        sc = w.SceneHost(el, device_bvh=device_bvh, bvh_width=width)
+        info = sc.info()
+        est = w.Estimator.from_config(sc, solver)
+        s, q = est.compute_domain(x, paths, seed=5)
+        cnt, ids, ts = sc.intersect_lines(o, d, max_hits=64)
+        res = [a.cpu().numpy() for a in (s, q, cnt)]
+        hits = [sorted(zip(i[:c].tolist(), t[:c].view(np.uint32).tolist()))
+                for i, t, c in zip(ids.cpu().numpy(), ts.cpu().numpy(), res[2])]
+        _say(f"{name} width {width} ({'device' if device_bvh else 'host'} build): {info['n_nodes']} nodes, "
+             f"depth {info['depth']}, stack bound {info['stack_need']}")
+        assert info["bvh_width"] == width
+        if ref is None:
+            ref = (res, hits)
+            continue
+        assert all(np.array_equal(a, b) for a, b in zip(ref[0], res)) and ref[1] == hits
+
+
+def test_bvh_width_tuner(gpu):
+    import wobtoolbox_amd as w
+    ico = G.icosphere_neumann_3d(n_points=4000, level=4)
+    pts = _dev(ico.points.astype(np.float32), gpu)
+    best, ms = w.tune_bvh_width(ico.elements, ico.solver, pts, 16)
+    _say(f"icosphere BVH width tuner: best {best}, walk ms by width {ms}")
+    assert best in (2, 3, 4) and np.isfinite(ms[best]) and ms[best] == min(v for v in ms.values() if np.isfinite(v))
+    with pytest.raises(w.WobError, match="volume"):
+        w.tune_bvh_width(ico.elements, dict(ico.solver, num_volume_samples=1), pts, 4)
+    with pytest.raises(w.WobError, match="width"):
+        w.SceneHost(ico.elements, bvh_width=5)

@@ -21,7 +21,8 @@ from . import _capi
 from ._capi import WobError, check, lib
 from .geometry import Elements
 
-__all__ = ["SceneHost", "Estimator", "Batch", "WobError", "Elements", "string_to_bie_model_type", "string_to_domain_type",
+__all__ = ["SceneHost", "Estimator", "Batch", "WobError", "Elements", "tune_bvh_width", "string_to_bie_model_type",
+           "string_to_domain_type",
            "string_to_estimator_type", "string_to_problem_type", "string_to_equation_kernel_type",
            "bie_model_type_to_string", "domain_type_to_string", "estimator_type_to_string",
            "problem_type_to_string", "Dirichlet", "Neumann", "Robin"]
@@ -73,21 +74,42 @@ def _stream_ptr(stream):
     return C.c_void_p(s.cuda_stream)
 
 
+def _scene_desc(el):
+    return _capi.SceneDesc(el.dim, el.n, el.vertices.ctypes.data, el.normals.ctypes.data,
+                           el.boundary_values.ctypes.data, el.boundary_types.ctypes.data, el.robin_alphas.ctypes.data)
+
+
+def tune_bvh_width(elements: Elements, solver: dict, points, paths_per_point, seed=0, device_bvh=False, stream=None):
+    """BVH width autotuner (wob_tune_bvh_width): times a walk of the solver (a reference JSON
+    config dict) at the device points on scenes with nodes of at most 2, 3 and 4 children.
+    Returns (best width, {width: walk ms or nan})."""
+    _torch()
+    el = elements.astype_contiguous()
+    d = _scene_desc(el)
+    ed = Estimator.from_config(None, solver).desc()
+    pts = points.contiguous()
+    best = C.c_uint32()
+    ms = (C.c_float * 3)()
+    check(lib().wob_tune_bvh_width(C.byref(d), _capi.SCENE_DEVICE_BVH if device_bvh else 0, C.byref(ed), _ptr(pts),
+                                   pts.shape[0], int(paths_per_point), int(seed), C.byref(best), ms,
+                                   _stream_ptr(stream)))
+    return best.value, {w: float(ms[w - 2]) for w in (2, 3, 4)}
+
+
 class SceneHost:
     """Device scene: wide BVH, CDFs and element data (SceneHost, scene.cuh:264-326)."""
 
-    def __init__(self, elements: Elements, device_bvh: bool = False):
+    def __init__(self, elements: Elements, device_bvh: bool = False, bvh_width: int = 4):
         """device_bvh: build the BVH on the GPU (wob_scene_create_ex, WOB_SCENE_DEVICE_BVH)
-        instead of the host's SAH build; walks give the same results either way."""
+        instead of the host's SAH build; bvh_width: the most children a node takes (2..4; see
+        tune_bvh_width).  Walks give the same results either way."""
         _torch()
         el = elements.astype_contiguous()
         self.elements = el
         self.dim = el.dim
-        d = _capi.SceneDesc(el.dim, el.n, el.vertices.ctypes.data, el.normals.ctypes.data,
-                            el.boundary_values.ctypes.data, el.boundary_types.ctypes.data,
-                            el.robin_alphas.ctypes.data)
+        d = _scene_desc(el)
         h = C.c_void_p()
-        flags = _capi.SCENE_DEVICE_BVH if device_bvh else 0
+        flags = (_capi.SCENE_DEVICE_BVH if device_bvh else 0) | _capi.scene_bvh_width(bvh_width)
         check(lib().wob_scene_create_ex(C.byref(d), flags, C.byref(h)))
         self._h = h
         self._has_volume = False

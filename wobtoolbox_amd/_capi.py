@@ -24,7 +24,7 @@ EXPORTS = ["wob_scene_create", "wob_scene_create_ex", "wob_scene_destroy", "wob_
            "wob_cv_set_params", "wob_cv_train", "wob_cv_gradient", "wob_cv_infer", "wob_scene_set_control_variate",
            "wob_sample_boundary", "wob_sample_lines", "wob_vpl_paths", "wob_vpl_aggregate",
            "wob_closest_points", "wob_wos_estimate", "wob_scene_fit_harmonic_cv", "wob_harmonic_cv_targets",
-           "wob_scene_set_harmonic_cv", "wob_scene_get_harmonic_cv", "wob_eval_harmonic_cv"]
+           "wob_scene_set_harmonic_cv", "wob_scene_get_harmonic_cv", "wob_eval_harmonic_cv", "wob_tune_bvh_width"]
 
 
 class WobError(RuntimeError):
@@ -41,7 +41,7 @@ class SceneDesc(C.Structure):
 class SceneInfo(C.Structure):
     _fields_ = [("dim", C.c_uint32), ("n_elements", C.c_uint32), ("n_nodes", C.c_uint32), ("depth", C.c_uint32),
                 ("total_area", C.c_float), ("total_boundary_value", C.c_float), ("device_bytes", C.c_uint64),
-                ("stack_need", C.c_uint32), ("reserved", C.c_uint32)]
+                ("stack_need", C.c_uint32), ("bvh_width", C.c_uint32)]
 
 
 class EstimatorDesc(C.Structure):
@@ -65,6 +65,12 @@ EST_INDEPENDENT_VOLUME_SAMPLES = 8
 EST_HARMONIC_CV = 16
 VOLUME_NO_MASK = 1
 SCENE_DEVICE_BVH = 1
+SCENE_BVH_WIDTH_SHIFT = 4
+
+
+def scene_bvh_width(w):
+    """WOB_SCENE_BVH_WIDTH(w)"""
+    return int(w) << SCENE_BVH_WIDTH_SHIFT
 
 
 class BatchEntry(C.Structure):
@@ -149,6 +155,8 @@ def lib():
         "wob_scene_set_harmonic_cv": ([P, U32, P, F], C.c_int),
         "wob_scene_get_harmonic_cv": ([P, C.POINTER(U32), P, C.POINTER(F)], C.c_int),
         "wob_eval_harmonic_cv": ([P, P, U32, P, P], C.c_int),
+        "wob_tune_bvh_width": ([C.POINTER(SceneDesc), U32, C.POINTER(EstimatorDesc), P, U32, U32, U64,
+                                C.POINTER(U32), P, P], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -185,7 +185,7 @@ std::vector<float> guide_records(const std::vector<float> &cdf, const std::vecto
     return r;
 }
 
-std::string build_scene(const wob_scene_desc &d, HostScene &S, bool with_bvh) {
+std::string build_scene(const wob_scene_desc &d, HostScene &S, bool with_bvh, int width) {
     if (d.dim != 2 && d.dim != 3) return "dim must be 2 or 3";
     if (d.n_elements == 0) return "scene has no elements";
     if (!d.vertices || !d.normals || !d.boundary_values || !d.boundary_types) return "null element array";
@@ -271,7 +271,7 @@ std::string build_scene(const wob_scene_desc &d, HostScene &S, bool with_bvh) {
         B.median_only = attempt == 1;
         root = B.build(0, (int)n, 0);
 
-        // ---- collapse to 4-wide
+        // ---- collapse to 4-wide nodes with at most `width` children
         std::vector<WNode> wn;
         std::vector<int> wn_of_bnode;
         // Each wide node is created for a binary internal node (or the root even if it is a leaf).
@@ -282,7 +282,7 @@ std::string build_scene(const wob_scene_desc &d, HostScene &S, bool with_bvh) {
             std::vector<int> kids;
             if (B.nodes[bn].leaf()) kids.push_back(bn);
             else { kids.push_back(B.nodes[bn].left); kids.push_back(B.nodes[bn].right); }
-            while (kids.size() < 4) {
+            while ((int)kids.size() < width) {
                 int best = -1;
                 float bestA = -1.f;
                 for (int i = 0; i < (int)kids.size(); ++i)

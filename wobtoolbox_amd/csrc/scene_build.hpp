@@ -55,12 +55,13 @@ struct HostScene {
     std::vector<uint32_t> area_guide, bv_guide;
     float total_area = 0, total_bv = 0;
     uint32_t n_nodes = 0, depth = 0, stack_need = 0;
+    uint32_t bvh_width = 4;     // the most children a node has (2..4)
 };
 
 // Returns an empty string on success, otherwise an error message.  Without with_bvh only the
 // element attributes and the CDFs are built (the BVH arrays stay empty: the device build
-// supplies them).
-std::string build_scene(const wob_scene_desc &desc, HostScene &out, bool with_bvh = true);
+// supplies them).  width: the most children a node takes (2..4; the layout stays 4-wide).
+std::string build_scene(const wob_scene_desc &desc, HostScene &out, bool with_bvh = true, int width = 4);
 
 // Quantized copy of float nodes (kNodeWords{2,3} each) in the kQNodeWords{2,3} layout.
 void quantize_nodes(int dim, const std::vector<float> &nodes, std::vector<uint32_t> &qnodes);

@@ -17,7 +17,7 @@
 //      depth-first order (primitive records follow that order);
 //   5. top-down, one level per launch: each binary node of the level becomes a wide node
 //      whose children are its binary children, expanded largest-measure-first until there
-//      are four (the host's collapse rule); child indices of the next level come from an
+//      are `width` (four by default; the host's collapse rule); child indices of the next level come from an
 //      exclusive scan of the internal-child counts, so the layout is deterministic; the
 //      float node and its quantized copy are written in the same kernel;
 //   6. the traversal stack bound, bottom-up over the levels;
@@ -401,7 +401,7 @@ __global__ __launch_bounds__(256) void order_kernel(int n, const uint32_t *__res
 }
 
 // children of binary node b as a wide node (host rule: expand the internal child of largest
-// measure, first one on ties, until there are four or only leaves)
+// measure, first one on ties, until there are `width` (2..4) or only leaves)
 // A subtree of at most kLeafPrims primitives (a contiguous range in depth-first order)
 // becomes one leaf.
 constexpr uint32_t kLeafPrims = 2;
@@ -411,12 +411,12 @@ __device__ __forceinline__ bool is_inner(uint32_t id, uint32_t n_int, const uint
 
 template <int D>
 __device__ __forceinline__ int expand(int n, uint32_t b, const uint32_t *left, const uint32_t *right, const Box6 *box,
-                                      const uint32_t *count, uint32_t kids[4]) {
+                                      const uint32_t *count, int width, uint32_t kids[4]) {
     const uint32_t n_int = (uint32_t)(n - 1);
     int nk;
     if (!is_inner(b, n_int, count)) { kids[0] = b; nk = 1; }
     else { kids[0] = left[b]; kids[1] = right[b]; nk = 2; }
-    while (nk < 4) {
+    while (nk < width) {
         int best = -1;
         float bestA = -1.f;
         for (int i = 0; i < nk; ++i)
@@ -436,11 +436,11 @@ template <int D>
 __global__ __launch_bounds__(256) void count_kernel(int n, uint32_t m, const uint32_t *__restrict__ level,
                                                     const uint32_t *__restrict__ left, const uint32_t *__restrict__ right,
                                                     const Box6 *__restrict__ box, const uint32_t *__restrict__ count,
-                                                    uint32_t *__restrict__ counts) {
+                                                    int width, uint32_t *__restrict__ counts) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m) return;
     uint32_t kids[4];
-    const int nk = expand<D>(n, level[t], left, right, box, count, kids);
+    const int nk = expand<D>(n, level[t], left, right, box, count, width, kids);
     uint32_t c = 0;
     for (int i = 0; i < nk; ++i) c += is_inner(kids[i], (uint32_t)(n - 1), count) ? 1u : 0u;
     counts[t] = c;
@@ -480,15 +480,16 @@ __global__ __launch_bounds__(256) void write_kernel(int n, uint32_t m, uint32_t 
                                                     const uint32_t *__restrict__ level, const uint32_t *__restrict__ offs,
                                                     const uint32_t *__restrict__ left, const uint32_t *__restrict__ right,
                                                     const Box6 *__restrict__ box, const uint32_t *__restrict__ first,
-                                                    const uint32_t *__restrict__ count, uint32_t *__restrict__ next,
-                                                    float *__restrict__ fnodes, uint32_t *__restrict__ qnodes) {
+                                                    const uint32_t *__restrict__ count, int width,
+                                                    uint32_t *__restrict__ next, float *__restrict__ fnodes,
+                                                    uint32_t *__restrict__ qnodes) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m) return;
     constexpr int NW = D == 2 ? kNodeWords2 : kNodeWords3;
     constexpr int QW = D == 2 ? kQNodeWords2 : kQNodeWords3;
     const uint32_t n_int = (uint32_t)(n - 1);
     uint32_t kids[4];
-    const int nk = expand<D>(n, level[t], left, right, box, count, kids);
+    const int nk = expand<D>(n, level[t], left, right, box, count, width, kids);
     const uint32_t w = base + t;
     float *nd = fnodes + (size_t)w * NW;
     uint32_t *q = qnodes + (size_t)w * QW;
@@ -764,7 +765,7 @@ hipError_t radix_sort(uint32_t n, uint32_t bits, uint32_t *k0, uint32_t *v0, uin
     return hipSuccess;
 }
 
-template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBvh &out) {
+template <int D> std::string build(uint32_t n, const float *h_vertices, int width, DeviceBvh &out) {
     constexpr int NW = D == 2 ? kNodeWords2 : kNodeWords3;
     constexpr int QW = D == 2 ? kQNodeWords2 : kQNodeWords3;
     constexpr int PW = D == 2 ? kPrimWords2 : kPrimWords3;
@@ -871,7 +872,7 @@ template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBv
     while (m > 0) {
         if (level_base.size() > 256) return "device BVH build: tree deeper than 256 levels";
         hipLaunchKernelGGL((count_kernel<D>), dim3(blocks(m)), dim3(256), 0, st, (int)n, m, cur, left, right, box, count,
-                           counts);
+                           width, counts);
         WOB_TRY(hipGetLastError());
         WOB_TRY(exclusive_scan(counts, offs, m, scan_tmp, st));
         uint32_t last[2] = {0u, 0u};
@@ -879,7 +880,7 @@ template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBv
         WOB_TRY(hipMemcpyAsync(&last[1], offs + (m - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         const uint32_t next_base = base + m;
         hipLaunchKernelGGL((write_kernel<D>), dim3(blocks(m)), dim3(256), 0, st, (int)n, m, base, next_base, cur, offs, left,
-                           right, box, first, count, nxt, fnodes, qnodes);
+                           right, box, first, count, width, nxt, fnodes, qnodes);
         WOB_TRY(hipGetLastError());
         WOB_TRY(hipStreamSynchronize(st));
         const uint32_t m_next = last[0] + last[1];
@@ -915,9 +916,10 @@ template <int D> std::string build(uint32_t n, const float *h_vertices, DeviceBv
 
 }  // namespace
 
-std::string build_bvh_device(int dim, uint32_t n, const float *h_vertices, DeviceBvh &out) {
+std::string build_bvh_device(int dim, uint32_t n, const float *h_vertices, DeviceBvh &out, int width) {
     out = DeviceBvh{};
-    std::string err = dim == 2 ? build<2>(n, h_vertices, out) : build<3>(n, h_vertices, out);
+    if (width < 2 || width > 4) return "BVH width must be 2, 3 or 4";
+    std::string err = dim == 2 ? build<2>(n, h_vertices, width, out) : build<3>(n, h_vertices, width, out);
     if (!err.empty()) {
         (void)hipFree(out.fnodes);
         (void)hipFree(out.qnodes);

@@ -506,13 +506,17 @@ wob_status wob_scene_create(const wob_scene_desc *desc, wob_scene **out_scene) {
 wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_scene **out_scene) {
     if (!desc || !out_scene) return fail(WOB_INVALID_ARG, "null argument");
     *out_scene = nullptr;
-    if (flags & ~(uint32_t)WOB_SCENE_DEVICE_BVH) return fail(WOB_INVALID_ARG, "unknown scene flags");
+    if (flags & ~(uint32_t)(WOB_SCENE_DEVICE_BVH | WOB_SCENE_BVH_WIDTH_MASK))
+        return fail(WOB_INVALID_ARG, "unknown scene flags");
     const bool device_bvh = (flags & WOB_SCENE_DEVICE_BVH) != 0u;
+    const uint32_t wfield = (flags & WOB_SCENE_BVH_WIDTH_MASK) >> WOB_SCENE_BVH_WIDTH_SHIFT;
+    const int width = wfield == 0 ? 4 : (int)wfield;
+    if (width < 2 || width > 4) return fail(WOB_INVALID_ARG, "BVH width must be 2, 3 or 4");
     wob_scene *s = new (std::nothrow) wob_scene();
     if (!s) return fail(WOB_OOM, "host allocation failed");
     std::string err;
     try {
-        err = wobhost::build_scene(*desc, s->host, !device_bvh);
+        err = wobhost::build_scene(*desc, s->host, !device_bvh, width);
     } catch (const std::bad_alloc &) {
         delete s;
         return fail(WOB_OOM, "host allocation failed during scene build");
@@ -522,9 +526,10 @@ wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_s
         return fail(WOB_INVALID_ARG, err);
     }
     (void)hipGetDevice(&s->device);
+    s->host.bvh_width = (uint32_t)width;
     wobl::DeviceBvh dv;
     if (device_bvh) {
-        err = wobl::build_bvh_device((int)desc->dim, desc->n_elements, desc->vertices, dv);
+        err = wobl::build_bvh_device((int)desc->dim, desc->n_elements, desc->vertices, dv, width);
         if (!err.empty()) {
             delete s;
             return fail(WOB_DEVICE, err);
@@ -627,7 +632,55 @@ wob_status wob_scene_get_info(const wob_scene *scene, wob_scene_info *info) {
     info->total_boundary_value = scene->host.total_bv;
     info->device_bytes = scene->device_bytes;
     info->stack_need = scene->host.stack_need;
-    info->reserved = 0;
+    info->bvh_width = scene->host.bvh_width;
+    return WOB_OK;
+}
+
+wob_status wob_tune_bvh_width(const wob_scene_desc *desc, uint32_t flags, const wob_estimator_desc *est,
+                              const float *d_points, uint32_t n_points, uint32_t paths_per_point, uint64_t seed,
+                              uint32_t *best_width, float *ms_by_width, void *stream) {
+    if (!desc || !est || !best_width) return fail(WOB_INVALID_ARG, "null argument");
+    if (flags & WOB_SCENE_BVH_WIDTH_MASK) return fail(WOB_INVALID_ARG, "the tuner chooses the BVH width");
+    if (est->num_volume_samples != 0)
+        return fail(WOB_UNSUPPORTED_COMBO, "the BVH width tuner times walks without a volume source");
+    if (est->flags & (WOB_EST_CONTROL_VARIATE | WOB_EST_HARMONIC_CV))
+        return fail(WOB_UNSUPPORTED_COMBO, "the BVH width tuner times plain walks");
+    if (n_points == 0 || paths_per_point == 0 || !d_points) return fail(WOB_INVALID_ARG, "empty point set");
+    const size_t V = (size_t)desc->dim + 1;
+    double *acc = nullptr;
+    WOB_HIP(hipMalloc((void **)&acc, 2 * (size_t)n_points * V * sizeof(double)), "hipMalloc tuner outputs");
+    float ms[3];
+    std::string last;
+    for (int w = 2; w <= 4; ++w) {
+        ms[w - 2] = NAN;
+        wob_scene *sc = nullptr;
+        if (wob_scene_create_ex(desc, flags | WOB_SCENE_BVH_WIDTH(w), &sc) != WOB_OK) {
+            last = g_last_error;
+            continue;
+        }
+        wob_launch_config lc{};
+        wob_status st = wob_estimate_domain(sc, est, d_points, n_points, 0, paths_per_point, seed, 0, 0, acc,
+                                            acc + (size_t)n_points * V, &lc, stream);
+        lc = wob_launch_config{};
+        lc.flags = WOB_LAUNCH_TIME_WALK;
+        if (st == WOB_OK)
+            st = wob_estimate_domain(sc, est, d_points, n_points, 0, paths_per_point, seed, paths_per_point, 0, acc,
+                                     acc + (size_t)n_points * V, &lc, stream);
+        wob_scene_destroy(sc);
+        if (st != WOB_OK) {
+            (void)hipFree(acc);
+            return st;
+        }
+        ms[w - 2] = lc.walk_ms;
+    }
+    (void)hipFree(acc);
+    int best = 0;
+    for (int w = 2; w <= 4; ++w)
+        if (ms[w - 2] == ms[w - 2] && (best == 0 || ms[w - 2] < ms[best - 2])) best = w;
+    if (ms_by_width)
+        for (int i = 0; i < 3; ++i) ms_by_width[i] = ms[i];
+    if (best == 0) return fail(WOB_INVALID_ARG, "no BVH width gives a usable scene: " + last);
+    *best_width = (uint32_t)best;
     return WOB_OK;
 }
 

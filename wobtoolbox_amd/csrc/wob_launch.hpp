@@ -130,9 +130,9 @@ bool supports_wavefront(int dim, bool boundary, bool volume, int bie, int est, i
 // LDS bytes of a walk launch (per-lane traversal stacks)
 size_t walk_lds_bytes(int flavour, bool volume, uint32_t block);
 
-// Device BVH build (wob_bvh.hip): the float and quantized 4-wide nodes, the primitive
-// records and the leaf-order element ids of a scene, built on the current device from host
-// vertex data (n * dim * dim floats).  The buffers are device allocations owned by the
+// Device BVH build (wob_bvh.hip): the float and quantized 4-wide nodes (at most `width` = 2..4
+// children each), the primitive records and the leaf-order element ids of a scene, built on
+// the current device from host vertex data (n * dim * dim floats).  The buffers are device allocations owned by the
 // caller afterwards.  Returns an empty string on success.
 struct DeviceBvh {
     float *fnodes = nullptr;
@@ -142,6 +142,6 @@ struct DeviceBvh {
     uint32_t n_nodes = 0, depth = 0, stack_need = 0;
     size_t bytes = 0;
 };
-std::string build_bvh_device(int dim, uint32_t n, const float *h_vertices, DeviceBvh &out);
+std::string build_bvh_device(int dim, uint32_t n, const float *h_vertices, DeviceBvh &out, int width = 4);
 
 }  // namespace wobl
