@@ -184,9 +184,11 @@ template <int D> struct Kern;
 
 template <> struct Kern<2> {
     float eps;
+    // The logarithm is correctly rounded (wobtrig::log_cr), so G stays within 2 FP32 ulps of
+    // any FP32 log that is within one ulp of the exact value, as the host C library's is.
     __device__ __forceinline__ float G(const float *y, const float *x) const {
         const float r0 = y[0] - x[0], r1 = y[1] - x[1];
-        return -1.0f / (2.0f * kPi) * logf(fmaxf(sqrtf(r0 * r0 + r1 * r1), eps));
+        return -1.0f / (2.0f * kPi) * wobtrig::log_cr(fmaxf(sqrtf(r0 * r0 + r1 * r1), eps));
     }
     __device__ __forceinline__ void dG_dx(const float *y, const float *x, float *out) const {
         const float r[2] = {y[0] - x[0], y[1] - x[1]};
@@ -1471,19 +1473,29 @@ template <int D> __device__ __forceinline__ void hcv_gradient(const DevScene &S,
 
 // The residual datum at a boundary point: the point's datum minus u_h's datum of the same
 // type (u_h at Dirichlet points, du_h/dn at Neumann points, du_h/dn + alpha u_h at Robin points).
+// Out of line, with the sources passed as scalars: the walks call it only with the control
+// variate on, and inlined its registers would count against every walk.
+template <int D>
+__device__ __noinline__ float hcv_residual_datum(const float4 *src, uint32_t m, float c0, float p0, float p1, float p2,
+                                                 float n0, float n1, float n2, int type, float alpha, float bv) {
+    DevScene S{};
+    S.hcv_src = src;
+    S.hcv_m = m;
+    S.hcv_c0 = c0;
+    const float p[3] = {p0, p1, p2}, n[3] = {n0, n1, n2};
+    if (type == kDirichlet) return bv - hcv_value<D>(S, p);
+    float g[D];
+    hcv_gradient<D>(S, p, g);
+    float dn = g[0] * n[0];
+#pragma unroll
+    for (int k = 1; k < D; ++k) dn += g[k] * n[k];
+    if (type == kNeumann) return bv - dn;
+    return bv - (dn + alpha * hcv_value<D>(S, p));
+}
 template <int D> __device__ __forceinline__ BP<D> hcv_residual(const DevScene &S, const BP<D> &y) {
     BP<D> r = y;
-    if (y.type == kDirichlet) {
-        r.bv = y.bv - hcv_value<D>(S, y.p);
-    } else {
-        float g[D];
-        hcv_gradient<D>(S, y.p, g);
-        float dn = g[0] * y.n[0];
-#pragma unroll
-        for (int k = 1; k < D; ++k) dn += g[k] * y.n[k];
-        if (y.type == kNeumann) r.bv = y.bv - dn;
-        else r.bv = y.bv - (dn + y.alpha * hcv_value<D>(S, y.p));
-    }
+    r.bv = hcv_residual_datum<D>(S.hcv_src, S.hcv_m, S.hcv_c0, y.p[0], y.p[1], D == 3 ? y.p[D - 1] : 0.f, y.n[0],
+                                 y.n[1], D == 3 ? y.n[D - 1] : 0.f, y.type, y.alpha, y.bv);
     return r;
 }
 

@@ -1,5 +1,6 @@
 // Double-precision sine and cosine of small non-negative arguments (0 <= x <= 8), for the
-// line directions and Box-Muller angles of the walk.
+// line directions and Box-Muller angles of the walk, and a correctly rounded FP32 natural
+// logarithm for the 2D fundamental solution (log_cr, at the end).
 //
 // The reference's CPU build evaluates a direction as std::cos(theta * M_PI) and
 // std::sin(theta * M_PI) in double and rounds each to float (scene.cuh:475-476, 785-786).
@@ -59,6 +60,45 @@ WOB_HD inline void sincos_small(double x, double &s, double &c) {
     const double s0 = (k & 1) ? cr : sr, c0 = (k & 1) ? sr : cr;
     s = (k & 2) ? -s0 : s0;
     c = (k == 1 || k == 2) ? -c0 : c0;
+}
+
+// Natural logarithm of a positive normal float, rounded to FP32 from a double evaluation
+// with relative error about 2^-47: correctly rounded except where the exact logarithm lies
+// that close to a rounding boundary (two of the floats in [2^-26, 2^12), where the result is
+// the neighbouring float).
+//   x = 2^e m, m in [1/sqrt 2, sqrt 2) (frexp, exact);
+//   log m = 2 atanh s = 2 s (1 + s^2/3 + s^4/5 + ... + s^18/19), s = (m - 1)/(m + 1), |s| < 0.172
+//     (truncation below 2^-50);
+//   1/(m + 1): the FP32 IEEE quotient refined by one Newton step in double (error ~2^-46);
+//   log x = 2 s p + e ln 2.
+// IEEE operations only, contraction off, so host and device compute the same bits.
+WOB_HD inline float log_cr(float x) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    int e;
+    float mf = __builtin_frexpf(x, &e);                       // mf in [0.5, 1)
+    if (mf < 0.70710678f) {
+        mf = 2.0f * mf;
+        e -= 1;
+    }
+    const double m = (double)mf, t = m - 1.0, d = m + 1.0;
+    double r = (double)(1.0f / (float)d);
+    const double dr = d * r;
+    r = r * (2.0 - dr);
+    const double s = t * r, z = s * s;
+    double p = 1.0 / 19.0;
+    p = 1.0 / 17.0 + z * p;
+    p = 1.0 / 15.0 + z * p;
+    p = 1.0 / 13.0 + z * p;
+    p = 1.0 / 11.0 + z * p;
+    p = 1.0 / 9.0 + z * p;
+    p = 1.0 / 7.0 + z * p;
+    p = 1.0 / 5.0 + z * p;
+    p = 1.0 / 3.0 + z * p;
+    p = 1.0 + z * p;
+    const double s2 = 2.0 * s;
+    return (float)(s2 * p + (double)e * 0.69314718055994530942);
 }
 
 }  // namespace wobtrig
