@@ -7,7 +7,10 @@ the large meshes are checked against the oracle at their real size:
   report, hit sets of 3,000 lines against brute force, and the single-layer RIS-64 walk of
   the control-variate workload path by path against the oracle;
 * the 64-scene star batch's largest polygon (14,974 segments): its single-layer RIS walk path
-  by path against the oracle.
+  by path against the oracle;
+* the Poisson workload on the same mesh (double layer, Dirichlet, Gaussian-mixture source
+  with the inside test) path by path against the oracle, plain and with the harmonic control
+  variate, and the control variate on the RIS walk of the control-variate workload.
 """
 import sys
 
@@ -89,8 +92,8 @@ def test_geodesic_200k_ris_walk_matches_oracle(geodesic, gpu, oracle):
 
 def test_largest_star_ris_walk_matches_oracle(gpu, oracle):
     """The batch workload's largest scene (single layer, exterior, RIS 64) path for path; its 2D
-    kernel takes a log (device logf against the host's), so a RIS choice decided by the last
-    bit of a weight may differ on a few paths."""
+    kernel takes a logarithm (correctly rounded on the device, the host C library's logf in the
+    oracle), so a RIS choice decided by the last bit of a weight may differ on a few paths."""
     import wobtoolbox_amd as w
     ws = G.star_exterior_batch_2d()
     big = max(ws, key=lambda x: x.elements.n)
@@ -107,3 +110,71 @@ def test_largest_star_ris_walk_matches_oracle(gpu, oracle):
     _say(f"largest star ({big.elements.n} segments, {big.solver['problem']}): points differing > 1e-5 rel "
          f"{frac:.4f}; max |z| {z.max():.3f}")
     assert frac <= 0.05 and z.max() < 0.5
+
+
+def _compare(m, se, mo, seo, rel=1e-5):
+    diff = np.abs(m - mo)
+    frac = (diff > rel * (np.abs(mo) + se + 1e-3)).mean()
+    z = diff / np.sqrt(se ** 2 + seo ** 2 + 1e-30)
+    return frac, z.max()
+
+
+def test_poisson_200k_walk_matches_oracle(gpu, oracle):
+    """The Poisson workload at its size (200,000 triangles, 16 Gaussians, double layer, path
+    length 4, one volume sample per term with the inside test), path for path against the
+    oracle, plain and with the harmonic control variate fitted on the device."""
+    import wobtoolbox_amd as w
+    wl = G.poisson_gaussian_3d(point_sets=[64])
+    el = wl.elements
+    src = (wl.extra["centers"], wl.extra["widths"], wl.extra["amplitudes"])
+    scene = w.SceneHost(el)
+    scene.set_volume_gaussians(*src)
+    osc = oracle.OracleScene(el)
+    osc.set_volume_gaussians(*src)
+    rms = scene.fit_harmonic_cv(n_sources=64, n_train=16384, volume_samples=16, eps=wl.solver["eps"], seed=7)
+    rec, c0 = scene.harmonic_cv()
+    osc.set_harmonic_cv(rec, c0)
+    pts = wl.points[:64]
+    paths = 32
+    for flags in (0, 16):
+        solver = dict(wl.solver, flags=flags)
+        est = w.Estimator.from_config(scene, solver)
+        s, q = est.compute_domain(_dev(pts, gpu), paths, seed=19)
+        so, qo = osc.estimate_domain(solver, pts, paths, seed=19, pick_mode=1, threads=8)
+        m, se = mean_se(s.cpu().numpy(), q.cpu().numpy(), paths)
+        mo, seo = mean_se(so, qo, paths)
+        frac, zmax = _compare(m, se, mo, seo)
+        _say(f"200k Poisson walk flags {flags}: points differing > 1e-5 rel {frac:.4f}; max |z| {zmax:.4f} "
+             f"(fit residual {rms:.4f})")
+        assert frac <= 0.02 and zmax < 0.05
+
+
+def test_geodesic_200k_harmonic_cv_walk_matches_oracle(geodesic, gpu, oracle):
+    """The control-variate workload's RIS walk with the harmonic control variate (128 sources)
+    on the full mesh, path for path.  u_h's terms take the hardware reciprocal square root on
+    the device and an IEEE division in the oracle (a few FP32 ulps apart), and the walk
+    multiplies the residual data (about 1e-5 of the data) by weights of order 200, so the two
+    sides agree to a small fraction of the standard error rather than to rounding."""
+    import wobtoolbox_amd as w
+    el = geodesic.elements
+    scene = w.SceneHost(el)
+    rms = scene.fit_harmonic_cv(n_sources=128, radius_scale=2.0, n_train=32768, seed=4)
+    rec, c0 = scene.harmonic_cv()
+    osc = oracle.OracleScene(el)
+    osc.set_harmonic_cv(rec, c0)
+    pts = geodesic.points[:64]
+    paths = 16
+    solver = dict(geodesic.solver, flags=16)
+    est = w.Estimator.from_config(scene, solver)
+    s, q = est.compute_domain(_dev(pts, gpu), paths, seed=29)
+    so, qo = osc.estimate_domain(solver, pts, paths, seed=29, pick_mode=1, threads=8)
+    m, se = mean_se(s.cpu().numpy(), q.cpu().numpy(), paths)
+    mo, seo = mean_se(so, qo, paths)
+    frac, zmax = _compare(m, se, mo, seo)
+    ex = geodesic.exact(pts)[:, 0]
+    med = np.median(np.abs(m - mo) / (np.abs(mo) + 1e-9))
+    _say(f"200k RIS walk with the harmonic control variate: points differing > 1e-5 rel {frac:.4f}; max |z| "
+         f"{zmax:.4f}; median rel diff {med:.2e}; fit residual {rms:.2e}; max |u - exact| "
+         f"{np.abs(m[:, 0] - ex).max():.2e} at {paths} paths")
+    assert zmax < 0.5 and med < 1e-5
+    assert np.abs((m[:, 0] - ex) / se[:, 0]).max() < 4.5
