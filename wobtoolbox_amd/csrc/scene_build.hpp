@@ -75,10 +75,11 @@ void build_cdf(const std::vector<float> &weights, std::vector<float> &cdf, float
 void build_guide(const std::vector<float> &cdf, std::vector<uint32_t> &guide, size_t entries = 0);
 
 // Fused area-guide records of a 3D scene: entries = kFusedPerElement * n; record k holds
-// (i = guide[k], cdf[i - 1], cdf[i], cdf[i + 1]) and then element i's geometry (the first
-// three rows of its kGeoWords3 record), 64 bytes in all.  A uniform boundary sample whose
-// search ends at i (about seven times in eight with four entries per element) reads this one
-// record.  The host builds the guide (area_fused_guide); the device assembles the records
+// (i = guide[k], cdf[i - 1], cdf[i], cdf[i + 1]) and then the geometry (the first three rows
+// of its kGeoWords3 record) of i or i + 1, whichever covers more of the entry's interval
+// (flagged in bit 31 of the index word), 64 bytes in all.  A uniform boundary sample whose
+// search ends at that element (about fifteen times in sixteen with four entries per element)
+// reads this one record.  The host builds the guide (area_fused_guide); the device assembles the records
 // from it, the area CDF and the geometry table (launch_fused_records).
 constexpr int kFusedPerElement = 4;
 

@@ -432,9 +432,10 @@ __device__ __forceinline__ GeomSample<D> uniform_geometry_of(const DevScene &S, 
     return g;
 }
 
-// 3D: the element from the fused record of entry floor(u1 n_fused), whose geometry rows
-// are element i's; the search usually ends at i, otherwise the element's own geometry
-// record is read.  The same draws and arithmetic as uniform_geometry_of.
+// 3D: the element from the fused record of entry floor(u1 n_fused), whose geometry rows are
+// those of element i or i + 1 (bit 31 of the index word), the one covering more of the entry;
+// the search usually ends there, otherwise the element's own geometry record is read.  The
+// same draws and arithmetic as uniform_geometry_of.
 template <int D> __device__ __forceinline__ GeomSample<D> uniform_geometry(const DevScene &S, Rng &rng) {
     const float u1 = rng.uniform();
     if constexpr (D == 2) {
@@ -446,8 +447,9 @@ template <int D> __device__ __forceinline__ GeomSample<D> uniform_geometry(const
         const float4 *rec = S.area_fused + 4 * (size_t)k;
         const float4 g = gld4(rec);
         float4 f0 = gld4(rec + 1), f1 = gld4(rec + 2), f2 = gld4(rec + 3);
-        const uint32_t e = cdf_search(S.area_cdf, S.n, u1, g);
-        if (e != __float_as_uint(g.x)) {
+        const uint32_t w = __float_as_uint(g.x), i = w & 0x7FFFFFFFu;
+        const uint32_t e = cdf_search(S.area_cdf, S.n, u1, make_float4(__uint_as_float(i), g.y, g.z, g.w));
+        if (e != i + (w >> 31)) {
             f0 = gld4(S.geo + 4 * e); f1 = gld4(S.geo + 4 * e + 1); f2 = gld4(S.geo + 4 * e + 2);
         }
         GeomSample<D> gs;

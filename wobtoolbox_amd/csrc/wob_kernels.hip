@@ -342,15 +342,24 @@ hipError_t launch_hcv_eval(int dim, const DevScene &S, uint32_t n, const float *
     return hipGetLastError();
 }
 
+// Record k of the fused area-guide table: row 0 is the guide row (i = guide[k], cdf[i - 1],
+// cdf[i], cdf[i + 1]); rows 1-3 are the geometry of the element that covers more of entry k's
+// interval [k/m, (k+1)/m), i or i + 1 (the search for a uniform in the interval ends there
+// more often); bit 31 of the index word says which (element indices stay below 2^27).
 __global__ void fused_records_kernel(const uint32_t *guide, uint32_t m, const float *cdf, uint32_t n, const float4 *geo,
                                      float4 *out) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m) return;
     const uint32_t i = guide[k];
-    out[4 * (size_t)k] = make_float4(__uint_as_float(i), i > 0 ? cdf[i - 1] : 0.f, cdf[i], i + 1 < n ? cdf[i + 1] : 0.f);
-    out[4 * (size_t)k + 1] = geo[4 * (size_t)i];
-    out[4 * (size_t)k + 2] = geo[4 * (size_t)i + 1];
-    out[4 * (size_t)k + 3] = geo[4 * (size_t)i + 2];
+    const float lo = (float)k / (float)m, hi = (float)(k + 1) / (float)m;
+    const float ci = cdf[i], cn = i + 1 < n ? cdf[i + 1] : ci;
+    const bool next = i + 1 < n && fminf(cn, hi) - fmaxf(ci, lo) > fminf(ci, hi) - lo;
+    const uint32_t g = next ? i + 1 : i;
+    out[4 * (size_t)k] = make_float4(__uint_as_float(i | (next ? 0x80000000u : 0u)), i > 0 ? cdf[i - 1] : 0.f, ci,
+                                     i + 1 < n ? cn : 0.f);
+    out[4 * (size_t)k + 1] = geo[4 * (size_t)g];
+    out[4 * (size_t)k + 2] = geo[4 * (size_t)g + 1];
+    out[4 * (size_t)k + 3] = geo[4 * (size_t)g + 2];
 }
 
 hipError_t launch_fused_records(const uint32_t *guide, uint32_t m, const float *cdf, uint32_t n, const float4 *geo,
