@@ -78,10 +78,11 @@ void build_guide(const std::vector<float> &cdf, std::vector<uint32_t> &guide, si
 // (i = guide[k], cdf[i - 1], cdf[i], cdf[i + 1]) and then the geometry (the first three rows
 // of its kGeoWords3 record) of i or i + 1, whichever covers more of the entry's interval
 // (flagged in bit 31 of the index word), 64 bytes in all.  A uniform boundary sample whose
-// search ends at that element (about fifteen times in sixteen with four entries per element)
-// reads this one record.  The host builds the guide (area_fused_guide); the device assembles the records
+// search ends at that element (about seven times in eight with two entries per element)
+// reads this one record.  Two entries per element (25.6 MB at 200k triangles) walked faster
+// than four (51 MB, one sample in sixteen reading a second record) and than one.  The host builds the guide (area_fused_guide); the device assembles the records
 // from it, the area CDF and the geometry table (launch_fused_records).
-constexpr int kFusedPerElement = 4;
+constexpr int kFusedPerElement = 2;
 
 // Guide records for the device search: per k four words (i = guide[k], cdf[i - 1], cdf[i],
 // cdf[i + 1]) (0 where the index is outside the CDF), so that the usual search that ends
