@@ -71,17 +71,17 @@ def workload_model(name, n_elements, path_length, candidates=64):
 # 262,161 KiB: x 2.0; WRITE_SIZE 524,288 KiB: x 1.0).  FETCH_SIZE counts L2 misses, served by
 # the MALL or HBM.
 #   geodesic_mixed_3d (1,000,000 points x 16 paths, single layer, RIS 64, 4 bounces):
-#     FETCH_SIZE 274,380,511 KiB, WRITE_SIZE 31,834 KiB per launch (75.1 ms)
+#     FETCH_SIZE 257,465,393 KiB, WRITE_SIZE 31,834 KiB per launch (69.2 ms)
 #   icosphere_neumann_3d (100,000 points x 256 paths, direct Neumann, 6 bounces):
 #     FETCH_SIZE 122,634 KiB, WRITE_SIZE 70,919 KiB per launch
 MEASURED_TRAFFIC = {
-    "geodesic_mixed_3d": (2.0 * 274380511 + 31834) * 1024,
+    "geodesic_mixed_3d": (2.0 * 257465393 + 31834) * 1024,
     "icosphere_neumann_3d": (2.0 * 122634 + 70919) * 1024,
 }
 
 # What bounds each workload's walk kernel (rocprofv3 counters): the RIS walk's scattered
-# candidate loads keep the texture data unit 98 % busy and read L2 misses at 94 % of the HBM
-# peak (about one 128-byte line per candidate against 92 algorithmic bytes); the line-sampled
+# candidate loads keep the texture data unit 98 % busy and read L2 misses at 95 % of the HBM
+# peak (0.8 lines of 128 bytes per candidate against 92 algorithmic bytes); the line-sampled
 # walk keeps the vector ALU about 90 % busy.
 BOUND = {"geodesic_mixed_3d": "hbm", "icosphere_neumann_3d": "mfma", "circle_dirichlet_2d": "mfma"}
 
