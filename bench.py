@@ -181,7 +181,7 @@ def time_to_rmse_leg(args, torch, wob, G, w, scene, pts):
     torch.cuda.synchronize()
     t_hfit = time.perf_counter() - t0h
     hest = wob.Estimator.from_config(scene, w.solver, flags=HCV)
-    curve_h = progressive.run(hest, pts, exact_u, args.cv_paths_per_pass, t_hfit + args.cv_budget, t0=t0h, seed=1,
+    curve_h = progressive.run(hest, pts, exact_u, args.hcv_paths_per_pass, t_hfit + args.cv_budget, t0=t0h, seed=1,
                               stop_rmse=1e-3)
     scene.set_harmonic_cv(None)
     target = max(curve_plain[-1][2], curve_cv[-1][2])
@@ -216,6 +216,7 @@ def time_to_rmse_leg(args, torch, wob, G, w, scene, pts):
             "harmonic_cv": {"s_to_target": th, "speedup_vs_plain": None if (tp is None or th is None) else tp / th,
                             "fit_s": t_hfit, "fit_rms_residual": hfit, "n_sources": args.hcv_sources,
                             "radius_scale": args.hcv_radius, "n_train": args.hcv_train,
+                            "paths_per_pass": args.hcv_paths_per_pass,
                             "paths_per_s": rate(curve_h, t_hfit),
                             "kind": "fundamental solutions outside the domain, FP64 least squares on the walk's "
                                     "boundary data at uniform boundary samples (value rows at Dirichlet points, "
@@ -550,6 +551,7 @@ def main():
     ap.add_argument("--hcv-sources", type=int, default=128, help="harmonic control variate: fundamental solutions")
     ap.add_argument("--hcv-radius", type=float, default=2.0, help="harmonic control variate: source radius / scene radius")
     ap.add_argument("--hcv-train", type=int, default=32768, help="harmonic control variate: boundary samples of the fit")
+    ap.add_argument("--hcv-paths-per-pass", type=int, default=1, help="harmonic control variate: paths per point per pass")
     ap.add_argument("--profile-copy-mib", type=int, default=0,
                     help="after the timed steps, copy this many MiB once (calibrates HBM byte counters)")
     args = ap.parse_args()

@@ -4,7 +4,8 @@
 // least squares to the walk's own known-term data at uniform boundary samples (the datum
 // plus the volume term as the estimator samples it, bie_models.cuh:301-316, 389-399), each
 // sample through the datum of its type (u_h, du_h/dn or du_h/dn + alpha u_h), and bound to
-// the scene for WOB_EST_HARMONIC_CV walks.
+// the scene for WOB_EST_HARMONIC_CV walks.  The samples and the FP64 Gram matrix of the normal
+// equations are formed on the device; the host scales the columns and solves by Cholesky.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -79,42 +80,6 @@ std::vector<double> source_positions(const wobhost::HostScene &h, uint32_t m, do
     return s;
 }
 
-inline double phi(int D, const double *x, const double *s) {
-    double z = 0;
-    for (int k = 0; k < D; ++k) {
-        const double d = x[k] - s[k];
-        z += d * d;
-    }
-    return D == 3 ? 1.0 / std::sqrt(z) : std::log2(z);
-}
-
-// d phi / dn at x: 3D -(x - s).n |x - s|^-3, 2D (2 / ln 2) (x - s).n / |x - s|^2
-inline double dphi_dn(int D, const double *x, const double *s, const double *n) {
-    double z = 0, dn = 0;
-    for (int k = 0; k < D; ++k) {
-        const double d = x[k] - s[k];
-        z += d * d;
-        dn += d * n[k];
-    }
-    return D == 3 ? -dn / (z * std::sqrt(z)) : 2.0 / std::log(2.0) * dn / z;
-}
-
-// The fit's row of one sample record [p, n, ia, target, type, alpha]: the datum of its type
-// of each basis function (1, phi_1 .. phi_m).
-void basis_row(int D, const float *rec, const std::vector<double> &src, uint32_t m, std::vector<double> &row) {
-    double x[3], n[3];
-    for (int k = 0; k < D; ++k) { x[k] = rec[k]; n[k] = rec[D + k]; }
-    const int type = (int)rec[2 * D + 2];
-    const double alpha = rec[2 * D + 3];
-    row[0] = type == WOB_DIRICHLET ? 1.0 : type == WOB_NEUMANN ? 0.0 : alpha;
-    for (uint32_t j = 0; j < m; ++j) {
-        const double *s = &src[(size_t)j * D];
-        if (type == WOB_DIRICHLET) row[1 + j] = phi(D, x, s);
-        else if (type == WOB_NEUMANN) row[1 + j] = dphi_dn(D, x, s, n);
-        else row[1 + j] = dphi_dn(D, x, s, n) + alpha * phi(D, x, s);
-    }
-}
-
 // In-place Cholesky solve of the SPD system A x = b (n x n, row-major); false if A is not
 // positive definite.
 bool cholesky_solve(std::vector<double> &A, std::vector<double> &b, int n) {
@@ -181,74 +146,74 @@ wob_status wob_scene_fit_harmonic_cv(wob_scene *scene, const wob_hcv_desc *d, fl
     if (st != WOB_OK) return st;
     const wobhost::HostScene &h = wob_scene_host(scene);
     const int D = (int)h.dim, W = 2 * D + 4;
-    const uint32_t n = d->n_train, m = d->n_sources, P = m + 1;
-    float *buf = nullptr;
-    WOB_HIP(hipMalloc((void **)&buf, (size_t)n * W * sizeof(float)), "hipMalloc harmonic control-variate samples");
-    std::vector<float> rec((size_t)n * W);
-    st = wob_harmonic_cv_targets(scene, d, buf, stream);
-    hipError_t e = hipSuccess;
-    if (st == WOB_OK) e = hipStreamSynchronize((hipStream_t)stream);
-    if (st == WOB_OK && e == hipSuccess) e = hipMemcpy(rec.data(), buf, rec.size() * sizeof(float), hipMemcpyDeviceToHost);
-    (void)hipFree(buf);
-    if (st != WOB_OK) return st;
-    WOB_HIP(e, "harmonic control-variate samples");
-
+    const uint32_t n = d->n_train, m = d->n_sources, P = m + 1, Q = m + 2;
+    const hipStream_t hs = (hipStream_t)stream;
     const std::vector<double> src = source_positions(h, m, d->radius_scale);
-    // normal equations of the rows, columns scaled to unit RMS
-    std::vector<double> row(P), G((size_t)P * P, 0.0), r(P, 0.0), scale(P, 0.0);
-    for (uint32_t i = 0; i < n; ++i) {
-        basis_row(D, &rec[(size_t)i * W], src, m, row);
-        for (uint32_t a = 0; a < P; ++a) scale[a] += row[a] * row[a];
+    // device buffers: the target records, the source positions, the Gram matrix, residuals
+    const size_t rec_bytes = ((size_t)n * W * sizeof(float) + 255) & ~(size_t)255,
+                 src_bytes = (src.size() * sizeof(double) + 255) & ~(size_t)255,
+                 g_bytes = ((size_t)Q * Q * sizeof(double) + 255) & ~(size_t)255,
+                 res_bytes = ((size_t)n * sizeof(float) + 255) & ~(size_t)255;
+    char *buf = nullptr;
+    WOB_HIP(hipMalloc((void **)&buf, rec_bytes + src_bytes + g_bytes + res_bytes + wobl::hcv_gram_scratch(n, m)),
+            "hipMalloc harmonic control-variate fit");
+    float *d_rec = (float *)buf;
+    double *d_src = (double *)(buf + rec_bytes), *d_G = (double *)(buf + rec_bytes + src_bytes);
+    float *d_res = (float *)(buf + rec_bytes + src_bytes + g_bytes);
+    double *d_part = (double *)(buf + rec_bytes + src_bytes + g_bytes + res_bytes);
+    std::vector<double> G((size_t)Q * Q);
+    hipError_t e = hipMemcpyAsync(d_src, src.data(), src.size() * sizeof(double), hipMemcpyHostToDevice, hs);
+    if (e == hipSuccess && (st = wob_harmonic_cv_targets(scene, d, d_rec, stream)) == WOB_OK) {
+        e = wobl::launch_hcv_gram(D, d_rec, n, d_src, m, d_part, d_G, hs);
+        if (e == hipSuccess) e = hipMemcpyAsync(G.data(), d_G, G.size() * sizeof(double), hipMemcpyDeviceToHost, hs);
+        if (e == hipSuccess) e = hipStreamSynchronize(hs);
     }
-    for (uint32_t a = 0; a < P; ++a) scale[a] = scale[a] > 0 ? 1.0 / std::sqrt(scale[a] / n) : 1.0;
-    for (uint32_t i = 0; i < n; ++i) {
-        const float *ri = &rec[(size_t)i * W];
-        basis_row(D, ri, src, m, row);
-        const double t = ri[2 * D + 1];
-        for (uint32_t a = 0; a < P; ++a) row[a] *= scale[a];
-        for (uint32_t a = 0; a < P; ++a) {
-            r[a] += row[a] * t;
-            const double ra = row[a];
-            double *g = &G[(size_t)a * P];
-            for (uint32_t b = 0; b <= a; ++b) g[b] += ra * row[b];
-        }
+    auto release = [&]() { (void)hipFree(buf); };
+    if (st != WOB_OK) { release(); return st; }
+    if (e != hipSuccess) { release(); return wob_hip_fail(e, "harmonic control-variate Gram matrix"); }
+
+    // normal equations with the columns scaled to unit RMS, and a ridge relative to the
+    // mean diagonal
+    std::vector<double> A((size_t)P * P), r(P), scale(P);
+    for (uint32_t a = 0; a < P; ++a) {
+        const double g = G[(size_t)a * Q + a];
+        scale[a] = g > 0 ? 1.0 / std::sqrt(g / n) : 1.0;
     }
     double tr = 0;
-    for (uint32_t a = 0; a < P; ++a) tr += G[(size_t)a * P + a];
-    const double lam = (double)d->ridge * tr / P;
     for (uint32_t a = 0; a < P; ++a) {
-        G[(size_t)a * P + a] += lam;
-        for (uint32_t b = a + 1; b < P; ++b) G[(size_t)a * P + b] = G[(size_t)b * P + a];
+        for (uint32_t b = 0; b < P; ++b) A[(size_t)a * P + b] = G[(size_t)a * Q + b] * scale[a] * scale[b];
+        r[a] = G[(size_t)a * Q + P] * scale[a];
+        tr += A[(size_t)a * P + a];
     }
-    if (!cholesky_solve(G, r, (int)P))
+    const double lam = (double)d->ridge * tr / P;
+    for (uint32_t a = 0; a < P; ++a) A[(size_t)a * P + a] += lam;
+    if (!cholesky_solve(A, r, (int)P)) {
+        release();
         return fail(WOB_INVALID_ARG, "harmonic control-variate fit: singular normal equations (raise ridge, or the "
                                      "samples hold no Dirichlet or Robin datum to fix the constant)");
+    }
     for (uint32_t a = 0; a < P; ++a) r[a] *= scale[a];
 
-    // device records (FP32) and the fit's residual with the coefficients as bound
+    // device records (FP32), bound to the scene
     std::vector<float> out((size_t)m * 4, 0.f);
-    std::vector<double> src32((size_t)m * D), coef(P);
-    coef[0] = (float)r[0];
     for (uint32_t j = 0; j < m; ++j) {
-        for (int k = 0; k < D; ++k) {
-            out[(size_t)j * 4 + k] = (float)src[(size_t)j * D + k];
-            src32[(size_t)j * D + k] = out[(size_t)j * 4 + k];
-        }
+        for (int k = 0; k < D; ++k) out[(size_t)j * 4 + k] = (float)src[(size_t)j * D + k];
         out[(size_t)j * 4 + D] = (float)r[1 + j];
-        coef[1 + j] = out[(size_t)j * 4 + D];
     }
-    if (rms_residual) {
+    st = bind_sources(scene, out, m, (float)r[0]);
+    if (st != WOB_OK) { release(); return st; }
+    if (rms_residual) {   // the residual datum of every sample as the walk computes it
+        std::vector<float> res(n);
+        e = wobl::launch_hcv_residual(D, wob_scene_dev(scene), d_rec, n, d_res, hs);
+        if (e == hipSuccess) e = hipMemcpyAsync(res.data(), d_res, n * sizeof(float), hipMemcpyDeviceToHost, hs);
+        if (e == hipSuccess) e = hipStreamSynchronize(hs);
+        if (e != hipSuccess) { release(); return wob_hip_fail(e, "harmonic control-variate residual"); }
         double ss = 0;
-        for (uint32_t i = 0; i < n; ++i) {
-            const float *ri = &rec[(size_t)i * W];
-            basis_row(D, ri, src32, m, row);
-            double v = 0;
-            for (uint32_t a = 0; a < P; ++a) v += row[a] * coef[a];
-            ss += (ri[2 * D + 1] - v) * (ri[2 * D + 1] - v);
-        }
+        for (uint32_t i = 0; i < n; ++i) ss += (double)res[i] * res[i];
         *rms_residual = (float)std::sqrt(ss / n);
     }
-    return bind_sources(scene, out, m, (float)r[0]);
+    release();
+    return WOB_OK;
 }
 
 wob_status wob_scene_set_harmonic_cv(wob_scene *scene, uint32_t n_sources, const float *sources, float c0) {

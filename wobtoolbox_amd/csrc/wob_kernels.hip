@@ -321,6 +321,107 @@ hipError_t launch_sample_boundary(int dim, const DevScene &S, uint32_t n, uint64
     return hipGetLastError();
 }
 
+// The harmonic control variate's fit: the augmented Gram matrix G = [B t]^T [B t] of the rows
+// B (basis 1, phi_1 .. phi_m, each as the datum of the record's type: value at Dirichlet,
+// normal derivative at Neumann, normal derivative + alpha value at Robin records) and the
+// targets t, in FP64, Q = m + 2 columns (column Q - 1 is the target).  Records are taken in
+// chunks; thread (a, b) of a chunk sums row a times row b over the chunk's records in order,
+// and the chunks' sums are added in order (deterministic).  src: m positions (D doubles).
+template <int D>
+__device__ __forceinline__ double hcv_row(const float *r, const double *src, uint32_t m, uint32_t c) {
+    if (c == m + 1) return (double)r[2 * D + 1];   // the target
+    const int type = (int)r[2 * D + 2];
+    const double alpha = (double)r[2 * D + 3];
+    if (c == 0) return type == kDirichlet ? 1.0 : type == kNeumann ? 0.0 : alpha;
+    const double *s = src + (size_t)(c - 1) * D;
+    double z = 0.0, dn = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double d = (double)r[k] - s[k];
+        z += d * d;
+        dn += d * (double)r[D + k];
+    }
+    double val, der;
+    if constexpr (D == 3) {
+        const double ri = 1.0 / sqrt(z);
+        val = ri;
+        der = -dn * ri * ri * ri;
+    } else {
+        val = log2(z);
+        der = 2.8853900817779268 * dn / z;   // 2 / ln 2
+    }
+    return type == kDirichlet ? val : type == kNeumann ? der : der + alpha * val;
+}
+
+// Block (a, b tile, chunk c) sums the products over the records of chunk c into the partial
+// matrix of that chunk; hcv_gram_sum_kernel then adds the chunks in order.
+constexpr uint32_t kGramChunk = 256;
+
+template <int D>
+__global__ __launch_bounds__(64) void hcv_gram_kernel(const float *rec, uint32_t n, const double *src, uint32_t m,
+                                                      double *partial) {
+    const uint32_t Q = m + 2;
+    const uint32_t a = blockIdx.x, b = blockIdx.y * 64 + threadIdx.x, c = blockIdx.z;
+    if (b >= Q || b < a) return;
+    const uint32_t i0 = c * kGramChunk, i1 = min(n, i0 + kGramChunk);
+    double acc = 0.0;
+    for (uint32_t i = i0; i < i1; ++i) {
+        const float *r = rec + (size_t)i * (2 * D + 4);
+        acc += hcv_row<D>(r, src, m, a) * hcv_row<D>(r, src, m, b);
+    }
+    partial[((size_t)c * Q + a) * Q + b] = acc;
+}
+
+__global__ void hcv_gram_sum_kernel(const double *partial, uint32_t Q, uint32_t chunks, double *G) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Q * Q) return;
+    const uint32_t a = t / Q, b = t - a * Q;
+    const uint32_t lo = min(a, b), hi = max(a, b);   // partials hold the upper triangle
+    double acc = 0.0;
+    for (uint32_t c = 0; c < chunks; ++c) acc += partial[((size_t)c * Q + lo) * Q + hi];
+    G[t] = acc;
+}
+
+size_t hcv_gram_scratch(uint32_t n, uint32_t m) {
+    const size_t Q = m + 2, chunks = (n + kGramChunk - 1) / kGramChunk;
+    return chunks * Q * Q * sizeof(double);
+}
+
+hipError_t launch_hcv_gram(int dim, const float *rec, uint32_t n, const double *src, uint32_t m, double *partial,
+                           double *G, hipStream_t st) {
+    const uint32_t Q = m + 2, chunks = (n + kGramChunk - 1) / kGramChunk;
+    const dim3 grid(Q, (Q + 63) / 64, chunks);
+    if (dim == 2) hipLaunchKernelGGL(hcv_gram_kernel<2>, grid, dim3(64), 0, st, rec, n, src, m, partial);
+    else hipLaunchKernelGGL(hcv_gram_kernel<3>, grid, dim3(64), 0, st, rec, n, src, m, partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(hcv_gram_sum_kernel, dim3((Q * Q + 255) / 256), dim3(256), 0, st, partial, Q, chunks, G);
+    return hipGetLastError();
+}
+
+// The walk's residual datum (hcv_residual) at each record: the fit's residual as the walk sees it
+template <int D> __global__ void hcv_residual_kernel(DevScene S, const float *rec, uint32_t n, float *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *r = rec + (size_t)i * (2 * D + 4);
+    BP<D> y;
+#pragma unroll
+    for (int k = 0; k < D; ++k) { y.p[k] = r[k]; y.n[k] = r[D + k]; y.bw[k] = 0.f; }
+    y.bv = r[2 * D + 1];
+    y.type = (int)r[2 * D + 2];
+    y.alpha = r[2 * D + 3];
+    y.elem = 0xFFFFFFFFu;
+    out[i] = hcv_residual<D>(S, y).bv;
+}
+
+hipError_t launch_hcv_residual(int dim, const DevScene &S, const float *rec, uint32_t n, float *out, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint32_t block = 256, grid = (n + block - 1) / block;
+    if (dim == 2) hipLaunchKernelGGL(hcv_residual_kernel<2>, dim3(grid), dim3(block), 0, st, S, rec, n, out);
+    else hipLaunchKernelGGL(hcv_residual_kernel<3>, dim3(grid), dim3(block), 0, st, S, rec, n, out);
+    return hipGetLastError();
+}
+
 // u_h and grad u_h of the scene's harmonic control variate (the walk's own functions)
 template <int D> __global__ void hcv_eval_kernel(DevScene S, uint32_t n, const float *pts, float *out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

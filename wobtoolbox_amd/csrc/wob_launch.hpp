@@ -120,6 +120,13 @@ hipError_t launch_hcv_targets_2(const wobdev::DevScene &S, uint32_t n, uint64_t 
                                 float eps, bool interior, float *rec, size_t lds_bytes, hipStream_t st);
 hipError_t launch_hcv_targets_3(const wobdev::DevScene &S, uint32_t n, uint64_t seed, uint32_t stream, uint32_t nvs,
                                 float eps, bool interior, float *rec, size_t lds_bytes, hipStream_t st);
+// The fit's augmented FP64 Gram matrix (G: (m + 2)^2 doubles) of n target records and m
+// source positions (src: m * dim doubles); the walk's residual datum at each record (out: n).
+size_t hcv_gram_scratch(uint32_t n, uint32_t m);   // bytes of `partial`
+hipError_t launch_hcv_gram(int dim, const float *rec, uint32_t n, const double *src, uint32_t m, double *partial,
+                           double *G, hipStream_t st);
+hipError_t launch_hcv_residual(int dim, const wobdev::DevScene &S, const float *rec, uint32_t n, float *out,
+                               hipStream_t st);
 // u_h and grad u_h of the scene's harmonic control variate at n points (out: n * (dim + 1))
 hipError_t launch_hcv_eval(int dim, const wobdev::DevScene &S, uint32_t n, const float *pts, float *out, hipStream_t st);
 
