@@ -57,12 +57,15 @@ def small_icosphere(level=3, kind="neumann"):
     if kind == "neumann":
         return G.mesh_elements(V, F, G.NEUMANN, lambda p, nr, ty: (G.exact_quadratic_3d(p)[:, 1:] * nr).sum(1)), V, F
     cz = V[F].mean(1)[:, 2]
-    types = np.where(cz > 0, G.DIRICHLET, G.NEUMANN).astype(np.int32)
+    # "mixed": Dirichlet for z > 0, Neumann below; "robin": Dirichlet for z > 0, Robin (alpha 1,
+    # data du/dn + u) below
+    types = np.where(cz > 0, G.DIRICHLET, G.ROBIN if kind == "robin" else G.NEUMANN).astype(np.int32)
 
     def data(p, nr, ty):
         e = G.exact_expcos_xz_3d(p)
-        return np.where(ty == G.DIRICHLET, e[:, 0], (e[:, 1:] * nr).sum(1))
-    return G.mesh_elements(V, F, types, data), V, F
+        dn = (e[:, 1:] * nr).sum(1)
+        return np.where(ty == G.DIRICHLET, e[:, 0], np.where(ty == G.ROBIN, dn + e[:, 0], dn))
+    return G.mesh_elements(V, F, types, data, robin_alpha=(types == G.ROBIN).astype(np.float64)), V, F
 
 
 def star_scene(seed=0, n_segments=400, kind=1):

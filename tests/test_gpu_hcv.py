@@ -7,7 +7,7 @@ single-layer mixed walks of the learned-control-variate workload) through the C 
   * the fit recovers a field it represents exactly, and agrees with an FP64 least-squares
     solve of its own targets;
   * the walk with the control variate against the oracle path by path, in single calls,
-    batches and the wavefront schedule;
+    batches and the wavefront schedule, with Dirichlet, Neumann and Robin data;
   * unbiasedness against the closed-form solution u = N_f + H and the variance it removes.
 """
 import sys
@@ -133,7 +133,8 @@ def test_hcv_fit_matches_f64_least_squares(gpu, oracle):
     el, srcv, _ = _poisson(3)
     import wobtoolbox_amd as wob
     el2, _, _ = small_icosphere(3, kind="mixed")
-    for e, vol in ((el, True), (el2, False)):
+    el3, _, _ = small_icosphere(3, kind="robin")
+    for e, vol in ((el, True), (el2, False), (el3, False)):
         scene = wob.SceneHost(e)
         if vol:
             scene.set_volume_gaussians(*srcv)
@@ -146,7 +147,8 @@ def test_hcv_fit_matches_f64_least_squares(gpu, oracle):
         pts = G.interior_points_star(256, np.random.default_rng(4), margin=0.02).astype(np.float32)
         dev = scene.eval_harmonic_cv(_dev(pts, gpu)).cpu().numpy()[:, 0]
         ref = hcv_eval_f64(pts, src, w, c0)[:, 0]
-        _say(f"fit vs FP64 least squares ({'Dirichlet + volume' if vol else 'mixed'}): "
+        _say(f"fit vs FP64 least squares ({'Dirichlet + volume' if vol else 'mixed'}, {e.n} elements, "
+             f"Robin {(e.boundary_types == G.ROBIN).any()}): "
              f"max |u_h - u_ref| {np.abs(dev - ref).max():.2e}; residual {rms:.4f}")
         assert np.abs(dev - ref).max() < 1e-4 * (1 + np.abs(ref).max())
 
@@ -279,6 +281,34 @@ def test_hcv_single_layer_walk_matches_oracle(gpu, oracle):
     frac = (np.abs(m - mo) > 1e-4 * (np.abs(mo) + se + 1e-3)).mean()
     _say(f"single-layer control-variate walk: max |z| {z.max():.3f}, points over 1e-4 rel {frac:.3f}")
     assert z.max() < 0.05 and frac < 0.02
+
+
+def test_hcv_single_layer_robin_walk_matches_oracle(gpu, oracle):
+    """Robin data (alpha 1 below z = 0, Dirichlet above) with the harmonic control variate:
+    the walk path by path against the oracle (the Robin residual g - (du_h/dn + alpha u_h))
+    and against the exact solution."""
+    import wobtoolbox_amd as wob
+    el, _, _ = small_icosphere(3, kind="robin")
+    scene, osc = _scenes(el, None, oracle)
+    scene.fit_harmonic_cv(n_sources=64, n_train=16384, seed=5)
+    rec, c0 = scene.harmonic_cv()
+    osc.set_harmonic_cv(rec, c0)
+    pts = G.interior_points_star(64, np.random.default_rng(7), margin=0.05).astype(np.float32)
+    paths = 128
+    solver = dict(SL_MIXED, flags=HCV)
+    est = wob.Estimator.from_config(scene, solver)
+    s, q = est.compute_domain(_dev(pts, gpu), paths, seed=11)
+    m, se = mean_se(s.cpu().numpy(), q.cpu().numpy(), paths)
+    so, qo = osc.estimate_domain(solver, pts, paths, seed=11, pick_mode=1)
+    mo, seo = mean_se(so, qo, paths)
+    z = np.abs(m - mo) / np.sqrt(se ** 2 + seo ** 2 + 1e-30)
+    frac = (np.abs(m - mo) > 1e-4 * (np.abs(mo) + se + 1e-3)).mean()
+    ex = G.exact_expcos_xz_3d(pts)
+    ze = np.abs(m - ex) / se
+    _say(f"Robin control-variate walk: max |z| vs oracle {z.max():.3f}, points over 1e-4 rel {frac:.3f}, "
+         f"max |z| vs exact {ze.max():.2f}")
+    assert z.max() < 0.05 and frac < 0.02
+    assert ze[:, 0].max() < 4.5 and ze[:, 1:].max() < 5.0
 
 
 def test_hcv_single_layer_unbiased_and_variance(gpu):
