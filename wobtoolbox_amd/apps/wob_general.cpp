@@ -3,6 +3,7 @@
 // Same command line, configuration keys, input meshes and output files as
 // apps/general_2d.cu and apps/general_3d.cu (rsugimoto/WoBToolbox):
 //   wob_general_<D>d config.json [--max-iterations N] [--dry-run] [--seed S]
+//                    [--harmonic-cv N_SOURCES [RADIUS_SCALE]]
 // reads input_obj_file, output_dir, path_length, num_sample_paths, num_resampling_candidates,
 // grid_res, eps, first_kind_equation_scaling_constant, max_time (minutes), equation_kernel,
 // bie_model, domain, estimator, problem (general_2d.cu:174-195); builds the boundary with
@@ -20,6 +21,11 @@
 //   - the RMSE is computed on the normalised estimate; general_3d.cu:367-368 passes the raw
 //     sums instead;
 //   - paths use the engine's Philox streams (pass i walks paths [(i-1) N, i N) of every point).
+// Addition: --harmonic-cv N fits the engine's harmonic control variate (N fundamental solutions
+// on a sphere or circle of RADIUS_SCALE x the scene radius, default 2, fitted to 32768 boundary
+// samples of the walk's data; wob_scene_fit_harmonic_cv) before the first pass and walks the
+// residual (WOB_EST_HARMONIC_CV).  The fit's time counts in the "seconds" column.  It needs the
+// single- or double-layer model, the backward estimator and an interior domain.
 // The 3D Robin elements keep the apps' robin alpha (1, 1, 0): the aggregate initialisation at
 // general_3d.cu:279-284 leaves the third vertex's alpha at 0.
 #include <hip/hip_runtime_api.h>
@@ -127,8 +133,10 @@ struct Run {
     uint32_t n, paths;
     uint64_t path_base;
     double *d_sum, *d_sq;
+    uint32_t flags;
     template <wob::BIEModelType B, wob::DomainType Dm, wob::EstimatorType E, wob::ProblemType P> void operator()() {
         wob::Estimator<float, D, wob::PoissonKernel, B, Dm, E, P> est(*scene, L, nrc, eps, fkesc);
+        est.desc().flags |= flags;
         est.compute_domain(reinterpret_cast<const wob::DomainPoint<float, D> *>(d_pts), n, paths, 0, path_base, d_sum,
                            d_sq);
     }
@@ -136,17 +144,26 @@ struct Run {
 
 int run_app(int argc, char **argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: %s config.json [--max-iterations N] [--dry-run] [--seed S]\n", argv[0]);
+        std::fprintf(stderr,
+                     "usage: %s config.json [--max-iterations N] [--dry-run] [--seed S] "
+                     "[--harmonic-cv N_SOURCES [RADIUS_SCALE]]\n",
+                     argv[0]);
         return 1;
     }
     const std::string config_path = argv[1];
     long max_iter = -1;
     bool dry = false;
     unsigned seed = 0;
+    unsigned hcv_sources = 0;
+    float hcv_radius = 2.0f;
     for (int a = 2; a < argc; ++a) {
         if (!std::strcmp(argv[a], "--max-iterations") && a + 1 < argc) max_iter = std::atol(argv[++a]);
         else if (!std::strcmp(argv[a], "--dry-run")) dry = true;
         else if (!std::strcmp(argv[a], "--seed") && a + 1 < argc) seed = (unsigned)std::atol(argv[++a]);
+        else if (!std::strcmp(argv[a], "--harmonic-cv") && a + 1 < argc) {
+            hcv_sources = (unsigned)std::atol(argv[++a]);
+            if (a + 1 < argc && argv[a + 1][0] != '-') hcv_radius = (float)std::atof(argv[++a]);
+        }
     }
     std::ifstream cf(config_path);
     if (!cf) {
@@ -170,6 +187,11 @@ int run_app(int argc, char **argv) {
     const auto domain = wob::string_to_domain_type(cfg.str("domain"));
     const auto est = wob::string_to_estimator_type(cfg.str("estimator"));
     const auto problem = wob::string_to_problem_type(cfg.str("problem"));
+    if (hcv_sources > 0 &&
+        ((bie != wob::IndirectSingleLayer && bie != wob::IndirectDoubleLayer) || est != wob::BackwardEstimator ||
+         domain != wob::InteriorDomain))
+        throw std::runtime_error("--harmonic-cv needs IndirectSingleLayer or IndirectDoubleLayer, BackwardEstimator "
+                                 "and InteriorDomain");
 
     std::vector<double> V;
     std::vector<int> F;
@@ -294,14 +316,25 @@ int run_app(int argc, char **argv) {
     HIP_OK(hipMemcpy(d_pts, pts.data(), pts.size() * sizeof(float), hipMemcpyHostToDevice));
     HIP_OK(hipMemset(d_acc, 0, 2 * m * V1 * sizeof(double)));
 
+    double elapsed = 0.0;
+    if (hcv_sources > 0) {
+        HIP_OK(hipDeviceSynchronize());
+        const auto t0 = std::chrono::steady_clock::now();
+        const float rms = scene.fit_harmonic_cv(hcv_sources, hcv_radius, 32768, 0, eps, 1e-10f, seed);
+        HIP_OK(hipDeviceSynchronize());
+        elapsed += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::cout << "harmonic control variate\t: " << hcv_sources << " sources at " << hcv_radius
+                  << " x the scene radius, fit residual " << rms << ", " << elapsed << " s" << std::endl;
+    }
+
     std::vector<double> acc(2 * m * V1);
     const unsigned ref_idx = domain == wob::InteriorDomain ? res * (res / 2) + res / 2 : 0u;   // shift_solution
-    double elapsed = 0.0;
     uint64_t total = 0;
     for (long it = 1; elapsed < max_time * 60.0 && (max_iter < 0 || it <= max_iter); ++it) {
         HIP_OK(hipDeviceSynchronize());
         const auto t0 = std::chrono::steady_clock::now();
-        Run r{&scene, L, nrc, eps, fkesc, d_pts, m, paths, total, d_acc, d_acc + m * V1};
+        Run r{&scene, L, nrc, eps, fkesc, d_pts, m, paths, total, d_acc, d_acc + m * V1,
+              hcv_sources > 0 ? (uint32_t)WOB_EST_HARMONIC_CV : 0u};
         if (m > 0 && !wob::run(bie, domain, est, problem, r)) throw std::runtime_error("invalid solver types");
         HIP_OK(hipDeviceSynchronize());
         elapsed += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
