@@ -267,3 +267,170 @@ def test_wos_app_both_modes_agree(tmp_path, gpu):
     assert np.all(np.abs(u_wos[m]) <= 1.0 + 1e-6)
     assert np.corrcoef(u_wos[m], u_wob[m])[0, 1] > 0.8
     assert np.mean(np.abs(u_wos[m] - u_wob[m])) < 0.15
+
+
+# ---------------------------------------------------------------------------------------
+# apps/poisson_equation.cu and apps/boundary_estimate.cu
+# ---------------------------------------------------------------------------------------
+def _json(tmp_path, name, **cfg):
+    p = tmp_path / name
+    p.write_text(json.dumps(cfg, indent=4))
+    return p
+
+
+def square_poisson_series(x, y, source=5.0, data=1.0, terms=4001):
+    """u with Laplacian(u) = source in (-1, 1)^2 and u = data on the boundary (FP64):
+    u = data + source w, w = (x^2 - 1) / 2 + sum_{k odd} b_k cos(k pi x / 2) cosh(k pi y / 2) /
+    cosh(k pi / 2), b_k = 2 (-1)^((k-1)/2) / (k pi / 2)^3 (the cosine series of (1 - x^2) / 2,
+    which w must cancel on the sides y = +-1); the cosh ratio is evaluated as
+    exp(a (|y| - 1)) (1 + exp(-2 a |y|)) / (1 + exp(-2 a))."""
+    x, y = np.asarray(x, np.float64), np.abs(np.asarray(y, np.float64))
+    w = (x * x - 1) / 2
+    for k in range(1, terms + 1, 2):
+        a = k * np.pi / 2
+        ratio = np.exp(a * (y - 1)) * (1 + np.exp(-2 * a * y)) / (1 + np.exp(-2 * a))
+        w = w + 2 * (-1) ** ((k - 1) // 2) / a ** 3 * np.cos(a * x) * ratio
+    return data + source * w
+
+
+def test_square_poisson_series_is_the_solution():
+    """The series against its defining properties: boundary values, and the five-point
+    Laplacian at interior points."""
+    t = np.linspace(-1, 1, 41)
+    for bx, by in ((t, np.ones_like(t)), (t, -np.ones_like(t)), (np.ones_like(t), t), (-np.ones_like(t), t)):
+        assert np.abs(square_poisson_series(bx, by) - 1).max() < 1e-6
+    rng = np.random.default_rng(0)
+    x, y = rng.uniform(-0.8, 0.8, (2, 50))
+    h = 1e-3
+    lap = (square_poisson_series(x + h, y) + square_poisson_series(x - h, y) + square_poisson_series(x, y + h)
+           + square_poisson_series(x, y - h) - 4 * square_poisson_series(x, y)) / (h * h)
+    assert np.abs(lap - 5).max() < 1e-3
+
+
+def test_poisson_app_dry_run(tmp_path):
+    cfg = _json(tmp_path, "poisson.json", output_dir=str(tmp_path / "p"), path_length=4, num_sample_paths=16,
+                num_volume_samples=16, grid_res=24, eps=1e-4, max_time=1)
+    out = _run(B.POISSON_APP, cfg, "--dry-run")
+    assert "evaluation points\t: 576" in out and "#num volume samples\t: 16" in out
+    assert json.loads((tmp_path / "p" / "config.json").read_text())["grid_res"] == 24
+
+
+def _boundary_cfg(tmp_path, **kw):
+    cfg = dict(output_dir=str(tmp_path / "b"), path_length=4, num_sample_paths_domain=64,
+               num_sample_paths_boundary=256, grid_res=32, max_time=1)
+    cfg.update(kw)
+    return _json(tmp_path, "boundary.json", **cfg)
+
+
+def test_boundary_estimate_app_dry_run(tmp_path):
+    out = _run(B.BOUNDARY_APP, _boundary_cfg(tmp_path), "--dry-run")
+    c = (np.arange(32) + 0.5) / 32 - 0.5
+    x, y = np.meshgrid(c.astype(np.float32), c.astype(np.float32))
+    inside = int((np.sqrt(x * x + y * y) <= 0.5).sum())
+    assert f"evaluation points\t: {inside} of 1024" in out
+
+
+@pytest.mark.gpu
+def test_poisson_app_matches_the_series_solution(tmp_path, gpu):
+    """poisson_equation.cu's setting (square, data 1, source 5 on the grid cache, double layer,
+    path length 4, 16 volume samples) against the series solution: the error is noise (its
+    mean within a few standard errors of 0, RMSE falling like 1 / sqrt(paths)), and the
+    gradient field points along the series' gradient."""
+    res = 16
+    cfg = _json(tmp_path, "poisson.json", output_dir=str(tmp_path / "p"), path_length=4, num_sample_paths=1024,
+                num_volume_samples=16, grid_res=res, eps=1e-4, max_time=1)
+    _run(B.POISSON_APP, cfg, "--max-iterations", "4")
+    c = ((np.arange(res, dtype=np.float32) + np.float32(0.5)) * np.float32(2.0 / res) - np.float32(1.0))
+    x, y = np.meshgrid(c.astype(np.float64), c.astype(np.float64))
+    ex = square_poisson_series(x, y)
+    errs = []
+    for it in (1, 4):
+        r, u = _read_field(tmp_path / "p" / f"potential{it}.scalar")
+        _, g = _read_field(tmp_path / "p" / f"gradient{it}.vector2", 2)
+        assert r == res and np.all(np.isfinite(u)) and np.all(np.isfinite(g))
+        errs.append((u - ex).ravel())
+    e1, e4 = errs
+    rmse1, rmse4 = np.sqrt(np.mean(e1 ** 2)), np.sqrt(np.mean(e4 ** 2))
+    print(f"Poisson app: RMSE vs series after 1024 / 4096 paths {rmse1:.4f} / {rmse4:.4f}, "
+          f"mean error {e4.mean():+.5f}", flush=True)
+    # per-path standard deviation about 7 (the oracle's on this scene): SE 0.11 at 4096 paths
+    assert abs(e4.mean()) < 4 * rmse4 / np.sqrt(e4.size)
+    assert 0.35 < rmse4 / rmse1 < 0.7 and rmse4 < 0.16
+    h = 1e-5
+    gx = (square_poisson_series(x + h, y) - square_poisson_series(x - h, y)) / (2 * h)
+    gy = (square_poisson_series(x, y + h) - square_poisson_series(x, y - h)) / (2 * h)
+    _, g = _read_field(tmp_path / "p" / "gradient4.vector2", 2)
+    m = np.hypot(gx, gy) > 2.0
+    cosang = (g[..., 0] * gx + g[..., 1] * gy)[m] / (np.hypot(g[..., 0], g[..., 1])[m] * np.hypot(gx, gy)[m])
+    print(f"Poisson app: median cosine between estimated and series gradients {np.median(cosang):.3f} "
+          f"over {m.sum()} points", flush=True)
+    assert np.median(cosang) > 0.8
+
+
+@pytest.mark.gpu
+def test_boundary_estimate_app_matches_the_library_and_its_symmetry(tmp_path, gpu):
+    """boundary_estimate.cu's scene and estimators: the app's fields equal the library's
+    estimates of the same scene built from the reference's recipe in numpy (same Philox
+    streams), and the boundary solution shares the data's symmetries within its noise: the
+    data repeat every 240 segments and change sign under a shift of 120, so the boundary
+    solution does too, up to the additive constant a Neumann solution leaves free."""
+    import torch
+    import wobtoolbox_amd as wob
+    res, pd, pb = 32, 64, 256
+    cfg = _boundary_cfg(tmp_path, grid_res=res, num_sample_paths_domain=pd, num_sample_paths_boundary=pb)
+    _run(B.BOUNDARY_APP, cfg, "--max-iterations", "2")
+    od = tmp_path / "b"
+    # the reference's scene (boundary_estimate.cu:153-172): vertices in double rounded to FP32
+    n_seg = 720
+    i = np.arange(n_seg + 1)
+    v = np.stack([np.float32(0.5) * np.sin(2 * np.pi * i / n_seg), np.float32(0.5) * np.cos(2 * np.pi * i / n_seg)],
+                 1).astype(np.float32)
+    v1, v2 = v[:-1], v[1:]
+    t = v1 - v2
+    t = t / np.sqrt(t[:, 0] * t[:, 0] + t[:, 1] * t[:, 1])[:, None]
+    nrm = np.stack([t[:, 1], -t[:, 0]], 1).astype(np.float32)
+    k = np.arange(n_seg) % 240
+    a_val = np.where((k == 0) | (k == 239), 1.0, np.where((k == 120) | (k == 119), -1.0, 0.0))
+    b_val = 200.0 * a_val
+    el = G.Elements(2, np.stack([v1, v2], 1).astype(np.float64), -nrm.astype(np.float64),
+                    np.stack([a_val, b_val], 1), np.full(n_seg, G.NEUMANN, np.int32),
+                    np.zeros((n_seg, 2))).astype_contiguous()
+    scene = wob.SceneHost(el)
+    solver = dict(bie_model="IndirectSingleLayer", domain="InteriorDomain", estimator="ForwardEstimator",
+                  problem="NeumannProblem", path_length=4, eps=float(np.finfo(np.float32).eps))
+    est = wob.Estimator.from_config(scene, solver)
+    c = (np.arange(res, dtype=np.float32) + np.float32(0.5)) * np.float32(1.0 / res) - np.float32(0.5)
+    gx, gy = np.meshgrid(c, c)
+    inside = np.sqrt(gx * gx + gy * gy) <= np.float32(0.5)
+    pts = np.stack([gx[inside], gy[inside]], 1).astype(np.float32)
+    s = None
+    for it in range(2):
+        s, _ = est.compute_domain(torch.as_tensor(pts).to(gpu), pd, seed=0, path_base=it * pd,
+                                  out=None if s is None else (s, _))
+    u_lib = (s[:, 0].cpu().numpy() / (2 * pd)).astype(np.float32)
+    _, u_app = _read_field(od / "potential2.scalar")
+    assert np.array_equal(np.isnan(u_app), ~inside)
+    assert np.array_equal(u_app[inside], u_lib)
+    rec = np.zeros((n_seg, 8), np.float32)
+    rec[:, :2] = (v1 + v2) / np.float32(2.0)
+    rec[:, 2:4] = nrm
+    rec[:, 4] = 0.5
+    rec[:, 6] = G.NEUMANN
+    sb = qb = None
+    for it in range(2):
+        sb, qb = est.compute_boundary(torch.as_tensor(rec).to(gpu), pb, seed=0, path_base=it * pb, stream_id=1,
+                                      out=None if sb is None else (sb, qb))
+    b_lib = (sb.cpu().numpy().ravel() / (2 * pb)).astype(np.float32)
+    raw = open(od / "boundary_solution2.scalar", "rb").read()
+    assert int(np.frombuffer(raw[:4], np.uint32)[0]) == n_seg
+    b_app = np.frombuffer(raw[4:], np.float32)
+    assert np.array_equal(b_app, b_lib)
+    m = sb.cpu().numpy().ravel() / (2 * pb)
+    se = np.sqrt(np.maximum(qb.cpu().numpy().ravel() / (2 * pb) - m * m, 0) / (2 * pb))
+    d3 = m - np.roll(m, 240)
+    z3 = (d3 - d3.mean()) / np.sqrt(se ** 2 + np.roll(se, 240) ** 2)
+    d6 = m + np.roll(m, 120)
+    z6 = (d6 - d6.mean()) / np.sqrt(se ** 2 + np.roll(se, 120) ** 2)
+    print(f"boundary app: max |z| of the 240-shift difference {np.abs(z3).max():.2f}, of the 120-shift sum "
+          f"{np.abs(z6).max():.2f}; boundary solution range {m.min():.3f} .. {m.max():.3f}", flush=True)
+    assert np.abs(z3).max() < 5.0 and np.abs(z6).max() < 5.0

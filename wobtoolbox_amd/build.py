@@ -103,14 +103,18 @@ def _build_shim_check():
 APPS = {2: os.path.join(LIBDIR, "wob_general_2d"), 3: os.path.join(LIBDIR, "wob_general_3d")}
 VPL_APP = os.path.join(LIBDIR, "wob_vpl")
 WOS_APP = os.path.join(LIBDIR, "wob_wos")
+POISSON_APP = os.path.join(LIBDIR, "wob_poisson")
+BOUNDARY_APP = os.path.join(LIBDIR, "wob_boundary_estimate")
 
 
 def _build_apps():
-    """The reference apps' runners: apps/wob_general.cpp for 2D and 3D, apps/wob_vpl.cpp and
-    apps/wob_wos.cpp."""
+    """The reference apps' runners: apps/wob_general.cpp for 2D and 3D, apps/wob_vpl.cpp,
+    apps/wob_wos.cpp, apps/wob_poisson.cpp and apps/wob_boundary_estimate.cpp."""
     jobs = [(os.path.join(PKG, "apps", "wob_general.cpp"), out, [f"-DWOB_APP_DIM={dim}"]) for dim, out in APPS.items()]
     jobs.append((os.path.join(PKG, "apps", "wob_vpl.cpp"), VPL_APP, []))
     jobs.append((os.path.join(PKG, "apps", "wob_wos.cpp"), WOS_APP, []))
+    jobs.append((os.path.join(PKG, "apps", "wob_poisson.cpp"), POISSON_APP, []))
+    jobs.append((os.path.join(PKG, "apps", "wob_boundary_estimate.cpp"), BOUNDARY_APP, []))
     procs = []
     for src, out, defs in jobs:
         cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", *defs,
