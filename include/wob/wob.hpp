@@ -153,11 +153,13 @@ template <typename S, unsigned Dim, bool IsVectorProblem = false> class SceneHos
     // Learned harmonic control variate (no counterpart in the reference; wob_c.h
     // wob_scene_fit_harmonic_cv): fitted to the walk's known-term data at n_train boundary
     // samples and bound to the scene; estimators whose desc().flags carry WOB_EST_HARMONIC_CV
-    // then walk the residual.  Returns the fit's RMS residual.
+    // then walk the residual.  exterior = true fits the exterior domain's (radius_scale in
+    // (0, 1), sources inside the body).  Returns the fit's RMS residual.
     float fit_harmonic_cv(uint32_t n_sources = 64, float radius_scale = 1.5f, uint32_t n_train = 16384,
                           uint32_t volume_samples = 0, float eps = 1e-4f, float ridge = 1e-10f, uint64_t seed = 0,
-                          void *stream = nullptr) {
-        const wob_hcv_desc d{n_sources, radius_scale, n_train, volume_samples, eps, ridge, seed, 0u};
+                          void *stream = nullptr, bool exterior = false) {
+        const wob_hcv_desc d{n_sources, radius_scale, n_train, volume_samples, eps, ridge, seed, 0u,
+                             exterior ? WOB_HCV_EXTERIOR : 0u};
         float rms = 0.f;
         check(wob_scene_fit_harmonic_cv(scene_, &d, &rms, stream));
         return rms;

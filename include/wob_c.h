@@ -378,29 +378,40 @@ wob_status wob_cv_infer(wob_cv *cv, const float *d_pos, uint32_t n, float *d_out
 wob_status wob_scene_set_control_variate(wob_scene *scene, wob_cv *cv, const float *d_table, uint32_t n_values,
                                          void *stream);
 
-/* Learned harmonic control variate (no counterpart in the reference) for interior
- * single-layer and double-layer walks (WOB_EST_HARMONIC_CV), e.g. the Poisson problem's
- * double-layer estimator (poisson_equation.cu:40-61) and the mixed single-layer workload:
+/* Learned harmonic control variate (no counterpart in the reference) for single-layer and
+ * double-layer walks of interior domains and single-layer walks of exterior domains
+ * (WOB_EST_HARMONIC_CV), e.g. the Poisson problem's double-layer estimator
+ * (poisson_equation.cu:40-61), the mixed single-layer workload and the batched exterior scenes:
  *   u_h(x) = c0 + sum_j w_j phi(|x - s_j|^2),  phi(z) = z^(-1/2) (3D) or log2 z (2D),
  * a combination of fundamental solutions with sources s_j on a sphere (circle) outside the
- * scene, so u_h is harmonic in the interior domain.  The variance of the walk falls with the
- * size of the residual data.
+ * scene (interior fits), so u_h is harmonic in the interior domain, or inside the body
+ * (WOB_HCV_EXTERIOR), so u_h is harmonic in the exterior domain; an exterior fit has c0 = 0,
+ * since a single-layer potential has no constant term at infinity.  The variance of the walk
+ * falls with the size of the residual data.
  * Fit: the targets are the walk's own known-term data at n_train uniform boundary samples
  * (sample i on the Philox stream (seed, path i, point 0, stream_id)): the boundary datum plus
  * the volume term as the estimator samples it (the mean of volume_samples independent
  * volume samples; 0 without a volume source); each sample contributes the row of its type
  * (u_h, du_h/dn or du_h/dn + alpha u_h at the sample), (c0, w) minimise the ridge-regularised
  * squared error in FP64 (columns scaled to unit RMS, ridge relative to the mean diagonal),
- * and the result is bound to the scene.  Interior domains only (the sources lie outside).   */
+ * and the result is bound to the scene.  An interior fit serves interior walks only, an
+ * exterior fit exterior single-layer walks only.
+ * Exterior source placement: a sphere (circle) of radius radius_scale * R_in about the centre
+ * c of the scene's vertex box, R_in the smallest distance from c to an element; the sources
+ * lie inside the body when c does and the body holds the ball of radius R_in about c (a body
+ * star-shaped about c does).                                                                */
+#define WOB_HCV_EXTERIOR 1u
 typedef struct wob_hcv_desc {
     uint32_t n_sources;       /* 1 .. 1024 */
-    float radius_scale;       /* source radius / the scene's bounding radius about its box centre, > 1 */
+    float radius_scale;       /* interior: source radius / the scene's bounding radius about its box
+                                 centre, > 1; exterior: source radius / R_in, in (0, 1) */
     uint32_t n_train;         /* boundary samples, >= 2 (n_sources + 1) */
     uint32_t volume_samples;  /* volume-term samples per target (the scene's source; 0: data only) */
     float eps;                /* kernel clamp of the volume term (the walk's eps) */
     float ridge;              /* relative Tikhonov weight, >= 0 */
     uint64_t seed;
     uint32_t stream_id;
+    uint32_t flags;           /* WOB_HCV_EXTERIOR or 0 */
 } wob_hcv_desc;
 /* rms_residual (host, may be NULL): RMS of target - (the sample's row) over the samples.  */
 wob_status wob_scene_fit_harmonic_cv(wob_scene *scene, const wob_hcv_desc *desc, float *rms_residual, void *stream);

@@ -109,14 +109,45 @@ def mean_se(s, q, n):
     return m, np.sqrt(np.maximum(q / n - m * m, 0) / n)
 
 
-def hcv_sources(el, m, radius_scale):
+def element_distances(el, c):
+    """Distance from c to every element (segment or triangle), brute force in FP64 (the
+    triangle's closest point as the minimum over a dense barycentric grid refined by
+    projection onto the plane where it falls inside)."""
+    V = el.vertices.astype(np.float64)
+    c = np.asarray(c, np.float64)
+    if el.dim == 2:
+        a, b = V[:, 0], V[:, 1]
+        ab = b - a
+        t = np.clip(((c - a) * ab).sum(1) / (ab * ab).sum(1), 0, 1)
+        return np.linalg.norm(a + t[:, None] * ab - c, axis=1)
+    a, b, t3 = V[:, 0], V[:, 1], V[:, 2]
+    nrm = np.cross(b - a, t3 - a)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    q = c - ((c - a) * nrm).sum(1)[:, None] * nrm            # projection onto the plane
+    # barycentrics of q; inside -> plane distance, else the closest point on the three edges
+    v0, v1, v2 = b - a, t3 - a, q - a
+    d00, d01, d11 = (v0 * v0).sum(1), (v0 * v1).sum(1), (v1 * v1).sum(1)
+    d20, d21 = (v2 * v0).sum(1), (v2 * v1).sum(1)
+    den = d00 * d11 - d01 * d01
+    l1, l2 = (d11 * d20 - d01 * d21) / den, (d00 * d21 - d01 * d20) / den
+    inside = (l1 >= 0) & (l2 >= 0) & (l1 + l2 <= 1)
+    def seg(p0, p1):
+        e = p1 - p0
+        t = np.clip(((c - p0) * e).sum(1) / (e * e).sum(1), 0, 1)
+        return np.linalg.norm(p0 + t[:, None] * e - c, axis=1)
+    edge = np.minimum(np.minimum(seg(a, b), seg(b, t3)), seg(t3, a))
+    return np.where(inside, np.linalg.norm(q - c, axis=1), edge)
+
+
+def hcv_sources(el, m, radius_scale, exterior=False):
     """Source positions of the harmonic control variate: a Fibonacci sphere (3D) or equal
-    angles on a circle (2D) of radius radius_scale x (largest vertex distance from the centre
-    of the vertex box), about that centre (FP64)."""
+    angles on a circle (2D) about the centre c of the vertex box (FP64), of radius
+    radius_scale x (largest vertex distance from c) for interior fits, or radius_scale x
+    (smallest distance from c to an element) for exterior fits."""
     D = el.dim
     V = el.vertices.reshape(-1, D).astype(np.float64)
     c = 0.5 * (V.min(0) + V.max(0))
-    R = radius_scale * np.sqrt(((V - c) ** 2).sum(1).max())
+    R = radius_scale * (element_distances(el, c).min() if exterior else np.sqrt(((V - c) ** 2).sum(1).max()))
     j = np.arange(m) + 0.5
     if D == 2:
         a = 2 * np.pi * j / m
@@ -163,14 +194,16 @@ def hcv_eval_f64(pos, src, w, c0):
     return np.concatenate([u[:, None], g], 1)
 
 
-def hcv_fit_numpy(rec, src, ridge=1e-10):
+def hcv_fit_numpy(rec, src, ridge=1e-10, constant=True):
     """Ridge least squares of the records' targets on their rows, unit-RMS columns (FP64):
-    returns (c0, w)."""
+    returns (c0, w); constant=False (exterior fits) leaves out the constant column (c0 = 0)."""
     B = hcv_rows(rec, src)
+    if not constant:
+        B = B[:, 1:]
     t = np.asarray(rec, np.float64)[:, 2 * src.shape[1] + 1]
     sc = 1 / np.sqrt((B ** 2).mean(0))
     Bs = B * sc
     A = Bs.T @ Bs
     A += ridge * np.trace(A) / len(A) * np.eye(len(A))
     coef = np.linalg.solve(A, Bs.T @ t) * sc
-    return coef[0], coef[1:]
+    return (coef[0], coef[1:]) if constant else (0.0, coef)

@@ -333,3 +333,86 @@ def test_hcv_single_layer_unbiased_and_variance(gpu):
              f"(fit residual {rms:.2e})")
         assert np.abs(z[:, 0]).max() < 4.5 and np.abs(z[:, 1:]).max() < 5.0
     assert var[0] / var[HCV] > 1e4, var
+
+
+def _star(kind, n_segments=1200):
+    from helpers import star_scene
+    return star_scene(seed=4, n_segments=n_segments, kind=kind)
+
+
+def test_hcv_exterior_fit_matches_f64_least_squares(gpu, oracle):
+    """The exterior fit (WOB_HCV_EXTERIOR): sources inside the body at 0.75 x the smallest
+    centre-to-element distance (FP64 point-segment distances), no constant, and u_h equal to
+    an FP64 least-squares solve of the same targets."""
+    import wobtoolbox_amd as wob
+    for kind in (0, 1, 2):
+        w = _star(kind)
+        scene = wob.SceneHost(w.elements)
+        rec = scene.harmonic_cv_targets(8192, seed=2, exterior=True).cpu().numpy()
+        rms = scene.fit_harmonic_cv(n_sources=64, n_train=8192, seed=2, exterior=True)
+        src = hcv_sources(w.elements, 64, 0.75, exterior=True)
+        c0, wt = hcv_fit_numpy(rec, src, constant=False)
+        got, got_c0 = scene.harmonic_cv()
+        assert got_c0 == 0.0
+        assert np.allclose(got[:, :2], src, rtol=0, atol=1e-6)
+        pts = w.points[:256]
+        dev = scene.eval_harmonic_cv(_dev(pts, gpu)).cpu().numpy()[:, 0]
+        ref = hcv_eval_f64(pts, src, wt, 0.0)[:, 0]
+        _say(f"exterior fit kind {kind}: max |u_h - u_ref| {np.abs(dev - ref).max():.2e}, residual {rms:.2e}, "
+             f"max |u_h - u| {np.abs(ref - w.exact(pts)[:, 0]).max():.2e}")
+        assert np.abs(dev - ref).max() < 1e-5 * (1 + np.abs(ref).max())
+
+
+def test_hcv_exterior_walk_matches_oracle(gpu, oracle):
+    """Exterior single-layer RIS walks (Neumann, Robin, mixed) with the exterior control
+    variate, path by path against the oracle, in single calls and in one batch launch."""
+    import wobtoolbox_amd as wob
+    ests, sets = [], []
+    for kind in (0, 1, 2):
+        w = _star(kind, 400)
+        scene, osc = _scenes(w.elements, None, oracle)
+        scene.fit_harmonic_cv(n_sources=64, n_train=8192, seed=3, exterior=True)
+        rec, c0 = scene.harmonic_cv()
+        osc.set_harmonic_cv(rec, c0)
+        pts = w.points[:64]
+        paths = 64
+        solver = dict(w.solver, flags=HCV)
+        est = wob.Estimator.from_config(scene, solver)
+        s, q = est.compute_domain(_dev(pts, gpu), paths, seed=11)
+        m, se = mean_se(s.cpu().numpy(), q.cpu().numpy(), paths)
+        so, qo = osc.estimate_domain(solver, pts, paths, seed=11, pick_mode=1)
+        mo, _ = mean_se(so, qo, paths)
+        scale = np.abs(mo).max(0) + 1e-6
+        frac = (np.abs(m - mo) > 1e-4 * scale).mean()
+        _say(f"exterior control-variate walk kind {kind}: max |m - m_oracle| / scale "
+             f"{(np.abs(m - mo) / scale).max():.2e}, points over 1e-4 {frac:.3f}")
+        assert frac < 0.03
+        ests.append((est, s, q))
+        sets.append(pts)
+    b = wob.Batch([(est, _dev(p, gpu), 64, 0) for (est, _, _), p in zip(ests, sets)], persistent=True)
+    b.run(1, seed=11)
+    for (_, s, q), (bs, bq) in zip(ests, b.outputs):
+        assert np.array_equal(s.cpu().numpy(), bs.cpu().numpy()) and np.array_equal(q.cpu().numpy(), bq.cpu().numpy())
+
+
+def test_hcv_exterior_errors(gpu):
+    import wobtoolbox_amd as wob
+    w = _star(0, 300)
+    scene = wob.SceneHost(w.elements)
+    pts = _dev(w.points[:4], gpu)
+    ext = wob.Estimator.from_config(scene, dict(w.solver, flags=HCV))
+    interior = wob.Estimator.from_config(scene, dict(w.solver, domain="InteriorDomain", flags=HCV))
+    with pytest.raises(wob.WobError, match="radius_scale"):
+        scene.fit_harmonic_cv(n_sources=8, radius_scale=1.5, n_train=64, exterior=True)
+    scene.fit_harmonic_cv(n_sources=8, n_train=64)   # interior fit
+    with pytest.raises(wob.WobError, match="exterior walks need"):
+        ext.compute_domain(pts, 4)
+    scene.fit_harmonic_cv(n_sources=8, n_train=64, exterior=True)
+    with pytest.raises(wob.WobError, match="interior walks need"):
+        interior.compute_domain(pts, 4)
+    s, _ = ext.compute_domain(pts, 4)
+    assert np.isfinite(s.cpu().numpy()).all()
+    src, c0 = scene.harmonic_cv()
+    scene.set_harmonic_cv(src, c0)   # explicit sources serve either side
+    ext.compute_domain(pts, 4)
+    interior.compute_domain(pts, 4)

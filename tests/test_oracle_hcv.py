@@ -203,3 +203,54 @@ def test_hcv_single_layer_robin_unbiased_and_variance(oracle):
         assert np.abs(z[:, 1:]).max() < 5.0, (fl, np.abs(z[:, 1:]).max())
         var[fl] = np.mean(se[:, 0] ** 2)
     assert var[0] / var[16] > 1e4, var
+
+
+def _replicates(sc, solver, pts, reps, paths):
+    """Per-point mean and standard error from `reps` independent estimates of `paths` paths
+    each (successive path ranges of the same streams).  With variances far below the square
+    of the value, the FP32 sums of squares cannot resolve the variance; the spread of the
+    replicate means can."""
+    ms = np.stack([sc.estimate_domain(solver, pts, paths, seed=4, path_base=r * paths, threads=8)[0] / paths
+                   for r in range(reps)])
+    return ms.mean(0), ms.std(0, ddof=1) / np.sqrt(reps)
+
+
+def test_hcv_exterior_single_layer_unbiased_and_variance(oracle):
+    """Exterior star scenes (the batched workload's solver: single layer, RIS 64; Neumann at
+    path length 7, Robin at 4, mixed at 6 with c = 2) against their log-source solutions.
+    The exterior fit puts the sources inside the body (0.75 x the smallest centre-to-element
+    distance) and leaves out the constant (a single-layer potential has none at infinity).
+    The estimate with the control variate has a variance four orders of magnitude below plain
+    WoB's, low enough to show the O(h^2) error of the linearly interpolated data: its error
+    against the exact solution falls about 16-fold from 300 to 1200 segments (a wrong sign or a
+    dropped alpha would not converge).  A constant c0 left in u_h shifts a Neumann estimate by
+    c0 exactly; with Robin or Dirichlet data it enters the residual data, where the exterior
+    single layer cannot represent it, and the estimate leaves the solution: hence c0 = 0."""
+    from helpers import star_scene
+    for kind in (0, 1, 2):
+        err = {}
+        for nseg in (300, 1200):
+            w = star_scene(seed=4, n_segments=nseg, kind=kind)
+            sc = oracle.OracleScene(w.elements)
+            rec = sc.hcv_targets(8192, seed=5, volume_samples=0)
+            src = hcv_sources(w.elements, 64, 0.75, exterior=True)
+            assert np.linalg.norm(src - src.mean(0), axis=1).max() < 0.75   # inside the star (r >= 0.8)
+            c0, wt = hcv_fit_numpy(rec, src, constant=False)
+            assert c0 == 0.0
+            sc.set_harmonic_cv(np.concatenate([src, wt[:, None]], 1), 0.0)
+            pts = w.points[:24]
+            ex = w.exact(pts)
+            m1, se1 = _replicates(sc, dict(w.solver, flags=16), pts, 8, 128)
+            err[nseg] = np.sqrt(np.mean((m1 - ex) ** 2, axis=0))
+        assert np.all(err[1200] < err[300] / 6), (kind, err)
+        m0, se0 = _replicates(sc, dict(w.solver, flags=0), pts, 8, 128)
+        assert np.mean(se0[:, 0] ** 2) / np.mean(se1[:, 0] ** 2) > 1e4, kind
+        sc.set_harmonic_cv(np.concatenate([src, wt[:, None]], 1), 1e-2)
+        s, _ = sc.estimate_domain(dict(w.solver, flags=16), pts, 128, seed=4, threads=8)
+        m_shift = s[:, 0] / 128
+        if kind == 0:
+            sc.set_harmonic_cv(np.concatenate([src, wt[:, None]], 1), 0.0)
+            s, _ = sc.estimate_domain(dict(w.solver, flags=16), pts, 128, seed=4, threads=8)
+            np.testing.assert_allclose(m_shift - s[:, 0] / 128, 1e-2, rtol=1e-3)
+        else:
+            assert np.sqrt(np.mean((m_shift - ex[:, 0]) ** 2)) > 100 * err[1200][0], (kind, err)

@@ -178,32 +178,38 @@ class SceneHost:
         self._has_volume = len(w) > 0
 
     @staticmethod
-    def _hcv_desc(n_sources, radius_scale, n_train, volume_samples, eps, ridge, seed, stream_id):
+    def _hcv_desc(n_sources, radius_scale, n_train, volume_samples, eps, ridge, seed, stream_id, exterior=False):
         return _capi.HcvDesc(int(n_sources), float(radius_scale), int(n_train), int(volume_samples), float(eps),
-                             float(ridge), int(seed), int(stream_id))
+                             float(ridge), int(seed), int(stream_id), _capi.HCV_EXTERIOR if exterior else 0)
 
-    def fit_harmonic_cv(self, n_sources=64, radius_scale=1.5, n_train=16384, volume_samples=16, eps=1e-4,
-                        ridge=1e-10, seed=0, stream_id=0, stream=None):
+    def fit_harmonic_cv(self, n_sources=64, radius_scale=None, n_train=16384, volume_samples=16, eps=1e-4,
+                        ridge=1e-10, seed=0, stream_id=0, exterior=False, stream=None):
         """Fit and bind the harmonic control variate u_h = c0 + sum_j w_j phi(|x - s_j|^2) of
         WOB_EST_HARMONIC_CV walks (wob_scene_fit_harmonic_cv) to the walk's known-term data
         at n_train uniform boundary samples; volume_samples is ignored without a volume
-        source.  Returns the RMS residual of the fit."""
+        source.  exterior=True fits the exterior domain's (sources inside the body at
+        radius_scale x the smallest centre-to-element distance, default 0.75; c0 = 0); the
+        interior fit's sources lie at radius_scale x the bounding radius (default 1.5).
+        Returns the RMS residual of the fit."""
         _torch()
         if not self._has_volume:
             volume_samples = 0
-        d = self._hcv_desc(n_sources, radius_scale, n_train, volume_samples, eps, ridge, seed, stream_id)
+        if radius_scale is None:
+            radius_scale = 0.75 if exterior else 1.5
+        d = self._hcv_desc(n_sources, radius_scale, n_train, volume_samples, eps, ridge, seed, stream_id, exterior)
         rms = C.c_float()
         check(lib().wob_scene_fit_harmonic_cv(self._h, C.byref(d), C.byref(rms), _stream_ptr(stream)))
         return rms.value
 
     def harmonic_cv_targets(self, n_train=16384, volume_samples=16, eps=1e-4, seed=0, stream_id=0, n_sources=1,
-                            stream=None):
+                            exterior=False, stream=None):
         """The fit's training data: BoundaryPoint records (n, 2 dim + 4) whose boundary value
         is the target."""
         torch = _torch()
         if not self._has_volume:
             volume_samples = 0
-        d = self._hcv_desc(n_sources, 1.5, n_train, volume_samples, eps, 0.0, seed, stream_id)
+        d = self._hcv_desc(n_sources, 0.75 if exterior else 1.5, n_train, volume_samples, eps, 0.0, seed, stream_id,
+                           exterior)
         rec = torch.zeros((n_train, 2 * self.dim + 4), dtype=torch.float32, device="cuda")
         check(lib().wob_harmonic_cv_targets(self._h, C.byref(d), _ptr(rec), _stream_ptr(stream)))
         return rec

@@ -55,9 +55,10 @@ class EstimatorDesc(C.Structure):
 class HcvDesc(C.Structure):
     _fields_ = [("n_sources", C.c_uint32), ("radius_scale", C.c_float), ("n_train", C.c_uint32),
                 ("volume_samples", C.c_uint32), ("eps", C.c_float), ("ridge", C.c_float), ("seed", C.c_uint64),
-                ("stream_id", C.c_uint32)]
+                ("stream_id", C.c_uint32), ("flags", C.c_uint32)]
 
 
+HCV_EXTERIOR = 1
 EST_EXCLUDE_ORIGIN_ELEMENT = 1
 EST_CONTROL_VARIATE = 2
 EST_BOUNDARY_DENSITY = 4

@@ -49,6 +49,7 @@ struct wob_scene {
     int device = 0;
     float *cv_table = nullptr;
     float4 *hcv_src = nullptr;
+    int hcv_side = 0;       // the bound harmonic control variate: kHcvInterior / kHcvExterior / kHcvAny
     void *vol_bufs[3] = {nullptr, nullptr, nullptr};
     uint64_t version = 0;   // bumped whenever device data referenced by launches changes
     double *aux = nullptr;  // FP64 accumulators of VPL path phases
@@ -58,9 +59,10 @@ struct wob_scene {
 const wobhost::HostScene &wob_scene_host(const wob_scene *scene) { return scene->host; }
 const wobdev::DevScene &wob_scene_dev(const wob_scene *scene) { return scene->dev; }
 
-wob_status wob_scene_bind_hcv(wob_scene *scene, float4 *d_src, uint32_t m, float c0) {
+wob_status wob_scene_bind_hcv(wob_scene *scene, float4 *d_src, uint32_t m, float c0, int side) {
     if (scene->hcv_src) (void)hipFree(scene->hcv_src);
     scene->hcv_src = d_src;
+    scene->hcv_side = side;
     scene->dev.hcv_src = d_src;
     scene->dev.hcv_m = d_src ? m : 0u;
     scene->dev.hcv_c0 = d_src ? c0 : 0.f;
@@ -158,12 +160,20 @@ wob_status check_est(const wob_scene *scene, const wob_estimator_desc *est, int 
             return fail(WOB_INVALID_ARG, "no control-variate table with path_length value functions is bound to the scene");
     }
     if (est->flags & WOB_EST_HARMONIC_CV) {
-        if (est->bie_model == WOB_DIRECT || est->estimator != WOB_BACKWARD || est->domain != WOB_INTERIOR)
-            return fail(WOB_UNSUPPORTED_COMBO, "the harmonic control variate needs the single-layer or double-layer "
-                                               "model, the backward estimator and an interior domain");
+        if (est->bie_model == WOB_DIRECT || est->estimator != WOB_BACKWARD ||
+            (est->domain != WOB_INTERIOR && est->bie_model != WOB_SINGLE_LAYER))
+            return fail(WOB_UNSUPPORTED_COMBO, "the harmonic control variate needs the backward estimator and the "
+                                               "single-layer model, or the double-layer model in an interior domain");
         if (scene->dev.hcv_m == 0)
             return fail(WOB_INVALID_ARG, "no harmonic control variate is bound to the scene "
                                          "(wob_scene_fit_harmonic_cv / wob_scene_set_harmonic_cv)");
+        const int want = est->domain == WOB_INTERIOR ? wob_hcv_interior : wob_hcv_exterior;
+        if (scene->hcv_side != wob_hcv_any && scene->hcv_side != want)
+            return fail(WOB_INVALID_ARG, est->domain == WOB_INTERIOR
+                                             ? "the scene's harmonic control variate was fitted for the exterior "
+                                               "domain (WOB_HCV_EXTERIOR); interior walks need an interior fit"
+                                             : "the scene's harmonic control variate was fitted for the interior "
+                                               "domain; exterior walks need a WOB_HCV_EXTERIOR fit");
     }
     if ((est->flags & WOB_EST_BOUNDARY_DENSITY) && est->estimator != WOB_BACKWARD)
         return fail(WOB_UNSUPPORTED_COMBO, "boundary density estimates need the backward estimator");

@@ -25,4 +25,7 @@ wob_status wob_scene_bind_cv_table(wob_scene *scene, float *d_table, uint32_t R)
 const wobdev::DevScene &wob_scene_dev(const wob_scene *scene);
 // takes ownership of d_src (device, m float4 records); frees the previous sources; m == 0
 // (d_src == nullptr) unbinds
-wob_status wob_scene_bind_hcv(wob_scene *scene, float4 *d_src, uint32_t m, float c0);
+// side of the domain a bound harmonic control variate serves: fitted for the interior, fitted
+// for the exterior, or given explicitly (wob_scene_set_harmonic_cv: either)
+enum { wob_hcv_interior = 0, wob_hcv_exterior = 1, wob_hcv_any = 2 };
+wob_status wob_scene_bind_hcv(wob_scene *scene, float4 *d_src, uint32_t m, float c0, int side);
