@@ -309,13 +309,101 @@ def batch_leg(args, torch, wob, G):
         rmse[kind] = float(np.sqrt(np.mean(e ** 2)))
         mean_se[kind] = float(np.mean(np.sqrt(v)))
         bias[kind] = float(np.sqrt(max(np.mean(e ** 2) - np.mean(v), 0.0)))
-    return {"workload": "star_exterior_batch_2d", "scenes": len(ws), "segments": int(sum(w.elements.n for w in ws)),
-            "points": n_points, "paths_per_point_per_pass": paths, "passes": args.batch_passes,
-            "paths_per_s": total / el, "ms_per_pass": el / args.batch_passes * 1e3, "tuned": best,
-            "tune_s": tune_s, "rmse_u_by_problem": rmse, "mean_se_u_by_problem": mean_se,
-            "bias_rms_estimate_by_problem": bias, "paths_per_point_total": b.passes * paths,
-            "solver": "IndirectSingleLayer, ExteriorDomain, BackwardEstimator, RIS 64, eps 1e-5; path_length 7/4/6 "
-                      "and fkesc 1/1/2 for Neumann/Robin/mixed scenes"}
+    out = {"workload": "star_exterior_batch_2d", "scenes": len(ws), "segments": int(sum(w.elements.n for w in ws)),
+           "points": n_points, "paths_per_point_per_pass": paths, "passes": args.batch_passes,
+           "paths_per_s": total / el, "ms_per_pass": el / args.batch_passes * 1e3, "tuned": best,
+           "tune_s": tune_s, "rmse_u_by_problem": rmse, "mean_se_u_by_problem": mean_se,
+           "bias_rms_estimate_by_problem": bias, "paths_per_point_total": b.passes * paths,
+           "solver": "IndirectSingleLayer, ExteriorDomain, BackwardEstimator, RIS 64, eps 1e-5; path_length 7/4/6 "
+                     "and fkesc 1/1/2 for Neumann/Robin/mixed scenes"}
+    if not args.no_batch_cv:
+        errs = [e for e, _ in per_kind.values()]
+        plain_rmse_all = float(np.sqrt(np.mean(np.concatenate([np.concatenate(e) for e in errs]) ** 2)))
+        plain_var = {kind: float(np.mean(np.concatenate(v))) * b.passes * paths for kind, (_, v) in per_kind.items()}
+        out["control_variate"] = batch_cv(args, torch, wob, ws, scenes, pts, best, plain_rmse_all,
+                                          b.passes * el / args.batch_passes, b.passes * paths, plain_var)
+    return out
+
+
+def batch_cv(args, torch, wob, ws, scenes, pts, best, plain_rmse_all, plain_s, plain_paths, plain_var):
+    """The harmonic control variate on the batched exterior scenes: one exterior fit per scene
+    (fundamental solutions inside the body, no constant; fits on the clock), then the same
+    batched launch with WOB_EST_HARMONIC_CV and the tuned configuration.  Reports paths/s, and
+    per problem kind the RMSE against the exact solution beside the standard error; the
+    standard errors come from the spread of the per-pass means (the control variate's
+    variances are far below the resolution of FP32 sums of squared values near u), and the
+    time to plain WoB's final RMSE over all points."""
+    print("[bench] batch control variate: 64 exterior fits", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fit_res = [sc.fit_harmonic_cv(n_sources=args.batch_cv_sources, n_train=8192, seed=7, exterior=True)
+               for sc in scenes]
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    paths = args.batch_paths
+    entries = [(wob.Estimator.from_config(sc, w.solver, flags=HCV), p, paths) for sc, w, p in zip(scenes, ws, pts)]
+    cfg = {k: best[k] for k in ("paths_per_lane", "block_size", "schedule", "regen_threshold")}
+    b = wob.Batch(entries, persistent=bool(best["persistent"]), use_graph=True, **cfg)
+    b.run(1, seed=1)   # warm-up (graph capture), then reset the sums: the timed passes start at 0
+    for s, q in b.outputs:
+        s.zero_()
+        q.zero_()
+    b.passes = 0
+    reps = max(args.batch_passes, 4)
+    snaps, el = [], 0.0
+    prev = [s[:, 0].clone() for s, _ in b.outputs]
+    for r in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.run(1, seed=0)
+        torch.cuda.synchronize()
+        el += time.perf_counter() - t0
+        cur = [s[:, 0].clone() for s, _ in b.outputs]
+        snaps.append([((c - p_) / paths).cpu().numpy() for c, p_ in zip(cur, prev)])
+        prev = cur
+    first_pass_s = el / reps
+    n_points = sum(len(w.points) for w in ws)
+    per_kind = {}
+    err1_all = []
+    for i, w in enumerate(ws):
+        reps_i = np.stack([snaps[r][i] for r in range(reps)])          # (reps, points)
+        ex = w.exact(w.points.astype(np.float64))[:, 0]
+        m = reps_i.mean(0)
+        se = reps_i.std(0, ddof=1) / np.sqrt(reps)
+        e, v, pv = per_kind.setdefault(w.solver["problem"], ([], [], []))
+        e.append(m - ex)
+        v.append(se ** 2)
+        pv.append(reps_i.var(0, ddof=1) * paths)
+        err1_all.append(reps_i[0] - ex)
+    rmse, mean_se, bias, var_path = {}, {}, {}, {}
+    for kind, (e, v, pv) in per_kind.items():
+        e, v = np.concatenate(e), np.concatenate(v)
+        rmse[kind] = float(np.sqrt(np.mean(e ** 2)))
+        mean_se[kind] = float(np.mean(np.sqrt(v)))
+        bias[kind] = float(np.sqrt(max(np.mean(e ** 2) - np.mean(v), 0.0)))
+        var_path[kind] = float(np.mean(np.concatenate(pv)))
+    rmse1_all = float(np.sqrt(np.mean(np.concatenate(err1_all) ** 2)))
+    t_cv = fit_s + first_pass_s
+    return {"kind": "harmonic, exterior (fundamental solutions inside each body, no constant term; FP64 least "
+                    "squares on the walk's boundary data)",
+            "n_sources": args.batch_cv_sources, "fits": len(scenes), "fit_s": fit_s,
+            "fit_rms_residual_max": float(max(fit_res)),
+            "paths_per_s": n_points * paths * reps / el, "ms_per_pass": el / reps * 1e3,
+            "paths_per_point_total": reps * paths, "rmse_u_by_problem": rmse, "mean_se_u_by_problem": mean_se,
+            "se_from": f"spread of {reps} per-pass means",
+            "bias_rms_estimate_by_problem": bias, "per_path_variance_by_problem": var_path,
+            "plain_per_path_variance_by_problem": plain_var,
+            "variance_ratio_by_problem": {k: plain_var[k] / var_path[k] for k in var_path if var_path[k] > 0},
+            "time_to_plain_rmse": {"plain_rmse_all": plain_rmse_all, "plain_s": plain_s,
+                                   "plain_paths_per_point": plain_paths,
+                                   "cv_rmse_all_after_one_pass": rmse1_all,
+                                   "cv_s": t_cv if rmse1_all <= plain_rmse_all else None,
+                                   "cv_paths_per_point": paths,
+                                   "speedup": plain_s / t_cv if rmse1_all <= plain_rmse_all else None,
+                                   "plain_s_to_cv_rmse_extrapolated": plain_s * (plain_rmse_all / rmse1_all) ** 2,
+                                   "note": "control variate: all fits plus its first pass; the plain arm's time to "
+                                           "the control variate's one-pass RMSE is noise-only (RMSE^2 scaling), "
+                                           "its truncation bias (Robin, mixed) would keep it above"}}
 
 
 def poisson_leg(args, torch, wob, G):
@@ -531,6 +619,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-paths", type=int, default=16, help="paths per point per batch pass")
     ap.add_argument("--batch-passes", type=int, default=4)
+    ap.add_argument("--no-batch-cv", action="store_true", help="skip the batch leg's harmonic control variate")
+    ap.add_argument("--batch-cv-sources", type=int, default=64)
     ap.add_argument("--no-poisson", action="store_true")
     ap.add_argument("--poisson-paths", type=int, default=16, help="paths per point per Poisson pass")
     ap.add_argument("--poisson-passes", type=int, default=4)
