@@ -149,7 +149,7 @@ def test_app_3d_progressive_run(tmp_path, gpu):
 
 def test_app_harmonic_cv_option_checks_the_solver(tmp_path):
     """--harmonic-cv is refused, before any GPU work, for solvers the control variate does not
-    cover (the direct model, the forward estimator, exterior domains)."""
+    cover (the direct model, the forward estimator, the exterior double layer)."""
     obj = tmp_path / "sphere.obj"
     _write_sphere_obj(obj)
     for kw in (dict(bie_model="Direct"), dict(estimator="ForwardEstimator"), dict(domain="ExteriorDomain")):
@@ -186,6 +186,26 @@ def test_app_harmonic_cv_lowers_rmse(tmp_path, gpu):
         assert rows[True][-1, 2] < 0.1 * rows[False][-1, 2]
         assert rows[True][-1, 2] <= rows[True][0, 2]
         assert rows[True][0, 1] > 0
+
+
+@pytest.mark.gpu
+def test_app_2d_exterior_harmonic_cv(tmp_path, gpu):
+    """The 2D app's exterior Neumann problem (single layer, RIS 16) on the circle with
+    --harmonic-cv: the sources go inside the circle, and the RMSE against the apps' exterior
+    solution x / |x|^2 falls far below plain WoB's."""
+    obj = tmp_path / "circle.obj"
+    _write_circle_obj(obj)
+    cfg = _config(tmp_path, obj, bie_model="IndirectSingleLayer", domain="ExteriorDomain", problem="NeumannProblem",
+                  path_length=6, num_sample_paths=64, num_resampling_candidates=16)
+    rows = {}
+    for cv in (False, True):
+        out = _run(B.APPS[2], cfg, "--max-iterations", "3", *(["--harmonic-cv", "32"] if cv else []))
+        rows[cv] = np.loadtxt(tmp_path / "out" / "rmse_list.csv", delimiter=",", ndmin=2)
+        if cv:
+            assert "(exterior)" in out
+    print(f"2D app exterior Neumann: RMSE u plain {rows[False][:, 2]}, with the control variate {rows[True][:, 2]}",
+          flush=True)
+    assert rows[True][-1, 2] < 0.1 * rows[False][-1, 2]
 
 
 def _vpl_config(tmp_path, **kw):

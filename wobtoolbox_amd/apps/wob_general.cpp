@@ -22,10 +22,12 @@
 //     sums instead;
 //   - paths use the engine's Philox streams (pass i walks paths [(i-1) N, i N) of every point).
 // Addition: --harmonic-cv N fits the engine's harmonic control variate (N fundamental solutions
-// on a sphere or circle of RADIUS_SCALE x the scene radius, default 2, fitted to 32768 boundary
-// samples of the walk's data; wob_scene_fit_harmonic_cv) before the first pass and walks the
-// residual (WOB_EST_HARMONIC_CV).  The fit's time counts in the "seconds" column.  It needs the
-// single- or double-layer model, the backward estimator and an interior domain.
+// fitted to 32768 boundary samples of the walk's data; wob_scene_fit_harmonic_cv) before the
+// first pass and walks the residual (WOB_EST_HARMONIC_CV).  Interior domains: sources on a
+// sphere or circle of RADIUS_SCALE x the scene radius (default 2); exterior domains: inside
+// the body at RADIUS_SCALE x the smallest centre-to-element distance (default 0.75).  The fit's
+// time counts in the "seconds" column.  It needs the backward estimator and the single-layer
+// model, or the double-layer model in an interior domain.
 // The 3D Robin elements keep the apps' robin alpha (1, 1, 0): the aggregate initialisation at
 // general_3d.cu:279-284 leaves the third vertex's alpha at 0.
 #include <hip/hip_runtime_api.h>
@@ -155,7 +157,7 @@ int run_app(int argc, char **argv) {
     bool dry = false;
     unsigned seed = 0;
     unsigned hcv_sources = 0;
-    float hcv_radius = 2.0f;
+    float hcv_radius = 0.0f;   // 0: the side's default
     for (int a = 2; a < argc; ++a) {
         if (!std::strcmp(argv[a], "--max-iterations") && a + 1 < argc) max_iter = std::atol(argv[++a]);
         else if (!std::strcmp(argv[a], "--dry-run")) dry = true;
@@ -187,11 +189,13 @@ int run_app(int argc, char **argv) {
     const auto domain = wob::string_to_domain_type(cfg.str("domain"));
     const auto est = wob::string_to_estimator_type(cfg.str("estimator"));
     const auto problem = wob::string_to_problem_type(cfg.str("problem"));
-    if (hcv_sources > 0 &&
-        ((bie != wob::IndirectSingleLayer && bie != wob::IndirectDoubleLayer) || est != wob::BackwardEstimator ||
-         domain != wob::InteriorDomain))
-        throw std::runtime_error("--harmonic-cv needs IndirectSingleLayer or IndirectDoubleLayer, BackwardEstimator "
-                                 "and InteriorDomain");
+    if (hcv_sources > 0 && (est != wob::BackwardEstimator ||
+                            (bie != wob::IndirectSingleLayer &&
+                             (bie != wob::IndirectDoubleLayer || domain != wob::InteriorDomain))))
+        throw std::runtime_error("--harmonic-cv needs BackwardEstimator and IndirectSingleLayer, or "
+                                 "IndirectDoubleLayer in the InteriorDomain");
+    const bool hcv_exterior = domain == wob::ExteriorDomain;
+    if (hcv_radius == 0.0f) hcv_radius = hcv_exterior ? 0.75f : 2.0f;
 
     std::vector<double> V;
     std::vector<int> F;
@@ -320,11 +324,13 @@ int run_app(int argc, char **argv) {
     if (hcv_sources > 0) {
         HIP_OK(hipDeviceSynchronize());
         const auto t0 = std::chrono::steady_clock::now();
-        const float rms = scene.fit_harmonic_cv(hcv_sources, hcv_radius, 32768, 0, eps, 1e-10f, seed);
+        const float rms = scene.fit_harmonic_cv(hcv_sources, hcv_radius, 32768, 0, eps, 1e-10f, seed, nullptr,
+                                                hcv_exterior);
         HIP_OK(hipDeviceSynchronize());
         elapsed += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         std::cout << "harmonic control variate\t: " << hcv_sources << " sources at " << hcv_radius
-                  << " x the scene radius, fit residual " << rms << ", " << elapsed << " s" << std::endl;
+                  << (hcv_exterior ? " x the inner radius (exterior)" : " x the scene radius") << ", fit residual "
+                  << rms << ", " << elapsed << " s" << std::endl;
     }
 
     std::vector<double> acc(2 * m * V1);
