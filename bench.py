@@ -638,7 +638,7 @@ def main():
     ap.add_argument("--cv-train-steps", type=int, default=1500)
     ap.add_argument("--cv-sigma", type=float, default=0.5, help="scale of the Gaussian Fourier frequencies")
     ap.add_argument("--cv-samples", type=int, default=16, help="samples of the domain-level control-variate integral")
-    ap.add_argument("--hcv-sources", type=int, default=128, help="harmonic control variate: fundamental solutions")
+    ap.add_argument("--hcv-sources", type=int, default=192, help="harmonic control variate: fundamental solutions")
     ap.add_argument("--hcv-radius", type=float, default=2.0, help="harmonic control variate: source radius / scene radius")
     ap.add_argument("--hcv-train", type=int, default=32768, help="harmonic control variate: boundary samples of the fit")
     ap.add_argument("--hcv-paths-per-pass", type=int, default=1, help="harmonic control variate: paths per point per pass")
