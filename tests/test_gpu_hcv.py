@@ -437,6 +437,8 @@ def test_hcv_exterior_3d_walk_matches_oracle(gpu, oracle):
     rms = scene.fit_harmonic_cv(n_sources=64, n_train=16384, seed=3, exterior=True)
     rec, c0 = scene.harmonic_cv()
     assert c0 == 0.0 and np.linalg.norm(rec[:, :3], axis=1).max() < 0.75
+    # the inner radius from FP64 point-triangle distances (host) against numpy's
+    assert np.allclose(rec[:, :3], hcv_sources(el, 64, 0.75, exterior=True), rtol=0, atol=1e-6)
     osc.set_harmonic_cv(rec, c0)
     rng = np.random.default_rng(8)
     d = rng.normal(size=(64, 3))
