@@ -274,7 +274,11 @@ wob_status wob_vpl_paths(const wob_scene *scene, const wob_estimator_desc *est, 
  * written by wob_vpl_paths (or any BoundaryPoint records and weights).  FP32 terms,
  * compensated FP32 sums over VPL ranges, FP64 accumulation into d_sum (deterministic).  The
  * caller divides by the number of paths, as the reference app does (:250-257), and skips
- * masked points by leaving them out of d_points.                                             */
+ * masked points by leaving them out of d_points.
+ * Concurrency: an aggregation and a path phase of the same scene work in disjoint scene
+ * buffers, so iteration k's aggregation may run on one stream while iteration k + 1's path
+ * phase runs on another (into other record / contribution buffers); two path phases, or two
+ * aggregations, of one scene must not overlap.  The first call of each may allocate.        */
 wob_status wob_vpl_aggregate(const wob_scene *scene, const wob_estimator_desc *est, const float *d_records,
                              const float *d_contrib, uint32_t n_vpl, const float *d_points, uint32_t n_points,
                              double *d_sum, void *stream);

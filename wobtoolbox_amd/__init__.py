@@ -358,15 +358,21 @@ class Estimator:
                                           _stream_ptr(stream)))
         return s, q
 
-    def compute_vpl_paths(self, n_paths, seed=0, path_base=0, stream_id=0, device="cuda", stream=None):
+    def compute_vpl_paths(self, n_paths, seed=0, path_base=0, stream_id=0, device="cuda", out=None, stream=None):
         """Path phase of the VPL method (apps/potential_flow_with_vpl.cu:73-88): n_paths uniform
         boundary start points and the boundary-density walk value at each.
 
-        Returns (records (n, 2*dim + 4) float32, contributions (n,) float32) device tensors."""
+        Returns (records (n, 2*dim + 4) float32, contributions (n,) float32) device tensors,
+        written into `out` = (records, contributions) if given."""
         torch = _torch()
         W = 2 * self.scene.dim + 4
-        rec = torch.empty((n_paths, W), dtype=torch.float32, device=device)
-        c = torch.empty(n_paths, dtype=torch.float32, device=device)
+        if out is not None:
+            rec, c = out
+            if rec.shape != (n_paths, W) or c.shape != (n_paths,) or not (rec.is_contiguous() and c.is_contiguous()):
+                raise ValueError("out must be contiguous ((n_paths, 2 dim + 4), (n_paths,)) float32 tensors")
+        else:
+            rec = torch.empty((n_paths, W), dtype=torch.float32, device=device)
+            c = torch.empty(n_paths, dtype=torch.float32, device=device)
         d, lc = self.desc(), self._launch()
         check(lib().wob_vpl_paths(self.scene.handle, C.byref(d), int(n_paths), seed, path_base, stream_id, _ptr(rec),
                                   _ptr(c), C.byref(lc), _stream_ptr(stream)))

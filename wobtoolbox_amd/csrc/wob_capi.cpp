@@ -54,6 +54,8 @@ struct wob_scene {
     uint64_t version = 0;   // bumped whenever device data referenced by launches changes
     double *aux = nullptr;  // FP64 accumulators of VPL path phases
     size_t aux_bytes = 0;
+    char *vpl = nullptr;    // VPL aggregation records and partial sums (apart from the walks' buffers)
+    size_t vpl_bytes = 0;
 };
 
 const wobhost::HostScene &wob_scene_host(const wob_scene *scene) { return scene->host; }
@@ -114,6 +116,8 @@ void free_scene(wob_scene *s) {
     s->hcv_src = nullptr;
     if (s->aux) (void)hipFree(s->aux);
     s->aux = nullptr;
+    if (s->vpl) (void)hipFree(s->vpl);
+    s->vpl = nullptr;
 }
 
 wob_status ensure_scratch(wob_scene *s, size_t bytes) {
@@ -139,6 +143,19 @@ wob_status ensure_aux(wob_scene *s, size_t bytes) {
     if (e != hipSuccess) return fail(WOB_OOM, std::string("hipMalloc aux: ") + hipGetErrorString(e));
     s->aux = (double *)p;
     s->aux_bytes = bytes;
+    return WOB_OK;
+}
+
+wob_status ensure_vpl(wob_scene *s, size_t bytes) {
+    if (bytes <= s->vpl_bytes) return WOB_OK;
+    if (s->vpl) WOB_HIP(hipFree(s->vpl), "hipFree VPL buffer");
+    s->vpl = nullptr;
+    s->vpl_bytes = 0;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(WOB_OOM, std::string("hipMalloc VPL buffer: ") + hipGetErrorString(e));
+    s->vpl = (char *)p;
+    s->vpl_bytes = bytes;
     return WOB_OK;
 }
 
@@ -962,10 +979,11 @@ wob_status wob_vpl_aggregate(const wob_scene *cscene, const wob_estimator_desc *
     wobl::vpl_plan(n_points, n_vpl, splits, chunk);
     const size_t rec_bytes = ((size_t)n_vpl * wobl::vpl_record_floats(D, est->bie_model) * sizeof(float) + 255) & ~(size_t)255;
     const size_t part_bytes = (size_t)splits * n_points * V * sizeof(double);
-    wob_status st = ensure_scratch(scene, rec_bytes + part_bytes);
+    // its own buffer: an aggregation may run beside a path phase of the same scene (another stream)
+    wob_status st = ensure_vpl(scene, rec_bytes + part_bytes);
     if (st != WOB_OK) return st;
-    float *rec = scene->scratch;
-    double *partial = (double *)((char *)scene->scratch + rec_bytes);
+    float *rec = (float *)scene->vpl;
+    double *partial = (double *)(scene->vpl + rec_bytes);
     hipStream_t hs = (hipStream_t)stream;
     // inv_pdf = total boundary area, as apps/potential_flow_with_vpl.cu:61-63 passes it
     WOB_HIP(wobl::launch_vpl_stage(D, est->bie_model, d_records, d_contrib, n_vpl, scene->host.total_area,
