@@ -154,12 +154,13 @@ template <typename S, unsigned Dim, bool IsVectorProblem = false> class SceneHos
     // wob_scene_fit_harmonic_cv): fitted to the walk's known-term data at n_train boundary
     // samples and bound to the scene; estimators whose desc().flags carry WOB_EST_HARMONIC_CV
     // then walk the residual.  exterior = true fits the exterior domain's (radius_scale in
-    // (0, 1), sources inside the body).  Returns the fit's RMS residual.
+    // (0, 1), sources inside the body); zero_charge = true with it keeps sum_j w_j = 0, as
+    // exterior double-layer walks need.  Returns the fit's RMS residual.
     float fit_harmonic_cv(uint32_t n_sources = 64, float radius_scale = 1.5f, uint32_t n_train = 16384,
                           uint32_t volume_samples = 0, float eps = 1e-4f, float ridge = 1e-10f, uint64_t seed = 0,
-                          void *stream = nullptr, bool exterior = false) {
+                          void *stream = nullptr, bool exterior = false, bool zero_charge = false) {
         const wob_hcv_desc d{n_sources, radius_scale, n_train, volume_samples, eps, ridge, seed, 0u,
-                             exterior ? WOB_HCV_EXTERIOR : 0u};
+                             (exterior ? WOB_HCV_EXTERIOR : 0u) | (zero_charge ? WOB_HCV_ZERO_CHARGE : 0u)};
         float rms = 0.f;
         check(wob_scene_fit_harmonic_cv(scene_, &d, &rms, stream));
         return rms;

@@ -383,14 +383,13 @@ wob_status wob_scene_set_control_variate(wob_scene *scene, wob_cv *cv, const flo
                                          void *stream);
 
 /* Learned harmonic control variate (no counterpart in the reference) for single-layer and
- * double-layer walks of interior domains and single-layer walks of exterior domains
- * (WOB_EST_HARMONIC_CV), e.g. the Poisson problem's double-layer estimator
+ * double-layer walks of interior and exterior domains (WOB_EST_HARMONIC_CV), e.g. the Poisson problem's double-layer estimator
  * (poisson_equation.cu:40-61), the mixed single-layer workload and the batched exterior scenes:
  *   u_h(x) = c0 + sum_j w_j phi(|x - s_j|^2),  phi(z) = z^(-1/2) (3D) or log2 z (2D),
  * a combination of fundamental solutions with sources s_j on a sphere (circle) outside the
  * scene (interior fits), so u_h is harmonic in the interior domain, or inside the body
  * (WOB_HCV_EXTERIOR), so u_h is harmonic in the exterior domain; an exterior fit has c0 = 0,
- * since a single-layer potential has no constant term at infinity.  The variance of the walk
+ * since a layer potential has no constant term at infinity.  The variance of the walk
  * falls with the size of the residual data.
  * Fit: the targets are the walk's own known-term data at n_train uniform boundary samples
  * (sample i on the Philox stream (seed, path i, point 0, stream_id)): the boundary datum plus
@@ -399,12 +398,16 @@ wob_status wob_scene_set_control_variate(wob_scene *scene, wob_cv *cv, const flo
  * (u_h, du_h/dn or du_h/dn + alpha u_h at the sample), (c0, w) minimise the ridge-regularised
  * squared error in FP64 (columns scaled to unit RMS, ridge relative to the mean diagonal),
  * and the result is bound to the scene.  An interior fit serves interior walks only, an
- * exterior fit exterior single-layer walks only.
+ * exterior fit exterior walks only (double-layer ones with WOB_HCV_ZERO_CHARGE).
  * Exterior source placement: a sphere (circle) of radius radius_scale * R_in about the centre
  * c of the scene's vertex box, R_in the smallest distance from c to an element; the sources
  * lie inside the body when c does and the body holds the ball of radius R_in about c (a body
  * star-shaped about c does).                                                                */
 #define WOB_HCV_EXTERIOR 1u
+/* With WOB_HCV_EXTERIOR: the fit keeps sum_j w_j = 0, so u_h decays at infinity like a
+ * double-layer potential (no log term in 2D, no 1/r term in 3D); exterior double-layer walks
+ * need such a fit, exterior single-layer walks take either.                                 */
+#define WOB_HCV_ZERO_CHARGE 2u
 typedef struct wob_hcv_desc {
     uint32_t n_sources;       /* 1 .. 1024 */
     float radius_scale;       /* interior: source radius / the scene's bounding radius about its box
@@ -415,7 +418,7 @@ typedef struct wob_hcv_desc {
     float ridge;              /* relative Tikhonov weight, >= 0 */
     uint64_t seed;
     uint32_t stream_id;
-    uint32_t flags;           /* WOB_HCV_EXTERIOR or 0 */
+    uint32_t flags;           /* 0, WOB_HCV_EXTERIOR, or WOB_HCV_EXTERIOR | WOB_HCV_ZERO_CHARGE */
 } wob_hcv_desc;
 /* rms_residual (host, may be NULL): RMS of target - (the sample's row) over the samples.  */
 wob_status wob_scene_fit_harmonic_cv(wob_scene *scene, const wob_hcv_desc *desc, float *rms_residual, void *stream);

@@ -194,16 +194,23 @@ def hcv_eval_f64(pos, src, w, c0):
     return np.concatenate([u[:, None], g], 1)
 
 
-def hcv_fit_numpy(rec, src, ridge=1e-10, constant=True):
+def hcv_fit_numpy(rec, src, ridge=1e-10, constant=True, zero_charge=False):
     """Ridge least squares of the records' targets on their rows, unit-RMS columns (FP64):
-    returns (c0, w); constant=False (exterior fits) leaves out the constant column (c0 = 0)."""
+    returns (c0, w); constant=False (exterior fits) leaves out the constant column (c0 = 0);
+    zero_charge=True (with constant=False) keeps sum_j w_j = 0, solving on the columns
+    b_j - b_last for all but the last source."""
     B = hcv_rows(rec, src)
     if not constant:
         B = B[:, 1:]
+    if zero_charge:
+        assert not constant
+        B = B[:, :-1] - B[:, -1:]
     t = np.asarray(rec, np.float64)[:, 2 * src.shape[1] + 1]
     sc = 1 / np.sqrt((B ** 2).mean(0))
     Bs = B * sc
     A = Bs.T @ Bs
     A += ridge * np.trace(A) / len(A) * np.eye(len(A))
     coef = np.linalg.solve(A, Bs.T @ t) * sc
+    if zero_charge:
+        coef = np.concatenate([coef, [-coef.sum()]])
     return (coef[0], coef[1:]) if constant else (0.0, coef)

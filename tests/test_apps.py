@@ -149,10 +149,10 @@ def test_app_3d_progressive_run(tmp_path, gpu):
 
 def test_app_harmonic_cv_option_checks_the_solver(tmp_path):
     """--harmonic-cv is refused, before any GPU work, for solvers the control variate does not
-    cover (the direct model, the forward estimator, the exterior double layer)."""
+    cover (the direct model, the forward estimator)."""
     obj = tmp_path / "sphere.obj"
     _write_sphere_obj(obj)
-    for kw in (dict(bie_model="Direct"), dict(estimator="ForwardEstimator"), dict(domain="ExteriorDomain")):
+    for kw in (dict(bie_model="Direct"), dict(estimator="ForwardEstimator")):
         cfg = _config(tmp_path, obj, **kw)
         r = subprocess.run([B.APPS[3], str(cfg), "--harmonic-cv", "16", "--dry-run"], capture_output=True, text=True,
                            timeout=60)
@@ -205,6 +205,15 @@ def test_app_2d_exterior_harmonic_cv(tmp_path, gpu):
             assert "(exterior)" in out
     print(f"2D app exterior Neumann: RMSE u plain {rows[False][:, 2]}, with the control variate {rows[True][:, 2]}",
           flush=True)
+    assert rows[True][-1, 2] < 0.1 * rows[False][-1, 2]
+    # exterior Dirichlet with the double layer (the reference's exterior Dirichlet configs):
+    # the fit keeps sum_j w_j = 0
+    cfg = _config(tmp_path, obj, domain="ExteriorDomain", path_length=6, num_sample_paths=64)
+    for cv in (False, True):
+        _run(B.APPS[2], cfg, "--max-iterations", "3", *(["--harmonic-cv", "32"] if cv else []))
+        rows[cv] = np.loadtxt(tmp_path / "out" / "rmse_list.csv", delimiter=",", ndmin=2)
+    print(f"2D app exterior Dirichlet (double layer): RMSE u plain {rows[False][:, 2]}, with the control variate "
+          f"{rows[True][:, 2]}", flush=True)
     assert rows[True][-1, 2] < 0.1 * rows[False][-1, 2]
 
 

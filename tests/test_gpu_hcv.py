@@ -462,3 +462,63 @@ def test_hcv_exterior_3d_walk_matches_oracle(gpu, oracle):
             assert frac < 0.03
     _say(f"3D exterior Neumann dipole: RMSE u plain {err[0]:.3e}, with the control variate {err[HCV]:.3e}")
     assert err[HCV] < 0.05 * err[0]
+
+
+def test_hcv_exterior_double_layer_walk_matches_oracle(gpu, oracle):
+    """Exterior double-layer Dirichlet walks (line sampling, path length 6) with a zero-charge
+    exterior fit: the device fit equals the FP64 constrained least squares (sum_j w_j = 0), the
+    walk matches the oracle path by path in 2D (star polygon) and 3D (the dipole z / |x|^3 on a
+    displaced icosphere), and the error against the exact solution is far below plain WoB's."""
+    import wobtoolbox_amd as wob
+    from test_oracle_hcv import _star_dirichlet_exterior
+    solver = dict(bie_model="IndirectDoubleLayer", domain="ExteriorDomain", estimator="BackwardEstimator",
+                  problem="DirichletProblem", path_length=6, eps=1e-5)
+    el2, ex2, pts2 = _star_dirichlet_exterior(400)
+    V, F = G.icosphere(3)
+
+    def dipole(p):
+        p = np.asarray(p, np.float64)
+        r2 = (p * p).sum(1)
+        return np.stack([p[:, 2] / r2 ** 1.5, -3 * p[:, 2] * p[:, 0] / r2 ** 2.5, -3 * p[:, 2] * p[:, 1] / r2 ** 2.5,
+                         1 / r2 ** 1.5 - 3 * p[:, 2] ** 2 / r2 ** 2.5], 1)
+    el3 = G.mesh_elements(G.displace_radial(V), F, G.DIRICHLET, lambda p, nr, ty: dipole(p)[:, 0])
+    rng = np.random.default_rng(8)
+    d = rng.normal(size=(48, 3))
+    pts3 = (d / np.linalg.norm(d, axis=1, keepdims=True) * rng.uniform(1.6, 3.0, (48, 1))).astype(np.float32)
+    for el, ex, pts in ((el2, ex2, pts2), (el3, dipole, pts3)):
+        D = el.dim
+        scene, osc = _scenes(el, None, oracle)
+        rms = scene.fit_harmonic_cv(n_sources=64, n_train=8192, seed=3, exterior=True, zero_charge=True)
+        rec, c0 = scene.harmonic_cv()
+        t = scene.harmonic_cv_targets(8192, seed=3, exterior=True).cpu().numpy()
+        src = hcv_sources(el, 64, 0.75, exterior=True)
+        _, wt = hcv_fit_numpy(t, src, constant=False, zero_charge=True)
+        u_dev = scene.eval_harmonic_cv(_dev(pts, gpu)).cpu().numpy()[:, 0]
+        u_ref = hcv_eval_f64(pts, src, wt, 0.0)[:, 0]
+        assert c0 == 0.0 and abs(rec[:, D].astype(np.float64).sum()) < 1e-5 * np.abs(rec[:, D]).sum()
+        assert np.abs(u_dev - u_ref).max() < 1e-5 * (1 + np.abs(u_ref).max())
+        osc.set_harmonic_cv(rec, c0)
+        paths = 64
+        err = {}
+        for flags in (0, HCV):
+            est = wob.Estimator.from_config(scene, dict(solver, flags=flags))
+            s, q = est.compute_domain(_dev(pts, gpu), paths, seed=5)
+            m, _ = mean_se(s.cpu().numpy(), q.cpu().numpy(), paths)
+            err[flags] = float(np.sqrt(np.mean((m[:, 0] - ex(pts)[:, 0]) ** 2)))
+            if flags:
+                so, qo = osc.estimate_domain(dict(solver, flags=flags), pts, paths, seed=5, pick_mode=1)
+                mo, _ = mean_se(so, qo, paths)
+                scale = np.abs(mo).max(0) + 1e-6
+                frac = (np.abs(m - mo) > 1e-4 * scale).mean()
+                assert frac < 0.03, frac
+        _say(f"{D}D exterior double layer: fit residual {rms:.2e}, RMSE u plain {err[0]:.3e}, "
+             f"with the control variate {err[HCV]:.3e}")
+        assert err[HCV] < 0.05 * err[0]
+    # a fit without the zero-charge constraint is refused for exterior double-layer walks
+    scene = wob.SceneHost(el2)
+    scene.fit_harmonic_cv(n_sources=16, n_train=256, seed=3, exterior=True)
+    est = wob.Estimator.from_config(scene, dict(solver, flags=HCV))
+    with pytest.raises(wob.WobError, match="ZERO_CHARGE"):
+        est.compute_domain(_dev(pts2, gpu), 4)
+    with pytest.raises(wob.WobError, match="needs WOB_HCV_EXTERIOR"):
+        scene.fit_harmonic_cv(n_sources=16, n_train=256, zero_charge=True)

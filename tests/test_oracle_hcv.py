@@ -254,3 +254,42 @@ def test_hcv_exterior_single_layer_unbiased_and_variance(oracle):
             np.testing.assert_allclose(m_shift - s[:, 0] / 128, 1e-2, rtol=1e-3)
         else:
             assert np.sqrt(np.mean((m_shift - ex[:, 0]) ** 2)) > 100 * err[1200][0], (kind, err)
+
+
+def _star_dirichlet_exterior(n_segments, seed=4):
+    """A star polygon with exterior Dirichlet data from four log sources inside it whose charges
+    sum to zero (u = O(1/r) at infinity, a double-layer potential's class), and 24 points at
+    1.5 .. 3 from the centre."""
+    rng = np.random.default_rng(seed)
+    ak, phk = rng.uniform(0, 0.08 / np.arange(1, 7)), rng.uniform(0, 2 * np.pi, 6)
+    th = 2 * np.pi * np.arange(n_segments) / n_segments
+    r = 1 + (ak[None, :] * np.cos(np.arange(1, 7)[None, :] * th[:, None] + phk[None, :])).sum(1)
+    src = np.array([[0.2, 0.1], [-0.3, 0.2], [0.1, -0.35], [-0.1, -0.1]])
+    ex = G.make_log_sources_2d(src, np.array([0.7, -0.4, 0.5, -0.8]))
+    el = G.polygon_elements(np.stack([r * np.cos(th), r * np.sin(th)], 1), G.DIRICHLET, lambda p, nr, ty: ex(p)[:, 0])
+    rr, tt = np.sqrt(rng.uniform(1.5 ** 2, 3.0 ** 2, 24)), rng.uniform(0, 2 * np.pi, 24)
+    return el, ex, np.stack([rr * np.cos(tt), rr * np.sin(tt)], 1).astype(np.float32)
+
+
+def test_hcv_exterior_double_layer_zero_charge(oracle):
+    """Exterior Dirichlet with the double layer (the reference's exterior Dirichlet solver, path
+    length 6): a double-layer potential decays at infinity, so the exterior fit keeps
+    sum_j w_j = 0 (and c0 = 0).  The estimate's error against the exact solution falls at the
+    O(h^2) rate of the interpolated data (300 -> 1200 segments) and the variance falls by
+    more than eight orders of magnitude."""
+    solver = dict(bie_model="IndirectDoubleLayer", domain="ExteriorDomain", estimator="BackwardEstimator",
+                  problem="DirichletProblem", path_length=6, eps=1e-5)
+    err = {}
+    for nseg in (300, 1200):
+        el, ex, pts = _star_dirichlet_exterior(nseg)
+        sc = oracle.OracleScene(el)
+        rec = sc.hcv_targets(8192, seed=5, volume_samples=0)
+        src = hcv_sources(el, 64, 0.75, exterior=True)
+        _, wt = hcv_fit_numpy(rec, src, constant=False, zero_charge=True)
+        assert abs(wt.sum()) < 1e-12 * np.abs(wt).sum()
+        sc.set_harmonic_cv(np.concatenate([src, wt[:, None]], 1), 0.0)
+        m1, se1 = _replicates(sc, dict(solver, flags=16), pts, 8, 128)
+        err[nseg] = np.sqrt(np.mean((m1 - ex(pts)) ** 2, axis=0))
+    assert np.all(err[1200] < err[300] / 6), err
+    m0, se0 = _replicates(sc, dict(solver, flags=0), pts, 8, 128)
+    assert np.mean(se0[:, 0] ** 2) / np.mean(se1[:, 0] ** 2) > 1e8

@@ -178,17 +178,20 @@ class SceneHost:
         self._has_volume = len(w) > 0
 
     @staticmethod
-    def _hcv_desc(n_sources, radius_scale, n_train, volume_samples, eps, ridge, seed, stream_id, exterior=False):
+    def _hcv_desc(n_sources, radius_scale, n_train, volume_samples, eps, ridge, seed, stream_id, exterior=False,
+                  zero_charge=False):
+        flags = (_capi.HCV_EXTERIOR if exterior else 0) | (_capi.HCV_ZERO_CHARGE if zero_charge else 0)
         return _capi.HcvDesc(int(n_sources), float(radius_scale), int(n_train), int(volume_samples), float(eps),
-                             float(ridge), int(seed), int(stream_id), _capi.HCV_EXTERIOR if exterior else 0)
+                             float(ridge), int(seed), int(stream_id), flags)
 
     def fit_harmonic_cv(self, n_sources=64, radius_scale=None, n_train=16384, volume_samples=16, eps=1e-4,
-                        ridge=1e-10, seed=0, stream_id=0, exterior=False, stream=None):
+                        ridge=1e-10, seed=0, stream_id=0, exterior=False, zero_charge=False, stream=None):
         """Fit and bind the harmonic control variate u_h = c0 + sum_j w_j phi(|x - s_j|^2) of
         WOB_EST_HARMONIC_CV walks (wob_scene_fit_harmonic_cv) to the walk's known-term data
         at n_train uniform boundary samples; volume_samples is ignored without a volume
         source.  exterior=True fits the exterior domain's (sources inside the body at
-        radius_scale x the smallest centre-to-element distance, default 0.75; c0 = 0); the
+        radius_scale x the smallest centre-to-element distance, default 0.75; c0 = 0), and
+        zero_charge=True with it keeps sum_j w_j = 0 (exterior double-layer walks need it); the
         interior fit's sources lie at radius_scale x the bounding radius (default 1.5).
         Returns the RMS residual of the fit."""
         _torch()
@@ -196,7 +199,8 @@ class SceneHost:
             volume_samples = 0
         if radius_scale is None:
             radius_scale = 0.75 if exterior else 1.5
-        d = self._hcv_desc(n_sources, radius_scale, n_train, volume_samples, eps, ridge, seed, stream_id, exterior)
+        d = self._hcv_desc(n_sources, radius_scale, n_train, volume_samples, eps, ridge, seed, stream_id, exterior,
+                           zero_charge)
         rms = C.c_float()
         check(lib().wob_scene_fit_harmonic_cv(self._h, C.byref(d), C.byref(rms), _stream_ptr(stream)))
         return rms.value

@@ -59,6 +59,7 @@ class HcvDesc(C.Structure):
 
 
 HCV_EXTERIOR = 1
+HCV_ZERO_CHARGE = 2
 EST_EXCLUDE_ORIGIN_ELEMENT = 1
 EST_CONTROL_VARIATE = 2
 EST_BOUNDARY_DENSITY = 4

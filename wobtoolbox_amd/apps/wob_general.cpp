@@ -26,8 +26,9 @@
 // first pass and walks the residual (WOB_EST_HARMONIC_CV).  Interior domains: sources on a
 // sphere or circle of RADIUS_SCALE x the scene radius (default 2); exterior domains: inside
 // the body at RADIUS_SCALE x the smallest centre-to-element distance (default 0.75).  The fit's
-// time counts in the "seconds" column.  It needs the backward estimator and the single-layer
-// model, or the double-layer model in an interior domain.
+// time counts in the "seconds" column.  It needs the backward estimator and the single- or
+// double-layer model; exterior double-layer runs fit with sum_j w_j = 0 (WOB_HCV_ZERO_CHARGE:
+// a double-layer potential decays at infinity).
 // The 3D Robin elements keep the apps' robin alpha (1, 1, 0): the aggregate initialisation at
 // general_3d.cu:279-284 leaves the third vertex's alpha at 0.
 #include <hip/hip_runtime_api.h>
@@ -190,10 +191,9 @@ int run_app(int argc, char **argv) {
     const auto est = wob::string_to_estimator_type(cfg.str("estimator"));
     const auto problem = wob::string_to_problem_type(cfg.str("problem"));
     if (hcv_sources > 0 && (est != wob::BackwardEstimator ||
-                            (bie != wob::IndirectSingleLayer &&
-                             (bie != wob::IndirectDoubleLayer || domain != wob::InteriorDomain))))
-        throw std::runtime_error("--harmonic-cv needs BackwardEstimator and IndirectSingleLayer, or "
-                                 "IndirectDoubleLayer in the InteriorDomain");
+                            (bie != wob::IndirectSingleLayer && bie != wob::IndirectDoubleLayer)))
+        throw std::runtime_error("--harmonic-cv needs BackwardEstimator and IndirectSingleLayer or "
+                                 "IndirectDoubleLayer");
     const bool hcv_exterior = domain == wob::ExteriorDomain;
     if (hcv_radius == 0.0f) hcv_radius = hcv_exterior ? 0.75f : 2.0f;
 
@@ -325,7 +325,7 @@ int run_app(int argc, char **argv) {
         HIP_OK(hipDeviceSynchronize());
         const auto t0 = std::chrono::steady_clock::now();
         const float rms = scene.fit_harmonic_cv(hcv_sources, hcv_radius, 32768, 0, eps, 1e-10f, seed, nullptr,
-                                                hcv_exterior);
+                                                hcv_exterior, hcv_exterior && bie == wob::IndirectDoubleLayer);
         HIP_OK(hipDeviceSynchronize());
         elapsed += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         std::cout << "harmonic control variate\t: " << hcv_sources << " sources at " << hcv_radius

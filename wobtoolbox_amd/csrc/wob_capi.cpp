@@ -177,20 +177,24 @@ wob_status check_est(const wob_scene *scene, const wob_estimator_desc *est, int 
             return fail(WOB_INVALID_ARG, "no control-variate table with path_length value functions is bound to the scene");
     }
     if (est->flags & WOB_EST_HARMONIC_CV) {
-        if (est->bie_model == WOB_DIRECT || est->estimator != WOB_BACKWARD ||
-            (est->domain != WOB_INTERIOR && est->bie_model != WOB_SINGLE_LAYER))
+        if (est->bie_model == WOB_DIRECT || est->estimator != WOB_BACKWARD)
             return fail(WOB_UNSUPPORTED_COMBO, "the harmonic control variate needs the backward estimator and the "
-                                               "single-layer model, or the double-layer model in an interior domain");
+                                               "single-layer or double-layer model");
         if (scene->dev.hcv_m == 0)
             return fail(WOB_INVALID_ARG, "no harmonic control variate is bound to the scene "
                                          "(wob_scene_fit_harmonic_cv / wob_scene_set_harmonic_cv)");
-        const int want = est->domain == WOB_INTERIOR ? wob_hcv_interior : wob_hcv_exterior;
-        if (scene->hcv_side != wob_hcv_any && scene->hcv_side != want)
-            return fail(WOB_INVALID_ARG, est->domain == WOB_INTERIOR
-                                             ? "the scene's harmonic control variate was fitted for the exterior "
-                                               "domain (WOB_HCV_EXTERIOR); interior walks need an interior fit"
-                                             : "the scene's harmonic control variate was fitted for the interior "
-                                               "domain; exterior walks need a WOB_HCV_EXTERIOR fit");
+        const int side = scene->hcv_side;
+        if (side != wob_hcv_any) {
+            if (est->domain == WOB_INTERIOR && side != wob_hcv_interior)
+                return fail(WOB_INVALID_ARG, "the scene's harmonic control variate was fitted for the exterior domain "
+                                             "(WOB_HCV_EXTERIOR); interior walks need an interior fit");
+            if (est->domain != WOB_INTERIOR && side == wob_hcv_interior)
+                return fail(WOB_INVALID_ARG, "the scene's harmonic control variate was fitted for the interior domain; "
+                                             "exterior walks need a WOB_HCV_EXTERIOR fit");
+            if (est->domain != WOB_INTERIOR && est->bie_model == WOB_DOUBLE_LAYER && side != wob_hcv_exterior_zero_charge)
+                return fail(WOB_INVALID_ARG, "exterior double-layer walks need a WOB_HCV_EXTERIOR | WOB_HCV_ZERO_CHARGE "
+                                             "fit (a double-layer potential decays at infinity)");
+        }
     }
     if ((est->flags & WOB_EST_BOUNDARY_DENSITY) && est->estimator != WOB_BACKWARD)
         return fail(WOB_UNSUPPORTED_COMBO, "boundary density estimates need the backward estimator");

@@ -27,5 +27,6 @@ const wobdev::DevScene &wob_scene_dev(const wob_scene *scene);
 // (d_src == nullptr) unbinds
 // side of the domain a bound harmonic control variate serves: fitted for the interior, fitted
 // for the exterior, or given explicitly (wob_scene_set_harmonic_cv: either)
-enum { wob_hcv_interior = 0, wob_hcv_exterior = 1, wob_hcv_any = 2 };
+// (an exterior fit with sum_j w_j = 0 serves exterior double-layer walks too)
+enum { wob_hcv_interior = 0, wob_hcv_exterior = 1, wob_hcv_any = 2, wob_hcv_exterior_zero_charge = 3 };
 wob_status wob_scene_bind_hcv(wob_scene *scene, float4 *d_src, uint32_t m, float c0, int side);

@@ -28,7 +28,12 @@ inline wob_status fail(wob_status s, const std::string &msg) { return wob_fail(s
 wob_status check_desc(const wob_scene *scene, const wob_hcv_desc *d) {
     if (!scene || !d) return fail(WOB_INVALID_ARG, "null scene or harmonic control-variate descriptor");
     if (d->n_sources == 0 || d->n_sources > kMaxSources) return fail(WOB_INVALID_ARG, "n_sources must be 1 .. 1024");
-    if (d->flags & ~WOB_HCV_EXTERIOR) return fail(WOB_INVALID_ARG, "unknown harmonic control-variate flags");
+    if (d->flags & ~(WOB_HCV_EXTERIOR | WOB_HCV_ZERO_CHARGE))
+        return fail(WOB_INVALID_ARG, "unknown harmonic control-variate flags");
+    if ((d->flags & WOB_HCV_ZERO_CHARGE) && !(d->flags & WOB_HCV_EXTERIOR))
+        return fail(WOB_INVALID_ARG, "WOB_HCV_ZERO_CHARGE needs WOB_HCV_EXTERIOR");
+    if ((d->flags & WOB_HCV_ZERO_CHARGE) && d->n_sources < 2)
+        return fail(WOB_INVALID_ARG, "a zero-charge fit needs n_sources >= 2");
     if ((d->flags & WOB_HCV_EXTERIOR) ? !(d->radius_scale > 0.f && d->radius_scale < 1.f) : !(d->radius_scale > 1.f))
         return fail(WOB_INVALID_ARG, (d->flags & WOB_HCV_EXTERIOR) ? "radius_scale must be in (0, 1) for an exterior fit"
                                                                     : "radius_scale must be > 1");
@@ -226,18 +231,31 @@ wob_status wob_scene_fit_harmonic_cv(wob_scene *scene, const wob_hcv_desc *d, fl
     if (e != hipSuccess) { release(); return wob_hip_fail(e, "harmonic control-variate Gram matrix"); }
 
     // normal equations with the columns scaled to unit RMS, and a ridge relative to the
-    // mean diagonal; an exterior fit leaves out column 0 (c0 = 0)
-    const uint32_t a0 = exterior ? 1u : 0u, P1 = P - a0;
+    // mean diagonal.  An exterior fit leaves out column 0 (c0 = 0); a zero-charge fit solves for
+    // w_0 .. w_{m-2} with w_{m-1} = -(w_0 + .. + w_{m-2}), i.e. on the columns b_j - b_{m-1}:
+    // Gram entries G_ij - G_i,L - G_L,j + G_LL and targets G_iT - G_LT (L the last source).
+    const bool zero = (d->flags & WOB_HCV_ZERO_CHARGE) != 0;
+    const uint32_t a0 = exterior ? 1u : 0u, P1 = P - a0 - (zero ? 1u : 0u), L = P - 1;
+    auto gram = [&](uint32_t i, uint32_t j) {   // unknowns i, j (Gram rows a0 + i, a0 + j)
+        const uint32_t gi = a0 + i, gj = a0 + j;
+        double v = G[(size_t)gi * Q + gj];
+        if (zero) v += -G[(size_t)gi * Q + L] - G[(size_t)L * Q + gj] + G[(size_t)L * Q + L];
+        return v;
+    };
+    auto rhs = [&](uint32_t i) {
+        double v = G[(size_t)(a0 + i) * Q + P];
+        if (zero) v -= G[(size_t)L * Q + P];
+        return v;
+    };
     std::vector<double> A((size_t)P1 * P1), rr(P1), scale(P1), r(P, 0.0);
     for (uint32_t a = 0; a < P1; ++a) {
-        const double g = G[(size_t)(a + a0) * Q + a + a0];
+        const double g = gram(a, a);
         scale[a] = g > 0 ? 1.0 / std::sqrt(g / n) : 1.0;
     }
     double tr = 0;
     for (uint32_t a = 0; a < P1; ++a) {
-        for (uint32_t b = 0; b < P1; ++b)
-            A[(size_t)a * P1 + b] = G[(size_t)(a + a0) * Q + b + a0] * scale[a] * scale[b];
-        rr[a] = G[(size_t)(a + a0) * Q + P] * scale[a];
+        for (uint32_t b = 0; b < P1; ++b) A[(size_t)a * P1 + b] = gram(a, b) * scale[a] * scale[b];
+        rr[a] = rhs(a) * scale[a];
         tr += A[(size_t)a * P1 + a];
     }
     const double lam = (double)d->ridge * tr / P1;
@@ -248,6 +266,11 @@ wob_status wob_scene_fit_harmonic_cv(wob_scene *scene, const wob_hcv_desc *d, fl
                                      "samples hold no Dirichlet or Robin datum to fix the constant)");
     }
     for (uint32_t a = 0; a < P1; ++a) r[a + a0] = rr[a] * scale[a];
+    if (zero) {
+        double sum = 0;
+        for (uint32_t a = 0; a < P1; ++a) sum += r[a + a0];
+        r[L] = -sum;
+    }
 
     // device records (FP32), bound to the scene
     std::vector<float> out((size_t)m * 4, 0.f);
@@ -255,7 +278,8 @@ wob_status wob_scene_fit_harmonic_cv(wob_scene *scene, const wob_hcv_desc *d, fl
         for (int k = 0; k < D; ++k) out[(size_t)j * 4 + k] = (float)src[(size_t)j * D + k];
         out[(size_t)j * 4 + D] = (float)r[1 + j];
     }
-    st = bind_sources(scene, out, m, (float)r[0], exterior ? wob_hcv_exterior : wob_hcv_interior);
+    st = bind_sources(scene, out, m, (float)r[0],
+                      !exterior ? wob_hcv_interior : zero ? wob_hcv_exterior_zero_charge : wob_hcv_exterior);
     if (st != WOB_OK) { release(); return st; }
     if (rms_residual) {   // the residual datum of every sample as the walk computes it
         std::vector<float> res(n);
