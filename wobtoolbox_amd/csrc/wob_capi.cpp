@@ -56,10 +56,27 @@ struct wob_scene {
     size_t aux_bytes = 0;
     char *vpl = nullptr;    // VPL aggregation records and partial sums (apart from the walks' buffers)
     size_t vpl_bytes = 0;
+    char *hcv_work = nullptr;   // harmonic control-variate fits: targets, Gram matrix, partial sums
+    size_t hcv_work_bytes = 0;
 };
 
 const wobhost::HostScene &wob_scene_host(const wob_scene *scene) { return scene->host; }
 const wobdev::DevScene &wob_scene_dev(const wob_scene *scene) { return scene->dev; }
+
+wob_status wob_scene_hcv_workspace(wob_scene *s, size_t bytes, char **out) {
+    if (bytes > s->hcv_work_bytes) {
+        if (s->hcv_work) WOB_HIP(hipFree(s->hcv_work), "hipFree fit workspace");
+        s->hcv_work = nullptr;
+        s->hcv_work_bytes = 0;
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) return fail(WOB_OOM, std::string("hipMalloc fit workspace: ") + hipGetErrorString(e));
+        s->hcv_work = (char *)p;
+        s->hcv_work_bytes = bytes;
+    }
+    *out = s->hcv_work;
+    return WOB_OK;
+}
 
 wob_status wob_scene_bind_hcv(wob_scene *scene, float4 *d_src, uint32_t m, float c0, int side) {
     if (scene->hcv_src) (void)hipFree(scene->hcv_src);
@@ -118,6 +135,8 @@ void free_scene(wob_scene *s) {
     s->aux = nullptr;
     if (s->vpl) (void)hipFree(s->vpl);
     s->vpl = nullptr;
+    if (s->hcv_work) (void)hipFree(s->hcv_work);
+    s->hcv_work = nullptr;
 }
 
 wob_status ensure_scratch(wob_scene *s, size_t bytes) {

@@ -213,8 +213,9 @@ wob_status wob_scene_fit_harmonic_cv(wob_scene *scene, const wob_hcv_desc *d, fl
                  g_bytes = ((size_t)Q * Q * sizeof(double) + 255) & ~(size_t)255,
                  res_bytes = ((size_t)n * sizeof(float) + 255) & ~(size_t)255;
     char *buf = nullptr;
-    WOB_HIP(hipMalloc((void **)&buf, rec_bytes + src_bytes + g_bytes + res_bytes + wobl::hcv_gram_scratch(n, m)),
-            "hipMalloc harmonic control-variate fit");
+    if ((st = wob_scene_hcv_workspace(scene, rec_bytes + src_bytes + g_bytes + res_bytes + wobl::hcv_gram_scratch(n, m),
+                                      &buf)) != WOB_OK)
+        return st;
     float *d_rec = (float *)buf;
     double *d_src = (double *)(buf + rec_bytes), *d_G = (double *)(buf + rec_bytes + src_bytes);
     float *d_res = (float *)(buf + rec_bytes + src_bytes + g_bytes);
@@ -226,9 +227,8 @@ wob_status wob_scene_fit_harmonic_cv(wob_scene *scene, const wob_hcv_desc *d, fl
         if (e == hipSuccess) e = hipMemcpyAsync(G.data(), d_G, G.size() * sizeof(double), hipMemcpyDeviceToHost, hs);
         if (e == hipSuccess) e = hipStreamSynchronize(hs);
     }
-    auto release = [&]() { (void)hipFree(buf); };
-    if (st != WOB_OK) { release(); return st; }
-    if (e != hipSuccess) { release(); return wob_hip_fail(e, "harmonic control-variate Gram matrix"); }
+    if (st != WOB_OK) { return st; }
+    if (e != hipSuccess) { return wob_hip_fail(e, "harmonic control-variate Gram matrix"); }
 
     // normal equations with the columns scaled to unit RMS, and a ridge relative to the
     // mean diagonal.  An exterior fit leaves out column 0 (c0 = 0); a zero-charge fit solves for
@@ -261,7 +261,6 @@ wob_status wob_scene_fit_harmonic_cv(wob_scene *scene, const wob_hcv_desc *d, fl
     const double lam = (double)d->ridge * tr / P1;
     for (uint32_t a = 0; a < P1; ++a) A[(size_t)a * P1 + a] += lam;
     if (!cholesky_solve(A, rr, (int)P1)) {
-        release();
         return fail(WOB_INVALID_ARG, "harmonic control-variate fit: singular normal equations (raise ridge, or the "
                                      "samples hold no Dirichlet or Robin datum to fix the constant)");
     }
@@ -280,18 +279,17 @@ wob_status wob_scene_fit_harmonic_cv(wob_scene *scene, const wob_hcv_desc *d, fl
     }
     st = bind_sources(scene, out, m, (float)r[0],
                       !exterior ? wob_hcv_interior : zero ? wob_hcv_exterior_zero_charge : wob_hcv_exterior);
-    if (st != WOB_OK) { release(); return st; }
+    if (st != WOB_OK) { return st; }
     if (rms_residual) {   // the residual datum of every sample as the walk computes it
         std::vector<float> res(n);
         e = wobl::launch_hcv_residual(D, wob_scene_dev(scene), d_rec, n, d_res, hs);
         if (e == hipSuccess) e = hipMemcpyAsync(res.data(), d_res, n * sizeof(float), hipMemcpyDeviceToHost, hs);
         if (e == hipSuccess) e = hipStreamSynchronize(hs);
-        if (e != hipSuccess) { release(); return wob_hip_fail(e, "harmonic control-variate residual"); }
+        if (e != hipSuccess) { return wob_hip_fail(e, "harmonic control-variate residual"); }
         double ss = 0;
         for (uint32_t i = 0; i < n; ++i) ss += (double)res[i] * res[i];
         *rms_residual = (float)std::sqrt(ss / n);
     }
-    release();
     return WOB_OK;
 }
 
