@@ -165,14 +165,17 @@ enum { WOB_EST_CONTROL_VARIATE = 2, WOB_EST_BOUNDARY_DENSITY = 4 };
  *   uses its own sample.  The reference draws one sample per term and adds it
  *   num_volume_samples times (bie_models.cuh:85-101), which leaves the variance unchanged.   */
 enum { WOB_EST_INDEPENDENT_VOLUME_SAMPLES = 8 };
-/* WOB_EST_HARMONIC_CV (single-layer or double-layer model, backward estimator, interior
- *   domain): use the scene's harmonic function u_h (wob_scene_fit_harmonic_cv).  Every
- *   boundary datum the walk reads is replaced by the residual's: g - u_h at Dirichlet points,
- *   g - du_h/dn at Neumann points, g - (du_h/dn + alpha u_h) at Robin points; each domain
- *   estimate adds u_h(x) and grad u_h(x).  u - u_h solves the same problem with the residual
- *   data, so the expectation is that of plain WoB for any u_h harmonic in the domain (the
- *   path-length truncation then applies to the residual's series).  Combines with
- *   WOB_EST_CONTROL_VARIATE (a value table of the residual walk).                          */
+/* WOB_EST_HARMONIC_CV (single-layer or double-layer model, backward or forward estimator,
+ *   domain estimates or boundary densities; the fit must serve the domain's side, see
+ *   wob_scene_fit_harmonic_cv): use the scene's harmonic function u_h.  Every boundary datum
+ *   the walk reads is replaced by the residual's: g - u_h at Dirichlet points, g - du_h/dn at
+ *   Neumann points, g - (du_h/dn + alpha u_h) at Robin points; each domain estimate adds
+ *   u_h(x) and grad u_h(x).  u - u_h solves the same problem with the residual data, so the
+ *   expectation is that of plain WoB for any u_h harmonic in the domain (of the layer
+ *   potential's class at infinity in an exterior domain; the path-length truncation then
+ *   applies to the residual's series).  The forward estimator then draws its start point
+ *   uniformly rather than by the data's density (the residual data need not vanish where the
+ *   data do).  Combines with WOB_EST_CONTROL_VARIATE (a value table of the residual walk).  */
 enum { WOB_EST_HARMONIC_CV = 16 };
 
 /* Launch tuning and timing (in/out; may be NULL).  paths_per_lane: paths one lane
@@ -383,8 +386,9 @@ wob_status wob_scene_set_control_variate(wob_scene *scene, wob_cv *cv, const flo
                                          void *stream);
 
 /* Learned harmonic control variate (no counterpart in the reference) for single-layer and
- * double-layer walks of interior and exterior domains (WOB_EST_HARMONIC_CV), e.g. the Poisson problem's double-layer estimator
- * (poisson_equation.cu:40-61), the mixed single-layer workload and the batched exterior scenes:
+ * double-layer walks of interior and exterior domains (WOB_EST_HARMONIC_CV), e.g. the Poisson
+ * problem's double-layer estimator (poisson_equation.cu:40-61), the mixed single-layer
+ * workload and the batched exterior scenes:
  *   u_h(x) = c0 + sum_j w_j phi(|x - s_j|^2),  phi(z) = z^(-1/2) (3D) or log2 z (2D),
  * a combination of fundamental solutions with sources s_j on a sphere (circle) outside the
  * scene (interior fits), so u_h is harmonic in the interior domain, or inside the body

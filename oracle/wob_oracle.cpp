@@ -1110,14 +1110,16 @@ template <int D> struct Estimator {
             }
             return;
         }
-        // forward :405-454
+        // forward :405-454.  With the harmonic control variate the start point is uniform
+        // (the residual data need not vanish where the data do, so the data's own density
+        // would miss part of them) and the known term reads the residual datum.
         Sampled<D> y = sample_uniform(*s, rng);
         out[0] = m.solution_domain_known(x, y.y, y.inv_pdf, rng);
         Vec<D> g = m.gradient_domain_known(x, y.y, y.inv_pdf, rng);
         const bool sl_neumann = m.bie == IndirectSingleLayer && problem == NeumannProblem;
-        y = sl_neumann ? sample_boundary_value(*s, rng) : sample_uniform(*s, rng);
+        y = sl_neumann && !opt.harmonic_cv ? sample_boundary_value(*s, rng) : sample_uniform(*s, rng);
         Sampled<D> y2 = sample_uniform(*s, rng);
-        float W = y.inv_pdf * m.fredholm_known(y.y, y2.y, y2.inv_pdf, rng);
+        float W = y.inv_pdf * m.fredholm_known(opt.harmonic_cv ? hcv_residual(*s, y.y) : y.y, y2.y, y2.inv_pdf, rng);
         BPoint<D> yc = y.y;
         for (unsigned it = 0; it < L; ++it) {
             if (it == L - 1) W *= 0.5f;
@@ -1130,6 +1132,12 @@ template <int D> struct Estimator {
             W = m.fredholm_unknown(yc, yp, q.inv_pdf, rng, false) * W;
         }
         for (int k = 0; k < D; ++k) out[1 + k] = g[k];
+        if (opt.harmonic_cv) {   // u = u_h + the walk's estimate of u - u_h
+            float gh[3];
+            hcv_gradient(*s, x.v, gh);
+            out[0] += hcv_value(*s, x.v);
+            for (int k = 0; k < D; ++k) out[1 + k] += gh[k];
+        }
     }
 
     float boundary_path(const BPoint<D> &x, Rng &rng) const {

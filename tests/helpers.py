@@ -206,7 +206,8 @@ def hcv_fit_numpy(rec, src, ridge=1e-10, constant=True, zero_charge=False):
         assert not constant
         B = B[:, :-1] - B[:, -1:]
     t = np.asarray(rec, np.float64)[:, 2 * src.shape[1] + 1]
-    sc = 1 / np.sqrt((B ** 2).mean(0))
+    ms = (B ** 2).mean(0)
+    sc = np.where(ms > 0, 1 / np.sqrt(np.where(ms > 0, ms, 1)), 1.0)   # an all-zero column keeps scale 1
     Bs = B * sc
     A = Bs.T @ Bs
     A += ridge * np.trace(A) / len(A) * np.eye(len(A))

@@ -1719,14 +1719,17 @@ __device__ __forceinline__ void domain_path(const DevScene &S, const Model<D, BI
             }
         }
     } else {                                                                                   // estimator.cuh:405-454
+        // With the harmonic control variate the start point is uniform (the residual data need
+        // not vanish where the data do, so the data's own density would miss part of them) and
+        // the known term reads the residual datum.
         Sample<D> y = sample_uniform<D>(S, rng);
         out[0] = m.solution_domain_known(x, y.y, y.inv_pdf, vt, rng);
         float g[D], gu[D];
         m.gradient_domain_known(x, y.y, y.inv_pdf, vt, rng, g);
-        if constexpr (FL == FL_FWD_LINE) y = sample_bv<D>(S, rng);
+        if (FL == FL_FWD_LINE && !wp.hcv) y = sample_bv<D>(S, rng);
         else y = sample_uniform<D>(S, rng);
         const Sample<D> y2 = sample_uniform<D>(S, rng);
-        float W = y.inv_pdf * m.fredholm_known(y.y, y2.y, y2.inv_pdf, vt, rng);
+        float W = y.inv_pdf * m.fredholm_known(wp.hcv ? hcv_residual<D>(S, y.y) : y.y, y2.y, y2.inv_pdf, vt, rng);
         BP<D> yc = y.y;
         for (uint32_t it = 0; it < wp.path_length; ++it) {
             if (it == wp.path_length - 1) W *= 0.5f;
@@ -1743,6 +1746,13 @@ __device__ __forceinline__ void domain_path(const DevScene &S, const Model<D, BI
         }
 #pragma unroll
         for (int k = 0; k < D; ++k) out[1 + k] = g[k];
+        if (wp.hcv) {   // u = u_h + the walk's estimate of u - u_h
+            float gh[D];
+            hcv_gradient<D>(S, x, gh);
+            out[0] += hcv_value<D>(S, x);
+#pragma unroll
+            for (int k = 0; k < D; ++k) out[1 + k] += gh[k];
+        }
     }
 }
 
