@@ -257,13 +257,30 @@ def icosphere_leg(args, torch, wob, G):
     var = np.maximum((q[:, 0] / k).cpu().numpy() - m * m, 0)
     d = m - w.exact(w.points)[:, 0]
     d -= d.mean()
-    return {"workload": w.name, "elements": w.elements.n, "points": len(w.points), "paths_per_step": len(w.points) * paths,
-            "paths_per_s": value, "ms_per_step": ms,
-            "roofline": roofline_block(w.name, w.elements.n, w.solver["path_length"], len(w.points) * paths, walk_ms),
-            "accuracy": {"rmse_u": float(np.sqrt((d ** 2).mean())), "mean_se_u": float(np.sqrt(var / k).mean()),
-                         "paths_per_point": k, "note": "Neumann problem: u compared after removing the mean offset"},
-            "bvh_width": {"tuned": width, "walk_ms_by_width": width_ms, "tune_s": tune_s},
-            "solver": "Direct, InteriorDomain, BackwardEstimator, NeumannProblem, path_length 6"}
+    out = {"workload": w.name, "elements": w.elements.n, "points": len(w.points), "paths_per_step": len(w.points) * paths,
+           "paths_per_s": value, "ms_per_step": ms,
+           "roofline": roofline_block(w.name, w.elements.n, w.solver["path_length"], len(w.points) * paths, walk_ms),
+           "accuracy": {"rmse_u": float(np.sqrt((d ** 2).mean())), "mean_se_u": float(np.sqrt(var / k).mean()),
+                        "paths_per_point": k, "note": "Neumann problem: u compared after removing the mean offset"},
+           "bvh_width": {"tuned": width, "walk_ms_by_width": width_ms, "tune_s": tune_s},
+           "solver": "Direct, InteriorDomain, BackwardEstimator, NeumannProblem, path_length 6"}
+    # the harmonic control variate on the same walk (fit on the clock, one step of paths)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rms = scene.fit_harmonic_cv(n_sources=64, n_train=16384, seed=7)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    hest = wob.Estimator.from_config(scene, w.solver, flags=HCV)
+    hv, hms, _, hs_, hq_, hk = walk_timing(hest, pts, paths, 1, 1)
+    hm = (hs_[:, 0] / hk).cpu().numpy()
+    hvar = np.maximum((hq_[:, 0] / hk).cpu().numpy() - hm * hm, 0)
+    hd = hm - w.exact(w.points)[:, 0]
+    hd -= hd.mean()
+    out["control_variate"] = {"kind": "harmonic (64 fundamental solutions outside the scene)", "fit_s": fit_s,
+                              "fit_rms_residual": rms, "paths_per_s": hv, "rmse_u": float(np.sqrt((hd ** 2).mean())),
+                              "mean_se_u": float(np.sqrt(hvar / hk).mean()), "paths_per_point": hk,
+                              "per_path_variance_ratio": float(np.mean(var) / max(np.mean(hvar), 1e-300))}
+    return out
 
 
 def batch_leg(args, torch, wob, G):

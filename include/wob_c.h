@@ -165,17 +165,19 @@ enum { WOB_EST_CONTROL_VARIATE = 2, WOB_EST_BOUNDARY_DENSITY = 4 };
  *   uses its own sample.  The reference draws one sample per term and adds it
  *   num_volume_samples times (bie_models.cuh:85-101), which leaves the variance unchanged.   */
 enum { WOB_EST_INDEPENDENT_VOLUME_SAMPLES = 8 };
-/* WOB_EST_HARMONIC_CV (single-layer or double-layer model, backward or forward estimator,
- *   domain estimates or boundary densities; the fit must serve the domain's side, see
- *   wob_scene_fit_harmonic_cv): use the scene's harmonic function u_h.  Every boundary datum
+/* WOB_EST_HARMONIC_CV (single-layer or double-layer model, or the direct model in an
+ *   interior domain; backward or forward estimator; domain estimates or boundary densities;
+ *   the fit must serve the domain's side, see wob_scene_fit_harmonic_cv): use the scene's
+ *   harmonic function u_h.  Every boundary datum
  *   the walk reads is replaced by the residual's: g - u_h at Dirichlet points, g - du_h/dn at
  *   Neumann points, g - (du_h/dn + alpha u_h) at Robin points; each domain estimate adds
  *   u_h(x) and grad u_h(x).  u - u_h solves the same problem with the residual data, so the
  *   expectation is that of plain WoB for any u_h harmonic in the domain (of the layer
  *   potential's class at infinity in an exterior domain; the path-length truncation then
- *   applies to the residual's series).  The forward estimator then draws its start point
- *   uniformly rather than by the data's density (the residual data need not vanish where the
- *   data do).  Combines with WOB_EST_CONTROL_VARIATE (a value table of the residual walk).  */
+ *   applies to the residual's series).  The forward estimator then draws its start point,
+ *   and the direct model its known terms' sample, uniformly rather than by the data's density
+ *   (the residual data need not vanish where the data do).  Combines with
+ *   WOB_EST_CONTROL_VARIATE (a value table of the residual walk).                          */
 enum { WOB_EST_HARMONIC_CV = 16 };
 
 /* Launch tuning and timing (in/out; may be NULL).  paths_per_lane: paths one lane

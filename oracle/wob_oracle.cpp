@@ -1057,9 +1057,12 @@ template <int D> struct Estimator {
                 W *= m.fredholm_unknown(yp, y, inv, rng, true);
                 yp = y;
             } else if (direct_neumann()) {
-                Sampled<D> q = sample_boundary_value(*s, rng);
+                // with the control variate the known term's sample is uniform (the residual data
+                // need not vanish where the data do) and both points carry residual data
+                Sampled<D> q = opt.harmonic_cv ? sample_uniform(*s, rng) : sample_boundary_value(*s, rng);
                 y = q.y; inv = q.inv_pdf;
-                spc += W * m.fredholm_known(yp, y, inv, rng);
+                if (opt.harmonic_cv) spc += W * m.fredholm_known(hcv_residual(*s, yp), hcv_residual(*s, y), inv, rng);
+                else spc += W * m.fredholm_known(yp, y, inv, rng);
                 q = sample_line_intersection(*s, yp.p, rng, opt, yp.elem);
                 y = q.y; inv = q.inv_pdf;
                 W = W * m.fredholm_unknown(yp, y, inv, rng, true);
@@ -1074,7 +1077,8 @@ template <int D> struct Estimator {
             } else {
                 Sampled<D> q = sample_uniform(*s, rng);
                 y = q.y; inv = q.inv_pdf;
-                spc += W * m.fredholm_known(yp, y, inv, rng);
+                if (opt.harmonic_cv) spc += W * m.fredholm_known(hcv_residual(*s, yp), hcv_residual(*s, y), inv, rng);
+                else spc += W * m.fredholm_known(yp, y, inv, rng);
                 q = sample_uniform(*s, rng);
                 y = q.y; inv = q.inv_pdf;
                 W = W * m.fredholm_unknown(yp, y, inv, rng, true);
@@ -1087,7 +1091,10 @@ template <int D> struct Estimator {
 
     void first_samples(const Vec<D> &x, Sampled<D> &y, Sampled<D> &y1, Rng &rng, int origin_elem = -1) const {
         if (dl_dirichlet()) { y = sample_line_intersection(*s, x, rng, opt, origin_elem); y1 = y; }
-        else if (direct_neumann()) { y = sample_line_intersection(*s, x, rng, opt, origin_elem); y1 = sample_boundary_value(*s, rng); }
+        else if (direct_neumann()) {
+            y = sample_line_intersection(*s, x, rng, opt, origin_elem);
+            y1 = opt.harmonic_cv ? sample_uniform(*s, rng) : sample_boundary_value(*s, rng);
+        }
         else { y = sample_uniform(*s, rng); y1 = sample_uniform(*s, rng); }
     }
 
@@ -1096,13 +1103,14 @@ template <int D> struct Estimator {
         if (est == BackwardEstimator) {                                                             // :154-191
             Sampled<D> y, y1;
             first_samples(x, y, y1, rng);
+            if (opt.harmonic_cv && m.bie == Direct) y1.y = hcv_residual(*s, y1.y);   // the known terms' datum
             out[0] = m.solution_domain_known(x, y1.y, y1.inv_pdf, rng);
             Vec<D> g = m.gradient_domain_known(x, y1.y, y1.inv_pdf, rng);
             const float spc = walk(y.y, rng);
             out[0] += spc * m.solution_domain_unknown(x, y.y, y.inv_pdf);
             const Vec<D> gu = m.gradient_domain_unknown(x, y.y, y.inv_pdf);
             for (int k = 0; k < D; ++k) out[1 + k] = g[k] + spc * gu[k];
-            if (opt.harmonic_cv && m.bie != Direct) {   // u = u_h + the walk's estimate of u - u_h
+            if (opt.harmonic_cv) {   // u = u_h + the walk's estimate of u - u_h
                 float gh[3];
                 hcv_gradient(*s, x.v, gh);
                 out[0] += hcv_value(*s, x.v);

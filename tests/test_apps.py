@@ -149,10 +149,10 @@ def test_app_3d_progressive_run(tmp_path, gpu):
 
 def test_app_harmonic_cv_option_checks_the_solver(tmp_path):
     """--harmonic-cv is refused, before any GPU work, for the solver the control variate does not
-    cover (the direct model)."""
+    cover (the direct model in an exterior domain)."""
     obj = tmp_path / "sphere.obj"
     _write_sphere_obj(obj)
-    for kw in (dict(bie_model="Direct"), dict(bie_model="Direct", domain="ExteriorDomain")):
+    for kw in (dict(bie_model="Direct", domain="ExteriorDomain", problem="NeumannProblem"),):
         cfg = _config(tmp_path, obj, **kw)
         r = subprocess.run([B.APPS[3], str(cfg), "--harmonic-cv", "16", "--dry-run"], capture_output=True, text=True,
                            timeout=60)

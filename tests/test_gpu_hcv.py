@@ -557,3 +557,40 @@ def test_hcv_forward_walk_matches_oracle(gpu, oracle):
             assert frac < 0.03 and z.max() < 5.0
     _say(f"forward Neumann: gradient variance plain {var[0]:.3e}, with the control variate {var[HCV]:.3e}")
     assert var[0] / var[HCV] > 100
+
+
+def test_hcv_direct_model_walk_matches_oracle(gpu, oracle):
+    """The direct model with the control variate (the line walk of the direct Neumann
+    workload, and the uniform flavour of mixed data): path by path against the oracle in the
+    per-lane and the wavefront schedules, and far less variance than plain WoB."""
+    import wobtoolbox_amd as wob
+    for kind, problem, exact in (("neumann", "NeumannProblem", G.exact_quadratic_3d),
+                                 ("mixed", "MixedProblem", G.exact_expcos_xz_3d)):
+        el, _, _ = small_icosphere(3, kind=kind)
+        scene, osc = _scenes(el, None, oracle)
+        scene.fit_harmonic_cv(n_sources=64, n_train=16384, seed=5)
+        rec, c0 = scene.harmonic_cv()
+        osc.set_harmonic_cv(rec, c0)
+        pts = G.interior_points_star(64, np.random.default_rng(7), margin=0.05).astype(np.float32)
+        solver = dict(bie_model="Direct", domain="InteriorDomain", estimator="BackwardEstimator", problem=problem,
+                      path_length=6, eps=1e-5)
+        paths = 128
+        var = {}
+        for flags in (0, HCV):
+            est = wob.Estimator.from_config(scene, dict(solver, flags=flags))
+            s, q = est.compute_domain(_dev(pts, gpu), paths, seed=11)
+            m, se = mean_se(s.cpu().numpy(), q.cpu().numpy(), paths)
+            var[flags] = float(np.mean(se ** 2))
+            if flags:
+                so, qo = osc.estimate_domain(dict(solver, flags=flags), pts, paths, seed=11, pick_mode=1)
+                mo, _ = mean_se(so, qo, paths)
+                scale = np.abs(mo).max(0) + 1e-6
+                frac = (np.abs(m - mo) > 1e-4 * scale).mean()
+                _say(f"direct {problem} control-variate walk: points over 1e-4 vs oracle {frac:.3f}")
+                assert frac < 0.05, frac
+                if problem == "NeumannProblem":   # the line walk also runs in the wavefront schedule
+                    est.schedule = 2
+                    sw, qw = est.compute_domain(_dev(pts, gpu), paths, seed=11)
+                    assert np.array_equal(s.cpu().numpy(), sw.cpu().numpy())
+        _say(f"direct {problem}: variance plain {var[0]:.3e}, with the control variate {var[HCV]:.3e}")
+        assert var[0] / var[HCV] > 100

@@ -26,9 +26,9 @@
 // first pass and walks the residual (WOB_EST_HARMONIC_CV).  Interior domains: sources on a
 // sphere or circle of RADIUS_SCALE x the scene radius (default 2); exterior domains: inside
 // the body at RADIUS_SCALE x the smallest centre-to-element distance (default 0.75).  The fit's
-// time counts in the "seconds" column.  It needs the single- or double-layer model; exterior
-// double-layer runs fit with sum_j w_j = 0 (WOB_HCV_ZERO_CHARGE: a double-layer potential
-// decays at infinity).
+// time counts in the "seconds" column.  It takes every model and estimator but the direct
+// model in an exterior domain; exterior double-layer runs fit with sum_j w_j = 0
+// (WOB_HCV_ZERO_CHARGE: a double-layer potential decays at infinity).
 // The 3D Robin elements keep the apps' robin alpha (1, 1, 0): the aggregate initialisation at
 // general_3d.cu:279-284 leaves the third vertex's alpha at 0.
 #include <hip/hip_runtime_api.h>
@@ -190,8 +190,8 @@ int run_app(int argc, char **argv) {
     const auto domain = wob::string_to_domain_type(cfg.str("domain"));
     const auto est = wob::string_to_estimator_type(cfg.str("estimator"));
     const auto problem = wob::string_to_problem_type(cfg.str("problem"));
-    if (hcv_sources > 0 && bie != wob::IndirectSingleLayer && bie != wob::IndirectDoubleLayer)
-        throw std::runtime_error("--harmonic-cv needs IndirectSingleLayer or IndirectDoubleLayer");
+    if (hcv_sources > 0 && bie == wob::Direct && domain != wob::InteriorDomain)
+        throw std::runtime_error("--harmonic-cv needs the InteriorDomain with the Direct model");
     const bool hcv_exterior = domain == wob::ExteriorDomain;
     if (hcv_radius == 0.0f) hcv_radius = hcv_exterior ? 0.75f : 2.0f;
 

@@ -196,9 +196,9 @@ wob_status check_est(const wob_scene *scene, const wob_estimator_desc *est, int 
             return fail(WOB_INVALID_ARG, "no control-variate table with path_length value functions is bound to the scene");
     }
     if (est->flags & WOB_EST_HARMONIC_CV) {
-        if (est->bie_model == WOB_DIRECT)
-            return fail(WOB_UNSUPPORTED_COMBO, "the harmonic control variate needs the single-layer or double-layer "
-                                               "model");
+        if (est->bie_model == WOB_DIRECT && est->domain != WOB_INTERIOR)
+            return fail(WOB_UNSUPPORTED_COMBO, "the harmonic control variate takes the direct model in interior domains "
+                                               "only");
         if (scene->dev.hcv_m == 0)
             return fail(WOB_INVALID_ARG, "no harmonic control variate is bound to the scene "
                                          "(wob_scene_fit_harmonic_cv / wob_scene_set_harmonic_cv)");

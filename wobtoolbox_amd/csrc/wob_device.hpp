@@ -1607,9 +1607,16 @@ __device__ __forceinline__ float walk_backward(const DevScene &S, const Model<D,
             W *= m.fredholm_unknown_backward(yp, q.y, q.inv_pdf, rng);
             yp = q.y;
         } else if constexpr (FL == FL_LINE_BV) {
-            Sample<D> q = sample_bv<D>(S, rng);
-            spc += W * m.fredholm_known(yp, q.y, q.inv_pdf, vt, rng);
-            q = line_sample<D, STACK>(S, yp.p, wp.skip(yp.elem), rng, stack, stride);
+            // with the control variate the known term's sample is uniform (the residual data need
+            // not vanish where the data do) and both points carry residual data
+            if (wp.hcv) {
+                const Sample<D> q = sample_uniform<D>(S, rng);
+                spc += W * m.fredholm_known(hcv_residual<D>(S, yp), hcv_residual<D>(S, q.y), q.inv_pdf, vt, rng);
+            } else {
+                const Sample<D> q = sample_bv<D>(S, rng);
+                spc += W * m.fredholm_known(yp, q.y, q.inv_pdf, vt, rng);
+            }
+            Sample<D> q = line_sample<D, STACK>(S, yp.p, wp.skip(yp.elem), rng, stack, stride);
             W = W * m.fredholm_unknown_backward(yp, q.y, q.inv_pdf, rng);
             yp = q.y;
         } else if constexpr (BIE == kSL) {
@@ -1641,7 +1648,8 @@ __device__ __forceinline__ float walk_backward(const DevScene &S, const Model<D,
             yp = q.y;
         } else {
             Sample<D> q = sample_uniform<D>(S, rng);
-            spc += W * m.fredholm_known(yp, q.y, q.inv_pdf, vt, rng);
+            if (wp.hcv) spc += W * m.fredholm_known(hcv_residual<D>(S, yp), hcv_residual<D>(S, q.y), q.inv_pdf, vt, rng);
+            else spc += W * m.fredholm_known(yp, q.y, q.inv_pdf, vt, rng);
             q = sample_uniform<D>(S, rng);
             W = W * m.fredholm_unknown_backward(yp, q.y, q.inv_pdf, rng);
             yp = q.y;
@@ -1652,15 +1660,17 @@ __device__ __forceinline__ float walk_backward(const DevScene &S, const Model<D,
 }
 
 // first samples of the backward estimators (estimator.cuh:158-168, 199-209)
+// (with the control variate the known terms' sample y1 is uniform, as in walk_backward)
 template <int D, int FL, int STACK>
 __device__ __forceinline__ void first_samples(const DevScene &S, const float *x, uint32_t skip, Sample<D> &y,
-                                              Sample<D> &y1, Rng &rng, uint32_t *stack, uint32_t stride) {
+                                              Sample<D> &y1, Rng &rng, uint32_t *stack, uint32_t stride,
+                                              bool hcv = false) {
     if constexpr (FL == FL_LINE_LINE) {
         y = line_sample<D, STACK>(S, x, skip, rng, stack, stride);
         y1 = y;
     } else if constexpr (FL == FL_LINE_BV) {
         y = line_sample<D, STACK>(S, x, skip, rng, stack, stride);
-        y1 = sample_bv<D>(S, rng);
+        y1 = hcv ? sample_uniform<D>(S, rng) : sample_bv<D>(S, rng);
     } else {
         y = sample_uniform<D>(S, rng);
         y1 = sample_uniform<D>(S, rng);
@@ -1674,7 +1684,10 @@ __device__ __forceinline__ void domain_path(const DevScene &S, const Model<D, BI
                                             float *out) {
     if constexpr (EST == kBackward) {                                                          // estimator.cuh:154-191
         Sample<D> y, y1;
-        first_samples<D, FL, STACK>(S, x, 0xFFFFFFFFu, y, y1, rng, stack, stride);
+        first_samples<D, FL, STACK>(S, x, 0xFFFFFFFFu, y, y1, rng, stack, stride, wp.hcv != 0);
+        if constexpr (BIE == kDirect) {
+            if (wp.hcv) y1.y = hcv_residual<D>(S, y1.y);   // the known terms' datum
+        }
         const float known = m.solution_domain_known(x, y1.y, y1.inv_pdf, vt, rng);
         float gk[D], gu[D];
         m.gradient_domain_known(x, y1.y, y1.inv_pdf, vt, rng, gk);
@@ -1684,14 +1697,12 @@ __device__ __forceinline__ void domain_path(const DevScene &S, const Model<D, BI
         out[0] = known + spc * unk;
 #pragma unroll
         for (int k = 0; k < D; ++k) out[1 + k] = gk[k] + spc * gu[k];
-        if constexpr (BIE != kDirect) {
-            if (wp.hcv) {   // u = u_h + the walk's estimate of u - u_h (residual data, the same source)
-                float gh[D];
-                hcv_gradient<D>(S, x, gh);
-                out[0] += hcv_value<D>(S, x);
+        if (wp.hcv) {   // u = u_h + the walk's estimate of u - u_h (residual data, the same source)
+            float gh[D];
+            hcv_gradient<D>(S, x, gh);
+            out[0] += hcv_value<D>(S, x);
 #pragma unroll
-                for (int k = 0; k < D; ++k) out[1 + k] += gh[k];
-            }
+            for (int k = 0; k < D; ++k) out[1 + k] += gh[k];
         }
         if constexpr (BIE == kSL && FL == FL_UNIFORM) {
             if (wp.cv) {

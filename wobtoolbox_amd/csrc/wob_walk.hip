@@ -238,9 +238,15 @@ __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
     auto start_bounce = [&](const WalkEntry &e) {
         const Model<D, BIE> m = model(e);
         if (it == e.path_length - 1) W = 0.5f * W;
-        if constexpr (FL == FL_LINE_BV) {
-            const Sample<D> qb = sample_bv<D>(e.scene, rng);
-            spc += W * m.fredholm_known(yp, qb.y, qb.inv_pdf, NoVol{}, rng);
+        if constexpr (FL == FL_LINE_BV) {   // as in walk_backward, with or without the control variate
+            if (e.flags & 16u) {
+                const Sample<D> qb = sample_uniform<D>(e.scene, rng);
+                spc += W * m.fredholm_known(hcv_residual<D>(e.scene, yp), hcv_residual<D>(e.scene, qb.y), qb.inv_pdf,
+                                            NoVol{}, rng);
+            } else {
+                const Sample<D> qb = sample_bv<D>(e.scene, rng);
+                spc += W * m.fredholm_known(yp, qb.y, qb.inv_pdf, NoVol{}, rng);
+            }
         }
         issue_query(yp.p, (e.flags & 1u) ? yp.elem : 0xFFFFFFFFu);
     };
@@ -257,7 +263,14 @@ __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
             bool done = false;
             if (phase == 0) {                                              // domain_path, first samples
                 Sample<D> y1 = q;
-                if constexpr (FL == FL_LINE_BV) y1 = sample_bv<D>(e.scene, rng);
+                if constexpr (FL == FL_LINE_BV) {
+                    if (e.flags & 16u) {
+                        y1 = sample_uniform<D>(e.scene, rng);
+                        y1.y = hcv_residual<D>(e.scene, y1.y);
+                    } else {
+                        y1 = sample_bv<D>(e.scene, rng);
+                    }
+                }
                 known = m.solution_domain_known(x, y1.y, y1.inv_pdf, NoVol{}, rng);
                 m.gradient_domain_known(x, y1.y, y1.inv_pdf, NoVol{}, rng, gk);
                 unk = m.solution_domain_unknown(x, q.y, q.inv_pdf);
@@ -283,7 +296,7 @@ __global__ __launch_bounds__(256) void walk_kernel_wf(WalkLaunch L) {
             out[0] = known + spc * unk;
 #pragma unroll
             for (int k = 0; k < D; ++k) out[1 + k] = gk[k] + spc * gu[k];
-            if constexpr (FL == FL_LINE_LINE) {
+            {
                 if (e.flags & 16u) {
                     float gh[D];
                     hcv_gradient<D>(e.scene, x, gh);
