@@ -203,116 +203,218 @@ __device__ __forceinline__ void grow(Box6 &a, const Box6 &b) {
 // their leaves' boxes, so every box stays the exact union of its primitives' boxes.  A node
 // only changes when the optimum is strictly cheaper, and each treelet depends only on its
 // own subtree, so the result does not depend on the order in which threads arrive.
+//
+// Each lane climbs from one leaf and grows the treelets it reaches in its own registers
+// (their loads are in flight across the lanes); the dynamic program then runs over the
+// whole wave, one ready treelet at a time: a lane per subset for the unions, and for each
+// subset size the (subset, partition) pairs spread over the lanes with an LDS minimum.
 constexpr int kTreeletLeaves = 7;
 constexpr int kTreeletPasses = 2;
 constexpr float kTreeCi = 1.2f, kTreeCt = 1.0f;
+constexpr int kTreeletSets = 1 << kTreeletLeaves;   // leaf subsets as bit masks
+constexpr int kTreeletLanes = 64;                   // one wave per block
 
-// The subset tables live in LDS, one column per lane: cost of subset s at c[s * kTreeletLanes],
-// its best split at split[s * kTreeletLanes].
-constexpr int kTreeletLanes = 64;
+constexpr int binom7(int m) {
+    int c = 1;
+    for (int i = 0; i < m; ++i) c = c * (kTreeletLeaves - i) / (i + 1);
+    return c;
+}
+// partitions {q, s ^ q} of a subset of m leaves, each once
+constexpr int partitions(int m) { return m < 2 ? 0 : (1 << (m - 1)) - 1; }
+constexpr int pair_begin(int m) {
+    int b = 0;
+    for (int i = 2; i < m; ++i) b += binom7(i) * partitions(i);
+    return b;
+}
+constexpr int set_begin(int m) {
+    int b = 0;
+    for (int i = 1; i < m; ++i) b += binom7(i);
+    return b;
+}
+constexpr int kTreeletPairs = pair_begin(kTreeletLeaves + 1);
+constexpr int pair_rounds(int m) { return (binom7(m) * partitions(m) + kTreeletLanes - 1) / kTreeletLanes; }
+constexpr int pair_slot(int m) {
+    int b = 0;
+    for (int i = 2; i < m; ++i) b += pair_rounds(i);
+    return b;
+}
+constexpr int kPairSlots = pair_slot(kTreeletLeaves + 1);
+static_assert(kTreeletPairs == 966 && kPairSlots == 17, "seven-leaf treelet plan");
 
+// The dynamic program's work list for seven leaves: the subsets grouped by size, and for
+// each subset of at least two leaves its partitions {q, s ^ q}, q holding the lowest leaf
+// of s and q = low | r for the submasks r of the other leaves below all of them, largest
+// first (the tie order of the sequential search).
+struct TreeletPlan {
+    uint8_t pair_s[kTreeletPairs], pair_q[kTreeletPairs];
+    uint8_t set[kTreeletSets - 1];
+};
+constexpr int popcount7(int s) {
+    int c = 0;
+    for (int t = 0; t < kTreeletLeaves; ++t) c += (s >> t) & 1;
+    return c;
+}
+constexpr TreeletPlan make_treelet_plan() {
+    TreeletPlan t{};
+    int np = 0, ns = 0;
+    for (int m = 1; m <= kTreeletLeaves; ++m)
+        for (int s = 1; s < kTreeletSets; ++s) {
+            if (popcount7(s) != m) continue;
+            t.set[ns++] = (uint8_t)s;
+            if (m < 2) continue;
+            const int low = s & -s, rest = s ^ low;
+            for (int r = (rest - 1) & rest;; r = (r - 1) & rest) {
+                t.pair_s[np] = (uint8_t)s;
+                t.pair_q[np] = (uint8_t)(low | r);
+                ++np;
+                if (r == 0) break;
+            }
+        }
+    return t;
+}
+__constant__ TreeletPlan d_treelet_plan = make_treelet_plan();
+
+// a treelet as its climbing lane grows it: leaf ids, boxes, measures, primitive counts and
+// subtree costs; the internal node ids it reuses (the root first)
+template <int D> struct TreeletLeaves {
+    uint32_t L[kTreeletLeaves], I[kTreeletLeaves - 1], LC[kTreeletLeaves];
+    Box6 LB[kTreeletLeaves];
+    float LA[kTreeletLeaves], LCost[kTreeletLeaves];
+    int k;       // leaves
+    float cur;   // the present topology's cost
+};
+
+struct TreeletShared {
+    unsigned long long best[kTreeletSets];   // (cost bits, 255 - q) of the best partition so far
+    Box6 u[kTreeletSets];                    // union box of subset s
+    float c[kTreeletSets];                   // least cost over subset s
+    float area[kTreeletSets];
+    uint32_t cnt[kTreeletSets];
+    uint8_t split[kTreeletSets];             // q of the best partition of s
+    Box6 lb[kTreeletLeaves];
+    uint32_t lc[kTreeletLeaves];
+    int k;
+};
+
+// Leaf arrays are only indexed by compile-time constants (unrolled loops with a predicate),
+// so they stay in registers.
 template <int D>
-__device__ void restructure(uint32_t n_int, uint32_t p, uint32_t *left, uint32_t *right, uint32_t *parent,
-                            uint32_t *count, Box6 *box, float *cost, float *c_col, uint8_t *split_col) {
-    // Leaf arrays are only indexed by compile-time constants (unrolled loops with a
-    // predicate), so they stay in registers.
+__device__ __forceinline__ void grow_treelet(uint32_t n_int, uint32_t p, const uint32_t *left, const uint32_t *right,
+                                             const uint32_t *count, const Box6 *box, const float *cost,
+                                             TreeletLeaves<D> &T) {
     constexpr int K = kTreeletLeaves;
-    uint32_t L[K], I[K - 1], LC[K];
-    Box6 LB[K];
-    float LA[K];
-    auto c = [&](uint32_t set) -> float & { return c_col[set * kTreeletLanes]; };
-    auto split = [&](uint32_t set) -> uint8_t & { return split_col[set * kTreeletLanes]; };
     auto load_leaf = [&](int i, uint32_t id) {
         const Box6 b = coherent_box(box + id);
         const float a = measure<D>(b);
         const uint32_t cn = id < n_int ? coherent_load(count + id) : 1u;
-        c(1u << i) = id < n_int ? coherent_load(cost + id) : kTreeCt * a;
+        const float cc = id < n_int ? coherent_load(cost + id) : kTreeCt * a;
 #pragma unroll
         for (int t = 0; t < K; ++t)
-            if (t == i) { L[t] = id; LB[t] = b; LA[t] = a; LC[t] = cn; }
-    };
-    auto leaf_of = [&](uint32_t bit_set) {   // the leaf of a one-leaf subset
-        uint32_t id = 0;
-#pragma unroll
-        for (int t = 0; t < K; ++t)
-            if (bit_set == (1u << t)) id = L[t];
-        return id;
-    };
-    auto union_of = [&](uint32_t set, Box6 &u, uint32_t &cn) {
-        for (int q = 0; q < 3; ++q) { u.lo[q] = INFINITY; u.hi[q] = -INFINITY; }
-        cn = 0;
-#pragma unroll
-        for (int t = 0; t < K; ++t)
-            if ((set >> t) & 1u) { grow(u, LB[t]); cn += LC[t]; }
+            if (t == i) { T.L[t] = id; T.LB[t] = b; T.LA[t] = a; T.LC[t] = cn; T.LCost[t] = cc; }
     };
     load_leaf(0, coherent_load(left + p));
     load_leaf(1, coherent_load(right + p));
-    Box6 bp = LB[0];
-    grow(bp, LB[1]);
-    const float cur = kTreeCi * measure<D>(bp) + c(1u) + c(2u);
+    Box6 bp = T.LB[0];
+    grow(bp, T.LB[1]);
+    T.cur = kTreeCi * measure<D>(bp) + T.LCost[0] + T.LCost[1];
     int k = 2;
-    I[0] = p;
-    if (LC[0] + LC[1] >= 3u) {
+    T.I[0] = p;
+    if (T.LC[0] + T.LC[1] >= 3u) {
         while (k < K) {
             int best = -1;
             float bestA = -1.f;
             uint32_t x = 0;
 #pragma unroll
             for (int t = 0; t < K; ++t)
-                if (t < k && L[t] < n_int && LA[t] > bestA) { bestA = LA[t]; best = t; x = L[t]; }
+                if (t < k && T.L[t] < n_int && T.LA[t] > bestA) { bestA = T.LA[t]; best = t; x = T.L[t]; }
             if (best < 0) break;
 #pragma unroll
             for (int t = 1; t < K - 1; ++t)
-                if (t == k - 1) I[t] = x;
+                if (t == k - 1) T.I[t] = x;
             load_leaf(best, coherent_load(left + x));
             load_leaf(k, coherent_load(right + x));
             ++k;
         }
     }
-    if (k < 3) {
-        cost[p] = cur;
-        return;
+    T.k = k;
+}
+
+// The least-cost topology over the leaves of lane `owner`'s treelet, by the whole wave.
+// On return sh.c / sh.split / sh.u / sh.cnt hold every subset's optimum.
+template <int D>
+__device__ __forceinline__ void treelet_dp(TreeletShared &sh, const TreeletLeaves<D> &T, bool owner,
+                                           const uint32_t (&pw)[kPairSlots], const uint32_t (&sets)[kTreeletLeaves - 1]) {
+    constexpr int K = kTreeletLeaves;
+    const int lane = (int)threadIdx.x;
+    if (owner) {
+#pragma unroll
+        for (int t = 0; t < K; ++t)
+            if (t < T.k) { sh.lb[t] = T.LB[t]; sh.lc[t] = T.LC[t]; sh.c[1u << t] = T.LCost[t]; }
+        sh.k = T.k;
     }
+    __syncthreads();
+    const int k = sh.k;
     const uint32_t full = (1u << k) - 1u;
-    for (uint32_t s = 1; s <= full; ++s) {
-        if ((s & (s - 1u)) == 0u) continue;   // one leaf: its subtree cost, set above
-        Box6 u;
-        uint32_t cn;
-        union_of(s, u, cn);
-        // each partition once: the side q holding the lowest leaf, q = low | r for the
-        // submasks r of the other leaves except all of them, largest first; four at a time so
-        // that their table loads are in flight together (a q past the last one is masked off;
-        // every index stays below 128)
-        const uint32_t low = s & (0u - s), rest = s ^ low;
-        float best = INFINITY;
-        uint32_t bs = 0;
-        uint32_t r = (rest - 1u) & rest;
-        bool more = true;
-        while (more) {
-            uint32_t qq[4];
-            bool ok[4];
+    // unions (in leaf order, as a sequential build would form them), one lane per subset
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                qq[u] = low | r;
-                ok[u] = more;
-                more = more && r != 0u;
-                r = (r - 1u) & rest;
-            }
-            float v[4];
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t s = (uint32_t)(lane + 1 + kTreeletLanes * h);
+        if (s < (uint32_t)kTreeletSets && (s & ~full) == 0u) {
+            Box6 u;
+            for (int q = 0; q < 3; ++q) { u.lo[q] = INFINITY; u.hi[q] = -INFINITY; }
+            uint32_t cn = 0;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = c(qq[u]) + c(s ^ qq[u]);
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (ok[u] && v[u] < best) { best = v[u]; bs = qq[u]; }
+            for (int t = 0; t < K; ++t)
+                if ((s >> t) & 1u) { grow(u, sh.lb[t]); cn += sh.lc[t]; }
+            sh.u[s] = u;
+            sh.area[s] = measure<D>(u);
+            sh.cnt[s] = cn;
+            sh.best[s] = ~0ull;
         }
-        c(s) = kTreeCi * measure<D>(u) + best;
-        split(s) = (uint8_t)bs;
     }
-    if (!(c(full) < cur)) {
-        cost[p] = cur;
+    __syncthreads();
+#pragma unroll
+    for (int m = 2; m <= K; ++m) {
+        if (m > k) break;
+#pragma unroll
+        for (int r = 0; r < pair_rounds(m); ++r) {
+            const uint32_t w = pw[pair_slot(m) + r];
+            const uint32_t s = w & 0xFFu, q = (w >> 8) & 0xFFu;
+            if (w != 0u && (s & ~full) == 0u) {
+                const float v = sh.c[q] + sh.c[s ^ q];
+                atomicMin(&sh.best[s], ((unsigned long long)__float_as_uint(v) << 32) | (255u - q));
+            }
+        }
+        __syncthreads();
+        const uint32_t s = sets[m - 2];
+        if (s != 0u && (s & ~full) == 0u) {
+            const unsigned long long b = sh.best[s];
+            sh.c[s] = kTreeCi * sh.area[s] + __uint_as_float((uint32_t)(b >> 32));
+            sh.split[s] = (uint8_t)(255u - (uint32_t)(b & 0xFFu));
+        }
+        __syncthreads();
+    }
+}
+
+// rebuild top-down from the subset tables; internal node ids come from T.I in order (k - 1
+// of them, p first)
+template <int D>
+__device__ __forceinline__ void treelet_rebuild(const TreeletShared &sh, const TreeletLeaves<D> &T, uint32_t p,
+                                                uint32_t *left, uint32_t *right, uint32_t *parent, uint32_t *count,
+                                                Box6 *box, float *cost) {
+    constexpr int K = kTreeletLeaves;
+    const uint32_t full = (1u << T.k) - 1u;
+    if (!(sh.c[full] < T.cur)) {
+        cost[p] = T.cur;
         return;
     }
-    // rebuild top-down; internal node ids come from I in order (k - 1 of them, p first)
+    auto leaf_of = [&](uint32_t bit_set) {   // the leaf of a one-leaf subset
+        uint32_t id = 0;
+#pragma unroll
+        for (int t = 0; t < K; ++t)
+            if (bit_set == (1u << t)) id = T.L[t];
+        return id;
+    };
     uint32_t stk_node[K - 1], stk_set[K - 1];
     int sp = 0, used = 1;
     stk_node[sp] = p;
@@ -320,7 +422,7 @@ __device__ void restructure(uint32_t n_int, uint32_t p, uint32_t *left, uint32_t
     while (sp > 0) {
         --sp;
         const uint32_t node = stk_node[sp], s = stk_set[sp];
-        const uint32_t sub[2] = {split(s), s ^ split(s)};
+        const uint32_t sub[2] = {sh.split[s], s ^ sh.split[s]};
         uint32_t ch[2];
         for (int t = 0; t < 2; ++t) {
             if ((sub[t] & (sub[t] - 1u)) == 0u) {
@@ -329,44 +431,73 @@ __device__ void restructure(uint32_t n_int, uint32_t p, uint32_t *left, uint32_t
                 uint32_t id = 0;
 #pragma unroll
                 for (int q = 1; q < K - 1; ++q)
-                    if (q == used) id = I[q];
+                    if (q == used) id = T.I[q];
                 ++used;
                 ch[t] = id;
                 stk_node[sp] = id;
                 stk_set[sp++] = sub[t];
             }
         }
-        Box6 u;
-        uint32_t cn;
-        union_of(s, u, cn);
         left[node] = ch[0];
         right[node] = ch[1];
         parent[ch[0]] = node;
         parent[ch[1]] = node;
-        box[node] = u;
-        count[node] = cn;
-        cost[node] = c(s);
+        box[node] = sh.u[s];
+        count[node] = sh.cnt[s];
+        cost[node] = sh.c[s];
     }
 }
 
 template <int D>
 __global__ __launch_bounds__(kTreeletLanes) void treelet_kernel(int n, uint32_t *left, uint32_t *right, uint32_t *parent,
                                                                 uint32_t *count, Box6 *box, float *cost, uint32_t *flag) {
-    __shared__ float c_tab[(1 << kTreeletLeaves) * kTreeletLanes];
-    __shared__ uint8_t split_tab[(1 << kTreeletLeaves) * kTreeletLanes];
-    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (j >= n || n < 2) return;
+    __shared__ TreeletShared sh;
+    const int lane = (int)threadIdx.x;
+    const int j = (int)(blockIdx.x * blockDim.x) + lane;
+    if (n < 3) return;
     const uint32_t n_int = (uint32_t)(n - 1);
-    uint32_t p = parent[n_int + (uint32_t)j];
+    // this lane's share of the work list, fixed for every treelet
+    uint32_t pw[kPairSlots], sets[kTreeletLeaves - 1];
+#pragma unroll
+    for (int m = 2; m <= kTreeletLeaves; ++m) {
+#pragma unroll
+        for (int r = 0; r < pair_rounds(m); ++r) {
+            const int g = lane + kTreeletLanes * r;
+            pw[pair_slot(m) + r] = g < binom7(m) * partitions(m)
+                                       ? (uint32_t)d_treelet_plan.pair_s[pair_begin(m) + g] |
+                                             ((uint32_t)d_treelet_plan.pair_q[pair_begin(m) + g] << 8)
+                                       : 0u;
+        }
+        sets[m - 2] = lane < binom7(m) ? (uint32_t)d_treelet_plan.set[set_begin(m) + lane] : 0u;
+    }
+    bool live = j < n;
+    uint32_t p = live ? parent[n_int + (uint32_t)j] : 0u;
     // every step goes to an ancestor in the tree as it was at launch (a node's parent only
-    // changes when a treelet above it is rebuilt, after this thread has passed), so the
-    // walk is at most the launch-time depth
+    // changes when a treelet above it is rebuilt, after this lane has passed), so a climb
+    // is at most the launch-time depth
     for (uint32_t up = 0; up < n_int; ++up) {
-        __threadfence();
-        if (atomicAdd(&flag[p], 1u) == 0u) return;   // the sibling subtree is not done yet
-        restructure<D>(n_int, p, left, right, parent, count, box, cost, c_tab + threadIdx.x, split_tab + threadIdx.x);
-        if (p == 0u) return;
-        p = coherent_load(parent + p);
+        bool ready = false;
+        if (live) {
+            __threadfence();
+            ready = atomicAdd(&flag[p], 1u) != 0u;   // the sibling subtree is done
+            live = ready;
+        }
+        TreeletLeaves<D> T;
+        T.k = 0;
+        if (ready) grow_treelet<D>(n_int, p, left, right, count, box, cost, T);
+        const bool dp = ready && T.k >= 3;
+        if (ready && !dp) cost[p] = T.cur;
+        for (unsigned long long todo = __ballot(dp); todo != 0ull; todo &= todo - 1ull) {
+            const int owner = __ffsll((long long)todo) - 1;
+            treelet_dp<D>(sh, T, lane == owner, pw, sets);
+            if (lane == owner) treelet_rebuild<D>(sh, T, p, left, right, parent, count, box, cost);
+            __syncthreads();   // the tables are the next treelet's
+        }
+        if (ready) {
+            if (p == 0u) live = false;
+            else p = coherent_load(parent + p);
+        }
+        if (__ballot(live) == 0ull) break;
     }
 }
 
