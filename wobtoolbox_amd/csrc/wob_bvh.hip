@@ -12,9 +12,10 @@
 //   3. the binary radix tree, one internal node per thread; equal codes are ordered by
 //      index (the delta function falls back to the index bits);
 //   4. boxes bottom-up: each leaf walks to the root, the second child to arrive at a node
-//      writes its box (device-scope fences and coherent loads); then treelet restructuring
-//      by the SAH over the same arrival order, and every node's first primitive in
-//      depth-first order (primitive records follow that order);
+//      writes its box (device-scope fences and coherent loads), and restructures the
+//      treelet below it by the SAH (two passes; the first one also writes the boxes); then
+//      every node's first primitive in depth-first order (primitive records follow that
+//      order);
 //   5. top-down, one level per launch: each binary node of the level becomes a wide node
 //      whose children are its binary children, expanded largest-measure-first until there
 //      are `width` (four by default; the host's collapse rule); child indices of the next level come from an
@@ -300,7 +301,7 @@ struct TreeletShared {
 // so they stay in registers.
 template <int D>
 __device__ __forceinline__ void grow_treelet(uint32_t n_int, uint32_t p, const uint32_t *left, const uint32_t *right,
-                                             const uint32_t *count, const Box6 *box, const float *cost,
+                                             const uint32_t *count, Box6 *box, const float *cost, bool write_box,
                                              TreeletLeaves<D> &T) {
     constexpr int K = kTreeletLeaves;
     auto load_leaf = [&](int i, uint32_t id) {
@@ -316,6 +317,7 @@ __device__ __forceinline__ void grow_treelet(uint32_t n_int, uint32_t p, const u
     load_leaf(1, coherent_load(right + p));
     Box6 bp = T.LB[0];
     grow(bp, T.LB[1]);
+    if (write_box) box[p] = bp;   // the first pass forms the boxes bottom-up as it climbs
     T.cur = kTreeCi * measure<D>(bp) + T.LCost[0] + T.LCost[1];
     int k = 2;
     T.I[0] = p;
@@ -448,14 +450,20 @@ __device__ __forceinline__ void treelet_rebuild(const TreeletShared &sh, const T
     }
 }
 
+// With pbox (the first pass) each lane first writes its leaf's box and every node's box is
+// written on arrival, as the union of its children's (step 4 folded into the climb).
 template <int D>
 __global__ __launch_bounds__(kTreeletLanes) void treelet_kernel(int n, uint32_t *left, uint32_t *right, uint32_t *parent,
-                                                                uint32_t *count, Box6 *box, float *cost, uint32_t *flag) {
+                                                                uint32_t *count, Box6 *box, float *cost, uint32_t *flag,
+                                                                const uint32_t *__restrict__ idx,
+                                                                const Box6 *__restrict__ pbox) {
     __shared__ TreeletShared sh;
     const int lane = (int)threadIdx.x;
     const int j = (int)(blockIdx.x * blockDim.x) + lane;
     if (n < 3) return;
     const uint32_t n_int = (uint32_t)(n - 1);
+    const bool write_box = pbox != nullptr;
+    if (write_box && j < n) box[n_int + (uint32_t)j] = pbox[idx[j]];
     // this lane's share of the work list, fixed for every treelet
     uint32_t pw[kPairSlots], sets[kTreeletLeaves - 1];
 #pragma unroll
@@ -484,7 +492,7 @@ __global__ __launch_bounds__(kTreeletLanes) void treelet_kernel(int n, uint32_t 
         }
         TreeletLeaves<D> T;
         T.k = 0;
-        if (ready) grow_treelet<D>(n_int, p, left, right, count, box, cost, T);
+        if (ready) grow_treelet<D>(n_int, p, left, right, count, box, cost, write_box, T);
         const bool dp = ready && T.k >= 3;
         if (ready && !dp) cost[p] = T.cur;
         for (unsigned long long todo = __ballot(dp); todo != 0ull; todo &= todo - 1ull) {
@@ -959,15 +967,18 @@ template <int D> std::string build(uint32_t n, const float *h_vertices, int widt
         WOB_TRY(hipGetLastError());
         WOB_TRY(hipMemsetAsync(flag, 0, (size_t)n_int * sizeof(uint32_t), st));
     }
-    hipLaunchKernelGGL(bounds_kernel, dim3(blocks(n)), dim3(256), 0, st, (int)n, idx2, pbox, left, right, parent, flag, box);
-    WOB_TRY(hipGetLastError());
-    if (n > 2) {
+    if (n > 2 && kTreeletPasses > 0) {
+        // boxes come from the first treelet pass
         for (int pass = 0; pass < kTreeletPasses; ++pass) {
-            WOB_TRY(hipMemsetAsync(flag, 0, (size_t)n_int * sizeof(uint32_t), st));
+            if (pass > 0) WOB_TRY(hipMemsetAsync(flag, 0, (size_t)n_int * sizeof(uint32_t), st));
             hipLaunchKernelGGL((treelet_kernel<D>), dim3((n + kTreeletLanes - 1) / kTreeletLanes), dim3(kTreeletLanes), 0, st,
-                               (int)n, left, right, parent, count, box, cost, flag);
+                               (int)n, left, right, parent, count, box, cost, flag, idx2, pass == 0 ? pbox : nullptr);
             WOB_TRY(hipGetLastError());
         }
+    } else {
+        hipLaunchKernelGGL(bounds_kernel, dim3(blocks(n)), dim3(256), 0, st, (int)n, idx2, pbox, left, right, parent, flag,
+                           box);
+        WOB_TRY(hipGetLastError());
     }
     if (n > 1) {
         hipLaunchKernelGGL(order_kernel, dim3(blocks(n_all)), dim3(256), 0, st, (int)n, left, right, parent, count, first,

@@ -13,6 +13,8 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "../../include/wob_c.h"
@@ -564,26 +566,48 @@ wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_s
     if (width < 2 || width > 4) return fail(WOB_INVALID_ARG, "BVH width must be 2, 3 or 4");
     wob_scene *s = new (std::nothrow) wob_scene();
     if (!s) return fail(WOB_OOM, "host allocation failed");
+    (void)hipGetDevice(&s->device);
+    // The device BVH build reads only the vertices: it runs on a second host thread (on this
+    // thread's device) while this one builds the attributes, CDFs and guides.
+    wobl::DeviceBvh dv;
+    std::string dev_err;
+    std::thread dev_build;
+    const bool shapes_ok = (desc->dim == 2u || desc->dim == 3u) && desc->n_elements > 0u &&
+                           desc->n_elements < (1u << 27) && desc->vertices != nullptr;
+    if (device_bvh && shapes_ok) {
+        try {
+            dev_build = std::thread([&dv, &dev_err, desc, width, device = s->device] {
+                if (hipSetDevice(device) != hipSuccess) {
+                    dev_err = "hipSetDevice on the build thread failed";
+                    return;
+                }
+                dev_err = wobl::build_bvh_device((int)desc->dim, desc->n_elements, desc->vertices, dv, width);
+            });
+        } catch (const std::system_error &) {   // no thread: build in sequence below
+        }
+    }
     std::string err;
+    bool oom = false;
     try {
         err = wobhost::build_scene(*desc, s->host, !device_bvh, width);
     } catch (const std::bad_alloc &) {
-        delete s;
-        return fail(WOB_OOM, "host allocation failed during scene build");
+        oom = true;
     }
-    if (!err.empty()) {
+    if (dev_build.joinable()) dev_build.join();
+    else if (device_bvh && !oom && err.empty())
+        dev_err = wobl::build_bvh_device((int)desc->dim, desc->n_elements, desc->vertices, dv, width);
+    if (oom || !err.empty() || !dev_err.empty()) {
+        (void)hipFree(dv.fnodes);
+        (void)hipFree(dv.qnodes);
+        (void)hipFree(dv.prims);
+        (void)hipFree(dv.prim_id);
         delete s;
-        return fail(WOB_INVALID_ARG, err);
+        if (oom) return fail(WOB_OOM, "host allocation failed during scene build");
+        if (!err.empty()) return fail(WOB_INVALID_ARG, err);
+        return fail(WOB_DEVICE, dev_err);
     }
-    (void)hipGetDevice(&s->device);
     s->host.bvh_width = (uint32_t)width;
-    wobl::DeviceBvh dv;
     if (device_bvh) {
-        err = wobl::build_bvh_device((int)desc->dim, desc->n_elements, desc->vertices, dv, width);
-        if (!err.empty()) {
-            delete s;
-            return fail(WOB_DEVICE, err);
-        }
         s->bufs[0] = dv.fnodes;
         s->bufs[1] = dv.prims;
         s->bufs[2] = dv.prim_id;
