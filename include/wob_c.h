@@ -86,8 +86,9 @@ wob_status wob_scene_create(const wob_scene_desc *desc, wob_scene **out_scene);
  * layouts (subtrees of at most two primitives become leaves).  Attributes and CDFs are built
  * on the host as in wob_scene_create.  Walk results, intersections and
  * closest points are the same as with the host-built scene: they do not depend on the tree.
- * WOB_DEVICE if the device build fails; WOB_INVALID_ARG for unknown flags or a stack bound
- * over 48 entries.                                                                          */
+ * A device tree whose traversal stack bound exceeds 48 entries is replaced by the host SAH
+ * build (which falls back to median splits).  WOB_DEVICE if the device build fails;
+ * WOB_INVALID_ARG for unknown flags or a stack bound over 48 entries after that.            */
 enum { WOB_SCENE_DEVICE_BVH = 1 };
 /* WOB_SCENE_BVH_WIDTH(w), w = 2, 3 or 4 (default 4): every BVH node takes at most w children
  * (the node layout stays 4-wide; a narrower tree is deeper).  wob_tune_bvh_width picks w.   */

@@ -139,6 +139,54 @@ def test_device_bvh_tiny_and_degenerate_scenes(dim, n, same, gpu):
     assert np.array_equal(e0, e1)
 
 
+def _comb_scene(cluster=32768):
+    """Tiny triangles whose Morton codes make the radix tree a comb 30 levels deep above a
+    balanced cluster: one triangle in each Morton cell with a single bit set (a coordinate
+    2^b cells from the origin, the others 0) and `cluster` triangles in the origin's cell; a
+    triangle at the far corner spans the Morton grid.  Collapsed to 4-wide nodes, the comb
+    alone puts three entries per level on the traversal stack."""
+    rng = np.random.default_rng(11)
+    cell = 1.0 / 1024.0
+    cents = [np.full(3, 1023.5 * cell)]
+    for b in range(10):
+        for a in range(3):
+            c = np.full(3, 0.5 * cell)
+            c[a] = (2 ** b + 0.5) * cell
+            cents.append(c)
+    cents = np.array(cents + list(rng.uniform(0.1 * cell, 0.9 * cell, (cluster, 3))))
+    n = len(cents)
+    V = np.repeat(cents[:, None, :], 3, 1) + rng.uniform(-1e-6, 1e-6, (n, 3, 3))
+    V[0] = [[1.0, 1.0, 1.0 - 1e-3], [1.0 - 1e-3, 1.0, 1.0], [1.0, 1.0 - 1e-3, 1.0]]
+    V = V.astype(np.float32)
+    nr = np.cross(V[:, 1] - V[:, 0], V[:, 2] - V[:, 0]).astype(np.float64)
+    nr /= np.maximum(np.linalg.norm(nr, axis=1, keepdims=True), 1e-30)
+    return G.Elements(3, V, nr.astype(np.float32), np.ones((n, 3), np.float32), np.zeros(n, np.int32),
+                      np.zeros((n, 3), np.float32))
+
+
+def test_device_bvh_deep_comb_scene(gpu):
+    """The comb scene, built to make the radix tree deep: the device build's stack bound stays
+    within the 48-entry LDS stack (39 measured; a deeper tree would be replaced by the host
+    SAH build) and its hit sets are the host-built scene's."""
+    el = _comb_scene()
+    host, dev = _scenes(el)
+    ih, idv = host.info(), dev.info()
+    _say(f"comb scene: {el.n} elements; host tree {ih['n_nodes']} nodes, depth {ih['depth']}, "
+         f"stack {ih['stack_need']}; device tree {idv['n_nodes']} nodes, depth {idv['depth']}, "
+         f"stack {idv['stack_need']}")
+    assert idv["depth"] > ih["depth"] and idv["stack_need"] <= 48
+    rng = np.random.default_rng(3)
+    m = 2000
+    o = rng.uniform(0.0, 2.0 / 1024.0, (m, 3)).astype(np.float32)
+    d = rng.normal(size=(m, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    (c0, i0, _), (c1, i1, _) = [[t.cpu().numpy() for t in sc.intersect_lines(_dev(o, gpu), _dev(d, gpu), max_hits=64)]
+                                for sc in (host, dev)]
+    assert np.array_equal(c0, c1)
+    for k in range(m):
+        assert sorted(i0[k, :c0[k]].tolist()) == sorted(i1[k, :c1[k]].tolist())
+
+
 def test_device_bvh_flags(gpu):
     import ctypes as C
     import wobtoolbox_amd as w

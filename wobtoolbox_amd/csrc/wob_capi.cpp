@@ -560,7 +560,7 @@ wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_s
     *out_scene = nullptr;
     if (flags & ~(uint32_t)(WOB_SCENE_DEVICE_BVH | WOB_SCENE_BVH_WIDTH_MASK))
         return fail(WOB_INVALID_ARG, "unknown scene flags");
-    const bool device_bvh = (flags & WOB_SCENE_DEVICE_BVH) != 0u;
+    bool device_bvh = (flags & WOB_SCENE_DEVICE_BVH) != 0u;
     const uint32_t wfield = (flags & WOB_SCENE_BVH_WIDTH_MASK) >> WOB_SCENE_BVH_WIDTH_SHIFT;
     const int width = wfield == 0 ? 4 : (int)wfield;
     if (width < 2 || width > 4) return fail(WOB_INVALID_ARG, "BVH width must be 2, 3 or 4");
@@ -607,6 +607,27 @@ wob_status wob_scene_create_ex(const wob_scene_desc *desc, uint32_t flags, wob_s
         return fail(WOB_DEVICE, dev_err);
     }
     s->host.bvh_width = (uint32_t)width;
+    if (device_bvh && dv.stack_need > (uint32_t)wobl::kStackEntries) {
+        // a tree too deep for the traversal stack: the host SAH build (which falls back to
+        // median splits) replaces it
+        (void)hipFree(dv.fnodes);
+        (void)hipFree(dv.qnodes);
+        (void)hipFree(dv.prims);
+        (void)hipFree(dv.prim_id);
+        dv = wobl::DeviceBvh{};
+        try {
+            err = wobhost::build_scene(*desc, s->host, true, width);
+        } catch (const std::bad_alloc &) {
+            delete s;
+            return fail(WOB_OOM, "host allocation failed during scene build");
+        }
+        if (!err.empty()) {
+            delete s;
+            return fail(WOB_INVALID_ARG, err);
+        }
+        s->host.bvh_width = (uint32_t)width;
+        device_bvh = false;
+    }
     if (device_bvh) {
         s->bufs[0] = dv.fnodes;
         s->bufs[1] = dv.prims;
