@@ -668,11 +668,11 @@ template <int D> struct Trav {
         node_step<STACK>(nodes, stack, stride, inv, oi);
     }
     // the same with the slab parameters of another lane's line (a lane working on a subtree
-    // taken from that lane's query)
-    template <int STACK>
+    // taken from that lane's query); RAY: only the ray t >= 0 is wanted
+    template <int STACK, bool RAY = false>
     __device__ __forceinline__ void node_step(const float4 *nodes, uint32_t *stack, uint32_t stride, const float *inv,
                                               const float *oi) {
-        node_apply<STACK>(node_load(nodes), stack, stride, inv, oi);
+        node_apply<STACK, RAY>(node_load(nodes), stack, stride, inv, oi);
     }
     // the current node's quantized record (kQNodeWords layout of scene_build.hpp): node box
     // corner lo and per-axis step s, 8-bit child bounds, child entries (0x80000000: none)
@@ -705,7 +705,10 @@ template <int D> struct Trav {
     // where inv >= 0 and fall where inv < 0 (the rounded FMA is monotone in q), so the near
     // bound of every child along an axis is its lower bound for inv >= 0 and its upper bound
     // otherwise: one selection per node and axis instead of a min and a max per child.
-    template <int STACK>
+    // RAY (a query that counts only hits with t > 0) also skips children whose slab interval
+    // ends before t = 0: a counted hit has t > 10 FLT_EPSILON and lies in its (inflated) box,
+    // so that box's interval reaches past 0.
+    template <int STACK, bool RAY = false>
     __device__ __forceinline__ void node_apply(const NodeRec &r, uint32_t *stack, uint32_t stride, const float *inv,
                                                const float *oi) {
         const int ch[4] = {(int)__float_as_uint(r.chf.x), (int)__float_as_uint(r.chf.y), (int)__float_as_uint(r.chf.z),
@@ -727,7 +730,7 @@ template <int D> struct Trav {
             for (int k = 0; k < D; ++k) {
                 const float n = (float)((qn[k] >> (8 * c)) & 0xFFu), f = (float)((qf[k] >> (8 * c)) & 0xFFu);
                 const float t1 = __builtin_fmaf(n, A[k], B[k]), t2 = __builtin_fmaf(f, A[k], B[k]);
-                tmin = k == 0 ? t1 : fmaxf(tmin, t1);
+                tmin = k == 0 && !RAY ? t1 : fmaxf(tmin, t1);
                 tmax = k == 0 ? t2 : fminf(tmax, t2);
             }
             // no overflow test: scene creation rejects BVHs whose bound (the pushes included)
@@ -927,7 +930,7 @@ __device__ __forceinline__ LineHit<D> line_query_ws(const DevScene &S, const flo
             float inv[D], oi[D];
 #pragma unroll
             for (int k = 0; k < D; ++k) { inv[k] = lane_read(tr.inv[k], own4); oi[k] = lane_read(tr.oi[k], own4); }
-            if (want_node) tr.template node_step<STACK>(S.nodes, stack, stride, inv, oi);
+            if (want_node) tr.template node_step<STACK, !PICK>(S.nodes, stack, stride, inv, oi);
         }
     }
     if (cnt != 0u) {
