@@ -136,12 +136,13 @@ __device__ __forceinline__ void walk_item(const WalkLaunch &L, uint32_t item, ui
 // Waves per SIMD the compiler must fit (the register budget): 4 (128 VGPRs) for the 2D
 // single-layer RIS domain walk (the batched exterior scenes), which fits them without spills;
 // 3 (168 VGPRs, as many as a 48-entry LDS stack allows) for the 3D direct Neumann walk and the
-// 3D double-layer walk with a volume term, whose line queries wait on L2 misses (the latter
-// spills but runs a third faster than at 2 waves); the others take the compiler's choice.
+// double-layer walks with a volume term, whose line queries and volume samples wait on memory
+// (they spill, but the 3D one runs a third faster than at 2 waves and the 2D grid-source one,
+// the Poisson app's, an eighth); the others take the compiler's choice.
 template <int D, int BIE, int EST, int FL, bool BOUNDARY, bool VOL> constexpr int walk_min_waves() {
     if (D == 2 && BIE == kSL && EST == kBackward && FL == FL_UNIFORM && !BOUNDARY && !VOL) return 4;
     if (D == 3 && BIE == kDirect && FL == FL_LINE_BV && !VOL) return 3;
-    if (D == 3 && BIE == kDL && FL == FL_LINE_LINE && VOL) return 3;
+    if (BIE == kDL && FL == FL_LINE_LINE && VOL) return 3;
     return 1;
 }
 
