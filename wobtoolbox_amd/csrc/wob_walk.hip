@@ -135,14 +135,16 @@ __device__ __forceinline__ void walk_item(const WalkLaunch &L, uint32_t item, ui
 // (point, chunk) work at once.
 // Waves per SIMD the compiler must fit (the register budget): 4 (128 VGPRs) for the 2D
 // single-layer RIS domain walk (the batched exterior scenes), which fits them without spills;
-// 3 (168 VGPRs, as many as a 48-entry LDS stack allows) for the 3D direct Neumann walk and the
-// double-layer walks with a volume term, whose line queries and volume samples wait on memory
-// (they spill, but the 3D one runs a third faster than at 2 waves and the 2D grid-source one,
-// the Poisson app's, an eighth); the others take the compiler's choice.
+// 3 (168 VGPRs, as many as a 48-entry LDS stack allows) for the walks with line-sampled known
+// terms (direct Neumann, forward single layer) and every walk with a volume term, whose line
+// queries and volume samples wait on memory.  Measured against the compiler's choice of 2
+// waves (some spill at 3): 3D double layer with the Gaussian source a third faster, 2D double
+// layer with a grid source (the Poisson app) an eighth, 2D direct Neumann 29%, 3D forward
+// single layer 21%.  The double-layer line walks without a volume term fit 3 waves by
+// themselves and run no faster with the bound; the others take the compiler's choice.
 template <int D, int BIE, int EST, int FL, bool BOUNDARY, bool VOL> constexpr int walk_min_waves() {
     if (D == 2 && BIE == kSL && EST == kBackward && FL == FL_UNIFORM && !BOUNDARY && !VOL) return 4;
-    if (D == 3 && BIE == kDirect && FL == FL_LINE_BV && !VOL) return 3;
-    if (BIE == kDL && FL == FL_LINE_LINE && VOL) return 3;
+    if (FL == FL_LINE_BV || FL == FL_FWD_LINE || VOL) return 3;
     return 1;
 }
 
