@@ -277,6 +277,17 @@ struct VolumeGauss {
     bool mask = true;
     unsigned n = 0;
     std::vector<float> c, amp, inv2w2, pn, cdf, w;   // c: n * 3; pn: mixture density coefficient
+    // Occupancy grid of the inside test (no counterpart in the reference): res^D cells over the
+    // vertex box, each outside (0), inside (1) or near the boundary (2).  A cell is near when it
+    // lies within one cell of an element's inflated box; otherwise the rays from two points of
+    // the cell (its centre and the point a quarter cell towards its low corner) decide it when
+    // their parities agree, and it is near when they do not.  A sample in an outside or inside
+    // cell takes the cell's parity (no element meets the cell, so the parity is the same at
+    // every point of it); a sample in a near cell, or within one cell outside the grid, casts
+    // its own ray; a sample further out is outside.
+    unsigned gres = 0;
+    float glo[3] = {0.f, 0.f, 0.f}, gh[3] = {0.f, 0.f, 0.f}, ginv[3] = {0.f, 0.f, 0.f};
+    std::vector<unsigned char> gflag;
 };
 
 template <int D> struct Scene {
@@ -748,6 +759,9 @@ static inline void idx_to_domain_point(unsigned idx, unsigned grid_res, float do
     for (int k = 0; k < D; ++k) out[k] = ((float)u[k] + 0.5f) * dx - (domain_size / 2);
 }
 
+// Cells per axis of the inside test's occupancy grid.
+constexpr unsigned kInsideGridRes2 = 512, kInsideGridRes3 = 64;
+
 // Fixed direction of the inside/outside ray (parity of the crossings with t > 0).
 static const float kRay2[2] = {0.58778525f, 0.80901699f};
 static const float kRay3[3] = {0.48218104f, 0.66748405f, 0.56749094f};
@@ -777,6 +791,73 @@ template <int D> static unsigned ray_crossings(const Scene<D> &s, const Vec<D> &
         }
     }
     return count;
+}
+
+// The inside-test grid's verdict at p: 0 outside, 1 inside, 2 unknown (the caller casts p's ray).
+template <int D> static int inside_grid_cell(const VolumeGauss &g, const float *p) {
+    if (g.gres == 0) return 2;
+    const float R = (float)g.gres;
+    size_t idx = 0, mul = 1;
+    bool out = false, edge = false;
+    for (int k = 0; k < D; ++k) {
+        const float sk = (p[k] - g.glo[k]) * g.ginv[k];
+        if (!(sk >= -1.f && sk < R + 1.f)) out = true;
+        else if (!(sk >= 0.f && sk < R)) edge = true;
+        else { idx += (size_t)sk * mul; mul *= g.gres; }
+    }
+    if (out) return 0;
+    if (edge) return 2;
+    return g.gflag[idx];
+}
+
+template <int D> static void build_inside_grid(const Scene<D> &s, VolumeGauss &g, unsigned res) {
+    g.gres = 0;
+    g.gflag.clear();
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (const auto &e : s.elems)
+        for (int j = 0; j < D; ++j)
+            for (int k = 0; k < D; ++k) { lo[k] = std::min(lo[k], e.v[j][k]); hi[k] = std::max(hi[k], e.v[j][k]); }
+    for (int k = 0; k < D; ++k)
+        if (!(hi[k] > lo[k])) return;
+    for (int k = 0; k < D; ++k) {
+        g.glo[k] = lo[k];
+        g.gh[k] = (hi[k] - lo[k]) / (float)res;
+        g.ginv[k] = (float)res / (hi[k] - lo[k]);
+    }
+    size_t n = 1;
+    for (int k = 0; k < D; ++k) n *= res;
+    g.gres = res;
+    g.gflag.assign(n, 255);
+    for (const auto &e : s.elems) {
+        int i0[3] = {0, 0, 0}, i1[3] = {0, 0, 0};
+        for (int k = 0; k < D; ++k) {
+            float bl = e.v[0][k], bh = e.v[0][k];
+            for (int j = 1; j < D; ++j) { bl = std::min(bl, e.v[j][k]); bh = std::max(bh, e.v[j][k]); }
+            const float pad = std::max(std::abs(bl), std::abs(bh)) * 4.0e-6f + 1.0e-7f;
+            bl -= pad;
+            bh += pad;
+            i0[k] = std::max(0, (int)std::floor((bl - g.glo[k]) * g.ginv[k]) - 1);
+            i1[k] = std::min((int)res - 1, (int)std::floor((bh - g.glo[k]) * g.ginv[k]) + 1);
+        }
+        const int z0 = D == 3 ? i0[2] : 0, z1 = D == 3 ? i1[2] : 0;
+        for (int z = z0; z <= z1; ++z)
+            for (int y = i0[1]; y <= i1[1]; ++y)
+                for (int x = i0[0]; x <= i1[0]; ++x) g.gflag[(size_t)x + ((size_t)y + (size_t)z * res) * res] = 2;
+    }
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ci = 0; ci < (long long)n; ++ci) {
+        if (g.gflag[(size_t)ci] == 2) continue;
+        size_t rem = (size_t)ci;
+        Vec<D> pc, pq;
+        for (int k = 0; k < D; ++k) {
+            const float ik = (float)(rem % res);
+            rem /= res;
+            pc[k] = g.glo[k] + (ik + 0.5f) * g.gh[k];
+            pq[k] = g.glo[k] + (ik + 0.25f) * g.gh[k];
+        }
+        const unsigned a = ray_crossings(s, pc) & 1u, b = ray_crossings(s, pq) & 1u;
+        g.gflag[(size_t)ci] = (unsigned char)(a == b ? a : 2u);
+    }
 }
 
 template <int D> static VolSample sample_volume(const Scene<D> &s, Rng &rng, bool interior) {
@@ -828,10 +909,13 @@ template <int D> static VolSample sample_volume(const Scene<D> &s, Rng &rng, boo
     r.value = f;
     r.inv_pdf = 1.0f / pdf;
     if (g.mask) {
-        Vec<D> o;
-        for (int k = 0; k < D; ++k) o[k] = r.p[k];
-        const bool inside = (ray_crossings(s, o) & 1u) != 0;
-        if (inside != interior) r.value = 0.f;
+        int cell = inside_grid_cell<D>(g, r.p);
+        if (cell == 2) {
+            Vec<D> o;
+            for (int k = 0; k < D; ++k) o[k] = r.p[k];
+            cell = (int)(ray_crossings(s, o) & 1u);
+        }
+        if ((cell != 0) != interior) r.value = 0.f;
     }
     r.valid = true;
     return r;
@@ -1345,6 +1429,10 @@ int orc_scene_set_volume_gaussians(void *p, unsigned n, const float *centers, co
         g.pn.push_back((float)(pj * std::pow(2.0 * M_PI * w * w, -0.5 * D)));
         g.cdf.push_back(j + 1 == n ? 1.0f : (float)acc);
     }
+    if (g.mask) {
+        if (D == 2) build_inside_grid(S->s2, g, kInsideGridRes2);
+        else build_inside_grid(S->s3, g, kInsideGridRes3);
+    }
     return 0;
 }
 
@@ -1352,6 +1440,26 @@ int orc_scene_set_volume_gaussians(void *p, unsigned n, const float *centers, co
 int orc_inside(void *p, const float *pt) {
     auto *S = (OrcScene *)p;
     if (S->dim == 2) { Vec<2> o{{pt[0], pt[1]}}; return (int)(ray_crossings(S->s2, o) & 1u); }
+    Vec<3> o{{pt[0], pt[1], pt[2]}};
+    return (int)(ray_crossings(S->s3, o) & 1u);
+}
+
+// Inside/outside verdict of a masked Gaussian source's volume sample at pt: the occupancy
+// grid's cell, or the point's own ray where the grid leaves it open (-1: no masked source).
+int orc_inside_grid(void *p, const float *pt, int *cell) {
+    auto *S = (OrcScene *)p;
+    if (S->dim == 2) {
+        const VolumeGauss &g = S->s2.gauss;
+        if (!g.present || !g.mask) return -1;
+        *cell = inside_grid_cell<2>(g, pt);
+        if (*cell != 2) return *cell;
+        Vec<2> o{{pt[0], pt[1]}};
+        return (int)(ray_crossings(S->s2, o) & 1u);
+    }
+    const VolumeGauss &g = S->s3.gauss;
+    if (!g.present || !g.mask) return -1;
+    *cell = inside_grid_cell<3>(g, pt);
+    if (*cell != 2) return *cell;
     Vec<3> o{{pt[0], pt[1], pt[2]}};
     return (int)(ray_crossings(S->s3, o) & 1u);
 }

@@ -54,6 +54,8 @@ def lib():
         L.orc_scene_set_volume_gaussians.argtypes = [P, C.c_uint, P, P, P, C.c_int]
         L.orc_inside.argtypes = [P, P]
         L.orc_inside.restype = C.c_int
+        L.orc_inside_grid.argtypes = [P, P, P]
+        L.orc_inside_grid.restype = C.c_int
         L.orc_intersect_line.argtypes = [P, P, P, C.c_int, C.c_int, P, P]
         L.orc_intersect_line.restype = C.c_int
         L.orc_estimate_domain.argtypes = [P, C.POINTER(EstDesc), P, P, C.c_int, C.c_uint, C.c_uint64, C.c_uint64,
@@ -156,6 +158,18 @@ class OracleScene:
         w = np.ascontiguousarray(widths, np.float32)
         a = np.ascontiguousarray(amplitudes, np.float32)
         lib().orc_scene_set_volume_gaussians(self.h, len(w), _p(c), _p(w), _p(a), int(mask))
+
+    def inside_grid(self, points):
+        """(verdict, cell) per point from the masked Gaussian source's occupancy grid (cell 2:
+        the point's own ray decided)."""
+        pts = np.ascontiguousarray(points, np.float32)
+        cell = C.c_int(0)
+        out = np.zeros(len(pts), np.int32)
+        cells = np.zeros(len(pts), np.int32)
+        for i in range(len(pts)):
+            out[i] = lib().orc_inside_grid(self.h, _p(pts[i]), C.byref(cell))
+            cells[i] = cell.value
+        return out, cells
 
     def inside(self, points):
         pts = np.ascontiguousarray(points, np.float32)

@@ -321,6 +321,26 @@ def test_inside_parity_matches_star_shape(oracle):
     assert np.array_equal(ins, (r < rs)[keep])
 
 
+def test_inside_grid_matches_ray_parity(oracle):
+    """The occupancy grid of a masked Gaussian source gives every point the parity its own ray
+    gives (points inside, outside, near the surface and just outside the grid), and decides
+    most of the points that are not near the surface without a ray."""
+    el, V, F = small_icosphere(level=4, kind="neumann")
+    sc = oracle.OracleScene(el)
+    sc.set_volume_gaussians(np.zeros((1, 3), np.float32), np.array([0.3], np.float32), np.array([1.0], np.float32))
+    rng = np.random.default_rng(8)
+    p = np.concatenate([rng.uniform(-1.5, 1.5, (6000, 3)), rng.normal(scale=0.4, size=(3000, 3))]).astype(np.float32)
+    verdict, cell = sc.inside_grid(p)
+    ray = sc.inside(p)
+    assert np.array_equal(verdict == 1, ray)
+    r = np.linalg.norm(p.astype(np.float64), axis=1)
+    rs = G.smooth_radius(p / r[:, None])
+    far = np.abs(r - rs) > 0.1
+    decided = cell[far] != 2
+    print(f"inside grid: {decided.mean():.3f} of the points 0.1 or more from the surface decided by their cell")
+    assert decided.mean() > 0.9
+
+
 def _poisson_mesh_workload(level=3):
     V, F = G.icosphere(level)
     V = G.displace_radial(V)

@@ -478,4 +478,49 @@ void quantize_nodes(int dim, const std::vector<float> &nodes, std::vector<uint32
     }
 }
 
+bool inside_grid_marks(const HostScene &S, uint32_t res, float lo[3], float h[3], float inv[3],
+                       std::vector<uint8_t> &flag) {
+    const int D = (int)S.dim;
+    const int AW = D == 2 ? kAttrWords2 : kAttrWords3;
+    float hi[3];
+    for (int k = 0; k < 3; ++k) { lo[k] = INFINITY; hi[k] = -INFINITY; h[k] = 0.f; inv[k] = 0.f; }
+    for (uint32_t e = 0; e < S.n; ++e)
+        for (int j = 0; j < D; ++j)
+            for (int k = 0; k < D; ++k) {
+                const float x = S.elem_attr[(size_t)e * AW + j * D + k];
+                lo[k] = std::min(lo[k], x);
+                hi[k] = std::max(hi[k], x);
+            }
+    for (int k = 0; k < D; ++k)
+        if (!(hi[k] > lo[k])) return false;
+    for (int k = 0; k < D; ++k) {
+        h[k] = (hi[k] - lo[k]) / (float)res;
+        inv[k] = (float)res / (hi[k] - lo[k]);
+    }
+    for (int k = D; k < 3; ++k) lo[k] = 0.f;
+    size_t n = 1;
+    for (int k = 0; k < D; ++k) n *= res;
+    flag.assign(n, 255u);
+    for (uint32_t e = 0; e < S.n; ++e) {
+        int i0[3] = {0, 0, 0}, i1[3] = {0, 0, 0};
+        for (int k = 0; k < D; ++k) {
+            float bl = S.elem_attr[(size_t)e * AW + k], bh = bl;
+            for (int j = 1; j < D; ++j) {
+                const float x = S.elem_attr[(size_t)e * AW + j * D + k];
+                bl = std::min(bl, x);
+                bh = std::max(bh, x);
+            }
+            const float pad = std::max(std::abs(bl), std::abs(bh)) * 4.0e-6f + 1.0e-7f;
+            bl -= pad;
+            bh += pad;
+            i0[k] = std::max(0, (int)std::floor((bl - lo[k]) * inv[k]) - 1);
+            i1[k] = std::min((int)res - 1, (int)std::floor((bh - lo[k]) * inv[k]) + 1);
+        }
+        for (int z = i0[2]; z <= i1[2]; ++z)
+            for (int y = i0[1]; y <= i1[1]; ++y)
+                for (int x = i0[0]; x <= i1[0]; ++x) flag[(size_t)x + ((size_t)y + (size_t)z * res) * res] = 2u;
+    }
+    return true;
+}
+
 }  // namespace wobhost

@@ -89,4 +89,14 @@ constexpr int kFusedPerElement = 2;
 // within one step of guide[k] reads one 16-byte record and no CDF entry.
 std::vector<float> guide_records(const std::vector<float> &cdf, const std::vector<uint32_t> &guide);
 
+// The inside test's occupancy grid (volume samples of a masked Gaussian source): res^D cells
+// over the vertex box (res = kInsideGridRes2 or kInsideGridRes3), cell (x, y, z) at
+// x + res (y + res z).  inside_grid_marks fills the layout (corner lo, cell size h, inv =
+// res / extent) and marks with 2 every cell within one cell of an element's inflated box (the
+// BVH's inflation rule), 255 the rest (the device classifies those by ray parity).  It returns
+// false, and no grid is used, when the vertex box is flat along an axis.
+constexpr uint32_t kInsideGridRes2 = 512, kInsideGridRes3 = 64;
+bool inside_grid_marks(const HostScene &S, uint32_t res, float lo[3], float h[3], float inv[3],
+                       std::vector<uint8_t> &flag);
+
 }  // namespace wobhost

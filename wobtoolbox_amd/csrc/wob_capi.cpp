@@ -870,7 +870,31 @@ wob_status wob_scene_set_volume_gaussians(wob_scene *scene, uint32_t n, const fl
     v.mask = (flags & WOB_VOLUME_NO_MASK) ? 0u : 1u;
     v.comp = (const float4 *)p[0];
     v.cdf = (const float *)p[1];
+    v.gflag = nullptr;
+    v.gres = 0u;
     scene->version += 1;
+    // the inside test's occupancy grid: near-boundary cells marked on the host, the others
+    // classified on the device by ray parity
+    std::vector<uint8_t> gflag;
+    const uint32_t res = D == 2 ? wobhost::kInsideGridRes2 : wobhost::kInsideGridRes3;
+    float glo[3], gh[3], ginv[3];
+    if (v.mask && wobhost::inside_grid_marks(scene->host, res, glo, gh, ginv, gflag)) {
+        void *g = nullptr;
+        if (hipMalloc(&g, gflag.size()) != hipSuccess) return fail(WOB_OOM, "hipMalloc inside-test grid");
+        scene->vol_bufs[2] = g;
+        WOB_HIP(hipMemcpy(g, gflag.data(), gflag.size(), hipMemcpyHostToDevice), "hipMemcpy inside-test grid");
+        v.gres = res;
+        for (int k = 0; k < 3; ++k) { v.glo[k] = glo[k]; v.gh[k] = gh[k]; v.ginv[k] = ginv[k]; }
+        const size_t lds = wobl::walk_lds_bytes(wobdev::FL_UNIFORM, true, 256);
+        hipError_t e = D == 2 ? wobl::launch_inside_grid_2(scene->dev, (uint8_t *)g, (uint32_t)gflag.size(), lds, nullptr)
+                              : wobl::launch_inside_grid_3(scene->dev, (uint8_t *)g, (uint32_t)gflag.size(), lds, nullptr);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e != hipSuccess) {
+            v.gres = 0u;
+            return fail(WOB_DEVICE, std::string("inside-test grid: ") + hipGetErrorString(e));
+        }
+        v.gflag = (const uint8_t *)g;
+    }
     return WOB_OK;
 }
 

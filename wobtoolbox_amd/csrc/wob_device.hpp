@@ -54,6 +54,11 @@ struct VolSrc {
     const float *cdf;        // grid cell CDF (n) or mixture CDF (n)
     const uint32_t *guide;   // grid guide table
     const float4 *comp;      // mixture: [cx cy cz w] [A pn inv2w2 0] per component
+    // mixture with mask: the inside test's occupancy grid, gres^D cells over the vertex box
+    // (cell (x, y, z) at x + gres (y + gres z)): 0 outside, 1 inside, 2 near the boundary
+    const uint8_t *gflag;
+    uint32_t gres;
+    float glo[3], gh[3], ginv[3];
 };
 
 struct DevScene {
@@ -1043,6 +1048,28 @@ template <int D> struct VolSample {
     float inv_pdf;
 };
 
+// The inside test's occupancy grid at p: 0 outside, 1 inside, 2 unknown (p casts its own ray).
+// An outside or inside cell lies more than a cell from every element's inflated box and the
+// rays from two of its points agreed, so the parity is the same at each of its points; near
+// cells, and points within a cell outside the grid, are unknown; points further out are
+// outside the closed boundary.
+template <int D> __device__ __forceinline__ uint32_t inside_grid_cell(const VolSrc &v, const float *p) {
+    if (v.gres == 0u) return 2u;
+    const float R = (float)v.gres;
+    uint32_t idx = 0u, mul = 1u;
+    bool out = false, edge = false;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const float sk = (p[k] - v.glo[k]) * v.ginv[k];
+        if (!(sk >= -1.f && sk < R + 1.f)) out = true;
+        else if (!(sk >= 0.f && sk < R)) edge = true;
+        else { idx += (uint32_t)sk * mul; mul *= v.gres; }
+    }
+    if (out) return 0u;
+    if (edge) return 2u;
+    return (uint32_t)v.gflag[idx];
+}
+
 // Crossings of the ray o + t r, t > 0, with the boundary along a fixed direction r; odd
 // means inside a closed boundary (a counting line query, the wave's lanes sharing the work).
 // Kept out of line: it is the only traversal of a volume sample, and inlining it at every
@@ -1131,9 +1158,10 @@ __device__ __forceinline__ VolSample<D> sample_gauss(const DevScene &S, Rng &rng
     r.value = f;
     r.inv_pdf = 1.0f / pdf;
     if (v.mask) {
-        const uint32_t c = ray_crossings<D, STACK>(S.nodes, S.prims, r.p[0], r.p[1], D == 3 ? r.p[D - 1] : 0.f, stack,
-                                                   stride);
-        if (((c & 1u) != 0u) != interior) r.value = 0.f;
+        uint32_t c = inside_grid_cell<D>(v, r.p);
+        if (c == 2u)
+            c = ray_crossings<D, STACK>(S.nodes, S.prims, r.p[0], r.p[1], D == 3 ? r.p[D - 1] : 0.f, stack, stride) & 1u;
+        if ((c != 0u) != interior) r.value = 0.f;
     }
     return r;
 }

@@ -120,6 +120,13 @@ hipError_t launch_hcv_targets_2(const wobdev::DevScene &S, uint32_t n, uint64_t 
                                 float eps, bool interior, float *rec, size_t lds_bytes, hipStream_t st);
 hipError_t launch_hcv_targets_3(const wobdev::DevScene &S, uint32_t n, uint64_t seed, uint32_t stream, uint32_t nvs,
                                 float eps, bool interior, float *rec, size_t lds_bytes, hipStream_t st);
+// The inside test's occupancy grid (wob_walk.hip, volume translation units): classifies every
+// cell whose flag is 255 by ray parity; S.vol carries the grid layout; lds_bytes =
+// walk_lds_bytes(FL_UNIFORM, true, 256).
+hipError_t launch_inside_grid_2(const wobdev::DevScene &S, uint8_t *flag, uint32_t n_cells, size_t lds_bytes,
+                                hipStream_t st);
+hipError_t launch_inside_grid_3(const wobdev::DevScene &S, uint8_t *flag, uint32_t n_cells, size_t lds_bytes,
+                                hipStream_t st);
 // The fit's augmented FP64 Gram matrix (G: (m + 2)^2 doubles) of n target records and m
 // source positions (src: m * dim doubles); the walk's residual datum at each record (out: n).
 size_t hcv_gram_scratch(uint32_t n, uint32_t m);   // bytes of `partial`

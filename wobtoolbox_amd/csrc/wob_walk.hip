@@ -419,6 +419,31 @@ __global__ __launch_bounds__(256) void hcv_targets_kernel(DevScene S, uint32_t n
     r[2 * D + 2] = (float)y.y.type;
     r[2 * D + 3] = y.y.alpha;
 }
+
+// The inside test's occupancy grid: every cell the host left unmarked (255) is classified by
+// the rays from its centre and from the point a quarter cell towards its low corner: their
+// common parity, or 2 (near: samples there cast their own rays) when they disagree.
+template <int D>
+__global__ __launch_bounds__(256) void inside_grid_kernel(DevScene S, uint8_t *flag, uint32_t n_cells) {
+#pragma clang fp contract(off)
+    extern __shared__ uint32_t lds_stack[];
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_cells || flag[c] != 255u) return;
+    const VolSrc &v = S.vol;
+    uint32_t rem = c;
+    float pc[3] = {0.f, 0.f, 0.f}, pq[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const float ik = (float)(rem % v.gres);
+        rem /= v.gres;
+        pc[k] = v.glo[k] + (ik + 0.5f) * v.gh[k];
+        pq[k] = v.glo[k] + (ik + 0.25f) * v.gh[k];
+    }
+    uint32_t *stack = lds_stack + threadIdx.x;
+    const uint32_t a = ray_crossings<D, wobl::kStackEntries>(S.nodes, S.prims, pc[0], pc[1], pc[2], stack, blockDim.x) & 1u;
+    const uint32_t b = ray_crossings<D, wobl::kStackEntries>(S.nodes, S.prims, pq[0], pq[1], pq[2], stack, blockDim.x) & 1u;
+    flag[c] = (uint8_t)(a == b ? a : 2u);
+}
 #endif
 
 }  // namespace wobk
@@ -514,6 +539,15 @@ hipError_t WOB_NAME(launch_hcv_targets_, WOB_WALK_DIM, )(const wobdev::DevScene 
     const uint32_t block = 256;
     hipLaunchKernelGGL((hcv_targets_kernel<WOB_WALK_DIM>), dim3((n + block - 1) / block), dim3(block), lds_bytes, st, S,
                        n, seed, stream, nvs, eps, interior ? 1u : 0u, rec);
+    return hipGetLastError();
+}
+
+hipError_t WOB_NAME(launch_inside_grid_, WOB_WALK_DIM, )(const wobdev::DevScene &S, uint8_t *flag, uint32_t n_cells,
+                                                         size_t lds_bytes, hipStream_t st) {
+    if (n_cells == 0) return hipSuccess;
+    const uint32_t block = 256;
+    hipLaunchKernelGGL((inside_grid_kernel<WOB_WALK_DIM>), dim3((n_cells + block - 1) / block), dim3(block), lds_bytes,
+                       st, S, flag, n_cells);
     return hipGetLastError();
 }
 #endif
