@@ -122,8 +122,10 @@ wob_status wob_scene_set_volume_grid(wob_scene *scene, uint32_t grid_res, float 
  * with weights |A_j| w_j^dim (inv_pdf = 1 / mixture density) and, unless
  * WOB_VOLUME_NO_MASK is given, a sample on the far side of the boundary (outside an interior
  * domain, inside the boundary for an exterior one; parity of the crossings of a fixed ray)
- * contributes 0, so the source is restricted to the domain.  centers: n*dim floats, widths
- * and amplitudes: n floats (host).  n == 0 removes the source.                              */
+ * contributes 0, so the source is restricted to the domain.  The masked source builds an
+ * occupancy grid over the vertex box (64^3 cells in 3D, 512^2 in 2D, one byte each, on the
+ * scene's device) that decides the samples away from the boundary without a ray.  centers:
+ * n*dim floats, widths and amplitudes: n floats (host).  n == 0 removes the source.          */
 enum { WOB_VOLUME_NO_MASK = 1 };
 wob_status wob_scene_set_volume_gaussians(wob_scene *scene, uint32_t n, const float *centers, const float *widths,
                                           const float *amplitudes, uint32_t flags);
