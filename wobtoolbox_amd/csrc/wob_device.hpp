@@ -1543,10 +1543,13 @@ __device__ __forceinline__ Sample<D> ris(const DevScene &S, const Model<D, BIE> 
     float w_sum = 0.f;
     ksum = 0.f;
     const int tp = yp.type;
+    // each kernel evaluated at most once per lane, so a wave whose lanes hold all three types
+    // runs G and dG/dn once each (a Robin branch of its own would run them a second time)
     auto signed_kernel = [&](const float *y) -> float {
-        if (tp == kDirichlet) return m.k.G(y, yp.p);
-        if (tp == kNeumann) return m.k.dG_dny(yp.p, y, yp.n);
-        return m.k.dG_dny(yp.p, y, yp.n) + yp.alpha * m.k.G(y, yp.p);
+        float g = 0.f, dg = 0.f;
+        if (tp != kNeumann) g = m.k.G(y, yp.p);
+        if (tp != kDirichlet) dg = m.k.dG_dny(yp.p, y, yp.n);
+        return tp == kDirichlet ? g : tp == kNeumann ? dg : dg + yp.alpha * g;
     };
     GeomSample<D> pick;
     pick.elem = 0xFFFFFFFFu;
